@@ -1,0 +1,82 @@
+#!/usr/bin/env bash
+# Build the reference SuLi-IC solver routines with LLVM flang, for one grid
+# configuration, together with the harness program in this directory.
+#
+# The reference fixes its grid, boundary codes and model switches as Fortran
+# `parameter`s (0_modules.f90), so every configuration is its own build.  This
+# script reads the sources where they lie (/root/reference/SuLi-IC), writes a
+# configured copy into oracle/_ref/<tag>/src (ignored by git, never committed)
+# and compiles it to oracle/_ref/<tag>/harness.  The edits are configuration
+# only, plus one rename that LLVM flang needs:
+#   0_modules.f90:7      dx, dy, dz
+#   0_modules.f90:15     nx, ny, nz
+#   0_modules.f90:20     uinicial = 0 (fluid at rest)
+#   0_modules.f90:31     m_turb
+#   0_modules.f90:53-61  boundary codes ccx0..cczf
+#   0_modules.f90:143    gx, gz
+#   0_modules.f90:181    csmag
+#   module `tempo` (0_modules.f90:153) -> `tempo_mod`: flang rejects
+#   `use tempo` inside 11_diversos.f90, whose SUBROUTINE tempo has the same
+#   name (11_diversos.f90:10,237).
+# 1_principal.f90 (the PROGRAM) is left out; harness.f90 replaces it and
+# calls the reference subroutines in the order of 1_principal.f90:64-108.
+#
+# usage: build_ref.sh TAG NX NY NZ DX DY DZ CCX0 CCXF CCY0 CCYF CCZ0 CCZF MTURB CSMAG GX GZ
+set -euo pipefail
+if [ $# -ne 17 ]; then
+  echo "usage: $0 TAG NX NY NZ DX DY DZ CCX0 CCXF CCY0 CCYF CCZ0 CCZF MTURB CSMAG GX GZ" >&2
+  exit 2
+fi
+TAG=$1; NX=$2; NY=$3; NZ=$4; DX=$5; DY=$6; DZ=$7
+CCX0=$8; CCXF=$9; CCY0=${10}; CCYF=${11}; CCZ0=${12}; CCZF=${13}
+MTURB=${14}; CSMAG=${15}; GX=${16}; GZ=${17}
+
+HERE=$(cd "$(dirname "$0")" && pwd)
+REF=${SULI_REFERENCE:-/root/reference}/SuLi-IC
+OUT=$HERE/../_ref/$TAG
+FC=${FC:-/opt/rocm/lib/llvm/bin/flang}
+if [ ! -d "$REF" ]; then echo "reference sources not found at $REF" >&2; exit 3; fi
+
+rm -rf "$OUT"
+mkdir -p "$OUT/src" "$OUT/obj"
+for f in 0_modules 2_iniciais 3_contorno 4_plot 5_viscosidade 6_convdiff 7_advectivo \
+         8_derivadas 9_grad 10_pos 11_diversos 12_paraview_3D 13_mms 14_level_set; do
+  cp "$REF/$f.f90" "$OUT/src/$f.f90"
+done
+chmod u+w "$OUT"/src/*.f90
+
+M=$OUT/src/0_modules.f90
+edit() {  # edit FILE SED-EXPR PATTERN-THAT-MUST-EXIST-AFTER
+  sed -i -e "$2" "$1"
+  grep -q -- "$3" "$1" || { echo "edit failed in $1: $2" >&2; exit 4; }
+}
+edit "$M" "s/^real(8),parameter :: dx = 0.02, dy = 0.02, dz = 0.02/real(8),parameter :: dx = $DX, dy = $DY, dz = $DZ/" "dx = $DX, dy = $DY, dz = $DZ"
+edit "$M" "s/^integer,parameter :: nx=int(.5\/dx) , ny=int(1.\/dy), nz=int(0.9\/dz)/integer,parameter :: nx=$NX, ny=$NY, nz=$NZ/" "nx=$NX, ny=$NY, nz=$NZ"
+edit "$M" "s/^real(8),parameter :: uinicial = 0.31/real(8),parameter :: uinicial = 0./" "uinicial = 0\."
+edit "$M" "s/^integer,parameter :: m_turb = 0/integer,parameter :: m_turb = $MTURB/" "m_turb = $MTURB"
+edit "$M" "s/^integer,parameter :: ccx0=0/integer,parameter :: ccx0=$CCX0/" "ccx0=$CCX0"
+edit "$M" "s/^integer,parameter :: ccxf=0/integer,parameter :: ccxf=$CCXF/" "ccxf=$CCXF"
+edit "$M" "s/^integer,parameter :: ccy0=2/integer,parameter :: ccy0=$CCY0/" "ccy0=$CCY0"
+edit "$M" "s/^integer,parameter :: ccyf=2/integer,parameter :: ccyf=$CCYF/" "ccyf=$CCYF"
+edit "$M" "s/^integer,parameter :: ccz0=2/integer,parameter :: ccz0=$CCZ0/" "ccz0=$CCZ0"
+edit "$M" "s/^integer,parameter :: cczf=1/integer,parameter :: cczf=$CCZF/" "cczf=$CCZF"
+edit "$M" "s/^real(8), parameter :: gx = 9.80665 \* sin(atan(decliv)) , gz = 9.80665 \* cos(atan(decliv))/real(8), parameter :: gx = $GX , gz = $GZ/" "gx = $GX , gz = $GZ"
+edit "$M" "s/^real(8), parameter :: csmag=0.13/real(8), parameter :: csmag=$CSMAG/" "csmag=$CSMAG"
+edit "$M" "s/^module tempo\$/module tempo_mod/" "^module tempo_mod"
+edit "$M" "s/^end module tempo\$/end module tempo_mod/" "^end module tempo_mod"
+for f in 2_iniciais 4_plot 5_viscosidade 11_diversos; do
+  sed -i -e 's/^\([[:space:]]*\)[Uu][Ss][Ee] tempo[[:space:]]*$/\1USE tempo_mod/' "$OUT/src/$f.f90"
+done
+if grep -n -i "^[[:space:]]*use tempo[[:space:]]*$" "$OUT"/src/*.f90; then
+  echo "unrenamed 'use tempo' left" >&2; exit 4
+fi
+
+FFLAGS="-O2 -fdefault-real-8 -fdefault-double-8"
+cd "$OUT/obj"
+for f in 0_modules 2_iniciais 3_contorno 4_plot 5_viscosidade 6_convdiff 7_advectivo \
+         8_derivadas 9_grad 10_pos 11_diversos 12_paraview_3D 13_mms 14_level_set; do
+  "$FC" $FFLAGS -c "$OUT/src/$f.f90" -o "$f.o"
+done
+"$FC" $FFLAGS -c "$HERE/harness.f90" -o harness.o
+"$FC" -O2 -o "$OUT/harness" ./*.o
+echo "built $OUT/harness (nx=$NX ny=$NY nz=$NZ)"

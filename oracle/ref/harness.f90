@@ -1,0 +1,180 @@
+! Test harness around the reference SuLi-IC routines (oracle only).
+!
+! Replaces the reference PROGRAM (1_principal.f90) for golden-vector and
+! CPU-baseline runs: it loads a full solver state from a raw little-endian
+! stream file, calls one reference subroutine (or the time step of
+! 1_principal.f90:64-108), and writes the state back in the same format.
+!
+!   harness IN OUT CMD [ARG]
+!
+! CMD: ls_ini | level_set | heaviside | mod_ls1 | reinic | der_weno | weno1 ARG(ihs)
+!      visco | convdiff | tempo ARG(tt) | contorno ARG(nlock) | graddin | posdin
+!      stage ARG(tt) | step ARG(nsteps)
+!
+! State file: int32 nx,ny,nz; 10 float64 scalars (dt, t, vol_ini, vol_ins,
+! rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1); then the arrays listed in
+! io_state, each in Fortran order with the reference extents (0_modules.f90).
+program harness
+  use disc
+  use velpre
+  use vartempo
+  use smag
+  use ls_param
+  use obst
+  implicit none
+  character(len=512) :: fin, fout, cmd, arg
+  integer :: iarg, nsteps, k, i, j
+  integer(8) :: c0, c1, crate, g0, g1, gsum, ls0, ls1, lssum
+  real(8) :: dt_base
+  real(8), dimension(nx,ny,nz) :: ta1, tb1, tc1, td1, te1, tf1
+  real(8), dimension(nx,ny,nz) :: s1, s2, s3
+  integer :: ihs
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  call get_command_argument(3, cmd)
+  iarg = 0
+  if (command_argument_count() >= 4) then
+    call get_command_argument(4, arg)
+    read(arg, *) iarg
+  endif
+
+  call io_state(fin, 0)
+
+  ! RK3-TVD coefficients and RK2 stage steps as set up by
+  ! 14_level_set.f90:34-44 and 2_iniciais.f90:432-437.
+  adtl(1) = 1.;     bdtl(1) = 0.;     gdtl(1) = 1.
+  adtl(2) = 3./4.;  bdtl(2) = 1./4.;  gdtl(2) = 1./4.
+  adtl(3) = 1./3.;  bdtl(3) = 2./3.;  gdtl(3) = 2./3.
+  ls_m = 0.
+  ntt = 2
+  dt_base = dt
+  a_dt(1) = 0.5*dt_base
+  a_dt(2) = 1.0*dt_base
+  a_dt(3) = 1.0*dt_base
+  tt = 1
+  it = 1
+
+  select case (trim(cmd))
+  case ('ls_ini')
+    call heaviside()
+    vol_ini = 0.
+    do k = 1, nz
+    do j = 1, ny
+    do i = 1, nx
+      vol_ini = vol_ini + (1.-(1.-hs(i,j,k)))*dx*dy*dz
+    enddo
+    enddo
+    enddo
+    call mod_ls1()
+  case ('level_set')
+    call level_set()
+  case ('heaviside')
+    call heaviside()
+  case ('mod_ls1')
+    call mod_ls1()
+  case ('reinic')
+    call reinic_weno(s1, s2, s3)
+  case ('der_weno')
+    call der_weno(ls, ta1, tb1, tc1, td1, te1, tf1, ihs)
+    hsx = ta1; hsy = tb1; hsz = tc1
+    ddlsdx = td1; ddlsdy = te1; ddlsdz = tf1
+    vol_ins = ihs
+  case ('weno1')
+    ihs = iarg
+    call weno1(ddlsdx(:,1,1), ddlsdy(:,1,1), nx, dx, ls(:,1,1), ihs)
+  case ('visco')
+    call visco()
+  case ('convdiff')
+    call convdiff()
+  case ('tempo')
+    tt = iarg
+    call tempo()
+  case ('contorno')
+    call contorno(iarg)
+  case ('graddin')
+    call graddin()
+  case ('posdin')
+    call posdin()
+  case ('stage')
+    tt = iarg
+    dt = a_dt(tt)
+    call visco()
+    call convdiff()
+    call tempo()
+    call contorno(2)
+    call graddin()
+    call posdin()
+    call contorno(1)
+    dt = dt_base
+  case ('step')
+    nsteps = max(iarg, 1)
+    gsum = 0
+    lssum = 0
+    call system_clock(c0, crate)
+    do it = 1, nsteps
+      t = it * dt
+      call system_clock(ls0)
+      call level_set()
+      call system_clock(ls1)
+      lssum = lssum + (ls1 - ls0)
+      call contorno(3)
+      do tt = 1, ntt
+        dt = a_dt(tt)
+        call visco()
+        call convdiff()
+        call tempo()
+        call contorno(2)
+        call system_clock(g0)
+        call graddin()
+        call system_clock(g1)
+        gsum = gsum + (g1 - g0)
+        call posdin()
+        call contorno(1)
+      enddo
+    enddo
+    call system_clock(c1)
+    write(*,'(A,I0,A,ES16.8,A,ES16.8,A,ES16.8)') 'REF_TIMING steps=', nsteps, &
+      ' seconds=', real(c1-c0,8)/real(crate,8), ' graddin_seconds=', real(gsum,8)/real(crate,8), &
+      ' level_set_seconds=', real(lssum,8)/real(crate,8)
+  case default
+    write(*,*) 'unknown command ', trim(cmd)
+    stop 2
+  end select
+
+  if (trim(fout) /= '-') call io_state(fout, 1)
+
+contains
+
+  subroutine io_state(fname, mode)
+    character(len=*), intent(in) :: fname
+    integer, intent(in) :: mode
+    integer :: hdr(3)
+    real(8) :: sc(10)
+    if (mode == 0) then
+      open(unit=21, file=trim(fname), access='stream', form='unformatted', status='old')
+      read(21) hdr
+      if (hdr(1) /= nx .or. hdr(2) /= ny .or. hdr(3) /= nz) then
+        write(*,*) 'grid mismatch: file ', hdr, ' build ', nx, ny, nz
+        stop 3
+      endif
+      read(21) sc
+      dt = sc(1); t = sc(2); vol_ini = sc(3); vol_ins = sc(4)
+      rho_f1 = sc(5); rho_f2 = sc(6); mi_f1 = sc(7); mi_f2 = sc(8)
+      sigma = sc(9); alpha1 = sc(10)
+      read(21) u, v, w, prd1, rho, ls_nu, ls, mod_ls, kurv, hs, hsx, hsy, hsz, &
+               ddlsdx, ddlsdy, ddlsdz, nut, xnut, ynut, znut, Fu, Fv, Fw, u0, v0, w0, ub, vb, wb
+      close(21)
+    else
+      open(unit=22, file=trim(fname), access='stream', form='unformatted', status='replace')
+      hdr = (/ nx, ny, nz /)
+      sc = (/ dt, t, vol_ini, vol_ins, rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1 /)
+      write(22) hdr
+      write(22) sc
+      write(22) u, v, w, prd1, rho, ls_nu, ls, mod_ls, kurv, hs, hsx, hsy, hsz, &
+                ddlsdx, ddlsdy, ddlsdz, nut, xnut, ynut, znut, Fu, Fv, Fw, u0, v0, w0, ub, vb, wb
+      close(22)
+    endif
+  end subroutine io_state
+
+end program harness

@@ -1,0 +1,1030 @@
+/* CPU oracle for the SuLi-IC time step (test infrastructure only).
+ *
+ * A plain-C FP64 restatement of the reference Fortran routines, written to
+ * evaluate every expression in the reference's own order (Fortran evaluates
+ * a+b+c as (a+b)+c and a*b*c as (a*b)*c), so that with IEEE double
+ * arithmetic and no FMA contraction it reproduces the reference bit for bit.
+ * Each routine cites the reference lines it follows (paths relative to
+ * SuLi-IC/).  Only tests/, the smoke check and the CPU-baseline leg of the
+ * benchmark use this file; the product library never does.
+ */
+#include "suli_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI 3.141592653589793115997963468544185161590576171875 /* acos(-1.), 0_modules.f90:4 */
+
+/* index maps of the reference extents (0_modules.f90:92-133) */
+#define DIMS const int nx = s->nx, ny = s->ny, nz = s->nz; (void)nx; (void)ny; (void)nz
+#define IU(i, j, k) ((int64_t)(i) + (int64_t)(nx + 3) * ((int64_t)(j) + (int64_t)(ny + 2) * (int64_t)(k)))
+#define IV(i, j, k) ((int64_t)(i) + (int64_t)(nx + 2) * ((int64_t)(j) + (int64_t)(ny + 3) * (int64_t)(k)))
+#define IW(i, j, k) ((int64_t)(i) + (int64_t)(nx + 2) * ((int64_t)(j) + (int64_t)(ny + 2) * (int64_t)(k)))
+#define IP(i, j, k) IW(i, j, k)
+#define IC(i, j, k) ((int64_t)((i)-1) + (int64_t)nx * ((int64_t)((j)-1) + (int64_t)ny * (int64_t)((k)-1)))
+#define IFX(i, j, k) ((int64_t)((i)-1) + (int64_t)(nx + 1) * ((int64_t)((j)-1) + (int64_t)ny * (int64_t)((k)-1)))
+#define IFY(i, j, k) ((int64_t)((i)-1) + (int64_t)nx * ((int64_t)((j)-1) + (int64_t)(ny + 1) * (int64_t)((k)-1)))
+#define IFZ(i, j, k) IC(i, j, k)
+/* generic 1-based (dimx,dimy,dimz) array */
+#define IA(i, j, k, dx_, dy_) ((int64_t)((i)-1) + (int64_t)(dx_) * ((int64_t)((j)-1) + (int64_t)(dy_) * (int64_t)((k)-1)))
+
+static double fmax_f(double a, double b) { return a > b ? a : b; }
+static double fmin_f(double a, double b) { return a < b ? a : b; }
+
+int64_t or_state_len(int nx, int ny, int nz) {
+  int64_t U = (int64_t)(nx + 3) * (ny + 2) * (nz + 2);
+  int64_t V = (int64_t)(nx + 2) * (ny + 3) * (nz + 2);
+  int64_t W = (int64_t)(nx + 2) * (ny + 2) * (nz + 3);
+  int64_t P = (int64_t)(nx + 2) * (ny + 2) * (nz + 2);
+  int64_t C = (int64_t)nx * ny * nz;
+  int64_t FX = (int64_t)(nx + 1) * ny * nz, FY = (int64_t)nx * (ny + 1) * nz, FZ = (int64_t)nx * ny * (nz + 1);
+  /* u v w prd1 | 12 cell arrays | nut | xnut ynut znut | Fu Fv Fw | u0 v0 w0 | ub vb wb */
+  return U + V + W + P + 12 * C + C + FX + FY + FZ + FX + FY + FZ + FX + FY + FZ + U + V + W;
+}
+
+void or_bind(or_state *s, double *b) {
+  DIMS;
+  int64_t U = (int64_t)(nx + 3) * (ny + 2) * (nz + 2);
+  int64_t V = (int64_t)(nx + 2) * (ny + 3) * (nz + 2);
+  int64_t W = (int64_t)(nx + 2) * (ny + 2) * (nz + 3);
+  int64_t P = (int64_t)(nx + 2) * (ny + 2) * (nz + 2);
+  int64_t C = (int64_t)nx * ny * nz;
+  int64_t FX = (int64_t)(nx + 1) * ny * nz, FY = (int64_t)nx * (ny + 1) * nz, FZ = (int64_t)nx * ny * (nz + 1);
+  s->u = b; b += U; s->v = b; b += V; s->w = b; b += W; s->prd1 = b; b += P;
+  s->rho = b; b += C; s->ls_nu = b; b += C; s->ls = b; b += C; s->mod_ls = b; b += C;
+  s->kurv = b; b += C; s->hs = b; b += C; s->hsx = b; b += C; s->hsy = b; b += C; s->hsz = b; b += C;
+  s->ddlsdx = b; b += C; s->ddlsdy = b; b += C; s->ddlsdz = b; b += C;
+  s->nut = b; b += C; s->xnut = b; b += FX; s->ynut = b; b += FY; s->znut = b; b += FZ;
+  s->Fu = b; b += FX; s->Fv = b; b += FY; s->Fw = b; b += FZ;
+  s->u0 = b; b += FX; s->v0 = b; b += FY; s->w0 = b; b += FZ;
+  s->ub = b; b += U; s->vb = b; b += V; s->wb = b; b += W;
+}
+
+/* ------------------------------------------------------------------------ */
+/* WENO5 one-sided derivatives of a pencil, 14_level_set.f90:511-610.        */
+void or_weno1(double *dphidxp, double *dphidxn, int n, double h, const double *phi0,
+              int64_t stride, int ihs) {
+  const double aux1 = 13. / 12., aux2 = 1. / 4., aux3 = 1. / 6.;
+  const double auxx[3] = {0.1, 0.6, 0.3};
+  const double aux6 = 0.00000001;
+  double *phi1_ = (double *)malloc(sizeof(double) * (size_t)(n + 6));
+  double *up_ = (double *)malloc(sizeof(double) * (size_t)(n + 4));
+  double *un_ = (double *)malloc(sizeof(double) * (size_t)(n + 4));
+#define PHI1(i) phi1_[(i) + 2]   /* phi1(-2:n+3) */
+#define UP(i) up_[(i) + 3]       /* up(-3:n)     */
+#define UN(i) un_[(i)-1]         /* un(1:n+4)    */
+  for (int i = 1; i <= n; i++) PHI1(i) = phi0[(int64_t)(i - 1) * stride];
+  if (ihs == 0) { /* periodic, :532-540 */
+    PHI1(0) = PHI1(n - 1);
+    PHI1(-1) = PHI1(n - 2);
+    PHI1(-2) = PHI1(n - 3);
+    PHI1(n + 1) = PHI1(2);
+    PHI1(n + 2) = PHI1(3);
+    PHI1(n + 3) = PHI1(4);
+  } else if (ihs == 1) { /* extrapolation, asymmetric 1/5 and 1/11, :541-556 */
+    PHI1(0) = 1. / 5. * (12. * PHI1(1) - 9. * PHI1(2) + 2. * PHI1(3));
+    PHI1(-1) = 1. / 5. * (12. * PHI1(0) - 9. * PHI1(1) + 2. * PHI1(2));
+    PHI1(-2) = 1. / 5. * (12. * PHI1(-1) - 9. * PHI1(0) + 2. * PHI1(1));
+    PHI1(n + 1) = 1. / 11. * (18. * PHI1(n) - 9. * PHI1(n - 1) + 2. * PHI1(n - 2));
+    PHI1(n + 2) = 1. / 11. * (18. * PHI1(n + 1) - 9. * PHI1(n) + 2. * PHI1(n - 1));
+    PHI1(n + 3) = 1. / 11. * (18. * PHI1(n + 2) - 9. * PHI1(n + 1) + 2. * PHI1(n));
+  } else if (ihs == 2) { /* zero derivative, 1/11 both sides, :557-569 */
+    PHI1(0) = 1. / 11. * (18. * PHI1(1) - 9. * PHI1(2) + 2. * PHI1(3));
+    PHI1(-1) = 1. / 11. * (18. * PHI1(0) - 9. * PHI1(1) + 2. * PHI1(2));
+    PHI1(-2) = 1. / 11. * (18. * PHI1(-1) - 9. * PHI1(0) + 2. * PHI1(1));
+    PHI1(n + 1) = 1. / 11. * (18. * PHI1(n) - 9. * PHI1(n - 1) + 2. * PHI1(n - 2));
+    PHI1(n + 2) = 1. / 11. * (18. * PHI1(n + 1) - 9. * PHI1(n) + 2. * PHI1(n - 1));
+    PHI1(n + 3) = 1. / 11. * (18. * PHI1(n + 2) - 9. * PHI1(n + 1) + 2. * PHI1(n));
+  }
+  for (int i = -3; i <= n; i++) UP(i) = (PHI1(i + 3) - PHI1(i + 2)) / h;
+  for (int i = 1; i <= n + 4; i++) UN(i) = (PHI1(i - 2) - PHI1(i - 3)) / h;
+  for (int i = 1; i <= n; i++) {
+    double isup[3], isun[3], alpup[3], alpun[3], omgup[3], omgun[3], a, b;
+    a = UP(i) - 2 * UP(i - 1) + UP(i - 2);
+    b = UP(i) - 4 * UP(i - 1) + 3 * UP(i - 2);
+    isup[0] = aux1 * a * a + aux2 * b * b;
+    a = UN(i) - 2 * UN(i + 1) + UN(i + 2);
+    b = UN(i) - 4 * UN(i + 1) + 3 * UN(i + 2);
+    isun[0] = aux1 * a * a + aux2 * b * b;
+    a = UP(i - 1) - 2 * UP(i - 2) + UP(i - 3);
+    b = UP(i - 1) - UP(i - 3);
+    isup[1] = aux1 * a * a + aux2 * b * b;
+    a = UN(i + 1) - 2 * UN(i + 2) + UN(i + 3);
+    b = UN(i + 1) - UN(i + 3);
+    isun[1] = aux1 * a * a + aux2 * b * b;
+    a = UP(i - 2) - 2 * UP(i - 3) + UP(i - 4);
+    b = 3 * UP(i - 2) - 4 * UP(i - 3) + UP(i - 4);
+    isup[2] = aux1 * a * a + aux2 * b * b;
+    a = UN(i + 2) - 2 * UN(i + 3) + UN(i + 4);
+    b = 3 * UN(i + 2) - 4 * UN(i + 3) + UN(i + 4);
+    isun[2] = aux1 * a * a + aux2 * b * b;
+    for (int q = 0; q < 3; q++) {
+      alpup[q] = auxx[q] / ((aux6 + isup[q]) * (aux6 + isup[q]));
+      alpun[q] = auxx[q] / ((aux6 + isun[q]) * (aux6 + isun[q]));
+    }
+    for (int q = 0; q < 3; q++) {
+      omgup[q] = alpup[q] / (alpup[0] + alpup[1] + alpup[2]);
+      omgun[q] = alpun[q] / (alpun[0] + alpun[1] + alpun[2]);
+    }
+    dphidxp[(int64_t)(i - 1) * stride] =
+        aux3 * (omgup[0] * (2 * UP(i) - 7 * UP(i - 1) + 11 * UP(i - 2)) +
+                omgup[1] * (-UP(i - 1) + 5 * UP(i - 2) + 2 * UP(i - 3)) +
+                omgup[2] * (2 * UP(i - 2) + 5 * UP(i - 3) - UP(i - 4)));
+    dphidxn[(int64_t)(i - 1) * stride] =
+        aux3 * (omgun[0] * (2 * UN(i) - 7 * UN(i + 1) + 11 * UN(i + 2)) +
+                omgun[1] * (-UN(i + 1) + 5 * UN(i + 2) + 2 * UN(i + 3)) +
+                omgun[2] * (2 * UN(i + 2) + 5 * UN(i + 3) - UN(i + 4)));
+  }
+#undef PHI1
+#undef UP
+#undef UN
+  free(phi1_);
+  free(up_);
+  free(un_);
+}
+
+/* der_weno, 14_level_set.f90:343-384: x and y pencils use ihs 0 (periodic)
+ * or 2, z pencils ihs 1; returns the ihs left in the caller (always 1). */
+int or_der_weno(const or_state *s, const double *ls, double *ta1, double *tb1, double *tc1,
+                double *td1, double *te1, double *tf1) {
+  DIMS;
+  int ihs = (s->ccx0 == 0) ? 0 : 2;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      or_weno1(ta1 + IC(1, j, k), td1 + IC(1, j, k), nx, s->dx, ls + IC(1, j, k), 1, ihs);
+  ihs = (s->ccy0 == 0) ? 0 : 2;
+  for (int k = 1; k <= nz; k++)
+    for (int i = 1; i <= nx; i++)
+      or_weno1(tb1 + IC(i, 1, k), te1 + IC(i, 1, k), ny, s->dy, ls + IC(i, 1, k), nx, ihs);
+  ihs = 1;
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++)
+      or_weno1(tc1 + IC(i, j, 1), tf1 + IC(i, j, 1), nz, s->dz, ls + IC(i, j, 1), (int64_t)nx * ny, ihs);
+  return ihs;
+}
+
+static double *cell_alloc(const or_state *s) {
+  return (double *)calloc((size_t)s->nx * s->ny * s->nz, sizeof(double));
+}
+
+/* SSP-RK3 stage update intt_ls, 14_level_set.f90:287-306, with the
+ * coefficients of 14_level_set.f90:34-44. */
+static const double ADTL[3] = {1., 3. / 4., 1. / 3.};
+static const double BDTL[3] = {0., 1. / 4., 2. / 3.};
+static const double GDTL[3] = {1., 1. / 4., 2. / 3.};
+static void intt_ls(const or_state *s, const double *hx, double *gx, double *ta1, int itrl, double *ls1,
+                    double dt1) {
+  int64_t n = (int64_t)s->nx * s->ny * s->nz;
+  double a = ADTL[itrl - 1], b = BDTL[itrl - 1], g = GDTL[itrl - 1];
+  if (a == 1.) memcpy(ta1, ls1, sizeof(double) * (size_t)n);
+  memcpy(gx, ls1, sizeof(double) * (size_t)n);
+  for (int64_t q = 0; q < n; q++) ls1[q] = a * ta1[q] + b * gx[q] + g * dt1 * hx[q];
+}
+
+/* conv_weno, 14_level_set.f90:308-341 */
+static void conv_weno(const or_state *s, double *sy7) {
+  DIMS;
+  double *ta1 = cell_alloc(s), *tb1 = cell_alloc(s), *tc1 = cell_alloc(s);
+  double *td1 = cell_alloc(s), *te1 = cell_alloc(s), *tf1 = cell_alloc(s);
+  or_der_weno(s, s->ls, ta1, tb1, tc1, td1, te1, tf1);
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        double apos = fmax_f((s->u[IU(i, j, k)] + s->u[IU(i + 1, j, k)]) * 0.5, 0.);
+        double aneg = fmin_f((s->u[IU(i, j, k)] + s->u[IU(i + 1, j, k)]) * 0.5, 0.);
+        double bpos = fmax_f((s->v[IV(i, j, k)] + s->v[IV(i, j + 1, k)]) * 0.5, 0.);
+        double bneg = fmin_f((s->v[IV(i, j, k)] + s->v[IV(i, j + 1, k)]) * 0.5, 0.);
+        double cpos = fmax_f((s->w[IW(i, j, k)] + s->w[IW(i, j, k + 1)]) * 0.5, 0.);
+        double cneg = fmin_f((s->w[IW(i, j, k)] + s->w[IW(i, j, k + 1)]) * 0.5, 0.);
+        int64_t c = IC(i, j, k);
+        sy7[c] = -(apos * td1[c] + aneg * ta1[c] + bpos * te1[c] + bneg * tb1[c] + cpos * tf1[c] +
+                   cneg * tc1[c]);
+      }
+  free(ta1); free(tb1); free(tc1); free(td1); free(te1); free(tf1);
+}
+
+/* Godunov upwinding with the 1e-8 floors, 14_level_set.f90:406-430 */
+static void godunov(double lsv, double *ta, double *tb, double *tc, double td, double te, double tf) {
+  double aux1, aux2;
+  if (lsv > 0.) {
+    aux1 = fmax_f(td, 0.00000001); aux2 = -fmin_f(*ta, 0.00000001); *ta = fmax_f(aux1, aux2);
+    aux1 = fmax_f(te, 0.00000001); aux2 = -fmin_f(*tb, 0.00000001); *tb = fmax_f(aux1, aux2);
+    aux1 = fmax_f(tf, 0.00000001); aux2 = -fmin_f(*tc, 0.00000001); *tc = fmax_f(aux1, aux2);
+  } else {
+    aux1 = fmax_f(*ta, 0.00000001); aux2 = -fmin_f(td, 0.00000001); *ta = fmax_f(aux1, aux2);
+    aux1 = fmax_f(*tb, 0.00000001); aux2 = -fmin_f(te, 0.00000001); *tb = fmax_f(aux1, aux2);
+    aux1 = fmax_f(*tc, 0.00000001); aux2 = -fmin_f(tf, 0.00000001); *tc = fmax_f(aux1, aux2);
+  }
+}
+
+/* reinic_weno, 14_level_set.f90:386-509 */
+void or_reinic_weno(or_state *s) {
+  DIMS;
+  const int64_t n = (int64_t)nx * ny * nz;
+  const double dx1 = fmax_f(fmax_f(s->dx, s->dy), s->dz);
+  double *ta1 = cell_alloc(s), *tb1 = cell_alloc(s), *tc1 = cell_alloc(s);
+  double *td1 = cell_alloc(s), *te1 = cell_alloc(s), *tf1 = cell_alloc(s);
+  double *func_s = cell_alloc(s), *sy7 = cell_alloc(s), *gx = cell_alloc(s), *ta1s = cell_alloc(s);
+  double *lsaux = cell_alloc(s);
+  const int l = 3;
+  or_der_weno(s, s->ls, ta1, tb1, tc1, td1, te1, tf1);
+  for (int64_t c = 0; c < n; c++) {
+    godunov(s->ls[c], &ta1[c], &tb1[c], &tc1[c], td1[c], te1[c], tf1[c]);
+    double m = sqrt(ta1[c] * ta1[c] + tb1[c] * tb1[c] + tc1[c] * tc1[c]);
+    func_s[c] = s->ls[c] / sqrt(s->ls[c] * s->ls[c] + m * m * dx1 * dx1);
+  }
+  int il = 0;
+  double error = 999.;
+  while (error > s->dt1 * dx1 * dx1) {
+    il = il + 1;
+    for (int itrl = 1; itrl <= 3; itrl++) {
+      or_der_weno(s, s->ls, ta1, tb1, tc1, td1, te1, tf1);
+      for (int64_t c = 0; c < n; c++) {
+        godunov(s->ls[c], &ta1[c], &tb1[c], &tc1[c], td1[c], te1[c], tf1[c]);
+        double m = sqrt(ta1[c] * ta1[c] + tb1[c] * tb1[c] + tc1[c] * tc1[c]);
+        sy7[c] = func_s[c] * (1. - m);
+        lsaux[c] = s->ls[c];
+      }
+      intt_ls(s, sy7, gx, ta1s, itrl, s->ls, s->dt1);
+    }
+    if (il == l) {
+      error = 0.;
+    } else if (il < 1) {
+      error = 999.;
+    } else {
+      error = 0.;
+      int nr = 0;
+      for (int64_t c = 0; c < n; c++)
+        if (fabs(s->ls[c]) <= 2. * s->alpha1 * dx1) {
+          error = error + s->ls[c] - lsaux[c];
+          nr = nr + 1;
+        }
+      error = error / nr;
+    }
+  }
+  s->reinit_iters = il;
+  free(ta1); free(tb1); free(tc1); free(td1); free(te1); free(tf1);
+  free(func_s); free(sy7); free(gx); free(ta1s); free(lsaux);
+}
+
+/* heaviside, 14_level_set.f90:612-707 (t_hs = 0) */
+void or_heaviside(or_state *s) {
+  DIMS;
+  const int64_t n = (int64_t)nx * ny * nz;
+  const double dx1 = fmax_f(fmax_f(s->dx, s->dy), s->dz);
+  double *ta1 = cell_alloc(s), *tb1 = cell_alloc(s), *tc1 = cell_alloc(s);
+  double *td1 = cell_alloc(s), *te1 = cell_alloc(s), *tf1 = cell_alloc(s);
+  or_der_weno(s, s->ls, ta1, tb1, tc1, td1, te1, tf1);
+  for (int64_t c = 0; c < n; c++) {
+    if (!(fabs(ta1[c]) > fabs(td1[c]))) ta1[c] = td1[c];
+    if (!(fabs(tb1[c]) > fabs(te1[c]))) tb1[c] = te1[c];
+    if (!(fabs(tc1[c]) > fabs(tf1[c]))) tc1[c] = tf1[c];
+  }
+  const double a = s->alpha1;
+  for (int64_t c = 0; c < n; c++) {
+    double lsv = s->ls[c];
+    if (lsv < -a * dx1) {
+      s->hs[c] = 0.; s->hsx[c] = 0.; s->hsy[c] = 0.; s->hsz[c] = 0.;
+    } else if (lsv > a * dx1) {
+      s->hs[c] = 1.; s->hsx[c] = 0.; s->hsy[c] = 0.; s->hsz[c] = 0.;
+    } else {
+      s->hs[c] = 0.5 * (1. + lsv / (a * dx1) + 1. / PI * sin(PI * lsv / (a * dx1)));
+      s->hsx[c] = 0.5 * ta1[c] * (1. + cos(PI * lsv / (a * dx1))) / (a * dx1);
+      s->hsy[c] = 0.5 * tb1[c] * (1. + cos(PI * lsv / (a * dx1))) / (a * dx1);
+      s->hsz[c] = 0.5 * tc1[c] * (1. + cos(PI * lsv / (a * dx1))) / (a * dx1);
+    }
+    s->rho[c] = s->rho_f1 * (1. - s->hs[c]) + s->rho_f2 * s->hs[c];
+    s->ls_nu[c] = s->mi_f1 * (1. - s->hs[c]) + s->mi_f2 * s->hs[c];
+  }
+  free(ta1); free(tb1); free(tc1); free(td1); free(te1); free(tf1);
+}
+
+/* mod_ls1, 14_level_set.f90:709-805: normals from the larger one-sided
+ * derivative, then WENO (ihs = 1 in every direction) of the normals and the
+ * curvature as the sum of the larger one-sided derivatives. */
+void or_mod_ls1(or_state *s) {
+  DIMS;
+  const int64_t n = (int64_t)nx * ny * nz;
+  double *ta1 = cell_alloc(s), *tb1 = cell_alloc(s), *tc1 = cell_alloc(s);
+  double *td1 = cell_alloc(s), *te1 = cell_alloc(s), *tf1 = cell_alloc(s);
+  double *nxa = cell_alloc(s), *nya = cell_alloc(s), *nza = cell_alloc(s);
+  int ihs = or_der_weno(s, s->ls, ta1, tb1, tc1, td1, te1, tf1);
+  for (int64_t c = 0; c < n; c++) {
+    if (!(fabs(ta1[c]) > fabs(td1[c]))) ta1[c] = td1[c];
+    if (!(fabs(tb1[c]) > fabs(te1[c]))) tb1[c] = te1[c];
+    if (!(fabs(tc1[c]) > fabs(tf1[c]))) tc1[c] = tf1[c];
+  }
+  for (int64_t c = 0; c < n; c++) {
+    s->mod_ls[c] = sqrt(ta1[c] * ta1[c] + tb1[c] * tb1[c] + tc1[c] * tc1[c]);
+    if (s->mod_ls[c] == 0) {
+      nxa[c] = 0.; nya[c] = 0.; nza[c] = 0.;
+    } else {
+      nxa[c] = ta1[c] / s->mod_ls[c];
+      nya[c] = tb1[c] / s->mod_ls[c];
+      nza[c] = tc1[c] / s->mod_ls[c];
+    }
+  }
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      or_weno1(ta1 + IC(1, j, k), td1 + IC(1, j, k), nx, s->dx, nxa + IC(1, j, k), 1, ihs);
+  for (int k = 1; k <= nz; k++)
+    for (int i = 1; i <= nx; i++)
+      or_weno1(tb1 + IC(i, 1, k), te1 + IC(i, 1, k), ny, s->dy, nya + IC(i, 1, k), nx, ihs);
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++)
+      or_weno1(tc1 + IC(i, j, 1), tf1 + IC(i, j, 1), nz, s->dz, nza + IC(i, j, 1), (int64_t)nx * ny, ihs);
+  for (int64_t c = 0; c < n; c++) {
+    s->ddlsdx[c] = (fabs(td1[c]) > fabs(ta1[c])) ? td1[c] : ta1[c];
+    s->ddlsdy[c] = (fabs(te1[c]) > fabs(tb1[c])) ? te1[c] : tb1[c];
+    s->ddlsdz[c] = (fabs(tf1[c]) > fabs(tc1[c])) ? tf1[c] : tc1[c];
+    s->kurv[c] = s->ddlsdx[c] + s->ddlsdy[c] + s->ddlsdz[c];
+  }
+  free(ta1); free(tb1); free(tc1); free(td1); free(te1); free(tf1);
+  free(nxa); free(nya); free(nza);
+}
+
+static double volume(const or_state *s) {
+  DIMS;
+  double vol = 0.;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) vol = vol + (1. - (1. - s->hs[IC(i, j, k)])) * s->dx * s->dy * s->dz;
+  return vol;
+}
+
+/* Tail of level_set_ini (14_level_set.f90:205-217): heaviside, initial
+ * volume, mod_ls1. */
+void or_level_set_ini_tail(or_state *s) {
+  or_heaviside(s);
+  s->vol_ini = volume(s);
+  or_mod_ls1(s);
+}
+
+/* level_set, 14_level_set.f90:221-285 */
+void or_level_set(or_state *s) {
+  DIMS;
+  const int64_t n = (int64_t)nx * ny * nz;
+  double *sy7 = cell_alloc(s), *gx = cell_alloc(s), *ta1 = cell_alloc(s);
+  for (int itrl = 1; itrl <= 3; itrl++) {
+    conv_weno(s, sy7);
+    intt_ls(s, sy7, gx, ta1, itrl, s->ls, s->dt);  /* dt1 := dt, :231-232 */
+  }
+  or_reinic_weno(s);
+  or_mod_ls1(s);
+  or_heaviside(s);
+  s->vol_ins = volume(s);
+  int nc = 0;
+  while (fabs(s->vol_ins - s->vol_ini) / s->vol_ins > 0.1) {
+    for (int64_t c = 0; c < n; c++)
+      s->ls[c] = s->ls[c] - s->dt1 * (s->vol_ins - s->vol_ini) * s->mod_ls[c] / s->vol_ini;
+    or_mod_ls1(s);
+    or_heaviside(s);
+    s->vol_ins = volume(s);
+    nc++;
+  }
+  s->mass_corr_iters = nc;
+  or_mod_ls1(s);
+  or_heaviside(s);
+  free(sy7); free(gx); free(ta1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* finite-difference primitives, 8_derivadas.f90:1-68,304-455 */
+
+/* centre -> face along x with end faces copied, (dimx,dimy,dimz)->(dimx+1,...) */
+static void interpx_cf(const double *a, int dx_, int dy_, int dz_, double *o) {
+  for (int k = 1; k <= dz_; k++)
+    for (int j = 1; j <= dy_; j++) {
+      for (int i = 2; i <= dx_; i++)
+        o[IA(i, j, k, dx_ + 1, dy_)] = (a[IA(i, j, k, dx_, dy_)] + a[IA(i - 1, j, k, dx_, dy_)]) * 0.5;
+      o[IA(1, j, k, dx_ + 1, dy_)] = o[IA(2, j, k, dx_ + 1, dy_)];
+      o[IA(dx_ + 1, j, k, dx_ + 1, dy_)] = o[IA(dx_, j, k, dx_ + 1, dy_)];
+    }
+}
+static void interpy_cf(const double *a, int dx_, int dy_, int dz_, double *o) {
+  for (int k = 1; k <= dz_; k++)
+    for (int j = 2; j <= dy_; j++)
+      for (int i = 1; i <= dx_; i++)
+        o[IA(i, j, k, dx_, dy_ + 1)] = (a[IA(i, j, k, dx_, dy_)] + a[IA(i, j - 1, k, dx_, dy_)]) * 0.5;
+  for (int k = 1; k <= dz_; k++)
+    for (int i = 1; i <= dx_; i++) {
+      o[IA(i, 1, k, dx_, dy_ + 1)] = o[IA(i, 2, k, dx_, dy_ + 1)];
+      o[IA(i, dy_ + 1, k, dx_, dy_ + 1)] = o[IA(i, dy_, k, dx_, dy_ + 1)];
+    }
+}
+static void interpz_cf(const double *a, int dx_, int dy_, int dz_, double *o) {
+  for (int k = 2; k <= dz_; k++)
+    for (int j = 1; j <= dy_; j++)
+      for (int i = 1; i <= dx_; i++)
+        o[IA(i, j, k, dx_, dy_)] = (a[IA(i, j, k, dx_, dy_)] + a[IA(i, j, k - 1, dx_, dy_)]) * 0.5;
+  for (int j = 1; j <= dy_; j++)
+    for (int i = 1; i <= dx_; i++) {
+      o[IA(i, j, 1, dx_, dy_)] = o[IA(i, j, 2, dx_, dy_)];
+      o[IA(i, j, dz_ + 1, dx_, dy_)] = o[IA(i, j, dz_, dx_, dy_)];
+    }
+}
+/* face -> centre, (dimx,...) -> (dimx-1,...) etc. */
+static void interpx_fc(const double *a, int dx_, int dy_, int dz_, double *o) {
+  for (int k = 1; k <= dz_; k++)
+    for (int j = 1; j <= dy_; j++)
+      for (int i = 1; i <= dx_ - 1; i++)
+        o[IA(i, j, k, dx_ - 1, dy_)] = (a[IA(i, j, k, dx_, dy_)] + a[IA(i + 1, j, k, dx_, dy_)]) * 0.5;
+}
+static void interpy_fc(const double *a, int dx_, int dy_, int dz_, double *o) {
+  for (int k = 1; k <= dz_; k++)
+    for (int j = 1; j <= dy_ - 1; j++)
+      for (int i = 1; i <= dx_; i++)
+        o[IA(i, j, k, dx_, dy_ - 1)] = (a[IA(i, j, k, dx_, dy_)] + a[IA(i, j + 1, k, dx_, dy_)]) * 0.5;
+}
+static void interpz_fc(const double *a, int dx_, int dy_, int dz_, double *o) {
+  for (int k = 1; k <= dz_ - 1; k++)
+    for (int j = 1; j <= dy_; j++)
+      for (int i = 1; i <= dx_; i++)
+        o[IA(i, j, k, dx_, dy_)] = (a[IA(i, j, k, dx_, dy_)] + a[IA(i, j, k + 1, dx_, dy_)]) * 0.5;
+}
+/* central derivatives of a ghosted (0:dimx+1,0:dimy+1,0:dimz+1) array */
+#define IG(i, j, k, dx_, dy_) ((int64_t)(i) + (int64_t)((dx_) + 2) * ((int64_t)(j) + (int64_t)((dy_) + 2) * (int64_t)(k)))
+static void derivax(const double *a, int dx_, int dy_, int dz_, double h, double *o) {
+  for (int k = 1; k <= dz_; k++)
+    for (int j = 1; j <= dy_; j++)
+      for (int i = 1; i <= dx_; i++)
+        o[IA(i, j, k, dx_, dy_)] = (a[IG(i + 1, j, k, dx_, dy_)] - a[IG(i - 1, j, k, dx_, dy_)]) / (2. * h);
+}
+static void derivay(const double *a, int dx_, int dy_, int dz_, double h, double *o) {
+  for (int k = 1; k <= dz_; k++)
+    for (int j = 1; j <= dy_; j++)
+      for (int i = 1; i <= dx_; i++)
+        o[IA(i, j, k, dx_, dy_)] = (a[IG(i, j + 1, k, dx_, dy_)] - a[IG(i, j - 1, k, dx_, dy_)]) / (2. * h);
+}
+static void derivaz(const double *a, int dx_, int dy_, int dz_, double h, double *o) {
+  for (int k = 1; k <= dz_; k++)
+    for (int j = 1; j <= dy_; j++)
+      for (int i = 1; i <= dx_; i++)
+        o[IA(i, j, k, dx_, dy_)] = (a[IG(i, j, k + 1, dx_, dy_)] - a[IG(i, j, k - 1, dx_, dy_)]) / (2. * h);
+}
+
+static double *alloc_n(int64_t n) { return (double *)calloc((size_t)n, sizeof(double)); }
+
+/* visco, 5_viscosidade.f90:12-219 (m_turb 0 or 1) */
+void or_visco(or_state *s) {
+  DIMS;
+  const int nx1 = nx + 1, ny1 = ny + 1, nz1 = nz + 1;
+  const int64_t C = (int64_t)nx * ny * nz, FX = (int64_t)nx1 * ny * nz, FY = (int64_t)nx * ny1 * nz,
+                FZ = (int64_t)nx * ny * nz1;
+  double *xnut_a = alloc_n(FX), *ynut_a = alloc_n(FY), *znut_a = alloc_n(FZ);
+  if (s->m_turb == 1) {
+    double *dudx_i = alloc_n(FX), *dudy_i = alloc_n(FX), *dudz_i = alloc_n(FX);
+    double *dvdx_i = alloc_n(FY), *dvdy_i = alloc_n(FY), *dvdz_i = alloc_n(FY);
+    double *dwdx_i = alloc_n(FZ), *dwdy_i = alloc_n(FZ), *dwdz_i = alloc_n(FZ);
+    derivax(s->u, nx1, ny, nz, s->dx, dudx_i); derivax(s->v, nx, ny1, nz, s->dx, dvdx_i); derivax(s->w, nx, ny, nz1, s->dx, dwdx_i);
+    derivay(s->u, nx1, ny, nz, s->dy, dudy_i); derivay(s->v, nx, ny1, nz, s->dy, dvdy_i); derivay(s->w, nx, ny, nz1, s->dy, dwdy_i);
+    derivaz(s->u, nx1, ny, nz, s->dz, dudz_i); derivaz(s->v, nx, ny1, nz, s->dz, dvdz_i); derivaz(s->w, nx, ny, nz1, s->dz, dwdz_i);
+    double *dudx = alloc_n(C), *dudy = alloc_n(C), *dudz = alloc_n(C);
+    double *dvdx = alloc_n(C), *dvdy = alloc_n(C), *dvdz = alloc_n(C);
+    double *dwdx = alloc_n(C), *dwdy = alloc_n(C), *dwdz = alloc_n(C);
+    interpx_fc(dudx_i, nx1, ny, nz, dudx); interpx_fc(dudy_i, nx1, ny, nz, dudy); interpx_fc(dudz_i, nx1, ny, nz, dudz);
+    interpy_fc(dvdx_i, nx, ny1, nz, dvdx); interpy_fc(dvdy_i, nx, ny1, nz, dvdy); interpy_fc(dvdz_i, nx, ny1, nz, dvdz);
+    interpz_fc(dwdx_i, nx, ny, nz1, dwdx); interpz_fc(dwdy_i, nx, ny, nz1, dwdy); interpz_fc(dwdz_i, nx, ny, nz1, dwdz);
+    const double c23 = pow(s->dx * s->dy * s->dz, 2. / 3.);
+    for (int64_t c = 0; c < C; c++) {
+      double p1 = dudx[c] + 0.5 * (dudy[c] + dvdx[c] + dudz[c] + dwdx[c]);
+      double p2 = dvdy[c] + 0.5 * (dvdx[c] + dudy[c] + dvdz[c] + dwdy[c]);
+      double p3 = dwdz[c] + 0.5 * (dwdy[c] + dvdz[c] + dudz[c] + dwdx[c]);
+      s->nut[c] = c23 * s->csmag * s->csmag * sqrt(2. * (p1 + p2 + p3) * (p1 + p2 + p3)) * s->rho[c];
+    }
+    interpx_cf(s->nut, nx, ny, nz, xnut_a);
+    interpy_cf(s->nut, nx, ny, nz, ynut_a);
+    interpz_cf(s->nut, nx, ny, nz, znut_a);
+    free(dudx_i); free(dudy_i); free(dudz_i); free(dvdx_i); free(dvdy_i); free(dvdz_i);
+    free(dwdx_i); free(dwdy_i); free(dwdz_i);
+    free(dudx); free(dudy); free(dudz); free(dvdx); free(dvdy); free(dvdz); free(dwdx); free(dwdy); free(dwdz);
+  }
+  double *lx = alloc_n(FX), *ly = alloc_n(FY), *lz = alloc_n(FZ);
+  interpx_cf(s->ls_nu, nx, ny, nz, lx);
+  interpy_cf(s->ls_nu, nx, ny, nz, ly);
+  interpz_cf(s->ls_nu, nx, ny, nz, lz);
+  for (int64_t c = 0; c < FX; c++) s->xnut[c] = lx[c] + xnut_a[c];
+  for (int64_t c = 0; c < FY; c++) s->ynut[c] = ly[c] + ynut_a[c];
+  for (int64_t c = 0; c < FZ; c++) s->znut[c] = lz[c] + znut_a[c];
+  free(lx); free(ly); free(lz); free(xnut_a); free(ynut_a); free(znut_a);
+}
+
+/* classico, der = 1 (first-order upwind), 7_advectivo.f90:33-113 */
+void or_classico(const or_state *s, double *uint_, double *vint_, double *wint_) {
+  DIMS;
+  const double *u = s->u, *v = s->v, *w = s->w;
+  const double dx = s->dx, dy = s->dy, dz = s->dz;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx + 1; i++) {
+        double bma = (v[IV(i, j, k)] + v[IV(i - 1, j, k)] + v[IV(i, j + 1, k)] + v[IV(i - 1, j + 1, k)]) * 0.25;
+        double dma = (w[IW(i, j, k)] + w[IW(i - 1, j, k)] + w[IW(i, j, k + 1)] + w[IW(i - 1, j, k + 1)]) * 0.25;
+        double a1 = fmax_f(u[IU(i, j, k)], 0.), a2 = fmin_f(u[IU(i, j, k)], 0.);
+        double dudx = a1 * (u[IU(i, j, k)] - u[IU(i - 1, j, k)]) / dx + a2 * (u[IU(i + 1, j, k)] - u[IU(i, j, k)]) / dx;
+        a1 = fmax_f(bma, 0.); a2 = fmin_f(bma, 0.);
+        double dudy = a1 * (u[IU(i, j, k)] - u[IU(i, j - 1, k)]) / dy + a2 * (u[IU(i, j + 1, k)] - u[IU(i, j, k)]) / dy;
+        a1 = fmax_f(dma, 0.); a2 = fmin_f(dma, 0.);
+        double dudz = a1 * (u[IU(i, j, k)] - u[IU(i, j, k - 1)]) / dz + a2 * (u[IU(i, j, k + 1)] - u[IU(i, j, k)]) / dz;
+        uint_[IFX(i, j, k)] = dudx + dudy + dudz;
+      }
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny + 1; j++)
+      for (int i = 1; i <= nx; i++) {
+        double amb = (u[IU(i, j, k)] + u[IU(i + 1, j, k)] + u[IU(i, j - 1, k)] + u[IU(i + 1, j - 1, k)]) * 0.25;
+        double dmb = (w[IW(i, j, k)] + w[IW(i, j - 1, k)] + w[IW(i, j, k + 1)] + w[IW(i, j - 1, k + 1)]) * 0.25;
+        double a1 = fmax_f(amb, 0.), a2 = fmin_f(amb, 0.);
+        double dvdx = a1 * (v[IV(i, j, k)] - v[IV(i - 1, j, k)]) / dx + a2 * (v[IV(i + 1, j, k)] - v[IV(i, j, k)]) / dx;
+        a1 = fmax_f(v[IV(i, j, k)], 0.); a2 = fmin_f(v[IV(i, j, k)], 0.);
+        double dvdy = a1 * (v[IV(i, j, k)] - v[IV(i, j - 1, k)]) / dy + a2 * (v[IV(i, j + 1, k)] - v[IV(i, j, k)]) / dy;
+        a1 = fmax_f(dmb, 0.); a2 = fmin_f(dmb, 0.);
+        double dvdz = a1 * (v[IV(i, j, k)] - v[IV(i, j, k - 1)]) / dz + a2 * (v[IV(i, j, k + 1)] - v[IV(i, j, k)]) / dz;
+        vint_[IFY(i, j, k)] = dvdx + dvdy + dvdz;
+      }
+  for (int k = 1; k <= nz + 1; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        double amd = (u[IU(i, j, k)] + u[IU(i + 1, j, k)] + u[IU(i, j, k - 1)] + u[IU(i + 1, j, k - 1)]) * 0.25;
+        double bmd = (v[IV(i, j, k)] + v[IV(i, j + 1, k)] + v[IV(i, j, k - 1)] + v[IV(i, j + 1, k - 1)]) * 0.25;
+        double a1 = fmax_f(amd, 0.), a2 = fmin_f(amd, 0.);
+        double dwdx = a1 * (w[IW(i, j, k)] - w[IW(i - 1, j, k)]) / dx + a2 * (w[IW(i + 1, j, k)] - w[IW(i, j, k)]) / dx;
+        a1 = fmax_f(bmd, 0.); a2 = fmin_f(bmd, 0.);
+        double dwdy = a1 * (w[IW(i, j, k)] - w[IW(i, j - 1, k)]) / dy + a2 * (w[IW(i, j + 1, k)] - w[IW(i, j, k)]) / dy;
+        a1 = fmax_f(w[IW(i, j, k)], 0.); a2 = fmin_f(w[IW(i, j, k)], 0.);
+        double dwdz = a1 * (w[IW(i, j, k)] - w[IW(i, j, k - 1)]) / dz + a2 * (w[IW(i, j, k + 1)] - w[IW(i, j, k)]) / dz;
+        wint_[IFZ(i, j, k)] = dwdx + dwdy + dwdz;
+      }
+}
+
+/* convdiff, 6_convdiff.f90:14-189 (adv_type = 1, no sponge layer) */
+void or_convdiff(or_state *s) {
+  DIMS;
+  const int nx1 = nx + 1, ny1 = ny + 1, nz1 = nz + 1;
+  const int64_t C = (int64_t)nx * ny * nz, FX = (int64_t)nx1 * ny * nz, FY = (int64_t)nx * ny1 * nz,
+                FZ = (int64_t)nx * ny * nz1;
+  const double dx = s->dx, dy = s->dy, dz = s->dz;
+  const double *u = s->u, *v = s->v, *w = s->w;
+  double *uint_ = alloc_n(FX), *vint_ = alloc_n(FY), *wint_ = alloc_n(FZ);
+  or_classico(s, uint_, vint_, wint_);
+  double *rhox = alloc_n(FX), *rhoy = alloc_n(FY), *rhoz = alloc_n(FZ);
+  interpx_cf(s->rho, nx, ny, nz, rhox);
+  interpy_cf(s->rho, nx, ny, nz, rhoy);
+  interpz_cf(s->rho, nx, ny, nz, rhoz);
+  double *tmp = alloc_n(C);
+  double *dhsdx = alloc_n(FX), *dhsdy = alloc_n(FY), *dhsdz = alloc_n(FZ);
+  for (int64_t c = 0; c < C; c++) tmp[c] = s->hsx[c] * s->kurv[c] / s->rho[c];
+  interpx_cf(tmp, nx, ny, nz, dhsdx);
+  for (int64_t c = 0; c < C; c++) tmp[c] = s->hsy[c] * s->kurv[c] / s->rho[c];
+  interpy_cf(tmp, nx, ny, nz, dhsdy);
+  for (int64_t c = 0; c < C; c++) tmp[c] = s->hsz[c] * s->kurv[c] / s->rho[c];
+  interpz_cf(tmp, nx, ny, nz, dhsdz);
+  /* edge viscosities, :118-145 */
+  int64_t EXY = (int64_t)nx1 * ny1 * nz, EXZ = (int64_t)nx1 * ny * nz1, EYZ = (int64_t)nx * ny1 * nz1;
+  double *bma = alloc_n(EXY), *dma = alloc_n(EXZ), *amb = alloc_n(EXY), *dmb = alloc_n(EYZ);
+  double *amd = alloc_n(EXZ), *bmd = alloc_n(EYZ);
+  interpy_cf(s->xnut, nx1, ny, nz, bma);
+  interpz_cf(s->xnut, nx1, ny, nz, dma);
+  interpx_cf(s->ynut, nx, ny1, nz, amb);
+  interpz_cf(s->ynut, nx, ny1, nz, dmb);
+  interpx_cf(s->znut, nx, ny, nz1, amd);
+  interpy_cf(s->znut, nx, ny, nz1, bmd);
+  for (int64_t c = 0; c < EXY; c++) { bma[c] = bma[c] / dy; amb[c] = amb[c] / dx; }
+  for (int64_t c = 0; c < EXZ; c++) { dma[c] = dma[c] / dz; amd[c] = amd[c] / dx; }
+  for (int64_t c = 0; c < EYZ; c++) { dmb[c] = dmb[c] / dz; bmd[c] = bmd[c] / dy; }
+  /* ama(0:nx1,ny,nz), bmb(nx,0:ny1,nz), dmd(nx,ny,0:nz1) */
+#define AMA(i, j, k) ama[(int64_t)(i) + (int64_t)(nx + 2) * ((int64_t)((j)-1) + (int64_t)ny * (int64_t)((k)-1))]
+#define BMB(i, j, k) bmb[(int64_t)((i)-1) + (int64_t)nx * ((int64_t)(j) + (int64_t)(ny + 2) * (int64_t)((k)-1))]
+#define DMD(i, j, k) dmd[(int64_t)((i)-1) + (int64_t)nx * ((int64_t)((j)-1) + (int64_t)ny * (int64_t)(k))]
+  double *ama = alloc_n((int64_t)(nx + 2) * ny * nz), *bmb = alloc_n((int64_t)nx * (ny + 2) * nz),
+         *dmd = alloc_n((int64_t)nx * ny * (nz + 2));
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        AMA(i, j, k) = s->nut[IC(i, j, k)] / dx;
+        BMB(i, j, k) = s->nut[IC(i, j, k)] / dy;
+        DMD(i, j, k) = s->nut[IC(i, j, k)] / dz;
+      }
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) { AMA(0, j, k) = AMA(1, j, k); AMA(nx1, j, k) = AMA(nx, j, k); }
+  for (int k = 1; k <= nz; k++)
+    for (int i = 1; i <= nx; i++) { BMB(i, 0, k) = BMB(i, 1, k); BMB(i, ny1, k) = BMB(i, ny, k); }
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) { DMD(i, j, 0) = DMD(i, j, 1); DMD(i, j, nz1) = DMD(i, j, nz); }
+#define EXYI(i, j, k) IA(i, j, k, nx1, ny1)
+#define EXZI(i, j, k) IA(i, j, k, nx1, ny)
+#define EYZI(i, j, k) IA(i, j, k, nx, ny1)
+  const double fr = s->gz / (s->chezy * s->chezy);
+  for (int k = 1; k <= nz; k++) {
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx1; i++) {
+        double tfu = s->tf_u ? s->tf_u[IFX(i, j, k)] : 0.;
+        double ubv = s->ub[IU(i, j, k)];
+        s->Fu[IFX(i, j, k)] =
+            -uint_[IFX(i, j, k)] +
+            ((AMA(i, j, k) * 2. * (u[IU(i + 1, j, k)] - u[IU(i, j, k)]) -
+              AMA(i - 1, j, k) * 2. * (u[IU(i, j, k)] - u[IU(i - 1, j, k)])) / dx +
+             bma[EXYI(i, j + 1, k)] * ((u[IU(i, j + 1, k)] - u[IU(i, j, k)]) / dy + (v[IV(i, j + 1, k)] - v[IV(i - 1, j + 1, k)]) / dx) -
+             bma[EXYI(i, j, k)] * ((u[IU(i, j, k)] - u[IU(i, j - 1, k)]) / dy + (v[IV(i, j, k)] - v[IV(i - 1, j, k)]) / dx) +
+             dma[EXZI(i, j, k + 1)] * ((u[IU(i, j, k + 1)] - u[IU(i, j, k)]) / dz + (w[IW(i, j, k + 1)] - w[IW(i - 1, j, k + 1)]) / dx) -
+             dma[EXZI(i, j, k)] * ((u[IU(i, j, k)] - u[IU(i, j, k - 1)]) / dz + (w[IW(i, j, k)] - w[IW(i - 1, j, k)]) / dx)) /
+                rhox[IFX(i, j, k)] +
+            s->gx - s->sigma * dhsdx[IFX(i, j, k)] + tfu - fr * sqrt(ubv * ubv) * u[IU(i, j, k)];
+      }
+    for (int j = 1; j <= ny1; j++)
+      for (int i = 1; i <= nx; i++) {
+        double tfv = s->tf_v ? s->tf_v[IFY(i, j, k)] : 0.;
+        double vbv = s->vb[IV(i, j, k)];
+        s->Fv[IFY(i, j, k)] =
+            -vint_[IFY(i, j, k)] +
+            (amb[EXYI(i + 1, j, k)] * ((v[IV(i + 1, j, k)] - v[IV(i, j, k)]) / dx + (u[IU(i + 1, j, k)] - u[IU(i + 1, j - 1, k)]) / dy) -
+             amb[EXYI(i, j, k)] * ((v[IV(i, j, k)] - v[IV(i - 1, j, k)]) / dx + (u[IU(i, j, k)] - u[IU(i, j - 1, k)]) / dy) +
+             (BMB(i, j, k) * 2. * (v[IV(i, j + 1, k)] - v[IV(i, j, k)]) -
+              BMB(i, j - 1, k) * 2. * (v[IV(i, j, k)] - v[IV(i, j - 1, k)])) / dy +
+             dmb[EYZI(i, j, k + 1)] * ((v[IV(i, j, k + 1)] - v[IV(i, j, k)]) / dz + (w[IW(i, j, k + 1)] - w[IW(i, j - 1, k + 1)]) / dy) -
+             dmb[EYZI(i, j, k)] * ((v[IV(i, j, k)] - v[IV(i, j, k - 1)]) / dz + (w[IW(i, j, k)] - w[IW(i, j - 1, k)]) / dy)) /
+                rhoy[IFY(i, j, k)] -
+            s->sigma * dhsdy[IFY(i, j, k)] + tfv - fr * sqrt(vbv * vbv) * v[IV(i, j, k)];
+      }
+  }
+  for (int k = 1; k <= nz1; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        double tfw = s->tf_w ? s->tf_w[IFZ(i, j, k)] : 0.;
+        double wbv = s->wb[IW(i, j, k)];
+        const double epis_z = 0.;
+        s->Fw[IFZ(i, j, k)] =
+            -wint_[IFZ(i, j, k)] +
+            (amd[EXZI(i + 1, j, k)] * ((w[IW(i + 1, j, k)] - w[IW(i, j, k)]) / dx + (u[IU(i + 1, j, k)] - u[IU(i + 1, j, k - 1)]) / dz) -
+             amd[EXZI(i, j, k)] * ((w[IW(i, j, k)] - w[IW(i - 1, j, k)]) / dx + (u[IU(i, j, k)] - u[IU(i, j, k - 1)]) / dz) +
+             bmd[EYZI(i, j + 1, k)] * ((w[IW(i, j + 1, k)] - w[IW(i, j, k)]) / dy + (v[IV(i, j + 1, k)] - v[IV(i, j + 1, k - 1)]) / dz) -
+             bmd[EYZI(i, j, k)] * ((w[IW(i, j, k)] - w[IW(i, j - 1, k)]) / dy + (v[IV(i, j, k)] - v[IV(i, j, k - 1)]) / dz) +
+             (DMD(i, j, k) * 2. * (w[IW(i, j, k + 1)] - w[IW(i, j, k)]) -
+              DMD(i, j, k - 1) * 2. * (w[IW(i, j, k)] - w[IW(i, j, k - 1)])) / dz) /
+                rhoz[IFZ(i, j, k)] -
+            (w[IW(i, j, k)] - 0.) * epis_z - s->gz - s->sigma * dhsdz[IFZ(i, j, k)] + tfw -
+            fr * sqrt(wbv * wbv) * w[IW(i, j, k)];
+      }
+#undef AMA
+#undef BMB
+#undef DMD
+  free(uint_); free(vint_); free(wint_); free(rhox); free(rhoy); free(rhoz); free(tmp);
+  free(dhsdx); free(dhsdy); free(dhsdz); free(bma); free(dma); free(amb); free(dmb); free(amd); free(bmd);
+  free(ama); free(bmb); free(dmd);
+}
+
+/* tempo, RK2 branch, 11_diversos.f90:54-111 */
+void or_tempo(or_state *s) {
+  DIMS;
+  const double aux1 = s->dt;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx + 1; i++) {
+        if (s->tt == 1) {
+          s->u0[IFX(i, j, k)] = s->u[IU(i, j, k)];
+          s->u[IU(i, j, k)] = s->u[IU(i, j, k)] + aux1 * s->Fu[IFX(i, j, k)];
+        } else {
+          s->u[IU(i, j, k)] = s->u0[IFX(i, j, k)] + aux1 * s->Fu[IFX(i, j, k)];
+        }
+      }
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny + 1; j++)
+      for (int i = 1; i <= nx; i++) {
+        if (s->tt == 1) {
+          s->v0[IFY(i, j, k)] = s->v[IV(i, j, k)];
+          s->v[IV(i, j, k)] = s->v[IV(i, j, k)] + aux1 * s->Fv[IFY(i, j, k)];
+        } else {
+          s->v[IV(i, j, k)] = s->v0[IFY(i, j, k)] + aux1 * s->Fv[IFY(i, j, k)];
+        }
+      }
+  for (int k = 1; k <= nz + 1; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        if (s->tt == 1) {
+          s->w0[IFZ(i, j, k)] = s->w[IW(i, j, k)];
+          s->w[IW(i, j, k)] = s->w[IW(i, j, k)] + aux1 * s->Fw[IFZ(i, j, k)];
+        } else {
+          s->w[IW(i, j, k)] = s->w0[IFZ(i, j, k)] + aux1 * s->Fw[IFZ(i, j, k)];
+        }
+      }
+}
+
+/* contorno, 3_contorno.f90:29-443 with prd_corr (:872-950, every dp ≡ 0);
+ * wall codes 0 (periodic), 1 (free slip), 2 (no slip), 4 (outflow at xf);
+ * obst_t = 0, so nlock = 3 does nothing. */
+void or_contorno(or_state *s, int nlock) {
+  DIMS;
+  const int nx1 = nx + 1, ny1 = ny + 1, nz1 = nz + 1;
+  double *u = s->u, *v = s->v, *w = s->w, *p = s->prd1;
+  int i, j, k;
+  if (nlock == 1 || nlock == 2) {
+    /* pressure ghosts, :37-54 */
+    for (k = 0; k <= nz + 1; k++)
+      for (j = 0; j <= ny + 1; j++) {
+        if (s->ccx0 == 0 && s->ccxf == 0) { p[IP(0, j, k)] = p[IP(nx, j, k)]; p[IP(nx + 1, j, k)] = p[IP(1, j, k)]; }
+        else { p[IP(0, j, k)] = p[IP(1, j, k)]; p[IP(nx + 1, j, k)] = p[IP(nx, j, k)]; }
+      }
+    for (k = 0; k <= nz + 1; k++)
+      for (i = 0; i <= nx + 1; i++) {
+        if (s->ccy0 == 0 && s->ccyf == 0) { p[IP(i, 0, k)] = p[IP(i, ny, k)]; p[IP(i, ny + 1, k)] = p[IP(i, 1, k)]; }
+        else { p[IP(i, 0, k)] = p[IP(i, 1, k)]; p[IP(i, ny + 1, k)] = p[IP(i, ny, k)]; }
+      }
+    for (j = 0; j <= ny + 1; j++)
+      for (i = 0; i <= nx + 1; i++) { p[IP(i, j, 0)] = p[IP(i, j, 1)]; p[IP(i, j, nz + 1)] = p[IP(i, j, nz)]; }
+    /* y = 0 wall, :58-85 */
+    if (s->ccy0 == 0 && s->ccyf == 0) {
+      for (k = 0; k <= nz + 1; k++) {
+        for (i = 0; i <= nx1 + 1; i++) u[IU(i, 0, k)] = u[IU(i, ny, k)];
+        for (i = 0; i <= nx + 1; i++) { v[IV(i, 0, k)] = v[IV(i, ny1 - 1, k)]; v[IV(i, 1, k)] = v[IV(i, ny1, k)]; }
+      }
+      for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) w[IW(i, 0, k)] = w[IW(i, ny, k)];
+    }
+    if (s->ccy0 == 1 || s->ccy0 == 2) {
+      double sg = (s->ccy0 == 1) ? 1. : -1.;
+      for (k = 0; k <= nz + 1; k++) {
+        for (i = 0; i <= nx1 + 1; i++) u[IU(i, 0, k)] = (s->ccy0 == 1) ? u[IU(i, 1, k)] : -u[IU(i, 1, k)];
+        for (i = 0; i <= nx + 1; i++) v[IV(i, 0, k)] = (s->ccy0 == 1) ? -v[IV(i, 2, k)] : v[IV(i, 2, k)];
+      }
+      for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) w[IW(i, 0, k)] = (s->ccy0 == 1) ? w[IW(i, 1, k)] : -w[IW(i, 1, k)];
+      (void)sg;
+    }
+    /* y = yf wall, :87-114 */
+    if (s->ccy0 == 0 && s->ccyf == 0) {
+      for (k = 0; k <= nz + 1; k++) {
+        for (i = 0; i <= nx1 + 1; i++) u[IU(i, ny + 1, k)] = u[IU(i, 1, k)];
+        for (i = 0; i <= nx + 1; i++) v[IV(i, ny1 + 1, k)] = v[IV(i, 2, k)];
+      }
+      for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) w[IW(i, ny + 1, k)] = w[IW(i, 1, k)];
+    }
+    if (s->ccyf == 1 || s->ccyf == 2) {
+      for (k = 0; k <= nz + 1; k++) {
+        for (i = 0; i <= nx1 + 1; i++) u[IU(i, ny + 1, k)] = (s->ccyf == 1) ? u[IU(i, ny, k)] : -u[IU(i, ny, k)];
+        for (i = 0; i <= nx + 1; i++) v[IV(i, ny1 + 1, k)] = (s->ccyf == 1) ? -v[IV(i, ny1 - 1, k)] : v[IV(i, ny1 - 1, k)];
+      }
+      for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) w[IW(i, ny + 1, k)] = (s->ccyf == 1) ? w[IW(i, ny, k)] : -w[IW(i, ny, k)];
+    }
+    /* x = 0 wall, :116-161 */
+    if (s->ccx0 == 0 && s->ccxf == 0) {
+      for (k = 0; k <= nz + 1; k++) {
+        for (j = 0; j <= ny + 1; j++) u[IU(0, j, k)] = u[IU(nx1 - 1, j, k)];
+        for (j = 0; j <= ny1 + 1; j++) v[IV(0, j, k)] = v[IV(nx, j, k)];
+      }
+      for (k = 0; k <= nz1 + 1; k++) for (j = 0; j <= ny + 1; j++) w[IW(0, j, k)] = w[IW(nx, j, k)];
+    }
+    if (s->ccx0 == 1 || s->ccx0 == 2) {
+      for (k = 0; k <= nz + 1; k++) {
+        for (j = 0; j <= ny + 1; j++) u[IU(0, j, k)] = (s->ccx0 == 1) ? -u[IU(2, j, k)] : u[IU(2, j, k)];
+        for (j = 0; j <= ny1 + 1; j++) v[IV(0, j, k)] = (s->ccx0 == 1) ? v[IV(1, j, k)] : -v[IV(1, j, k)];
+      }
+      for (k = 0; k <= nz1 + 1; k++) for (j = 0; j <= ny + 1; j++) w[IW(0, j, k)] = (s->ccx0 == 1) ? w[IW(1, j, k)] : -w[IW(1, j, k)];
+    }
+    /* x = xf wall, :163-197 */
+    if (s->ccx0 == 0 && s->ccxf == 0) {
+      for (k = 0; k <= nz + 1; k++) {
+        for (j = 0; j <= ny + 1; j++) u[IU(nx1 + 1, j, k)] = u[IU(2, j, k)];
+        for (j = 0; j <= ny1 + 1; j++) v[IV(nx + 1, j, k)] = v[IV(1, j, k)];
+      }
+      for (k = 0; k <= nz1 + 1; k++) for (j = 0; j <= ny + 1; j++) w[IW(nx + 1, j, k)] = w[IW(1, j, k)];
+    }
+    if (s->ccxf == 1 || s->ccxf == 2 || s->ccxf == 4) {
+      for (k = 0; k <= nz + 1; k++) {
+        for (j = 0; j <= ny + 1; j++)
+          u[IU(nx1 + 1, j, k)] = (s->ccxf == 1) ? -u[IU(nx1 - 1, j, k)] : (s->ccxf == 2) ? u[IU(nx1 - 1, j, k)] : u[IU(nx1, j, k)];
+        for (j = 0; j <= ny1 + 1; j++) v[IV(nx + 1, j, k)] = (s->ccxf == 2) ? -v[IV(nx, j, k)] : v[IV(nx, j, k)];
+      }
+      for (k = 0; k <= nz1 + 1; k++)
+        for (j = 0; j <= ny + 1; j++) w[IW(nx + 1, j, k)] = (s->ccxf == 2) ? -w[IW(nx, j, k)] : w[IW(nx, j, k)];
+    }
+    /* bottom, :199-219 */
+    if (s->ccz0 == 1 || s->ccz0 == 2) {
+      for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx1 + 1; i++) u[IU(i, j, 0)] = (s->ccz0 == 1) ? u[IU(i, j, 1)] : -u[IU(i, j, 1)];
+      for (j = 0; j <= ny1 + 1; j++) for (i = 0; i <= nx + 1; i++) v[IV(i, j, 0)] = (s->ccz0 == 1) ? v[IV(i, j, 1)] : -v[IV(i, j, 1)];
+      for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, 0)] = (s->ccz0 == 1) ? -w[IW(i, j, 2)] : w[IW(i, j, 2)];
+    }
+    /* top, :221-241 */
+    if (s->cczf == 1 || s->cczf == 2) {
+      for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx1 + 1; i++) u[IU(i, j, nz + 1)] = (s->cczf == 1) ? u[IU(i, j, nz)] : -u[IU(i, j, nz)];
+      for (j = 0; j <= ny1 + 1; j++) for (i = 0; i <= nx + 1; i++) v[IV(i, j, nz + 1)] = (s->cczf == 1) ? v[IV(i, j, nz)] : -v[IV(i, j, nz)];
+      for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, nz1 + 1)] = (s->cczf == 1) ? -w[IW(i, j, nz1 - 1)] : w[IW(i, j, nz1 - 1)];
+    }
+    /* friction planes, obst_t = 0, :247-264 */
+    if (s->ccz0 == 2) {
+      for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx1 + 1; i++) s->ub[IU(i, j, 2)] = u[IU(i, j, 2)];
+      for (j = 0; j <= ny1 + 1; j++) for (i = 0; i <= nx + 1; i++) s->vb[IV(i, j, 2)] = v[IV(i, j, 2)];
+      for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) s->wb[IW(i, j, 2)] = w[IW(i, j, 2)];
+    }
+    if (s->ccy0 == 2) {
+      for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx1 + 1; i++) s->ub[IU(i, 2, k)] = u[IU(i, 2, k)];
+      for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) s->vb[IV(i, 2, k)] = v[IV(i, 2, k)];
+      for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) s->wb[IW(i, 2, k)] = w[IW(i, 2, k)];
+    }
+    if (s->ccyf == 2) {
+      for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx1 + 1; i++) s->ub[IU(i, ny1 - 1, k)] = u[IU(i, ny1 - 1, k)];
+      for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) s->vb[IV(i, ny1 - 1, k)] = v[IV(i, ny1 - 1, k)];
+      for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) s->wb[IW(i, ny1 - 1, k)] = w[IW(i, ny1 - 1, k)];
+    }
+  }
+  if (nlock == 2) { /* wall-normal velocities, :297-443 */
+    if (s->ccy0 == 0 && s->ccyf == 0) {
+      for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) v[IV(i, 1, k)] = v[IV(i, ny1, k)];
+    }
+    if (s->ccy0 == 1 || s->ccy0 == 2) for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) v[IV(i, 1, k)] = 0. + 0.;
+    if (s->ccy0 == 0 && s->ccyf == 0) {
+      for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) v[IV(i, ny1, k)] = v[IV(i, 1, k)];
+    }
+    if (s->ccyf == 1 || s->ccyf == 2) for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) v[IV(i, ny1, k)] = 0. + 0.;
+    if (s->ccx0 == 0 && s->ccxf == 0) {
+      for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) u[IU(1, j, k)] = u[IU(nx1, j, k)];
+    }
+    if (s->ccx0 == 1 || s->ccx0 == 2) for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) u[IU(1, j, k)] = 0. + 0.;
+    if (s->ccx0 == 0 && s->ccxf == 0) {
+      for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) u[IU(nx1, j, k)] = u[IU(1, j, k)];
+    }
+    if (s->ccxf == 1) for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) u[IU(nx1, j, k)] = 0. + 0.;
+    if (s->ccxf == 2) {
+      for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) {
+        u[IU(nx1, j, k)] = 0. + 0.;
+        u[IU(nx1 + 1, j, k)] = u[IU(nx1 - 1, j, k)];
+      }
+      for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny1 + 1; j++) v[IV(nx + 1, j, k)] = -v[IV(nx, j, k)];
+      for (k = 0; k <= nz1 + 1; k++) for (j = 0; j <= ny + 1; j++) w[IW(nx + 1, j, k)] = -w[IW(nx, j, k)];
+    }
+    if (s->ccxf == 4) for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) u[IU(nx1, j, k)] = u[IU(nx1 - 1, j, k)];
+    if (s->ccz0 == 1 || s->ccz0 == 2) for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, 1)] = 0. + 0.;
+    if (s->cczf == 1 || s->cczf == 2) for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, nz1)] = 0. + 0.;
+  }
+}
+
+/* graddin, 9_grad.f90:10-278: symmetrically scaled CG on the
+ * variable-density pressure equation, warm-started from prd1, stopped when
+ * the sum of squared scaled residuals drops to 1e-4/(nx*ny*nz) or after
+ * 10000 iterations. */
+void or_graddin(or_state *s) {
+  DIMS;
+  const int nx1 = nx + 1, ny1 = ny + 1, nz1 = nz + 1;
+  const double dt = s->dt, dx = s->dx, dy = s->dy, dz = s->dz;
+  const int64_t FX = (int64_t)nx1 * ny * nz, FY = (int64_t)nx * ny1 * nz, FZ = (int64_t)nx * ny * nz1;
+  const int64_t C = (int64_t)nx * ny * nz, G = (int64_t)(nx + 2) * (ny + 2) * (nz + 2);
+  double *mi = alloc_n(FX), *mj = alloc_n(FY), *mk = alloc_n(FZ);
+  double *rhox = alloc_n(FX), *rhoy = alloc_n(FY), *rhoz = alloc_n(FZ);
+  double *q = alloc_n(C), *aip = alloc_n(C), *ain = alloc_n(C), *ajp = alloc_n(C), *ajn = alloc_n(C),
+         *akp = alloc_n(C), *akn = alloc_n(C);
+  double *d = alloc_n(G), *e = alloc_n(G), *r = alloc_n(G), *pp = alloc_n(G), *mp = alloc_n(G);
+  for (int64_t c = 0; c < FX; c++) mi[c] = dt / (dx * dx);
+  for (int64_t c = 0; c < FY; c++) mj[c] = dt / (dy * dy);
+  for (int64_t c = 0; c < FZ; c++) mk[c] = dt / (dz * dz);
+  interpx_cf(s->rho, nx, ny, nz, rhox);
+  interpy_cf(s->rho, nx, ny, nz, rhoy);
+  interpz_cf(s->rho, nx, ny, nz, rhoz);
+  for (int64_t c = 0; c < FX; c++) mi[c] = mi[c] / rhox[c];
+  for (int64_t c = 0; c < FY; c++) mj[c] = mj[c] / rhoy[c];
+  for (int64_t c = 0; c < FZ; c++) mk[c] = mk[c] / rhoz[c];
+  const double *u = s->u, *v = s->v, *w = s->w;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        d[IP(i, j, k)] = mi[IFX(i + 1, j, k)] + mi[IFX(i, j, k)] + mj[IFY(i, j + 1, k)] + mj[IFY(i, j, k)] +
+                         mk[IFZ(i, j, k + 1)] + mk[IFZ(i, j, k)];
+        q[IC(i, j, k)] = (u[IU(i, j, k)] - u[IU(i + 1, j, k)]) / dx + (v[IV(i, j, k)] - v[IV(i, j + 1, k)]) / dy +
+                         (w[IW(i, j, k)] - w[IW(i, j, k + 1)]) / dz;
+      }
+  /* mirror a ghosted vector (0:nx1,0:ny1,0:nz1), :98-116 */
+#define MIRROR(a)                                                                              \
+  do {                                                                                         \
+    for (int k_ = 0; k_ <= nz1; k_++)                                                          \
+      for (int j_ = 0; j_ <= ny1; j_++) {                                                      \
+        if (s->ccx0 == 0) { a[IP(0, j_, k_)] = a[IP(nx, j_, k_)]; a[IP(nx1, j_, k_)] = a[IP(1, j_, k_)]; } \
+        else { a[IP(0, j_, k_)] = a[IP(1, j_, k_)]; a[IP(nx1, j_, k_)] = a[IP(nx, j_, k_)]; }  \
+      }                                                                                        \
+    for (int k_ = 0; k_ <= nz1; k_++)                                                          \
+      for (int i_ = 0; i_ <= nx1; i_++) {                                                      \
+        if (s->ccy0 == 0) { a[IP(i_, 0, k_)] = a[IP(i_, ny, k_)]; a[IP(i_, ny1, k_)] = a[IP(i_, 1, k_)]; } \
+        else { a[IP(i_, 0, k_)] = a[IP(i_, 1, k_)]; a[IP(i_, ny1, k_)] = a[IP(i_, ny, k_)]; }  \
+      }                                                                                        \
+    for (int j_ = 0; j_ <= ny1; j_++)                                                          \
+      for (int i_ = 0; i_ <= nx1; i_++) { a[IP(i_, j_, 0)] = a[IP(i_, j_, 1)]; a[IP(i_, j_, nz1)] = a[IP(i_, j_, nz)]; } \
+  } while (0)
+  MIRROR(d);
+  for (int64_t c = 0; c < G; c++) e[c] = s->prd1[c] * sqrt(d[c]);
+  double alfamupr = 0.;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        int64_t c = IC(i, j, k), g = IP(i, j, k);
+        aip[c] = mi[IFX(i + 1, j, k)] / sqrt(d[g] * d[IP(i + 1, j, k)]);
+        ain[c] = mi[IFX(i, j, k)] / sqrt(d[g] * d[IP(i - 1, j, k)]);
+        ajp[c] = mj[IFY(i, j + 1, k)] / sqrt(d[g] * d[IP(i, j + 1, k)]);
+        ajn[c] = mj[IFY(i, j, k)] / sqrt(d[g] * d[IP(i, j - 1, k)]);
+        akp[c] = mk[IFZ(i, j, k + 1)] / sqrt(d[g] * d[IP(i, j, k + 1)]);
+        akn[c] = mk[IFZ(i, j, k)] / sqrt(d[g] * d[IP(i, j, k - 1)]);
+        r[g] = e[g] - aip[c] * e[IP(i + 1, j, k)] - ain[c] * e[IP(i - 1, j, k)] - ajp[c] * e[IP(i, j + 1, k)] -
+               ajn[c] * e[IP(i, j - 1, k)] - akp[c] * e[IP(i, j, k + 1)] - akn[c] * e[IP(i, j, k - 1)] -
+               q[c] / sqrt(d[g]);
+        alfamupr = alfamupr + r[g] * r[g];
+      }
+  MIRROR(r);
+  memcpy(pp, r, sizeof(double) * (size_t)G);
+  int cont = 0;
+  const double thr = 0.0001 / (nx * ny * nz);
+  while (fabs(alfamupr) > thr && cont < 10000) {
+    cont = cont + 1;
+    double alfadipr = 0., betamupr = 0.;
+    alfamupr = 0.;
+    memset(mp, 0, sizeof(double) * (size_t)G);
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+          int64_t c = IC(i, j, k), g = IP(i, j, k);
+          mp[g] = pp[g] - pp[IP(i + 1, j, k)] * aip[c] - pp[IP(i - 1, j, k)] * ain[c] - pp[IP(i, j + 1, k)] * ajp[c] -
+                  pp[IP(i, j - 1, k)] * ajn[c] - pp[IP(i, j, k + 1)] * akp[c] - pp[IP(i, j, k - 1)] * akn[c];
+          alfamupr = alfamupr + r[g] * r[g];
+          alfadipr = alfadipr + pp[g] * mp[g];
+        }
+    double alfapr = alfamupr / alfadipr;
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+          int64_t g = IP(i, j, k);
+          e[g] = e[g] - alfapr * pp[g];
+          r[g] = r[g] - alfapr * mp[g];
+          betamupr = betamupr + r[g] * r[g];
+        }
+    double betapr = betamupr / alfamupr;
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+          int64_t g = IP(i, j, k);
+          pp[g] = r[g] + betapr * pp[g];
+        }
+    MIRROR(e);
+    MIRROR(pp);
+  }
+  s->cg_iters = cont;
+  s->cg_res2 = alfamupr;
+  for (int64_t c = 0; c < G; c++) s->prd1[c] = e[c] / sqrt(d[c]);
+#undef MIRROR
+  free(mi); free(mj); free(mk); free(rhox); free(rhoy); free(rhoz);
+  free(q); free(aip); free(ain); free(ajp); free(ajn); free(akp); free(akn);
+  free(d); free(e); free(r); free(pp); free(mp);
+}
+
+/* posdin, 10_pos.f90:1-96 */
+void or_posdin(or_state *s) {
+  DIMS;
+  const int nx1 = nx + 1, ny1 = ny + 1, nz1 = nz + 1;
+  const double dt = s->dt;
+  const int64_t FX = (int64_t)nx1 * ny * nz, FY = (int64_t)nx * ny1 * nz, FZ = (int64_t)nx * ny * nz1;
+  double *rhox = alloc_n(FX), *rhoy = alloc_n(FY), *rhoz = alloc_n(FZ);
+  interpx_cf(s->rho, nx, ny, nz, rhox);
+  interpy_cf(s->rho, nx, ny, nz, rhoy);
+  interpz_cf(s->rho, nx, ny, nz, rhoz);
+  const double aux1 = dt / s->dx, aux2 = dt / s->dy;
+  double *p = s->prd1;
+  for (int k = 1; k <= nz; k++) {
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx1; i++)
+        s->u[IU(i, j, k)] = s->u[IU(i, j, k)] - aux1 * (p[IP(i, j, k)] - p[IP(i - 1, j, k)]) / rhox[IFX(i, j, k)];
+    for (int j = 1; j <= ny1; j++)
+      for (int i = 1; i <= nx; i++)
+        s->v[IV(i, j, k)] = s->v[IV(i, j, k)] - aux2 * (p[IP(i, j, k)] - p[IP(i, j - 1, k)]) / rhoy[IFY(i, j, k)];
+  }
+  for (int k = 1; k <= nz1; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++)
+        s->w[IW(i, j, k)] = s->w[IW(i, j, k)] - dt / s->dz * (p[IP(i, j, k)] - p[IP(i, j, k - 1)]) / rhoz[IFZ(i, j, k)];
+  free(rhox); free(rhoy); free(rhoz);
+}
+
+/* est, 4_plot.f90:500-525: max abs divergence */
+double or_est_div(const or_state *s) {
+  DIMS;
+  double m = 0.;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        double dv = (s->u[IU(i + 1, j, k)] - s->u[IU(i, j, k)]) / s->dx + (s->v[IV(i, j + 1, k)] - s->v[IV(i, j, k)]) / s->dy +
+                    (s->w[IW(i, j, k + 1)] - s->w[IW(i, j, k)]) / s->dz;
+        if (fabs(dv) > m) m = fabs(dv);
+      }
+  return m;
+}
+
+/* one RK2 stage of 1_principal.f90:77-95 */
+void or_stage(or_state *s, int tt, double dt_base) {
+  s->tt = tt;
+  s->dt = (tt == 1) ? 0.5 * dt_base : 1.0 * dt_base;
+  or_visco(s);
+  or_convdiff(s);
+  or_tempo(s);
+  or_contorno(s, 2);
+  or_graddin(s);
+  or_posdin(s);
+  or_contorno(s, 1);
+}
+
+/* one time step, 1_principal.f90:74-95 (level set with the full dt) */
+void or_step(or_state *s, double dt_base) {
+  s->dt = dt_base;
+  or_level_set(s);
+  or_contorno(s, 3);
+  for (int tt = 1; tt <= 2; tt++) or_stage(s, tt, dt_base);
+  s->dt = dt_base;
+}
