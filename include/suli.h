@@ -1,0 +1,157 @@
+/* suli.h - C ABI of the MI355X-native SuLi-IC time step (libsuli.so).
+ *
+ * Drop-in boundary for the solver loop of the reference Fortran driver
+ * (SuLi-IC/1_principal.f90:64-108).  Each entry point replaces one
+ * argument-less reference subroutine that works on module globals; here the
+ * state lives on one GPU, allocated by suli_init and freed by suli_finalize,
+ * and the host exchanges whole arrays in the reference's own extents and
+ * Fortran order (0_modules.f90:87-135,177-201) through suli_upload /
+ * suli_download.  The driver's mutable scalar dt is passed by value.
+ *
+ * Conventions: every function returns 0 on success and a nonzero status on
+ * error; the message is available from suli_last_error.  One host thread,
+ * one HIP stream; calls are serialised and the library is not re-entrant.
+ * Calls that return scalars synchronise the stream; the others may return
+ * before the GPU work has finished.
+ */
+#ifndef SULI_H
+#define SULI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* precision of the fields and stencils */
+#define SULI_PREC_FP64 0   /* everything in FP64 (parity with the reference)  */
+#define SULI_PREC_MIXED 1  /* FP32 fields, stencils and Krylov sweeps; FP64
+                              pressure, residuals, dot products, integrals    */
+
+/* pressure solver of suli_graddin */
+#define SULI_POISSON_REF 0 /* the reference CG: FP64, scaled residual sum of
+                              squares <= tol_abs (default 1e-4/(nx*ny*nz)),
+                              warm start, at most max_it iterations
+                              (9_grad.f90:175)                                */
+#define SULI_POISSON_FCG 1 /* flexible CG with FP64 iterative refinement,
+                              stopped on the FP64 true relative residual
+                              ||D^-1/2 (b - A p)|| / ||D^-1/2 b|| <= tol_rel  */
+
+/* preconditioners of SULI_POISSON_FCG */
+#define SULI_PRECOND_JACOBI 0  /* symmetric diagonal scaling (the reference's) */
+#define SULI_PRECOND_GMG 1     /* geometric multigrid V-cycle                  */
+#define SULI_PRECOND_LEARNED 2 /* 3-D convolutional U-Net, falls back to GMG   */
+
+typedef struct {
+  /* grid (0_modules.f90:7,15) */
+  int32_t nx, ny, nz;
+  double dx, dy, dz;
+  /* wall codes (0_modules.f90:53-61): 0 periodic, 1 free slip, 2 no slip,
+     4 outflow (x = xf only) */
+  int32_t ccx0, ccxf, ccy0, ccyf, ccz0, cczf;
+  /* model switches: m_turb 0 none, 1 Smagorinsky (0_modules.f90:31) */
+  int32_t m_turb;
+  /* fluid and model constants (14_level_set.f90:25-30, 0_modules.f90:141-143,181) */
+  double rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1;
+  double csmag, gx, gz, chezy;
+  /* numerics */
+  int32_t precision;  /* SULI_PREC_* */
+  int32_t poisson;    /* SULI_POISSON_* */
+  int32_t precond;    /* SULI_PRECOND_* */
+  int32_t max_it;     /* Krylov iteration cap (0: 10000) */
+  double tol_abs;     /* SULI_POISSON_REF threshold (0: 1e-4/(nx*ny*nz)) */
+  double tol_rel;     /* SULI_POISSON_FCG threshold (0: 1e-8) */
+  int32_t device;     /* HIP device ordinal */
+} suli_params;
+
+/* Field identifiers of suli_upload / suli_download; the host array has the
+ * reference extents (name: Fortran declaration). */
+enum {
+  SULI_U = 0,      /* u(0:nx+2,0:ny+1,0:nz+1)  */
+  SULI_V = 1,      /* v(0:nx+1,0:ny+2,0:nz+1)  */
+  SULI_W = 2,      /* w(0:nx+1,0:ny+1,0:nz+2)  */
+  SULI_PRD1 = 3,   /* prd1(0:nx+1,0:ny+1,0:nz+1) */
+  SULI_RHO = 4,    /* rho(nx,ny,nz)            */
+  SULI_LS_NU = 5,  /* ls_nu(nx,ny,nz)          */
+  SULI_LS = 6,     /* ls(nx,ny,nz): level set, positive in water */
+  SULI_MOD_LS = 7,
+  SULI_KURV = 8,
+  SULI_HS = 9,
+  SULI_HSX = 10,
+  SULI_HSY = 11,
+  SULI_HSZ = 12,
+  SULI_DDLSDX = 13,
+  SULI_DDLSDY = 14,
+  SULI_DDLSDZ = 15,
+  SULI_NUT = 16,   /* nut(nx,ny,nz)            */
+  SULI_XNUT = 17,  /* xnut(nx+1,ny,nz): download only, rebuilt from nut, ls_nu */
+  SULI_YNUT = 18,
+  SULI_ZNUT = 19,
+  SULI_FU = 20,    /* Fu(nx+1,ny,nz): download only, after suli_convdiff_forces */
+  SULI_FV = 21,
+  SULI_FW = 22,
+  SULI_U0 = 23,    /* u0(nx+1,ny,nz)           */
+  SULI_V0 = 24,
+  SULI_W0 = 25,
+  SULI_NFIELDS = 26
+};
+
+typedef struct {
+  int32_t iters;          /* Krylov iterations (inner iterations for FCG) */
+  int32_t outer;          /* FP64 refinement passes (FCG)                 */
+  int32_t fallbacks;      /* learned-preconditioner fallbacks to GMG      */
+  int32_t converged;      /* 1 if the stopping rule was met               */
+  double res0;            /* initial residual (REF: sum of squares; FCG: relative 2-norm) */
+  double res;             /* final residual, same measure                 */
+  double ms;              /* wall time of the solve                       */
+} suli_poisson_stats;
+
+typedef struct {
+  double vol_ini, vol_ins;  /* level-set volume, initial and current     */
+  int32_t reinit_iters;     /* reinitialisation iterations (1..3)        */
+  int32_t mass_corr_iters;  /* volume-correction passes                  */
+} suli_ls_stats;
+
+/* lifetime */
+int suli_init(const suli_params *p);
+int suli_finalize(void);
+const char *suli_version(void);
+int suli_last_error(char *buf, int len);
+
+/* host <-> device, arrays in reference extents and Fortran order (FP64) */
+int suli_upload(int field, const double *host, int64_t n);
+int suli_download(int field, double *host, int64_t n);
+int64_t suli_field_len(int field);
+
+/* Tail of level_set_ini (14_level_set.f90:205-217): heaviside, initial
+ * volume, mod_ls1, for a level set uploaded with suli_upload(SULI_LS). */
+int suli_level_set_ini(double *vol_ini);
+/* Dam-break initial level set: signed distance (positive in water) to a
+ * column x <= width, z <= height spanning y; then suli_level_set_ini. */
+int suli_init_dambreak(double width, double height, double *vol_ini);
+
+/* the time step, one call per reference subroutine (1_principal.f90:74-92) */
+int suli_level_set(double dt, suli_ls_stats *st);  /* level_set, 14_level_set.f90:221 */
+int suli_contorno(int nlock);                      /* contorno, 3_contorno.f90:10      */
+int suli_visco(void);                              /* visco, 5_viscosidade.f90:12      */
+int suli_convdiff(void);                           /* convdiff, 6_convdiff.f90:14      */
+int suli_tempo(int tt, double dt);                 /* tempo, 11_diversos.f90:10        */
+int suli_graddin(double dt, suli_poisson_stats *st);/* graddin, 9_grad.f90:10          */
+int suli_posdin(double dt);                        /* posdin, 10_pos.f90:1             */
+int suli_est_div(double *maxdiv);                  /* est, 4_plot.f90:500              */
+
+/* Fu, Fv, Fw of convdiff materialised for inspection (suli_convdiff leaves
+ * them implicit: the forces are fused into suli_tempo's update). */
+int suli_convdiff_forces(void);
+
+/* One RK stage (visco .. contorno(1)) and one whole time step
+ * (1_principal.f90:74-95) with the RK2 stage steps dt/2, dt. */
+int suli_stage(int tt, double dt_base, suli_poisson_stats *st);
+int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps);
+
+/* wait for all queued GPU work */
+int suli_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,587 @@
+// C ABI of the SuLi-IC MI355X library: state management, host <-> device
+// exchange in the reference's array extents, and the time-step entry points
+// that replace the reference subroutines called by 1_principal.f90:64-108.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstddef>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+
+Ctx *g_ctx = nullptr;
+static std::string g_err;
+
+void suli_set_error(const std::string &m) { g_err = m; }
+
+#define NEEDCTX()                                              \
+  do {                                                         \
+    if (!g_ctx) {                                              \
+      suli_set_error("library not initialised (suli_init)");   \
+      return 2;                                                \
+    }                                                          \
+  } while (0)
+
+extern "C" const char *suli_version(void) { return "suli-mi355x 0.1 (gfx950)"; }
+
+extern "C" int suli_last_error(char *buf, int len) {
+  if (!buf || len <= 0) return 1;
+  std::snprintf(buf, (size_t)len, "%s", g_err.c_str());
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// allocation
+
+static int dalloc(Ctx &c, void **p, size_t bytes) {
+  HIPCHK(hipMalloc(p, bytes));
+  HIPCHK(hipMemsetAsync(*p, 0, bytes, c.st));
+  c.allocs.push_back(*p);
+  return 0;
+}
+
+static void free_all(Ctx &c) {
+  for (void *p : c.allocs) (void)hipFree(p);
+  c.allocs.clear();
+  if (c.ctl_h) (void)hipHostFree(c.ctl_h);
+  if (c.res_h) (void)hipHostFree(c.res_h);
+  if (c.st) (void)hipStreamDestroy(c.st);
+}
+
+extern "C" int suli_finalize(void) {
+  if (!g_ctx) return 0;
+  (void)hipStreamSynchronize(g_ctx->st);
+  fcg_destroy(*g_ctx);
+  free_all(*g_ctx);
+  delete g_ctx;
+  g_ctx = nullptr;
+  return 0;
+}
+
+extern "C" int suli_init(const suli_params *p) {
+  if (!p) {
+    suli_set_error("suli_init: null params");
+    return 1;
+  }
+  if (g_ctx) suli_finalize();
+  if (p->nx < 4 || p->ny < 4 || p->nz < 4) {
+    suli_set_error("suli_init: every grid dimension must be at least 4 (WENO5 stencils)");
+    return 1;
+  }
+  auto okc = [](int v, bool x) { return v == 0 || v == 1 || v == 2 || (x && v == 4); };
+  if (!okc(p->ccx0, false) || !okc(p->ccxf, true) || !okc(p->ccy0, false) || !okc(p->ccyf, false) ||
+      p->ccz0 < 1 || p->ccz0 > 2 || p->cczf < 1 || p->cczf > 2) {
+    suli_set_error("suli_init: supported wall codes are 0 periodic, 1 free slip, 2 no slip, 4 outflow at xf; "
+                   "z walls 1 or 2");
+    return 1;
+  }
+  if ((p->ccx0 == 0) != (p->ccxf == 0) || (p->ccy0 == 0) != (p->ccyf == 0)) {
+    suli_set_error("suli_init: a periodic wall needs its opposite wall periodic too");
+    return 1;
+  }
+  if (p->precision != SULI_PREC_FP64 && p->precision != SULI_PREC_MIXED) {
+    suli_set_error("suli_init: unknown precision");
+    return 1;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    suli_set_error("suli_init: no HIP device visible");
+    return 3;
+  }
+  Ctx *c = new Ctx();
+  c->P = *p;
+  if (c->P.max_it <= 0) c->P.max_it = 10000;
+  if (c->P.tol_abs <= 0) c->P.tol_abs = 0.0001 / ((double)p->nx * p->ny * p->nz);
+  if (c->P.tol_rel <= 0) c->P.tol_rel = 1e-8;
+  g_ctx = c;
+  HIPCHK(hipSetDevice(p->device));
+  HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  c->prec = p->precision;
+  c->esz = c->prec == SULI_PREC_FP64 ? 8 : 4;
+  Grid &g = c->g;
+  g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
+  g.sx = ((p->nx + 3 + 31) / 32) * 32;
+  g.sy = p->ny + 3;
+  g.sz = (int64_t)g.sx * g.sy;
+  g.off = 31;
+  g.total = g.off + g.sz * (p->nz + 3) + 64;
+  g.dx = p->dx; g.dy = p->dy; g.dz = p->dz;
+  g.ccx0 = p->ccx0; g.ccxf = p->ccxf; g.ccy0 = p->ccy0; g.ccyf = p->ccyf; g.ccz0 = p->ccz0; g.cczf = p->cczf;
+  Phys &ph = c->ph;
+  ph.rho_f1 = p->rho_f1; ph.rho_f2 = p->rho_f2; ph.mi_f1 = p->mi_f1; ph.mi_f2 = p->mi_f2;
+  ph.sigma = p->sigma; ph.alpha1 = p->alpha1; ph.csmag = p->csmag; ph.gx = p->gx; ph.gz = p->gz;
+  ph.chezy = p->chezy;
+  ph.c23 = std::pow(p->dx * p->dy * p->dz, 2. / 3.);
+  ph.dx1 = std::fmax(std::fmax(p->dx, p->dy), p->dz);
+  ph.dt1 = 0.1 * ph.dx1;
+  ph.m_turb = p->m_turb;
+  const size_t fb = (size_t)g.total * c->esz, db = (size_t)g.total * 8;
+  void **tf[] = {&c->u, &c->v, &c->w, &c->un, &c->vn, &c->wn, &c->u0, &c->v0, &c->w0, &c->ls, &c->lsA,
+                 &c->lsB, &c->S, &c->hs, &c->hsx, &c->hsy, &c->hsz, &c->rho, &c->lsnu, &c->modls,
+                 &c->kurv, &c->nut, &c->nrx, &c->nry, &c->nrz, &c->ddx, &c->ddy, &c->ddz, &c->Fu,
+                 &c->Fv, &c->Fw};
+  for (void **q : tf)
+    if (dalloc(*c, q, fb)) return 1;
+  double **td[] = {&c->prd1, &c->D, &c->sD, &c->q, &c->ax, &c->ay, &c->az, &c->e, &c->r, &c->pp, &c->mp};
+  for (double **q : td)
+    if (dalloc(*c, (void **)q, db)) return 1;
+  if (dalloc(*c, (void **)&c->ctl, sizeof(CGCtl))) return 1;
+  const size_t maxblocks = (size_t)((g.nx + 3 + 63) / 64) * ((g.ny + 3 + 3) / 4) * (g.nz + 3) + 1024;
+  if (dalloc(*c, (void **)&c->partials, maxblocks * 4 * sizeof(double))) return 1;
+  if (dalloc(*c, (void **)&c->counters, 64 * sizeof(unsigned int))) return 1;
+  if (dalloc(*c, (void **)&c->res, RS_NSLOT * sizeof(double))) return 1;
+  HIPCHK(hipHostMalloc((void **)&c->ctl_h, sizeof(CGCtl), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&c->res_h, RS_NSLOT * sizeof(double), hipHostMallocDefault));
+  // density and viscosity start as the light fluid until a level set is set
+  if (fcg_create(*c)) return 1;
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+extern "C" int suli_sync(void) {
+  NEEDCTX();
+  HIPCHK(hipStreamSynchronize(g_ctx->st));
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// host <-> device in the reference extents
+
+struct Ext {
+  int i0, i1, j0, j1, k0, k1;
+};
+enum Kind { K_U, K_V, K_W, K_P, K_C, K_FX, K_FY, K_FZ };
+
+static Ext ext_of(const Grid &g, Kind k) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  switch (k) {
+    case K_U: return {0, nx + 2, 0, ny + 1, 0, nz + 1};
+    case K_V: return {0, nx + 1, 0, ny + 2, 0, nz + 1};
+    case K_W: return {0, nx + 1, 0, ny + 1, 0, nz + 2};
+    case K_P: return {0, nx + 1, 0, ny + 1, 0, nz + 1};
+    case K_C: return {1, nx, 1, ny, 1, nz};
+    case K_FX: return {1, nx + 1, 1, ny, 1, nz};
+    case K_FY: return {1, nx, 1, ny + 1, 1, nz};
+    default: return {1, nx, 1, ny, 1, nz + 1};
+  }
+}
+static int64_t ext_len(const Ext &e) {
+  return (int64_t)(e.i1 - e.i0 + 1) * (e.j1 - e.j0 + 1) * (e.k1 - e.k0 + 1);
+}
+
+struct FieldRef {
+  Kind kind;
+  void *dev;
+  bool f64;      // stored as double regardless of precision
+  bool derived;  // download-only
+};
+
+static bool field_ref(Ctx &c, int f, FieldRef &r) {
+  switch (f) {
+    case SULI_U: r = {K_U, c.u, false, false}; return true;
+    case SULI_V: r = {K_V, c.v, false, false}; return true;
+    case SULI_W: r = {K_W, c.w, false, false}; return true;
+    case SULI_PRD1: r = {K_P, c.prd1, true, false}; return true;
+    case SULI_RHO: r = {K_C, c.rho, false, false}; return true;
+    case SULI_LS_NU: r = {K_C, c.lsnu, false, false}; return true;
+    case SULI_LS: r = {K_C, c.ls, false, false}; return true;
+    case SULI_MOD_LS: r = {K_C, c.modls, false, false}; return true;
+    case SULI_KURV: r = {K_C, c.kurv, false, false}; return true;
+    case SULI_HS: r = {K_C, c.hs, false, false}; return true;
+    case SULI_HSX: r = {K_C, c.hsx, false, false}; return true;
+    case SULI_HSY: r = {K_C, c.hsy, false, false}; return true;
+    case SULI_HSZ: r = {K_C, c.hsz, false, false}; return true;
+    case SULI_DDLSDX: r = {K_C, c.ddx, false, false}; return true;
+    case SULI_DDLSDY: r = {K_C, c.ddy, false, false}; return true;
+    case SULI_DDLSDZ: r = {K_C, c.ddz, false, false}; return true;
+    case SULI_NUT: r = {K_C, c.nut, false, false}; return true;
+    case SULI_XNUT: r = {K_FX, nullptr, false, true}; return true;
+    case SULI_YNUT: r = {K_FY, nullptr, false, true}; return true;
+    case SULI_ZNUT: r = {K_FZ, nullptr, false, true}; return true;
+    case SULI_FU: r = {K_FX, c.Fu, false, false}; return true;
+    case SULI_FV: r = {K_FY, c.Fv, false, false}; return true;
+    case SULI_FW: r = {K_FZ, c.Fw, false, false}; return true;
+    case SULI_U0: r = {K_FX, c.u0, false, false}; return true;
+    case SULI_V0: r = {K_FY, c.v0, false, false}; return true;
+    case SULI_W0: r = {K_FZ, c.w0, false, false}; return true;
+    default: return false;
+  }
+}
+
+extern "C" int64_t suli_field_len(int field) {
+  if (!g_ctx) return -1;
+  FieldRef r;
+  if (!field_ref(*g_ctx, field, r)) return -1;
+  return ext_len(ext_of(g_ctx->g, r.kind));
+}
+
+// whole frame field to host doubles
+static int frame_to_host(Ctx &c, const void *dev, bool f64, std::vector<double> &out) {
+  const Grid &g = c.g;
+  out.resize((size_t)g.total);
+  if (f64 || c.esz == 8) {
+    HIPCHK(hipMemcpyAsync(out.data(), dev, (size_t)g.total * 8, hipMemcpyDeviceToHost, c.st));
+  } else {
+    std::vector<float> tmp((size_t)g.total);
+    HIPCHK(hipMemcpyAsync(tmp.data(), dev, (size_t)g.total * 4, hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    for (size_t q = 0; q < tmp.size(); q++) out[q] = tmp[q];
+  }
+  HIPCHK(hipStreamSynchronize(c.st));
+  return 0;
+}
+static int host_to_frame(Ctx &c, void *dev, bool f64, const std::vector<double> &in) {
+  const Grid &g = c.g;
+  if (f64 || c.esz == 8) {
+    HIPCHK(hipMemcpyAsync(dev, in.data(), (size_t)g.total * 8, hipMemcpyHostToDevice, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+  } else {
+    std::vector<float> tmp((size_t)g.total);
+    for (size_t q = 0; q < tmp.size(); q++) tmp[q] = (float)in[q];
+    HIPCHK(hipMemcpyAsync(dev, tmp.data(), (size_t)g.total * 4, hipMemcpyHostToDevice, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+  }
+  return 0;
+}
+
+extern "C" int suli_upload(int field, const double *host, int64_t n) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  FieldRef r;
+  if (!field_ref(c, field, r) || r.derived) {
+    suli_set_error("suli_upload: field " + std::to_string(field) + " cannot be uploaded");
+    return 1;
+  }
+  const Ext e = ext_of(c.g, r.kind);
+  if (n != ext_len(e)) {
+    suli_set_error("suli_upload: field " + std::to_string(field) + " expects " + std::to_string(ext_len(e)) +
+                   " values, got " + std::to_string(n));
+    return 1;
+  }
+  std::vector<double> fr;
+  if (frame_to_host(c, r.dev, r.f64, fr)) return 1;
+  int64_t q = 0;
+  for (int k = e.k0; k <= e.k1; k++)
+    for (int j = e.j0; j <= e.j1; j++)
+      for (int i = e.i0; i <= e.i1; i++) fr[(size_t)c.g.idx(i, j, k)] = host[q++];
+  if (host_to_frame(c, r.dev, r.f64, fr)) return 1;
+  if (field == SULI_U || field == SULI_V || field == SULI_W) c.convdiff_pending = false;
+  return 0;
+}
+
+extern "C" int suli_download(int field, double *host, int64_t n) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  FieldRef r;
+  if (!field_ref(c, field, r)) {
+    suli_set_error("suli_download: unknown field " + std::to_string(field));
+    return 1;
+  }
+  const Ext e = ext_of(c.g, r.kind);
+  if (n != ext_len(e)) {
+    suli_set_error("suli_download: field " + std::to_string(field) + " expects " +
+                   std::to_string(ext_len(e)) + " values, got " + std::to_string(n));
+    return 1;
+  }
+  const Grid &g = c.g;
+  if (r.derived) {
+    // face viscosities xnut/ynut/znut = interp*_cf(ls_nu) + interp*_cf(nut)
+    // (5_viscosidade.f90:99-101,178-211), rebuilt from the cell arrays
+    std::vector<double> nu, nt;
+    if (frame_to_host(c, c.lsnu, false, nu) || frame_to_host(c, c.nut, false, nt)) return 1;
+    int64_t q = 0;
+    for (int k = e.k0; k <= e.k1; k++)
+      for (int j = e.j0; j <= e.j1; j++)
+        for (int i = e.i0; i <= e.i1; i++) {
+          int a = i, b = j, cc = k, da = 0, db = 0, dc = 0, n1;
+          if (field == SULI_XNUT) { n1 = g.nx; a = a < 2 ? 2 : (a > n1 ? n1 : a); da = 1; }
+          else if (field == SULI_YNUT) { n1 = g.ny; b = b < 2 ? 2 : (b > n1 ? n1 : b); db = 1; }
+          else { n1 = g.nz; cc = cc < 2 ? 2 : (cc > n1 ? n1 : cc); dc = 1; }
+          const size_t p0 = (size_t)g.idx(a, b, cc), p1 = (size_t)g.idx(a - da, b - db, cc - dc);
+          host[q++] = (nu[p0] + nu[p1]) * 0.5 + (nt[p0] + nt[p1]) * 0.5;
+        }
+    return 0;
+  }
+  std::vector<double> fr;
+  if (frame_to_host(c, r.dev, r.f64, fr)) return 1;
+  int64_t q = 0;
+  for (int k = e.k0; k <= e.k1; k++)
+    for (int j = e.j0; j <= e.j1; j++)
+      for (int i = e.i0; i <= e.i1; i++) host[q++] = fr[(size_t)g.idx(i, j, k)];
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// level set
+
+static int read_res(Ctx &c, int off, int n) {
+  HIPCHK(hipMemcpyAsync(c.res_h + off, c.res + off, sizeof(double) * n, hipMemcpyDeviceToHost, c.st));
+  HIPCHK(hipStreamSynchronize(c.st));
+  return 0;
+}
+
+static int heavi_curv(Ctx &c, bool curv, double *vol) {
+  DISPATCH(c, Launch<TT>::heavi(c.g, c.ph, c.st, c.ls, c.hs, c.hsx, c.hsy, c.hsz, c.rho, c.lsnu, c.modls,
+                                c.nrx, c.nry, c.nrz, c.slot(RS_VOL, 0)));
+  if (curv) DISPATCH(c, Launch<TT>::curv(c.g, c.st, c.nrx, c.nry, c.nrz, c.ddx, c.ddy, c.ddz, c.kurv));
+  HIPCHK(hipGetLastError());
+  if (vol) {
+    if (read_res(c, RS_VOL, 1)) return 1;
+    *vol = c.res_h[RS_VOL];
+  }
+  return 0;
+}
+
+extern "C" int suli_level_set_ini(double *vol_ini) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  double v = 0;
+  if (heavi_curv(c, true, &v)) return 1;
+  c.vol_ini = v;
+  if (vol_ini) *vol_ini = v;
+  return 0;
+}
+
+extern "C" int suli_init_dambreak(double width, double height, double *vol_ini) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  DISPATCH(c, Launch<TT>::dambreak(c.g, c.st, width, height, c.ls));
+  HIPCHK(hipGetLastError());
+  return suli_level_set_ini(vol_ini);
+}
+
+extern "C" int suli_level_set(double dt, suli_ls_stats *st) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  const Grid &g = c.g;
+  // SSP-RK3 advection with the full dt (14_level_set.f90:231-239)
+  DISPATCH(c, Launch<TT>::ls_adv(g, c.st, c.ls, c.ls, c.u, c.v, c.w, c.lsA, 1., 0., 1., dt));
+  DISPATCH(c, Launch<TT>::ls_adv(g, c.st, c.lsA, c.ls, c.u, c.v, c.w, c.lsB, 3. / 4., 1. / 4., 1. / 4., dt));
+  DISPATCH(c, Launch<TT>::ls_adv(g, c.st, c.lsB, c.ls, c.u, c.v, c.w, c.lsA, 1. / 3., 2. / 3., 2. / 3., dt));
+  std::swap(c.ls, c.lsA);
+  // reinitialisation (:386-509)
+  const double dt1 = c.ph.dt1, dx1 = c.ph.dx1;
+  DISPATCH(c, Launch<TT>::reinit_sign(g, c.ph, c.st, c.ls, c.S));
+  int il = 0;
+  double error = 999.;
+  while (error > dt1 * dx1 * dx1) {
+    il = il + 1;
+    DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.ls, c.ls, c.S, c.lsA, 1., 0., 1. * dt1, 0,
+                                         c.slot(RS_BAND, 1)));
+    DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.lsA, c.ls, c.S, c.lsB, 3. / 4., 1. / 4.,
+                                         1. / 4. * dt1, 0, c.slot(RS_BAND, 1)));
+    DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.lsB, c.ls, c.S, c.lsA, 1. / 3., 2. / 3.,
+                                         2. / 3. * dt1, il < 3 ? 1 : 0, c.slot(RS_BAND, 1)));
+    std::swap(c.ls, c.lsA);
+    HIPCHK(hipGetLastError());
+    if (il == 3) {
+      error = 0.;
+    } else {
+      if (read_res(c, RS_BAND, 2)) return 1;
+      const double nr = c.res_h[RS_BAND + 1];
+      error = c.res_h[RS_BAND] / nr;  // nr = 0 gives NaN and stops, as in the reference
+    }
+  }
+  // mod_ls1 + heaviside, volume, volume correction (:243-282)
+  double vol = 0;
+  if (heavi_curv(c, false, &vol)) return 1;
+  int nc = 0;
+  while (std::fabs(vol - c.vol_ini) / vol > 0.1) {
+    DISPATCH(c, Launch<TT>::masscorr(g, c.st, c.ls, c.modls, dt1 * (vol - c.vol_ini), c.vol_ini));
+    if (heavi_curv(c, false, &vol)) return 1;
+    nc++;
+    if (nc > 1000) {
+      suli_set_error("suli_level_set: volume correction did not converge");
+      return 4;
+    }
+  }
+  DISPATCH(c, Launch<TT>::curv(g, c.st, c.nrx, c.nry, c.nrz, c.ddx, c.ddy, c.ddz, c.kurv));
+  HIPCHK(hipGetLastError());
+  c.vol_ins = vol;
+  if (st) {
+    st->vol_ini = c.vol_ini;
+    st->vol_ins = vol;
+    st->reinit_iters = il;
+    st->mass_corr_iters = nc;
+  }
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// predictor
+
+extern "C" int suli_visco(void) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  if (c.ph.m_turb == 1) {
+    DISPATCH(c, Launch<TT>::visco(c.g, c.ph, c.st, c.u, c.v, c.w, c.rho, c.nut));
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+extern "C" int suli_convdiff(void) {
+  NEEDCTX();
+  g_ctx->convdiff_pending = true;
+  return 0;
+}
+
+static MomPtrs mom_ptrs(Ctx &c) {
+  MomPtrs m;
+  m.u = c.u; m.v = c.v; m.w = c.w; m.u0 = c.u0; m.v0 = c.v0; m.w0 = c.w0;
+  m.rho = c.rho; m.lsnu = c.lsnu; m.nut = c.nut; m.hsx = c.hsx; m.hsy = c.hsy; m.hsz = c.hsz;
+  m.kurv = c.kurv; m.tfu = c.tfu; m.tfv = c.tfv; m.tfw = c.tfw;
+  m.un = c.un; m.vn = c.vn; m.wn = c.wn; m.u0o = c.u0; m.v0o = c.v0; m.w0o = c.w0;
+  m.Fu = nullptr; m.Fv = nullptr; m.Fw = nullptr;
+  return m;
+}
+
+extern "C" int suli_convdiff_forces(void) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  MomPtrs m = mom_ptrs(c);
+  m.u0 = c.u; m.v0 = c.v; m.w0 = c.w;  // stage-2 form: writes only the scratch swap buffers
+  m.Fu = c.Fu; m.Fv = c.Fv; m.Fw = c.Fw;
+  DISPATCH(c, Launch<TT>::momentum(c.g, c.ph, c.st, m, 0.0, 2));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int suli_tempo(int tt, double dt) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  if (!c.convdiff_pending) {
+    suli_set_error("suli_tempo: needs a preceding suli_convdiff (the forces are fused into the update)");
+    return 1;
+  }
+  if (tt != 1 && tt != 2) {
+    suli_set_error("suli_tempo: RK2 stage must be 1 or 2");
+    return 1;
+  }
+  MomPtrs m = mom_ptrs(c);
+  DISPATCH(c, Launch<TT>::momentum(c.g, c.ph, c.st, m, dt, tt));
+  HIPCHK(hipGetLastError());
+  std::swap(c.u, c.un);
+  std::swap(c.v, c.vn);
+  std::swap(c.w, c.wn);
+  c.convdiff_pending = false;
+  return 0;
+}
+
+extern "C" int suli_contorno(int nlock) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  DISPATCH(c, Launch<TT>::contorno(c.g, c.st, nlock, c.u, c.v, c.w, c.prd1));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// projection
+
+static int poisson_ref(Ctx &c, double dt, suli_poisson_stats *stt) {
+  const Grid &g = c.g;
+  auto t0 = std::chrono::steady_clock::now();
+  DISPATCH(c, Launch<TT>::pD(g, c.st, dt, c.u, c.v, c.w, c.rho, c.D, c.sD, c.q));
+  DISPATCH(c, Launch<TT>::pcoef(g, c.st, dt, c.rho, c.D, c.ax, c.ay, c.az));
+  const dim3 b(64, 4, 1);
+  const dim3 pg((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4, g.nz + 2);
+  hipLaunchKernelGGL(k_pe, pg, b, 0, c.st, g, c.prd1, c.sD, c.e);
+  std::memset(c.ctl_h, 0, sizeof(CGCtl));
+  c.ctl_h->thr = c.P.tol_abs;
+  c.ctl_h->maxit = c.P.max_it;
+  HIPCHK(hipMemcpyAsync(c.ctl, c.ctl_h, sizeof(CGCtl), hipMemcpyHostToDevice, c.st));
+  const dim3 cg = cell_grid(g);
+  RedSlot r0{c.partials, c.counters + 4, (double *)((char *)c.ctl + offsetof(CGCtl, r0))};
+  hipLaunchKernelGGL(k_pr0, cg, b, 0, c.st, g, c.e, c.ax, c.ay, c.az, c.q, c.sD, c.r, c.pp, c.ctl, r0);
+  RedSlot r1{c.partials, c.counters + 5, (double *)((char *)c.ctl + offsetof(CGCtl, alfa))};
+  RedSlot r2{c.partials, c.counters + 6, (double *)((char *)c.ctl + offsetof(CGCtl, beta))};
+  int chunk = 8;
+  for (;;) {
+    for (int n = 0; n < chunk; n++) {
+      hipLaunchKernelGGL(k_cg1, cg, b, 0, c.st, g, c.ax, c.ay, c.az, c.r, c.pp, c.mp, c.ctl, r1);
+      hipLaunchKernelGGL(k_cg2, cg, b, 0, c.st, g, c.e, c.r, c.pp, c.mp, c.ctl, r2);
+      hipLaunchKernelGGL(k_cg3, cg, b, 0, c.st, g, c.r, c.pp, c.ctl, c.counters + 7);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c.ctl_h, c.ctl, sizeof(CGCtl), hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    if (!c.ctl_h->active) break;
+    if (chunk < 64) chunk *= 2;
+  }
+  hipLaunchKernelGGL(k_pfinal, pg, b, 0, c.st, g, c.e, c.sD, c.prd1, c.ctl);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c.st));
+  auto t1 = std::chrono::steady_clock::now();
+  if (stt) {
+    std::memset(stt, 0, sizeof(*stt));
+    stt->iters = c.ctl_h->iters;
+    stt->outer = 0;
+    stt->converged = !(std::fabs(c.ctl_h->alfa[0]) > c.ctl_h->thr) ? 1 : 0;
+    stt->res0 = c.ctl_h->r0[0];
+    stt->res = c.ctl_h->alfa[0];
+    stt->ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+  return 0;
+}
+
+extern "C" int suli_graddin(double dt, suli_poisson_stats *st) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  if (c.P.poisson == SULI_POISSON_REF) return poisson_ref(c, dt, st);
+  return fcg_solve(c, dt, st);
+}
+
+extern "C" int suli_posdin(double dt) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  DISPATCH(c, Launch<TT>::posdin(c.g, c.st, dt, c.prd1, c.rho, c.u, c.v, c.w));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int suli_est_div(double *maxdiv) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  DISPATCH(c, Launch<TT>::divmax(c.g, c.st, c.u, c.v, c.w, c.slot(RS_DIV, 2)));
+  HIPCHK(hipGetLastError());
+  if (read_res(c, RS_DIV, 1)) return 1;
+  if (maxdiv) *maxdiv = c.res_h[RS_DIV];
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// composed step (1_principal.f90:74-95)
+
+extern "C" int suli_stage(int tt, double dt_base, suli_poisson_stats *st) {
+  NEEDCTX();
+  const double dt = tt == 1 ? 0.5 * dt_base : 1.0 * dt_base;  // a_dt, 2_iniciais.f90:432-437
+  int rc;
+  if ((rc = suli_visco())) return rc;
+  if ((rc = suli_convdiff())) return rc;
+  if ((rc = suli_tempo(tt, dt))) return rc;
+  if ((rc = suli_contorno(2))) return rc;
+  if ((rc = suli_graddin(dt, st))) return rc;
+  if ((rc = suli_posdin(dt))) return rc;
+  if ((rc = suli_contorno(1))) return rc;
+  return 0;
+}
+
+extern "C" int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps) {
+  NEEDCTX();
+  int rc;
+  if ((rc = suli_level_set(dt, ls))) return rc;
+  if ((rc = suli_contorno(3))) return rc;
+  suli_poisson_stats a{}, b{};
+  if ((rc = suli_stage(1, dt, &a))) return rc;
+  if ((rc = suli_stage(2, dt, &b))) return rc;
+  if (ps) {
+    ps[0] = a;
+    ps[1] = b;
+  }
+  return 0;
+}

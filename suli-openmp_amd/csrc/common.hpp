@@ -1,0 +1,262 @@
+// Shared device-side definitions of the SuLi-IC MI355X library.
+//
+// Field layout in HBM ("frame"): every 3-D field of the time step - the
+// staggered velocities with their ghost layers, the pressure, cell arrays and
+// face arrays - lives in one common frame indexed with the reference's own
+// Fortran indices i in [0,nx+2], j in [0,ny+2], k in [0,nz+2] (the union of
+// the extents declared in 0_modules.f90:92-133), i fastest:
+//
+//     idx(i,j,k) = off + i + sx*(j + sy*k)
+//
+// sx is nx+3 rounded up to 32 elements and off = 31, so the first interior
+// point of every row (i = 1) sits on a 128-byte boundary for FP32 fields and
+// every wavefront of 64 consecutive i reads whole cache lines.  Because all
+// fields share one index map, a fused kernel computes one address per thread
+// and offsets it by the same strides (1, sx, sx*sy) for every field.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SULI_LDS_RED 16  // max wavefronts per block in the reductions below
+
+struct Grid {
+  int nx, ny, nz;
+  int sx, sy;        // pitches in i and j
+  int64_t sz;        // sx*sy (k stride)
+  int64_t off;       // offset of (0,0,0)
+  int64_t total;     // elements per frame field
+  double dx, dy, dz;
+  int ccx0, ccxf, ccy0, ccyf, ccz0, cczf;
+  __host__ __device__ __forceinline__ int64_t idx(int i, int j, int k) const {
+    return off + (int64_t)i + (int64_t)sx * ((int64_t)j + (int64_t)sy * (int64_t)k);
+  }
+};
+
+// Fluid properties and model constants (14_level_set.f90:25-31,
+// 0_modules.f90:141-143,181,197-198).
+struct Phys {
+  double rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1;
+  double csmag, gx, gz, chezy;
+  double c23;   // (dx*dy*dz)**(2/3), evaluated on the host (5_viscosidade.f90:94)
+  double dx1;   // max(dx,dy,dz)
+  double dt1;   // pseudo time step of the reinitialisation, 0.1*dx1
+  int m_turb;
+};
+
+// Thread layout of the stencil kernels: 64 lanes along i (one wavefront per
+// row segment), 4 rows along j, one k plane per block column.
+#define SULI_BX 64
+#define SULI_BY 4
+
+template <typename T>
+__device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
+
+// ---------------------------------------------------------------------------
+// Deterministic block reductions in FP64.
+//
+// A kernel that produces a global sum writes one partial per block; the last
+// block to finish (counted with a global atomic) adds the partials in block
+// order and stores the result, so the sum is the same on every run.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_down(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+// Sum (or max) over the block of nv values per thread; result valid in thread 0.
+template <int NV, bool MAX = false>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double *lds /* NV*SULI_LDS_RED */) {
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int nt = blockDim.x * blockDim.y * blockDim.z;
+  const int lane = tid & 63, wid = tid >> 6, nw = (nt + 63) >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    double r = MAX ? wave_max(v[q]) : wave_sum(v[q]);
+    if (lane == 0) lds[q * SULI_LDS_RED + wid] = r;
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      double r = lds[q * SULI_LDS_RED];
+      for (int w = 1; w < nw; w++) {
+        double x = lds[q * SULI_LDS_RED + w];
+        r = MAX ? (x > r ? x : r) : r + x;
+      }
+      v[q] = r;
+    }
+  }
+}
+
+// Reduction slots: partials[NV*maxblocks], a completion counter, results[NV].
+struct RedSlot {
+  double *partials;
+  unsigned int *counter;
+  double *result;
+};
+
+// Called by every block after block_reduce; thread 0 of the last block writes
+// result[q] = sum/max over blocks of the block values, in block order.
+// Returns true in every thread of the last block.
+template <int NV, bool MAX = false>
+__device__ __forceinline__ bool grid_reduce_finish(const double (&v)[NV], RedSlot rs) {
+  __shared__ bool last;
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const unsigned int nb = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (tid == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) rs.partials[(size_t)q * nb + b] = v[q];
+    __threadfence();
+    unsigned int prev = atomicAdd(rs.counter, 1u);
+    last = (prev == nb - 1);
+  }
+  __syncthreads();
+  if (last) {
+    __threadfence();
+    const int nt = blockDim.x * blockDim.y * blockDim.z;
+    __shared__ double lds[NV * SULI_LDS_RED];
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      double a = MAX ? -1.0e300 : 0.0;
+      for (unsigned int t = tid; t < nb; t += nt) {
+        double x = rs.partials[(size_t)q * nb + t];
+        a = MAX ? (x > a ? x : a) : a + x;
+      }
+      acc[q] = a;
+    }
+    block_reduce<NV, MAX>(acc, lds);
+    if (tid == 0) {
+#pragma unroll
+      for (int q = 0; q < NV; q++) rs.result[q] = acc[q];
+      *rs.counter = 0u;
+    }
+  }
+  return last;
+}
+
+// ---------------------------------------------------------------------------
+// WENO5-JS one-sided derivative from five consecutive divided differences
+// (14_level_set.f90:580-608), evaluated in the reference's operation order.
+template <typename T>
+__device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5) {
+  const T aux1 = T(13.) / T(12.), aux2 = T(1.) / T(4.), aux3 = T(1.) / T(6.);
+  const T eps = T(0.00000001);
+  T a, b;
+  a = v1 - 2 * v2 + v3;
+  b = v1 - 4 * v2 + 3 * v3;
+  const T is1 = aux1 * a * a + aux2 * b * b;
+  a = v2 - 2 * v3 + v4;
+  b = v2 - v4;
+  const T is2 = aux1 * a * a + aux2 * b * b;
+  a = v3 - 2 * v4 + v5;
+  b = 3 * v3 - 4 * v4 + v5;
+  const T is3 = aux1 * a * a + aux2 * b * b;
+  const T al1 = T(0.1) / ((eps + is1) * (eps + is1));
+  const T al2 = T(0.6) / ((eps + is2) * (eps + is2));
+  const T al3 = T(0.3) / ((eps + is3) * (eps + is3));
+  const T o1 = al1 / (al1 + al2 + al3);
+  const T o2 = al2 / (al1 + al2 + al3);
+  const T o3 = al3 / (al1 + al2 + al3);
+  return aux3 * (o1 * (2 * v1 - 7 * v2 + 11 * v3) + o2 * (-v2 + 5 * v3 + 2 * v4) +
+                 o3 * (2 * v3 + 5 * v4 - v5));
+}
+
+// Seven values phi(pos-3..pos+3) of a pencil of n points with stride st,
+// centred on the point at address c (1 <= pos <= n).  Points outside 1..n are
+// the ghost values of 14_level_set.f90:532-570: ihs 0 periodic with period
+// n-1, ihs 1 extrapolation (1/5 on the low side, 1/11 on the high side), ihs 2
+// the 1/11 formula on both sides.  The ghosts depend only on phi(1..3) and
+// phi(n-2..n), which always lie inside the thread's own window.
+template <typename T>
+__device__ __forceinline__ void window7(const T *__restrict__ f, int64_t c, int64_t st, int pos,
+                                        int n, int ihs, T (&w)[7]) {
+  if (ihs == 0) {
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+      int q = pos - 3 + m;
+      int qs = q < 1 ? q + n - 1 : (q > n ? q - n + 1 : q);
+      w[m] = f[c + (int64_t)(qs - pos) * st];
+    }
+    return;
+  }
+#pragma unroll
+  for (int m = 0; m < 7; m++) {
+    int q = pos - 3 + m;
+    int qc = q < 1 ? 1 : (q > n ? n : q);
+    w[m] = f[c + (int64_t)(qc - pos) * st];
+  }
+  if (pos <= 3) {
+    const T p1 = f[c + (int64_t)(1 - pos) * st], p2 = f[c + (int64_t)(2 - pos) * st],
+            p3 = f[c + (int64_t)(3 - pos) * st];
+    T g0, gm1, gm2;
+    if (ihs == 1) {
+      g0 = T(1.) / T(5.) * (T(12.) * p1 - T(9.) * p2 + T(2.) * p3);
+      gm1 = T(1.) / T(5.) * (T(12.) * g0 - T(9.) * p1 + T(2.) * p2);
+      gm2 = T(1.) / T(5.) * (T(12.) * gm1 - T(9.) * g0 + T(2.) * p1);
+    } else {
+      g0 = T(1.) / T(11.) * (T(18.) * p1 - T(9.) * p2 + T(2.) * p3);
+      gm1 = T(1.) / T(11.) * (T(18.) * g0 - T(9.) * p1 + T(2.) * p2);
+      gm2 = T(1.) / T(11.) * (T(18.) * gm1 - T(9.) * g0 + T(2.) * p1);
+    }
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+      int q = pos - 3 + m;
+      w[m] = q == 0 ? g0 : (q == -1 ? gm1 : (q == -2 ? gm2 : w[m]));
+    }
+  }
+  if (pos >= n - 2) {
+    const T pn = f[c + (int64_t)(n - pos) * st], pn1 = f[c + (int64_t)(n - 1 - pos) * st],
+            pn2 = f[c + (int64_t)(n - 2 - pos) * st];
+    const T g1 = T(1.) / T(11.) * (T(18.) * pn - T(9.) * pn1 + T(2.) * pn2);
+    const T g2 = T(1.) / T(11.) * (T(18.) * g1 - T(9.) * pn + T(2.) * pn1);
+    const T g3 = T(1.) / T(11.) * (T(18.) * g2 - T(9.) * g1 + T(2.) * pn);
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+      int q = pos - 3 + m;
+      w[m] = q == n + 1 ? g1 : (q == n + 2 ? g2 : (q == n + 3 ? g3 : w[m]));
+    }
+  }
+}
+
+// Both one-sided WENO derivatives at the window centre:
+// dp = phi_x^+ (dphidxp), dn = phi_x^- (dphidxn), 14_level_set.f90:572-608.
+template <typename T>
+__device__ __forceinline__ void weno_pm(const T (&w)[7], T h, T &dp, T &dn) {
+  T d[6];
+#pragma unroll
+  for (int m = 0; m < 6; m++) d[m] = (w[m + 1] - w[m]) / h;
+  dp = weno5<T>(d[5], d[4], d[3], d[2], d[1]);
+  dn = weno5<T>(d[0], d[1], d[2], d[3], d[4]);
+}
+
+// der_weno at one cell: the six one-sided derivatives of f, with the ihs of
+// 14_level_set.f90:353-377 (x, y: 0 if periodic else 2; z: 1) or, when
+// all_ihs1 is set, ihs 1 in every direction (as mod_ls1 uses it).
+template <typename T>
+__device__ __forceinline__ void der_weno_cell(const T *__restrict__ f, const Grid &g, int i, int j,
+                                              int k, int64_t c, bool all_ihs1, T &ta, T &tb,
+                                              T &tc, T &td, T &te, T &tf) {
+  T w[7];
+  const int ihx = all_ihs1 ? 1 : (g.ccx0 == 0 ? 0 : 2);
+  const int ihy = all_ihs1 ? 1 : (g.ccy0 == 0 ? 0 : 2);
+  window7<T>(f, c, 1, i, g.nx, ihx, w);
+  weno_pm<T>(w, T(g.dx), ta, td);
+  window7<T>(f, c, g.sx, j, g.ny, ihy, w);
+  weno_pm<T>(w, T(g.dy), tb, te);
+  window7<T>(f, c, g.sz, k, g.nz, 1, w);
+  weno_pm<T>(w, T(g.dz), tc, tf);
+}

@@ -1,0 +1,52 @@
+// Host-side launchers of the templated stencil kernels.  Each is defined in
+// stencils_impl.hpp and instantiated for double (inst_f64.hip, built without
+// FMA contraction) and for float (inst_f32.hip).
+#pragma once
+#include "common.hpp"
+
+struct MomPtrs {
+  const void *u, *v, *w, *u0, *v0, *w0;
+  const void *rho, *lsnu, *nut, *hsx, *hsy, *hsz, *kurv;
+  const void *tfu, *tfv, *tfw;
+  void *un, *vn, *wn, *u0o, *v0o, *w0o;
+  void *Fu, *Fv, *Fw;
+};
+
+template <typename T>
+struct Launch {
+  static void ls_adv(const Grid &g, hipStream_t s, const void *phi, const void *phin, const void *u,
+                     const void *v, const void *w, void *out, double ca, double cb, double cg, double dt);
+  static void reinit_sign(const Grid &g, const Phys &ph, hipStream_t s, const void *phi, void *S);
+  static void reinit_stage(const Grid &g, const Phys &ph, hipStream_t s, const void *phi,
+                           const void *phit, const void *S, void *out, double ca, double cb,
+                           double cgdt, int band, RedSlot rs);
+  static void heavi(const Grid &g, const Phys &ph, hipStream_t s, const void *phi, void *hs, void *hsx,
+                    void *hsy, void *hsz, void *rho, void *lsnu, void *modls, void *nrx, void *nry,
+                    void *nrz, RedSlot rs);
+  static void curv(const Grid &g, hipStream_t s, const void *nrx, const void *nry, const void *nrz,
+                   void *ddx, void *ddy, void *ddz, void *kurv);
+  static void masscorr(const Grid &g, hipStream_t s, void *phi, const void *modls, double f, double vol0);
+  static void visco(const Grid &g, const Phys &ph, hipStream_t s, const void *u, const void *v,
+                    const void *w, const void *rho, void *nut);
+  static void momentum(const Grid &g, const Phys &ph, hipStream_t s, const MomPtrs &m, double dt,
+                       int stage);
+  static void contorno(const Grid &g, hipStream_t s, int nlock, void *u, void *v, void *w, double *p);
+  static void posdin(const Grid &g, hipStream_t s, double dt, const double *p, const void *rho,
+                     void *u, void *v, void *w);
+  static void divmax(const Grid &g, hipStream_t s, const void *u, const void *v, const void *w,
+                     RedSlot rs);
+  static void dambreak(const Grid &g, hipStream_t s, double width, double height, void *phi);
+  static void cast_from_f64(hipStream_t s, const double *src, void *dst, int64_t n);
+  static void cast_to_f64(hipStream_t s, const void *src, double *dst, int64_t n);
+  // projection set-up shared by the Poisson solvers
+  static void pD(const Grid &g, hipStream_t s, double dt, const void *u, const void *v, const void *w,
+                 const void *rho, double *D, double *sD, double *q);
+  static void pcoef(const Grid &g, hipStream_t s, double dt, const void *rho, const double *D,
+                    double *ax, double *ay, double *az);
+};
+
+// grid helpers
+inline dim3 cell_grid(const Grid &g) {
+  return dim3((g.nx + SULI_BX - 1) / SULI_BX, (g.ny + SULI_BY - 1) / SULI_BY, g.nz);
+}
+inline dim3 blk2() { return dim3(SULI_BX, SULI_BY, 1); }

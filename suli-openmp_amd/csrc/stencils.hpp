@@ -1,0 +1,636 @@
+// Stencil kernels of the SuLi-IC time step, templated on the field type T
+// (double for the FP64 path, float for the mixed-precision path).  The two
+// instantiations live in separate translation units: the FP64 one is built
+// without FMA contraction so that it evaluates every expression in the
+// reference's order.  Reference lines are cited as file:line in SuLi-IC/.
+#pragma once
+#include "common.hpp"
+
+#define SULI_PI 3.141592653589793115997963468544185161590576171875
+
+// ---------------------------------------------------------------------------
+// Level set (14_level_set.f90)
+
+// One SSP-RK3 stage of the level-set advection: conv_weno (:308-341) fused
+// with intt_ls (:287-306).  out = ca*phin + cb*phi + (cg*dt)*L(phi).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ls_adv(Grid g, const T *__restrict__ phi,
+                                                const T *__restrict__ phin, const T *__restrict__ u,
+                                                const T *__restrict__ v, const T *__restrict__ w,
+                                                T *__restrict__ out, T ca, T cb, T cg, T dt) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (i > g.nx || j > g.ny) return;
+  const int64_t c = g.idx(i, j, k);
+  T ta, tb, tc, td, te, tf;
+  der_weno_cell<T>(phi, g, i, j, k, c, false, ta, tb, tc, td, te, tf);
+  const T a = (u[c] + u[c + 1]) * T(0.5);
+  const T b = (v[c] + v[c + g.sx]) * T(0.5);
+  const T cc = (w[c] + w[c + g.sz]) * T(0.5);
+  const T apos = tmax(a, T(0)), aneg = tmin(a, T(0));
+  const T bpos = tmax(b, T(0)), bneg = tmin(b, T(0));
+  const T cpos = tmax(cc, T(0)), cneg = tmin(cc, T(0));
+  const T sy7 = -(apos * td + aneg * ta + bpos * te + bneg * tb + cpos * tf + cneg * tc);
+  out[c] = ca * phin[c] + cb * phi[c] + cg * dt * sy7;
+}
+
+// Godunov upwinding with the 1e-8 floors of reinic_weno (:406-430).
+template <typename T>
+__device__ __forceinline__ T godunov_mod(T lsv, T ta, T tb, T tc, T td, T te, T tf) {
+  const T fl = T(0.00000001);
+  T x, y, z;
+  if (lsv > T(0)) {
+    x = tmax(tmax(td, fl), -tmin(ta, fl));
+    y = tmax(tmax(te, fl), -tmin(tb, fl));
+    z = tmax(tmax(tf, fl), -tmin(tc, fl));
+  } else {
+    x = tmax(tmax(ta, fl), -tmin(td, fl));
+    y = tmax(tmax(tb, fl), -tmin(te, fl));
+    z = tmax(tmax(tc, fl), -tmin(tf, fl));
+  }
+  return sqrt(x * x + y * y + z * z);
+}
+
+// Sign function S(phi0) of the reinitialisation (:400-435).
+template <typename T>
+__global__ __launch_bounds__(256) void k_reinit_sign(Grid g, T dx1, const T *__restrict__ phi,
+                                                     T *__restrict__ S) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (i > g.nx || j > g.ny) return;
+  const int64_t c = g.idx(i, j, k);
+  T ta, tb, tc, td, te, tf;
+  der_weno_cell<T>(phi, g, i, j, k, c, false, ta, tb, tc, td, te, tf);
+  const T lsv = phi[c];
+  const T m = godunov_mod<T>(lsv, ta, tb, tc, td, te, tf);
+  S[c] = lsv / sqrt(lsv * lsv + m * m * dx1 * dx1);
+}
+
+// One SSP-RK3 stage of the reinitialisation (:443-484).  On the last stage
+// (band != 0) it also sums, in FP64, the change phi_new - phi_old and the
+// number of cells in the band abs(phi_new) <= 2*alpha1*dx1 (:493-503).
+template <typename T>
+__global__ __launch_bounds__(256) void k_reinit_stage(Grid g, const T *__restrict__ phi,
+                                                      const T *__restrict__ phit,
+                                                      const T *__restrict__ S, T *__restrict__ out,
+                                                      T ca, T cb, T cgdt, int band, double bandw,
+                                                      RedSlot rs) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  double acc[2] = {0.0, 0.0};
+  if (i <= g.nx && j <= g.ny) {
+    const int64_t c = g.idx(i, j, k);
+    T ta, tb, tc, td, te, tf;
+    der_weno_cell<T>(phi, g, i, j, k, c, false, ta, tb, tc, td, te, tf);
+    const T lsv = phi[c];
+    const T m = godunov_mod<T>(lsv, ta, tb, tc, td, te, tf);
+    const T sy7 = S[c] * (T(1.) - m);
+    const T nv = ca * phit[c] + cb * lsv + cgdt * sy7;
+    out[c] = nv;
+    if (band) {
+      const double nd = (double)nv;
+      if ((nd < 0 ? -nd : nd) <= bandw) {
+        acc[0] = nd - (double)lsv;
+        acc[1] = 1.0;
+      }
+    }
+  }
+  if (band) {
+    __shared__ double lds[2 * SULI_LDS_RED];
+    block_reduce<2>(acc, lds);
+    grid_reduce_finish<2>(acc, rs);
+  }
+}
+
+// heaviside (:612-707) fused with the first half of mod_ls1 (:717-759) and the
+// volume integral (:246-253): both take the larger-magnitude one-sided WENO
+// derivative in each direction.  Writes H, grad H, rho, mu, |grad phi| and the
+// unit normal; sums (1-(1-H))*dV in FP64.
+template <typename T>
+__global__ __launch_bounds__(256) void k_heavi(Grid g, Phys ph, const T *__restrict__ phi,
+                                               T *__restrict__ hs, T *__restrict__ hsx,
+                                               T *__restrict__ hsy, T *__restrict__ hsz,
+                                               T *__restrict__ rho, T *__restrict__ lsnu,
+                                               T *__restrict__ modls, T *__restrict__ nrx,
+                                               T *__restrict__ nry, T *__restrict__ nrz,
+                                               RedSlot rs) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  double acc[1] = {0.0};
+  if (i <= g.nx && j <= g.ny) {
+    const int64_t c = g.idx(i, j, k);
+    T ta, tb, tc, td, te, tf;
+    der_weno_cell<T>(phi, g, i, j, k, c, false, ta, tb, tc, td, te, tf);
+    if (!(tabs(ta) > tabs(td))) ta = td;
+    if (!(tabs(tb) > tabs(te))) tb = te;
+    if (!(tabs(tc) > tabs(tf))) tc = tf;
+    const T m = sqrt(ta * ta + tb * tb + tc * tc);
+    modls[c] = m;
+    if (m == T(0)) {
+      nrx[c] = T(0); nry[c] = T(0); nrz[c] = T(0);
+    } else {
+      nrx[c] = ta / m; nry[c] = tb / m; nrz[c] = tc / m;
+    }
+    const T lsv = phi[c];
+    const T a = T(ph.alpha1), dx1 = T(ph.dx1);
+    const T pi = T(SULI_PI);
+    T h, hx, hy, hz;
+    if (lsv < -a * dx1) {
+      h = T(0); hx = T(0); hy = T(0); hz = T(0);
+    } else if (lsv > a * dx1) {
+      h = T(1); hx = T(0); hy = T(0); hz = T(0);
+    } else {
+      h = T(0.5) * (T(1.) + lsv / (a * dx1) + T(1.) / pi * sin(pi * lsv / (a * dx1)));
+      const T cs = cos(pi * lsv / (a * dx1));
+      hx = T(0.5) * ta * (T(1.) + cs) / (a * dx1);
+      hy = T(0.5) * tb * (T(1.) + cs) / (a * dx1);
+      hz = T(0.5) * tc * (T(1.) + cs) / (a * dx1);
+    }
+    hs[c] = h; hsx[c] = hx; hsy[c] = hy; hsz[c] = hz;
+    rho[c] = T(ph.rho_f1) * (T(1.) - h) + T(ph.rho_f2) * h;
+    lsnu[c] = T(ph.mi_f1) * (T(1.) - h) + T(ph.mi_f2) * h;
+    acc[0] = (1. - (1. - (double)h)) * g.dx * g.dy * g.dz;
+  }
+  __shared__ double lds[SULI_LDS_RED];
+  block_reduce<1>(acc, lds);
+  grid_reduce_finish<1>(acc, rs);
+}
+
+// Second half of mod_ls1 (:761-803): WENO (ihs = 1 everywhere) of each normal
+// component along its own direction, larger-magnitude selection, curvature.
+template <typename T>
+__global__ __launch_bounds__(256) void k_curv(Grid g, const T *__restrict__ nrx,
+                                              const T *__restrict__ nry, const T *__restrict__ nrz,
+                                              T *__restrict__ ddx, T *__restrict__ ddy,
+                                              T *__restrict__ ddz, T *__restrict__ kurv) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (i > g.nx || j > g.ny) return;
+  const int64_t c = g.idx(i, j, k);
+  T w[7], ta, tb, tc, td, te, tf;
+  window7<T>(nrx, c, 1, i, g.nx, 1, w);
+  weno_pm<T>(w, T(g.dx), ta, td);
+  window7<T>(nry, c, g.sx, j, g.ny, 1, w);
+  weno_pm<T>(w, T(g.dy), tb, te);
+  window7<T>(nrz, c, g.sz, k, g.nz, 1, w);
+  weno_pm<T>(w, T(g.dz), tc, tf);
+  const T a = tabs(td) > tabs(ta) ? td : ta;
+  const T b = tabs(te) > tabs(tb) ? te : tb;
+  const T cc = tabs(tf) > tabs(tc) ? tf : tc;
+  ddx[c] = a; ddy[c] = b; ddz[c] = cc;
+  kurv[c] = a + b + cc;
+}
+
+// Volume correction step (:259-265): phi -= (dt1*dV)*|grad phi|/V0.
+template <typename T>
+__global__ __launch_bounds__(256) void k_masscorr(Grid g, T *__restrict__ phi,
+                                                  const T *__restrict__ modls, T f, T vol0) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (i > g.nx || j > g.ny) return;
+  const int64_t c = g.idx(i, j, k);
+  phi[c] = phi[c] - f * modls[c] / vol0;
+}
+
+// ---------------------------------------------------------------------------
+// Smagorinsky eddy viscosity, visco m_turb = 1 (5_viscosidade.f90:61-97):
+// central derivatives at the faces (8_derivadas.f90:1-68) averaged to the
+// cell centre (8_derivadas.f90:331-455) and nu_t = c23*Cs^2*sqrt(2 S^2)*rho.
+template <typename T>
+__global__ __launch_bounds__(256) void k_visco(Grid g, T c23, T cs, const T *__restrict__ u,
+                                               const T *__restrict__ v, const T *__restrict__ w,
+                                               const T *__restrict__ rho, T *__restrict__ nut) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (i > g.nx || j > g.ny) return;
+  const int64_t c = g.idx(i, j, k);
+  const int64_t X = 1, Y = g.sx, Z = g.sz;
+  const T h2x = T(2.) * T(g.dx), h2y = T(2.) * T(g.dy), h2z = T(2.) * T(g.dz);
+  // u at faces i (c) and i+1 (c+X)
+  const T dudx = ((u[c + X] - u[c - X]) / h2x + (u[c + 2 * X] - u[c]) / h2x) * T(0.5);
+  const T dudy = ((u[c + Y] - u[c - Y]) / h2y + (u[c + X + Y] - u[c + X - Y]) / h2y) * T(0.5);
+  const T dudz = ((u[c + Z] - u[c - Z]) / h2z + (u[c + X + Z] - u[c + X - Z]) / h2z) * T(0.5);
+  // v at faces j (c) and j+1 (c+Y)
+  const T dvdx = ((v[c + X] - v[c - X]) / h2x + (v[c + Y + X] - v[c + Y - X]) / h2x) * T(0.5);
+  const T dvdy = ((v[c + Y] - v[c - Y]) / h2y + (v[c + 2 * Y] - v[c]) / h2y) * T(0.5);
+  const T dvdz = ((v[c + Z] - v[c - Z]) / h2z + (v[c + Y + Z] - v[c + Y - Z]) / h2z) * T(0.5);
+  // w at faces k (c) and k+1 (c+Z)
+  const T dwdx = ((w[c + X] - w[c - X]) / h2x + (w[c + Z + X] - w[c + Z - X]) / h2x) * T(0.5);
+  const T dwdy = ((w[c + Y] - w[c - Y]) / h2y + (w[c + Z + Y] - w[c + Z - Y]) / h2y) * T(0.5);
+  const T dwdz = ((w[c + Z] - w[c - Z]) / h2z + (w[c + 2 * Z] - w[c]) / h2z) * T(0.5);
+  const T p1 = dudx + T(0.5) * (dudy + dvdx + dudz + dwdx);
+  const T p2 = dvdy + T(0.5) * (dvdx + dudy + dvdz + dwdy);
+  const T p3 = dwdz + T(0.5) * (dwdy + dvdz + dudz + dwdx);
+  nut[c] = c23 * cs * cs * sqrt(T(2.) * (p1 + p2 + p3) * (p1 + p2 + p3)) * rho[c];
+}
+
+// ---------------------------------------------------------------------------
+// Momentum: convdiff (6_convdiff.f90:14-189) with first-order upwind
+// advection classico der=1 (7_advectivo.f90:33-113), fused with the RK2
+// stage update tempo (11_diversos.f90:54-111).  The face viscosities
+// xnut/ynut/znut (5_viscosidade.f90:178-211) and their edge averages
+// (6_convdiff.f90:118-145) are rebuilt on the fly from the cell arrays nut
+// and ls_nu with the reference's end-face copies.
+//
+// The friction term -(gz/C^2)*|ub|*u (6_convdiff.f90:156,168,183) reads ub
+// from u itself: contorno copies u into ub on the planes k=2, j=2, j=ny
+// (3_contorno.f90:247-264) and contorno(1) always runs between the last
+// change of u and convdiff (1_principal.f90:46,92), so ub equals u there and
+// is zero elsewhere.
+template <typename T>
+struct MomArgs {
+  const T *u, *v, *w, *u0, *v0, *w0;
+  const T *rho, *lsnu, *nut, *hsx, *hsy, *hsz, *kurv;
+  const T *tfu, *tfv, *tfw;  // optional source terms (may be null)
+  T *un, *vn, *wn, *u0o, *v0o, *w0o;
+  T *Fu, *Fv, *Fw;           // optional force output (may be null)
+  T dt;
+  int stage;                 // 1 or 2
+};
+
+template <typename T>
+__device__ __forceinline__ T interp_face(const T *__restrict__ a, int64_t cf, int64_t st, int f,
+                                         int n) {
+  // interp*_cf at face f (8_derivadas.f90:304-434): average of cells fc-1, fc
+  // with fc = clamp(f, 2, n); cf is the address of face f's cell (f).
+  const int fc = f < 2 ? 2 : (f > n ? n : f);
+  const int64_t cc = cf + (int64_t)(fc - f) * st;
+  return (a[cc] + a[cc - st]) * T(0.5);
+}
+template <typename T>
+__device__ __forceinline__ T cell_clamp(const T *__restrict__ a, int64_t cf, int64_t st, int f,
+                                        int n) {
+  const int fc = f < 1 ? 1 : (f > n ? n : f);
+  return a[cf + (int64_t)(fc - f) * st];
+}
+// face viscosity along d1 at (face f1 along d1), e.g. xnut(i,j,k) = ls_nux + nutx
+template <typename T>
+__device__ __forceinline__ T face_visc(const T *__restrict__ lsnu, const T *__restrict__ nut,
+                                       int64_t cf, int64_t st, int f, int n) {
+  return interp_face<T>(lsnu, cf, st, f, n) + interp_face<T>(nut, cf, st, f, n);
+}
+// edge value: interp along d2 of the face viscosity along d1, divided by h2
+template <typename T>
+__device__ __forceinline__ T edge_visc(const T *__restrict__ lsnu, const T *__restrict__ nut,
+                                       int64_t ce, int64_t st1, int f1, int n1, int64_t st2, int f2,
+                                       int n2, T h2) {
+  const int fc = f2 < 2 ? 2 : (f2 > n2 ? n2 : f2);
+  const int64_t cc = ce + (int64_t)(fc - f2) * st2;
+  return ((face_visc<T>(lsnu, nut, cc, st1, f1, n1) + face_visc<T>(lsnu, nut, cc - st2, st1, f1, n1)) *
+          T(0.5)) / h2;
+}
+template <typename T>
+__device__ __forceinline__ T surf_cell(const T *__restrict__ hd, const T *__restrict__ kurv,
+                                       const T *__restrict__ rho, int64_t cc) {
+  return hd[cc] * kurv[cc] / rho[cc];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_momentum(Grid g, Phys ph, MomArgs<T> A) {
+  const int i = blockIdx.x * SULI_BX + threadIdx.x, j = blockIdx.y * SULI_BY + threadIdx.y,
+            k = blockIdx.z;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i > nx + 2 || j > ny + 2) return;
+  const int64_t c = g.idx(i, j, k);
+  const int64_t X = 1, Y = g.sx, Z = g.sz;
+  const T dx = T(g.dx), dy = T(g.dy), dz = T(g.dz);
+  const T *__restrict__ u = A.u;
+  const T *__restrict__ v = A.v;
+  const T *__restrict__ w = A.w;
+  const T fr = T(ph.gz / (ph.chezy * ph.chezy));
+  const T sigma = T(ph.sigma);
+  // ---- u faces: i in [1,nx+1], j in [1,ny], k in [1,nz]
+  if (i <= nx + 2 && j <= ny + 1 && k <= nz + 1) {
+    if (i >= 1 && i <= nx + 1 && j >= 1 && j <= ny && k >= 1 && k <= nz) {
+      const T uc = u[c];
+      const T bma_a = (v[c] + v[c - X] + v[c + Y] + v[c - X + Y]) * T(0.25);
+      const T dma_a = (w[c] + w[c - X] + w[c + Z] + w[c - X + Z]) * T(0.25);
+      T a1 = tmax(uc, T(0)), a2 = tmin(uc, T(0));
+      const T dudx = a1 * (uc - u[c - X]) / dx + a2 * (u[c + X] - uc) / dx;
+      a1 = tmax(bma_a, T(0)); a2 = tmin(bma_a, T(0));
+      const T dudy = a1 * (uc - u[c - Y]) / dy + a2 * (u[c + Y] - uc) / dy;
+      a1 = tmax(dma_a, T(0)); a2 = tmin(dma_a, T(0));
+      const T dudz = a1 * (uc - u[c - Z]) / dz + a2 * (u[c + Z] - uc) / dz;
+      const T uint_ = dudx + dudy + dudz;
+      const T amai = cell_clamp<T>(A.nut, c, X, i, nx) / dx;
+      const T amam = cell_clamp<T>(A.nut, c - X, X, i - 1, nx) / dx;
+      const T bmap = edge_visc<T>(A.lsnu, A.nut, c + Y, X, i, nx, Y, j + 1, ny, dy);
+      const T bmac = edge_visc<T>(A.lsnu, A.nut, c, X, i, nx, Y, j, ny, dy);
+      const T dmap = edge_visc<T>(A.lsnu, A.nut, c + Z, X, i, nx, Z, k + 1, nz, dz);
+      const T dmac = edge_visc<T>(A.lsnu, A.nut, c, X, i, nx, Z, k, nz, dz);
+      const T rhox = interp_face<T>(A.rho, c, X, i, nx);
+      const int ic = i < 2 ? 2 : (i > nx ? nx : i);
+      const int64_t cc = c + (int64_t)(ic - i);
+      const T dhsdx = (surf_cell<T>(A.hsx, A.kurv, A.rho, cc) + surf_cell<T>(A.hsx, A.kurv, A.rho, cc - X)) * T(0.5);
+      const bool pl = (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
+      const T ub = pl ? uc : T(0);
+      const T tf = A.tfu ? A.tfu[c] : T(0);
+      const T F = -uint_ +
+                  ((amai * T(2.) * (u[c + X] - uc) - amam * T(2.) * (uc - u[c - X])) / dx +
+                   bmap * ((u[c + Y] - uc) / dy + (v[c + Y] - v[c - X + Y]) / dx) -
+                   bmac * ((uc - u[c - Y]) / dy + (v[c] - v[c - X]) / dx) +
+                   dmap * ((u[c + Z] - uc) / dz + (w[c + Z] - w[c - X + Z]) / dx) -
+                   dmac * ((uc - u[c - Z]) / dz + (w[c] - w[c - X]) / dx)) /
+                      rhox +
+                  T(ph.gx) - sigma * dhsdx + tf - fr * sqrt(ub * ub) * uc;
+      if (A.Fu) A.Fu[c] = F;
+      if (A.stage == 1) {
+        A.u0o[c] = uc;
+        A.un[c] = uc + A.dt * F;
+      } else {
+        A.un[c] = A.u0[c] + A.dt * F;
+      }
+    } else {
+      A.un[c] = u[c];
+    }
+  }
+  // ---- v faces: i in [1,nx], j in [1,ny+1], k in [1,nz]
+  if (i <= nx + 1 && j <= ny + 2 && k <= nz + 1) {
+    if (i >= 1 && i <= nx && j >= 1 && j <= ny + 1 && k >= 1 && k <= nz) {
+      const T vc = v[c];
+      const T amb_a = (u[c] + u[c + X] + u[c - Y] + u[c + X - Y]) * T(0.25);
+      const T dmb_a = (w[c] + w[c - Y] + w[c + Z] + w[c - Y + Z]) * T(0.25);
+      T a1 = tmax(amb_a, T(0)), a2 = tmin(amb_a, T(0));
+      const T dvdx = a1 * (vc - v[c - X]) / dx + a2 * (v[c + X] - vc) / dx;
+      a1 = tmax(vc, T(0)); a2 = tmin(vc, T(0));
+      const T dvdy = a1 * (vc - v[c - Y]) / dy + a2 * (v[c + Y] - vc) / dy;
+      a1 = tmax(dmb_a, T(0)); a2 = tmin(dmb_a, T(0));
+      const T dvdz = a1 * (vc - v[c - Z]) / dz + a2 * (v[c + Z] - vc) / dz;
+      const T vint_ = dvdx + dvdy + dvdz;
+      const T ambp = edge_visc<T>(A.lsnu, A.nut, c + X, Y, j, ny, X, i + 1, nx, dx);
+      const T ambc = edge_visc<T>(A.lsnu, A.nut, c, Y, j, ny, X, i, nx, dx);
+      const T bmbj = cell_clamp<T>(A.nut, c, Y, j, ny) / dy;
+      const T bmbm = cell_clamp<T>(A.nut, c - Y, Y, j - 1, ny) / dy;
+      const T dmbp = edge_visc<T>(A.lsnu, A.nut, c + Z, Y, j, ny, Z, k + 1, nz, dz);
+      const T dmbc = edge_visc<T>(A.lsnu, A.nut, c, Y, j, ny, Z, k, nz, dz);
+      const T rhoy = interp_face<T>(A.rho, c, Y, j, ny);
+      const int jc = j < 2 ? 2 : (j > ny ? ny : j);
+      const int64_t cc = c + (int64_t)(jc - j) * Y;
+      const T dhsdy = (surf_cell<T>(A.hsy, A.kurv, A.rho, cc) + surf_cell<T>(A.hsy, A.kurv, A.rho, cc - Y)) * T(0.5);
+      const bool pl = (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
+      const T vb = pl ? vc : T(0);
+      const T tf = A.tfv ? A.tfv[c] : T(0);
+      const T F = -vint_ +
+                  (ambp * ((v[c + X] - vc) / dx + (u[c + X] - u[c + X - Y]) / dy) -
+                   ambc * ((vc - v[c - X]) / dx + (u[c] - u[c - Y]) / dy) +
+                   (bmbj * T(2.) * (v[c + Y] - vc) - bmbm * T(2.) * (vc - v[c - Y])) / dy +
+                   dmbp * ((v[c + Z] - vc) / dz + (w[c + Z] - w[c - Y + Z]) / dy) -
+                   dmbc * ((vc - v[c - Z]) / dz + (w[c] - w[c - Y]) / dy)) /
+                      rhoy -
+                  sigma * dhsdy + tf - fr * sqrt(vb * vb) * vc;
+      if (A.Fv) A.Fv[c] = F;
+      if (A.stage == 1) {
+        A.v0o[c] = vc;
+        A.vn[c] = vc + A.dt * F;
+      } else {
+        A.vn[c] = A.v0[c] + A.dt * F;
+      }
+    } else {
+      A.vn[c] = v[c];
+    }
+  }
+  // ---- w faces: i in [1,nx], j in [1,ny], k in [1,nz+1]
+  if (i <= nx + 1 && j <= ny + 1 && k <= nz + 2) {
+    if (i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz + 1) {
+      const T wc = w[c];
+      const T amd_a = (u[c] + u[c + X] + u[c - Z] + u[c + X - Z]) * T(0.25);
+      const T bmd_a = (v[c] + v[c + Y] + v[c - Z] + v[c + Y - Z]) * T(0.25);
+      T a1 = tmax(amd_a, T(0)), a2 = tmin(amd_a, T(0));
+      const T dwdx = a1 * (wc - w[c - X]) / dx + a2 * (w[c + X] - wc) / dx;
+      a1 = tmax(bmd_a, T(0)); a2 = tmin(bmd_a, T(0));
+      const T dwdy = a1 * (wc - w[c - Y]) / dy + a2 * (w[c + Y] - wc) / dy;
+      a1 = tmax(wc, T(0)); a2 = tmin(wc, T(0));
+      const T dwdz = a1 * (wc - w[c - Z]) / dz + a2 * (w[c + Z] - wc) / dz;
+      const T wint_ = dwdx + dwdy + dwdz;
+      const T amdp = edge_visc<T>(A.lsnu, A.nut, c + X, Z, k, nz, X, i + 1, nx, dx);
+      const T amdc = edge_visc<T>(A.lsnu, A.nut, c, Z, k, nz, X, i, nx, dx);
+      const T bmdp = edge_visc<T>(A.lsnu, A.nut, c + Y, Z, k, nz, Y, j + 1, ny, dy);
+      const T bmdc = edge_visc<T>(A.lsnu, A.nut, c, Z, k, nz, Y, j, ny, dy);
+      const T dmdk = cell_clamp<T>(A.nut, c, Z, k, nz) / dz;
+      const T dmdm = cell_clamp<T>(A.nut, c - Z, Z, k - 1, nz) / dz;
+      const T rhoz = interp_face<T>(A.rho, c, Z, k, nz);
+      const int kc = k < 2 ? 2 : (k > nz ? nz : k);
+      const int64_t cc = c + (int64_t)(kc - k) * Z;
+      const T dhsdz = (surf_cell<T>(A.hsz, A.kurv, A.rho, cc) + surf_cell<T>(A.hsz, A.kurv, A.rho, cc - Z)) * T(0.5);
+      const bool pl = (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
+      const T wb = pl ? wc : T(0);
+      const T tf = A.tfw ? A.tfw[c] : T(0);
+      const T epis_z = T(0);
+      const T F = -wint_ +
+                  (amdp * ((w[c + X] - wc) / dx + (u[c + X] - u[c + X - Z]) / dz) -
+                   amdc * ((wc - w[c - X]) / dx + (u[c] - u[c - Z]) / dz) +
+                   bmdp * ((w[c + Y] - wc) / dy + (v[c + Y] - v[c + Y - Z]) / dz) -
+                   bmdc * ((wc - w[c - Y]) / dy + (v[c] - v[c - Z]) / dz) +
+                   (dmdk * T(2.) * (w[c + Z] - wc) - dmdm * T(2.) * (wc - w[c - Z])) / dz) /
+                      rhoz -
+                  (wc - T(0.)) * epis_z - T(ph.gz) - sigma * dhsdz + tf - fr * sqrt(wb * wb) * wc;
+      if (A.Fw) A.Fw[c] = F;
+      if (A.stage == 1) {
+        A.w0o[c] = wc;
+        A.wn[c] = wc + A.dt * F;
+      } else {
+        A.wn[c] = A.w0[c] + A.dt * F;
+      }
+    } else {
+      A.wn[c] = w[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary conditions contorno(nlock) (3_contorno.f90:29-443) with prd_corr
+// (:872-950, whose corrections are all zero).  Wall codes: 0 periodic,
+// 1 free slip, 2 no slip, 4 outflow (x = xf only).  The reference fills the
+// ghost planes in the order y walls, x walls, z walls (pressure: x, y, z;
+// plain copies commute), so each phase is one launch over its ghost planes.
+template <typename T>
+__global__ void k_bc_y(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
+                       double *__restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i > nx + 2 || k > nz + 2) return;
+  const bool per = g.ccy0 == 0 && g.ccyf == 0;
+  if (i <= nx + 1 && k <= nz + 1) {
+    if (per) { p[g.idx(i, 0, k)] = p[g.idx(i, ny, k)]; p[g.idx(i, ny + 1, k)] = p[g.idx(i, 1, k)]; }
+    else { p[g.idx(i, 0, k)] = p[g.idx(i, 1, k)]; p[g.idx(i, ny + 1, k)] = p[g.idx(i, ny, k)]; }
+  }
+  if (k <= nz + 1) {  // u: i in [0,nx+2]
+    if (per) u[g.idx(i, 0, k)] = u[g.idx(i, ny, k)];
+    else if (g.ccy0 == 1) u[g.idx(i, 0, k)] = u[g.idx(i, 1, k)];
+    else if (g.ccy0 == 2) u[g.idx(i, 0, k)] = -u[g.idx(i, 1, k)];
+    if (per) u[g.idx(i, ny + 1, k)] = u[g.idx(i, 1, k)];
+    else if (g.ccyf == 1) u[g.idx(i, ny + 1, k)] = u[g.idx(i, ny, k)];
+    else if (g.ccyf == 2) u[g.idx(i, ny + 1, k)] = -u[g.idx(i, ny, k)];
+  }
+  if (i <= nx + 1 && k <= nz + 1) {  // v
+    if (per) { v[g.idx(i, 0, k)] = v[g.idx(i, ny, k)]; v[g.idx(i, 1, k)] = v[g.idx(i, ny + 1, k)]; }
+    else if (g.ccy0 == 1) v[g.idx(i, 0, k)] = -v[g.idx(i, 2, k)];
+    else if (g.ccy0 == 2) v[g.idx(i, 0, k)] = v[g.idx(i, 2, k)];
+    if (per) v[g.idx(i, ny + 2, k)] = v[g.idx(i, 2, k)];
+    else if (g.ccyf == 1) v[g.idx(i, ny + 2, k)] = -v[g.idx(i, ny, k)];
+    else if (g.ccyf == 2) v[g.idx(i, ny + 2, k)] = v[g.idx(i, ny, k)];
+  }
+  if (i <= nx + 1) {  // w: k in [0,nz+2]
+    if (per) w[g.idx(i, 0, k)] = w[g.idx(i, ny, k)];
+    else if (g.ccy0 == 1) w[g.idx(i, 0, k)] = w[g.idx(i, 1, k)];
+    else if (g.ccy0 == 2) w[g.idx(i, 0, k)] = -w[g.idx(i, 1, k)];
+    if (per) w[g.idx(i, ny + 1, k)] = w[g.idx(i, 1, k)];
+    else if (g.ccyf == 1) w[g.idx(i, ny + 1, k)] = w[g.idx(i, ny, k)];
+    else if (g.ccyf == 2) w[g.idx(i, ny + 1, k)] = -w[g.idx(i, ny, k)];
+  }
+}
+
+template <typename T>
+__global__ void k_bc_x(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
+                       double *__restrict__ p) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (j > ny + 2 || k > nz + 2) return;
+  const bool per = g.ccx0 == 0 && g.ccxf == 0;
+  if (j <= ny + 1 && k <= nz + 1) {
+    if (per) { p[g.idx(0, j, k)] = p[g.idx(nx, j, k)]; p[g.idx(nx + 1, j, k)] = p[g.idx(1, j, k)]; }
+    else { p[g.idx(0, j, k)] = p[g.idx(1, j, k)]; p[g.idx(nx + 1, j, k)] = p[g.idx(nx, j, k)]; }
+    // u: j in [0,ny+1], k in [0,nz+1]
+    if (per) u[g.idx(0, j, k)] = u[g.idx(nx, j, k)];
+    else if (g.ccx0 == 1) u[g.idx(0, j, k)] = -u[g.idx(2, j, k)];
+    else if (g.ccx0 == 2) u[g.idx(0, j, k)] = u[g.idx(2, j, k)];
+    if (per) u[g.idx(nx + 2, j, k)] = u[g.idx(2, j, k)];
+    else if (g.ccxf == 1) u[g.idx(nx + 2, j, k)] = -u[g.idx(nx, j, k)];
+    else if (g.ccxf == 2) u[g.idx(nx + 2, j, k)] = u[g.idx(nx, j, k)];
+    else if (g.ccxf == 4) u[g.idx(nx + 2, j, k)] = u[g.idx(nx + 1, j, k)];
+  }
+  if (k <= nz + 1) {  // v: j in [0,ny+2]
+    if (per) v[g.idx(0, j, k)] = v[g.idx(nx, j, k)];
+    else if (g.ccx0 == 1) v[g.idx(0, j, k)] = v[g.idx(1, j, k)];
+    else if (g.ccx0 == 2) v[g.idx(0, j, k)] = -v[g.idx(1, j, k)];
+    if (per) v[g.idx(nx + 1, j, k)] = v[g.idx(1, j, k)];
+    else if (g.ccxf == 1 || g.ccxf == 4) v[g.idx(nx + 1, j, k)] = v[g.idx(nx, j, k)];
+    else if (g.ccxf == 2) v[g.idx(nx + 1, j, k)] = -v[g.idx(nx, j, k)];
+  }
+  if (j <= ny + 1) {  // w: k in [0,nz+2]
+    if (per) w[g.idx(0, j, k)] = w[g.idx(nx, j, k)];
+    else if (g.ccx0 == 1) w[g.idx(0, j, k)] = w[g.idx(1, j, k)];
+    else if (g.ccx0 == 2) w[g.idx(0, j, k)] = -w[g.idx(1, j, k)];
+    if (per) w[g.idx(nx + 1, j, k)] = w[g.idx(1, j, k)];
+    else if (g.ccxf == 1 || g.ccxf == 4) w[g.idx(nx + 1, j, k)] = w[g.idx(nx, j, k)];
+    else if (g.ccxf == 2) w[g.idx(nx + 1, j, k)] = -w[g.idx(nx, j, k)];
+  }
+}
+
+template <typename T>
+__global__ void k_bc_z(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
+                       double *__restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i > nx + 2 || j > ny + 2) return;
+  if (i <= nx + 1 && j <= ny + 1) {
+    p[g.idx(i, j, 0)] = p[g.idx(i, j, 1)];
+    p[g.idx(i, j, nz + 1)] = p[g.idx(i, j, nz)];
+  }
+  if (j <= ny + 1) {  // u: i in [0,nx+2]
+    if (g.ccz0 == 1) u[g.idx(i, j, 0)] = u[g.idx(i, j, 1)];
+    else if (g.ccz0 == 2) u[g.idx(i, j, 0)] = -u[g.idx(i, j, 1)];
+    if (g.cczf == 1) u[g.idx(i, j, nz + 1)] = u[g.idx(i, j, nz)];
+    else if (g.cczf == 2) u[g.idx(i, j, nz + 1)] = -u[g.idx(i, j, nz)];
+  }
+  if (i <= nx + 1) {  // v: j in [0,ny+2]
+    if (g.ccz0 == 1) v[g.idx(i, j, 0)] = v[g.idx(i, j, 1)];
+    else if (g.ccz0 == 2) v[g.idx(i, j, 0)] = -v[g.idx(i, j, 1)];
+    if (g.cczf == 1) v[g.idx(i, j, nz + 1)] = v[g.idx(i, j, nz)];
+    else if (g.cczf == 2) v[g.idx(i, j, nz + 1)] = -v[g.idx(i, j, nz)];
+  }
+  if (i <= nx + 1 && j <= ny + 1) {  // w
+    if (g.ccz0 == 1) w[g.idx(i, j, 0)] = -w[g.idx(i, j, 2)];
+    else if (g.ccz0 == 2) w[g.idx(i, j, 0)] = w[g.idx(i, j, 2)];
+    if (g.cczf == 1) w[g.idx(i, j, nz + 2)] = -w[g.idx(i, j, nz)];
+    else if (g.cczf == 2) w[g.idx(i, j, nz + 2)] = w[g.idx(i, j, nz)];
+  }
+}
+
+// contorno(2) wall-normal velocities (3_contorno.f90:297-443), in three
+// launches: y walls, x walls (with the ccxf=2 ghost refresh), z walls.
+template <typename T>
+__global__ void k_bcn_y(Grid g, T *__restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.nx + 1 || k > g.nz + 1) return;
+  const bool per = g.ccy0 == 0 && g.ccyf == 0;
+  if (per) v[g.idx(i, 1, k)] = v[g.idx(i, g.ny + 1, k)];
+  else if (g.ccy0 == 1 || g.ccy0 == 2) v[g.idx(i, 1, k)] = T(0.) + T(0.);
+  if (per) v[g.idx(i, g.ny + 1, k)] = v[g.idx(i, 1, k)];
+  else if (g.ccyf == 1 || g.ccyf == 2) v[g.idx(i, g.ny + 1, k)] = T(0.) + T(0.);
+}
+template <typename T>
+__global__ void k_bcn_x(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  const int nx = g.nx;
+  if (j > g.ny + 2 || k > g.nz + 2) return;
+  const bool per = g.ccx0 == 0 && g.ccxf == 0;
+  if (j <= g.ny + 1 && k <= g.nz + 1) {
+    if (per) u[g.idx(1, j, k)] = u[g.idx(nx + 1, j, k)];
+    else if (g.ccx0 == 1 || g.ccx0 == 2) u[g.idx(1, j, k)] = T(0.) + T(0.);
+    if (per) u[g.idx(nx + 1, j, k)] = u[g.idx(1, j, k)];
+    else if (g.ccxf == 1) u[g.idx(nx + 1, j, k)] = T(0.) + T(0.);
+    else if (g.ccxf == 2) {
+      u[g.idx(nx + 1, j, k)] = T(0.) + T(0.);
+      u[g.idx(nx + 2, j, k)] = u[g.idx(nx, j, k)];
+    } else if (g.ccxf == 4) u[g.idx(nx + 1, j, k)] = u[g.idx(nx, j, k)];
+  }
+  if (g.ccxf == 2) {
+    if (k <= g.nz + 1) v[g.idx(nx + 1, j, k)] = -v[g.idx(nx, j, k)];
+    if (j <= g.ny + 1) w[g.idx(nx + 1, j, k)] = -w[g.idx(nx, j, k)];
+  }
+}
+template <typename T>
+__global__ void k_bcn_z(Grid g, T *__restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.nx + 1 || j > g.ny + 1) return;
+  if (g.ccz0 == 1 || g.ccz0 == 2) w[g.idx(i, j, 1)] = T(0.) + T(0.);
+  if (g.cczf == 1 || g.cczf == 2) w[g.idx(i, j, g.nz + 1)] = T(0.) + T(0.);
+}
+
+// ---------------------------------------------------------------------------
+// Velocity correction posdin (10_pos.f90:17-45): u -= (dt/h)*(p_i - p_{i-1})/rho_face
+// over every face including the walls.  The pressure is FP64; the update is
+// done in FP64 and rounded to T.
+template <typename T>
+__global__ __launch_bounds__(256) void k_posdin(Grid g, double dt, const double *__restrict__ p,
+                                                const T *__restrict__ rho, T *__restrict__ u,
+                                                T *__restrict__ v, T *__restrict__ w) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i > nx + 1 || j > ny + 1) return;
+  const int64_t c = g.idx(i, j, k);
+  const double a1 = dt / g.dx, a2 = dt / g.dy, a3 = dt / g.dz;
+  if (j <= ny && k <= nz) {
+    const double rx = (double)interp_face<T>(rho, c, 1, i, nx);
+    u[c] = (T)((double)u[c] - a1 * (p[c] - p[c - 1]) / rx);
+  }
+  if (i <= nx && k <= nz) {
+    const double ry = (double)interp_face<T>(rho, c, g.sx, j, ny);
+    v[c] = (T)((double)v[c] - a2 * (p[c] - p[c - g.sx]) / ry);
+  }
+  if (i <= nx && j <= ny) {
+    const double rz = (double)interp_face<T>(rho, c, g.sz, k, nz);
+    w[c] = (T)((double)w[c] - a3 * (p[c] - p[c - g.sz]) / rz);
+  }
+}
+
+// max abs(div u) over the cells (est, 4_plot.f90:509-514), FP64.
+template <typename T>
+__global__ __launch_bounds__(256) void k_divmax(Grid g, const T *__restrict__ u,
+                                                const T *__restrict__ v, const T *__restrict__ w,
+                                                RedSlot rs) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  double acc[1] = {0.0};
+  if (i <= g.nx && j <= g.ny) {
+    const int64_t c = g.idx(i, j, k);
+    const double d = ((double)u[c + 1] - (double)u[c]) / g.dx + ((double)v[c + g.sx] - (double)v[c]) / g.dy +
+                     ((double)w[c + g.sz] - (double)w[c]) / g.dz;
+    acc[0] = d < 0 ? -d : d;
+  }
+  __shared__ double lds[SULI_LDS_RED];
+  block_reduce<1, true>(acc, lds);
+  grid_reduce_finish<1, true>(acc, rs);
+}

@@ -1,0 +1,222 @@
+"""ctypes binding of libsuli.so (include/suli.h) for Python callers.
+
+Mirrors the reference's calling convention: the solver state lives on the
+GPU; whole arrays in the reference's extents and Fortran order are exchanged
+with upload()/download(); the time step is driven by one call per reference
+subroutine (1_principal.f90:74-92).  Every call raises SuliError when the
+library reports a nonzero status, and loading fails loudly when libsuli.so
+is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsuli.so")
+
+PREC_FP64, PREC_MIXED = 0, 1
+POISSON_REF, POISSON_FCG = 0, 1
+PRECOND_JACOBI, PRECOND_GMG, PRECOND_LEARNED = 0, 1, 2
+
+FIELDS = ["u", "v", "w", "prd1", "rho", "ls_nu", "ls", "mod_ls", "kurv", "hs", "hsx", "hsy", "hsz",
+          "ddlsdx", "ddlsdy", "ddlsdz", "nut", "xnut", "ynut", "znut", "Fu", "Fv", "Fw", "u0", "v0", "w0"]
+FIELD_ID = {n: q for q, n in enumerate(FIELDS)}
+KIND = {"u": "U", "v": "V", "w": "W", "prd1": "P", "xnut": "FX", "ynut": "FY", "znut": "FZ",
+        "Fu": "FX", "Fv": "FY", "Fw": "FZ", "u0": "FX", "v0": "FY", "w0": "FZ"}
+
+
+def extent_shape(kind: str, nx: int, ny: int, nz: int) -> tuple:
+    """Fortran shape of the reference arrays (0_modules.f90:92-133)."""
+    return {
+        "U": (nx + 3, ny + 2, nz + 2), "V": (nx + 2, ny + 3, nz + 2), "W": (nx + 2, ny + 2, nz + 3),
+        "P": (nx + 2, ny + 2, nz + 2), "C": (nx, ny, nz), "FX": (nx + 1, ny, nz),
+        "FY": (nx, ny + 1, nz), "FZ": (nx, ny, nz + 1),
+    }[kind]
+
+
+class SuliError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("ccx0", C.c_int32), ("ccxf", C.c_int32), ("ccy0", C.c_int32), ("ccyf", C.c_int32),
+        ("ccz0", C.c_int32), ("cczf", C.c_int32),
+        ("m_turb", C.c_int32),
+        ("rho_f1", C.c_double), ("rho_f2", C.c_double), ("mi_f1", C.c_double), ("mi_f2", C.c_double),
+        ("sigma", C.c_double), ("alpha1", C.c_double),
+        ("csmag", C.c_double), ("gx", C.c_double), ("gz", C.c_double), ("chezy", C.c_double),
+        ("precision", C.c_int32), ("poisson", C.c_int32), ("precond", C.c_int32), ("max_it", C.c_int32),
+        ("tol_abs", C.c_double), ("tol_rel", C.c_double), ("device", C.c_int32),
+    ]
+
+
+class PoissonStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("outer", C.c_int32), ("fallbacks", C.c_int32),
+                ("converged", C.c_int32), ("res0", C.c_double), ("res", C.c_double), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LsStats(C.Structure):
+    _fields_ = [("vol_ini", C.c_double), ("vol_ins", C.c_double), ("reinit_iters", C.c_int32),
+                ("mass_corr_iters", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_LIB = None
+
+
+def lib():
+    """Load libsuli.so (built in-tree); raises if it is absent."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise SuliError(f"{LIB_PATH} is not built; run `make -C {HERE}` or __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        i32, f64, p = C.c_int, C.c_double, C.POINTER
+        sig = {
+            "suli_init": ([p(Params)], i32), "suli_finalize": ([], i32),
+            "suli_last_error": ([C.c_char_p, i32], i32), "suli_version": ([], C.c_char_p),
+            "suli_upload": ([i32, p(f64), C.c_int64], i32), "suli_download": ([i32, p(f64), C.c_int64], i32),
+            "suli_field_len": ([i32], C.c_int64),
+            "suli_level_set_ini": ([p(f64)], i32), "suli_init_dambreak": ([f64, f64, p(f64)], i32),
+            "suli_level_set": ([f64, p(LsStats)], i32), "suli_contorno": ([i32], i32),
+            "suli_visco": ([], i32), "suli_convdiff": ([], i32), "suli_tempo": ([i32, f64], i32),
+            "suli_graddin": ([f64, p(PoissonStats)], i32), "suli_posdin": ([f64], i32),
+            "suli_est_div": ([p(f64)], i32), "suli_convdiff_forces": ([], i32),
+            "suli_stage": ([i32, f64, p(PoissonStats)], i32),
+            "suli_step": ([f64, p(LsStats), p(PoissonStats)], i32), "suli_sync": ([], i32),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        buf = C.create_string_buffer(1024)
+        lib().suli_last_error(buf, 1024)
+        raise SuliError(f"{what} failed ({rc}): {buf.value.decode(errors='replace')}")
+
+
+class Solver:
+    """One SuLi-IC state resident on the GPU."""
+
+    def __init__(self, nx, ny, nz, dx, dy=None, dz=None, *, ccx0=2, ccxf=2, ccy0=2, ccyf=2, ccz0=2, cczf=2,
+                 m_turb=1, rho_f1=1.2, rho_f2=1000.0, mi_f1=1.8e-5, mi_f2=1.0e-3, sigma=0.0728, alpha1=1.5,
+                 csmag=0.1, gx=0.0, gz=9.81, chezy=44.5, precision=PREC_FP64, poisson=POISSON_REF,
+                 precond=PRECOND_JACOBI, max_it=0, tol_abs=0.0, tol_rel=0.0, device=0):
+        dy = dx if dy is None else dy
+        dz = dx if dz is None else dz
+        self.p = Params(nx, ny, nz, dx, dy, dz, ccx0, ccxf, ccy0, ccyf, ccz0, cczf, m_turb, rho_f1, rho_f2,
+                        mi_f1, mi_f2, sigma, alpha1, csmag, gx, gz, chezy, precision, poisson, precond,
+                        max_it, tol_abs, tol_rel, device)
+        _check(lib().suli_init(C.byref(self.p)), "suli_init")
+        self.nx, self.ny, self.nz = nx, ny, nz
+
+    @classmethod
+    def from_oracle_params(cls, op, **kw):
+        f = {k: getattr(op, k) for k in ["nx", "ny", "nz", "dx", "dy", "dz", "ccx0", "ccxf", "ccy0", "ccyf",
+                                         "ccz0", "cczf", "m_turb", "rho_f1", "rho_f2", "mi_f1", "mi_f2",
+                                         "sigma", "alpha1", "csmag", "gx", "gz", "chezy"]}
+        f.update(kw)
+        nx, ny, nz, dx = f.pop("nx"), f.pop("ny"), f.pop("nz"), f.pop("dx")
+        return cls(nx, ny, nz, dx, **f)
+
+    def close(self):
+        lib().suli_finalize()
+
+    def shape(self, name):
+        return extent_shape(KIND.get(name, "C"), self.nx, self.ny, self.nz)
+
+    def upload(self, name, arr):
+        a = np.asfortranarray(np.asarray(arr, dtype=np.float64))
+        assert a.shape == self.shape(name), (name, a.shape, self.shape(name))
+        flat = a.ravel(order="F")
+        _check(lib().suli_upload(FIELD_ID[name], flat.ctypes.data_as(C.POINTER(C.c_double)), flat.size),
+               f"suli_upload({name})")
+
+    def download(self, name):
+        shp = self.shape(name)
+        flat = np.zeros(int(np.prod(shp)), dtype=np.float64)
+        _check(lib().suli_download(FIELD_ID[name], flat.ctypes.data_as(C.POINTER(C.c_double)), flat.size),
+               f"suli_download({name})")
+        return flat.reshape(shp, order="F")
+
+    # -- reference subroutines ------------------------------------------------
+    def level_set_ini(self):
+        v = C.c_double()
+        _check(lib().suli_level_set_ini(C.byref(v)), "suli_level_set_ini")
+        return v.value
+
+    def init_dambreak(self, width, height):
+        v = C.c_double()
+        _check(lib().suli_init_dambreak(width, height, C.byref(v)), "suli_init_dambreak")
+        return v.value
+
+    def level_set(self, dt):
+        st = LsStats()
+        _check(lib().suli_level_set(dt, C.byref(st)), "suli_level_set")
+        return st.as_dict()
+
+    def contorno(self, nlock):
+        _check(lib().suli_contorno(nlock), "suli_contorno")
+
+    def visco(self):
+        _check(lib().suli_visco(), "suli_visco")
+
+    def convdiff(self):
+        _check(lib().suli_convdiff(), "suli_convdiff")
+
+    def convdiff_forces(self):
+        _check(lib().suli_convdiff_forces(), "suli_convdiff_forces")
+
+    def tempo(self, tt, dt):
+        _check(lib().suli_tempo(tt, dt), "suli_tempo")
+
+    def graddin(self, dt):
+        st = PoissonStats()
+        _check(lib().suli_graddin(dt, C.byref(st)), "suli_graddin")
+        return st.as_dict()
+
+    def posdin(self, dt):
+        _check(lib().suli_posdin(dt), "suli_posdin")
+
+    def est_div(self):
+        v = C.c_double()
+        _check(lib().suli_est_div(C.byref(v)), "suli_est_div")
+        return v.value
+
+    def stage(self, tt, dt_base):
+        st = PoissonStats()
+        _check(lib().suli_stage(tt, dt_base, C.byref(st)), "suli_stage")
+        return st.as_dict()
+
+    def step(self, dt):
+        ls = LsStats()
+        ps = (PoissonStats * 2)()
+        _check(lib().suli_step(dt, C.byref(ls), ps), "suli_step")
+        return ls.as_dict(), [ps[0].as_dict(), ps[1].as_dict()]
+
+    def sync(self):
+        _check(lib().suli_sync(), "suli_sync")
+
+
+def load_state(solver: Solver, state, names=None):
+    """Upload an oracle State (reference layout) into the GPU solver."""
+    names = names or ["u", "v", "w", "prd1", "rho", "ls_nu", "ls", "mod_ls", "kurv", "hs", "hsx", "hsy",
+                      "hsz", "nut", "u0", "v0", "w0"]
+    for n in names:
+        solver.upload(n, state.arr(n))
