@@ -244,10 +244,12 @@ class State:
             L.or_graddin(C.byref(st))
         elif cmd == "posdin":
             L.or_posdin(C.byref(st))
-        elif cmd == "stage":
+        elif cmd == "stage":  # like the harness, leave dt at its base value afterwards
             L.or_stage(C.byref(st), arg, self.scal["dt"])
+            st.dt = self.scal["dt"]
         elif cmd == "step":
-            for _ in range(max(arg, 1)):
+            for it in range(1, max(arg, 1) + 1):
+                st.t = it * st.dt  # 1_principal.f90:65
                 L.or_step(C.byref(st), st.dt)
         else:
             raise ValueError(cmd)

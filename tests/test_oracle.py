@@ -1,0 +1,239 @@
+"""The CPU oracle, pinned three ways:
+
+1. golden vectors written by the flang-built reference harness
+   (tests/golden/*.npz, generated by tests/golden/gen_golden.py) must be
+   reproduced bit for bit, routine by routine;
+2. where the reference build is available here (oracle/_ref), a fresh
+   seeded state is run through both and must agree bit for bit;
+3. closed forms that a plausible slip (wrong ghost formula, swapped
+   one-sided derivative, dropped term) would break.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+def _params_from(d):
+    nx, ny, nz, ccx0, ccxf, ccy0, ccyf, ccz0, cczf = [int(x) for x in d["__params"]]
+    dx, dy, dz = [float(x) for x in d["__h"]]
+    return O.Params(nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, ccx0=ccx0, ccxf=ccxf, ccy0=ccy0, ccyf=ccyf,
+                    ccz0=ccz0, cczf=cczf)
+
+
+@pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p)[:-4] for p in GOLD])
+def test_golden_bitexact(path):
+    d = np.load(path)
+    p = _params_from(d)
+    s = O.State(p)
+    for name, _ in O.ARRAYS:
+        s.arr(name)[...] = d["in_" + name]
+    s.scal = dict(zip(O.SCALARS, [float(x) for x in d["__scal_in"]]))
+    tag = os.path.basename(path)[:-4].split("_", 1)[1]
+    cmd, arg = tag, 0
+    for base in ("tempo", "contorno", "stage", "step"):
+        if tag.startswith(base) and tag[len(base):].isdigit():
+            cmd, arg = base, int(tag[len(base):])
+    s.run(cmd, arg)
+    n_changed = 0
+    for name, _ in O.ARRAYS:
+        want = d["out_" + name] if ("out_" + name) in d else d["in_" + name]
+        n_changed += ("out_" + name) in d
+        got = s.arr(name)
+        assert np.array_equal(got, want), f"{name}: max diff {np.max(np.abs(got - want)):.3e}"
+    assert n_changed > 0
+    for q, k in enumerate(O.SCALARS):
+        assert s.scal[k] == float(d["__scal_out"][q]), k
+
+
+@pytest.mark.skipif(not O.ref_available(O.Params()), reason="flang reference harness not built here")
+@pytest.mark.parametrize("cmd,arg", [("level_set", None), ("stage", 1), ("step", 2)])
+def test_live_reference(cmd, arg):
+    p = O.Params()
+    s = O.random_state(p, seed=77, perturb=0.3)
+    s.run("visco")
+    s.run("contorno", 1)
+    a, b = s.copy(), s.copy()
+    O.ref_run(a, cmd, arg)
+    b.run(cmd, arg or 0)
+    assert np.array_equal(a.buf, b.buf)
+
+
+# ---- closed forms -------------------------------------------------------------
+@pytest.mark.parametrize("ihs", [1, 2])
+def test_weno_cubic_exact(ihs):
+    """Every WENO5 candidate differentiates a cubic exactly and the weights
+    sum to one, so both one-sided derivatives equal the exact derivative
+    wherever the stencil stays inside the pencil (14_level_set.f90:572-608)."""
+    n, h = 24, 0.1
+    x = (np.arange(1, n + 1) - 0.5) * h
+    phi = 0.3 * x**3 - x**2 + 2 * x - 1
+    dphi = 0.9 * x**2 - 2 * x + 2
+    dp, dn = O.weno1(phi, h, ihs)
+    inner = slice(3, n - 3)
+    assert np.allclose(dp[inner], dphi[inner], rtol=0, atol=1e-11)
+    assert np.allclose(dn[inner], dphi[inner], rtol=0, atol=1e-11)
+
+
+def test_weno_fifth_order():
+    errs = []
+    for n in (40, 80, 160):
+        h = 1.0 / (n - 1)
+        x = np.arange(n) * h
+        dp, dn = O.weno1(np.sin(2 * np.pi * x), h, 0)
+        exact = 2 * np.pi * np.cos(2 * np.pi * x)
+        errs.append(np.max(np.abs(dp - exact)))
+    order = np.log2(errs[0] / errs[1]), np.log2(errs[1] / errs[2])
+    assert min(order) > 4.5, order
+
+
+def test_weno_ghost_formulas():
+    """Ghost extrapolation (14_level_set.f90:541-569): for ihs=1 the low side
+    uses (12,-9,2)/5, exact for linear data, and the high side (18,-9,2)/11,
+    which is not; ihs=2 uses the 1/11 formula on both sides, mirror-symmetric.
+    A swapped or symmetric-by-mistake formula breaks one of these."""
+    phi = np.arange(1, 11, dtype=float)
+    dp, dn = O.weno1(phi, 1.0, 1)
+    assert np.allclose(dn, 1.0, rtol=0, atol=1e-14)  # low side exact for linear data
+    assert np.allclose(dp[:-1], 1.0, rtol=0, atol=1e-14) and dp[-1] < 0.9  # high side is not
+    dp2, dn2 = O.weno1(phi, 1.0, 2)
+    assert dn2[0] == pytest.approx(dp2[-1], abs=1e-14) and dp2[-1] == pytest.approx(dp[-1], abs=1e-14)
+    assert np.allclose(dn2[1:], 1.0, rtol=0, atol=1e-14) and np.allclose(dp2[:-1], 1.0, rtol=0, atol=1e-14)
+    dp0, dn0 = O.weno1(np.sin(2 * np.pi * np.arange(10) / 9.0), 1.0, 0)
+    # periodic with period n-1: points 1 and n coincide
+    assert dp0[0] == pytest.approx(dp0[-1], abs=1e-12) and dn0[0] == pytest.approx(dn0[-1], abs=1e-12)
+
+
+def test_heaviside_planar():
+    """For a planar level set phi = z0 - z, H, grad H and rho follow the
+    smoothed Heaviside in closed form (14_level_set.f90:659-698)."""
+    p = O.Params(nx=8, ny=8, nz=24, dx=1 / 24, dy=1 / 24, dz=1 / 24)
+    s = O.State(p)
+    z = (np.arange(1, p.nz + 1) - 0.5) * p.dz
+    s.ls[...] = (0.5 - z)[None, None, :]
+    s.run("heaviside")
+    eps = p.alpha1 * p.dx1
+    ph = 0.5 - z
+    H = np.where(ph < -eps, 0.0, np.where(ph > eps, 1.0, 0.5 * (1 + ph / eps + np.sin(np.pi * ph / eps) / np.pi)))
+    Hz = np.where(np.abs(ph) <= eps, 0.5 * (-1.0) * (1 + np.cos(np.pi * ph / eps)) / eps, 0.0)
+    assert np.allclose(s.hs[3, 3, :], H, atol=1e-14)
+    assert np.allclose(s.hsz[3, 3, 3:-3], Hz[3:-3], atol=1e-9)
+    assert np.allclose(s.rho, p.rho_f1 * (1 - s.hs) + p.rho_f2 * s.hs, rtol=1e-15)
+
+
+def _assemble(s, dt):
+    """Dense scaled operator and right-hand side of graddin (9_grad.f90:46-151)."""
+    p = s.p
+    nx, ny, nz = p.nx, p.ny, p.nz
+    rho = s.rho
+    def rf(axis, f, j, k, i):  # face density with end faces copied
+        a = np.moveaxis(rho, axis, 0)
+        n = a.shape[0]
+        fc = min(max(f, 2), n)
+        return 0.5 * (a[fc - 1] + a[fc - 2])
+    N = nx * ny * nz
+    idx = lambda i, j, k: (i - 1) + nx * ((j - 1) + ny * (k - 1))
+    c = {}
+    for k in range(1, nz + 1):
+        for j in range(1, ny + 1):
+            for i in range(1, nx + 2):
+                c[("x", i, j, k)] = dt / p.dx**2 / (0.5 * (rho[min(max(i, 2), nx) - 1, j - 1, k - 1] + rho[min(max(i, 2), nx) - 2, j - 1, k - 1]))
+    for k in range(1, nz + 1):
+        for j in range(1, ny + 2):
+            for i in range(1, nx + 1):
+                c[("y", i, j, k)] = dt / p.dy**2 / (0.5 * (rho[i - 1, min(max(j, 2), ny) - 1, k - 1] + rho[i - 1, min(max(j, 2), ny) - 2, k - 1]))
+    for k in range(1, nz + 2):
+        for j in range(1, ny + 1):
+            for i in range(1, nx + 1):
+                c[("z", i, j, k)] = dt / p.dz**2 / (0.5 * (rho[i - 1, j - 1, min(max(k, 2), nz) - 1] + rho[i - 1, j - 1, min(max(k, 2), nz) - 2]))
+    A = np.zeros((N, N))
+    D = np.zeros(N)
+    q = np.zeros(N)
+    u, v, w = s.u, s.v, s.w
+    for k in range(1, nz + 1):
+        for j in range(1, ny + 1):
+            for i in range(1, nx + 1):
+                r = idx(i, j, k)
+                D[r] = (c[("x", i + 1, j, k)] + c[("x", i, j, k)] + c[("y", i, j + 1, k)] + c[("y", i, j, k)]
+                        + c[("z", i, j, k + 1)] + c[("z", i, j, k)])
+                q[r] = (u[i, j, k] - u[i + 1, j, k]) / p.dx + (v[i, j, k] - v[i, j + 1, k]) / p.dy + (w[i, j, k] - w[i, j, k + 1]) / p.dz
+    for k in range(1, nz + 1):
+        for j in range(1, ny + 1):
+            for i in range(1, nx + 1):
+                r = idx(i, j, k)
+                A[r, r] += D[r]
+                for (di, dj, dk, key) in [(1, 0, 0, ("x", i + 1, j, k)), (-1, 0, 0, ("x", i, j, k)),
+                                          (0, 1, 0, ("y", i, j + 1, k)), (0, -1, 0, ("y", i, j, k)),
+                                          (0, 0, 1, ("z", i, j, k + 1)), (0, 0, -1, ("z", i, j, k))]:
+                    ii, jj, kk = i + di, j + dj, k + dk
+                    ii, jj, kk = min(max(ii, 1), nx), min(max(jj, 1), ny), min(max(kk, 1), nz)
+                    A[r, idx(ii, jj, kk)] -= c[key]
+    return A, D, q
+
+
+def test_graddin_dense_solve():
+    """The projection solves sum_f c_f (p_P - p_nb) = -div u* with Neumann
+    walls; at 6x5x4 the dense least-squares solution (the operator is
+    singular: constants) must match the CG result up to a constant."""
+    p = O.Params(nx=6, ny=5, nz=4, dx=0.2, dy=0.2, dz=0.2)
+    s = O.random_state(p, seed=5, width=0.5, height=0.4)
+    s.run("contorno", 2)
+    dt = 1e-3
+    s.scal["dt"] = dt
+    A, D, q = _assemble(s, dt)
+    assert np.allclose(A, A.T, rtol=1e-13, atol=0)          # symmetric
+    assert np.allclose(A @ np.ones(len(D)), 0, atol=1e-9 * np.max(D))  # constant null space
+    s.prd1[...] = 0.0
+    s.run("graddin")
+    pcg = s.prd1[1:-1, 1:-1, 1:-1].ravel(order="F")
+    pls = np.linalg.lstsq(A, q, rcond=None)[0]
+    dcg = pcg - pcg.mean()
+    dls = pls - pls.mean()
+    assert np.max(np.abs(dcg - dls)) <= 1e-3 * np.max(np.abs(dls)), np.max(np.abs(dcg - dls))
+
+
+def test_graddin_uniform_density_dct():
+    """Uniform density: the operator is the Neumann Laplacian times dt/rho,
+    diagonalised by the DCT-II."""
+    from scipy.fft import dctn, idctn
+    p = O.Params(nx=8, ny=6, nz=6, dx=0.1, dy=0.1, dz=0.1)
+    s = O.random_state(p, seed=9, width=2.0, height=2.0)  # all water
+    assert np.all(s.rho == p.rho_f2)
+    s.run("contorno", 2)
+    dt = 1e-3
+    s.scal["dt"] = dt
+    s.prd1[...] = 0.0
+    s.run("graddin")
+    q = ((s.u[1:-2, 1:-1, 1:-1] - s.u[2:-1, 1:-1, 1:-1]) / p.dx + (s.v[1:-1, 1:-2, 1:-1] - s.v[1:-1, 2:-1, 1:-1]) / p.dy
+         + (s.w[1:-1, 1:-1, 1:-2] - s.w[1:-1, 1:-1, 2:-1]) / p.dz)
+    c = dt / p.rho_f2 / p.dx**2
+    Q = dctn(q, type=2, norm="ortho")
+    lam = [2 - 2 * np.cos(np.pi * np.arange(n) / n) for n in (p.nx, p.ny, p.nz)]
+    L = c * (lam[0][:, None, None] + lam[1][None, :, None] + lam[2][None, None, :])
+    L[0, 0, 0] = 1.0
+    P = Q / L
+    P[0, 0, 0] = 0.0
+    pex = idctn(P, type=2, norm="ortho")
+    pcg = s.prd1[1:-1, 1:-1, 1:-1]
+    err = np.max(np.abs((pcg - pcg.mean()) - pex)) / np.max(np.abs(pex))
+    assert err < 1e-3, err
+
+
+def test_contorno_noslip_mirror():
+    """No-slip walls: tangential ghosts are the negated first interior value,
+    normal ghosts mirror across the wall face (3_contorno.f90:74-78,132-136)."""
+    p = O.Params()
+    s = O.random_state(p, seed=3)
+    s.run("contorno", 1)
+    assert np.array_equal(s.u[2:-2, 0, 1:-1], -s.u[2:-2, 1, 1:-1])
+    assert np.array_equal(s.v[1:-1, 0, 1:-1], s.v[1:-1, 2, 1:-1])
+    assert np.array_equal(s.u[0, 1:-1, 1:-1], s.u[2, 1:-1, 1:-1])
+    assert np.array_equal(s.w[1:-1, 1:-1, 0], s.w[1:-1, 1:-1, 2])
+    s.run("contorno", 2)
+    assert np.all(s.u[1, :, :] == 0) and np.all(s.u[p.nx + 1, :, :] == 0)
+    assert np.all(s.w[:, :, 1] == 0) and np.all(s.w[:, :, p.nz + 1] == 0)
