@@ -125,6 +125,10 @@ int64_t suli_field_len(int field);
 /* Tail of level_set_ini (14_level_set.f90:205-217): heaviside, initial
  * volume, mod_ls1, for a level set uploaded with suli_upload(SULI_LS). */
 int suli_level_set_ini(double *vol_ini);
+/* Reference volume of the level set (ls_param vol_ini, 0_modules.f90:194),
+ * for a state restored from the host instead of built by suli_level_set_ini. */
+int suli_set_vol_ini(double vol_ini);
+int suli_get_volumes(double *vol_ini, double *vol_ins);
 /* Dam-break initial level set: signed distance (positive in water) to a
  * column x <= width, z <= height spanning y; then suli_level_set_ini. */
 int suli_init_dambreak(double width, double height, double *vol_ini);

@@ -346,6 +346,23 @@ extern "C" int suli_level_set_ini(double *vol_ini) {
   return 0;
 }
 
+extern "C" int suli_set_vol_ini(double vol_ini) {
+  NEEDCTX();
+  if (!(vol_ini > 0)) {
+    suli_set_error("suli_set_vol_ini: the reference volume must be positive");
+    return 1;
+  }
+  g_ctx->vol_ini = vol_ini;
+  return 0;
+}
+
+extern "C" int suli_get_volumes(double *vol_ini, double *vol_ins) {
+  NEEDCTX();
+  if (vol_ini) *vol_ini = g_ctx->vol_ini;
+  if (vol_ins) *vol_ins = g_ctx->vol_ins;
+  return 0;
+}
+
 extern "C" int suli_init_dambreak(double width, double height, double *vol_ini) {
   NEEDCTX();
   Ctx &c = *g_ctx;
@@ -357,6 +374,11 @@ extern "C" int suli_init_dambreak(double width, double height, double *vol_ini) 
 extern "C" int suli_level_set(double dt, suli_ls_stats *st) {
   NEEDCTX();
   Ctx &c = *g_ctx;
+  if (!(c.vol_ini > 0)) {
+    suli_set_error("suli_level_set: no reference volume; call suli_level_set_ini, suli_init_dambreak "
+                   "or suli_set_vol_ini first");
+    return 1;
+  }
   const Grid &g = c.g;
   // SSP-RK3 advection with the full dt (14_level_set.f90:231-239)
   DISPATCH(c, Launch<TT>::ls_adv(g, c.st, c.ls, c.ls, c.u, c.v, c.w, c.lsA, 1., 0., 1., dt));

@@ -95,6 +95,7 @@ def lib():
             "suli_est_div": ([p(f64)], i32), "suli_convdiff_forces": ([], i32),
             "suli_stage": ([i32, f64, p(PoissonStats)], i32),
             "suli_step": ([f64, p(LsStats), p(PoissonStats)], i32), "suli_sync": ([], i32),
+            "suli_set_vol_ini": ([f64], i32), "suli_get_volumes": ([p(f64), p(f64)], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -161,6 +162,14 @@ class Solver:
         _check(lib().suli_level_set_ini(C.byref(v)), "suli_level_set_ini")
         return v.value
 
+    def set_vol_ini(self, v):
+        _check(lib().suli_set_vol_ini(v), "suli_set_vol_ini")
+
+    def volumes(self):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().suli_get_volumes(C.byref(a), C.byref(b)), "suli_get_volumes")
+        return a.value, b.value
+
     def init_dambreak(self, width, height):
         v = C.c_double()
         _check(lib().suli_init_dambreak(width, height, C.byref(v)), "suli_init_dambreak")
@@ -220,3 +229,5 @@ def load_state(solver: Solver, state, names=None):
                       "hsz", "nut", "u0", "v0", "w0"]
     for n in names:
         solver.upload(n, state.arr(n))
+    if state.scal.get("vol_ini", 0) > 0:
+        solver.set_vol_ini(state.scal["vol_ini"])
