@@ -152,6 +152,13 @@ int suli_convdiff_forces(void);
 int suli_stage(int tt, double dt_base, suli_poisson_stats *st);
 int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps);
 
+/* Per-kernel GPU timing: HIP events recorded on the library's stream around
+ * every kernel launch while enabled.  suli_prof_enable(1) clears the totals;
+ * suli_prof_get returns kernel idx's name, launch count and summed time. */
+int suli_prof_enable(int on);
+int suli_prof_count(void);
+int suli_prof_get(int idx, char *name, int len, int64_t *launches, double *total_ms);
+
 /* wait for all queued GPU work */
 int suli_sync(void);
 

@@ -71,12 +71,12 @@ if grep -n -i "^[[:space:]]*use tempo[[:space:]]*$" "$OUT"/src/*.f90; then
   echo "unrenamed 'use tempo' left" >&2; exit 4
 fi
 
-FFLAGS="-O2 -fdefault-real-8 -fdefault-double-8"
+FFLAGS="-O2 -fdefault-real-8 -fdefault-double-8 -mcmodel=medium"
 cd "$OUT/obj"
 for f in 0_modules 2_iniciais 3_contorno 4_plot 5_viscosidade 6_convdiff 7_advectivo \
          8_derivadas 9_grad 10_pos 11_diversos 12_paraview_3D 13_mms 14_level_set; do
   "$FC" $FFLAGS -c "$OUT/src/$f.f90" -o "$f.o"
 done
 "$FC" $FFLAGS -c "$HERE/harness.f90" -o harness.o
-"$FC" -O2 -o "$OUT/harness" ./*.o
+"$FC" -O2 -mcmodel=medium -o "$OUT/harness" ./*.o
 echo "built $OUT/harness (nx=$NX ny=$NY nz=$NZ)"

@@ -45,6 +45,9 @@ static int dalloc(Ctx &c, void **p, size_t bytes) {
 }
 
 static void free_all(Ctx &c) {
+  prof_flush(c);
+  for (hipEvent_t e : c.prof.pool) (void)hipEventDestroy(e);
+  c.prof.pool.clear();
   for (void *p : c.allocs) (void)hipFree(p);
   c.allocs.clear();
   if (c.ctl_h) (void)hipHostFree(c.ctl_h);
@@ -139,6 +142,74 @@ extern "C" int suli_init(const suli_params *p) {
   // density and viscosity start as the light fluid until a level set is set
   if (fcg_create(*c)) return 1;
   HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// kernel timing
+
+int prof_id(Ctx &c, const char *name) {
+  for (size_t q = 0; q < c.prof.names.size(); q++)
+    if (c.prof.names[q] == name) return (int)q;
+  c.prof.names.push_back(name);
+  c.prof.total_ms.push_back(0.0);
+  c.prof.launches.push_back(0);
+  return (int)c.prof.names.size() - 1;
+}
+
+hipEvent_t prof_event(Ctx &c) {
+  if (c.prof.pool.empty()) {
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  hipEvent_t e = c.prof.pool.back();
+  c.prof.pool.pop_back();
+  return e;
+}
+
+void prof_flush(Ctx &c) {
+  (void)hipStreamSynchronize(c.st);
+  for (auto &p : c.prof.pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      c.prof.total_ms[p.id] += ms;
+      c.prof.launches[p.id] += 1;
+    }
+    c.prof.pool.push_back(p.a);
+    c.prof.pool.push_back(p.b);
+  }
+  c.prof.pending.clear();
+}
+
+extern "C" int suli_prof_enable(int on) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  prof_flush(c);
+  c.prof.on = on != 0;
+  c.prof.names.clear();
+  c.prof.total_ms.clear();
+  c.prof.launches.clear();
+  return 0;
+}
+
+extern "C" int suli_prof_count(void) {
+  if (!g_ctx) return -1;
+  prof_flush(*g_ctx);
+  return (int)g_ctx->prof.names.size();
+}
+
+extern "C" int suli_prof_get(int idx, char *name, int len, int64_t *launches, double *total_ms) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  prof_flush(c);
+  if (idx < 0 || idx >= (int)c.prof.names.size()) {
+    suli_set_error("suli_prof_get: index out of range");
+    return 1;
+  }
+  if (name && len > 0) std::snprintf(name, (size_t)len, "%s", c.prof.names[idx].c_str());
+  if (launches) *launches = c.prof.launches[idx];
+  if (total_ms) *total_ms = c.prof.total_ms[idx];
   return 0;
 }
 
@@ -325,9 +396,11 @@ static int read_res(Ctx &c, int off, int n) {
 }
 
 static int heavi_curv(Ctx &c, bool curv, double *vol) {
+  { ProfScope ps_(c, "heaviside_normals");
   DISPATCH(c, Launch<TT>::heavi(c.g, c.ph, c.st, c.ls, c.hs, c.hsx, c.hsy, c.hsz, c.rho, c.lsnu, c.modls,
-                                c.nrx, c.nry, c.nrz, c.slot(RS_VOL, 0)));
-  if (curv) DISPATCH(c, Launch<TT>::curv(c.g, c.st, c.nrx, c.nry, c.nrz, c.ddx, c.ddy, c.ddz, c.kurv));
+                                c.nrx, c.nry, c.nrz, c.slot(RS_VOL, 0))); }
+  if (curv) { ProfScope ps_(c, "curvature");
+    DISPATCH(c, Launch<TT>::curv(c.g, c.st, c.nrx, c.nry, c.nrz, c.ddx, c.ddy, c.ddz, c.kurv)); }
   HIPCHK(hipGetLastError());
   if (vol) {
     if (read_res(c, RS_VOL, 1)) return 1;
@@ -366,7 +439,7 @@ extern "C" int suli_get_volumes(double *vol_ini, double *vol_ins) {
 extern "C" int suli_init_dambreak(double width, double height, double *vol_ini) {
   NEEDCTX();
   Ctx &c = *g_ctx;
-  DISPATCH(c, Launch<TT>::dambreak(c.g, c.st, width, height, c.ls));
+  { ProfScope ps_(c, "init_dambreak"); DISPATCH(c, Launch<TT>::dambreak(c.g, c.st, width, height, c.ls)); }
   HIPCHK(hipGetLastError());
   return suli_level_set_ini(vol_ini);
 }
@@ -381,23 +454,23 @@ extern "C" int suli_level_set(double dt, suli_ls_stats *st) {
   }
   const Grid &g = c.g;
   // SSP-RK3 advection with the full dt (14_level_set.f90:231-239)
-  DISPATCH(c, Launch<TT>::ls_adv(g, c.st, c.ls, c.ls, c.u, c.v, c.w, c.lsA, 1., 0., 1., dt));
-  DISPATCH(c, Launch<TT>::ls_adv(g, c.st, c.lsA, c.ls, c.u, c.v, c.w, c.lsB, 3. / 4., 1. / 4., 1. / 4., dt));
-  DISPATCH(c, Launch<TT>::ls_adv(g, c.st, c.lsB, c.ls, c.u, c.v, c.w, c.lsA, 1. / 3., 2. / 3., 2. / 3., dt));
+  { ProfScope ps_(c, "ls_advect_rk3"); DISPATCH(c, Launch<TT>::ls_adv(g, c.st, c.ls, c.ls, c.u, c.v, c.w, c.lsA, 1., 0., 1., dt)); }
+  { ProfScope ps_(c, "ls_advect_rk3"); DISPATCH(c, Launch<TT>::ls_adv(g, c.st, c.lsA, c.ls, c.u, c.v, c.w, c.lsB, 3. / 4., 1. / 4., 1. / 4., dt)); }
+  { ProfScope ps_(c, "ls_advect_rk3"); DISPATCH(c, Launch<TT>::ls_adv(g, c.st, c.lsB, c.ls, c.u, c.v, c.w, c.lsA, 1. / 3., 2. / 3., 2. / 3., dt)); }
   std::swap(c.ls, c.lsA);
   // reinitialisation (:386-509)
   const double dt1 = c.ph.dt1, dx1 = c.ph.dx1;
-  DISPATCH(c, Launch<TT>::reinit_sign(g, c.ph, c.st, c.ls, c.S));
+  { ProfScope ps_(c, "reinit_sign"); DISPATCH(c, Launch<TT>::reinit_sign(g, c.ph, c.st, c.ls, c.S)); }
   int il = 0;
   double error = 999.;
   while (error > dt1 * dx1 * dx1) {
     il = il + 1;
-    DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.ls, c.ls, c.S, c.lsA, 1., 0., 1. * dt1, 0,
-                                         c.slot(RS_BAND, 1)));
-    DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.lsA, c.ls, c.S, c.lsB, 3. / 4., 1. / 4.,
-                                         1. / 4. * dt1, 0, c.slot(RS_BAND, 1)));
-    DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.lsB, c.ls, c.S, c.lsA, 1. / 3., 2. / 3.,
-                                         2. / 3. * dt1, il < 3 ? 1 : 0, c.slot(RS_BAND, 1)));
+    { ProfScope ps_(c, "reinit_rk3"); DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.ls, c.ls, c.S, c.lsA, 1., 0., 1. * dt1, 0,
+                                         c.slot(RS_BAND, 1))); }
+    { ProfScope ps_(c, "reinit_rk3"); DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.lsA, c.ls, c.S, c.lsB, 3. / 4., 1. / 4.,
+                                         1. / 4. * dt1, 0, c.slot(RS_BAND, 1))); }
+    { ProfScope ps_(c, "reinit_rk3"); DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.lsB, c.ls, c.S, c.lsA, 1. / 3., 2. / 3.,
+                                         2. / 3. * dt1, il < 3 ? 1 : 0, c.slot(RS_BAND, 1))); }
     std::swap(c.ls, c.lsA);
     HIPCHK(hipGetLastError());
     if (il == 3) {
@@ -413,7 +486,7 @@ extern "C" int suli_level_set(double dt, suli_ls_stats *st) {
   if (heavi_curv(c, false, &vol)) return 1;
   int nc = 0;
   while (std::fabs(vol - c.vol_ini) / vol > 0.1) {
-    DISPATCH(c, Launch<TT>::masscorr(g, c.st, c.ls, c.modls, dt1 * (vol - c.vol_ini), c.vol_ini));
+    { ProfScope ps_(c, "volume_correction"); DISPATCH(c, Launch<TT>::masscorr(g, c.st, c.ls, c.modls, dt1 * (vol - c.vol_ini), c.vol_ini)); }
     if (heavi_curv(c, false, &vol)) return 1;
     nc++;
     if (nc > 1000) {
@@ -421,7 +494,7 @@ extern "C" int suli_level_set(double dt, suli_ls_stats *st) {
       return 4;
     }
   }
-  DISPATCH(c, Launch<TT>::curv(g, c.st, c.nrx, c.nry, c.nrz, c.ddx, c.ddy, c.ddz, c.kurv));
+  { ProfScope ps_(c, "curvature"); DISPATCH(c, Launch<TT>::curv(g, c.st, c.nrx, c.nry, c.nrz, c.ddx, c.ddy, c.ddz, c.kurv)); }
   HIPCHK(hipGetLastError());
   c.vol_ins = vol;
   if (st) {
@@ -440,7 +513,7 @@ extern "C" int suli_visco(void) {
   NEEDCTX();
   Ctx &c = *g_ctx;
   if (c.ph.m_turb == 1) {
-    DISPATCH(c, Launch<TT>::visco(c.g, c.ph, c.st, c.u, c.v, c.w, c.rho, c.nut));
+    { ProfScope ps_(c, "visco"); DISPATCH(c, Launch<TT>::visco(c.g, c.ph, c.st, c.u, c.v, c.w, c.rho, c.nut)); }
     HIPCHK(hipGetLastError());
   }
   return 0;
@@ -468,7 +541,7 @@ extern "C" int suli_convdiff_forces(void) {
   MomPtrs m = mom_ptrs(c);
   m.u0 = c.u; m.v0 = c.v; m.w0 = c.w;  // stage-2 form: writes only the scratch swap buffers
   m.Fu = c.Fu; m.Fv = c.Fv; m.Fw = c.Fw;
-  DISPATCH(c, Launch<TT>::momentum(c.g, c.ph, c.st, m, 0.0, 2));
+  { ProfScope ps_(c, "convdiff_tempo"); DISPATCH(c, Launch<TT>::momentum(c.g, c.ph, c.st, m, 0.0, 2)); }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -485,7 +558,7 @@ extern "C" int suli_tempo(int tt, double dt) {
     return 1;
   }
   MomPtrs m = mom_ptrs(c);
-  DISPATCH(c, Launch<TT>::momentum(c.g, c.ph, c.st, m, dt, tt));
+  { ProfScope ps_(c, "convdiff_tempo"); DISPATCH(c, Launch<TT>::momentum(c.g, c.ph, c.st, m, dt, tt)); }
   HIPCHK(hipGetLastError());
   std::swap(c.u, c.un);
   std::swap(c.v, c.vn);
@@ -497,7 +570,7 @@ extern "C" int suli_tempo(int tt, double dt) {
 extern "C" int suli_contorno(int nlock) {
   NEEDCTX();
   Ctx &c = *g_ctx;
-  DISPATCH(c, Launch<TT>::contorno(c.g, c.st, nlock, c.u, c.v, c.w, c.prd1));
+  { ProfScope ps_(c, "contorno"); DISPATCH(c, Launch<TT>::contorno(c.g, c.st, nlock, c.u, c.v, c.w, c.prd1)); }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -505,63 +578,17 @@ extern "C" int suli_contorno(int nlock) {
 // --------------------------------------------------------------------------
 // projection
 
-static int poisson_ref(Ctx &c, double dt, suli_poisson_stats *stt) {
-  const Grid &g = c.g;
-  auto t0 = std::chrono::steady_clock::now();
-  DISPATCH(c, Launch<TT>::pD(g, c.st, dt, c.u, c.v, c.w, c.rho, c.D, c.sD, c.q));
-  DISPATCH(c, Launch<TT>::pcoef(g, c.st, dt, c.rho, c.D, c.ax, c.ay, c.az));
-  const dim3 b(64, 4, 1);
-  const dim3 pg((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4, g.nz + 2);
-  hipLaunchKernelGGL(k_pe, pg, b, 0, c.st, g, c.prd1, c.sD, c.e);
-  std::memset(c.ctl_h, 0, sizeof(CGCtl));
-  c.ctl_h->thr = c.P.tol_abs;
-  c.ctl_h->maxit = c.P.max_it;
-  HIPCHK(hipMemcpyAsync(c.ctl, c.ctl_h, sizeof(CGCtl), hipMemcpyHostToDevice, c.st));
-  const dim3 cg = cell_grid(g);
-  RedSlot r0{c.partials, c.counters + 4, (double *)((char *)c.ctl + offsetof(CGCtl, r0))};
-  hipLaunchKernelGGL(k_pr0, cg, b, 0, c.st, g, c.e, c.ax, c.ay, c.az, c.q, c.sD, c.r, c.pp, c.ctl, r0);
-  RedSlot r1{c.partials, c.counters + 5, (double *)((char *)c.ctl + offsetof(CGCtl, alfa))};
-  RedSlot r2{c.partials, c.counters + 6, (double *)((char *)c.ctl + offsetof(CGCtl, beta))};
-  int chunk = 8;
-  for (;;) {
-    for (int n = 0; n < chunk; n++) {
-      hipLaunchKernelGGL(k_cg1, cg, b, 0, c.st, g, c.ax, c.ay, c.az, c.r, c.pp, c.mp, c.ctl, r1);
-      hipLaunchKernelGGL(k_cg2, cg, b, 0, c.st, g, c.e, c.r, c.pp, c.mp, c.ctl, r2);
-      hipLaunchKernelGGL(k_cg3, cg, b, 0, c.st, g, c.r, c.pp, c.ctl, c.counters + 7);
-    }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(c.ctl_h, c.ctl, sizeof(CGCtl), hipMemcpyDeviceToHost, c.st));
-    HIPCHK(hipStreamSynchronize(c.st));
-    if (!c.ctl_h->active) break;
-    if (chunk < 64) chunk *= 2;
-  }
-  hipLaunchKernelGGL(k_pfinal, pg, b, 0, c.st, g, c.e, c.sD, c.prd1, c.ctl);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(c.st));
-  auto t1 = std::chrono::steady_clock::now();
-  if (stt) {
-    std::memset(stt, 0, sizeof(*stt));
-    stt->iters = c.ctl_h->iters;
-    stt->outer = 0;
-    stt->converged = !(std::fabs(c.ctl_h->alfa[0]) > c.ctl_h->thr) ? 1 : 0;
-    stt->res0 = c.ctl_h->r0[0];
-    stt->res = c.ctl_h->alfa[0];
-    stt->ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  }
-  return 0;
-}
-
 extern "C" int suli_graddin(double dt, suli_poisson_stats *st) {
   NEEDCTX();
   Ctx &c = *g_ctx;
-  if (c.P.poisson == SULI_POISSON_REF) return poisson_ref(c, dt, st);
+  if (c.P.poisson == SULI_POISSON_REF) return poisson_ref_solve(c, dt, st);
   return fcg_solve(c, dt, st);
 }
 
 extern "C" int suli_posdin(double dt) {
   NEEDCTX();
   Ctx &c = *g_ctx;
-  DISPATCH(c, Launch<TT>::posdin(c.g, c.st, dt, c.prd1, c.rho, c.u, c.v, c.w));
+  { ProfScope ps_(c, "posdin"); DISPATCH(c, Launch<TT>::posdin(c.g, c.st, dt, c.prd1, c.rho, c.u, c.v, c.w)); }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -569,7 +596,7 @@ extern "C" int suli_posdin(double dt) {
 extern "C" int suli_est_div(double *maxdiv) {
   NEEDCTX();
   Ctx &c = *g_ctx;
-  DISPATCH(c, Launch<TT>::divmax(c.g, c.st, c.u, c.v, c.w, c.slot(RS_DIV, 2)));
+  { ProfScope ps_(c, "est_div"); DISPATCH(c, Launch<TT>::divmax(c.g, c.st, c.u, c.v, c.w, c.slot(RS_DIV, 2))); }
   HIPCHK(hipGetLastError());
   if (read_res(c, RS_DIV, 1)) return 1;
   if (maxdiv) *maxdiv = c.res_h[RS_DIV];

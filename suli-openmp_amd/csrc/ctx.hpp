@@ -9,6 +9,20 @@
 
 struct PoissonFCG;  // poisson_fcg.hip
 
+// Per-kernel GPU timing with HIP events recorded on the library's stream
+// around each launch (enabled by suli_prof_enable).  Event pairs are
+// recycled: when many are pending, the stream is synchronised and their
+// elapsed times are folded into per-kernel totals.
+struct Prof {
+  bool on = false;
+  std::vector<std::string> names;
+  std::vector<double> total_ms;
+  std::vector<long long> launches;
+  struct Pending { int id; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+};
+
 // reduction result slots (indices into Ctx::res)
 enum { RS_VOL = 0, RS_BAND = 2, RS_DIV = 4, RS_POIS = 8, RS_NSLOT = 64 };
 
@@ -40,6 +54,7 @@ struct Ctx {
   double vol_ini = 0, vol_ins = 0;
   PoissonFCG *fcg = nullptr;
   std::vector<void *> allocs;
+  Prof prof;
 
   RedSlot slot(int off, int ctr) const { return RedSlot{partials, counters + ctr, res + off}; }
 };
@@ -69,7 +84,33 @@ void suli_set_error(const std::string &m);
     }                                    \
   } while (0)
 
-// Poisson solvers (poisson_fcg.hip)
+// kernel timing scope: records events around the launches in its lifetime
+int prof_id(Ctx &c, const char *name);
+hipEvent_t prof_event(Ctx &c);
+void prof_flush(Ctx &c);
+struct ProfScope {
+  Ctx &c;
+  int id = -1;
+  hipEvent_t a = nullptr;
+  ProfScope(Ctx &cc, const char *name) : c(cc) {
+    if (c.prof.on) {
+      id = prof_id(c, name);
+      a = prof_event(c);
+      (void)hipEventRecord(a, c.st);
+    }
+  }
+  ~ProfScope() {
+    if (id >= 0) {
+      hipEvent_t b = prof_event(c);
+      (void)hipEventRecord(b, c.st);
+      c.prof.pending.push_back({id, a, b});
+      if (c.prof.pending.size() > 4096) prof_flush(c);
+    }
+  }
+};
+
+// Poisson solvers (poisson_ref.hip, poisson_fcg.hip)
+int poisson_ref_solve(Ctx &c, double dt, suli_poisson_stats *st);
 int fcg_create(Ctx &c);
 void fcg_destroy(Ctx &c);
 int fcg_solve(Ctx &c, double dt, suli_poisson_stats *st);

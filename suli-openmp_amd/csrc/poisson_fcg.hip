@@ -1,10 +1,712 @@
-// Mixed-precision flexible CG for the projection (placeholder until built).
-#include "ctx.hpp"
+// Mixed-precision pressure projection: flexible conjugate gradients (FCG(1),
+// Notay 2000) in FP32 inside an FP64 iterative-refinement loop.
+//
+// The system is the reference's symmetrically scaled one (9_grad.f90:118-151):
+// A_s e = b_s with A_s = D^-1/2 A D^-1/2, b_s = q/sqrt(D), p = e/sqrt(D),
+// set up in FP64 by the same kernels as the reference CG.  Each refinement
+// pass computes the true residual r = b_s - A_s x in FP64 (FP64 dot
+// products), removes its component along the null space sqrt(D) of the
+// singular Neumann operator, and solves A_s d = r approximately in FP32 with
+// FCG; x += d in FP64.  The solve stops on the FP64 relative residual
+// ||b_s - A_s x|| / ||b_s|| <= tol.
+//
+// Preconditioners: Jacobi (identity in the scaled system: the reference's own
+// symmetric diagonal scaling), a geometric multigrid V-cycle on the unscaled
+// operator (z = D^1/2 V(D^1/2 r)), or the learned U-Net (unet.hip).  Loop
+// control lives in device memory: every kernel returns at once when the
+// solve has converged, so the host enqueues iterations in chunks and reads
+// the state back once per chunk.
+#include <hip/hip_runtime.h>
 
-int fcg_create(Ctx &c) { (void)c; return 0; }
-void fcg_destroy(Ctx &c) { (void)c; }
-int fcg_solve(Ctx &c, double dt, suli_poisson_stats *st) {
-  (void)c; (void)dt; (void)st;
-  suli_set_error("SULI_POISSON_FCG is not available in this build");
-  return 5;
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <vector>
+
+#include "ctx.hpp"
+#include "fcg.hpp"
+
+// ---------------------------------------------------------------------------
+// device control block
+struct FcgCtl {
+  double bb[1];      // ||b_s||^2
+  double rout[3];    // outer: r.r, r.n, n.n (n = sqrt(D))
+  double rin0[1];    // inner: initial r.r
+  double k1[2];      // p.q, p.r
+  double k2[2];      // r.r, r.q (next beta numerator for Jacobi)
+  double zq[1];      // z.q_prev (preconditioned beta numerator)
+  double thr_in;     // inner stop: r.r <= thr_in
+  int maxit_in;
+  int active;        // inner loop running
+  int it;            // inner iterations of this pass
+  int first;         // next K1 is the first of the pass (beta = 0)
+};
+
+#define CTL_PTR(c, member) ((double *)((char *)(c) + offsetof(FcgCtl, member)))
+
+__device__ __forceinline__ int64_t nbx(const Grid &g, int64_t c, int q, int n, int64_t st, int dir) {
+  const int t = q + dir;
+  const int m = t < 1 ? 1 : (t > n ? n : t);
+  return c + (int64_t)(m - q) * st;
+}
+
+// A_s v at cell (i,j,k) with the reference's mirrored Neumann ghosts
+// (periodic walls wrap), coefficients of type C, vector of type V.
+template <typename C, typename V, typename A>
+__device__ __forceinline__ A apply_As(const Grid &g, int64_t c, int i, int j, int k, const C *__restrict__ ax,
+                                      const C *__restrict__ ay, const C *__restrict__ az, const V *__restrict__ v) {
+  const bool px = g.ccx0 == 0, py = g.ccy0 == 0;
+  auto nb = [&](int q, int n, int64_t st, int dir, bool per) -> int64_t {
+    int t = q + dir;
+    int m = t < 1 ? (per ? n : 1) : (t > n ? (per ? 1 : n) : t);
+    return c + (int64_t)(m - q) * st;
+  };
+  const int64_t ie = nb(i, g.nx, 1, 1, px), iw = nb(i, g.nx, 1, -1, px);
+  const int64_t jn = nb(j, g.ny, g.sx, 1, py), js = nb(j, g.ny, g.sx, -1, py);
+  const int64_t kt = nb(k, g.nz, g.sz, 1, false), kb = nb(k, g.nz, g.sz, -1, false);
+  return (A)v[c] - (A)ax[c + 1] * (A)v[ie] - (A)ax[c] * (A)v[iw] - (A)ay[c + g.sx] * (A)v[jn] -
+         (A)ay[c] * (A)v[js] - (A)az[c + g.sz] * (A)v[kt] - (A)az[c] * (A)v[kb];
+}
+
+#define CELL_IJK                                                                             \
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y; \
+  const int kz0 = 1 + blockIdx.z;
+
+// ---- outer (FP64) ---------------------------------------------------------
+// b_s = q/sqrt(D) norm
+__global__ __launch_bounds__(256) void k_bnorm(Grid g, const double *__restrict__ q, const double *__restrict__ sD,
+                                               double *__restrict__ b, RedSlot rs) {
+  CELL_IJK;
+  double acc[1] = {0.0};
+  if (i <= g.nx && j <= g.ny)
+    for (int k = kz0; k <= g.nz; k += gridDim.z) {
+      const int64_t c = g.idx(i, j, k);
+      const double bv = q[c] / sD[c];
+      b[c] = bv;
+      acc[0] += bv * bv;
+    }
+  __shared__ double lds[SULI_LDS_RED];
+  block_reduce<1>(acc, lds);
+  grid_reduce_finish<1>(acc, rs);
+}
+
+// r = b - A_s x in FP64; sums r.r, r.n, n.n
+__global__ __launch_bounds__(256) void k_res64(Grid g, const double *__restrict__ ax, const double *__restrict__ ay,
+                                               const double *__restrict__ az, const double *__restrict__ x,
+                                               const double *__restrict__ b, const double *__restrict__ sD,
+                                               double *__restrict__ r, RedSlot rs) {
+  CELL_IJK;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i <= g.nx && j <= g.ny)
+    for (int k = kz0; k <= g.nz; k += gridDim.z) {
+      const int64_t c = g.idx(i, j, k);
+      const double rv = b[c] - apply_As<double, double, double>(g, c, i, j, k, ax, ay, az, x);
+      r[c] = rv;
+      acc[0] += rv * rv;
+      acc[1] += rv * sD[c];
+      acc[2] += sD[c] * sD[c];
+    }
+  __shared__ double lds[3 * SULI_LDS_RED];
+  block_reduce<3>(acc, lds);
+  grid_reduce_finish<3>(acc, rs);
+}
+
+// r32 = r - (r.n/n.n) n; d = 0; sums r32.r32 and arms the inner loop
+__global__ __launch_bounds__(256) void k_r32(Grid g, const double *__restrict__ r, const double *__restrict__ sD,
+                                             float *__restrict__ r32, float *__restrict__ d32, FcgCtl *ctl,
+                                             double rel_in, int maxit, RedSlot rs) {
+  CELL_IJK;
+  const double mu = ctl->rout[1] / ctl->rout[2];
+  double acc[1] = {0.0};
+  if (i <= g.nx && j <= g.ny)
+    for (int k = kz0; k <= g.nz; k += gridDim.z) {
+      const int64_t c = g.idx(i, j, k);
+      const float rv = (float)(r[c] - mu * sD[c]);
+      r32[c] = rv;
+      d32[c] = 0.f;
+      acc[0] += (double)rv * (double)rv;
+    }
+  __shared__ double lds[SULI_LDS_RED];
+  block_reduce<1>(acc, lds);
+  if (grid_reduce_finish<1>(acc, rs) && threadIdx.x == 0 && threadIdx.y == 0) {
+    ctl->thr_in = ctl->rin0[0] * rel_in * rel_in;
+    ctl->maxit_in = maxit;
+    ctl->it = 0;
+    ctl->first = 1;
+    ctl->active = ctl->rin0[0] > 0 ? 1 : 0;
+  }
+}
+
+// x += d (FP64 += FP32)
+__global__ __launch_bounds__(256) void k_xupd(Grid g, double *__restrict__ x, const float *__restrict__ d) {
+  CELL_IJK;
+  if (i > g.nx || j > g.ny) return;
+  for (int k = kz0; k <= g.nz; k += gridDim.z) {
+    const int64_t c = g.idx(i, j, k);
+    x[c] += (double)d[c];
+  }
+}
+
+// ---- inner FCG (FP32 vectors) --------------------------------------------------
+// K1: beta from the previous reductions, p = z + beta p, q = A_s p; sums p.q, p.r
+// z is r for Jacobi.  p at the neighbours is rebuilt on the fly from z and p_old.
+__global__ __launch_bounds__(256) void k_fcg1(Grid g, const float *__restrict__ ax, const float *__restrict__ ay,
+                                              const float *__restrict__ az, const float *__restrict__ z,
+                                              const float *__restrict__ r, const float *__restrict__ pold,
+                                              float *__restrict__ pnew, float *__restrict__ q, FcgCtl *ctl,
+                                              int precond, RedSlot rs) {
+  if (!ctl->active) return;
+  CELL_IJK;
+  // FCG(1): beta = -(z.q_prev)/(p_prev.q_prev); with z = r that is -(r.q_prev)/(p.q)
+  float beta = 0.f;
+  if (!ctl->first) beta = (float)(-(precond ? ctl->zq[0] : ctl->k2[1]) / ctl->k1[0]);
+  double acc[2] = {0.0, 0.0};
+  if (i <= g.nx && j <= g.ny)
+    for (int k = kz0; k <= g.nz; k += gridDim.z) {
+      const int64_t c = g.idx(i, j, k);
+      const bool px = g.ccx0 == 0, py = g.ccy0 == 0;
+      auto nb = [&](int qq, int n, int64_t st, int dir, bool per) -> int64_t {
+        int t = qq + dir;
+        int m = t < 1 ? (per ? n : 1) : (t > n ? (per ? 1 : n) : t);
+        return c + (int64_t)(m - qq) * st;
+      };
+      const int64_t ie = nb(i, g.nx, 1, 1, px), iw = nb(i, g.nx, 1, -1, px);
+      const int64_t jn = nb(j, g.ny, g.sx, 1, py), js = nb(j, g.ny, g.sx, -1, py);
+      const int64_t kt = nb(k, g.nz, g.sz, 1, false), kb = nb(k, g.nz, g.sz, -1, false);
+      auto P = [&](int64_t a) { return z[a] + beta * pold[a]; };
+      const float pc = P(c);
+      const float qv = pc - ax[c + 1] * P(ie) - ax[c] * P(iw) - ay[c + g.sx] * P(jn) - ay[c] * P(js) -
+                       az[c + g.sz] * P(kt) - az[c] * P(kb);
+      pnew[c] = pc;
+      q[c] = qv;
+      acc[0] += (double)pc * (double)qv;
+      acc[1] += (double)pc * (double)r[c];
+    }
+  __shared__ double lds[2 * SULI_LDS_RED];
+  block_reduce<2>(acc, lds);
+  if (grid_reduce_finish<2>(acc, rs) && threadIdx.x == 0 && threadIdx.y == 0) ctl->first = 0;
+}
+
+// K2: alpha = p.r/p.q; d += alpha p; r -= alpha q; sums r.r, r.q; loop control
+__global__ __launch_bounds__(256) void k_fcg2(Grid g, const float *__restrict__ p, const float *__restrict__ q,
+                                              float *__restrict__ r, float *__restrict__ d, FcgCtl *ctl,
+                                              RedSlot rs) {
+  if (!ctl->active) return;
+  CELL_IJK;
+  const double pq = ctl->k1[0];
+  const float alpha = pq != 0.0 ? (float)(ctl->k1[1] / pq) : 0.f;
+  double acc[2] = {0.0, 0.0};
+  if (i <= g.nx && j <= g.ny)
+    for (int k = kz0; k <= g.nz; k += gridDim.z) {
+      const int64_t c = g.idx(i, j, k);
+      const float qv = q[c];
+      d[c] += alpha * p[c];
+      const float rv = r[c] - alpha * qv;
+      r[c] = rv;
+      acc[0] += (double)rv * (double)rv;
+      acc[1] += (double)rv * (double)qv;
+    }
+  __shared__ double lds[2 * SULI_LDS_RED];
+  block_reduce<2>(acc, lds);
+  if (grid_reduce_finish<2>(acc, rs) && threadIdx.x == 0 && threadIdx.y == 0) {
+    ctl->it += 1;
+    const bool stop = !(ctl->k2[0] > ctl->thr_in) || ctl->it >= ctl->maxit_in || !(pq > 0.0) ||
+                      !(ctl->k2[0] == ctl->k2[0]);
+    ctl->active = stop ? 0 : 1;
+  }
+}
+
+// z.q_prev for the preconditioned beta
+__global__ __launch_bounds__(256) void k_zq(Grid g, const float *__restrict__ z, const float *__restrict__ q,
+                                            FcgCtl *ctl, RedSlot rs) {
+  if (!ctl->active) return;
+  CELL_IJK;
+  double acc[1] = {0.0};
+  if (i <= g.nx && j <= g.ny && !ctl->first)
+    for (int k = kz0; k <= g.nz; k += gridDim.z) {
+      const int64_t c = g.idx(i, j, k);
+      acc[0] += (double)z[c] * (double)q[c];
+    }
+  __shared__ double lds[SULI_LDS_RED];
+  block_reduce<1>(acc, lds);
+  grid_reduce_finish<1>(acc, rs);
+}
+
+// ---- geometric multigrid on the unscaled operator ------------------------------
+// Level l: cells 1..n, faces 1..n+1 in a frame of its own; face conductances
+// cx (face i between cells i-1 and i), zero on the walls (Neumann); diagonal
+// dg = sum of the six face conductances.
+
+// level-0 conductances c_f = dt/(h^2 rho_f) from the FP64 set-up (a_f sqrt(D_P D_Q))
+__global__ void k_mg_c0(Grid g, const double *__restrict__ ax, const double *__restrict__ ay,
+                        const double *__restrict__ az, const double *__restrict__ D, float *__restrict__ cx,
+                        float *__restrict__ cy, float *__restrict__ cz, float *__restrict__ dg,
+                        float *__restrict__ ax32, float *__restrict__ ay32, float *__restrict__ az32,
+                        const double *__restrict__ sD, float *__restrict__ sD32) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i > nx + 1 || j > ny + 1 || k > nz + 1) return;
+  const int64_t c = g.idx(i, j, k);
+  if (j <= ny && k <= nz) {
+    ax32[c] = (float)ax[c];
+    cx[c] = (i == 1 || i == nx + 1) ? 0.f : (float)(ax[c] * sqrt(D[c] * D[c - 1]));
+  }
+  if (i <= nx && k <= nz) {
+    ay32[c] = (float)ay[c];
+    cy[c] = (j == 1 || j == ny + 1) ? 0.f : (float)(ay[c] * sqrt(D[c] * D[c - g.sx]));
+  }
+  if (i <= nx && j <= ny) {
+    az32[c] = (float)az[c];
+    cz[c] = (k == 1 || k == nz + 1) ? 0.f : (float)(az[c] * sqrt(D[c] * D[c - g.sz]));
+  }
+  if (i <= nx && j <= ny && k <= nz) sD32[c] = (float)sD[c];
+}
+
+__global__ void k_mg_diag(Grid g, const float *__restrict__ cx, const float *__restrict__ cy,
+                          const float *__restrict__ cz, float *__restrict__ dg) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (i > g.nx || j > g.ny || k > g.nz) return;
+  const int64_t c = g.idx(i, j, k);
+  dg[c] = cx[c] + cx[c + 1] + cy[c] + cy[c + g.sx] + cz[c] + cz[c + g.sz];
+}
+
+// coarse face conductance: half the sum of the four fine faces on it
+__global__ void k_mg_coarsen(Grid gf, Grid gc, const float *__restrict__ fx, const float *__restrict__ fy,
+                             const float *__restrict__ fz, float *__restrict__ cx, float *__restrict__ cy,
+                             float *__restrict__ cz) {
+  const int I = 1 + blockIdx.x * SULI_BX + threadIdx.x, J = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            K = 1 + blockIdx.z;
+  if (I > gc.nx + 1 || J > gc.ny + 1 || K > gc.nz + 1) return;
+  const int64_t c = gc.idx(I, J, K);
+  if (J <= gc.ny && K <= gc.nz) {
+    const int fi = 2 * I - 1, fj = 2 * J - 1, fk = 2 * K - 1;
+    cx[c] = 0.5f * (fx[gf.idx(fi, fj, fk)] + fx[gf.idx(fi, fj + 1, fk)] + fx[gf.idx(fi, fj, fk + 1)] +
+                    fx[gf.idx(fi, fj + 1, fk + 1)]);
+  }
+  if (I <= gc.nx && K <= gc.nz) {
+    const int fi = 2 * I - 1, fj = 2 * J - 1, fk = 2 * K - 1;
+    cy[c] = 0.5f * (fy[gf.idx(fi, fj, fk)] + fy[gf.idx(fi + 1, fj, fk)] + fy[gf.idx(fi, fj, fk + 1)] +
+                    fy[gf.idx(fi + 1, fj, fk + 1)]);
+  }
+  if (I <= gc.nx && J <= gc.ny) {
+    const int fi = 2 * I - 1, fj = 2 * J - 1, fk = 2 * K - 1;
+    cz[c] = 0.5f * (fz[gf.idx(fi, fj, fk)] + fz[gf.idx(fi + 1, fj, fk)] + fz[gf.idx(fi, fj + 1, fk)] +
+                    fz[gf.idx(fi + 1, fj + 1, fk)]);
+  }
+}
+
+__device__ __forceinline__ float mg_Ax(const Grid &g, int64_t c, const float *__restrict__ cx,
+                                       const float *__restrict__ cy, const float *__restrict__ cz,
+                                       const float *__restrict__ x) {
+  // sum_f c_f x_nb (wall faces have c = 0; their ghost neighbours hold 0)
+  return cx[c + 1] * x[c + 1] + cx[c] * x[c - 1] + cy[c + g.sx] * x[c + g.sx] + cy[c] * x[c - g.sx] +
+         cz[c + g.sz] * x[c + g.sz] + cz[c] * x[c - g.sz];
+}
+
+// red-black Gauss-Seidel half sweep on the cells of one colour; when zero_x
+// is set the other colour is taken as zero (first sweep from x = 0).
+__global__ __launch_bounds__(256) void k_mg_gs(Grid g, const float *__restrict__ cx, const float *__restrict__ cy,
+                                               const float *__restrict__ cz, const float *__restrict__ dg,
+                                               const float *__restrict__ b, float *__restrict__ x, int color,
+                                               int zero_x, const int *active) {
+  if (!*active) return;
+  const int ti = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (j > g.ny || k > g.nz) return;
+  const int i = 1 + 2 * ti + ((1 + j + k + color) & 1);
+  if (i > g.nx) return;
+  const int64_t c = g.idx(i, j, k);
+  const float s = zero_x ? 0.f : mg_Ax(g, c, cx, cy, cz, x);
+  x[c] = (b[c] + s) / dg[c];
+}
+
+// zero the other colour after a zero_x first half sweep
+__global__ __launch_bounds__(256) void k_mg_zero_color(Grid g, float *__restrict__ x, int color,
+                                                       const int *active) {
+  if (!*active) return;
+  const int ti = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (j > g.ny || k > g.nz) return;
+  const int i = 1 + 2 * ti + ((1 + j + k + color) & 1);
+  if (i > g.nx) return;
+  x[g.idx(i, j, k)] = 0.f;
+}
+
+// coarse rhs = sum over the 8 children of the fine residual b - A x; coarse x = 0
+__global__ __launch_bounds__(256) void k_mg_restrict(Grid gf, Grid gc, const float *__restrict__ cx,
+                                                     const float *__restrict__ cy, const float *__restrict__ cz,
+                                                     const float *__restrict__ dg, const float *__restrict__ b,
+                                                     const float *__restrict__ x, float *__restrict__ bc,
+                                                     float *__restrict__ xc, const int *active) {
+  if (!*active) return;
+  const int I = 1 + blockIdx.x * SULI_BX + threadIdx.x, J = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            K = 1 + blockIdx.z;
+  if (I > gc.nx || J > gc.ny) return;
+  float s = 0.f;
+#pragma unroll
+  for (int d = 0; d < 8; d++) {
+    const int64_t c = gf.idx(2 * I - 1 + (d & 1), 2 * J - 1 + ((d >> 1) & 1), 2 * K - 1 + (d >> 2));
+    s += b[c] - (dg[c] * x[c] - mg_Ax(gf, c, cx, cy, cz, x));
+  }
+  const int64_t cc = gc.idx(I, J, K);
+  bc[cc] = s;
+  xc[cc] = 0.f;
+}
+
+// x_fine += x_coarse(parent)
+__global__ __launch_bounds__(256) void k_mg_prolong(Grid gf, Grid gc, const float *__restrict__ xc,
+                                                    float *__restrict__ x, const int *active) {
+  if (!*active) return;
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (i > gf.nx || j > gf.ny) return;
+  x[gf.idx(i, j, k)] += xc[gc.idx((i + 1) >> 1, (j + 1) >> 1, (k + 1) >> 1)];
+}
+
+// coarsest level: many red-black sweeps in one work-group
+__global__ __launch_bounds__(1024) void k_mg_coarse(Grid g, const float *__restrict__ cx,
+                                                    const float *__restrict__ cy, const float *__restrict__ cz,
+                                                    const float *__restrict__ dg, const float *__restrict__ b,
+                                                    float *__restrict__ x, int sweeps, const int *active) {
+  if (!*active) return;
+  const int n = g.nx * g.ny * g.nz;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int i = 1 + t % g.nx, j = 1 + (t / g.nx) % g.ny, k = 1 + t / (g.nx * g.ny);
+    x[g.idx(i, j, k)] = 0.f;
+  }
+  __syncthreads();
+  for (int s = 0; s < sweeps; s++)
+    for (int color = 0; color < 2; color++) {
+      for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int i = 1 + t % g.nx, j = 1 + (t / g.nx) % g.ny, k = 1 + t / (g.nx * g.ny);
+        if (((i + j + k) & 1) != color) continue;
+        const int64_t c = g.idx(i, j, k);
+        x[c] = (b[c] + mg_Ax(g, c, cx, cy, cz, x)) / dg[c];
+      }
+      __syncthreads();
+    }
+}
+
+// level-0 input b0 = sqrt(D) r and output z = sqrt(D) x0
+__global__ __launch_bounds__(256) void k_scale(Grid g, const float *__restrict__ s, const float *__restrict__ a,
+                                               float *__restrict__ o, const int *active) {
+  if (!*active) return;
+  CELL_IJK;
+  if (i > g.nx || j > g.ny) return;
+  for (int k = kz0; k <= g.nz; k += gridDim.z) {
+    const int64_t c = g.idx(i, j, k);
+    o[c] = s[c] * a[c];
+  }
+}
+
+// final pressure p = x/sqrt(D) with mirrored ghosts (9_grad.f90:268-274)
+__global__ void k_pout(Grid g, const double *__restrict__ x, const double *__restrict__ sD, double *__restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y,
+            k = blockIdx.z;
+  if (i > g.nx + 1 || j > g.ny + 1 || k > g.nz + 1) return;
+  auto m = [](int q, int n, bool per) { return q < 1 ? (per ? n : 1) : (q > n ? (per ? 1 : n) : q); };
+  const int64_t s = g.idx(m(i, g.nx, g.ccx0 == 0), m(j, g.ny, g.ccy0 == 0), m(k, g.nz, false));
+  p[g.idx(i, j, k)] = x[s] / sD[s];
+}
+
+__global__ void k_xinit(Grid g, const double *__restrict__ p, const double *__restrict__ sD, double *__restrict__ x) {
+  CELL_IJK;
+  if (i > g.nx || j > g.ny) return;
+  for (int k = kz0; k <= g.nz; k += gridDim.z) {
+    const int64_t c = g.idx(i, j, k);
+    x[c] = p[c] * sD[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct MgLevel {
+  Grid g;
+  float *cx = nullptr, *cy = nullptr, *cz = nullptr, *dg = nullptr, *b = nullptr, *x = nullptr;
+};
+
+struct PoissonFCG {
+  FcgCtl *ctl = nullptr, *ctl_h = nullptr;
+  double *b = nullptr, *x = nullptr, *r = nullptr;          // FP64 outer vectors (frame)
+  float *r32 = nullptr, *d32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr, *z32 = nullptr;
+  float *ax32 = nullptr, *ay32 = nullptr, *az32 = nullptr, *sD32 = nullptr;
+  std::vector<MgLevel> lv;  // lv[0] shares the main frame
+  int kz = 1;               // grid.z of the k-looped reduction kernels
+  UNetPrecond *unet = nullptr;
+};
+
+static int falloc(Ctx &c, float **p, int64_t n) {
+  HIPCHK(hipMalloc((void **)p, (size_t)n * sizeof(float)));
+  HIPCHK(hipMemsetAsync(*p, 0, (size_t)n * sizeof(float), c.st));
+  c.allocs.push_back(*p);
+  return 0;
+}
+static int dalloc64(Ctx &c, double **p, int64_t n) {
+  HIPCHK(hipMalloc((void **)p, (size_t)n * sizeof(double)));
+  HIPCHK(hipMemsetAsync(*p, 0, (size_t)n * sizeof(double), c.st));
+  c.allocs.push_back(*p);
+  return 0;
+}
+
+int fcg_create(Ctx &c) {
+  PoissonFCG *f = new PoissonFCG();
+  c.fcg = f;
+  const Grid &g = c.g;
+  const int64_t n = g.total;
+  HIPCHK(hipMalloc((void **)&f->ctl, sizeof(FcgCtl)));
+  c.allocs.push_back(f->ctl);
+  HIPCHK(hipMemsetAsync(f->ctl, 0, sizeof(FcgCtl), c.st));
+  HIPCHK(hipHostMalloc((void **)&f->ctl_h, sizeof(FcgCtl), hipHostMallocDefault));
+  if (dalloc64(c, &f->b, n) || dalloc64(c, &f->x, n) || dalloc64(c, &f->r, n)) return 1;
+  float **fl[] = {&f->r32, &f->d32, &f->p32, &f->p32b, &f->q32, &f->z32, &f->ax32, &f->ay32, &f->az32, &f->sD32};
+  for (float **q : fl)
+    if (falloc(c, q, n)) return 1;
+  // k-loop depth of the reduction kernels: about 2048 blocks in total
+  const int64_t plane_blocks = (int64_t)((g.nx + SULI_BX - 1) / SULI_BX) * ((g.ny + SULI_BY - 1) / SULI_BY);
+  f->kz = (int)std::max<int64_t>(1, std::min<int64_t>(g.nz, 2048 / std::max<int64_t>(plane_blocks, 1)));
+  // multigrid hierarchy
+  MgLevel l0;
+  l0.g = g;
+  if (falloc(c, &l0.cx, n) || falloc(c, &l0.cy, n) || falloc(c, &l0.cz, n) || falloc(c, &l0.dg, n) ||
+      falloc(c, &l0.b, n) || falloc(c, &l0.x, n))
+    return 1;
+  f->lv.push_back(l0);
+  for (;;) {
+    const Grid &gf = f->lv.back().g;
+    if (gf.nx % 2 || gf.ny % 2 || gf.nz % 2 || gf.nx < 4 || gf.ny < 4 || gf.nz < 4) break;
+    if ((int64_t)gf.nx * gf.ny * gf.nz <= 512) break;
+    MgLevel l;
+    Grid gc = gf;
+    gc.nx = gf.nx / 2; gc.ny = gf.ny / 2; gc.nz = gf.nz / 2;
+    gc.sx = gc.nx + 3; gc.sy = gc.ny + 3; gc.sz = (int64_t)gc.sx * gc.sy; gc.off = 0;
+    gc.total = gc.sz * (gc.nz + 3) + 8;
+    gc.dx = gf.dx * 2; gc.dy = gf.dy * 2; gc.dz = gf.dz * 2;
+    l.g = gc;
+    if (falloc(c, &l.cx, gc.total) || falloc(c, &l.cy, gc.total) || falloc(c, &l.cz, gc.total) ||
+        falloc(c, &l.dg, gc.total) || falloc(c, &l.b, gc.total) || falloc(c, &l.x, gc.total))
+      return 1;
+    f->lv.push_back(l);
+  }
+  if (unet_create(c, &f->unet)) return 1;
+  return 0;
+}
+
+void fcg_destroy(Ctx &c) {
+  if (!c.fcg) return;
+  unet_destroy(c, c.fcg->unet);
+  if (c.fcg->ctl_h) (void)hipHostFree(c.fcg->ctl_h);
+  delete c.fcg;
+  c.fcg = nullptr;
+}
+
+static dim3 cells_kloop(const Grid &g, int kz) {
+  return dim3((g.nx + SULI_BX - 1) / SULI_BX, (g.ny + SULI_BY - 1) / SULI_BY, kz);
+}
+static dim3 cells_full(const Grid &g) {
+  return dim3((g.nx + SULI_BX - 1) / SULI_BX, (g.ny + SULI_BY - 1) / SULI_BY, g.nz);
+}
+static dim3 faces_full(const Grid &g) {
+  return dim3((g.nx + 1 + SULI_BX - 1) / SULI_BX, (g.ny + 1 + SULI_BY - 1) / SULI_BY, g.nz + 1);
+}
+static dim3 half_cells(const Grid &g) {
+  return dim3(((g.nx + 1) / 2 + SULI_BX - 1) / SULI_BX, (g.ny + SULI_BY - 1) / SULI_BY, g.nz);
+}
+
+// Build the FP32 operator copies and the multigrid hierarchy for this dt/rho.
+static int mg_setup(Ctx &c) {
+  PoissonFCG &f = *c.fcg;
+  const dim3 b2(SULI_BX, SULI_BY, 1);
+  {
+    ProfScope ps(c, "mg_setup");
+    MgLevel &l0 = f.lv[0];
+    hipLaunchKernelGGL(k_mg_c0, faces_full(c.g), b2, 0, c.st, c.g, c.ax, c.ay, c.az, c.D, l0.cx, l0.cy, l0.cz,
+                       l0.dg, f.ax32, f.ay32, f.az32, c.sD, f.sD32);
+    hipLaunchKernelGGL(k_mg_diag, cells_full(c.g), b2, 0, c.st, c.g, l0.cx, l0.cy, l0.cz, l0.dg);
+    for (size_t q = 1; q < f.lv.size(); q++) {
+      MgLevel &lf = f.lv[q - 1], &lc = f.lv[q];
+      hipLaunchKernelGGL(k_mg_coarsen, faces_full(lc.g), b2, 0, c.st, lf.g, lc.g, lf.cx, lf.cy, lf.cz, lc.cx,
+                         lc.cy, lc.cz);
+      hipLaunchKernelGGL(k_mg_diag, cells_full(lc.g), b2, 0, c.st, lc.g, lc.cx, lc.cy, lc.cz, lc.dg);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// One multigrid V-cycle: z = sqrt(D) V(sqrt(D) r).
+static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
+  PoissonFCG &f = *c.fcg;
+  const dim3 b2(SULI_BX, SULI_BY, 1);
+  const int nu1 = 2, nu2 = 2, coarse_sweeps = 40;
+  const int L = (int)f.lv.size();
+  {
+    ProfScope ps(c, "mg_scale");
+    hipLaunchKernelGGL(k_scale, cells_kloop(c.g, f.kz), b2, 0, c.st, c.g, f.sD32, r, f.lv[0].b, active);
+  }
+  for (int l = 0; l < L - 1; l++) {
+    MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
+    ProfScope ps(c, l == 0 ? "mg_smooth_restrict_l0" : "mg_smooth_restrict");
+    for (int s = 0; s < nu1; s++) {
+      hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 0,
+                         s == 0 ? 1 : 0, active);
+      if (s == 0)
+        hipLaunchKernelGGL(k_mg_zero_color, half_cells(lf.g), b2, 0, c.st, lf.g, lf.x, 1, active);
+      hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 1,
+                         0, active);
+    }
+    hipLaunchKernelGGL(k_mg_restrict, cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, lf.cx, lf.cy, lf.cz, lf.dg,
+                       lf.b, lf.x, lc.b, lc.x, active);
+  }
+  {
+    MgLevel &lc = f.lv[L - 1];
+    ProfScope ps(c, "mg_coarse");
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, c.st, lc.g, lc.cx, lc.cy, lc.cz, lc.dg, lc.b, lc.x,
+                       coarse_sweeps, active);
+  }
+  for (int l = L - 2; l >= 0; l--) {
+    MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
+    ProfScope ps(c, l == 0 ? "mg_prolong_smooth_l0" : "mg_prolong_smooth");
+    hipLaunchKernelGGL(k_mg_prolong, cells_full(lf.g), b2, 0, c.st, lf.g, lc.g, lc.x, lf.x, active);
+    for (int s = 0; s < nu2; s++) {
+      hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 1,
+                         0, active);
+      hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 0,
+                         0, active);
+    }
+  }
+  {
+    ProfScope ps(c, "mg_scale");
+    hipLaunchKernelGGL(k_scale, cells_kloop(c.g, f.kz), b2, 0, c.st, c.g, f.sD32, f.lv[0].x, z, active);
+  }
+}
+
+// Apply the preconditioner selected for this solve: z = M r.
+static void apply_precond(Ctx &c, int pc, const float *r, float *z, const int *active) {
+  if (pc == SULI_PRECOND_LEARNED && c.fcg->unet && unet_ready(c.fcg->unet)) {
+    unet_apply(c, c.fcg->unet, r, z, active);
+    return;
+  }
+  mg_vcycle(c, r, z, active);
+}
+
+int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
+  PoissonFCG &f = *c.fcg;
+  const Grid &g = c.g;
+  auto t0 = std::chrono::steady_clock::now();
+  const dim3 b2(SULI_BX, SULI_BY, 1);
+  const dim3 gk = cells_kloop(g, f.kz);
+  const double tol = c.P.tol_rel;
+  const int pc = c.P.precond;
+  const bool use_pc = pc != SULI_PRECOND_JACOBI;
+  // FP64 set-up shared with the reference CG
+  {
+    ProfScope ps(c, "poisson_setup");
+    DISPATCH(c, Launch<TT>::pD(g, c.st, dt, c.u, c.v, c.w, c.rho, c.D, c.sD, c.q));
+    DISPATCH(c, Launch<TT>::pcoef(g, c.st, dt, c.rho, c.D, c.ax, c.ay, c.az));
+    hipLaunchKernelGGL(k_xinit, gk, b2, 0, c.st, g, c.prd1, c.sD, f.x);
+    hipLaunchKernelGGL(k_bnorm, gk, b2, 0, c.st, g, c.q, c.sD, f.b,
+                       RedSlot{c.partials, c.counters + 10, CTL_PTR(f.ctl, bb)});
+  }
+  if (mg_setup(c)) return 1;
+  if (use_pc && pc == SULI_PRECOND_LEARNED && f.unet) unet_prepare(c, f.unet);
+  int outer = 0, inner_total = 0, fallbacks = 0;
+  double res0 = -1, res = 0, bnorm = 0;
+  bool conv = false;
+  const int max_outer = 12;
+  const double rel_in = 1e-4;
+  const int maxit_in = c.P.max_it > 0 ? c.P.max_it : 10000;
+  int *active = (int *)((char *)f.ctl + offsetof(FcgCtl, active));
+  for (;;) {
+    {
+      ProfScope ps(c, "fcg_residual64");
+      hipLaunchKernelGGL(k_res64, gk, b2, 0, c.st, g, c.ax, c.ay, c.az, f.x, f.b, c.sD, f.r,
+                         RedSlot{c.partials, c.counters + 11, CTL_PTR(f.ctl, rout)});
+    }
+    HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    bnorm = std::sqrt(f.ctl_h->bb[0]);
+    const double rn = std::sqrt(f.ctl_h->rout[0]);
+    res = bnorm > 0 ? rn / bnorm : 0.0;
+    if (res0 < 0) res0 = res;
+    if (!(res > tol) || outer >= max_outer) {
+      conv = !(res > tol);
+      break;
+    }
+    outer++;
+    // inner target: rel_in of this pass, but no further than the overall tolerance needs
+    const double want = 0.3 * tol * bnorm / rn;
+    const double rin = std::max(rel_in, std::min(want, 1.0));
+    {
+      ProfScope ps(c, "fcg_project");
+      hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rin, maxit_in - inner_total,
+                         RedSlot{c.partials, c.counters + 12, CTL_PTR(f.ctl, rin0)});
+    }
+    int chunk = use_pc ? 4 : 16;
+    for (;;) {
+      for (int n = 0; n < chunk; n++) {
+        const float *zin = f.r32;
+        if (use_pc) {
+          apply_precond(c, pc, f.r32, f.z32, active);
+          ProfScope ps(c, "fcg_zq");
+          hipLaunchKernelGGL(k_zq, gk, b2, 0, c.st, g, f.z32, f.q32, f.ctl,
+                             RedSlot{c.partials, c.counters + 13, CTL_PTR(f.ctl, zq)});
+          zin = f.z32;
+        }
+        {
+          ProfScope ps(c, "fcg_spmv_dot");
+          hipLaunchKernelGGL(k_fcg1, gk, b2, 0, c.st, g, f.ax32, f.ay32, f.az32, zin, f.r32, f.p32, f.p32b, f.q32,
+                             f.ctl, use_pc ? 1 : 0, RedSlot{c.partials, c.counters + 14, CTL_PTR(f.ctl, k1)});
+        }
+        std::swap(f.p32, f.p32b);
+        {
+          ProfScope ps(c, "fcg_update");
+          hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.p32, f.q32, f.r32, f.d32, f.ctl,
+                             RedSlot{c.partials, c.counters + 15, CTL_PTR(f.ctl, k2)});
+        }
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
+      HIPCHK(hipStreamSynchronize(c.st));
+      if (!f.ctl_h->active) break;
+      if (chunk < 64) chunk *= 2;
+    }
+    inner_total += f.ctl_h->it;
+    {
+      ProfScope ps(c, "fcg_xupdate");
+      hipLaunchKernelGGL(k_xupd, gk, b2, 0, c.st, g, f.x, f.d32);
+    }
+    if (inner_total >= maxit_in) {
+      // iteration budget spent: report the residual after this pass
+      hipLaunchKernelGGL(k_res64, gk, b2, 0, c.st, g, c.ax, c.ay, c.az, f.x, f.b, c.sD, f.r,
+                         RedSlot{c.partials, c.counters + 11, CTL_PTR(f.ctl, rout)});
+      HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
+      HIPCHK(hipStreamSynchronize(c.st));
+      res = bnorm > 0 ? std::sqrt(f.ctl_h->rout[0]) / bnorm : 0.0;
+      conv = !(res > tol);
+      break;
+    }
+  }
+  {
+    ProfScope ps(c, "poisson_final");
+    const dim3 pg((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4, g.nz + 2);
+    hipLaunchKernelGGL(k_pout, pg, dim3(64, 4, 1), 0, c.st, g, f.x, c.sD, c.prd1);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c.st));
+  auto t1 = std::chrono::steady_clock::now();
+  if (stt) {
+    std::memset(stt, 0, sizeof(*stt));
+    stt->iters = inner_total;
+    stt->outer = outer;
+    stt->fallbacks = fallbacks;
+    stt->converged = conv ? 1 : 0;
+    stt->res0 = res0;
+    stt->res = res;
+    stt->ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+  return 0;
 }

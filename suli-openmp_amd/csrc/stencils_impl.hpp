@@ -154,7 +154,7 @@ void Launch<T>::pD(const Grid &g, hipStream_t s, double dt, const void *u, const
               (const T *)rho, D, sD, q);
   dim3 b(64, 4, 1);
   dim3 gr((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4, g.nz + 2);
-  SULI_LAUNCH(k_mirror_ghosts, gr, b, s, g, D, sD);
+  SULI_LAUNCH(k_mirror_ghosts<0>, gr, b, s, g, D, sD);
 }
 
 template <typename T>

@@ -30,3 +30,58 @@ def gpu_from_state(state, **kw):
     sv = pysuli.Solver.from_oracle_params(state.p, **kw)
     pysuli.load_state(sv, state)
     return sv
+
+
+def assemble_projection(p, rho, u, v, w, dt):
+    """Sparse form of the projection system of graddin (9_grad.f90:46-116)
+    for non-periodic walls: A p = sum over interior faces c_f (p_P - p_Q),
+    c_f = dt/(h^2 rho_f) with rho_f the face average (end faces copied);
+    D = sum of all six c_f (wall faces included, as the reference's scaling);
+    q = -div u*.  Returns (A csr, D, q) over cells in Fortran order."""
+    import scipy.sparse as sp
+    nx, ny, nz = p.nx, p.ny, p.nz
+    rho = np.asarray(rho)
+
+    def face_rho(axis):
+        a = np.moveaxis(rho, axis, 0)
+        n = a.shape[0]
+        f = np.empty((n + 1,) + a.shape[1:])
+        f[1:n] = 0.5 * (a[1:] + a[:-1])
+        f[0] = f[1]
+        f[n] = f[n - 1]
+        return np.moveaxis(f, 0, axis)
+
+    cx = dt / p.dx**2 / face_rho(0)
+    cy = dt / p.dy**2 / face_rho(1)
+    cz = dt / p.dz**2 / face_rho(2)
+    D = cx[1:] + cx[:-1] + cy[:, 1:] + cy[:, :-1] + cz[:, :, 1:] + cz[:, :, :-1]
+    q = ((u[1:-2, 1:-1, 1:-1] - u[2:-1, 1:-1, 1:-1]) / p.dx + (v[1:-1, 1:-2, 1:-1] - v[1:-1, 2:-1, 1:-1]) / p.dy
+         + (w[1:-1, 1:-1, 1:-2] - w[1:-1, 1:-1, 2:-1]) / p.dz)
+    idx = np.arange(nx * ny * nz).reshape((nx, ny, nz), order="F")
+    rows, cols, vals = [], [], []
+    diag = np.zeros((nx, ny, nz))
+    for axis, c in ((0, cx), (1, cy), (2, cz)):
+        cin = np.take(c, np.arange(1, c.shape[axis] - 1), axis=axis)  # interior faces
+        a = np.take(idx, np.arange(0, idx.shape[axis] - 1), axis=axis)
+        b = np.take(idx, np.arange(1, idx.shape[axis]), axis=axis)
+        rows += [a.ravel(), b.ravel()]
+        cols += [b.ravel(), a.ravel()]
+        vals += [-cin.ravel(), -cin.ravel()]
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[axis] = slice(0, -1)
+        hi[axis] = slice(1, None)
+        diag[tuple(lo)] += cin
+        diag[tuple(hi)] += cin
+    rows.append(idx.ravel())
+    cols.append(idx.ravel())
+    vals.append(diag.ravel())
+    N = nx * ny * nz
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+    return A, D.ravel(order="F"), q.ravel(order="F")
+
+
+def scaled_relres(A, D, q, pcell):
+    """||D^-1/2 (q - A p)|| / ||D^-1/2 q||, the solver's stopping measure."""
+    s = 1.0 / np.sqrt(D)
+    return float(np.linalg.norm(s * (q - A @ pcell)) / np.linalg.norm(s * q))
