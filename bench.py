@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Benchmark of the SuLi-IC time step on one MI355X.
+
+Workload: 3-D dam break, water/air (rho 1000/1.2 kg/m^3, mu 1e-3/1.8e-5
+Pa s, g = 9.81 in -z), 1.0 x 0.5 x 0.5 m tank on a 256x128x128 staggered
+grid (h = 1/256), no-slip walls, Smagorinsky Cs = 0.1, WENO5 level set.
+Water column width/height drawn from the seed (splitmix64): width in
+[0.2, 0.4] m, height in [0.3, 0.45] m.  Fixed dt = 0.2 h / (2 sqrt(g H)).
+Fields FP32 (stencils and Krylov sweeps), FP64 pressure, residuals, dot
+products and volume integrals; Poisson solve = FCG with FP64 iterative
+refinement to a 1e-8 FP64 relative residual, geometric-multigrid
+preconditioner.  Synthetic data: the tank is initialised in place.
+
+A "step" is one full time step of 1_principal.f90:74-95 (level set, two RK2
+stages each with visco, convdiff+tempo, contorno, projection, posdin).
+After --warmup untimed steps, --steps steps are timed between two stream
+synchronisations.  value = nx*ny*nz*steps/seconds/1e6 (Mcell-steps/s).
+
+The JSON line also carries the roofline of the dominant kernel (algorithmic
+bytes per launch / its average HIP-event duration on the library's stream
+over the timed region) and a CPU baseline: the flang-built reference
+harness (oracle/_ref) timed on one host core on a bounded sample of the same
+workload (one level_set plus one RK stage from the GPU's post-warmup state;
+per-step time = level_set + 2 x stage).
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "suli-openmp_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+# Algorithmic bytes per cell of each timed kernel (each distinct array read
+# once and written once per cell; FP32 fields = 4 B, FP64 = 8 B).  Kernels
+# whose grid is a coarse multigrid level use that level's cells.
+BYTES_PER_CELL = {
+    "ls_advect_rk3": 6 * 4,        # phi, phi_n, u, v, w -> phi
+    "reinit_sign": 2 * 4,          # phi -> S
+    "reinit_rk3": 4 * 4,           # phi, phi_it, S -> phi
+    "heaviside_normals": 11 * 4,   # phi -> H, Hx, Hy, Hz, rho, mu, |grad phi|, n (3)
+    "curvature": 7 * 4,            # n (3) -> d (3), kappa
+    "volume_correction": 3 * 4,
+    "visco": 5 * 4,                # u, v, w, rho -> nu_t
+    "convdiff_tempo": 16 * 4,      # u,v,w,nu_t,mu,rho,kappa,Hx,Hy,Hz (+u0,v0,w0) -> u*,v*,w* (+u0,v0,w0)
+    "posdin": 8 + 7 * 4,           # p (FP64), rho, u, v, w -> u, v, w
+    "est_div": 3 * 4,
+    "fcg_spmv_dot": 8 * 4,         # z, p_old, r, ax, ay, az -> p, q
+    "fcg_update": 6 * 4,           # p, q, r, d -> d, r
+    "fcg_zq": 2 * 4,
+    "fcg_residual64": 7 * 8,       # x, b, sqrt(D), ax, ay, az (FP64) -> r (FP64)
+    "fcg_project": 8 + 8 + 4 + 4,  # r, sqrt(D) (FP64) -> r32, d
+    "fcg_xupdate": 8 + 4 + 8,
+    "mg_gs_l0": 5 * 4 + 2,         # b, dg (half), c_x,c_y,c_z, x -> x (half)
+    "mg_restrict_l0": 7 * 4,
+    "mg_prolong_l0": 2 * 4,
+    "mg_scale": 3 * 4,
+}
+
+
+def splitmix64(seed):
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        z = z ^ (z >> 31)
+        yield (z >> 11) / float(1 << 53)
+
+
+def column_geometry(seed):
+    r = splitmix64(seed)
+    width = 0.2 + 0.2 * next(r)
+    height = 0.3 + 0.15 * next(r)
+    return width, height
+
+
+def prof_table(L):
+    import ctypes as C
+    n = L.suli_prof_count()
+    out = {}
+    for q in range(n):
+        name = C.create_string_buffer(128)
+        cnt = C.c_int64()
+        tot = C.c_double()
+        L.suli_prof_get(q, name, 128, C.byref(cnt), C.byref(tot))
+        out[name.value.decode()] = (int(cnt.value), float(tot.value))
+    return out
+
+
+def cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, budget_s):
+    """Time the flang-built reference on a bounded sample of the workload."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    p = O.Params(nx=nx, ny=ny, nz=nz, dx=h, dy=h, dz=h)
+    exe = os.path.join(O.REF_DIR, O.ref_tag(p), "harness")
+    if not os.path.exists(exe):
+        return None
+    st = O.State(p)
+    for n in ["u", "v", "w", "prd1", "ls", "nut"]:
+        st.arr(n)[...] = sv.download(n)
+    st.scal.update(dt=dt, vol_ini=vol_ini)
+    tmp = tempfile.mkdtemp(prefix="suli_cpu_")
+    fin = os.path.join(tmp, "state.bin")
+    st.to_file(fin)
+    del st
+    t0 = time.time()
+    try:
+        r = subprocess.run(["taskset", "-c", "0", exe, fin, "-", "bench"], capture_output=True, text=True,
+                           timeout=budget_s, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        out = r.stdout
+    except subprocess.TimeoutExpired:
+        out = ""
+    finally:
+        os.remove(fin)
+        os.rmdir(tmp)
+    wall = time.time() - t0
+    vals = {}
+    for tok in out.split():
+        if "=" in tok:
+            k, v = tok.split("=")
+            vals[k] = float(v)
+    if "stage1_seconds" not in vals:
+        return {"value": None, "unit": "Mcell-steps/s", "cores": 1, "kind": "reference",
+                "sample": f"level_set + RK stage 1 at {nx}x{ny}x{nz} did not finish within {budget_s}s"}
+    step_s = vals["level_set_seconds"] + 2 * vals["stage1_seconds"]
+    return {"value": nx * ny * nz / step_s / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": "reference",
+            "sample": (f"flang -O2 reference harness, 1 core: one level_set ({vals['level_set_seconds']:.1f}s) + one "
+                       f"RK stage ({vals['stage1_seconds']:.1f}s, of which graddin {vals['graddin_seconds']:.1f}s) at "
+                       f"{nx}x{ny}x{nz} from the GPU post-warmup state; step = level_set + 2 stages = {step_s:.1f}s"),
+            "cpu_model": _cpu_model(), "host_nproc": os.cpu_count(), "wall_s": round(wall, 1)}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--precond", default="gmg", choices=["jacobi", "gmg", "learned"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=150.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the time step on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU slab decomposition is a design, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+    import pysuli
+    nx = args.nx
+    ny, nz = nx // 2, nx // 2
+    h = 1.0 / nx
+    width, height = column_geometry(args.seed)
+    g = 9.81
+    dt = 0.2 * h / (2.0 * math.sqrt(g * height))
+    pc = {"jacobi": pysuli.PRECOND_JACOBI, "gmg": pysuli.PRECOND_GMG, "learned": pysuli.PRECOND_LEARNED}[args.precond]
+    sv = pysuli.Solver(nx, ny, nz, h, precision=pysuli.PREC_MIXED, poisson=pysuli.POISSON_FCG, precond=pc,
+                       tol_rel=1e-8)
+    L = pysuli.lib()
+    vol_ini = sv.init_dambreak(width, height)
+    sv.contorno(1)
+    for _ in range(args.warmup):
+        sv.step(dt)
+    sv.sync()
+    L.suli_prof_enable(1)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ls, ps = sv.step(dt)
+        stats.append((ls, ps))
+    sv.sync()
+    t1 = time.perf_counter()
+    prof = prof_table(L)
+    L.suli_prof_enable(0)
+    secs = t1 - t0
+    ncell = nx * ny * nz
+    value = ncell * args.steps / secs / 1e6
+    solves = [p for _, ps in stats for p in ps]
+    pois_ms = sum(p["ms"] for p in solves) / len(solves)
+    pois_it = sum(p["iters"] for p in solves) / len(solves)
+    # dominant kernel and its roofline
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    dname, (dcount, dms) = dom
+    bpc = BYTES_PER_CELL.get(dname)
+    roof = None
+    if bpc is not None and dcount > 0:
+        avg_s = dms / dcount / 1e3
+        achieved = bpc * ncell / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dname,
+                "bytes_per_launch": bpc * ncell, "avg_launch_us": round(avg_s * 1e6, 2), "launches": dcount}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, args.cpu_budget)
+    vol_now = sv.volumes()[1]
+    maxdiv = sv.est_div()
+    sv.close()
+    kernels = {k: {"launches": c, "ms_total": round(t, 3), "share": round(t / secs / 1e3, 4)}
+               for k, (c, t) in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    line = {
+        "metric": "Mcell-steps/s at 256x128x128",
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 pressure, residuals, dots, integrals)",
+        "data": "synthetic (seeded dam-break tank initialised in place)",
+        "config": {"workload": f"dam break {nx}x{ny}x{nz}, h=1/{nx}, column {width:.4f} x {height:.4f} m (seed {args.seed})",
+                   "grid": [nx, ny, nz], "dt": dt, "poisson": f"FCG+FP64 refinement to 1e-8, {args.precond}",
+                   "parallelism": "single GPU"},
+        "poisson": {"ms_per_solve": round(pois_ms, 3), "iters_per_solve": round(pois_it, 2),
+                    "outer_per_solve": round(sum(p["outer"] for p in solves) / len(solves), 2),
+                    "fallbacks": sum(p["fallbacks"] for p in solves),
+                    "all_converged": all(p["converged"] for p in solves),
+                    "max_relres": max(p["res"] for p in solves), "precond": args.precond},
+        "diagnostics": {"volume_drift": (vol_now - vol_ini) / vol_ini, "max_div": maxdiv},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -9,7 +9,7 @@
 !
 ! CMD: ls_ini | level_set | heaviside | mod_ls1 | reinic | der_weno | weno1 ARG(ihs)
 !      visco | convdiff | tempo ARG(tt) | contorno ARG(nlock) | graddin | posdin
-!      stage ARG(tt) | step ARG(nsteps)
+!      stage ARG(tt) | step ARG(nsteps) | bench
 !
 ! State file: int32 nx,ny,nz; 10 float64 scalars (dt, t, vol_ini, vol_ins,
 ! rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1); then the arrays listed in
@@ -137,6 +137,32 @@ program harness
     write(*,'(A,I0,A,ES16.8,A,ES16.8,A,ES16.8)') 'REF_TIMING steps=', nsteps, &
       ' seconds=', real(c1-c0,8)/real(crate,8), ' graddin_seconds=', real(gsum,8)/real(crate,8), &
       ' level_set_seconds=', real(lssum,8)/real(crate,8)
+  case ('bench')
+    ! CPU-baseline sample: rebuild the level-set fields from ls
+    ! (14_level_set.f90:205,217), set the ghost and friction planes
+    ! (1_principal.f90:46), then time one level_set and one RK stage.
+    call heaviside()
+    call mod_ls1()
+    call contorno(1)
+    call system_clock(c0, crate)
+    call level_set()
+    call contorno(3)
+    call system_clock(ls1)
+    tt = 1
+    dt = a_dt(tt)
+    call visco()
+    call convdiff()
+    call tempo()
+    call contorno(2)
+    call system_clock(g0)
+    call graddin()
+    call system_clock(g1)
+    call posdin()
+    call contorno(1)
+    call system_clock(c1)
+    dt = dt_base
+    write(*,'(A,ES16.8,A,ES16.8,A,ES16.8)') 'REF_BENCH level_set_seconds=', real(ls1-c0,8)/real(crate,8), &
+      ' stage1_seconds=', real(c1-ls1,8)/real(crate,8), ' graddin_seconds=', real(g1-g0,8)/real(crate,8)
   case default
     write(*,*) 'unknown command ', trim(cmd)
     stop 2

@@ -549,15 +549,22 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
   }
   for (int l = 0; l < L - 1; l++) {
     MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
-    ProfScope ps(c, l == 0 ? "mg_smooth_restrict_l0" : "mg_smooth_restrict");
+    const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
     for (int s = 0; s < nu1; s++) {
-      hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 0,
-                         s == 0 ? 1 : 0, active);
-      if (s == 0)
+      {
+        ProfScope ps(c, gsn);
+        hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 0,
+                           s == 0 ? 1 : 0, active);
+      }
+      if (s == 0) {
+        ProfScope ps(c, "mg_zero");
         hipLaunchKernelGGL(k_mg_zero_color, half_cells(lf.g), b2, 0, c.st, lf.g, lf.x, 1, active);
+      }
+      ProfScope ps(c, gsn);
       hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 1,
                          0, active);
     }
+    ProfScope ps(c, l == 0 ? "mg_restrict_l0" : "mg_restrict");
     hipLaunchKernelGGL(k_mg_restrict, cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, lf.cx, lf.cy, lf.cz, lf.dg,
                        lf.b, lf.x, lc.b, lc.x, active);
   }
@@ -569,11 +576,16 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
   }
   for (int l = L - 2; l >= 0; l--) {
     MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
-    ProfScope ps(c, l == 0 ? "mg_prolong_smooth_l0" : "mg_prolong_smooth");
-    hipLaunchKernelGGL(k_mg_prolong, cells_full(lf.g), b2, 0, c.st, lf.g, lc.g, lc.x, lf.x, active);
+    const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
+    {
+      ProfScope ps(c, l == 0 ? "mg_prolong_l0" : "mg_prolong");
+      hipLaunchKernelGGL(k_mg_prolong, cells_full(lf.g), b2, 0, c.st, lf.g, lc.g, lc.x, lf.x, active);
+    }
     for (int s = 0; s < nu2; s++) {
+      ProfScope ps(c, gsn);
       hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 1,
                          0, active);
+      ProfScope ps2(c, gsn);
       hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 0,
                          0, active);
     }

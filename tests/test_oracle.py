@@ -237,3 +237,45 @@ def test_contorno_noslip_mirror():
     s.run("contorno", 2)
     assert np.all(s.u[1, :, :] == 0) and np.all(s.u[p.nx + 1, :, :] == 0)
     assert np.all(s.w[:, :, 1] == 0) and np.all(s.w[:, :, p.nz + 1] == 0)
+
+
+def test_cg_summation_order_sensitivity():
+    """Scale of the reference CG's round-off sensitivity, which sets the GPU
+    tolerance of whole steps: the same scaled CG (9_grad.f90:175-264) with
+    numpy's pairwise sums instead of sequential ones stops after the same
+    number of iterations on a cold-start dam-break solve, and its pressure
+    differs from the oracle's by less than 2e-6 relative (but far more than
+    the 1e-15 of a single operation)."""
+    import scipy.sparse as sp
+    from helpers import assemble_projection
+    p = O.Params(nx=16, ny=8, nz=8, dx=1 / 16, dy=1 / 16, dz=1 / 16)
+    st = O.State(p)
+    st.ls[...] = O.dambreak_phi(p, 0.25, 0.25)
+    st.scal["dt"] = 1e-3
+    st.run("ls_ini")
+    st.run("contorno", 1)
+    st.run("level_set")
+    st.scal["dt"] = 5e-4
+    for cmd, arg in (("visco", 0), ("convdiff", 0), ("tempo", 1), ("contorno", 2)):
+        st.run(cmd, arg)
+    a = st.copy().run("graddin")
+    A, D, q = assemble_projection(p, st.rho, st.u, st.v, st.w, 5e-4)
+    sD = np.sqrt(D)
+    As = sp.diags(1 / sD) @ A @ sp.diags(1 / sD)
+    e = st.prd1[1:-1, 1:-1, 1:-1].ravel(order="F") * sD
+    r = As @ e - q / sD
+    pp = r.copy()
+    am = float(np.sum(r * r))
+    thr, it = 1e-4 / len(D), 0
+    while abs(am) > thr and it < 10000:
+        it += 1
+        mp = As @ pp
+        am = float(np.sum(r * r))
+        al = am / float(np.sum(pp * mp))
+        e = e - al * pp
+        r = r - al * mp
+        pp = r + float(np.sum(r * r)) / am * pp
+    po = a.prd1[1:-1, 1:-1, 1:-1].ravel(order="F")
+    d = np.max(np.abs(e / sD - po)) / np.max(np.abs(po))
+    assert it == a.diag["cg_iters"]
+    assert 1e-12 < d < 2e-6, d
