@@ -113,12 +113,13 @@ def cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, budget_s):
     st.to_file(fin)
     del st
     t0 = time.time()
+    proc = subprocess.Popen(["taskset", "-c", "0", exe, fin, "-", "bench"], stdout=subprocess.PIPE,
+                            stderr=subprocess.DEVNULL, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
     try:
-        r = subprocess.run(["taskset", "-c", "0", exe, fin, "-", "bench"], capture_output=True, text=True,
-                           timeout=budget_s, env=dict(os.environ, OMP_NUM_THREADS="1"))
-        out = r.stdout
+        out, _ = proc.communicate(timeout=budget_s)
     except subprocess.TimeoutExpired:
-        out = ""
+        proc.kill()
+        out, _ = proc.communicate()
     finally:
         os.remove(fin)
         os.rmdir(tmp)
@@ -128,15 +129,28 @@ def cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, budget_s):
         if "=" in tok:
             k, v = tok.split("=")
             vals[k] = float(v)
+    info = {"unit": "Mcell-steps/s", "cores": 1, "kind": "reference", "cpu_model": _cpu_model(),
+            "host_nproc": os.cpu_count(), "wall_s": round(wall, 1)}
+    if "level_set_seconds" not in vals:
+        info.update(value=None, sample=f"reference level_set at {nx}x{ny}x{nz} did not finish within {budget_s:.0f}s")
+        return info
+    ls_s = vals["level_set_seconds"]
     if "stage1_seconds" not in vals:
-        return {"value": None, "unit": "Mcell-steps/s", "cores": 1, "kind": "reference",
-                "sample": f"level_set + RK stage 1 at {nx}x{ny}x{nz} did not finish within {budget_s}s"}
-    step_s = vals["level_set_seconds"] + 2 * vals["stage1_seconds"]
-    return {"value": nx * ny * nz / step_s / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": "reference",
-            "sample": (f"flang -O2 reference harness, 1 core: one level_set ({vals['level_set_seconds']:.1f}s) + one "
-                       f"RK stage ({vals['stage1_seconds']:.1f}s, of which graddin {vals['graddin_seconds']:.1f}s) at "
-                       f"{nx}x{ny}x{nz} from the GPU post-warmup state; step = level_set + 2 stages = {step_s:.1f}s"),
-            "cpu_model": _cpu_model(), "host_nproc": os.cpu_count(), "wall_s": round(wall, 1)}
+        # the stage (with its CG) ran past the budget: the step takes at least
+        # level_set + 2 x (budget left for the stage), so the value is an upper bound
+        stage_min = max(wall - ls_s - 5.0, 0.0)
+        step_min = ls_s + 2 * stage_min
+        info.update(value=nx * ny * nz / step_min / 1e6, bound="upper",
+                    sample=(f"flang -O2 reference harness, 1 core, {nx}x{ny}x{nz} from the GPU post-warmup state: "
+                            f"level_set {ls_s:.1f}s; RK stage 1 still running after {stage_min:.0f}s (stopped); "
+                            f"step >= level_set + 2 x {stage_min:.0f}s = {step_min:.0f}s, so the value is an upper bound"))
+        return info
+    step_s = ls_s + 2 * vals["stage1_seconds"]
+    info.update(value=nx * ny * nz / step_s / 1e6,
+                sample=(f"flang -O2 reference harness, 1 core: one level_set ({ls_s:.1f}s) + one RK stage "
+                        f"({vals['stage1_seconds']:.1f}s, of which graddin {vals['graddin_seconds']:.1f}s) at "
+                        f"{nx}x{ny}x{nz} from the GPU post-warmup state; step = level_set + 2 stages = {step_s:.1f}s"))
+    return info
 
 
 def _cpu_model():
@@ -158,7 +172,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precond", default="gmg", choices=["jacobi", "gmg", "learned"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-budget", type=float, default=150.0)
+    ap.add_argument("--cpu-budget", type=float, default=180.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the time step on one GPU; --gpus {args.gpus} is not supported "
@@ -197,7 +211,8 @@ def main():
     pois_ms = sum(p["ms"] for p in solves) / len(solves)
     pois_it = sum(p["iters"] for p in solves) / len(solves)
     # dominant kernel and its roofline
-    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    known = {k: v for k, v in prof.items() if k in BYTES_PER_CELL}
+    dom = max(known.items(), key=lambda kv: kv[1][1])
     dname, (dcount, dms) = dom
     bpc = BYTES_PER_CELL.get(dname)
     roof = None
@@ -206,7 +221,9 @@ def main():
         achieved = bpc * ncell / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dname,
-                "bytes_per_launch": bpc * ncell, "avg_launch_us": round(avg_s * 1e6, 2), "launches": dcount}
+                "bytes_per_launch": bpc * ncell, "avg_launch_us": round(avg_s * 1e6, 2), "launches": dcount,
+                "note": "largest fine-grid kernel by total time; coarse multigrid levels (mg_gs, mg_restrict, "
+                        "mg_prolong on levels >= 1) are latency-bound and reported under kernels"}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, args.cpu_budget)
