@@ -113,13 +113,19 @@ def cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, budget_s):
     st.to_file(fin)
     del st
     t0 = time.time()
+    def _big_stack():  # the reference keeps large automatic arrays on the stack
+        import resource
+        hard = resource.getrlimit(resource.RLIMIT_STACK)[1]
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
     proc = subprocess.Popen(["taskset", "-c", "0", exe, fin, "-", "bench"], stdout=subprocess.PIPE,
-                            stderr=subprocess.DEVNULL, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+                            stderr=subprocess.PIPE, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"),
+                            preexec_fn=_big_stack)
     try:
-        out, _ = proc.communicate(timeout=budget_s)
+        out, err = proc.communicate(timeout=budget_s)
     except subprocess.TimeoutExpired:
         proc.kill()
-        out, _ = proc.communicate()
+        out, err = proc.communicate()
     finally:
         os.remove(fin)
         os.rmdir(tmp)
@@ -132,7 +138,8 @@ def cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, budget_s):
     info = {"unit": "Mcell-steps/s", "cores": 1, "kind": "reference", "cpu_model": _cpu_model(),
             "host_nproc": os.cpu_count(), "wall_s": round(wall, 1)}
     if "level_set_seconds" not in vals:
-        info.update(value=None, sample=f"reference level_set at {nx}x{ny}x{nz} did not finish within {budget_s:.0f}s")
+        info.update(value=None, sample=(f"reference level_set at {nx}x{ny}x{nz} did not finish within {budget_s:.0f}s "
+                                        f"(rc {proc.returncode}: {(err or '').strip()[-200:]})"))
         return info
     ls_s = vals["level_set_seconds"]
     if "stage1_seconds" not in vals:

@@ -135,7 +135,6 @@ extern "C" int suli_init(const suli_params *p) {
   if (dalloc(*c, (void **)&c->ctl, sizeof(CGCtl))) return 1;
   const size_t maxblocks = (size_t)((g.nx + 3 + 63) / 64) * ((g.ny + 3 + 3) / 4) * (g.nz + 3) + 1024;
   if (dalloc(*c, (void **)&c->partials, maxblocks * 4 * sizeof(double))) return 1;
-  if (dalloc(*c, (void **)&c->counters, 64 * sizeof(unsigned int))) return 1;
   if (dalloc(*c, (void **)&c->res, RS_NSLOT * sizeof(double))) return 1;
   HIPCHK(hipHostMalloc((void **)&c->ctl_h, sizeof(CGCtl), hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void **)&c->res_h, RS_NSLOT * sizeof(double), hipHostMallocDefault));

@@ -100,52 +100,63 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double *lds /* NV*
   }
 }
 
-// Reduction slots: partials[NV*maxblocks], a completion counter, results[NV].
+// Reduction slots: per-block partials (NV values per block, value-major) and
+// the NV results.  A producer kernel writes its block partials with
+// block_partial (no fence, no atomic); a one-block finish kernel launched
+// right after it on the same stream adds the partials in block order, so the
+// sum is the same on every run.  (A last-block-done scheme needs a
+// device-scope fence per block, which on this multi-die part writes the L2
+// back and costs far more than the extra launch.)
 struct RedSlot {
   double *partials;
-  unsigned int *counter;
   double *result;
 };
 
-// Called by every block after block_reduce; thread 0 of the last block writes
-// result[q] = sum/max over blocks of the block values, in block order.
-// Returns true in every thread of the last block.
-template <int NV, bool MAX = false>
-__device__ __forceinline__ bool grid_reduce_finish(const double (&v)[NV], RedSlot rs) {
-  __shared__ bool last;
-  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-  const unsigned int nb = gridDim.x * gridDim.y * gridDim.z;
-  const unsigned int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  if (tid == 0) {
+template <int NV>
+__device__ __forceinline__ void block_partial(const double (&v)[NV], double *partials) {
+  if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) {
+    const unsigned int nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 #pragma unroll
-    for (int q = 0; q < NV; q++) rs.partials[(size_t)q * nb + b] = v[q];
-    __threadfence();
-    unsigned int prev = atomicAdd(rs.counter, 1u);
-    last = (prev == nb - 1);
+    for (int q = 0; q < NV; q++) partials[(size_t)q * nb + b] = v[q];
   }
-  __syncthreads();
-  if (last) {
-    __threadfence();
-    const int nt = blockDim.x * blockDim.y * blockDim.z;
-    __shared__ double lds[NV * SULI_LDS_RED];
-    double acc[NV];
+}
+
+// Epilogue of a plain reduction: nothing beyond storing the result.
+struct EpiNone {
+  __device__ bool enabled() const { return true; }
+  __device__ void operator()(const double *) const {}
+};
+
+// Sum (or max) of the nb block partials of NV values, in block order; thread 0
+// stores result[0..NV) and runs the epilogue (loop control of the solvers).
+template <int NV, bool MAX, typename Epi>
+__global__ __launch_bounds__(256) void k_finish(const double *__restrict__ partials, int nb,
+                                                double *__restrict__ result, Epi epi) {
+  if (!epi.enabled()) return;
+  __shared__ double lds[NV * SULI_LDS_RED];
+  double acc[NV];
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-      double a = MAX ? -1.0e300 : 0.0;
-      for (unsigned int t = tid; t < nb; t += nt) {
-        double x = rs.partials[(size_t)q * nb + t];
-        a = MAX ? (x > a ? x : a) : a + x;
-      }
-      acc[q] = a;
+  for (int q = 0; q < NV; q++) {
+    double a = MAX ? -1.0e300 : 0.0;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+      const double x = partials[(size_t)q * nb + t];
+      a = MAX ? (x > a ? x : a) : a + x;
     }
-    block_reduce<NV, MAX>(acc, lds);
-    if (tid == 0) {
-#pragma unroll
-      for (int q = 0; q < NV; q++) rs.result[q] = acc[q];
-      *rs.counter = 0u;
-    }
+    acc[q] = a;
   }
-  return last;
+  block_reduce<NV, MAX>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) result[q] = acc[q];
+    epi(result);
+  }
+}
+
+template <int NV, bool MAX = false, typename Epi = EpiNone>
+inline void launch_finish(hipStream_t s, dim3 producer_grid, RedSlot rs, Epi epi = Epi()) {
+  const int nb = (int)(producer_grid.x * producer_grid.y * producer_grid.z);
+  hipLaunchKernelGGL((k_finish<NV, MAX, Epi>), dim3(1), dim3(256), 0, s, rs.partials, nb, rs.result, epi);
 }
 
 // ---------------------------------------------------------------------------

@@ -48,7 +48,6 @@ struct Ctx {
   CGCtl *ctl = nullptr, *ctl_h = nullptr;
   // reductions
   double *partials = nullptr;
-  unsigned int *counters = nullptr;
   double *res = nullptr, *res_h = nullptr;
   bool convdiff_pending = false;
   double vol_ini = 0, vol_ins = 0;
@@ -56,7 +55,7 @@ struct Ctx {
   std::vector<void *> allocs;
   Prof prof;
 
-  RedSlot slot(int off, int ctr) const { return RedSlot{partials, counters + ctr, res + off}; }
+  RedSlot slot(int off, int ctr) const { (void)ctr; return RedSlot{partials, res + off}; }
 };
 
 extern Ctx *g_ctx;

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_bnorm(Grid g, const double *__restrict_
     }
   __shared__ double lds[SULI_LDS_RED];
   block_reduce<1>(acc, lds);
-  grid_reduce_finish<1>(acc, rs);
+  block_partial<1>(acc, rs.partials);
 }
 
 // r = b - A_s x in FP64; sums r.r, r.n, n.n
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void k_res64(Grid g, const double *__restrict_
     }
   __shared__ double lds[3 * SULI_LDS_RED];
   block_reduce<3>(acc, lds);
-  grid_reduce_finish<3>(acc, rs);
+  block_partial<3>(acc, rs.partials);
 }
 
 // r32 = r - (r.n/n.n) n; d = 0; sums r32.r32 and arms the inner loop
@@ -129,14 +129,43 @@ __global__ __launch_bounds__(256) void k_r32(Grid g, const double *__restrict__ 
     }
   __shared__ double lds[SULI_LDS_RED];
   block_reduce<1>(acc, lds);
-  if (grid_reduce_finish<1>(acc, rs) && threadIdx.x == 0 && threadIdx.y == 0) {
-    ctl->thr_in = ctl->rin0[0] * rel_in * rel_in;
+  block_partial<1>(acc, rs.partials);
+}
+
+// loop-control epilogues of the reduction finishes (run by one thread)
+struct EpiR32 {  // arm the inner loop: stop when r.r <= rel_in^2 r0.r0
+  FcgCtl *ctl;
+  double rel_in;
+  int maxit;
+  __device__ bool enabled() const { return true; }
+  __device__ void operator()(const double *res) const {
+    ctl->thr_in = res[0] * rel_in * rel_in;
     ctl->maxit_in = maxit;
     ctl->it = 0;
     ctl->first = 1;
-    ctl->active = ctl->rin0[0] > 0 ? 1 : 0;
+    ctl->active = res[0] > 0 ? 1 : 0;
   }
-}
+};
+struct EpiActive {  // plain result, only while the inner loop runs
+  FcgCtl *ctl;
+  __device__ bool enabled() const { return ctl->active != 0; }
+  __device__ void operator()(const double *) const {}
+};
+struct EpiK1 {
+  FcgCtl *ctl;
+  __device__ bool enabled() const { return ctl->active != 0; }
+  __device__ void operator()(const double *) const { ctl->first = 0; }
+};
+struct EpiK2 {  // count the iteration and decide whether the next one runs
+  FcgCtl *ctl;
+  __device__ bool enabled() const { return ctl->active != 0; }
+  __device__ void operator()(const double *res) const {
+    ctl->it += 1;
+    const double pq = ctl->k1[0];
+    const bool stop = !(res[0] > ctl->thr_in) || ctl->it >= ctl->maxit_in || !(pq > 0.0) || !(res[0] == res[0]);
+    ctl->active = stop ? 0 : 1;
+  }
+};
 
 // x += d (FP64 += FP32)
 __global__ __launch_bounds__(256) void k_xupd(Grid g, double *__restrict__ x, const float *__restrict__ d) {
@@ -185,7 +214,7 @@ __global__ __launch_bounds__(256) void k_fcg1(Grid g, const float *__restrict__ 
     }
   __shared__ double lds[2 * SULI_LDS_RED];
   block_reduce<2>(acc, lds);
-  if (grid_reduce_finish<2>(acc, rs) && threadIdx.x == 0 && threadIdx.y == 0) ctl->first = 0;
+  block_partial<2>(acc, rs.partials);
 }
 
 // K2: alpha = p.r/p.q; d += alpha p; r -= alpha q; sums r.r, r.q; loop control
@@ -209,12 +238,7 @@ __global__ __launch_bounds__(256) void k_fcg2(Grid g, const float *__restrict__ 
     }
   __shared__ double lds[2 * SULI_LDS_RED];
   block_reduce<2>(acc, lds);
-  if (grid_reduce_finish<2>(acc, rs) && threadIdx.x == 0 && threadIdx.y == 0) {
-    ctl->it += 1;
-    const bool stop = !(ctl->k2[0] > ctl->thr_in) || ctl->it >= ctl->maxit_in || !(pq > 0.0) ||
-                      !(ctl->k2[0] == ctl->k2[0]);
-    ctl->active = stop ? 0 : 1;
-  }
+  block_partial<2>(acc, rs.partials);
 }
 
 // z.q_prev for the preconditioned beta
@@ -230,7 +254,7 @@ __global__ __launch_bounds__(256) void k_zq(Grid g, const float *__restrict__ z,
     }
   __shared__ double lds[SULI_LDS_RED];
   block_reduce<1>(acc, lds);
-  grid_reduce_finish<1>(acc, rs);
+  block_partial<1>(acc, rs.partials);
 }
 
 // ---- geometric multigrid on the unscaled operator ------------------------------
@@ -306,8 +330,11 @@ __device__ __forceinline__ float mg_Ax(const Grid &g, int64_t c, const float *__
          cz[c + g.sz] * x[c + g.sz] + cz[c] * x[c - g.sz];
 }
 
-// red-black Gauss-Seidel half sweep on the cells of one colour; when zero_x
-// is set the other colour is taken as zero (first sweep from x = 0).
+// red-black Gauss-Seidel half sweep on the cells of one colour.  The 7-point
+// stencil couples a cell only to cells of the other colour, so a half sweep
+// reads only the other colour; with zero_x set that colour is taken as zero
+// (the first sweep from x = 0), and its stale values are overwritten by the
+// next half sweep before anything reads them.
 __global__ __launch_bounds__(256) void k_mg_gs(Grid g, const float *__restrict__ cx, const float *__restrict__ cy,
                                                const float *__restrict__ cz, const float *__restrict__ dg,
                                                const float *__restrict__ b, float *__restrict__ x, int color,
@@ -321,18 +348,6 @@ __global__ __launch_bounds__(256) void k_mg_gs(Grid g, const float *__restrict__
   const int64_t c = g.idx(i, j, k);
   const float s = zero_x ? 0.f : mg_Ax(g, c, cx, cy, cz, x);
   x[c] = (b[c] + s) / dg[c];
-}
-
-// zero the other colour after a zero_x first half sweep
-__global__ __launch_bounds__(256) void k_mg_zero_color(Grid g, float *__restrict__ x, int color,
-                                                       const int *active) {
-  if (!*active) return;
-  const int ti = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
-            k = 1 + blockIdx.z;
-  if (j > g.ny || k > g.nz) return;
-  const int i = 1 + 2 * ti + ((1 + j + k + color) & 1);
-  if (i > g.nx) return;
-  x[g.idx(i, j, k)] = 0.f;
 }
 
 // coarse rhs = sum over the 8 children of the fine residual b - A x; coarse x = 0
@@ -366,28 +381,78 @@ __global__ __launch_bounds__(256) void k_mg_prolong(Grid gf, Grid gc, const floa
   x[gf.idx(i, j, k)] += xc[gc.idx((i + 1) >> 1, (j + 1) >> 1, (k + 1) >> 1)];
 }
 
-// coarsest level: many red-black sweeps in one work-group
-__global__ __launch_bounds__(1024) void k_mg_coarse(Grid g, const float *__restrict__ cx,
-                                                    const float *__restrict__ cy, const float *__restrict__ cz,
-                                                    const float *__restrict__ dg, const float *__restrict__ b,
-                                                    float *__restrict__ x, int sweeps, const int *active) {
-  if (!*active) return;
+// V-cycle over the small levels s..L-1 (each at most SULI_MG_SMALL cells) in
+// one work-group, levels separated by barriers: the coarse end of the
+// hierarchy costs one launch instead of a dozen latency-bound ones.
+#define SULI_MG_MAXLV 12
+#define SULI_MG_SMALL 8192
+struct MgSmall {
+  Grid g[SULI_MG_MAXLV];
+  float *cx[SULI_MG_MAXLV], *cy[SULI_MG_MAXLV], *cz[SULI_MG_MAXLV], *dg[SULI_MG_MAXLV];
+  float *b[SULI_MG_MAXLV], *x[SULI_MG_MAXLV];
+  int s, L, nu, coarse_sweeps;
+};
+
+__device__ __forceinline__ void cell_of(const Grid &g, int t, int &i, int &j, int &k) {
+  i = 1 + t % g.nx;
+  j = 1 + (t / g.nx) % g.ny;
+  k = 1 + t / (g.nx * g.ny);
+}
+
+__device__ void small_gs(const MgSmall &m, int l, int color, bool zero_x) {
+  const Grid &g = m.g[l];
   const int n = g.nx * g.ny * g.nz;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
-    const int i = 1 + t % g.nx, j = 1 + (t / g.nx) % g.ny, k = 1 + t / (g.nx * g.ny);
-    x[g.idx(i, j, k)] = 0.f;
+    int i, j, k;
+    cell_of(g, t, i, j, k);
+    if (((i + j + k) & 1) != color) continue;
+    const int64_t c = g.idx(i, j, k);
+    const float s = zero_x ? 0.f : mg_Ax(g, c, m.cx[l], m.cy[l], m.cz[l], m.x[l]);
+    m.x[l][c] = (m.b[l][c] + s) / m.dg[l][c];
   }
   __syncthreads();
-  for (int s = 0; s < sweeps; s++)
-    for (int color = 0; color < 2; color++) {
-      for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const int i = 1 + t % g.nx, j = 1 + (t / g.nx) % g.ny, k = 1 + t / (g.nx * g.ny);
-        if (((i + j + k) & 1) != color) continue;
-        const int64_t c = g.idx(i, j, k);
-        x[c] = (b[c] + mg_Ax(g, c, cx, cy, cz, x)) / dg[c];
-      }
-      __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_mg_small(MgSmall m, const int *active) {
+  if (!*active) return;
+  for (int l = m.s; l < m.L - 1; l++) {
+    for (int sw = 0; sw < m.nu; sw++) {
+      small_gs(m, l, 0, sw == 0);
+      small_gs(m, l, 1, false);
     }
+    const Grid &gf = m.g[l], &gc = m.g[l + 1];
+    const int nc = gc.nx * gc.ny * gc.nz;
+    for (int t = threadIdx.x; t < nc; t += blockDim.x) {
+      int I, J, K;
+      cell_of(gc, t, I, J, K);
+      float s = 0.f;
+      for (int d = 0; d < 8; d++) {
+        const int64_t c = gf.idx(2 * I - 1 + (d & 1), 2 * J - 1 + ((d >> 1) & 1), 2 * K - 1 + (d >> 2));
+        s += m.b[l][c] - (m.dg[l][c] * m.x[l][c] - mg_Ax(gf, c, m.cx[l], m.cy[l], m.cz[l], m.x[l]));
+      }
+      m.b[l + 1][gc.idx(I, J, K)] = s;
+    }
+    __syncthreads();
+  }
+  const int lc = m.L - 1;
+  for (int sw = 0; sw < m.coarse_sweeps; sw++) {
+    small_gs(m, lc, 0, sw == 0);
+    small_gs(m, lc, 1, false);
+  }
+  for (int l = m.L - 2; l >= m.s; l--) {
+    const Grid &gf = m.g[l], &gc = m.g[l + 1];
+    const int nf = gf.nx * gf.ny * gf.nz;
+    for (int t = threadIdx.x; t < nf; t += blockDim.x) {
+      int i, j, k;
+      cell_of(gf, t, i, j, k);
+      m.x[l][gf.idx(i, j, k)] += m.x[l + 1][gc.idx((i + 1) >> 1, (j + 1) >> 1, (k + 1) >> 1)];
+    }
+    __syncthreads();
+    for (int sw = 0; sw < m.nu; sw++) {
+      small_gs(m, l, 1, false);
+      small_gs(m, l, 0, false);
+    }
+  }
 }
 
 // level-0 input b0 = sqrt(D) r and output z = sqrt(D) x0
@@ -478,7 +543,7 @@ int fcg_create(Ctx &c) {
   for (;;) {
     const Grid &gf = f->lv.back().g;
     if (gf.nx % 2 || gf.ny % 2 || gf.nz % 2 || gf.nx < 4 || gf.ny < 4 || gf.nz < 4) break;
-    if ((int64_t)gf.nx * gf.ny * gf.nz <= 512) break;
+    if ((int64_t)gf.nx * gf.ny * gf.nz <= 512 || f->lv.size() >= SULI_MG_MAXLV) break;
     MgLevel l;
     Grid gc = gf;
     gc.nx = gf.nx / 2; gc.ny = gf.ny / 2; gc.nz = gf.nz / 2;
@@ -537,17 +602,20 @@ static int mg_setup(Ctx &c) {
   return 0;
 }
 
-// One multigrid V-cycle: z = sqrt(D) V(sqrt(D) r).
+// One multigrid V-cycle: z = sqrt(D) V(sqrt(D) r).  Levels with more than
+// SULI_MG_SMALL cells run one launch per pass; the rest run in k_mg_small.
 static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
   PoissonFCG &f = *c.fcg;
   const dim3 b2(SULI_BX, SULI_BY, 1);
-  const int nu1 = 2, nu2 = 2, coarse_sweeps = 40;
+  const int nu1 = 2, nu2 = 2;
   const int L = (int)f.lv.size();
+  int S = 0;
+  while (S < L - 1 && (int64_t)f.lv[S].g.nx * f.lv[S].g.ny * f.lv[S].g.nz > SULI_MG_SMALL) S++;
   {
     ProfScope ps(c, "mg_scale");
     hipLaunchKernelGGL(k_scale, cells_kloop(c.g, f.kz), b2, 0, c.st, c.g, f.sD32, r, f.lv[0].b, active);
   }
-  for (int l = 0; l < L - 1; l++) {
+  for (int l = 0; l < S; l++) {
     MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
     for (int s = 0; s < nu1; s++) {
@@ -555,10 +623,6 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
         ProfScope ps(c, gsn);
         hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 0,
                            s == 0 ? 1 : 0, active);
-      }
-      if (s == 0) {
-        ProfScope ps(c, "mg_zero");
-        hipLaunchKernelGGL(k_mg_zero_color, half_cells(lf.g), b2, 0, c.st, lf.g, lf.x, 1, active);
       }
       ProfScope ps(c, gsn);
       hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 1,
@@ -569,12 +633,22 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
                        lf.b, lf.x, lc.b, lc.x, active);
   }
   {
-    MgLevel &lc = f.lv[L - 1];
-    ProfScope ps(c, "mg_coarse");
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, c.st, lc.g, lc.cx, lc.cy, lc.cz, lc.dg, lc.b, lc.x,
-                       coarse_sweeps, active);
+    MgSmall m;
+    std::memset(&m, 0, sizeof(m));
+    for (int l = S; l < L && l - S < SULI_MG_MAXLV; l++) {
+      const int q = l;
+      m.g[q] = f.lv[l].g;
+      m.cx[q] = f.lv[l].cx; m.cy[q] = f.lv[l].cy; m.cz[q] = f.lv[l].cz; m.dg[q] = f.lv[l].dg;
+      m.b[q] = f.lv[l].b; m.x[q] = f.lv[l].x;
+    }
+    m.s = S;
+    m.L = L;
+    m.nu = nu1;
+    m.coarse_sweeps = 24;
+    ProfScope ps(c, "mg_small_levels");
+    hipLaunchKernelGGL(k_mg_small, dim3(1), dim3(1024), 0, c.st, m, active);
   }
-  for (int l = L - 2; l >= 0; l--) {
+  for (int l = S - 1; l >= 0; l--) {
     MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
     {
@@ -620,8 +694,9 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     DISPATCH(c, Launch<TT>::pD(g, c.st, dt, c.u, c.v, c.w, c.rho, c.D, c.sD, c.q));
     DISPATCH(c, Launch<TT>::pcoef(g, c.st, dt, c.rho, c.D, c.ax, c.ay, c.az));
     hipLaunchKernelGGL(k_xinit, gk, b2, 0, c.st, g, c.prd1, c.sD, f.x);
-    hipLaunchKernelGGL(k_bnorm, gk, b2, 0, c.st, g, c.q, c.sD, f.b,
-                       RedSlot{c.partials, c.counters + 10, CTL_PTR(f.ctl, bb)});
+    const RedSlot rb{c.partials, CTL_PTR(f.ctl, bb)};
+    hipLaunchKernelGGL(k_bnorm, gk, b2, 0, c.st, g, c.q, c.sD, f.b, rb);
+    launch_finish<1>(c.st, gk, rb);
   }
   if (mg_setup(c)) return 1;
   if (use_pc && pc == SULI_PRECOND_LEARNED && f.unet) unet_prepare(c, f.unet);
@@ -635,8 +710,9 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   for (;;) {
     {
       ProfScope ps(c, "fcg_residual64");
-      hipLaunchKernelGGL(k_res64, gk, b2, 0, c.st, g, c.ax, c.ay, c.az, f.x, f.b, c.sD, f.r,
-                         RedSlot{c.partials, c.counters + 11, CTL_PTR(f.ctl, rout)});
+      const RedSlot rr{c.partials, CTL_PTR(f.ctl, rout)};
+      hipLaunchKernelGGL(k_res64, gk, b2, 0, c.st, g, c.ax, c.ay, c.az, f.x, f.b, c.sD, f.r, rr);
+      launch_finish<3>(c.st, gk, rr);
     }
     HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
     HIPCHK(hipStreamSynchronize(c.st));
@@ -654,8 +730,9 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     const double rin = std::max(rel_in, std::min(want, 1.0));
     {
       ProfScope ps(c, "fcg_project");
-      hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rin, maxit_in - inner_total,
-                         RedSlot{c.partials, c.counters + 12, CTL_PTR(f.ctl, rin0)});
+      const RedSlot r3{c.partials, CTL_PTR(f.ctl, rin0)};
+      hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rin, maxit_in - inner_total, r3);
+      launch_finish<1>(c.st, gk, r3, EpiR32{f.ctl, rin, maxit_in - inner_total});
     }
     int chunk = use_pc ? 4 : 16;
     for (;;) {
@@ -664,20 +741,24 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
         if (use_pc) {
           apply_precond(c, pc, f.r32, f.z32, active);
           ProfScope ps(c, "fcg_zq");
-          hipLaunchKernelGGL(k_zq, gk, b2, 0, c.st, g, f.z32, f.q32, f.ctl,
-                             RedSlot{c.partials, c.counters + 13, CTL_PTR(f.ctl, zq)});
+          const RedSlot rz{c.partials, CTL_PTR(f.ctl, zq)};
+          hipLaunchKernelGGL(k_zq, gk, b2, 0, c.st, g, f.z32, f.q32, f.ctl, rz);
+          launch_finish<1>(c.st, gk, rz, EpiActive{f.ctl});
           zin = f.z32;
         }
         {
           ProfScope ps(c, "fcg_spmv_dot");
+          const RedSlot r1{c.partials, CTL_PTR(f.ctl, k1)};
           hipLaunchKernelGGL(k_fcg1, gk, b2, 0, c.st, g, f.ax32, f.ay32, f.az32, zin, f.r32, f.p32, f.p32b, f.q32,
-                             f.ctl, use_pc ? 1 : 0, RedSlot{c.partials, c.counters + 14, CTL_PTR(f.ctl, k1)});
+                             f.ctl, use_pc ? 1 : 0, r1);
+          launch_finish<2>(c.st, gk, r1, EpiK1{f.ctl});
         }
         std::swap(f.p32, f.p32b);
         {
           ProfScope ps(c, "fcg_update");
-          hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.p32, f.q32, f.r32, f.d32, f.ctl,
-                             RedSlot{c.partials, c.counters + 15, CTL_PTR(f.ctl, k2)});
+          const RedSlot r2{c.partials, CTL_PTR(f.ctl, k2)};
+          hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.p32, f.q32, f.r32, f.d32, f.ctl, r2);
+          launch_finish<2>(c.st, gk, r2, EpiK2{f.ctl});
         }
       }
       HIPCHK(hipGetLastError());
@@ -693,8 +774,9 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     }
     if (inner_total >= maxit_in) {
       // iteration budget spent: report the residual after this pass
-      hipLaunchKernelGGL(k_res64, gk, b2, 0, c.st, g, c.ax, c.ay, c.az, f.x, f.b, c.sD, f.r,
-                         RedSlot{c.partials, c.counters + 11, CTL_PTR(f.ctl, rout)});
+      const RedSlot rr{c.partials, CTL_PTR(f.ctl, rout)};
+      hipLaunchKernelGGL(k_res64, gk, b2, 0, c.st, g, c.ax, c.ay, c.az, f.x, f.b, c.sD, f.r, rr);
+      launch_finish<3>(c.st, gk, rr);
       HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
       HIPCHK(hipStreamSynchronize(c.st));
       res = bnorm > 0 ? std::sqrt(f.ctl_h->rout[0]) / bnorm : 0.0;

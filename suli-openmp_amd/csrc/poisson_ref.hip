@@ -43,16 +43,33 @@ __global__ __launch_bounds__(256) void k_pr0(Grid g, const double *__restrict__ 
   }
   __shared__ double lds[SULI_LDS_RED];
   block_reduce<1>(acc, lds);
-  if (grid_reduce_finish<1>(acc, rs)) {
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      const double a = ctl->r0[0];
-      ctl->alfa[0] = a;
-      ctl->iters = 0;
-      ctl->active = (fabs(a) > ctl->thr && 0 < ctl->maxit) ? 1 : 0;
-      ctl->next = ctl->active;
-    }
-  }
+  block_partial<1>(acc, rs.partials);
 }
+
+// loop-control epilogues of the reduction finishes (run by one thread)
+struct EpiCgR0 {  // after the initial residual: start the loop if sum r^2 > thr
+  CGCtl *ctl;
+  __device__ bool enabled() const { return true; }
+  __device__ void operator()(const double *res) const {
+    ctl->alfa[0] = res[0];
+    ctl->iters = 0;
+    ctl->active = (fabs(res[0]) > ctl->thr && 0 < ctl->maxit) ? 1 : 0;
+    ctl->next = ctl->active;
+  }
+};
+struct EpiCgAlfa {  // after r.r and pp.mp: count the iteration, decide the next one (:175)
+  CGCtl *ctl;
+  __device__ bool enabled() const { return ctl->active != 0; }
+  __device__ void operator()(const double *) const {
+    ctl->iters += 1;
+    ctl->next = (fabs(ctl->alfa[0]) > ctl->thr && ctl->iters < ctl->maxit) ? 1 : 0;
+  }
+};
+struct EpiCgBeta {  // after the new r.r: the direction update of a final iteration is not needed
+  CGCtl *ctl;
+  __device__ bool enabled() const { return ctl->active != 0; }
+  __device__ void operator()(const double *) const { ctl->active = ctl->next; }
+};
 
 // neighbour address with the mirrored-ghost convention
 __device__ __forceinline__ int64_t nb_addr(const Grid &g, int64_t c, int q, int n, int64_t st,
@@ -86,12 +103,7 @@ __global__ __launch_bounds__(256) void k_cg1(Grid g, const double *__restrict__ 
   }
   __shared__ double lds[2 * SULI_LDS_RED];
   block_reduce<2>(acc, lds);
-  if (grid_reduce_finish<2>(acc, rs)) {
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      ctl->iters += 1;
-      ctl->next = (fabs(ctl->alfa[0]) > ctl->thr && ctl->iters < ctl->maxit) ? 1 : 0;
-    }
-  }
+  block_partial<2>(acc, rs.partials);
 }
 
 // e -= alpha pp; r -= alpha mp; sum r.r (:205-219).
@@ -113,13 +125,12 @@ __global__ __launch_bounds__(256) void k_cg2(Grid g, double *__restrict__ e, dou
   }
   __shared__ double lds[SULI_LDS_RED];
   block_reduce<1>(acc, lds);
-  grid_reduce_finish<1>(acc, rs);
+  block_partial<1>(acc, rs.partials);
 }
 
-// pp = r + beta pp (:221-230); hands the loop condition to the next iteration.
-__global__ __launch_bounds__(256) void k_cg3(Grid g, const double *__restrict__ r,
-                                             double *__restrict__ pp, CGCtl *ctl,
-                                             unsigned int *__restrict__ done_counter) {
+// pp = r + beta pp (:221-230).
+__global__ __launch_bounds__(256) void k_cg3(Grid g, const double *__restrict__ r, double *__restrict__ pp,
+                                             const CGCtl *ctl) {
 #pragma clang fp contract(off)
   if (!ctl->active) return;
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
@@ -128,18 +139,6 @@ __global__ __launch_bounds__(256) void k_cg3(Grid g, const double *__restrict__ 
   if (i <= g.nx && j <= g.ny) {
     const int64_t c = g.idx(i, j, k);
     pp[c] = r[c] + betapr * pp[c];
-  }
-  // the last block to finish switches the loop flag for the next iteration
-  __shared__ bool last;
-  __syncthreads();
-  if (threadIdx.x == 0 && threadIdx.y == 0) {
-    __threadfence();
-    const unsigned int nb = gridDim.x * gridDim.y * gridDim.z;
-    last = atomicAdd(done_counter, 1u) == nb - 1;
-    if (last) {
-      *done_counter = 0u;
-      ctl->active = ctl->next;
-    }
   }
 }
 
@@ -170,16 +169,19 @@ int poisson_ref_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   c.ctl_h->maxit = c.P.max_it;
   HIPCHK(hipMemcpyAsync(c.ctl, c.ctl_h, sizeof(CGCtl), hipMemcpyHostToDevice, c.st));
   const dim3 cg = cell_grid(g);
-  RedSlot r0{c.partials, c.counters + 4, (double *)((char *)c.ctl + offsetof(CGCtl, r0))};
-  { ProfScope ps_(c, "cg_residual0"); hipLaunchKernelGGL(k_pr0, cg, b, 0, c.st, g, c.e, c.ax, c.ay, c.az, c.q, c.sD, c.r, c.pp, c.ctl, r0); }
-  RedSlot r1{c.partials, c.counters + 5, (double *)((char *)c.ctl + offsetof(CGCtl, alfa))};
-  RedSlot r2{c.partials, c.counters + 6, (double *)((char *)c.ctl + offsetof(CGCtl, beta))};
+  RedSlot r0{c.partials, (double *)((char *)c.ctl + offsetof(CGCtl, r0))};
+  { ProfScope ps_(c, "cg_residual0"); hipLaunchKernelGGL(k_pr0, cg, b, 0, c.st, g, c.e, c.ax, c.ay, c.az, c.q, c.sD, c.r, c.pp, c.ctl, r0);
+    launch_finish<1>(c.st, cg, r0, EpiCgR0{c.ctl}); }
+  RedSlot r1{c.partials, (double *)((char *)c.ctl + offsetof(CGCtl, alfa))};
+  RedSlot r2{c.partials, (double *)((char *)c.ctl + offsetof(CGCtl, beta))};
   int chunk = 8;
   for (;;) {
     for (int n = 0; n < chunk; n++) {
-      { ProfScope ps_(c, "cg_spmv_dot"); hipLaunchKernelGGL(k_cg1, cg, b, 0, c.st, g, c.ax, c.ay, c.az, c.r, c.pp, c.mp, c.ctl, r1); }
-      { ProfScope ps_(c, "cg_update"); hipLaunchKernelGGL(k_cg2, cg, b, 0, c.st, g, c.e, c.r, c.pp, c.mp, c.ctl, r2); }
-      { ProfScope ps_(c, "cg_direction"); hipLaunchKernelGGL(k_cg3, cg, b, 0, c.st, g, c.r, c.pp, c.ctl, c.counters + 7); }
+      { ProfScope ps_(c, "cg_spmv_dot"); hipLaunchKernelGGL(k_cg1, cg, b, 0, c.st, g, c.ax, c.ay, c.az, c.r, c.pp, c.mp, c.ctl, r1);
+        launch_finish<2>(c.st, cg, r1, EpiCgAlfa{c.ctl}); }
+      { ProfScope ps_(c, "cg_update"); hipLaunchKernelGGL(k_cg2, cg, b, 0, c.st, g, c.e, c.r, c.pp, c.mp, c.ctl, r2);
+        launch_finish<1>(c.st, cg, r2, EpiCgBeta{c.ctl}); }
+      { ProfScope ps_(c, "cg_direction"); hipLaunchKernelGGL(k_cg3, cg, b, 0, c.st, g, c.r, c.pp, c.ctl); }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c.ctl_h, c.ctl, sizeof(CGCtl), hipMemcpyDeviceToHost, c.st));

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_reinit_stage(Grid g, const T *__restric
   if (band) {
     __shared__ double lds[2 * SULI_LDS_RED];
     block_reduce<2>(acc, lds);
-    grid_reduce_finish<2>(acc, rs);
+    block_partial<2>(acc, rs.partials);
   }
 }
 
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_heavi(Grid g, Phys ph, const T *__restr
   }
   __shared__ double lds[SULI_LDS_RED];
   block_reduce<1>(acc, lds);
-  grid_reduce_finish<1>(acc, rs);
+  block_partial<1>(acc, rs.partials);
 }
 
 // Second half of mod_ls1 (:761-803): WENO (ihs = 1 everywhere) of each normal
@@ -632,5 +632,5 @@ __global__ __launch_bounds__(256) void k_divmax(Grid g, const T *__restrict__ u,
   }
   __shared__ double lds[SULI_LDS_RED];
   block_reduce<1, true>(acc, lds);
-  grid_reduce_finish<1, true>(acc, rs);
+  block_partial<1>(acc, rs.partials);
 }

@@ -27,6 +27,7 @@ void Launch<T>::reinit_stage(const Grid &g, const Phys &ph, hipStream_t s, const
   const double bandw = 2. * ph.alpha1 * ph.dx1;
   SULI_LAUNCH(k_reinit_stage<T>, cell_grid(g), blk2(), s, g, (const T *)phi, (const T *)phit,
               (const T *)S, (T *)out, (T)ca, (T)cb, (T)cgdt, band, bandw, rs);
+  if (band) launch_finish<2>(s, cell_grid(g), rs);
 }
 
 template <typename T>
@@ -35,6 +36,7 @@ void Launch<T>::heavi(const Grid &g, const Phys &ph, hipStream_t s, const void *
                       void *nrz, RedSlot rs) {
   SULI_LAUNCH(k_heavi<T>, cell_grid(g), blk2(), s, g, ph, (const T *)phi, (T *)hs, (T *)hsx, (T *)hsy,
               (T *)hsz, (T *)rho, (T *)lsnu, (T *)modls, (T *)nrx, (T *)nry, (T *)nrz, rs);
+  launch_finish<1>(s, cell_grid(g), rs);
 }
 
 template <typename T>
@@ -102,6 +104,7 @@ template <typename T>
 void Launch<T>::divmax(const Grid &g, hipStream_t s, const void *u, const void *v, const void *w,
                        RedSlot rs) {
   SULI_LAUNCH(k_divmax<T>, cell_grid(g), blk2(), s, g, (const T *)u, (const T *)v, (const T *)w, rs);
+  launch_finish<1, true>(s, cell_grid(g), rs);
 }
 
 // Dam-break level set: signed distance (positive in water) to the column

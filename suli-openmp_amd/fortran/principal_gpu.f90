@@ -1,0 +1,112 @@
+! SuLi-IC driver on the MI355X library: the time loop of the reference
+! PROGRAM PNH (1_principal.f90:64-108) with its CALL lines unchanged; the
+! subroutines come from module suli_gpu and run on the GPU.
+!
+!   principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME]]]
+!
+! Tank 1.0 x 0.5 x 0.5 m scaled to the grid (h = 1/NX), dam break of a water
+! column WIDTH x HEIGHT in the x = 0 corner, water/air 1000/1.2 kg/m^3, no-slip
+! walls, Smagorinsky Cs = 0.1, mixed precision, FCG projection to 1e-8 with
+! PRECOND 0 Jacobi, 1 multigrid (default), 2 learned.  Every FRAME steps the
+! velocities, pressure and level set are copied back into the host arrays
+! u, v, w, prd1, ls with the reference extents (0_modules.f90:92-122), the
+! frame cadence of plot_f (4_plot.f90:260).
+program principal_gpu
+  use suli_gpu
+  implicit none
+  integer :: nx, ny, nz, ts, precond, frame, nargs
+  real(8) :: width, height, h, div, vol_ini
+  character(len=64) :: arg
+  integer(8) :: c0, c1, crate
+  real(8), allocatable :: u(:,:,:), v(:,:,:), w(:,:,:), prd1(:,:,:), ls(:,:,:)
+
+  nargs = command_argument_count()
+  if (nargs < 4) then
+    write(*,*) 'usage: principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME]]]'
+    stop 2
+  endif
+  call get_command_argument(1, arg); read(arg, *) nx
+  call get_command_argument(2, arg); read(arg, *) ny
+  call get_command_argument(3, arg); read(arg, *) nz
+  call get_command_argument(4, arg); read(arg, *) ts
+  precond = 1
+  width = 0.25d0
+  height = 0.25d0
+  frame = 0
+  if (nargs >= 5) then
+    call get_command_argument(5, arg); read(arg, *) precond
+  endif
+  if (nargs >= 7) then
+    call get_command_argument(6, arg); read(arg, *) width
+    call get_command_argument(7, arg); read(arg, *) height
+  endif
+  if (nargs >= 8) then
+    call get_command_argument(8, arg); read(arg, *) frame
+  endif
+  h = 1.0d0 / nx
+
+  ! iniciais (2_iniciais.f90:10-106) and level_set_ini (14_level_set.f90:10-219)
+  prm%nx = nx; prm%ny = ny; prm%nz = nz
+  prm%dx = h; prm%dy = h; prm%dz = h
+  prm%ccx0 = 2; prm%ccxf = 2; prm%ccy0 = 2; prm%ccyf = 2; prm%ccz0 = 2; prm%cczf = 2
+  prm%m_turb = 1
+  prm%rho_f1 = 1.2d0; prm%rho_f2 = 1000.0d0; prm%mi_f1 = 1.8d-5; prm%mi_f2 = 1.0d-3
+  prm%sigma = 0.0728d0; prm%alpha1 = 1.5d0
+  prm%csmag = 0.1d0; prm%gx = 0.0d0; prm%gz = 9.81d0; prm%chezy = 44.5d0
+  prm%precision = 1; prm%poisson = 1; prm%precond = precond; prm%max_it = 0
+  prm%tol_abs = 0.0d0; prm%tol_rel = 1.0d-8; prm%device = 0
+  call check(c_suli_init(prm), 'init')
+  call check(c_suli_init_dambreak(width, height, vol_ini), 'init_dambreak')
+  dt = 0.2d0 * h / (2.0d0 * sqrt(prm%gz * height))
+  ntt = 2
+  a_dt(1) = 0.5d0 * dt
+  a_dt(2) = 1.0d0 * dt
+  a_dt(3) = 1.0d0 * dt
+  allocate(u(0:nx+2,0:ny+1,0:nz+1), v(0:nx+1,0:ny+2,0:nz+1), w(0:nx+1,0:ny+1,0:nz+2))
+  allocate(prd1(0:nx+1,0:ny+1,0:nz+1), ls(nx,ny,nz))
+
+  call contorno(1)
+  call contorno(3)
+
+  call system_clock(c0, crate)
+  do it = 1, ts
+    t = it * dt
+
+    call level_set()
+    call contorno(3)
+
+    do tt = 1, ntt
+      dt = a_dt(tt)
+
+      call visco()
+      call convdiff()
+      call tempo()
+
+      call contorno(2)
+
+      call graddin()
+      call posdin()
+      call contorno(1)
+    enddo
+
+    call est(div)
+    write(*,'(A,I6,A,ES12.5,A,ES12.5,A,ES12.5,A,I5)') 'step', it, ' t=', t, ' vol_err=', &
+      (ls_st%vol_ins - ls_st%vol_ini) / ls_st%vol_ini, ' maxdiv=', div, ' poisson_iters=', pois_st%iters
+    if (frame > 0) then
+      if (mod(it, frame) == 0) then
+        call download(SULI_U, u)
+        call download(SULI_V, v)
+        call download(SULI_W, w)
+        call download(SULI_PRD1, prd1)
+        call download(SULI_LS, ls)
+        write(*,'(A,I6,A,ES12.5,A,ES12.5)') 'frame', it, ' max|u|=', maxval(abs(u)), ' min ls=', minval(ls)
+      endif
+    endif
+  enddo
+  call check(c_suli_sync(), 'sync')
+  call system_clock(c1)
+  write(*,'(A,F10.3,A,F10.2,A,F8.2)') 'seconds=', real(c1 - c0, 8) / real(crate, 8), &
+    ' Mcell-steps/s=', real(nx, 8) * ny * nz * ts / (real(c1 - c0, 8) / real(crate, 8)) / 1.0d6, &
+    ' poisson_iters/solve=', real(pois_iters, 8) / max(pois_solves, 1_8)
+  call check(c_suli_finalize(), 'finalize')
+end program principal_gpu

@@ -1,0 +1,187 @@
+! ISO_C_BINDING shim between the SuLi-IC Fortran driver and libsuli.so.
+!
+! The reference driver (1_principal.f90:64-108) calls argument-less
+! subroutines that work on module globals.  This module keeps the same
+! subroutine names and the global time step dt (0_modules.f90:9, mutated per
+! RK stage at 1_principal.f90:78), and forwards each call to the C ABI of
+! include/suli.h.  Errors print the library's message and STOP, the
+! reference's own convention (e.g. 6_convdiff.f90:81-82).
+module suli_gpu
+  use iso_c_binding
+  implicit none
+
+  type, bind(C) :: suli_params
+    integer(c_int32_t) :: nx, ny, nz
+    real(c_double) :: dx, dy, dz
+    integer(c_int32_t) :: ccx0, ccxf, ccy0, ccyf, ccz0, cczf
+    integer(c_int32_t) :: m_turb
+    real(c_double) :: rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1
+    real(c_double) :: csmag, gx, gz, chezy
+    integer(c_int32_t) :: precision, poisson, precond, max_it
+    real(c_double) :: tol_abs, tol_rel
+    integer(c_int32_t) :: device
+  end type suli_params
+
+  type, bind(C) :: suli_poisson_stats
+    integer(c_int32_t) :: iters, outer, fallbacks, converged
+    real(c_double) :: res0, res, ms
+  end type suli_poisson_stats
+
+  type, bind(C) :: suli_ls_stats
+    real(c_double) :: vol_ini, vol_ins
+    integer(c_int32_t) :: reinit_iters, mass_corr_iters
+  end type suli_ls_stats
+
+  ! field identifiers (include/suli.h)
+  integer(c_int), parameter :: SULI_U = 0, SULI_V = 1, SULI_W = 2, SULI_PRD1 = 3, SULI_RHO = 4, &
+                               SULI_LS = 6, SULI_HS = 9, SULI_NUT = 16
+
+  interface
+    integer(c_int) function c_suli_init(p) bind(C, name='suli_init')
+      import :: suli_params, c_int
+      type(suli_params), intent(in) :: p
+    end function
+    integer(c_int) function c_suli_finalize() bind(C, name='suli_finalize')
+      import :: c_int
+    end function
+    integer(c_int) function c_suli_last_error(buf, n) bind(C, name='suli_last_error')
+      import :: c_int, c_char
+      character(kind=c_char), dimension(*) :: buf
+      integer(c_int), value :: n
+    end function
+    integer(c_int) function c_suli_upload(f, a, n) bind(C, name='suli_upload')
+      import :: c_int, c_double, c_int64_t
+      integer(c_int), value :: f
+      real(c_double), dimension(*), intent(in) :: a
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function c_suli_download(f, a, n) bind(C, name='suli_download')
+      import :: c_int, c_double, c_int64_t
+      integer(c_int), value :: f
+      real(c_double), dimension(*) :: a
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function c_suli_init_dambreak(w, h, v) bind(C, name='suli_init_dambreak')
+      import :: c_int, c_double
+      real(c_double), value :: w, h
+      real(c_double) :: v
+    end function
+    integer(c_int) function c_suli_level_set(dt, st) bind(C, name='suli_level_set')
+      import :: c_int, c_double, suli_ls_stats
+      real(c_double), value :: dt
+      type(suli_ls_stats) :: st
+    end function
+    integer(c_int) function c_suli_contorno(nlock) bind(C, name='suli_contorno')
+      import :: c_int
+      integer(c_int), value :: nlock
+    end function
+    integer(c_int) function c_suli_visco() bind(C, name='suli_visco')
+      import :: c_int
+    end function
+    integer(c_int) function c_suli_convdiff() bind(C, name='suli_convdiff')
+      import :: c_int
+    end function
+    integer(c_int) function c_suli_tempo(tt, dt) bind(C, name='suli_tempo')
+      import :: c_int, c_double
+      integer(c_int), value :: tt
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function c_suli_graddin(dt, st) bind(C, name='suli_graddin')
+      import :: c_int, c_double, suli_poisson_stats
+      real(c_double), value :: dt
+      type(suli_poisson_stats) :: st
+    end function
+    integer(c_int) function c_suli_posdin(dt) bind(C, name='suli_posdin')
+      import :: c_int, c_double
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function c_suli_est_div(d) bind(C, name='suli_est_div')
+      import :: c_int, c_double
+      real(c_double) :: d
+    end function
+    integer(c_int) function c_suli_sync() bind(C, name='suli_sync')
+      import :: c_int
+    end function
+  end interface
+
+  ! driver state mirroring module disc (0_modules.f90:9,39-41)
+  real(8) :: dt, t
+  integer :: it, tt, ntt
+  real(8), dimension(3) :: a_dt
+  type(suli_params) :: prm
+  type(suli_ls_stats) :: ls_st
+  type(suli_poisson_stats) :: pois_st
+  integer(8) :: pois_iters = 0, pois_solves = 0
+
+contains
+
+  subroutine check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    character(kind=c_char), dimension(512) :: buf
+    integer :: n, k
+    integer(c_int) :: r2
+    if (rc == 0) return
+    buf = c_null_char
+    r2 = c_suli_last_error(buf, 512_c_int)
+    n = 0
+    do k = 1, 512
+      if (buf(k) == c_null_char) exit
+      n = k
+    enddo
+    write(*,*) 'suli: ', what, ' failed: ', (buf(k), k = 1, n)
+    stop 1
+  end subroutine check
+
+  ! ---- the reference subroutine names (1_principal.f90:74-92) ----------------
+  subroutine level_set()
+    call check(c_suli_level_set(dt, ls_st), 'level_set')
+  end subroutine level_set
+
+  subroutine contorno(nlock)
+    integer, intent(in) :: nlock
+    call check(c_suli_contorno(int(nlock, c_int)), 'contorno')
+  end subroutine contorno
+
+  subroutine visco()
+    call check(c_suli_visco(), 'visco')
+  end subroutine visco
+
+  subroutine convdiff()
+    call check(c_suli_convdiff(), 'convdiff')
+  end subroutine convdiff
+
+  subroutine tempo()
+    call check(c_suli_tempo(int(tt, c_int), dt), 'tempo')
+  end subroutine tempo
+
+  subroutine graddin()
+    call check(c_suli_graddin(dt, pois_st), 'graddin')
+    pois_iters = pois_iters + pois_st%iters
+    pois_solves = pois_solves + 1
+  end subroutine graddin
+
+  subroutine posdin()
+    call check(c_suli_posdin(dt), 'posdin')
+  end subroutine posdin
+
+  ! est (4_plot.f90:500-525): max abs divergence and volume, one line per step
+  subroutine est(div)
+    real(8), intent(out) :: div
+    call check(c_suli_est_div(div), 'est')
+  end subroutine est
+
+  ! host copy of a field in the reference extents, for output
+  subroutine download(field, a)
+    integer(c_int), intent(in) :: field
+    real(8), dimension(:,:,:), intent(inout) :: a
+    call check(c_suli_download(field, a, int(size(a), c_int64_t)), 'download')
+  end subroutine download
+
+  subroutine upload(field, a)
+    integer(c_int), intent(in) :: field
+    real(8), dimension(:,:,:), intent(in) :: a
+    call check(c_suli_upload(field, a, int(size(a), c_int64_t)), 'upload')
+  end subroutine upload
+
+end module suli_gpu
