@@ -130,11 +130,8 @@ def cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, budget_s):
         os.remove(fin)
         os.rmdir(tmp)
     wall = time.time() - t0
-    vals = {}
-    for tok in out.split():
-        if "=" in tok:
-            k, v = tok.split("=")
-            vals[k] = float(v)
+    import re
+    vals = {k: float(v) for k, v in re.findall(r"(\w+)=\s*([-+0-9.Ee]+)", out)}
     info = {"unit": "Mcell-steps/s", "cores": 1, "kind": "reference", "cpu_model": _cpu_model(),
             "host_nproc": os.cpu_count(), "wall_s": round(wall, 1)}
     if "level_set_seconds" not in vals:
