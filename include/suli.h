@@ -152,6 +152,45 @@ int suli_convdiff_forces(void);
 int suli_stage(int tt, double dt_base, suli_poisson_stats *st);
 int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps);
 
+/* Switch the pressure solver of later suli_graddin calls (the fields stay
+ * as they are).  tol_rel <= 0 or max_it <= 0 keeps the current value.  No
+ * counterpart in the reference: its solver is fixed at compile time. */
+int suli_set_solver(int poisson, int precond, double tol_rel, int max_it);
+
+/* Learned preconditioner (SULI_PRECOND_LEARNED).  No counterpart in the
+ * reference (9_grad.f90 is Jacobi-scaled CG).  A 3-level 3-D convolutional
+ * U-Net (channels 16/32/64, 3x3x3 kernels, ReLU, average pooling, nearest
+ * upsampling with additive skips) maps (r_s/rms(r_s), normalised log rho) to
+ * the search direction z; FCG is invariant to the scale of z.  Weights are
+ * FP32, convolutions run in BF16 with FP32 accumulation.  The grid
+ * dimensions must be multiples of 4.  A solve that stalls (64 inner
+ * iterations in one refinement pass without reaching its target), breaks
+ * down (p.Ap <= 0) or produces a non-finite residual finishes on multigrid
+ * and counts one fallback in suli_poisson_stats.fallbacks. */
+int64_t suli_unet_nparams(void);
+/* He-uniform initial weights from a seed; clears the Adam moments. */
+int suli_unet_init(uint64_t seed);
+/* weights as one FP32 vector: for layer l = c1a,c1b,c2a,c2b,c3a,c3b,c4,c5,c6
+ * the kernel W[27][c_out][c_in] (tap = (dz+1)*9+(dy+1)*3+(dx+1)), then the
+ * bias[c_out] */
+int suli_unet_set_weights(const float *w, int64_t n);
+int suli_unet_get_weights(float *w, int64_t n);
+/* max_samples > 0: restart the training set and arm the capture, so that
+ * every following FCG refinement pass stores its scaled residual and the
+ * density (FP32, on the device) until max_samples are held.  max_samples 0:
+ * stop capturing and keep the set. */
+int suli_unet_capture(int max_samples);
+int suli_unet_dataset_size(void);
+/* iters Adam steps (lr, beta 0.9/0.999) on samples drawn with the seed from
+ * the captured set; loss = 1 - (r.Az)^2/(|r|^2 |Az|^2), the squared residual
+ * left by one optimally scaled step along z.  *loss = mean of the last
+ * min(iters, 32) steps. */
+int suli_unet_train(int iters, double lr, uint64_t seed, double *loss);
+/* Loss, weight gradient (n = suli_unet_nparams, may be null) and network
+ * output z (may be null) for one sample given on the host in cell order
+ * (i fastest, nx*ny*nz values): r is the scaled residual, rho the density. */
+int suli_unet_loss_grad(const double *r, const double *rho, double *loss, float *grad, float *z);
+
 /* Per-kernel GPU timing: HIP events recorded on the library's stream around
  * every kernel launch while enabled.  suli_prof_enable(1) clears the totals;
  * suli_prof_get returns kernel idx's name, launch count and summed time. */

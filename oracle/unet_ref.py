@@ -1,0 +1,263 @@
+"""NumPy restatement of the learned Poisson preconditioner (test
+infrastructure only: the tests use it to check the GPU network; the product
+never imports it).
+
+The network and its training loss have no counterpart in the reference,
+whose projection (9_grad.f90:10-278) is Jacobi-scaled CG.  What this module
+shares with the reference is the operator the loss is taken through: the
+symmetrically scaled projection matrix of graddin, A_s = D^-1/2 A D^-1/2,
+with face coefficients dt/(h^2 rho_f) (9_grad.f90:46-76), face densities
+averaged from the two cells and copied at the end faces
+(8_derivadas.f90:304-434), and the wall faces counted in the diagonal D
+(9_grad.f90:80-87) while the mirrored ghost makes their off-diagonal term the
+cell itself (9_grad.f90:99-116).  The tests pin that operator against a
+sparse assembly of the reference's system, the convolution against a
+scalar-loop definition, and the gradient against finite differences.
+
+Arrays are channels-last, indexed [k, j, i, c], i.e. cell v = i + nx*(j +
+ny*k) with the channel fastest.  The weight vector is, for the layers c1a,
+c1b, c2a, c2b, c3a, c3b, c4, c5, c6 in turn, the kernel W[27][c_out][c_in]
+(tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) followed by the bias[c_out].
+
+With ``bf16=True`` the functions round at the points where the GPU stores
+BF16 (layer outputs, pooled/upsampled tensors, weights of the MFMA layers,
+backward gradient tensors) so that the remaining difference is the
+accumulation order; with ``bf16=False`` everything is float64.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+LAYERS = [(2, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 32), (32, 16), (16, 16), (16, 1)]
+MFMA_LAYERS = range(1, 8)  # BF16 weights; c1a and c6 use FP32 weights
+
+
+def nparams() -> int:
+    return sum(27 * ci * co + co for ci, co in LAYERS)
+
+
+def unpack(wvec):
+    """Split the flat weight vector into [(W[27,co,ci], b[co])]."""
+    out, off = [], 0
+    for ci, co in LAYERS:
+        W = np.asarray(wvec[off:off + 27 * co * ci], dtype=np.float64).reshape(27, co, ci)
+        off += 27 * co * ci
+        b = np.asarray(wvec[off:off + co], dtype=np.float64)
+        off += co
+        out.append((W, b))
+    assert off == len(wvec)
+    return out
+
+
+def pack(layers):
+    return np.concatenate([np.concatenate([W.ravel(), b.ravel()]) for W, b in layers])
+
+
+def bf16(x):
+    """Round float64 values to BF16 (through FP32, round to nearest even)."""
+    f = np.asarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    nan = (u & 0x7F800000) == 0x7F800000
+    r = np.where(nan, u, r).astype(np.uint32)
+    return r.view(np.float32).astype(np.float64)
+
+
+def f32(x):
+    return np.asarray(x, dtype=np.float32).astype(np.float64)
+
+
+def taps():
+    for t in range(27):
+        yield t, t // 9 - 1, (t // 3) % 3 - 1, t % 3 - 1  # tap, dz, dy, dx
+
+
+def conv(x, W, b=None):
+    """3x3x3 convolution, zero padding: out[v,co] = b + sum_tap,ci x[v+tap,ci] W[tap,co,ci]."""
+    nz, ny, nx, _ = x.shape
+    xp = np.pad(x, ((1, 1), (1, 1), (1, 1), (0, 0)))
+    out = np.zeros((nz, ny, nx, W.shape[1]))
+    for t, dz, dy, dx in taps():
+        out += xp[1 + dz:1 + dz + nz, 1 + dy:1 + dy + ny, 1 + dx:1 + dx + nx, :] @ W[t].T
+    if b is not None:
+        out += b
+    return out
+
+
+def conv_t(g, W):
+    """Adjoint of conv in its input: din[u,ci] = sum_tap g[u-tap,co] W[tap,co,ci]."""
+    nz, ny, nx, _ = g.shape
+    gp = np.pad(g, ((1, 1), (1, 1), (1, 1), (0, 0)))
+    out = np.zeros((nz, ny, nx, W.shape[2]))
+    for t, dz, dy, dx in taps():
+        out += gp[1 - dz:1 - dz + nz, 1 - dy:1 - dy + ny, 1 - dx:1 - dx + nx, :] @ W[t]
+    return out
+
+
+def conv_w(g, x):
+    """Weight gradient of conv: dW[tap,co,ci] = sum_v g[v,co] x[v+tap,ci]."""
+    nz, ny, nx, _ = x.shape
+    xp = np.pad(x, ((1, 1), (1, 1), (1, 1), (0, 0)))
+    dW = np.zeros((27, g.shape[3], x.shape[3]))
+    G = g.reshape(-1, g.shape[3])
+    for t, dz, dy, dx in taps():
+        dW[t] = G.T @ xp[1 + dz:1 + dz + nz, 1 + dy:1 + dy + ny, 1 + dx:1 + dx + nx, :].reshape(-1, x.shape[3])
+    return dW
+
+
+def pool(x):
+    nz, ny, nx, C = x.shape
+    return x.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2, C).mean(axis=(1, 3, 5))
+
+
+def up(x):
+    return x.repeat(2, axis=0).repeat(2, axis=1).repeat(2, axis=2)
+
+
+def down_sum(x):
+    nz, ny, nx, C = x.shape
+    return x.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2, C).sum(axis=(1, 3, 5))
+
+
+def relu(x):
+    return np.maximum(x, 0.0)
+
+
+def inputs(r, rho, rho_f1, rho_f2, bf=False):
+    """x0 = [r/rms(r), (log rho - l1)/l2], l1 the mean and l2 the half
+    difference of the two fluids' log densities; r, rho as [k, j, i]."""
+    r = f32(r) if bf else np.asarray(r, dtype=np.float64)
+    rho = f32(rho) if bf else np.asarray(rho, dtype=np.float64)
+    l1 = 0.5 * (np.log(rho_f1) + np.log(rho_f2))
+    l2 = max(0.5 * abs(np.log(rho_f2) - np.log(rho_f1)), 1e-6)
+    rms = np.sqrt(np.sum(r * r) / r.size)
+    x0 = np.stack([r / rms if rms > 0 else 0 * r, (np.log(rho) - l1) / l2], axis=-1)
+    return f32(x0) if bf else x0
+
+
+def forward(wvec, x0, bf=False):
+    """Network output z[k,j,i] and the activations (for backward)."""
+    L = unpack(wvec)
+    q = bf16 if bf else (lambda a: a)
+    Wm = [(bf16(W) if (bf and l in MFMA_LAYERS) else (f32(W) if bf else W), f32(b) if bf else b)
+          for l, (W, b) in enumerate(L)]
+    A = {"x0": x0}
+    A["a1"] = q(relu(conv(x0, *Wm[0])))
+    A["s1"] = q(relu(conv(A["a1"], *Wm[1])))
+    A["p1"] = q(pool(A["s1"]))
+    A["a2"] = q(relu(conv(A["p1"], *Wm[2])))
+    A["s2"] = q(relu(conv(A["a2"], *Wm[3])))
+    A["p2"] = q(pool(A["s2"]))
+    A["a3"] = q(relu(conv(A["p2"], *Wm[4])))
+    A["b3"] = q(relu(conv(A["a3"], *Wm[5])))
+    A["u2"] = q(up(A["b3"]) + A["s2"])
+    A["a4"] = q(relu(conv(A["u2"], *Wm[6])))
+    A["u1"] = q(up(A["a4"]) + A["s1"])
+    A["a5"] = q(relu(conv(A["u1"], *Wm[7])))
+    z = conv(A["a5"], *Wm[8])[..., 0]
+    A["W"] = Wm
+    return (f32(z) if bf else z), A
+
+
+def backward(dz, A, bf=False):
+    """Gradient of a scalar with dL/dz = dz through the network; returns the
+    flat gradient vector."""
+    q = bf16 if bf else (lambda a: a)
+    Wm = A["W"]
+    G = [None] * 9
+
+    def layer(l, g, x):
+        G[l] = (conv_w(g, x), g.reshape(-1, g.shape[-1]).sum(axis=0))
+
+    dz = dz[..., None]
+    layer(8, dz, A["a5"])
+    da5 = q(conv_t(dz, Wm[8][0]) * (A["a5"] > 0))
+    layer(7, da5, A["u1"])
+    du1 = q(conv_t(da5, Wm[7][0]))
+    da4 = q(down_sum(du1) * (A["a4"] > 0))
+    layer(6, da4, A["u2"])
+    du2 = q(conv_t(da4, Wm[6][0]))
+    db3 = q(down_sum(du2) * (A["b3"] > 0))
+    layer(5, db3, A["a3"])
+    da3 = q(conv_t(db3, Wm[5][0]) * (A["a3"] > 0))
+    layer(4, da3, A["p2"])
+    dp2 = q(conv_t(da3, Wm[4][0]))
+    ds2 = q((du2 + up(dp2) / 8.0) * (A["s2"] > 0))
+    layer(3, ds2, A["a2"])
+    da2 = q(conv_t(ds2, Wm[3][0]) * (A["a2"] > 0))
+    layer(2, da2, A["p1"])
+    dp1 = q(conv_t(da2, Wm[2][0]))
+    ds1 = q((du1 + up(dp1) / 8.0) * (A["s1"] > 0))
+    layer(1, ds1, A["a1"])
+    da1 = q(conv_t(ds1, Wm[1][0]) * (A["a1"] > 0))
+    layer(0, da1, A["x0"])
+    return pack(G)
+
+
+# ---- the scaled projection operator on the cell grid ------------------------
+
+def face_coefs(rho, dx, dy, dz):
+    """Face coefficients c_f = (1/h^2)/rho_f along x, y, z; rho as [k, j, i].
+    Returns arrays with n+1 faces along their own axis (face f between cells
+    f-1 and f, end faces copied from their neighbour)."""
+    def along(axis, h):
+        a = np.moveaxis(rho, axis, 0)
+        n = a.shape[0]
+        f = np.empty((n + 1,) + a.shape[1:])
+        f[1:n] = 0.5 * (a[1:] + a[:-1])
+        f[0] = f[1]
+        f[n] = f[n - 1]
+        return np.moveaxis(1.0 / (h * h) / f, 0, axis)
+    return along(2, dx), along(1, dy), along(0, dz)  # x is the last axis
+
+
+def apply_As(z, rho, dx, dy, dz):
+    """A_s z = z - sum_f c_f/sqrt(D_P D_Q) z_Q, a wall face coupling the cell
+    to itself (Neumann ghost)."""
+    cx, cy, cz = face_coefs(rho, dx, dy, dz)
+    D = cx[:, :, 1:] + cx[:, :, :-1] + cy[:, 1:] + cy[:, :-1] + cz[1:] + cz[:-1]
+    s = np.asarray(z, dtype=np.float64).copy()
+
+    def term(c_hi, c_lo, axis):
+        zp = np.moveaxis(z, axis, 0)
+        Dp = np.moveaxis(D, axis, 0)
+        chi = np.moveaxis(c_hi, axis, 0)
+        clo = np.moveaxis(c_lo, axis, 0)
+        # neighbour values with the mirrored ghost at the walls
+        zn_hi = np.concatenate([zp[1:], zp[-1:]], axis=0)
+        Dn_hi = np.concatenate([Dp[1:], Dp[-1:]], axis=0)
+        zn_lo = np.concatenate([zp[:1], zp[:-1]], axis=0)
+        Dn_lo = np.concatenate([Dp[:1], Dp[:-1]], axis=0)
+        t = chi / np.sqrt(Dp * Dn_hi) * zn_hi + clo / np.sqrt(Dp * Dn_lo) * zn_lo
+        return np.moveaxis(t, 0, axis)
+
+    s -= term(cx[:, :, 1:], cx[:, :, :-1], 2)
+    s -= term(cy[:, 1:], cy[:, :-1], 1)
+    s -= term(cz[1:], cz[:-1], 0)
+    return s
+
+
+def loss_and_grad(wvec, r, rho, rho_f1, rho_f2, dx, dy, dz, bf=False):
+    """L = 1 - (r.s)^2/(|r|^2 |s|^2), s = A_s z(r, rho); returns (L, dL/dw, z)."""
+    r64 = f32(r) if bf else np.asarray(r, dtype=np.float64)
+    rho64 = f32(rho) if bf else np.asarray(rho, dtype=np.float64)
+    x0 = inputs(r, rho, rho_f1, rho_f2, bf)
+    z, A = forward(wvec, x0, bf)
+    s = apply_As(z, rho64, dx, dy, dz)
+    rs, ss, rr = float(np.sum(r64 * s)), float(np.sum(s * s)), float(np.sum(r64 * r64))
+    L = 1.0 - rs * rs / (rr * ss)
+    t = -2.0 * rs / (rr * ss) * r64 + 2.0 * rs * rs / (rr * ss * ss) * s
+    if bf:
+        t = f32(t)
+    dz_ = apply_As(t, rho64, dx, dy, dz)
+    if bf:
+        dz_ = f32(dz_)
+    return L, backward(dz_, A, bf), z
+
+
+def loss(wvec, r, rho, rho_f1, rho_f2, dx, dy, dz):
+    x0 = inputs(r, rho, rho_f1, rho_f2)
+    z, _ = forward(wvec, x0)
+    s = apply_As(z, np.asarray(rho, dtype=np.float64), dx, dy, dz)
+    rs, ss, rr = float(np.sum(r * s)), float(np.sum(s * s)), float(np.sum(r * r))
+    return 1.0 - rs * rs / (rr * ss)

@@ -90,6 +90,14 @@ extern "C" int suli_init(const suli_params *p) {
     suli_set_error("suli_init: unknown precision");
     return 1;
   }
+  if (p->poisson != SULI_POISSON_REF && p->poisson != SULI_POISSON_FCG) {
+    suli_set_error("suli_init: unknown poisson solver");
+    return 1;
+  }
+  if (p->precond != SULI_PRECOND_JACOBI && p->precond != SULI_PRECOND_GMG && p->precond != SULI_PRECOND_LEARNED) {
+    suli_set_error("suli_init: unknown preconditioner");
+    return 1;
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     suli_set_error("suli_init: no HIP device visible");
@@ -582,6 +590,24 @@ extern "C" int suli_graddin(double dt, suli_poisson_stats *st) {
   Ctx &c = *g_ctx;
   if (c.P.poisson == SULI_POISSON_REF) return poisson_ref_solve(c, dt, st);
   return fcg_solve(c, dt, st);
+}
+
+extern "C" int suli_set_solver(int poisson, int precond, double tol_rel, int max_it) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  if (poisson != SULI_POISSON_REF && poisson != SULI_POISSON_FCG) {
+    suli_set_error("suli_set_solver: unknown poisson solver");
+    return 1;
+  }
+  if (precond != SULI_PRECOND_JACOBI && precond != SULI_PRECOND_GMG && precond != SULI_PRECOND_LEARNED) {
+    suli_set_error("suli_set_solver: unknown preconditioner");
+    return 1;
+  }
+  c.P.poisson = poisson;
+  c.P.precond = precond;
+  if (tol_rel > 0) c.P.tol_rel = tol_rel;
+  if (max_it > 0) c.P.max_it = max_it;
+  return 0;
 }
 
 extern "C" int suli_posdin(double dt) {

@@ -7,7 +7,13 @@ struct UNetPrecond;  // unet.hip
 int unet_create(Ctx &c, UNetPrecond **u);
 void unet_destroy(Ctx &c, UNetPrecond *u);
 bool unet_ready(const UNetPrecond *u);
-// per-solve preparation (log-density input for this rho field)
-void unet_prepare(Ctx &c, UNetPrecond *u);
-// z = M(r) with the network; kernels return at once when *active == 0
+// per-solve check and activation buffers; nonzero (error set) when the
+// network has no weights or the grid does not fit it
+int unet_prepare(Ctx &c, UNetPrecond *u);
+// z = M(r) with the network on the scaled FP32 residual (frame layout);
+// every kernel returns at once when *active == 0
 void unet_apply(Ctx &c, UNetPrecond *u, const float *r, float *z, const int *active);
+// append (r, rho) to the training set while capture is armed
+void unet_capture(Ctx &c, UNetPrecond *u, const float *r);
+// the solver's network (poisson_fcg.hip)
+UNetPrecond *fcg_unet(Ctx &c);

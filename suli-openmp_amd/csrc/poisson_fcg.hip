@@ -560,6 +560,8 @@ int fcg_create(Ctx &c) {
   return 0;
 }
 
+UNetPrecond *fcg_unet(Ctx &c) { return c.fcg ? c.fcg->unet : nullptr; }
+
 void fcg_destroy(Ctx &c) {
   if (!c.fcg) return;
   unet_destroy(c, c.fcg->unet);
@@ -672,12 +674,16 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
 
 // Apply the preconditioner selected for this solve: z = M r.
 static void apply_precond(Ctx &c, int pc, const float *r, float *z, const int *active) {
-  if (pc == SULI_PRECOND_LEARNED && c.fcg->unet && unet_ready(c.fcg->unet)) {
+  if (pc == SULI_PRECOND_LEARNED) {
     unet_apply(c, c.fcg->unet, r, z, active);
     return;
   }
   mg_vcycle(c, r, z, active);
 }
+
+// inner iterations a learned-preconditioner pass may take before the solve
+// falls back to multigrid
+static constexpr int kLearnedPassCap = 64;
 
 int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   PoissonFCG &f = *c.fcg;
@@ -686,8 +692,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   const dim3 b2(SULI_BX, SULI_BY, 1);
   const dim3 gk = cells_kloop(g, f.kz);
   const double tol = c.P.tol_rel;
-  const int pc = c.P.precond;
-  const bool use_pc = pc != SULI_PRECOND_JACOBI;
+  int pc = c.P.precond;
   // FP64 set-up shared with the reference CG
   {
     ProfScope ps(c, "poisson_setup");
@@ -699,7 +704,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     launch_finish<1>(c.st, gk, rb);
   }
   if (mg_setup(c)) return 1;
-  if (use_pc && pc == SULI_PRECOND_LEARNED && f.unet) unet_prepare(c, f.unet);
+  if (pc == SULI_PRECOND_LEARNED && unet_prepare(c, f.unet)) return 1;
   int outer = 0, inner_total = 0, fallbacks = 0;
   double res0 = -1, res = 0, bnorm = 0;
   bool conv = false;
@@ -728,12 +733,16 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     // inner target: rel_in of this pass, but no further than the overall tolerance needs
     const double want = 0.3 * tol * bnorm / rn;
     const double rin = std::max(rel_in, std::min(want, 1.0));
+    const bool use_pc = pc != SULI_PRECOND_JACOBI;
+    const int pass_max = pc == SULI_PRECOND_LEARNED ? std::min(kLearnedPassCap, maxit_in - inner_total)
+                                                    : maxit_in - inner_total;
     {
       ProfScope ps(c, "fcg_project");
       const RedSlot r3{c.partials, CTL_PTR(f.ctl, rin0)};
-      hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rin, maxit_in - inner_total, r3);
-      launch_finish<1>(c.st, gk, r3, EpiR32{f.ctl, rin, maxit_in - inner_total});
+      hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rin, pass_max, r3);
+      launch_finish<1>(c.st, gk, r3, EpiR32{f.ctl, rin, pass_max});
     }
+    unet_capture(c, f.unet, f.r32);
     int chunk = use_pc ? 4 : 16;
     for (;;) {
       for (int n = 0; n < chunk; n++) {
@@ -768,9 +777,23 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       if (chunk < 64) chunk *= 2;
     }
     inner_total += f.ctl_h->it;
-    {
+    const double rr_end = f.ctl_h->k2[0];
+    const bool finite = rr_end == rr_end && rr_end < 1e300;
+    if (finite) {
       ProfScope ps(c, "fcg_xupdate");
       hipLaunchKernelGGL(k_xupd, gk, b2, 0, c.st, g, f.x, f.d32);
+    }
+    if (pc == SULI_PRECOND_LEARNED) {
+      // stagnation, breakdown (p.q <= 0) or a non-finite residual: the rest of
+      // this solve runs on multigrid
+      const bool stalled = f.ctl_h->it >= pass_max && rr_end > f.ctl_h->thr_in;
+      if (!finite || stalled || !(f.ctl_h->k1[0] > 0.0)) {
+        pc = SULI_PRECOND_GMG;
+        fallbacks++;
+      }
+    } else if (!finite) {
+      suli_set_error("fcg_solve: non-finite residual in the inner iteration");
+      return 1;
     }
     if (inner_total >= maxit_in) {
       // iteration budget spent: report the residual after this pass

@@ -1,0 +1,424 @@
+// Kernels of the learned preconditioner: a 3-level 3-D convolutional U-Net
+// whose 3x3x3 convolutions run as BF16 implicit GEMM on the CDNA4 matrix
+// cores (FP32 accumulation), plus its backward pass, pooling, upsampling and
+// the Adam optimiser.
+//
+// Activations are channels-last ("NDHWC"): element (v, c) of a C-channel
+// tensor on an nx*ny*nz grid is at v*C + c with v = i + nx*(j + ny*k)
+// (0-based), stored as BF16 bit patterns (uint16_t).  A 3x3x3 convolution is
+// the GEMM out[v][co] = sum_{tap,ci} in[v+tap][ci] W[tap][co][ci] with
+// M = voxels, N = C_out, K = 27*C_in; zero padding outside the grid.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  // round to nearest even; NaN stays NaN
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+struct Dims {
+  int nx, ny, nz;
+  __host__ __device__ __forceinline__ int64_t n() const { return (int64_t)nx * ny * nz; }
+};
+
+__device__ __forceinline__ void tap_offsets(int tap, int &dx, int &dy, int &dz) {
+  dz = tap / 9 - 1;
+  dy = (tap / 3) % 3 - 1;
+  dx = tap % 3 - 1;
+}
+
+enum { CONV_BIAS = 1, CONV_RELU = 2, CONV_MASK = 4 };
+
+// 3x3x3 convolution on MFMA.  Block = 4 wavefronts; wavefront w computes 16
+// consecutive voxels along x (x0..x0+15) of row (j,k) for all COUT outputs as
+// COUT/16 tiles of 16x16.  K steps: 27 taps x (CIN/KC) channel chunks; KC=32
+// uses v_mfma_f32_16x16x32_bf16 (8 BF16 per lane), KC=16 the 16x16x16 form.
+// Lane l holds A[l%16][(l/16)*KC/4 + 0..KC/4-1] (voxel, input channels),
+// B[(l/16)*KC/4 + ..][l%16] (input channels, output channel) and
+// C[(l/16)*4 + 0..3][l%16].  Epilogue: + bias, ReLU, and (MASK) zero where the
+// gate tensor is not positive (ReLU derivative in the backward pass).
+template <int CIN, int COUT, int FLAGS>
+__global__ __launch_bounds__(256) void k_conv_mfma(Dims d, const uint16_t *__restrict__ in,
+                                                   const uint16_t *__restrict__ w, const float *__restrict__ bias,
+                                                   const uint16_t *__restrict__ gate, uint16_t *__restrict__ out,
+                                                   const int *__restrict__ active) {
+  if (active && !*active) return;
+  constexpr int KC = (CIN % 32 == 0) ? 32 : 16;
+  constexpr int KL = KC / 4;  // BF16 per lane per MFMA
+  constexpr int NT = COUT / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * 64 + wave * 16;
+  const int j = blockIdx.y, k = blockIdx.z;
+  const int m = lane & 15, kg = lane >> 4;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 27; tap++) {
+    int dx, dy, dz;
+    tap_offsets(tap, dx, dy, dz);
+    const int xs = x0 + m + dx, ys = j + dy, zs = k + dz;
+    const bool ok = xs >= 0 && xs < d.nx && ys >= 0 && ys < d.ny && zs >= 0 && zs < d.nz;
+    const int64_t vs = ok ? (xs + (int64_t)d.nx * (ys + (int64_t)d.ny * zs)) : 0;
+#pragma unroll
+    for (int ch = 0; ch < CIN; ch += KC) {
+      if constexpr (KC == 32) {
+        bf16x8 a;
+        if (ok) a = *(const bf16x8 *)(in + vs * CIN + ch + kg * KL);
+        else a = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const bf16x8 b = *(const bf16x8 *)(w + ((int64_t)(tap * COUT + t * 16 + m) * CIN + ch + kg * KL));
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        }
+      } else {
+        bf16x4 a;
+        if (ok) a = *(const bf16x4 *)(in + vs * CIN + ch + kg * KL);
+        else a = bf16x4{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const bf16x4 b = *(const bf16x4 *)(w + ((int64_t)(tap * COUT + t * 16 + m) * CIN + ch + kg * KL));
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int co = t * 16 + m;
+    const float bv = (FLAGS & CONV_BIAS) ? bias[co] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int x = x0 + kg * 4 + r;
+      if (x >= d.nx) continue;
+      const int64_t v = x + (int64_t)d.nx * (j + (int64_t)d.ny * k);
+      float val = acc[t][r] + bv;
+      if (FLAGS & CONV_RELU) val = val > 0.f ? val : 0.f;
+      if (FLAGS & CONV_MASK) val = bf2f(gate[v * COUT + co]) > 0.f ? val : 0.f;
+      out[v * COUT + co] = f2bf(val);
+    }
+  }
+}
+
+// Weight gradient dW[tap][co][ci] = sum_v dy[v][co] x[v+tap][ci] on MFMA:
+// for one tap and one chunk of voxels, A = dy^T (COUT x voxels), B = x shifted
+// (voxels x CIN), staged through LDS transposed ([channel][32 voxels]) so each
+// lane reads its 8 consecutive voxels as one 16-byte word.  Partial results
+// per chunk go to part[(chunk*27 + tap)*COUT*CIN + co*CIN + ci].
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_wgrad_mfma(Dims d, const uint16_t *__restrict__ dy,
+                                                    const uint16_t *__restrict__ x, float *__restrict__ part,
+                                                    int chunk_vox) {
+  constexpr int NTM = COUT / 16, NTN = CIN / 16, NTILE = NTM * NTN;
+  constexpr int TPW = (NTILE + 3) / 4;  // tiles per wavefront
+  __shared__ uint16_t dyT[COUT][32 + 8];
+  __shared__ uint16_t xT[CIN][32 + 8];
+  const int tap = blockIdx.x, chunk = blockIdx.y;
+  int dx, dyo, dz;
+  tap_offsets(tap, dx, dyo, dz);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kg = lane >> 4;
+  f32x4 acc[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; q++) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int64_t n = d.n();
+  const int64_t v0 = (int64_t)chunk * chunk_vox;
+  const int64_t v1 = v0 + chunk_vox < n ? v0 + chunk_vox : n;
+  for (int64_t vb = v0; vb < v1; vb += 32) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * COUT; e += 256) {
+      const int vv = e / COUT, co = e % COUT;
+      const int64_t v = vb + vv;
+      dyT[co][vv] = v < v1 ? dy[v * COUT + co] : (uint16_t)0;
+    }
+    for (int e = threadIdx.x; e < 32 * CIN; e += 256) {
+      const int vv = e / CIN, ci = e % CIN;
+      const int64_t v = vb + vv;
+      uint16_t val = 0;
+      if (v < v1) {
+        const int i = (int)(v % d.nx), jj = (int)((v / d.nx) % d.ny), kk = (int)(v / ((int64_t)d.nx * d.ny));
+        const int xs = i + dx, ys = jj + dyo, zs = kk + dz;
+        if (xs >= 0 && xs < d.nx && ys >= 0 && ys < d.ny && zs >= 0 && zs < d.nz)
+          val = x[(xs + (int64_t)d.nx * (ys + (int64_t)d.ny * zs)) * CIN + ci];
+      }
+      xT[ci][vv] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int tile = wave + 4 * q;
+      if (tile < NTILE) {
+        const int tm = tile / NTN, tn = tile % NTN;
+        bf16x8 a, b;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          a[e] = __builtin_bit_cast(__bf16, dyT[tm * 16 + m][kg * 8 + e]);
+          b[e] = __builtin_bit_cast(__bf16, xT[tn * 16 + m][kg * 8 + e]);
+        }
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[q], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < TPW; q++) {
+    const int tile = wave + 4 * q;
+    if (tile < NTILE) {
+      const int tm = tile / NTN, tn = tile % NTN;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int co = tm * 16 + kg * 4 + r, ci = tn * 16 + m;
+        part[((int64_t)(chunk * 27 + tap) * COUT + co) * CIN + ci] = acc[q][r];
+      }
+    }
+  }
+}
+
+// out[i] = sum over chunks of part[chunk*len + i]
+__global__ void k_sum_chunks(const float *__restrict__ part, int nchunk, int64_t len, float *__restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int c = 0; c < nchunk; c++) s += part[(int64_t)c * len + i];
+    out[i] = s;
+  }
+}
+
+// bias gradient partials: part[chunk*C + c] = sum over the chunk of dy[v][c]
+template <typename TY>
+__global__ __launch_bounds__(256) void k_bias_grad(Dims d, int C, const TY *__restrict__ dy, float *__restrict__ part,
+                                                   int chunk_vox) {
+  __shared__ float lds[256];
+  const int64_t n = d.n();
+  const int64_t v0 = (int64_t)blockIdx.x * chunk_vox;
+  const int64_t v1 = v0 + chunk_vox < n ? v0 + chunk_vox : n;
+  for (int c = 0; c < C; c++) {
+    float s = 0.f;
+    for (int64_t v = v0 + threadIdx.x; v < v1; v += 256) {
+      if constexpr (sizeof(TY) == 2) s += bf2f(((const uint16_t *)dy)[v * C + c]);
+      else s += ((const float *)dy)[v * C + c];
+    }
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * C + c] = lds[0];
+    __syncthreads();
+  }
+}
+
+// First layer, 2 -> 16 channels, FP32 input x0[v][2], BF16 output, bias + ReLU.
+__global__ __launch_bounds__(256) void k_conv_in(Dims d, const float *__restrict__ x0, const float *__restrict__ w,
+                                                 const float *__restrict__ bias, uint16_t *__restrict__ out,
+                                                 const int *__restrict__ active) {
+  if (active && !*active) return;
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= d.n()) return;
+  const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+  float acc[16];
+#pragma unroll
+  for (int co = 0; co < 16; co++) acc[co] = bias[co];
+  for (int tap = 0; tap < 27; tap++) {
+    int dx, dy, dz;
+    tap_offsets(tap, dx, dy, dz);
+    const int xs = i + dx, ys = j + dy, zs = k + dz;
+    if (xs < 0 || xs >= d.nx || ys < 0 || ys >= d.ny || zs < 0 || zs >= d.nz) continue;
+    const int64_t vs = xs + (int64_t)d.nx * (ys + (int64_t)d.ny * zs);
+    const float a0 = x0[vs * 2], a1 = x0[vs * 2 + 1];
+#pragma unroll
+    for (int co = 0; co < 16; co++) acc[co] += w[(tap * 16 + co) * 2] * a0 + w[(tap * 16 + co) * 2 + 1] * a1;
+  }
+#pragma unroll
+  for (int co = 0; co < 16; co++) out[v * 16 + co] = f2bf(acc[co] > 0.f ? acc[co] : 0.f);
+}
+
+// Last layer, 16 -> 1 channel, BF16 input, FP32 output (bias, no activation).
+__global__ __launch_bounds__(256) void k_conv_out(Dims d, const uint16_t *__restrict__ a, const float *__restrict__ w,
+                                                  const float *__restrict__ bias, float *__restrict__ z,
+                                                  const int *__restrict__ active) {
+  if (active && !*active) return;
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= d.n()) return;
+  const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+  float acc = bias[0];
+  for (int tap = 0; tap < 27; tap++) {
+    int dx, dy, dz;
+    tap_offsets(tap, dx, dy, dz);
+    const int xs = i + dx, ys = j + dy, zs = k + dz;
+    if (xs < 0 || xs >= d.nx || ys < 0 || ys >= d.ny || zs < 0 || zs >= d.nz) continue;
+    const int64_t vs = xs + (int64_t)d.nx * (ys + (int64_t)d.ny * zs);
+#pragma unroll
+    for (int ci = 0; ci < 16; ci++) acc += w[tap * 16 + ci] * bf2f(a[vs * 16 + ci]);
+  }
+  z[v] = acc;
+}
+
+// Backward of the last layer: da[v][ci] = sum_tap W[tap][ci] dz[v - tap],
+// masked by a[v][ci] > 0 (ReLU of the layer below).
+__global__ __launch_bounds__(256) void k_conv_out_bwd(Dims d, const float *__restrict__ dz, const float *__restrict__ w,
+                                                      const uint16_t *__restrict__ gate, uint16_t *__restrict__ da) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= d.n()) return;
+  const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+  float acc[16];
+#pragma unroll
+  for (int ci = 0; ci < 16; ci++) acc[ci] = 0.f;
+  for (int tap = 0; tap < 27; tap++) {
+    int dx, dy, dzz;
+    tap_offsets(tap, dx, dy, dzz);
+    const int xs = i - dx, ys = j - dy, zs = k - dzz;
+    if (xs < 0 || xs >= d.nx || ys < 0 || ys >= d.ny || zs < 0 || zs >= d.nz) continue;
+    const float g = dz[xs + (int64_t)d.nx * (ys + (int64_t)d.ny * zs)];
+#pragma unroll
+    for (int ci = 0; ci < 16; ci++) acc[ci] += w[tap * 16 + ci] * g;
+  }
+#pragma unroll
+  for (int ci = 0; ci < 16; ci++) da[v * 16 + ci] = f2bf(bf2f(gate[v * 16 + ci]) > 0.f ? acc[ci] : 0.f);
+}
+
+// Weight-gradient partials of a small layer (first or last), one tap per
+// block column: part[(chunk*27 + tap)*CO*CI + co*CI + ci] =
+// sum over the chunk of dy[v][co] x[v+tap][ci].
+template <int CI, int CO, typename TY, typename TX>
+__global__ __launch_bounds__(256) void k_wgrad_small(Dims d, const TY *__restrict__ dy, const TX *__restrict__ x,
+                                                     float *__restrict__ part, int chunk_vox) {
+  __shared__ float lds[256];
+  const int tap = blockIdx.x, chunk = blockIdx.y;
+  int dx, dyo, dz;
+  tap_offsets(tap, dx, dyo, dz);
+  float acc[CO * CI];
+#pragma unroll
+  for (int q = 0; q < CO * CI; q++) acc[q] = 0.f;
+  const int64_t n = d.n();
+  const int64_t v0 = (int64_t)chunk * chunk_vox;
+  const int64_t v1 = v0 + chunk_vox < n ? v0 + chunk_vox : n;
+  for (int64_t v = v0 + threadIdx.x; v < v1; v += 256) {
+    const int i = (int)(v % d.nx), jj = (int)((v / d.nx) % d.ny), kk = (int)(v / ((int64_t)d.nx * d.ny));
+    const int xs = i + dx, ys = jj + dyo, zs = kk + dz;
+    if (xs < 0 || xs >= d.nx || ys < 0 || ys >= d.ny || zs < 0 || zs >= d.nz) continue;
+    const int64_t vs = xs + (int64_t)d.nx * (ys + (int64_t)d.ny * zs);
+    float xv[CI];
+#pragma unroll
+    for (int ci = 0; ci < CI; ci++) {
+      if constexpr (sizeof(TX) == 2) xv[ci] = bf2f(((const uint16_t *)x)[vs * CI + ci]);
+      else xv[ci] = ((const float *)x)[vs * CI + ci];
+    }
+#pragma unroll
+    for (int co = 0; co < CO; co++) {
+      float g;
+      if constexpr (sizeof(TY) == 2) g = bf2f(((const uint16_t *)dy)[v * CO + co]);
+      else g = ((const float *)dy)[v * CO + co];
+#pragma unroll
+      for (int ci = 0; ci < CI; ci++) acc[co * CI + ci] += g * xv[ci];
+    }
+  }
+  for (int q = 0; q < CO * CI; q++) {
+    lds[threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)(chunk * 27 + tap) * CO * CI + q] = lds[0];
+    __syncthreads();
+  }
+}
+
+// 2x2x2 average pooling of a C-channel BF16 tensor (fine dims d)
+__global__ void k_pool(Dims d, int C, const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
+                       const int *__restrict__ active) {
+  if (active && !*active) return;
+  const Dims dc{d.nx / 2, d.ny / 2, d.nz / 2};
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= dc.n() * C) return;
+  const int c = (int)(e % C);
+  const int64_t V = e / C;
+  const int I = (int)(V % dc.nx), J = (int)((V / dc.nx) % dc.ny), K = (int)(V / ((int64_t)dc.nx * dc.ny));
+  float s = 0.f;
+  for (int q = 0; q < 8; q++) {
+    const int i = 2 * I + (q & 1), j = 2 * J + ((q >> 1) & 1), k = 2 * K + (q >> 2);
+    s += bf2f(in[(i + (int64_t)d.nx * (j + (int64_t)d.ny * k)) * C + c]);
+  }
+  out[e] = f2bf(0.125f * s);
+}
+
+// out = nearest-upsample(coarse) + skip  (fine dims d)
+__global__ void k_up_add(Dims d, int C, const uint16_t *__restrict__ coarse, const uint16_t *__restrict__ skip,
+                         uint16_t *__restrict__ out, const int *__restrict__ active) {
+  if (active && !*active) return;
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= d.n() * C) return;
+  const int c = (int)(e % C);
+  const int64_t v = e / C;
+  const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+  const int64_t V = (i / 2) + (int64_t)(d.nx / 2) * ((j / 2) + (int64_t)(d.ny / 2) * (k / 2));
+  out[e] = f2bf(bf2f(coarse[V * C + c]) + bf2f(skip[e]));
+}
+
+// adjoint of nearest upsampling: coarse[V][c] = sum of the 8 children,
+// optionally masked by gate > 0 (fine dims d)
+__global__ void k_down_sum(Dims d, int C, const uint16_t *__restrict__ fine, const uint16_t *__restrict__ gate,
+                           uint16_t *__restrict__ out) {
+  const Dims dc{d.nx / 2, d.ny / 2, d.nz / 2};
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= dc.n() * C) return;
+  const int c = (int)(e % C);
+  const int64_t V = e / C;
+  const int I = (int)(V % dc.nx), J = (int)((V / dc.nx) % dc.ny), K = (int)(V / ((int64_t)dc.nx * dc.ny));
+  float s = 0.f;
+  for (int q = 0; q < 8; q++) {
+    const int i = 2 * I + (q & 1), j = 2 * J + ((q >> 1) & 1), k = 2 * K + (q >> 2);
+    s += bf2f(fine[(i + (int64_t)d.nx * (j + (int64_t)d.ny * k)) * C + c]);
+  }
+  if (gate && !(bf2f(gate[e]) > 0.f)) s = 0.f;
+  out[e] = f2bf(s);
+}
+
+// out = (a + coarse(parent)/8) masked by gate > 0: the gradient of a skip
+// tensor that also fed an average pool (fine dims d)
+__global__ void k_add_upavg_mask(Dims d, int C, const uint16_t *__restrict__ a, const uint16_t *__restrict__ coarse,
+                                 const uint16_t *__restrict__ gate, uint16_t *__restrict__ out) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= d.n() * C) return;
+  const int c = (int)(e % C);
+  const int64_t v = e / C;
+  const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+  const int64_t V = (i / 2) + (int64_t)(d.nx / 2) * ((j / 2) + (int64_t)(d.ny / 2) * (k / 2));
+  const float s = bf2f(a[e]) + 0.125f * bf2f(coarse[V * C + c]);
+  out[e] = f2bf(bf2f(gate[e]) > 0.f ? s : 0.f);
+}
+
+// BF16 copies of an FP32 weight tensor W[27][CO][CI]: the forward copy and
+// the flipped, transposed copy Wt[tap][ci][co] = W[26-tap][co][ci] used by the
+// data-gradient convolution.
+__global__ void k_weights_bf16(const float *__restrict__ w, int co_n, int ci_n, uint16_t *__restrict__ wf,
+                               uint16_t *__restrict__ wt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = co_n * ci_n;
+  if (e >= 27 * per) return;
+  const int tap = e / per, co = (e % per) / ci_n, ci = e % ci_n;
+  const uint16_t b = f2bf(w[e]);
+  wf[e] = b;
+  wt[((26 - tap) * ci_n + ci) * co_n + co] = b;
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master weights
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       int64_t n, float lr, float b1, float b2, float eps, float c1, float c2, float gscale) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i] * gscale;
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}

@@ -96,6 +96,13 @@ def lib():
             "suli_stage": ([i32, f64, p(PoissonStats)], i32),
             "suli_step": ([f64, p(LsStats), p(PoissonStats)], i32), "suli_sync": ([], i32),
             "suli_set_vol_ini": ([f64], i32), "suli_get_volumes": ([p(f64), p(f64)], i32),
+            "suli_set_solver": ([i32, i32, f64, i32], i32),
+            "suli_unet_nparams": ([], C.c_int64), "suli_unet_init": ([C.c_uint64], i32),
+            "suli_unet_set_weights": ([p(C.c_float), C.c_int64], i32),
+            "suli_unet_get_weights": ([p(C.c_float), C.c_int64], i32),
+            "suli_unet_capture": ([i32], i32), "suli_unet_dataset_size": ([], i32),
+            "suli_unet_train": ([i32, f64, C.c_uint64, p(f64)], i32),
+            "suli_unet_loss_grad": ([p(f64), p(f64), p(f64), p(C.c_float), p(C.c_float)], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -221,6 +228,54 @@ class Solver:
 
     def sync(self):
         _check(lib().suli_sync(), "suli_sync")
+
+    def set_solver(self, poisson, precond, tol_rel=0.0, max_it=0):
+        _check(lib().suli_set_solver(poisson, precond, tol_rel, max_it), "suli_set_solver")
+
+    # ---- learned preconditioner -------------------------------------------
+    def unet_nparams(self):
+        return int(lib().suli_unet_nparams())
+
+    def unet_init(self, seed):
+        _check(lib().suli_unet_init(seed), "suli_unet_init")
+
+    def unet_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(lib().suli_unet_set_weights(w.ctypes.data_as(C.POINTER(C.c_float)), w.size),
+               "suli_unet_set_weights")
+
+    def unet_get_weights(self):
+        w = np.empty(self.unet_nparams(), dtype=np.float32)
+        _check(lib().suli_unet_get_weights(w.ctypes.data_as(C.POINTER(C.c_float)), w.size),
+               "suli_unet_get_weights")
+        return w
+
+    def unet_capture(self, max_samples):
+        _check(lib().suli_unet_capture(max_samples), "suli_unet_capture")
+
+    def unet_dataset_size(self):
+        return int(lib().suli_unet_dataset_size())
+
+    def unet_train(self, iters, lr, seed=0):
+        v = C.c_double()
+        _check(lib().suli_unet_train(iters, lr, seed, C.byref(v)), "suli_unet_train")
+        return v.value
+
+    def unet_loss_grad(self, r, rho):
+        """(loss, gradient, z) for one sample; r, rho as [k, j, i] arrays."""
+        r = np.ascontiguousarray(r, dtype=np.float64).ravel()
+        rho = np.ascontiguousarray(rho, dtype=np.float64).ravel()
+        n = self.nx * self.ny * self.nz
+        if r.size != n or rho.size != n:
+            raise SuliError(f"unet_loss_grad: expected {n} cells")
+        loss = C.c_double()
+        g = np.empty(self.unet_nparams(), dtype=np.float32)
+        z = np.empty(n, dtype=np.float32)
+        P = C.POINTER
+        _check(lib().suli_unet_loss_grad(r.ctypes.data_as(P(C.c_double)), rho.ctypes.data_as(P(C.c_double)),
+                                         C.byref(loss), g.ctypes.data_as(P(C.c_float)),
+                                         z.ctypes.data_as(P(C.c_float))), "suli_unet_loss_grad")
+        return loss.value, g, z.reshape(self.nz, self.ny, self.nx)
 
 
 def load_state(solver: Solver, state, names=None):
