@@ -158,28 +158,39 @@ int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps);
 int suli_set_solver(int poisson, int precond, double tol_rel, int max_it);
 
 /* Learned preconditioner (SULI_PRECOND_LEARNED).  No counterpart in the
- * reference (9_grad.f90 is Jacobi-scaled CG).  A 3-level 3-D convolutional
- * U-Net (channels 16/32/64, 3x3x3 kernels, ReLU, average pooling, nearest
+ * reference (9_grad.f90 is Jacobi-scaled CG).  An L-level 3-D convolutional
+ * U-Net (channels 16/32/64/64.., 3x3x3 kernels, ReLU, average pooling, nearest
  * upsampling with additive skips) maps (r_s/rms(r_s), normalised log rho) to
- * the search direction z; FCG is invariant to the scale of z.  Weights are
+ * the search direction z; FCG is invariant to the scale of z, and each new
+ * direction is A-orthogonalised against the last four (FCG(4)), which keeps
+ * the iteration convergent for a nonlinear preconditioner.  Weights are
  * FP32, convolutions run in BF16 with FP32 accumulation.  The grid
- * dimensions must be multiples of 4.  A solve that stalls (64 inner
+ * dimensions must be multiples of 2^(L-1).  A solve that stalls (64 inner
  * iterations in one refinement pass without reaching its target), breaks
  * down (p.Ap <= 0) or produces a non-finite residual finishes on multigrid
  * and counts one fallback in suli_poisson_stats.fallbacks. */
+/* Number of levels L (2..6, default 3): channels min(16*2^l, 64) at level
+ * l; the grid dimensions must be multiples of 2^(L-1).  Discards the
+ * weights (suli_unet_init or suli_unet_set_weights follows). */
+int suli_unet_configure(int levels);
+int suli_unet_levels(void);
 int64_t suli_unet_nparams(void);
 /* He-uniform initial weights from a seed; clears the Adam moments. */
 int suli_unet_init(uint64_t seed);
-/* weights as one FP32 vector: for layer l = c1a,c1b,c2a,c2b,c3a,c3b,c4,c5,c6
- * the kernel W[27][c_out][c_in] (tap = (dz+1)*9+(dy+1)*3+(dx+1)), then the
- * bias[c_out] */
+/* Weights as one FP32 vector: for each layer in the order encoder (two per
+ * level 0..L-2), bottom (two), decoder (levels L-2 down to 0), output, the
+ * kernel W[27][c_out][c_in] (tap = (dz+1)*9+(dy+1)*3+(dx+1)) then the
+ * bias[c_out].  L = 3: c_in->c_out = 2->16, 16->16, 16->32, 32->32, 32->64,
+ * 64->32, 32->16, 16->16, 16->1. */
 int suli_unet_set_weights(const float *w, int64_t n);
 int suli_unet_get_weights(float *w, int64_t n);
-/* max_samples > 0: restart the training set and arm the capture, so that
- * every following FCG refinement pass stores its scaled residual and the
- * density (FP32, on the device) until max_samples are held.  max_samples 0:
- * stop capturing and keep the set. */
-int suli_unet_capture(int max_samples);
+/* max_samples > 0: arm the capture of training samples: the following FCG
+ * solves append the scaled residual and the density (FP32, on the device)
+ * at the start of every refinement pass and, when every > 0, at every
+ * every-th inner iteration, until the set holds max_samples.  max_samples
+ * 0: stop capturing.  suli_unet_clear empties the set and stops. */
+int suli_unet_capture(int max_samples, int every);
+int suli_unet_clear(void);
 int suli_unet_dataset_size(void);
 /* iters Adam steps (lr, beta 0.9/0.999) on samples drawn with the seed from
  * the captured set; loss = 1 - (r.Az)^2/(|r|^2 |Az|^2), the squared residual

@@ -15,31 +15,57 @@ sparse assembly of the reference's system, the convolution against a
 scalar-loop definition, and the gradient against finite differences.
 
 Arrays are channels-last, indexed [k, j, i, c], i.e. cell v = i + nx*(j +
-ny*k) with the channel fastest.  The weight vector is, for the layers c1a,
-c1b, c2a, c2b, c3a, c3b, c4, c5, c6 in turn, the kernel W[27][c_out][c_in]
-(tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) followed by the bias[c_out].
+ny*k) with the channel fastest.  The network has L levels (channels
+min(16*2^l, 64)); the weight vector holds, layer by layer in the order of
+arch(L), the kernel W[27][c_out][c_in] (tap = (dz+1)*9 + (dy+1)*3 + (dx+1))
+followed by the bias[c_out]; L is recovered from its length.
 
 With ``bf16=True`` the functions round at the points where the GPU stores
-BF16 (layer outputs, pooled/upsampled tensors, weights of the MFMA layers,
-backward gradient tensors) so that the remaining difference is the
+BF16 (layer outputs, pooled/upsampled tensors, weights of all layers but the
+first and the last, backward gradient tensors) so that the remaining difference is the
 accumulation order; with ``bf16=False`` everything is float64.
 """
 from __future__ import annotations
 
 import numpy as np
 
-LAYERS = [(2, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 32), (32, 16), (16, 16), (16, 1)]
-MFMA_LAYERS = range(1, 8)  # BF16 weights; c1a and c6 use FP32 weights
+def channels(L):
+    return [min(16 << l, 64) for l in range(L)]
 
 
-def nparams() -> int:
-    return sum(27 * ci * co + co for ci, co in LAYERS)
+def arch(L=3):
+    """Layer (c_in, c_out) list of an L-level network in weight-vector order:
+    encoder (a, b per level 0..L-2), bottom (a, b), decoder (levels L-2 down
+    to 0), output."""
+    C = channels(L)
+    lay = [(2, C[0]), (C[0], C[0])]
+    for l in range(1, L - 1):
+        lay += [(C[l - 1], C[l]), (C[l], C[l])]
+    lay += [(C[L - 2], C[L - 1]), (C[L - 1], C[L - 2])]
+    for l in range(L - 2, 0, -1):
+        lay.append((C[l], C[l - 1]))
+    lay += [(C[0], C[0]), (C[0], 1)]
+    return lay
+
+
+LAYERS = arch(3)
+
+
+def nparams(L=3) -> int:
+    return sum(27 * ci * co + co for ci, co in arch(L))
+
+
+def levels_of(nw: int) -> int:
+    for L in range(2, 7):
+        if nparams(L) == nw:
+            return L
+    raise ValueError(f"no architecture has {nw} parameters")
 
 
 def unpack(wvec):
     """Split the flat weight vector into [(W[27,co,ci], b[co])]."""
     out, off = [], 0
-    for ci, co in LAYERS:
+    for ci, co in arch(levels_of(len(wvec))):
         W = np.asarray(wvec[off:off + 27 * co * ci], dtype=np.float64).reshape(27, co, ci)
         off += 27 * co * ci
         b = np.asarray(wvec[off:off + co], dtype=np.float64)
@@ -137,25 +163,33 @@ def inputs(r, rho, rho_f1, rho_f2, bf=False):
 
 def forward(wvec, x0, bf=False):
     """Network output z[k,j,i] and the activations (for backward)."""
-    L = unpack(wvec)
+    L = levels_of(len(wvec))
+    lay = unpack(wvec)
+    n = len(lay)
     q = bf16 if bf else (lambda a: a)
-    Wm = [(bf16(W) if (bf and l in MFMA_LAYERS) else (f32(W) if bf else W), f32(b) if bf else b)
-          for l, (W, b) in enumerate(L)]
-    A = {"x0": x0}
-    A["a1"] = q(relu(conv(x0, *Wm[0])))
-    A["s1"] = q(relu(conv(A["a1"], *Wm[1])))
-    A["p1"] = q(pool(A["s1"]))
-    A["a2"] = q(relu(conv(A["p1"], *Wm[2])))
-    A["s2"] = q(relu(conv(A["a2"], *Wm[3])))
-    A["p2"] = q(pool(A["s2"]))
-    A["a3"] = q(relu(conv(A["p2"], *Wm[4])))
-    A["b3"] = q(relu(conv(A["a3"], *Wm[5])))
-    A["u2"] = q(up(A["b3"]) + A["s2"])
-    A["a4"] = q(relu(conv(A["u2"], *Wm[6])))
-    A["u1"] = q(up(A["a4"]) + A["s1"])
-    A["a5"] = q(relu(conv(A["u1"], *Wm[7])))
-    z = conv(A["a5"], *Wm[8])[..., 0]
-    A["W"] = Wm
+    Wm = [((bf16(W) if (bf and 0 < i < n - 1) else (f32(W) if bf else W)), (f32(b) if bf else b))
+          for i, (W, b) in enumerate(lay)]
+    ea = lambda l: 2 * l
+    eb = lambda l: 2 * l + 1
+    bot = 2 * (L - 1)
+    dec = lambda l: 2 * L + (L - 2 - l)
+    A = {"x0": x0, "W": Wm, "L": L}
+    A["A0"] = q(relu(conv(x0, *Wm[ea(0)])))
+    A["S0"] = q(relu(conv(A["A0"], *Wm[eb(0)])))
+    for l in range(1, L):
+        A[f"P{l}"] = q(pool(A[f"S{l - 1}"]))
+        if l < L - 1:
+            A[f"A{l}"] = q(relu(conv(A[f"P{l}"], *Wm[ea(l)])))
+            A[f"S{l}"] = q(relu(conv(A[f"A{l}"], *Wm[eb(l)])))
+        else:
+            A[f"A{l}"] = q(relu(conv(A[f"P{l}"], *Wm[bot])))
+            A["B"] = q(relu(conv(A[f"A{l}"], *Wm[bot + 1])))
+    prev = A["B"]
+    for l in range(L - 2, -1, -1):
+        A[f"U{l}"] = q(up(prev) + A[f"S{l}"])
+        A[f"D{l}"] = q(relu(conv(A[f"U{l}"], *Wm[dec(l)])))
+        prev = A[f"D{l}"]
+    z = conv(A["D0"], *Wm[n - 1])[..., 0]
     return (f32(z) if bf else z), A
 
 
@@ -163,34 +197,42 @@ def backward(dz, A, bf=False):
     """Gradient of a scalar with dL/dz = dz through the network; returns the
     flat gradient vector."""
     q = bf16 if bf else (lambda a: a)
-    Wm = A["W"]
-    G = [None] * 9
+    Wm, L = A["W"], A["L"]
+    n = len(Wm)
+    G = [None] * n
+    ea = lambda l: 2 * l
+    eb = lambda l: 2 * l + 1
+    bot = 2 * (L - 1)
+    dec = lambda l: 2 * L + (L - 2 - l)
 
-    def layer(l, g, x):
-        G[l] = (conv_w(g, x), g.reshape(-1, g.shape[-1]).sum(axis=0))
+    def layer(i, g, x):
+        G[i] = (conv_w(g, x), g.reshape(-1, g.shape[-1]).sum(axis=0))
 
     dz = dz[..., None]
-    layer(8, dz, A["a5"])
-    da5 = q(conv_t(dz, Wm[8][0]) * (A["a5"] > 0))
-    layer(7, da5, A["u1"])
-    du1 = q(conv_t(da5, Wm[7][0]))
-    da4 = q(down_sum(du1) * (A["a4"] > 0))
-    layer(6, da4, A["u2"])
-    du2 = q(conv_t(da4, Wm[6][0]))
-    db3 = q(down_sum(du2) * (A["b3"] > 0))
-    layer(5, db3, A["a3"])
-    da3 = q(conv_t(db3, Wm[5][0]) * (A["a3"] > 0))
-    layer(4, da3, A["p2"])
-    dp2 = q(conv_t(da3, Wm[4][0]))
-    ds2 = q((du2 + up(dp2) / 8.0) * (A["s2"] > 0))
-    layer(3, ds2, A["a2"])
-    da2 = q(conv_t(ds2, Wm[3][0]) * (A["a2"] > 0))
-    layer(2, da2, A["p1"])
-    dp1 = q(conv_t(da2, Wm[2][0]))
-    ds1 = q((du1 + up(dp1) / 8.0) * (A["s1"] > 0))
-    layer(1, ds1, A["a1"])
-    da1 = q(conv_t(ds1, Wm[1][0]) * (A["a1"] > 0))
-    layer(0, da1, A["x0"])
+    layer(n - 1, dz, A["D0"])
+    gD = {0: q(conv_t(dz, Wm[n - 1][0]) * (A["D0"] > 0))}
+    gU = {}
+    for l in range(0, L - 1):
+        layer(dec(l), gD[l], A[f"U{l}"])
+        gU[l] = q(conv_t(gD[l], Wm[dec(l)][0]))
+        if l + 1 == L - 1:
+            gB = q(down_sum(gU[l]) * (A["B"] > 0))
+        else:
+            gD[l + 1] = q(down_sum(gU[l]) * (A[f"D{l + 1}"] > 0))
+    l = L - 1
+    layer(bot + 1, gB, A[f"A{l}"])
+    gA = q(conv_t(gB, Wm[bot + 1][0]) * (A[f"A{l}"] > 0))
+    layer(bot, gA, A[f"P{l}"])
+    gP = q(conv_t(gA, Wm[bot][0]))
+    for l in range(L - 2, -1, -1):
+        gS = q((gU[l] + up(gP) / 8.0) * (A[f"S{l}"] > 0))
+        layer(eb(l), gS, A[f"A{l}"])
+        gA = q(conv_t(gS, Wm[eb(l)][0]) * (A[f"A{l}"] > 0))
+        if l > 0:
+            layer(ea(l), gA, A[f"P{l}"])
+            gP = q(conv_t(gA, Wm[ea(l)][0]))
+        else:
+            layer(ea(0), gA, A["x0"])
     return pack(G)
 
 

@@ -27,6 +27,11 @@
 #include "ctx.hpp"
 #include "fcg.hpp"
 
+// truncation of the FCG recurrence used with the learned preconditioner: the
+// new direction is A-orthogonalised against the last SULI_FCG_M directions,
+// which keeps the method convergent for a nonlinear preconditioner
+#define SULI_FCG_M 4
+
 // ---------------------------------------------------------------------------
 // device control block
 struct FcgCtl {
@@ -41,6 +46,10 @@ struct FcgCtl {
   int active;        // inner loop running
   int it;            // inner iterations of this pass
   int first;         // next K1 is the first of the pass (beta = 0)
+  // FCG(m) with the learned preconditioner: the last SULI_FCG_M directions
+  int nk;                    // directions held (0..SULI_FCG_M)
+  double zqm[SULI_FCG_M];    // z.q_j of the current z
+  double pqm[SULI_FCG_M];    // p_j.q_j of the held directions
 };
 
 #define CTL_PTR(c, member) ((double *)((char *)(c) + offsetof(FcgCtl, member)))
@@ -143,6 +152,7 @@ struct EpiR32 {  // arm the inner loop: stop when r.r <= rel_in^2 r0.r0
     ctl->maxit_in = maxit;
     ctl->it = 0;
     ctl->first = 1;
+    ctl->nk = 0;
     ctl->active = res[0] > 0 ? 1 : 0;
   }
 };
@@ -256,6 +266,80 @@ __global__ __launch_bounds__(256) void k_zq(Grid g, const float *__restrict__ z,
   block_reduce<1>(acc, lds);
   block_partial<1>(acc, rs.partials);
 }
+
+// ---- FCG(m) steps of the learned preconditioner ---------------------------------
+// LZ: s = A_s z; sums z.q_j over the held directions (slots j < nk)
+__global__ __launch_bounds__(256) void k_fcgm_lz(Grid g, const float *__restrict__ ax, const float *__restrict__ ay,
+                                                 const float *__restrict__ az, const float *__restrict__ z,
+                                                 float *__restrict__ sv, const float *__restrict__ q0,
+                                                 const float *__restrict__ q1, const float *__restrict__ q2,
+                                                 const float *__restrict__ q3, FcgCtl *ctl, RedSlot rs) {
+  if (!ctl->active) return;
+  CELL_IJK;
+  const int nk = ctl->nk;
+  const float *qs[SULI_FCG_M] = {q0, q1, q2, q3};
+  double acc[SULI_FCG_M] = {0.0, 0.0, 0.0, 0.0};
+  if (i <= g.nx && j <= g.ny)
+    for (int k = kz0; k <= g.nz; k += gridDim.z) {
+      const int64_t c = g.idx(i, j, k);
+      sv[c] = apply_As<float, float, float>(g, c, i, j, k, ax, ay, az, z);
+      const double zc = z[c];
+#pragma unroll
+      for (int m = 0; m < SULI_FCG_M; m++)
+        if (m < nk) acc[m] += zc * (double)qs[m][c];
+    }
+  __shared__ double lds[SULI_FCG_M * SULI_LDS_RED];
+  block_reduce<SULI_FCG_M>(acc, lds);
+  block_partial<SULI_FCG_M>(acc, rs.partials);
+}
+
+// LP: p = z - sum_j (z.q_j/p_j.q_j) p_j, q = s - sum_j (...) q_j, written
+// into slot (pw, qw), which is the oldest held direction once all slots are
+// in use (each element is read before it is overwritten); sums p.q, p.r
+__global__ __launch_bounds__(256) void k_fcgm_lp(Grid g, const float *__restrict__ z, const float *__restrict__ sv,
+                                                 const float *__restrict__ r, const float *p0, const float *p1,
+                                                 const float *p2, const float *p3, const float *q0, const float *q1,
+                                                 const float *q2, const float *q3, float *pw, float *qw, FcgCtl *ctl,
+                                                 RedSlot rs) {
+  if (!ctl->active) return;
+  CELL_IJK;
+  const int nk = ctl->nk;
+  const float *ps[SULI_FCG_M] = {p0, p1, p2, p3};
+  const float *qs[SULI_FCG_M] = {q0, q1, q2, q3};
+  float beta[SULI_FCG_M];
+#pragma unroll
+  for (int m = 0; m < SULI_FCG_M; m++) beta[m] = m < nk ? (float)(ctl->zqm[m] / ctl->pqm[m]) : 0.f;
+  double acc[2] = {0.0, 0.0};
+  if (i <= g.nx && j <= g.ny)
+    for (int k = kz0; k <= g.nz; k += gridDim.z) {
+      const int64_t c = g.idx(i, j, k);
+      float pv = z[c], qv = sv[c];
+#pragma unroll
+      for (int m = 0; m < SULI_FCG_M; m++)
+        if (m < nk) {
+          pv -= beta[m] * ps[m][c];
+          qv -= beta[m] * qs[m][c];
+        }
+      pw[c] = pv;
+      qw[c] = qv;
+      acc[0] += (double)pv * (double)qv;
+      acc[1] += (double)pv * (double)r[c];
+    }
+  __shared__ double lds[2 * SULI_LDS_RED];
+  block_reduce<2>(acc, lds);
+  block_partial<2>(acc, rs.partials);
+}
+struct EpiLP {
+  FcgCtl *ctl;
+  int slot;
+  __device__ bool enabled() const { return ctl->active != 0; }
+  __device__ void operator()(const double *res) const {
+    ctl->k1[0] = res[0];
+    ctl->k1[1] = res[1];
+    ctl->pqm[slot] = res[0];
+    if (ctl->nk < SULI_FCG_M) ctl->nk += 1;
+  }
+};
 
 // ---- geometric multigrid on the unscaled operator ------------------------------
 // Level l: cells 1..n, faces 1..n+1 in a frame of its own; face conductances
@@ -502,6 +586,7 @@ struct PoissonFCG {
   std::vector<MgLevel> lv;  // lv[0] shares the main frame
   int kz = 1;               // grid.z of the k-looped reduction kernels
   UNetPrecond *unet = nullptr;
+  float *pm[SULI_FCG_M] = {}, *qm[SULI_FCG_M] = {};  // FCG(m) directions and their images
 };
 
 static int falloc(Ctx &c, float **p, int64_t n) {
@@ -685,6 +770,34 @@ static void apply_precond(Ctx &c, int pc, const float *r, float *z, const int *a
 // falls back to multigrid
 static constexpr int kLearnedPassCap = 64;
 
+// One FCG(m) iteration with the learned preconditioner; the new direction
+// goes to slot `slot` of the ring of held directions.
+static void fcgm_iteration(Ctx &c, PoissonFCG &f, int slot, dim3 gk, const int *active) {
+  const Grid &g = c.g;
+  const dim3 b2(SULI_BX, SULI_BY, 1);
+  unet_apply(c, f.unet, f.r32, f.z32, active);
+  {
+    ProfScope ps(c, "fcgm_spmv_dots");
+    const RedSlot rz{c.partials, CTL_PTR(f.ctl, zqm)};
+    hipLaunchKernelGGL(k_fcgm_lz, gk, b2, 0, c.st, g, f.ax32, f.ay32, f.az32, f.z32, f.q32, f.qm[0], f.qm[1],
+                       f.qm[2], f.qm[3], f.ctl, rz);
+    launch_finish<SULI_FCG_M>(c.st, gk, rz, EpiActive{f.ctl});
+  }
+  {
+    ProfScope ps(c, "fcgm_orthogonalise");
+    const RedSlot rp{c.partials, CTL_PTR(f.ctl, k1)};
+    hipLaunchKernelGGL(k_fcgm_lp, gk, b2, 0, c.st, g, f.z32, f.q32, f.r32, f.pm[0], f.pm[1], f.pm[2], f.pm[3],
+                       f.qm[0], f.qm[1], f.qm[2], f.qm[3], f.pm[slot], f.qm[slot], f.ctl, rp);
+    launch_finish<2>(c.st, gk, rp, EpiLP{f.ctl, slot});
+  }
+  {
+    ProfScope ps(c, "fcg_update");
+    const RedSlot r2{c.partials, CTL_PTR(f.ctl, k2)};
+    hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.pm[slot], f.qm[slot], f.r32, f.d32, f.ctl, r2);
+    launch_finish<2>(c.st, gk, r2, EpiK2{f.ctl});
+  }
+}
+
 int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   PoissonFCG &f = *c.fcg;
   const Grid &g = c.g;
@@ -704,7 +817,11 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     launch_finish<1>(c.st, gk, rb);
   }
   if (mg_setup(c)) return 1;
-  if (pc == SULI_PRECOND_LEARNED && unet_prepare(c, f.unet)) return 1;
+  if (pc == SULI_PRECOND_LEARNED) {
+    if (unet_prepare(c, f.unet)) return 1;
+    for (int m = 0; m < SULI_FCG_M; m++)
+      if (!f.pm[m] && (falloc(c, &f.pm[m], g.total) || falloc(c, &f.qm[m], g.total))) return 1;
+  }
   int outer = 0, inner_total = 0, fallbacks = 0;
   double res0 = -1, res = 0, bnorm = 0;
   bool conv = false;
@@ -742,10 +859,16 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rin, pass_max, r3);
       launch_finish<1>(c.st, gk, r3, EpiR32{f.ctl, rin, pass_max});
     }
-    unet_capture(c, f.unet, f.r32);
+    unet_capture(c, f.unet, f.r32, active, 0);
     int chunk = use_pc ? 4 : 16;
+    int iter = 0;
     for (;;) {
-      for (int n = 0; n < chunk; n++) {
+      for (int n = 0; n < chunk; n++, iter++) {
+        if (iter > 0) unet_capture(c, f.unet, f.r32, active, iter);
+        if (pc == SULI_PRECOND_LEARNED) {
+          fcgm_iteration(c, f, iter % SULI_FCG_M, gk, active);
+          continue;
+        }
         const float *zin = f.r32;
         if (use_pc) {
           apply_precond(c, pc, f.r32, f.z32, active);

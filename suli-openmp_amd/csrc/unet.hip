@@ -1,15 +1,25 @@
-// Learned preconditioner for the projection's flexible CG: a 3-level 3-D
-// convolutional U-Net (16/32/64 channels, 181k parameters) applied to the
-// scaled residual r_s and the log-density, z = ||r|| Net(r/||r||, log rho).
-// Training and inference use this library's own kernels (unet_kernels.hpp):
-// BF16 implicit-GEMM convolutions on MFMA with FP32 accumulation for
-// forward, data gradient and weight gradient, FP32 master weights, Adam.
+// Learned preconditioner for the projection's flexible CG: a 3-D
+// convolutional U-Net applied to the scaled residual r_s and the
+// log-density, z = rms(r) Net(r/rms(r), log rho).  Training and inference use
+// this library's own kernels (unet_kernels.hpp): BF16 implicit-GEMM
+// convolutions on MFMA with FP32 accumulation for forward, data gradient and
+// weight gradient, FP32 master weights, Adam.
 //
-// Architecture (channels; c1a and c6 are FP32 direct kernels):
-//   x0(2) -c1a-> a1(16) -c1b-> s1(16) -pool-> p1(16) -c2a-> a2(32) -c2b-> s2(32)
-//   -pool-> p2(32) -c3a-> a3(64) -c3b-> b3(32) -up,+s2-> u2(32) -c4-> a4(16)
-//   -up,+s1-> u1(16) -c5-> a5(16) -c6-> z(1)
-// 3x3x3 convolutions with bias and ReLU except c6 (linear).
+// Architecture with L levels (default 3) and C_l = min(16*2^l, 64) channels
+// at level l (grid halved per level); all convolutions 3x3x3 with bias and
+// ReLU except the output layer:
+//   encoder  l = 0:        A0 = conv(x0: 2 -> C0)  S0 = conv(A0: C0 -> C0)
+//            l = 1..L-2:   P_l = pool(S_{l-1})  A_l = conv(P_l)  S_l = conv(A_l)
+//   bottom   l = L-1:      P = pool(S_{L-2})  A = conv(P: C_{L-2} -> C_{L-1})
+//                          B = conv(A: C_{L-1} -> C_{L-2})
+//   decoder  l = L-2..1:   U_l = up(prev) + S_l  D_l = conv(U_l: C_l -> C_{l-1})
+//            l = 0:        U_0 = up(prev) + S_0  D_0 = conv(U_0: C0 -> C0)
+//   output                 z = conv(D_0: C0 -> 1), no activation
+// (pool: 2x2x2 average; up: nearest).  With L = 3 that is 16/32/64 channels
+// and 181,233 parameters.  The first and last layers run as FP32 direct
+// kernels, the others on MFMA.  Weight vector order: encoder (a, b per level),
+// bottom (a, b), decoder from level L-2 down to 0, output; each layer's
+// kernel W[27][c_out][c_in] then bias[c_out].
 //
 // Training loss on snapshots (r, rho) captured from the solver's own solves:
 // the relative residual left after one preconditioned step with the optimal
@@ -29,36 +39,70 @@
 
 namespace {
 
+constexpr int kMaxLevels = 6;
+constexpr int kMaxLayers = 3 * kMaxLevels;
+
 struct LayerDesc {
   int ci, co;
 };
-// c1a c1b c2a c2b c3a c3b c4 c5 c6
-const LayerDesc kLayers[9] = {{2, 16}, {16, 16}, {16, 32}, {32, 32}, {32, 64}, {64, 32}, {32, 16}, {16, 16}, {16, 1}};
-
-inline int64_t wcount(int l) { return 27LL * kLayers[l].ci * kLayers[l].co; }
 
 inline dim3 g1(int64_t n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+// layer numbering of an L-level network
+struct Arch {
+  int L = 3;
+  int C[kMaxLevels];
+  int n = 0;
+  LayerDesc lay[kMaxLayers];
+  int ea(int l) const { return 2 * l; }
+  int eb(int l) const { return 2 * l + 1; }
+  int bot_a() const { return 2 * (L - 1); }
+  int bot_b() const { return 2 * (L - 1) + 1; }
+  int dec(int l) const { return 2 * L + (L - 2 - l); }
+  int out() const { return 3 * L - 1; }
+  void set(int levels) {
+    L = levels;
+    for (int l = 0; l < L; l++) C[l] = std::min(16 << l, 64);
+    n = 3 * L;
+    lay[ea(0)] = {2, C[0]};
+    lay[eb(0)] = {C[0], C[0]};
+    for (int l = 1; l <= L - 2; l++) {
+      lay[ea(l)] = {C[l - 1], C[l]};
+      lay[eb(l)] = {C[l], C[l]};
+    }
+    lay[bot_a()] = {C[L - 2], C[L - 1]};
+    lay[bot_b()] = {C[L - 1], C[L - 2]};
+    for (int l = L - 2; l >= 1; l--) lay[dec(l)] = {C[l], C[l - 1]};
+    lay[dec(0)] = {C[0], C[0]};
+    lay[out()] = {C[0], 1};
+  }
+  int64_t wcount(int q) const { return 27LL * lay[q].ci * lay[q].co; }
+};
 
 }  // namespace
 
 struct UNetPrecond {
   int ready = 0;
+  Arch a;
   int64_t nparam = 0;
-  int64_t woff[9], boff[9];
+  int64_t woff[kMaxLayers], boff[kMaxLayers];
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-  uint16_t *wf[9] = {}, *wt[9] = {};
-  // activations / gradients at the allocated size
-  Dims d1{0, 0, 0};
-  int64_t cap1 = 0;  // voxels allocated at level 1
+  uint16_t *wf[kMaxLayers] = {}, *wt[kMaxLayers] = {};
+  // activations and their gradients for a level-0 grid of cap0 voxels
+  Dims d0{0, 0, 0};
+  int64_t cap0 = 0;
   float *x0 = nullptr, *z = nullptr, *dz = nullptr, *part = nullptr, *tmpf = nullptr;
-  uint16_t *a1, *s1, *p1, *a2, *s2, *p2, *a3, *b3, *u2, *a4, *u1, *a5;
-  uint16_t *ga, *gb, *gc, *gd;  // gradient scratch (level-1 sized, 32 channels)
-  double *red = nullptr;        // reduction partials / results
+  uint16_t *A[kMaxLevels] = {}, *S[kMaxLevels] = {}, *P[kMaxLevels] = {}, *U[kMaxLevels] = {}, *D[kMaxLevels] = {},
+           *B = nullptr;
+  uint16_t *gA[kMaxLevels] = {}, *gS[kMaxLevels] = {}, *gP[kMaxLevels] = {}, *gU[kMaxLevels] = {},
+           *gD[kMaxLevels] = {}, *gB = nullptr;
+  double *red = nullptr;  // reduction partials / results
   // training operator (compact grid) and scratch
   float *cx = nullptr, *cy = nullptr, *cz = nullptr, *s = nullptr, *t = nullptr, *rho1 = nullptr;
   // dataset captured from the solver (compact grid)
   Dims dsd{0, 0, 0};
-  int ds_n = 0, ds_max = 0, ds_cap = 0;
+  int ds_max = 0, ds_cap = 0, ds_every = 0;  // armed while ds_max > 0
+  int *ds_cnt = nullptr;                       // samples held (device counter)
   float *ds_r = nullptr, *ds_rho = nullptr;
   double lr1 = 0.0, lr2 = 0.0;  // log-density normalisation: (log rho - lr1)/lr2
   int step = 0;
@@ -119,16 +163,26 @@ __global__ void k_scatter_frame(Grid g, Dims d, const float *__restrict__ zc, co
   z[g.idx(i + 1, j + 1, k + 1)] = rms * zc[v];
 }
 
-// compact copies of the residual and density for the dataset
+// compact copies of the residual and density into sample *cnt of the set
 template <typename T>
 __global__ void k_capture(Grid g, Dims d, const float *__restrict__ r, const T *__restrict__ rho,
-                          float *__restrict__ rs, float *__restrict__ rhos) {
+                          float *__restrict__ rs0, float *__restrict__ rhos0, const int *__restrict__ cnt, int max,
+                          const int *__restrict__ active) {
+  if (active && !*active) return;
+  const int sidx = *cnt;
+  if (sidx >= max) return;
   const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (v >= d.n()) return;
+  float *rs = rs0 + (int64_t)sidx * d.n(), *rhos = rhos0 + (int64_t)sidx * d.n();
   const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
   const int64_t c = g.idx(i + 1, j + 1, k + 1);
   rs[v] = r[c];
   rhos[v] = (float)rho[c];
+}
+
+__global__ void k_capture_bump(int *cnt, int max, const int *__restrict__ active) {
+  if (active && !*active) return;
+  if (threadIdx.x == 0 && *cnt < max) *cnt += 1;
 }
 
 // Inverse face densities of the scaled operator on the compact grid: face
@@ -221,11 +275,31 @@ namespace {
 constexpr int kRedPart = 1 << 16;  // reduction partials; results follow
 constexpr int kResNorm = kRedPart, kResDots = kRedPart + 8;
 
+inline Dims level_dims(Dims d0, int l) { return Dims{d0.nx >> l, d0.ny >> l, d0.nz >> l}; }
+
 template <int CIN, int COUT, int FLAGS>
-void conv(hipStream_t st, Dims d, const uint16_t *in, const uint16_t *w, const float *bias, const uint16_t *gate,
-          uint16_t *out, const int *active = nullptr) {
+void conv_t(hipStream_t st, Dims d, const uint16_t *in, const uint16_t *w, const float *bias, const uint16_t *gate,
+            uint16_t *out, const int *active) {
   hipLaunchKernelGGL((k_conv_mfma<CIN, COUT, FLAGS>), dim3((d.nx + 63) / 64, d.ny, d.nz), dim3(256), 0, st, d, in, w,
                      bias, gate, out, active);
+}
+
+// run-time dispatch over the channel pairs of the architecture
+enum ConvKind { FWD = CONV_BIAS | CONV_RELU, DGRAD = 0, DGRAD_MASK = CONV_MASK };
+template <int FLAGS>
+void conv_ci_co(int ci, int co, hipStream_t st, Dims d, const uint16_t *in, const uint16_t *w, const float *bias,
+                const uint16_t *gate, uint16_t *out, const int *active) {
+#define SULI_CONV_CASE(A, B) \
+  if (ci == A && co == B) return conv_t<A, B, FLAGS>(st, d, in, w, bias, gate, out, active);
+  SULI_CONV_CASE(16, 16) SULI_CONV_CASE(16, 32) SULI_CONV_CASE(32, 16) SULI_CONV_CASE(32, 32)
+  SULI_CONV_CASE(32, 64) SULI_CONV_CASE(64, 32) SULI_CONV_CASE(64, 64)
+#undef SULI_CONV_CASE
+}
+void conv(ConvKind kind, int ci, int co, hipStream_t st, Dims d, const uint16_t *in, const uint16_t *w,
+          const float *bias, const uint16_t *gate, uint16_t *out, const int *active = nullptr) {
+  if (kind == FWD) conv_ci_co<FWD>(ci, co, st, d, in, w, bias, gate, out, active);
+  else if (kind == DGRAD) conv_ci_co<DGRAD>(ci, co, st, d, in, w, bias, gate, out, active);
+  else conv_ci_co<DGRAD_MASK>(ci, co, st, d, in, w, bias, gate, out, active);
 }
 
 // chunking of the weight-gradient reductions: at most 64 chunks of whole
@@ -233,15 +307,22 @@ void conv(hipStream_t st, Dims d, const uint16_t *in, const uint16_t *w, const f
 inline int wchunk(int64_t n) { return (int)(((n + 63) / 64 + 31) / 32 * 32); }
 inline int wnchunk(int64_t n) { return (int)((n + wchunk(n) - 1) / wchunk(n)); }
 
-template <int CIN, int COUT>
-void wgrad(UNetPrecond &u, hipStream_t st, Dims d, const uint16_t *dy, const uint16_t *x, int layer) {
-  const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+// dW, db of layer q from the pre-activation gradient dy (c_out channels) and
+// the layer input x (c_in channels), both BF16 at dims d
+void wgrad(UNetPrecond &u, hipStream_t st, Dims d, const uint16_t *dy, const uint16_t *x, int q) {
+  const int64_t n = d.n();
   const int chunk = wchunk(n), nch = wnchunk(n);
-  hipLaunchKernelGGL((k_wgrad_mfma<CIN, COUT>), dim3(27, nch), dim3(256), 0, st, d, dy, x, u.part, chunk);
-  const int64_t len = wcount(layer);
-  hipLaunchKernelGGL(k_sum_chunks, g1(len), dim3(256), 0, st, u.part, nch, len, u.g + u.woff[layer]);
-  hipLaunchKernelGGL((k_bias_grad<uint16_t>), dim3(nch), dim3(256), 0, st, d, COUT, dy, u.part, chunk);
-  hipLaunchKernelGGL(k_sum_chunks, g1(COUT), dim3(256), 0, st, u.part, nch, (int64_t)COUT, u.g + u.boff[layer]);
+  const int ci = u.a.lay[q].ci, co = u.a.lay[q].co;
+  const dim3 grid(27, nch);
+#define SULI_WG_CASE(A, B)                                                                                    \
+  if (ci == A && co == B) hipLaunchKernelGGL((k_wgrad_mfma<A, B>), grid, dim3(256), 0, st, d, dy, x, u.part, chunk);
+  SULI_WG_CASE(16, 16) SULI_WG_CASE(16, 32) SULI_WG_CASE(32, 16) SULI_WG_CASE(32, 32)
+  SULI_WG_CASE(32, 64) SULI_WG_CASE(64, 32) SULI_WG_CASE(64, 64)
+#undef SULI_WG_CASE
+  const int64_t len = u.a.wcount(q);
+  hipLaunchKernelGGL(k_sum_chunks, g1(len), dim3(256), 0, st, u.part, nch, len, u.g + u.woff[q]);
+  hipLaunchKernelGGL((k_bias_grad<uint16_t>), dim3(nch), dim3(256), 0, st, d, co, dy, u.part, chunk);
+  hipLaunchKernelGGL(k_sum_chunks, g1(co), dim3(256), 0, st, u.part, nch, (int64_t)co, u.g + u.boff[q]);
 }
 
 void finish_sum(hipStream_t st, double *part, int nb, double *res, int nv) {
@@ -262,134 +343,155 @@ int alloc_h(Ctx &c, uint16_t **p, int64_t n) {
   return 0;
 }
 
-// activations and training scratch for a level-1 grid d (dimensions
-// multiples of 4); grown on demand, never shrunk
+bool dims_ok(const Grid &g, int L) {
+  const int m = 1 << (L - 1);
+  return g.nx % m == 0 && g.ny % m == 0 && g.nz % m == 0;
+}
+
+// activations, gradients and training scratch for a level-0 grid d (its
+// dimensions divisible by 2^(L-1)); grown on demand, never shrunk
 int alloc_acts(Ctx &c, UNetPrecond &u, Dims d) {
-  const int64_t n1 = (int64_t)d.nx * d.ny * d.nz;
-  if (n1 <= u.cap1) {
-    u.d1 = d;
+  const int64_t n0 = d.n();
+  if (n0 <= u.cap0) {
+    u.d0 = d;
     return 0;
   }
-  const int64_t n2 = n1 / 8, n3 = n1 / 64;
-  if (alloc_f(c, &u.x0, 2 * n1) || alloc_f(c, &u.z, n1) || alloc_f(c, &u.dz, n1) || alloc_f(c, &u.tmpf, n1)) return 1;
-  uint16_t **l1[] = {&u.a1, &u.s1, &u.u1, &u.a5};
-  for (auto q : l1)
-    if (alloc_h(c, q, 16 * n1)) return 1;
-  uint16_t **l2[] = {&u.p1, &u.a4};
-  for (auto q : l2)
-    if (alloc_h(c, q, 16 * n2)) return 1;
-  uint16_t **l2b[] = {&u.a2, &u.s2, &u.u2};
-  for (auto q : l2b)
-    if (alloc_h(c, q, 32 * n2)) return 1;
-  if (alloc_h(c, &u.p2, 32 * n3) || alloc_h(c, &u.a3, 64 * n3) || alloc_h(c, &u.b3, 32 * n3)) return 1;
-  // gradient scratch: the largest gradient tensor is 16 channels at level 1
-  uint16_t **gs[] = {&u.ga, &u.gb, &u.gc, &u.gd};
-  for (auto q : gs)
-    if (alloc_h(c, q, 16 * n1)) return 1;
-  // face arrays: (n+1) faces along one direction, at most 2 n1 entries
-  if (alloc_f(c, &u.cx, 2 * n1) || alloc_f(c, &u.cy, 2 * n1) || alloc_f(c, &u.cz, 2 * n1) || alloc_f(c, &u.s, n1) ||
-      alloc_f(c, &u.t, n1) || alloc_f(c, &u.rho1, n1))
+  const Arch &a = u.a;
+  const int L = a.L;
+  if (alloc_f(c, &u.x0, 2 * n0) || alloc_f(c, &u.z, n0) || alloc_f(c, &u.dz, n0) || alloc_f(c, &u.tmpf, n0)) return 1;
+  for (int l = 0; l < L; l++) {
+    const int64_t nl = n0 >> (3 * l);
+    const int Cl = a.C[l];
+    const int Cd = l == 0 ? a.C[0] : a.C[l - 1];  // decoder output channels at level l
+    if (alloc_h(c, &u.A[l], Cl * nl) || alloc_h(c, &u.gA[l], Cl * nl)) return 1;
+    if (l >= 1 && (alloc_h(c, &u.P[l], a.C[l - 1] * nl) || alloc_h(c, &u.gP[l], a.C[l - 1] * nl))) return 1;
+    if (l <= L - 2 && (alloc_h(c, &u.S[l], Cl * nl) || alloc_h(c, &u.gS[l], Cl * nl) || alloc_h(c, &u.U[l], Cl * nl) ||
+                       alloc_h(c, &u.gU[l], Cl * nl) || alloc_h(c, &u.D[l], Cd * nl) || alloc_h(c, &u.gD[l], Cd * nl)))
+      return 1;
+  }
+  const int64_t nb = n0 >> (3 * (L - 1));
+  if (alloc_h(c, &u.B, a.C[L - 2] * nb) || alloc_h(c, &u.gB, a.C[L - 2] * nb)) return 1;
+  // face arrays: (n+1) faces along one direction, at most 2 n0 entries
+  if (alloc_f(c, &u.cx, 2 * n0) || alloc_f(c, &u.cy, 2 * n0) || alloc_f(c, &u.cz, 2 * n0) || alloc_f(c, &u.s, n0) ||
+      alloc_f(c, &u.t, n0) || alloc_f(c, &u.rho1, n0))
     return 1;
-  u.cap1 = n1;
-  u.d1 = d;
+  u.cap0 = n0;
+  u.d0 = d;
   return 0;
 }
 
 // forward pass from x0 (already prepared); z_net to u.z
 void forward(Ctx &c, UNetPrecond &u, const int *active) {
   hipStream_t st = c.st;
-  const Dims d1 = u.d1, d2{d1.nx / 2, d1.ny / 2, d1.nz / 2}, d3{d1.nx / 4, d1.ny / 4, d1.nz / 4};
-  const int64_t n1 = (int64_t)d1.nx * d1.ny * d1.nz, n2 = n1 / 8, n3 = n1 / 64;
+  const Arch &a = u.a;
+  const int L = a.L;
   const float *W = u.w;
-  hipLaunchKernelGGL(k_conv_in, g1(n1), dim3(256), 0, st, d1, u.x0, W + u.woff[0], W + u.boff[0], u.a1, active);
-  conv<16, 16, CONV_BIAS | CONV_RELU>(st, d1, u.a1, u.wf[1], W + u.boff[1], nullptr, u.s1, active);
-  hipLaunchKernelGGL(k_pool, g1(n2 * 16), dim3(256), 0, st, d1, 16, u.s1, u.p1, active);
-  conv<16, 32, CONV_BIAS | CONV_RELU>(st, d2, u.p1, u.wf[2], W + u.boff[2], nullptr, u.a2, active);
-  conv<32, 32, CONV_BIAS | CONV_RELU>(st, d2, u.a2, u.wf[3], W + u.boff[3], nullptr, u.s2, active);
-  hipLaunchKernelGGL(k_pool, g1(n3 * 32), dim3(256), 0, st, d2, 32, u.s2, u.p2, active);
-  conv<32, 64, CONV_BIAS | CONV_RELU>(st, d3, u.p2, u.wf[4], W + u.boff[4], nullptr, u.a3, active);
-  conv<64, 32, CONV_BIAS | CONV_RELU>(st, d3, u.a3, u.wf[5], W + u.boff[5], nullptr, u.b3, active);
-  hipLaunchKernelGGL(k_up_add, g1(n2 * 32), dim3(256), 0, st, d2, 32, u.b3, u.s2, u.u2, active);
-  conv<32, 16, CONV_BIAS | CONV_RELU>(st, d2, u.u2, u.wf[6], W + u.boff[6], nullptr, u.a4, active);
-  hipLaunchKernelGGL(k_up_add, g1(n1 * 16), dim3(256), 0, st, d1, 16, u.a4, u.s1, u.u1, active);
-  conv<16, 16, CONV_BIAS | CONV_RELU>(st, d1, u.u1, u.wf[7], W + u.boff[7], nullptr, u.a5, active);
-  hipLaunchKernelGGL(k_conv_out, g1(n1), dim3(256), 0, st, d1, u.a5, W + u.woff[8], W + u.boff[8], u.z, active);
+  const Dims d0 = u.d0;
+  const int q0 = a.ea(0);
+  hipLaunchKernelGGL(k_conv_in, g1(d0.n()), dim3(256), 0, st, d0, u.x0, W + u.woff[q0], W + u.boff[q0], u.A[0],
+                     active);
+  auto fc = [&](int q, Dims d, const uint16_t *in, uint16_t *out) {
+    conv(FWD, a.lay[q].ci, a.lay[q].co, st, d, in, u.wf[q], W + u.boff[q], nullptr, out, active);
+  };
+  fc(a.eb(0), d0, u.A[0], u.S[0]);
+  for (int l = 1; l < L; l++) {
+    const Dims dl = level_dims(d0, l), du = level_dims(d0, l - 1);
+    const int Cin = a.C[l - 1];
+    hipLaunchKernelGGL(k_pool, g1(dl.n() * Cin), dim3(256), 0, st, du, Cin, u.S[l - 1], u.P[l], active);
+    if (l < L - 1) {
+      fc(a.ea(l), dl, u.P[l], u.A[l]);
+      fc(a.eb(l), dl, u.A[l], u.S[l]);
+    } else {
+      fc(a.bot_a(), dl, u.P[l], u.A[l]);
+      fc(a.bot_b(), dl, u.A[l], u.B);
+    }
+  }
+  const uint16_t *prev = u.B;
+  for (int l = L - 2; l >= 0; l--) {
+    const Dims dl = level_dims(d0, l);
+    hipLaunchKernelGGL(k_up_add, g1(dl.n() * a.C[l]), dim3(256), 0, st, dl, a.C[l], prev, u.S[l], u.U[l], active);
+    fc(a.dec(l), dl, u.U[l], u.D[l]);
+    prev = u.D[l];
+  }
+  const int qo = a.out();
+  hipLaunchKernelGGL(k_conv_out, g1(d0.n()), dim3(256), 0, st, d0, u.D[0], W + u.woff[qo], W + u.boff[qo], u.z,
+                     active);
 }
 
 // backward pass from u.dz; gradients into u.g (overwritten)
 void backward(Ctx &c, UNetPrecond &u) {
   hipStream_t st = c.st;
-  const Dims d1 = u.d1, d2{d1.nx / 2, d1.ny / 2, d1.nz / 2}, d3{d1.nx / 4, d1.ny / 4, d1.nz / 4};
-  const int64_t n1 = (int64_t)d1.nx * d1.ny * d1.nz, n2 = n1 / 8, n3 = n1 / 64;
-  const int chunk = wchunk(n1), nch = wnchunk(n1);
-  // c6: z = conv(a5) + b
-  hipLaunchKernelGGL((k_wgrad_small<16, 1, float, uint16_t>), dim3(27, nch), dim3(256), 0, st, d1, u.dz, u.a5, u.part,
-                     chunk);
-  hipLaunchKernelGGL(k_sum_chunks, g1(wcount(8)), dim3(256), 0, st, u.part, nch, wcount(8), u.g + u.woff[8]);
-  hipLaunchKernelGGL((k_bias_grad<float>), dim3(nch), dim3(256), 0, st, d1, 1, u.dz, u.part, chunk);
-  hipLaunchKernelGGL(k_sum_chunks, g1(1), dim3(256), 0, st, u.part, nch, (int64_t)1, u.g + u.boff[8]);
-  uint16_t *da5 = u.ga;
-  hipLaunchKernelGGL(k_conv_out_bwd, g1(n1), dim3(256), 0, st, d1, u.dz, u.w + u.woff[8], u.a5, da5);
-  // c5: a5 = relu(conv(u1))
-  wgrad<16, 16>(u, st, d1, da5, u.u1, 7);
-  uint16_t *du1 = u.gb;
-  conv<16, 16, 0>(st, d1, da5, u.wt[7], nullptr, nullptr, du1);
-  // u1 = up(a4) + s1
-  uint16_t *da4 = u.gc;
-  hipLaunchKernelGGL(k_down_sum, g1(n2 * 16), dim3(256), 0, st, d1, 16, du1, u.a4, da4);
-  // c4: a4 = relu(conv(u2))
-  wgrad<32, 16>(u, st, d2, da4, u.u2, 6);
-  uint16_t *du2 = u.gd;
-  conv<16, 32, 0>(st, d2, da4, u.wt[6], nullptr, nullptr, du2);
-  // u2 = up(b3) + s2
-  uint16_t *db3 = u.ga;
-  hipLaunchKernelGGL(k_down_sum, g1(n3 * 32), dim3(256), 0, st, d2, 32, du2, u.b3, db3);
-  // c3b: b3 = relu(conv(a3))
-  wgrad<64, 32>(u, st, d3, db3, u.a3, 5);
-  uint16_t *da3 = u.gc;
-  conv<32, 64, CONV_MASK>(st, d3, db3, u.wt[5], nullptr, u.a3, da3);
-  // c3a: a3 = relu(conv(p2))
-  wgrad<32, 64>(u, st, d3, da3, u.p2, 4);
-  uint16_t *dp2 = u.ga;
-  conv<64, 32, 0>(st, d3, da3, u.wt[4], nullptr, nullptr, dp2);
-  // p2 = pool(s2) and s2 feeds u2: ds2 = (du2 + up(dp2)/8) where s2 > 0
-  uint16_t *ds2 = u.gc;
-  hipLaunchKernelGGL(k_add_upavg_mask, g1(n2 * 32), dim3(256), 0, st, d2, 32, du2, dp2, u.s2, ds2);
-  // c2b: s2 = relu(conv(a2))
-  wgrad<32, 32>(u, st, d2, ds2, u.a2, 3);
-  uint16_t *da2 = u.gd;
-  conv<32, 32, CONV_MASK>(st, d2, ds2, u.wt[3], nullptr, u.a2, da2);
-  // c2a: a2 = relu(conv(p1))
-  wgrad<16, 32>(u, st, d2, da2, u.p1, 2);
-  uint16_t *dp1 = u.ga;
-  conv<32, 16, 0>(st, d2, da2, u.wt[2], nullptr, nullptr, dp1);
-  // p1 = pool(s1) and s1 feeds u1: ds1 = (du1 + up(dp1)/8) where s1 > 0
-  uint16_t *ds1 = u.gc;
-  hipLaunchKernelGGL(k_add_upavg_mask, g1(n1 * 16), dim3(256), 0, st, d1, 16, du1, dp1, u.s1, ds1);
-  // c1b: s1 = relu(conv(a1))
-  wgrad<16, 16>(u, st, d1, ds1, u.a1, 1);
-  uint16_t *da1 = u.gd;
-  conv<16, 16, CONV_MASK>(st, d1, ds1, u.wt[1], nullptr, u.a1, da1);
-  // c1a: a1 = relu(conv(x0) + b), FP32 input
-  hipLaunchKernelGGL((k_wgrad_small<2, 16, uint16_t, float>), dim3(27, nch), dim3(256), 0, st, d1, da1, u.x0, u.part,
-                     chunk);
-  hipLaunchKernelGGL(k_sum_chunks, g1(wcount(0)), dim3(256), 0, st, u.part, nch, wcount(0), u.g + u.woff[0]);
-  hipLaunchKernelGGL((k_bias_grad<uint16_t>), dim3(nch), dim3(256), 0, st, d1, 16, da1, u.part, chunk);
-  hipLaunchKernelGGL(k_sum_chunks, g1(16), dim3(256), 0, st, u.part, nch, (int64_t)16, u.g + u.boff[0]);
+  const Arch &a = u.a;
+  const int L = a.L;
+  const Dims d0 = u.d0;
+  const int64_t n0 = d0.n();
+  const int chunk = wchunk(n0), nch = wnchunk(n0);
+  auto dg = [&](ConvKind kind, int q, Dims d, const uint16_t *dy, const uint16_t *gate, uint16_t *out) {
+    // data gradient of layer q: the flipped, transposed kernel maps c_out -> c_in
+    conv(kind, a.lay[q].co, a.lay[q].ci, st, d, dy, u.wt[q], nullptr, gate, out);
+  };
+  // output layer: z = conv(D0) + b
+  const int qo = a.out();
+  hipLaunchKernelGGL((k_wgrad_small<16, 1, float, uint16_t>), dim3(27, nch), dim3(256), 0, st, d0, u.dz, u.D[0],
+                     u.part, chunk);
+  hipLaunchKernelGGL(k_sum_chunks, g1(a.wcount(qo)), dim3(256), 0, st, u.part, nch, a.wcount(qo), u.g + u.woff[qo]);
+  hipLaunchKernelGGL((k_bias_grad<float>), dim3(nch), dim3(256), 0, st, d0, 1, u.dz, u.part, chunk);
+  hipLaunchKernelGGL(k_sum_chunks, g1(1), dim3(256), 0, st, u.part, nch, (int64_t)1, u.g + u.boff[qo]);
+  hipLaunchKernelGGL(k_conv_out_bwd, g1(n0), dim3(256), 0, st, d0, u.dz, u.w + u.woff[qo], u.D[0], u.gD[0]);
+  // decoder, top down: D_l = relu(conv(U_l)), U_l = up(prev) + S_l
+  for (int l = 0; l <= L - 2; l++) {
+    const Dims dl = level_dims(d0, l), dn = level_dims(d0, l + 1);
+    const int q = a.dec(l);
+    wgrad(u, st, dl, u.gD[l], u.U[l], q);
+    dg(DGRAD, q, dl, u.gD[l], nullptr, u.gU[l]);
+    const bool bottom = l + 1 == L - 1;
+    hipLaunchKernelGGL(k_down_sum, g1(dn.n() * a.C[l]), dim3(256), 0, st, dl, a.C[l], u.gU[l],
+                       bottom ? u.B : u.D[l + 1], bottom ? u.gB : u.gD[l + 1]);
+  }
+  // bottom: B = relu(conv(A)), A = relu(conv(P))
+  {
+    const int l = L - 1;
+    const Dims dl = level_dims(d0, l);
+    wgrad(u, st, dl, u.gB, u.A[l], a.bot_b());
+    dg(DGRAD_MASK, a.bot_b(), dl, u.gB, u.A[l], u.gA[l]);
+    wgrad(u, st, dl, u.gA[l], u.P[l], a.bot_a());
+    dg(DGRAD, a.bot_a(), dl, u.gA[l], nullptr, u.gP[l]);
+  }
+  // encoder, bottom up: S_l feeds the pool of level l+1 and the skip of U_l
+  for (int l = L - 2; l >= 0; l--) {
+    const Dims dl = level_dims(d0, l);
+    hipLaunchKernelGGL(k_add_upavg_mask, g1(dl.n() * a.C[l]), dim3(256), 0, st, dl, a.C[l], u.gU[l], u.gP[l + 1],
+                       u.S[l], u.gS[l]);
+    wgrad(u, st, dl, u.gS[l], u.A[l], a.eb(l));
+    dg(DGRAD_MASK, a.eb(l), dl, u.gS[l], u.A[l], u.gA[l]);
+    if (l > 0) {
+      wgrad(u, st, dl, u.gA[l], u.P[l], a.ea(l));
+      dg(DGRAD, a.ea(l), dl, u.gA[l], nullptr, u.gP[l]);
+    }
+  }
+  // first layer: A0 = relu(conv(x0) + b), FP32 input
+  const int q0 = a.ea(0);
+  hipLaunchKernelGGL((k_wgrad_small<2, 16, uint16_t, float>), dim3(27, nch), dim3(256), 0, st, d0, u.gA[0], u.x0,
+                     u.part, chunk);
+  hipLaunchKernelGGL(k_sum_chunks, g1(a.wcount(q0)), dim3(256), 0, st, u.part, nch, a.wcount(q0), u.g + u.woff[q0]);
+  hipLaunchKernelGGL((k_bias_grad<uint16_t>), dim3(nch), dim3(256), 0, st, d0, 16, u.gA[0], u.part, chunk);
+  hipLaunchKernelGGL(k_sum_chunks, g1(16), dim3(256), 0, st, u.part, nch, (int64_t)16, u.g + u.boff[q0]);
 }
 
 void refresh_bf16(Ctx &c, UNetPrecond &u) {
-  for (int l = 1; l <= 7; l++)
-    hipLaunchKernelGGL(k_weights_bf16, g1(wcount(l)), dim3(256), 0, c.st, u.w + u.woff[l], kLayers[l].co,
-                       kLayers[l].ci, u.wf[l], u.wt[l]);
+  const Arch &a = u.a;
+  for (int q = 0; q < a.n; q++)
+    if (u.wf[q])
+      hipLaunchKernelGGL(k_weights_bf16, g1(a.wcount(q)), dim3(256), 0, c.st, u.w + u.woff[q], a.lay[q].co,
+                         a.lay[q].ci, u.wf[q], u.wt[q]);
 }
 
-// Loss terms and dL/dz for a compact (r, rho) pair at dims u.d1: dots
+// Loss terms and dL/dz for a compact (r, rho) pair at dims u.d0: dots
 // (r.s, s.s, r.r) with s = A_s z land at red[kResDots..+2], dL/dz in u.dz.
 void loss_and_dz(Ctx &c, UNetPrecond &u, const float *r, const float *rho) {
   hipStream_t st = c.st;
-  const Dims d = u.d1;
-  const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+  const Dims d = u.d0;
+  const int64_t n = d.n();
   const int64_t nf = (int64_t)(d.nx + 1) * d.ny * d.nz + (int64_t)d.nx * (d.ny + 1) * d.nz +
                      (int64_t)d.nx * d.ny * (d.nz + 1);
   const float mx = (float)(1.0 / (c.g.dx * c.g.dx)), my = (float)(1.0 / (c.g.dy * c.g.dy)),
@@ -407,7 +509,29 @@ void loss_and_dz(Ctx &c, UNetPrecond &u, const float *r, const float *rho) {
   hipLaunchKernelGGL(k_train_As, g1(n), dim3(256), 0, st, d, u.cx, u.cy, u.cz, u.t, u.dz);
 }
 
-bool dims_ok(const Grid &g) { return g.nx % 4 == 0 && g.ny % 4 == 0 && g.nz % 4 == 0; }
+// (re)build the parameter storage for an L-level network
+int configure(Ctx &c, UNetPrecond &u, int L) {
+  u.a.set(L);
+  int64_t off = 0;
+  for (int q = 0; q < u.a.n; q++) {
+    u.woff[q] = off;
+    off += u.a.wcount(q);
+    u.boff[q] = off;
+    off += u.a.lay[q].co;
+  }
+  u.nparam = off;
+  if (alloc_f(c, &u.w, off) || alloc_f(c, &u.g, off) || alloc_f(c, &u.m, off) || alloc_f(c, &u.v, off)) return 1;
+  for (int q = 0; q < u.a.n; q++) {
+    u.wf[q] = u.wt[q] = nullptr;
+    if (q == u.a.ea(0) || q == u.a.out()) continue;  // FP32 direct kernels
+    if (alloc_h(c, &u.wf[q], u.a.wcount(q)) || alloc_h(c, &u.wt[q], u.a.wcount(q))) return 1;
+  }
+  // activations are rebuilt for the new architecture on the next use
+  u.cap0 = 0;
+  u.ready = 0;
+  u.step = 0;
+  return 0;
+}
 
 }  // namespace
 
@@ -415,23 +539,13 @@ bool dims_ok(const Grid &g) { return g.nx % 4 == 0 && g.ny % 4 == 0 && g.nz % 4 
 int unet_create(Ctx &c, UNetPrecond **out) {
   UNetPrecond *u = new UNetPrecond();
   *out = u;
-  int64_t off = 0;
-  for (int l = 0; l < 9; l++) {
-    u->woff[l] = off;
-    off += wcount(l);
-    u->boff[l] = off;
-    off += kLayers[l].co;
-  }
-  u->nparam = off;
-  if (alloc_f(c, &u->w, off) || alloc_f(c, &u->g, off) || alloc_f(c, &u->m, off) || alloc_f(c, &u->v, off)) return 1;
-  for (int l = 1; l <= 7; l++)
-    if (alloc_h(c, &u->wf[l], wcount(l)) || alloc_h(c, &u->wt[l], wcount(l))) return 1;
+  if (configure(c, *u, 3)) return 1;
   HIPCHK(hipMalloc((void **)&u->red, (kRedPart + 64) * sizeof(double)));
   c.allocs.push_back(u->red);
   u->lr1 = 0.5 * (std::log(c.ph.rho_f1) + std::log(c.ph.rho_f2));
   u->lr2 = std::max(0.5 * std::fabs(std::log(c.ph.rho_f2) - std::log(c.ph.rho_f1)), 1e-6);
-  // weight-gradient partials: up to 64 chunks x 27 taps x the largest layer (64x32)
-  if (alloc_f(c, &u->part, 64LL * 27 * 64 * 32)) return 1;
+  // weight-gradient partials: up to 64 chunks x 27 taps x the largest layer (64x64)
+  if (alloc_f(c, &u->part, 64LL * 27 * 64 * 64)) return 1;
   return 0;
 }
 
@@ -448,8 +562,9 @@ int unet_prepare(Ctx &c, UNetPrecond *u) {
                    "suli_unet_set_weights)");
     return 1;
   }
-  if (!dims_ok(c.g)) {
-    suli_set_error("learned preconditioner: nx, ny and nz must be multiples of 4");
+  if (!dims_ok(c.g, u->a.L)) {
+    suli_set_error("learned preconditioner: nx, ny and nz must be multiples of 2^(levels-1) = " +
+                   std::to_string(1 << (u->a.L - 1)));
     return 1;
   }
   return alloc_acts(c, *u, Dims{c.g.nx, c.g.ny, c.g.nz});
@@ -458,7 +573,7 @@ int unet_prepare(Ctx &c, UNetPrecond *u) {
 void unet_apply(Ctx &c, UNetPrecond *u, const float *r, float *z, const int *active) {
   const Grid &g = c.g;
   const Dims d{g.nx, g.ny, g.nz};
-  const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+  const int64_t n = d.n();
   ProfScope ps(c, "unet_apply");
   const int64_t plane = (int64_t)((g.nx + SULI_BX - 1) / SULI_BX) * ((g.ny + SULI_BY - 1) / SULI_BY);
   const int kz = (int)std::max<int64_t>(1, std::min<int64_t>(g.nz, 2048 / std::max<int64_t>(plane, 1)));
@@ -475,18 +590,28 @@ void unet_apply(Ctx &c, UNetPrecond *u, const float *r, float *z, const int *act
   hipLaunchKernelGGL(k_scatter_frame, g1(n), dim3(256), 0, c.st, g, d, u->z, u->red + kResNorm, z, active);
 }
 
-void unet_capture(Ctx &c, UNetPrecond *u, const float *r) {
-  if (!u || u->ds_n >= u->ds_max) return;
+void unet_capture(Ctx &c, UNetPrecond *u, const float *r, const int *active, int iter) {
+  if (!u || u->ds_max <= 0) return;
+  if (iter > 0 && (u->ds_every <= 0 || iter % u->ds_every != 0)) return;
   const Grid &g = c.g;
   const Dims d{g.nx, g.ny, g.nz};
-  const int64_t n = (int64_t)d.nx * d.ny * d.nz;
-  if ((int64_t)u->dsd.nx * u->dsd.ny * u->dsd.nz != n) return;
-  float *rs = u->ds_r + (int64_t)u->ds_n * n, *rh = u->ds_rho + (int64_t)u->ds_n * n;
+  const int64_t n = d.n();
   if (c.prec == SULI_PREC_FP64)
-    hipLaunchKernelGGL((k_capture<double>), g1(n), dim3(256), 0, c.st, g, d, r, (const double *)c.rho, rs, rh);
+    hipLaunchKernelGGL((k_capture<double>), g1(n), dim3(256), 0, c.st, g, d, r, (const double *)c.rho, u->ds_r,
+                       u->ds_rho, u->ds_cnt, u->ds_max, active);
   else
-    hipLaunchKernelGGL((k_capture<float>), g1(n), dim3(256), 0, c.st, g, d, r, (const float *)c.rho, rs, rh);
-  u->ds_n++;
+    hipLaunchKernelGGL((k_capture<float>), g1(n), dim3(256), 0, c.st, g, d, r, (const float *)c.rho, u->ds_r,
+                       u->ds_rho, u->ds_cnt, u->ds_max, active);
+  hipLaunchKernelGGL(k_capture_bump, dim3(1), dim3(64), 0, c.st, u->ds_cnt, u->ds_max, active);
+}
+
+// samples held (synchronises the stream)
+static int dataset_count(Ctx &c, UNetPrecond &u, int *n) {
+  *n = 0;
+  if (!u.ds_cnt) return 0;
+  HIPCHK(hipMemcpyAsync(n, u.ds_cnt, sizeof(int), hipMemcpyDeviceToHost, c.st));
+  HIPCHK(hipStreamSynchronize(c.st));
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -500,6 +625,21 @@ void unet_capture(Ctx &c, UNetPrecond *u, const float *r) {
     }                                                        \
   } while (0)
 
+extern "C" int suli_unet_configure(int levels) {
+  NEEDU();
+  Ctx &c = *g_ctx;
+  if (levels < 2 || levels > kMaxLevels) {
+    suli_set_error("suli_unet_configure: levels must be in 2.." + std::to_string(kMaxLevels));
+    return 1;
+  }
+  return configure(c, *fcg_unet(c), levels);
+}
+
+extern "C" int suli_unet_levels(void) {
+  if (!g_ctx || !g_ctx->fcg) return -1;
+  return fcg_unet(*g_ctx)->a.L;
+}
+
 extern "C" int64_t suli_unet_nparams(void) {
   if (!g_ctx || !g_ctx->fcg) return -1;
   return fcg_unet(*g_ctx)->nparam;
@@ -511,12 +651,12 @@ extern "C" int suli_unet_init(uint64_t seed) {
   UNetPrecond &u = *fcg_unet(c);
   std::vector<float> h((size_t)u.nparam, 0.f);
   std::mt19937_64 rng(seed);
-  for (int l = 0; l < 9; l++) {
+  for (int l = 0; l < u.a.n; l++) {
     // He-uniform for the ReLU layers; the linear output layer starts small
-    const double fan_in = 27.0 * kLayers[l].ci;
-    const double a = (l == 8 ? 0.1 : 1.0) * std::sqrt(6.0 / fan_in);
+    const double fan_in = 27.0 * u.a.lay[l].ci;
+    const double a = (l == u.a.out() ? 0.1 : 1.0) * std::sqrt(6.0 / fan_in);
     std::uniform_real_distribution<double> U(-a, a);
-    for (int64_t q = 0; q < wcount(l); q++) h[(size_t)(u.woff[l] + q)] = (float)U(rng);
+    for (int64_t q = 0; q < u.a.wcount(l); q++) h[(size_t)(u.woff[l] + q)] = (float)U(rng);
   }
   HIPCHK(hipMemcpyAsync(u.w, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c.st));
   HIPCHK(hipMemsetAsync(u.m, 0, h.size() * sizeof(float), c.st));
@@ -561,40 +701,65 @@ extern "C" int suli_unet_get_weights(float *w, int64_t n) {
   return 0;
 }
 
-extern "C" int suli_unet_capture(int max_samples) {
+extern "C" int suli_unet_capture(int max_samples, int every) {
   NEEDU();
   Ctx &c = *g_ctx;
   UNetPrecond &u = *fcg_unet(c);
-  if (!dims_ok(c.g)) {
-    suli_set_error("suli_unet_capture: nx, ny and nz must be multiples of 4");
+  if (!dims_ok(c.g, u.a.L)) {
+    suli_set_error("suli_unet_capture: nx, ny and nz must be multiples of 2^(levels-1)");
     return 1;
   }
-  if (max_samples < 0) {
-    suli_set_error("suli_unet_capture: max_samples must be >= 0");
+  if (max_samples < 0 || every < 0) {
+    suli_set_error("suli_unet_capture: max_samples and every must be >= 0");
     return 1;
   }
   if (max_samples == 0) {  // stop capturing, keep the set
-    u.ds_max = u.ds_n;
+    u.ds_max = 0;
     return 0;
   }
   const Dims d{c.g.nx, c.g.ny, c.g.nz};
-  const int64_t n = (int64_t)d.nx * d.ny * d.nz;
-  if (max_samples > u.ds_cap) {
+  const int64_t n = d.n();
+  if (!u.ds_cnt) {
+    HIPCHK(hipMalloc((void **)&u.ds_cnt, sizeof(int)));
+    c.allocs.push_back(u.ds_cnt);
+    HIPCHK(hipMemsetAsync(u.ds_cnt, 0, sizeof(int), c.st));
+  }
+  if (max_samples > u.ds_cap) {  // grow, keeping the samples held
+    int held = 0;
+    if (dataset_count(c, u, &held)) return 1;
+    held = std::min(held, u.ds_cap);
     float *a = nullptr, *b = nullptr;
     if (alloc_f(c, &a, (int64_t)max_samples * n) || alloc_f(c, &b, (int64_t)max_samples * n)) return 1;
+    if (held > 0) {
+      HIPCHK(hipMemcpyAsync(a, u.ds_r, (size_t)held * n * sizeof(float), hipMemcpyDeviceToDevice, c.st));
+      HIPCHK(hipMemcpyAsync(b, u.ds_rho, (size_t)held * n * sizeof(float), hipMemcpyDeviceToDevice, c.st));
+      HIPCHK(hipStreamSynchronize(c.st));
+    }
+    // the old buffers stay in the context's allocation list until suli_finalize
     u.ds_r = a;
     u.ds_rho = b;
     u.ds_cap = max_samples;
   }
   u.dsd = d;
-  u.ds_n = 0;
   u.ds_max = max_samples;
+  u.ds_every = every;
+  return 0;
+}
+
+extern "C" int suli_unet_clear(void) {
+  NEEDU();
+  Ctx &c = *g_ctx;
+  UNetPrecond &u = *fcg_unet(c);
+  u.ds_max = 0;
+  if (u.ds_cnt) HIPCHK(hipMemsetAsync(u.ds_cnt, 0, sizeof(int), c.st));
   return 0;
 }
 
 extern "C" int suli_unet_dataset_size(void) {
   if (!g_ctx || !g_ctx->fcg) return -1;
-  return fcg_unet(*g_ctx)->ds_n;
+  int n = 0;
+  if (dataset_count(*g_ctx, *fcg_unet(*g_ctx), &n)) return -1;
+  return n;
 }
 
 extern "C" int suli_unet_loss_grad(const double *r, const double *rho, double *loss, float *grad, float *z_out) {
@@ -605,8 +770,8 @@ extern "C" int suli_unet_loss_grad(const double *r, const double *rho, double *l
     suli_set_error("suli_unet_loss_grad: no weights loaded (suli_unet_init or suli_unet_set_weights)");
     return 1;
   }
-  if (!dims_ok(c.g)) {
-    suli_set_error("suli_unet_loss_grad: nx, ny and nz must be multiples of 4");
+  if (!dims_ok(c.g, u.a.L)) {
+    suli_set_error("suli_unet_loss_grad: nx, ny and nz must be multiples of 2^(levels-1)");
     return 1;
   }
   const Dims d{c.g.nx, c.g.ny, c.g.nz};
@@ -639,7 +804,9 @@ extern "C" int suli_unet_train(int iters, double lr, uint64_t seed, double *loss
     suli_set_error("suli_unet_train: no weights loaded (suli_unet_init or suli_unet_set_weights)");
     return 1;
   }
-  if (u.ds_n <= 0) {
+  int nset = 0;
+  if (dataset_count(c, u, &nset)) return 1;
+  if (nset <= 0) {
     suli_set_error("suli_unet_train: the training set is empty (suli_unet_capture, then solves)");
     return 1;
   }
@@ -657,7 +824,7 @@ extern "C" int suli_unet_train(int iters, double lr, uint64_t seed, double *loss
   double *rep_d = nullptr;
   HIPCHK(hipMalloc((void **)&rep_d, std::max<size_t>(1, rep.size()) * sizeof(double)));
   for (int it = 0; it < iters; it++) {
-    const int s = (int)(rng() % (uint64_t)u.ds_n);
+    const int s = (int)(rng() % (uint64_t)nset);
     loss_and_dz(c, u, u.ds_r + (int64_t)s * n, u.ds_rho + (int64_t)s * n);
     backward(c, u);
     u.step++;

@@ -97,10 +97,11 @@ def lib():
             "suli_step": ([f64, p(LsStats), p(PoissonStats)], i32), "suli_sync": ([], i32),
             "suli_set_vol_ini": ([f64], i32), "suli_get_volumes": ([p(f64), p(f64)], i32),
             "suli_set_solver": ([i32, i32, f64, i32], i32),
-            "suli_unet_nparams": ([], C.c_int64), "suli_unet_init": ([C.c_uint64], i32),
+            "suli_unet_nparams": ([], C.c_int64), "suli_unet_configure": ([i32], i32), "suli_unet_levels": ([], i32),
+            "suli_unet_init": ([C.c_uint64], i32),
             "suli_unet_set_weights": ([p(C.c_float), C.c_int64], i32),
             "suli_unet_get_weights": ([p(C.c_float), C.c_int64], i32),
-            "suli_unet_capture": ([i32], i32), "suli_unet_dataset_size": ([], i32),
+            "suli_unet_capture": ([i32, i32], i32), "suli_unet_clear": ([], i32), "suli_unet_dataset_size": ([], i32),
             "suli_unet_train": ([i32, f64, C.c_uint64, p(f64)], i32),
             "suli_unet_loss_grad": ([p(f64), p(f64), p(f64), p(C.c_float), p(C.c_float)], i32),
         }
@@ -236,6 +237,12 @@ class Solver:
     def unet_nparams(self):
         return int(lib().suli_unet_nparams())
 
+    def unet_configure(self, levels):
+        _check(lib().suli_unet_configure(levels), "suli_unet_configure")
+
+    def unet_levels(self):
+        return int(lib().suli_unet_levels())
+
     def unet_init(self, seed):
         _check(lib().suli_unet_init(seed), "suli_unet_init")
 
@@ -250,8 +257,11 @@ class Solver:
                "suli_unet_get_weights")
         return w
 
-    def unet_capture(self, max_samples):
-        _check(lib().suli_unet_capture(max_samples), "suli_unet_capture")
+    def unet_capture(self, max_samples, every=0):
+        _check(lib().suli_unet_capture(max_samples, every), "suli_unet_capture")
+
+    def unet_clear(self):
+        _check(lib().suli_unet_clear(), "suli_unet_clear")
 
     def unet_dataset_size(self):
         return int(lib().suli_unet_dataset_size())

@@ -99,12 +99,12 @@ def test_gradient_matches_finite_differences():
     assert 0.0 <= L0 <= 1.0
     for seed in range(3):
         d = np.random.default_rng(100 + seed).standard_normal(w.size)
-        eps = 1e-6
+        eps = 1e-7
         fd = (U.loss(w + eps * d, *args) - U.loss(w - eps * d, *args)) / (2 * eps)
         assert np.isclose(fd, g @ d, rtol=1e-5, atol=1e-9), (fd, g @ d)
     # per layer: a bias and a kernel entry of every layer
     off = 0
-    for ci, co in U.LAYERS:
+    for ci, co in U.arch(3):
         for q in (off + 5 % (27 * ci * co), off + 27 * ci * co):
             e = np.zeros_like(w)
             e[q] = 1e-6
@@ -141,3 +141,19 @@ def test_bf16_forward_close_to_fp64():
     z64, _ = U.forward(w, U.inputs(r, rho, 1.2, 1000.0))
     z16, _ = U.forward(w, U.inputs(r, rho, 1.2, 1000.0, bf=True), bf=True)
     assert np.linalg.norm(z16 - z64) / np.linalg.norm(z64) < 3e-2
+
+
+@pytest.mark.parametrize("L", [2, 4])
+def test_gradient_matches_finite_differences_other_depths(L):
+    nz, ny, nx = 8, 8, 16
+    assert U.levels_of(U.nparams(L)) == L
+    w = np.random.default_rng(L).standard_normal(U.nparams(L)) * 0.1
+    r = RNG.standard_normal((nz, ny, nx))
+    rho = _two_phase_rho(nz, ny, nx)
+    args = (r, rho, 1.2, 1000.0, 0.1, 0.1, 0.1)
+    _, g, _ = U.loss_and_grad(w, *args)
+    for seed in range(2):
+        d = np.random.default_rng(200 + seed).standard_normal(w.size)
+        eps = 1e-7
+        fd = (U.loss(w + eps * d, *args) - U.loss(w - eps * d, *args)) / (2 * eps)
+        assert np.isclose(fd, g @ d, rtol=1e-5, atol=1e-9), (fd, g @ d)
