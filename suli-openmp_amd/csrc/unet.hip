@@ -280,8 +280,13 @@ inline Dims level_dims(Dims d0, int l) { return Dims{d0.nx >> l, d0.ny >> l, d0.
 template <int CIN, int COUT, int FLAGS>
 void conv_t(hipStream_t st, Dims d, const uint16_t *in, const uint16_t *w, const float *bias, const uint16_t *gate,
             uint16_t *out, const int *active) {
-  hipLaunchKernelGGL((k_conv_mfma<CIN, COUT, FLAGS>), dim3((d.nx + 63) / 64, d.ny, d.nz), dim3(256), 0, st, d, in, w,
-                     bias, gate, out, active);
+  if constexpr (CIN == 16 && COUT == 16) {
+    const dim3 grid((d.nx + C16_TX - 1) / C16_TX, (d.ny + C16_TY - 1) / C16_TY, (d.nz + C16_TZ - 1) / C16_TZ);
+    hipLaunchKernelGGL((k_conv16<FLAGS>), grid, dim3(256), 0, st, d, in, w, bias, gate, out, active);
+  } else {
+    hipLaunchKernelGGL((k_conv_mfma<CIN, COUT, FLAGS>), dim3((d.nx + 63) / 64, d.ny, d.nz), dim3(256), 0, st, d, in,
+                       w, bias, gate, out, active);
+  }
 }
 
 // run-time dispatch over the channel pairs of the architecture
@@ -397,7 +402,7 @@ void forward(Ctx &c, UNetPrecond &u, const int *active) {
   for (int l = 1; l < L; l++) {
     const Dims dl = level_dims(d0, l), du = level_dims(d0, l - 1);
     const int Cin = a.C[l - 1];
-    hipLaunchKernelGGL(k_pool, g1(dl.n() * Cin), dim3(256), 0, st, du, Cin, u.S[l - 1], u.P[l], active);
+    hipLaunchKernelGGL(k_pool, g1(dl.n() * Cin / 8), dim3(256), 0, st, du, Cin, u.S[l - 1], u.P[l], active);
     if (l < L - 1) {
       fc(a.ea(l), dl, u.P[l], u.A[l]);
       fc(a.eb(l), dl, u.A[l], u.S[l]);
@@ -409,7 +414,7 @@ void forward(Ctx &c, UNetPrecond &u, const int *active) {
   const uint16_t *prev = u.B;
   for (int l = L - 2; l >= 0; l--) {
     const Dims dl = level_dims(d0, l);
-    hipLaunchKernelGGL(k_up_add, g1(dl.n() * a.C[l]), dim3(256), 0, st, dl, a.C[l], prev, u.S[l], u.U[l], active);
+    hipLaunchKernelGGL(k_up_add, g1(dl.n() * a.C[l] / 8), dim3(256), 0, st, dl, a.C[l], prev, u.S[l], u.U[l], active);
     fc(a.dec(l), dl, u.U[l], u.D[l]);
     prev = u.D[l];
   }
@@ -445,7 +450,7 @@ void backward(Ctx &c, UNetPrecond &u) {
     wgrad(u, st, dl, u.gD[l], u.U[l], q);
     dg(DGRAD, q, dl, u.gD[l], nullptr, u.gU[l]);
     const bool bottom = l + 1 == L - 1;
-    hipLaunchKernelGGL(k_down_sum, g1(dn.n() * a.C[l]), dim3(256), 0, st, dl, a.C[l], u.gU[l],
+    hipLaunchKernelGGL(k_down_sum, g1(dn.n() * a.C[l] / 8), dim3(256), 0, st, dl, a.C[l], u.gU[l],
                        bottom ? u.B : u.D[l + 1], bottom ? u.gB : u.gD[l + 1]);
   }
   // bottom: B = relu(conv(A)), A = relu(conv(P))
@@ -460,7 +465,7 @@ void backward(Ctx &c, UNetPrecond &u) {
   // encoder, bottom up: S_l feeds the pool of level l+1 and the skip of U_l
   for (int l = L - 2; l >= 0; l--) {
     const Dims dl = level_dims(d0, l);
-    hipLaunchKernelGGL(k_add_upavg_mask, g1(dl.n() * a.C[l]), dim3(256), 0, st, dl, a.C[l], u.gU[l], u.gP[l + 1],
+    hipLaunchKernelGGL(k_add_upavg_mask, g1(dl.n() * a.C[l] / 8), dim3(256), 0, st, dl, a.C[l], u.gU[l], u.gP[l + 1],
                        u.S[l], u.gS[l]);
     wgrad(u, st, dl, u.gS[l], u.A[l], a.eb(l));
     dg(DGRAD_MASK, a.eb(l), dl, u.gS[l], u.A[l], u.gA[l]);

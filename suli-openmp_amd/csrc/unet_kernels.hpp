@@ -25,6 +25,22 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   return (uint16_t)(u >> 16);
 }
 
+// eight BF16 values <-> one 16-byte word
+__device__ __forceinline__ void unpack8(const uint4 q, float (&f)[8]) {
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    f[2 * e] = __uint_as_float(w[e] << 16);
+    f[2 * e + 1] = __uint_as_float(w[e] & 0xffff0000u);
+  }
+}
+__device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int e = 0; e < 4; e++) w[e] = (uint32_t)f2bf(f[2 * e]) | ((uint32_t)f2bf(f[2 * e + 1]) << 16);
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 struct Dims {
   int nx, ny, nz;
   __host__ __device__ __forceinline__ int64_t n() const { return (int64_t)nx * ny * nz; }
@@ -104,6 +120,77 @@ __global__ __launch_bounds__(256) void k_conv_mfma(Dims d, const uint16_t *__res
       if (FLAGS & CONV_RELU) val = val > 0.f ? val : 0.f;
       if (FLAGS & CONV_MASK) val = bf2f(gate[v * COUT + co]) > 0.f ? val : 0.f;
       out[v * COUT + co] = f2bf(val);
+    }
+  }
+}
+
+// 16 -> 16 channel convolution, the layers on the finest grid.  A block of
+// 256 threads computes a 64 x 4 x 2 (x, y, z) tile: the input tile with its
+// one-voxel halo (66 x 6 x 4 voxels x 16 channels, 50.7 KB) is staged once
+// in LDS with 16-byte loads, and every wavefront keeps the whole kernel in
+// registers as 27 MFMA B fragments (W[tap][co = lane%16][ci = 4*(lane/16)..]).
+// Wavefront w computes row y0 + w for z0 and z0 + 1, two 16-voxel groups at
+// a time with v_mfma_f32_16x16x16_bf16 (FP32 accumulation); the A fragment of
+// a tap is the 16 voxels shifted by the tap, read from LDS (64 lanes x 8 B,
+// contiguous, no bank conflicts).  Same epilogue as k_conv_mfma.
+constexpr int C16_TX = 64, C16_TY = 4, C16_TZ = 2;
+template <int FLAGS>
+__global__ __launch_bounds__(256) void k_conv16(Dims d, const uint16_t *__restrict__ in,
+                                                const uint16_t *__restrict__ w, const float *__restrict__ bias,
+                                                const uint16_t *__restrict__ gate, uint16_t *__restrict__ out,
+                                                const int *__restrict__ active) {
+  if (active && !*active) return;
+  constexpr int HX = C16_TX + 2, HY = C16_TY + 2, HZ = C16_TZ + 2;
+  __shared__ __attribute__((aligned(16))) uint16_t tile[HZ * HY * HX * 16];
+  const int x0 = blockIdx.x * C16_TX, y0 = blockIdx.y * C16_TY, z0 = blockIdx.z * C16_TZ;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // stage the halo tile: each voxel is two 16-byte chunks
+  constexpr int CPR = HX * 2;  // chunks per (y, z) row
+  for (int e = tid; e < HZ * HY * CPR; e += 256) {
+    const int row = e / CPR, ch = e % CPR;
+    const int zz = row / HY, yy = row % HY, xx = ch >> 1, half = ch & 1;
+    const int gx = x0 - 1 + xx, gy = y0 - 1 + yy, gz = z0 - 1 + zz;
+    uint4 val = make_uint4(0u, 0u, 0u, 0u);
+    if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+      val = *(const uint4 *)(in + (gx + (int64_t)d.nx * (gy + (int64_t)d.ny * gz)) * 16 + half * 8);
+    *(uint4 *)(tile + ((zz * HY + yy) * HX + xx) * 16 + half * 8) = val;
+  }
+  const int m = lane & 15, kg = lane >> 4;
+  bf16x4 b[27];
+#pragma unroll
+  for (int tap = 0; tap < 27; tap++) b[tap] = *(const bf16x4 *)(w + (tap * 16 + m) * 16 + kg * 4);
+  __syncthreads();
+  const int yy = wave;
+  const int y = y0 + yy;
+  for (int zr = 0; zr < C16_TZ; zr++) {
+    const int z = z0 + zr;
+    for (int gq = 0; gq < C16_TX / 16; gq += 2) {
+      f32x4 acc0 = f32x4{0.f, 0.f, 0.f, 0.f}, acc1 = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int tap = 0; tap < 27; tap++) {
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const int base = (((zr + 1 + dz) * HY + (yy + 1 + dy)) * HX + (1 + dx + gq * 16 + m)) * 16 + kg * 4;
+        const bf16x4 a0 = *(const bf16x4 *)(tile + base);
+        const bf16x4 a1 = *(const bf16x4 *)(tile + base + 16 * 16);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b[tap], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b[tap], acc1, 0, 0, 0);
+      }
+      if (y >= d.ny || z >= d.nz) continue;
+      const float bv = (FLAGS & CONV_BIAS) ? bias[m] : 0.f;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const f32x4 acc = h ? acc1 : acc0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int x = x0 + (gq + h) * 16 + kg * 4 + r;
+          if (x >= d.nx) continue;
+          const int64_t v = x + (int64_t)d.nx * (y + (int64_t)d.ny * z);
+          float val = acc[r] + bv;
+          if (FLAGS & CONV_RELU) val = val > 0.f ? val : 0.f;
+          if (FLAGS & CONV_MASK) val = bf2f(gate[v * 16 + m]) > 0.f ? val : 0.f;
+          out[v * 16 + m] = f2bf(val);
+        }
+      }
     }
   }
 }
@@ -237,7 +324,12 @@ __global__ __launch_bounds__(256) void k_conv_in(Dims d, const float *__restrict
     for (int co = 0; co < 16; co++) acc[co] += w[(tap * 16 + co) * 2] * a0 + w[(tap * 16 + co) * 2 + 1] * a1;
   }
 #pragma unroll
-  for (int co = 0; co < 16; co++) out[v * 16 + co] = f2bf(acc[co] > 0.f ? acc[co] : 0.f);
+  for (int h = 0; h < 2; h++) {
+    float f[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) f[e] = acc[8 * h + e] > 0.f ? acc[8 * h + e] : 0.f;
+    *(uint4 *)(out + v * 16 + 8 * h) = pack8(f);
+  }
 }
 
 // Last layer, 16 -> 1 channel, BF16 input, FP32 output (bias, no activation).
@@ -256,7 +348,12 @@ __global__ __launch_bounds__(256) void k_conv_out(Dims d, const uint16_t *__rest
     if (xs < 0 || xs >= d.nx || ys < 0 || ys >= d.ny || zs < 0 || zs >= d.nz) continue;
     const int64_t vs = xs + (int64_t)d.nx * (ys + (int64_t)d.ny * zs);
 #pragma unroll
-    for (int ci = 0; ci < 16; ci++) acc += w[tap * 16 + ci] * bf2f(a[vs * 16 + ci]);
+    for (int h = 0; h < 2; h++) {
+      float f[8];
+      unpack8(*(const uint4 *)(a + vs * 16 + 8 * h), f);
+#pragma unroll
+      for (int e = 0; e < 8; e++) acc += w[tap * 16 + 8 * h + e] * f[e];
+    }
   }
   z[v] = acc;
 }
@@ -281,7 +378,13 @@ __global__ __launch_bounds__(256) void k_conv_out_bwd(Dims d, const float *__res
     for (int ci = 0; ci < 16; ci++) acc[ci] += w[tap * 16 + ci] * g;
   }
 #pragma unroll
-  for (int ci = 0; ci < 16; ci++) da[v * 16 + ci] = f2bf(bf2f(gate[v * 16 + ci]) > 0.f ? acc[ci] : 0.f);
+  for (int h = 0; h < 2; h++) {
+    float gt[8], f[8];
+    unpack8(*(const uint4 *)(gate + v * 16 + 8 * h), gt);
+#pragma unroll
+    for (int e = 0; e < 8; e++) f[e] = gt[e] > 0.f ? acc[8 * h + e] : 0.f;
+    *(uint4 *)(da + v * 16 + 8 * h) = pack8(f);
+  }
 }
 
 // Weight-gradient partials of a small layer (first or last), one tap per
@@ -332,35 +435,51 @@ __global__ __launch_bounds__(256) void k_wgrad_small(Dims d, const TY *__restric
   }
 }
 
+// The resampling kernels below work on eight channels per thread (C is a
+// multiple of 8): element e of the grid-stride index covers channels
+// 8*(e % (C/8)) .. +7 of voxel e / (C/8).
+
 // 2x2x2 average pooling of a C-channel BF16 tensor (fine dims d)
 __global__ void k_pool(Dims d, int C, const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
                        const int *__restrict__ active) {
   if (active && !*active) return;
   const Dims dc{d.nx / 2, d.ny / 2, d.nz / 2};
+  const int C8 = C / 8;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= dc.n() * C) return;
-  const int c = (int)(e % C);
-  const int64_t V = e / C;
+  if (e >= dc.n() * C8) return;
+  const int c = (int)(e % C8) * 8;
+  const int64_t V = e / C8;
   const int I = (int)(V % dc.nx), J = (int)((V / dc.nx) % dc.ny), K = (int)(V / ((int64_t)dc.nx * dc.ny));
-  float s = 0.f;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   for (int q = 0; q < 8; q++) {
     const int i = 2 * I + (q & 1), j = 2 * J + ((q >> 1) & 1), k = 2 * K + (q >> 2);
-    s += bf2f(in[(i + (int64_t)d.nx * (j + (int64_t)d.ny * k)) * C + c]);
+    float f[8];
+    unpack8(*(const uint4 *)(in + (i + (int64_t)d.nx * (j + (int64_t)d.ny * k)) * C + c), f);
+#pragma unroll
+    for (int t = 0; t < 8; t++) s[t] += f[t];
   }
-  out[e] = f2bf(0.125f * s);
+#pragma unroll
+  for (int t = 0; t < 8; t++) s[t] *= 0.125f;
+  *(uint4 *)(out + V * C + c) = pack8(s);
 }
 
 // out = nearest-upsample(coarse) + skip  (fine dims d)
 __global__ void k_up_add(Dims d, int C, const uint16_t *__restrict__ coarse, const uint16_t *__restrict__ skip,
                          uint16_t *__restrict__ out, const int *__restrict__ active) {
   if (active && !*active) return;
+  const int C8 = C / 8;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= d.n() * C) return;
-  const int c = (int)(e % C);
-  const int64_t v = e / C;
+  if (e >= d.n() * C8) return;
+  const int c = (int)(e % C8) * 8;
+  const int64_t v = e / C8;
   const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
   const int64_t V = (i / 2) + (int64_t)(d.nx / 2) * ((j / 2) + (int64_t)(d.ny / 2) * (k / 2));
-  out[e] = f2bf(bf2f(coarse[V * C + c]) + bf2f(skip[e]));
+  float a[8], b[8];
+  unpack8(*(const uint4 *)(coarse + V * C + c), a);
+  unpack8(*(const uint4 *)(skip + v * C + c), b);
+#pragma unroll
+  for (int t = 0; t < 8; t++) a[t] += b[t];
+  *(uint4 *)(out + v * C + c) = pack8(a);
 }
 
 // adjoint of nearest upsampling: coarse[V][c] = sum of the 8 children,
@@ -368,32 +487,47 @@ __global__ void k_up_add(Dims d, int C, const uint16_t *__restrict__ coarse, con
 __global__ void k_down_sum(Dims d, int C, const uint16_t *__restrict__ fine, const uint16_t *__restrict__ gate,
                            uint16_t *__restrict__ out) {
   const Dims dc{d.nx / 2, d.ny / 2, d.nz / 2};
+  const int C8 = C / 8;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= dc.n() * C) return;
-  const int c = (int)(e % C);
-  const int64_t V = e / C;
+  if (e >= dc.n() * C8) return;
+  const int c = (int)(e % C8) * 8;
+  const int64_t V = e / C8;
   const int I = (int)(V % dc.nx), J = (int)((V / dc.nx) % dc.ny), K = (int)(V / ((int64_t)dc.nx * dc.ny));
-  float s = 0.f;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   for (int q = 0; q < 8; q++) {
     const int i = 2 * I + (q & 1), j = 2 * J + ((q >> 1) & 1), k = 2 * K + (q >> 2);
-    s += bf2f(fine[(i + (int64_t)d.nx * (j + (int64_t)d.ny * k)) * C + c]);
+    float f[8];
+    unpack8(*(const uint4 *)(fine + (i + (int64_t)d.nx * (j + (int64_t)d.ny * k)) * C + c), f);
+#pragma unroll
+    for (int t = 0; t < 8; t++) s[t] += f[t];
   }
-  if (gate && !(bf2f(gate[e]) > 0.f)) s = 0.f;
-  out[e] = f2bf(s);
+  if (gate) {
+    float g[8];
+    unpack8(*(const uint4 *)(gate + V * C + c), g);
+#pragma unroll
+    for (int t = 0; t < 8; t++) s[t] = g[t] > 0.f ? s[t] : 0.f;
+  }
+  *(uint4 *)(out + V * C + c) = pack8(s);
 }
 
 // out = (a + coarse(parent)/8) masked by gate > 0: the gradient of a skip
 // tensor that also fed an average pool (fine dims d)
 __global__ void k_add_upavg_mask(Dims d, int C, const uint16_t *__restrict__ a, const uint16_t *__restrict__ coarse,
                                  const uint16_t *__restrict__ gate, uint16_t *__restrict__ out) {
+  const int C8 = C / 8;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= d.n() * C) return;
-  const int c = (int)(e % C);
-  const int64_t v = e / C;
+  if (e >= d.n() * C8) return;
+  const int c = (int)(e % C8) * 8;
+  const int64_t v = e / C8;
   const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
   const int64_t V = (i / 2) + (int64_t)(d.nx / 2) * ((j / 2) + (int64_t)(d.ny / 2) * (k / 2));
-  const float s = bf2f(a[e]) + 0.125f * bf2f(coarse[V * C + c]);
-  out[e] = f2bf(bf2f(gate[e]) > 0.f ? s : 0.f);
+  float fa[8], fc[8], g[8];
+  unpack8(*(const uint4 *)(a + v * C + c), fa);
+  unpack8(*(const uint4 *)(coarse + V * C + c), fc);
+  unpack8(*(const uint4 *)(gate + v * C + c), g);
+#pragma unroll
+  for (int t = 0; t < 8; t++) fa[t] = g[t] > 0.f ? fa[t] + 0.125f * fc[t] : 0.f;
+  *(uint4 *)(out + v * C + c) = pack8(fa);
 }
 
 // BF16 copies of an FP32 weight tensor W[27][CO][CI]: the forward copy and
