@@ -118,16 +118,28 @@ def cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, budget_s):
         hard = resource.getrlimit(resource.RLIMIT_STACK)[1]
         resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
 
-    proc = subprocess.Popen(["taskset", "-c", "0", exe, fin, "-", "bench"], stdout=subprocess.PIPE,
-                            stderr=subprocess.PIPE, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"),
-                            preexec_fn=_big_stack)
+    logf = os.path.join(tmp, "harness.out")
     try:
-        out, err = proc.communicate(timeout=budget_s)
-    except subprocess.TimeoutExpired:
-        proc.kill()
-        out, err = proc.communicate()
+        with open(logf, "w") as fo:
+            proc = subprocess.Popen(["taskset", "-c", "0", exe, fin, "-", "bench"], stdout=fo,
+                                    stderr=subprocess.STDOUT, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"),
+                                    preexec_fn=_big_stack)
+            while True:
+                try:
+                    proc.wait(timeout=15)
+                    break
+                except subprocess.TimeoutExpired:
+                    el = time.time() - t0
+                    progress(f"CPU baseline (reference harness, 1 core) running {el:.0f}s")
+                    if el > budget_s:
+                        proc.kill()
+                        proc.wait()
+                        break
+        out = err = open(logf).read()
     finally:
-        os.remove(fin)
+        for f in (fin, logf):
+            if os.path.exists(f):
+                os.remove(f)
         os.rmdir(tmp)
     wall = time.time() - t0
     import re
@@ -157,6 +169,10 @@ def cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, budget_s):
     return info
 
 
+def progress(msg):
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -165,6 +181,148 @@ def _cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+WEIGHTS_DIR = os.path.join(ROOT, "suli-openmp_amd", "weights")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_256x128x128.json")
+
+
+def learned_weights():
+    """(levels, weights) of the shipped learned preconditioner, or None."""
+    import numpy as np
+    best = None
+    for f in sorted(os.listdir(WEIGHTS_DIR)) if os.path.isdir(WEIGHTS_DIR) else []:
+        if f.endswith(".npy") and os.path.exists(os.path.join(WEIGHTS_DIR, f[:-4] + ".json")):
+            meta = json.load(open(os.path.join(WEIGHTS_DIR, f[:-4] + ".json")))
+            best = (int(meta["levels"]), np.load(os.path.join(WEIGHTS_DIR, f)), f)
+    return best
+
+
+def poisson_comparison(sv, dt, repeats):
+    """ms per solve of the same projection (the first RK stage of the next
+    step, warm-started from the current pressure) with each preconditioner."""
+    import pysuli
+    sv.level_set(dt)
+    sv.contorno(3)
+    sv.visco()
+    sv.convdiff()
+    sv.tempo(1, 0.5 * dt)
+    sv.contorno(2)
+    p0 = sv.download("prd1")
+    out = {}
+    cases = [("jacobi", pysuli.PRECOND_JACOBI), ("gmg", pysuli.PRECOND_GMG)]
+    lw = learned_weights()
+    if lw is not None:
+        levels, w, fname = lw
+        sv.unet_configure(levels)
+        sv.unet_set_weights(w)
+        cases.append(("learned", pysuli.PRECOND_LEARNED))
+    for name, pc in cases:
+        sv.set_solver(pysuli.POISSON_FCG, pc)
+        runs = []
+        for _ in range(repeats):
+            sv.upload("prd1", p0)
+            runs.append(sv.graddin(0.5 * dt))
+        ms = sorted(r["ms"] for r in runs)
+        out[name] = {"ms_per_solve": round(ms[len(ms) // 2], 3), "ms_min": round(ms[0], 3),
+                     "iters": runs[-1]["iters"], "outer": runs[-1]["outer"], "fallbacks": runs[-1]["fallbacks"],
+                     "converged": runs[-1]["converged"], "relres": runs[-1]["res"]}
+    if "learned" in out:
+        out["learned"]["weights"] = f"suli-openmp_amd/weights/{lw[2]} ({lw[0]} levels)"
+        out["plain_over_learned"] = round(out["jacobi"]["ms_per_solve"] / out["learned"]["ms_per_solve"], 3)
+    out["plain_over_gmg"] = round(out["jacobi"]["ms_per_solve"] / out["gmg"]["ms_per_solve"], 3)
+    out["note"] = ("one first-stage projection of the 256x128x128 state after the timed steps, warm-started "
+                   f"from the current pressure, median of {repeats} solves, FP64 relative residual <= 1e-8; "
+                   "plain = Jacobi-FCG (the reference's symmetric scaling)")
+    return out
+
+
+def hbm_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed rocprof counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel), or None."""
+    try:
+        t = json.load(open(TRAFFIC_FILE))
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("bytes_per_launch")
+
+
+# timing scope -> kernel of that scope (the finest-grid dispatches of it)
+PMC_KERNELS = {"mg_gs_l0": "k_mg_gs", "mg_restrict_l0": "k_mg_restrict", "mg_prolong_l0": "k_mg_prolong",
+               "mg_scale": "k_scale", "fcg_spmv_dot": "k_fcg1", "fcg_update": "k_fcg2", "fcg_zq": "k_zq",
+               "fcg_residual64": "k_res64", "fcg_project": "k_r32", "fcg_xupdate": "k_xupd",
+               "convdiff_tempo": "k_momentum", "ls_advect_rk3": "k_ls_adv", "reinit_rk3": "k_reinit_stage",
+               "reinit_sign": "k_reinit_sign", "heaviside_normals": "k_heavi", "curvature": "k_curv",
+               "visco": "k_visco", "posdin": "k_posdin", "est_div": "k_divmax"}
+
+
+def _kernel_ident(name):
+    """'void k_momentum<float>(Grid, ...)' -> 'k_momentum'."""
+    import re
+    n = name[5:] if name.startswith("void ") else name
+    return re.split(r"[<(]", n, maxsplit=1)[0].strip()
+
+
+def _pmc_rows(d):
+    """{dispatch: (kernel, grid, {counter: value})} from a rocprofv3 --pmc csv directory."""
+    import csv
+    import glob
+    out = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            k = out.setdefault(r["Dispatch_Id"], [r["Kernel_Name"], int(r["Grid_Size"]), {}])
+            k[2][r["Counter_Name"]] = k[2].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return out
+
+
+def pmc_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, ncell):
+    """HBM bytes per launch of each timed kernel: rocprof FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) scaled by what they read for a 256 MiB
+    device copy, written to the traffic file read by the benchmark."""
+    cal_bytes = (64 << 20) * 4
+
+    def factor(d, counter):
+        vals = [c[counter] for _, _, c in _pmc_rows(d).values() if c.get(counter, 0) * 1024 > 0.5 * cal_bytes]
+        return cal_bytes / (1024 * sum(vals) / len(vals)), len(vals)
+
+    ff, nf = factor(cal_fetch_dir, "FETCH_SIZE")
+    fw, nw = factor(cal_write_dir, "WRITE_SIZE")
+    fetch, write = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
+    res = {"factor_fetch": ff, "factor_write": fw, "calibration": f"256 MiB torch copy, {nf}/{nw} dispatches",
+           "cells": ncell, "kernels": {}}
+    for scope, kern in PMC_KERNELS.items():
+        def per_launch(rows, counter):
+            ks = [(g, c.get(counter)) for name, g, c in rows.values() if _kernel_ident(name) == kern]
+            ks = [(g, v) for g, v in ks if v is not None]
+            if not ks:
+                return None, 0
+            gmax = max(g for g, _ in ks)
+            v = [x for g, x in ks if g == gmax]
+            return sum(v) / len(v), len(v)
+        fk, n1 = per_launch(fetch, "FETCH_SIZE")
+        wk, n2 = per_launch(write, "WRITE_SIZE")
+        if fk is None or wk is None:
+            continue
+        b = (fk * ff + wk * fw) * 1024
+        res["kernels"][scope] = {"kernel": kern, "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+                                 "bytes_per_launch": round(b), "bytes_per_cell": round(b / ncell, 2),
+                                 "algorithmic_bytes_per_cell": BYTES_PER_CELL.get(scope), "dispatches": [n1, n2]}
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res, indent=1))
+
+
+def pmc_calibrate():
+    """A plain device copy of known size, for scaling the rocprof counters."""
+    import torch
+    n = 64 << 20  # 64 Mi floats = 256 MiB each way
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 5}))
 
 
 def main():
@@ -177,7 +335,19 @@ def main():
     ap.add_argument("--precond", default="gmg", choices=["jacobi", "gmg", "learned"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=180.0)
+    ap.add_argument("--no-poisson", action="store_true", help="skip the preconditioner comparison")
+    ap.add_argument("--poisson-repeats", type=int, default=3)
+    ap.add_argument("--pmc-calibrate", action="store_true",
+                    help="only run a 256 MiB device copy (scale factor of the rocprof byte counters)")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "CAL_FETCH_DIR", "CAL_WRITE_DIR"),
+                    help="summarise rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs into " + TRAFFIC_FILE)
     args = ap.parse_args()
+    if args.pmc_calibrate:
+        pmc_calibrate()
+        return
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary, ncell=args.nx * (args.nx // 2) ** 2)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs the time step on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU slab decomposition is a design, not an implementation)", file=sys.stderr)
@@ -195,10 +365,20 @@ def main():
     L = pysuli.lib()
     vol_ini = sv.init_dambreak(width, height)
     sv.contorno(1)
-    for _ in range(args.warmup):
+    progress(f"{nx}x{ny}x{nz} initialised; {args.warmup} warm-up steps")
+    for _ in range(max(args.warmup - 1, 0)):
         sv.step(dt)
-    sv.sync()
+    # one untimed step with every kernel timed: the per-kernel table and the
+    # choice of the dominant kernel (events around every launch slow a step,
+    # so the timed steps below record events around that kernel only)
     L.suli_prof_enable(1)
+    sv.step(dt)
+    sv.sync()
+    prof = prof_table(L)
+    known = {k: v for k, v in prof.items() if k in BYTES_PER_CELL}
+    dname = max(known.items(), key=lambda kv: kv[1][1])[0]
+    L.suli_prof_enable(1)
+    L.suli_prof_select(dname.encode())
     stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -206,7 +386,8 @@ def main():
         stats.append((ls, ps))
     sv.sync()
     t1 = time.perf_counter()
-    prof = prof_table(L)
+    progress(f"timed {args.steps} steps in {t1 - t0:.3f}s")
+    dcount, dms = prof_table(L).get(dname, (0, 0.0))
     L.suli_prof_enable(0)
     secs = t1 - t0
     ncell = nx * ny * nz
@@ -214,30 +395,31 @@ def main():
     solves = [p for _, ps in stats for p in ps]
     pois_ms = sum(p["ms"] for p in solves) / len(solves)
     pois_it = sum(p["iters"] for p in solves) / len(solves)
-    # dominant kernel and its roofline
-    known = {k: v for k, v in prof.items() if k in BYTES_PER_CELL}
-    dom = max(known.items(), key=lambda kv: kv[1][1])
-    dname, (dcount, dms) = dom
+    # roofline of the dominant kernel over the timed steps
     bpc = BYTES_PER_CELL.get(dname)
     roof = None
     if bpc is not None and dcount > 0:
         avg_s = dms / dcount / 1e3
         achieved = bpc * ncell / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dname,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": hbm_traffic(dname), "kernel": dname,
                 "bytes_per_launch": bpc * ncell, "avg_launch_us": round(avg_s * 1e6, 2), "launches": dcount,
                 "note": "largest fine-grid kernel by total time; coarse multigrid levels (mg_gs, mg_restrict, "
                         "mg_prolong on levels >= 1) are latency-bound and reported under kernels"}
+    vol_now = sv.volumes()[1]
+    maxdiv = sv.est_div()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, args.cpu_budget)
-    vol_now = sv.volumes()[1]
-    maxdiv = sv.est_div()
+    if not args.no_poisson:
+        progress("preconditioner comparison")
+    comparison = None if args.no_poisson else poisson_comparison(sv, dt, args.poisson_repeats)
     sv.close()
-    kernels = {k: {"launches": c, "ms_total": round(t, 3), "share": round(t / secs / 1e3, 4)}
+    step_ms = sum(t for _, t in prof.values())
+    kernels = {k: {"launches": c, "ms_total": round(t, 3), "share_of_timed_kernels": round(t / step_ms, 4)}
                for k, (c, t) in sorted(prof.items(), key=lambda kv: -kv[1][1])}
     line = {
-        "metric": "Mcell-steps/s at 256x128x128",
+        "metric": "Mcell-steps/s at 256x128x128; Poisson ms/solve to 1e-8, learned vs plain",
         "value": round(value, 3),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
@@ -256,11 +438,12 @@ def main():
                     "outer_per_solve": round(sum(p["outer"] for p in solves) / len(solves), 2),
                     "fallbacks": sum(p["fallbacks"] for p in solves),
                     "all_converged": all(p["converged"] for p in solves),
-                    "max_relres": max(p["res"] for p in solves), "precond": args.precond},
+                    "max_relres": max(p["res"] for p in solves), "precond": args.precond,
+                    "comparison": comparison},
         "diagnostics": {"volume_drift": (vol_now - vol_ini) / vol_ini, "max_div": maxdiv},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "kernels": kernels,
+        "kernels": {"note": "one untimed step after the warm-up, every launch timed with HIP events", **kernels},
     }
     print(json.dumps(line))
 

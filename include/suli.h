@@ -152,6 +152,14 @@ int suli_convdiff_forces(void);
 int suli_stage(int tt, double dt_base, suli_poisson_stats *st);
 int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps);
 
+/* Time-dependent horizontal body force (sloshing tanks): convdiff uses
+ * gx(t) = gx + amplitude*sin(2 pi t/period + phase) in place of the constant
+ * gx of 6_convdiff.f90:155, with t the time of the current step.
+ * suli_set_time sets t (the driver's t = it*dt, 1_principal.f90:65);
+ * suli_step advances it by dt before the step. */
+int suli_set_forcing(double amplitude, double period, double phase);
+int suli_set_time(double t);
+
 /* Switch the pressure solver of later suli_graddin calls (the fields stay
  * as they are).  tol_rel <= 0 or max_it <= 0 keeps the current value.  No
  * counterpart in the reference: its solver is fixed at compile time. */
@@ -206,6 +214,9 @@ int suli_unet_loss_grad(const double *r, const double *rho, double *loss, float 
  * every kernel launch while enabled.  suli_prof_enable(1) clears the totals;
  * suli_prof_get returns kernel idx's name, launch count and summed time. */
 int suli_prof_enable(int on);
+/* Time only the scope of this name (null or "": all), so that the events
+ * of the other launches do not slow a timed run; keeps the totals. */
+int suli_prof_select(const char *name);
 int suli_prof_count(void);
 int suli_prof_get(int idx, char *name, int len, int64_t *launches, double *total_ms);
 

@@ -194,9 +194,18 @@ extern "C" int suli_prof_enable(int on) {
   Ctx &c = *g_ctx;
   prof_flush(c);
   c.prof.on = on != 0;
+  c.prof.only.clear();
   c.prof.names.clear();
   c.prof.total_ms.clear();
   c.prof.launches.clear();
+  return 0;
+}
+
+extern "C" int suli_prof_select(const char *name) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  prof_flush(c);
+  c.prof.only = name ? name : "";
   return 0;
 }
 
@@ -548,7 +557,7 @@ extern "C" int suli_convdiff_forces(void) {
   MomPtrs m = mom_ptrs(c);
   m.u0 = c.u; m.v0 = c.v; m.w0 = c.w;  // stage-2 form: writes only the scratch swap buffers
   m.Fu = c.Fu; m.Fv = c.Fv; m.Fw = c.Fw;
-  { ProfScope ps_(c, "convdiff_tempo"); DISPATCH(c, Launch<TT>::momentum(c.g, c.ph, c.st, m, 0.0, 2)); }
+  { ProfScope ps_(c, "convdiff_tempo"); DISPATCH(c, Launch<TT>::momentum(c.g, c.phys_now(), c.st, m, 0.0, 2)); }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -565,7 +574,7 @@ extern "C" int suli_tempo(int tt, double dt) {
     return 1;
   }
   MomPtrs m = mom_ptrs(c);
-  { ProfScope ps_(c, "convdiff_tempo"); DISPATCH(c, Launch<TT>::momentum(c.g, c.ph, c.st, m, dt, tt)); }
+  { ProfScope ps_(c, "convdiff_tempo"); DISPATCH(c, Launch<TT>::momentum(c.g, c.phys_now(), c.st, m, dt, tt)); }
   HIPCHK(hipGetLastError());
   std::swap(c.u, c.un);
   std::swap(c.v, c.vn);
@@ -645,8 +654,28 @@ extern "C" int suli_stage(int tt, double dt_base, suli_poisson_stats *st) {
   return 0;
 }
 
+extern "C" int suli_set_time(double t) {
+  NEEDCTX();
+  g_ctx->t_now = t;
+  return 0;
+}
+
+extern "C" int suli_set_forcing(double amplitude, double period, double phase) {
+  NEEDCTX();
+  if (!(period > 0.0)) {
+    suli_set_error("suli_set_forcing: period must be positive");
+    return 1;
+  }
+  Ctx &c = *g_ctx;
+  c.force_amp = amplitude;
+  c.force_period = period;
+  c.force_phase = phase;
+  return 0;
+}
+
 extern "C" int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps) {
   NEEDCTX();
+  g_ctx->t_now += dt;  // t = it*dt of the step (1_principal.f90:65) when started from t = 0
   int rc;
   if ((rc = suli_level_set(dt, ls))) return rc;
   if ((rc = suli_contorno(3))) return rc;

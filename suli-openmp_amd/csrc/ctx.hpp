@@ -1,5 +1,6 @@
 // Host-side state of the library: one GPU, one stream, every field resident.
 #pragma once
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -15,6 +16,7 @@ struct PoissonFCG;  // poisson_fcg.hip
 // elapsed times are folded into per-kernel totals.
 struct Prof {
   bool on = false;
+  std::string only;  // when set, only the scope of this name is timed
   std::vector<std::string> names;
   std::vector<double> total_ms;
   std::vector<long long> launches;
@@ -30,6 +32,13 @@ struct Ctx {
   suli_params P{};
   Grid g{};
   Phys ph{};
+  // time-dependent horizontal forcing gx(t) = gx + amp*sin(2 pi t/period + phase)
+  double force_amp = 0.0, force_period = 1.0, force_phase = 0.0, t_now = 0.0;
+  Phys phys_now() const {
+    Phys q = ph;
+    if (force_amp != 0.0) q.gx = ph.gx + force_amp * std::sin(2.0 * M_PI * t_now / force_period + force_phase);
+    return q;
+  }
   int prec = SULI_PREC_FP64;
   size_t esz = 8;
   hipStream_t st = nullptr;
@@ -92,7 +101,7 @@ struct ProfScope {
   int id = -1;
   hipEvent_t a = nullptr;
   ProfScope(Ctx &cc, const char *name) : c(cc) {
-    if (c.prof.on) {
+    if (c.prof.on && (c.prof.only.empty() || c.prof.only == name)) {
       id = prof_id(c, name);
       a = prof_event(c);
       (void)hipEventRecord(a, c.st);

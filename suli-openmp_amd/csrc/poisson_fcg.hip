@@ -587,6 +587,7 @@ struct PoissonFCG {
   int kz = 1;               // grid.z of the k-looped reduction kernels
   UNetPrecond *unet = nullptr;
   float *pm[SULI_FCG_M] = {}, *qm[SULI_FCG_M] = {};  // FCG(m) directions and their images
+  int pass_iters[4] = {0, 0, 0, 0};                   // inner iterations of the last solve's passes
 };
 
 static int falloc(Ctx &c, float **p, int64_t n) {
@@ -860,7 +861,12 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       launch_finish<1>(c.st, gk, r3, EpiR32{f.ctl, rin, pass_max});
     }
     unet_capture(c, f.unet, f.r32, active, 0);
-    int chunk = use_pc ? 4 : 16;
+    // Preconditioned passes take few iterations and consecutive solves of a
+    // run take about the same number: enqueue that many first, then two at a
+    // time, so that few iterations are enqueued past the end (those still
+    // cost their launches).  Jacobi passes take hundreds: chunks of 16..64.
+    const int pidx = std::min(outer - 1, 3);
+    int chunk = use_pc ? (f.pass_iters[pidx] > 0 ? f.pass_iters[pidx] : 4) : 16;
     int iter = 0;
     for (;;) {
       for (int n = 0; n < chunk; n++, iter++) {
@@ -897,9 +903,11 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
       HIPCHK(hipStreamSynchronize(c.st));
       if (!f.ctl_h->active) break;
-      if (chunk < 64) chunk *= 2;
+      if (use_pc) chunk = 2;
+      else if (chunk < 64) chunk *= 2;
     }
     inner_total += f.ctl_h->it;
+    if (use_pc) f.pass_iters[pidx] = f.ctl_h->it;
     const double rr_end = f.ctl_h->k2[0];
     const bool finite = rr_end == rr_end && rr_end < 1e300;
     if (finite) {

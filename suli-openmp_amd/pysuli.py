@@ -97,6 +97,9 @@ def lib():
             "suli_step": ([f64, p(LsStats), p(PoissonStats)], i32), "suli_sync": ([], i32),
             "suli_set_vol_ini": ([f64], i32), "suli_get_volumes": ([p(f64), p(f64)], i32),
             "suli_set_solver": ([i32, i32, f64, i32], i32),
+            "suli_set_forcing": ([f64, f64, f64], i32), "suli_set_time": ([f64], i32),
+            "suli_prof_enable": ([i32], i32), "suli_prof_select": ([C.c_char_p], i32),
+            "suli_prof_count": ([], i32), "suli_prof_get": ([i32, C.c_char_p, i32, p(C.c_int64), p(f64)], i32),
             "suli_unet_nparams": ([], C.c_int64), "suli_unet_configure": ([i32], i32), "suli_unet_levels": ([], i32),
             "suli_unet_init": ([C.c_uint64], i32),
             "suli_unet_set_weights": ([p(C.c_float), C.c_int64], i32),
@@ -229,6 +232,12 @@ class Solver:
 
     def sync(self):
         _check(lib().suli_sync(), "suli_sync")
+
+    def set_forcing(self, amplitude, period, phase=0.0):
+        _check(lib().suli_set_forcing(amplitude, period, phase), "suli_set_forcing")
+
+    def set_time(self, t):
+        _check(lib().suli_set_time(t), "suli_set_time")
 
     def set_solver(self, poisson, precond, tol_rel=0.0, max_it=0):
         _check(lib().suli_set_solver(poisson, precond, tol_rel, max_it), "suli_set_solver")

@@ -195,3 +195,23 @@ def test_mixed_precision_step():
     assert errs["velocity"] <= 1e-3, errs
     print("mixed-precision relative errors after 3 steps:", errs)
     sv.close()
+
+
+def test_sloshing_forcing_matches_constant_gx(grid):
+    """gx(t) = amp*sin(2 pi t/period + phase) at a time where the sine is 1
+    must give the forces of the reference with gx = amp (6_convdiff.f90:155)."""
+    import dataclasses
+    amp, period = 0.05 * 9.81, 1.2
+    st = consistent_state(dataclasses.replace(grid, gx=amp), seed=5)
+    sv = gpu_from_state(st, gx=0.0)
+    sv.set_forcing(amp, period, 0.0)
+    sv.set_time(0.25 * period)
+    sv.convdiff_forces()
+    st.run("convdiff")
+    _cmp(sv, st, ["Fu", "Fv", "Fw"], 1e-12, "forcing")
+    sv.set_time(0.75 * period)  # sine -1: the forces move by -2 amp in x only
+    sv.convdiff_forces()
+    Fu = sv.download("Fu")
+    assert np.allclose(Fu, st.arr("Fu") - 2 * amp, rtol=0, atol=1e-12 * max(1.0, np.abs(Fu).max()))
+    assert np.array_equal(sv.download("Fw"), st.arr("Fw"))
+    sv.close()
