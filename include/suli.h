@@ -92,7 +92,11 @@ enum {
   SULI_U0 = 23,    /* u0(nx+1,ny,nz)           */
   SULI_V0 = 24,
   SULI_W0 = 25,
-  SULI_NFIELDS = 26
+  SULI_TFU = 26,   /* tf_u(nx+1,ny,nz): momentum source of convdiff (6_convdiff.f90:156),
+                      the manufactured-solution forcing; zero until uploaded */
+  SULI_TFV = 27,
+  SULI_TFW = 28,
+  SULI_NFIELDS = 29
 };
 
 typedef struct {

@@ -134,7 +134,7 @@ extern "C" int suli_init(const suli_params *p) {
   void **tf[] = {&c->u, &c->v, &c->w, &c->un, &c->vn, &c->wn, &c->u0, &c->v0, &c->w0, &c->ls, &c->lsA,
                  &c->lsB, &c->S, &c->hs, &c->hsx, &c->hsy, &c->hsz, &c->rho, &c->lsnu, &c->modls,
                  &c->kurv, &c->nut, &c->nrx, &c->nry, &c->nrz, &c->ddx, &c->ddy, &c->ddz, &c->Fu,
-                 &c->Fv, &c->Fw};
+                 &c->Fv, &c->Fw, &c->tfu, &c->tfv, &c->tfw};
   for (void **q : tf)
     if (dalloc(*c, q, fb)) return 1;
   double **td[] = {&c->prd1, &c->D, &c->sD, &c->q, &c->ax, &c->ay, &c->az, &c->e, &c->r, &c->pp, &c->mp};
@@ -295,6 +295,9 @@ static bool field_ref(Ctx &c, int f, FieldRef &r) {
     case SULI_U0: r = {K_FX, c.u0, false, false}; return true;
     case SULI_V0: r = {K_FY, c.v0, false, false}; return true;
     case SULI_W0: r = {K_FZ, c.w0, false, false}; return true;
+    case SULI_TFU: r = {K_FX, c.tfu, false, false}; return true;
+    case SULI_TFV: r = {K_FY, c.tfv, false, false}; return true;
+    case SULI_TFW: r = {K_FZ, c.tfw, false, false}; return true;
     default: return false;
   }
 }
@@ -357,6 +360,7 @@ extern "C" int suli_upload(int field, const double *host, int64_t n) {
       for (int i = e.i0; i <= e.i1; i++) fr[(size_t)c.g.idx(i, j, k)] = host[q++];
   if (host_to_frame(c, r.dev, r.f64, fr)) return 1;
   if (field == SULI_U || field == SULI_V || field == SULI_W) c.convdiff_pending = false;
+  if (field == SULI_TFU || field == SULI_TFV || field == SULI_TFW) c.tf_on = true;
   return 0;
 }
 
@@ -545,7 +549,11 @@ static MomPtrs mom_ptrs(Ctx &c) {
   MomPtrs m;
   m.u = c.u; m.v = c.v; m.w = c.w; m.u0 = c.u0; m.v0 = c.v0; m.w0 = c.w0;
   m.rho = c.rho; m.lsnu = c.lsnu; m.nut = c.nut; m.hsx = c.hsx; m.hsy = c.hsy; m.hsz = c.hsz;
-  m.kurv = c.kurv; m.tfu = c.tfu; m.tfv = c.tfv; m.tfw = c.tfw;
+  m.kurv = c.kurv;
+  // momentum sources only once some were uploaded (no traffic otherwise)
+  m.tfu = c.tf_on ? c.tfu : nullptr;
+  m.tfv = c.tf_on ? c.tfv : nullptr;
+  m.tfw = c.tf_on ? c.tfw : nullptr;
   m.un = c.un; m.vn = c.vn; m.wn = c.wn; m.u0o = c.u0; m.v0o = c.v0; m.w0o = c.w0;
   m.Fu = nullptr; m.Fv = nullptr; m.Fw = nullptr;
   return m;

@@ -49,7 +49,8 @@ struct Ctx {
   void *hs = nullptr, *hsx = nullptr, *hsy = nullptr, *hsz = nullptr, *rho = nullptr, *lsnu = nullptr;
   void *modls = nullptr, *kurv = nullptr, *nut = nullptr, *nrx = nullptr, *nry = nullptr, *nrz = nullptr;
   void *ddx = nullptr, *ddy = nullptr, *ddz = nullptr, *Fu = nullptr, *Fv = nullptr, *Fw = nullptr;
-  void *tfu = nullptr, *tfv = nullptr, *tfw = nullptr;  // optional momentum sources
+  void *tfu = nullptr, *tfv = nullptr, *tfw = nullptr;  // momentum sources tf_u, tf_v, tf_w
+  bool tf_on = false;                                    // set by the first upload of a source
   double *prd1 = nullptr;
   // projection (FP64)
   double *D = nullptr, *sD = nullptr, *q = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr;

@@ -283,6 +283,9 @@ void conv_t(hipStream_t st, Dims d, const uint16_t *in, const uint16_t *w, const
   if constexpr (CIN == 16 && COUT == 16) {
     const dim3 grid((d.nx + C16_TX - 1) / C16_TX, (d.ny + C16_TY - 1) / C16_TY, (d.nz + C16_TZ - 1) / C16_TZ);
     hipLaunchKernelGGL((k_conv16<FLAGS>), grid, dim3(256), 0, st, d, in, w, bias, gate, out, active);
+  } else if (d.n() < (int64_t)1 << 18) {  // coarse grids: split the taps over the block's wavefronts
+    hipLaunchKernelGGL((k_conv_mfma_split<CIN, COUT, FLAGS>), dim3((d.nx + 15) / 16, d.ny, d.nz), dim3(256), 0, st, d,
+                       in, w, bias, gate, out, active);
   } else {
     hipLaunchKernelGGL((k_conv_mfma<CIN, COUT, FLAGS>), dim3((d.nx + 63) / 64, d.ny, d.nz), dim3(256), 0, st, d, in,
                        w, bias, gate, out, active);
@@ -393,7 +396,8 @@ void forward(Ctx &c, UNetPrecond &u, const int *active) {
   const float *W = u.w;
   const Dims d0 = u.d0;
   const int q0 = a.ea(0);
-  hipLaunchKernelGGL(k_conv_in, g1(d0.n()), dim3(256), 0, st, d0, u.x0, W + u.woff[q0], W + u.boff[q0], u.A[0],
+  const dim3 tiles((d0.nx + C16_TX - 1) / C16_TX, (d0.ny + C16_TY - 1) / C16_TY, (d0.nz + C16_TZ - 1) / C16_TZ);
+  hipLaunchKernelGGL(k_conv_in_tile, tiles, dim3(256), 0, st, d0, u.x0, W + u.woff[q0], W + u.boff[q0], u.A[0],
                      active);
   auto fc = [&](int q, Dims d, const uint16_t *in, uint16_t *out) {
     conv(FWD, a.lay[q].ci, a.lay[q].co, st, d, in, u.wf[q], W + u.boff[q], nullptr, out, active);
@@ -419,7 +423,7 @@ void forward(Ctx &c, UNetPrecond &u, const int *active) {
     prev = u.D[l];
   }
   const int qo = a.out();
-  hipLaunchKernelGGL(k_conv_out, g1(d0.n()), dim3(256), 0, st, d0, u.D[0], W + u.woff[qo], W + u.boff[qo], u.z,
+  hipLaunchKernelGGL(k_conv_out_tile, tiles, dim3(256), 0, st, d0, u.D[0], W + u.woff[qo], W + u.boff[qo], u.z,
                      active);
 }
 

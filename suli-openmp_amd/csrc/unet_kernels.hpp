@@ -78,6 +78,9 @@ __global__ __launch_bounds__(256) void k_conv_mfma(Dims d, const uint16_t *__res
   f32x4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // unrolled in x so that the three loads of a row are in flight together
+  // (the coarse levels have few waves per CU and are load-latency bound)
+#pragma unroll 3
   for (int tap = 0; tap < 27; tap++) {
     int dx, dy, dz;
     tap_offsets(tap, dx, dy, dz);
@@ -109,6 +112,81 @@ __global__ __launch_bounds__(256) void k_conv_mfma(Dims d, const uint16_t *__res
   }
 #pragma unroll
   for (int t = 0; t < NT; t++) {
+    const int co = t * 16 + m;
+    const float bv = (FLAGS & CONV_BIAS) ? bias[co] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int x = x0 + kg * 4 + r;
+      if (x >= d.nx) continue;
+      const int64_t v = x + (int64_t)d.nx * (j + (int64_t)d.ny * k);
+      float val = acc[t][r] + bv;
+      if (FLAGS & CONV_RELU) val = val > 0.f ? val : 0.f;
+      if (FLAGS & CONV_MASK) val = bf2f(gate[v * COUT + co]) > 0.f ? val : 0.f;
+      out[v * COUT + co] = f2bf(val);
+    }
+  }
+}
+
+// The same convolution for coarse grids, where few wavefronts are in flight
+// and each would run 27*CIN/KC dependent load-MFMA steps: the 4 wavefronts
+// of a block share one 16-voxel group and split the taps (wave w takes taps
+// w, w+4, ...), then add their partial tiles through LDS in a fixed order.
+template <int CIN, int COUT, int FLAGS>
+__global__ __launch_bounds__(256) void k_conv_mfma_split(Dims d, const uint16_t *__restrict__ in,
+                                                         const uint16_t *__restrict__ w,
+                                                         const float *__restrict__ bias,
+                                                         const uint16_t *__restrict__ gate,
+                                                         uint16_t *__restrict__ out, const int *__restrict__ active) {
+  if (active && !*active) return;
+  constexpr int KC = (CIN % 32 == 0) ? 32 : 16;
+  constexpr int KL = KC / 4;
+  constexpr int NT = COUT / 16;
+  __shared__ f32x4 part[3][NT][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * 16;
+  const int j = blockIdx.y, k = blockIdx.z;
+  const int m = lane & 15, kg = lane >> 4;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = wave; tap < 27; tap += 4) {
+    int dx, dy, dz;
+    tap_offsets(tap, dx, dy, dz);
+    const int xs = x0 + m + dx, ys = j + dy, zs = k + dz;
+    const bool ok = xs >= 0 && xs < d.nx && ys >= 0 && ys < d.ny && zs >= 0 && zs < d.nz;
+    const int64_t vs = ok ? (xs + (int64_t)d.nx * (ys + (int64_t)d.ny * zs)) : 0;
+#pragma unroll
+    for (int ch = 0; ch < CIN; ch += KC) {
+      if constexpr (KC == 32) {
+        bf16x8 a;
+        if (ok) a = *(const bf16x8 *)(in + vs * CIN + ch + kg * KL);
+        else a = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const bf16x8 b = *(const bf16x8 *)(w + ((int64_t)(tap * COUT + t * 16 + m) * CIN + ch + kg * KL));
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        }
+      } else {
+        bf16x4 a;
+        if (ok) a = *(const bf16x4 *)(in + vs * CIN + ch + kg * KL);
+        else a = bf16x4{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const bf16x4 b = *(const bf16x4 *)(w + ((int64_t)(tap * COUT + t * 16 + m) * CIN + ch + kg * KL));
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  if (wave > 0)
+#pragma unroll
+    for (int t = 0; t < NT; t++) part[wave - 1][t][lane] = acc[t];
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) acc[t] += part[q][t][lane];
     const int co = t * 16 + m;
     const float bv = (FLAGS & CONV_BIAS) ? bias[co] : 0.f;
 #pragma unroll
@@ -356,6 +434,105 @@ __global__ __launch_bounds__(256) void k_conv_out(Dims d, const uint16_t *__rest
     }
   }
   z[v] = acc;
+}
+
+// The same layer with the input tile staged in LDS (64 x 4 x 2 outputs per
+// block of 256 threads, two voxels per thread; 66 x 6 x 4 halo voxels x 16
+// channels = 50.7 KB): each voxel's 27 x 16 inputs come from LDS instead of
+// 27 separate L1 reads, the kernel taps stay in scalar registers.
+__global__ __launch_bounds__(256) void k_conv_out_tile(Dims d, const uint16_t *__restrict__ a,
+                                                       const float *__restrict__ w, const float *__restrict__ bias,
+                                                       float *__restrict__ z, const int *__restrict__ active) {
+  if (active && !*active) return;
+  constexpr int HX = C16_TX + 2, HY = C16_TY + 2, HZ = C16_TZ + 2, CPR = HX * 2;
+  __shared__ __attribute__((aligned(16))) uint16_t tile[HZ * HY * HX * 16];
+  const int x0 = blockIdx.x * C16_TX, y0 = blockIdx.y * C16_TY, z0 = blockIdx.z * C16_TZ;
+  for (int e = threadIdx.x; e < HZ * HY * CPR; e += 256) {
+    const int row = e / CPR, ch = e % CPR;
+    const int zz = row / HY, yy = row % HY, xx = ch >> 1, half = ch & 1;
+    const int gx = x0 - 1 + xx, gy = y0 - 1 + yy, gz = z0 - 1 + zz;
+    uint4 val = make_uint4(0u, 0u, 0u, 0u);
+    if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+      val = *(const uint4 *)(a + (gx + (int64_t)d.nx * (gy + (int64_t)d.ny * gz)) * 16 + half * 8);
+    *(uint4 *)(tile + ((zz * HY + yy) * HX + xx) * 16 + half * 8) = val;
+  }
+  __syncthreads();
+  const int xo = threadIdx.x & 63, yo = (threadIdx.x >> 6) & 3;
+  float acc[C16_TZ];
+#pragma unroll
+  for (int zr = 0; zr < C16_TZ; zr++) acc[zr] = bias[0];
+  for (int tap = 0; tap < 27; tap++) {
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+    for (int zr = 0; zr < C16_TZ; zr++) {
+      const uint16_t *src = tile + (((zr + 1 + dz) * HY + (yo + 1 + dy)) * HX + (xo + 1 + dx)) * 16;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        float f[8];
+        unpack8(*(const uint4 *)(src + 8 * h), f);
+#pragma unroll
+        for (int e = 0; e < 8; e++) acc[zr] += w[tap * 16 + 8 * h + e] * f[e];
+      }
+    }
+  }
+  const int x = x0 + xo, y = y0 + yo;
+  if (x >= d.nx || y >= d.ny) return;
+#pragma unroll
+  for (int zr = 0; zr < C16_TZ; zr++)
+    if (z0 + zr < d.nz) z[x + (int64_t)d.nx * (y + (int64_t)d.ny * (z0 + zr))] = acc[zr];
+}
+
+// First layer with the FP32 input tile in LDS (same tiling): x0 is 8 B per
+// voxel, 66 x 6 x 4 x 8 B = 12.7 KB.
+__global__ __launch_bounds__(256) void k_conv_in_tile(Dims d, const float *__restrict__ x0,
+                                                      const float *__restrict__ w, const float *__restrict__ bias,
+                                                      uint16_t *__restrict__ out, const int *__restrict__ active) {
+  if (active && !*active) return;
+  constexpr int HX = C16_TX + 2, HY = C16_TY + 2, HZ = C16_TZ + 2;
+  __shared__ float2 tile[HZ * HY * HX];
+  const int bx = blockIdx.x * C16_TX, by = blockIdx.y * C16_TY, bz = blockIdx.z * C16_TZ;
+  for (int e = threadIdx.x; e < HZ * HY * HX; e += 256) {
+    const int xx = e % HX, yy = (e / HX) % HY, zz = e / (HX * HY);
+    const int gx = bx - 1 + xx, gy = by - 1 + yy, gz = bz - 1 + zz;
+    float2 val = make_float2(0.f, 0.f);
+    if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+      val = *(const float2 *)(x0 + 2 * (gx + (int64_t)d.nx * (gy + (int64_t)d.ny * gz)));
+    tile[e] = val;
+  }
+  __syncthreads();
+  const int xo = threadIdx.x & 63, yo = (threadIdx.x >> 6) & 3;
+  const int x = bx + xo, y = by + yo;
+  float acc[C16_TZ][16];
+#pragma unroll
+  for (int zr = 0; zr < C16_TZ; zr++)
+#pragma unroll
+    for (int co = 0; co < 16; co++) acc[zr][co] = bias[co];
+  for (int tap = 0; tap < 27; tap++) {
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    float2 a[C16_TZ];
+#pragma unroll
+    for (int zr = 0; zr < C16_TZ; zr++) a[zr] = tile[((zr + 1 + dz) * HY + (yo + 1 + dy)) * HX + (xo + 1 + dx)];
+#pragma unroll
+    for (int co = 0; co < 16; co++) {
+      const float w0 = w[(tap * 16 + co) * 2], w1 = w[(tap * 16 + co) * 2 + 1];
+#pragma unroll
+      for (int zr = 0; zr < C16_TZ; zr++) acc[zr][co] += w0 * a[zr].x + w1 * a[zr].y;
+    }
+  }
+  if (x >= d.nx || y >= d.ny) return;
+#pragma unroll
+  for (int zr = 0; zr < C16_TZ; zr++) {
+    const int zg = bz + zr;
+    if (zg >= d.nz) continue;
+    const int64_t v = x + (int64_t)d.nx * (y + (int64_t)d.ny * zg);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      float f[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) f[e] = acc[zr][8 * h + e] > 0.f ? acc[zr][8 * h + e] : 0.f;
+      *(uint4 *)(out + v * 16 + 8 * h) = pack8(f);
+    }
+  }
 }
 
 // Backward of the last layer: da[v][ci] = sum_tap W[tap][ci] dz[v - tap],

@@ -22,10 +22,12 @@ POISSON_REF, POISSON_FCG = 0, 1
 PRECOND_JACOBI, PRECOND_GMG, PRECOND_LEARNED = 0, 1, 2
 
 FIELDS = ["u", "v", "w", "prd1", "rho", "ls_nu", "ls", "mod_ls", "kurv", "hs", "hsx", "hsy", "hsz",
-          "ddlsdx", "ddlsdy", "ddlsdz", "nut", "xnut", "ynut", "znut", "Fu", "Fv", "Fw", "u0", "v0", "w0"]
+          "ddlsdx", "ddlsdy", "ddlsdz", "nut", "xnut", "ynut", "znut", "Fu", "Fv", "Fw", "u0", "v0", "w0",
+          "tf_u", "tf_v", "tf_w"]
 FIELD_ID = {n: q for q, n in enumerate(FIELDS)}
 KIND = {"u": "U", "v": "V", "w": "W", "prd1": "P", "xnut": "FX", "ynut": "FY", "znut": "FZ",
-        "Fu": "FX", "Fv": "FY", "Fw": "FZ", "u0": "FX", "v0": "FY", "w0": "FZ"}
+        "Fu": "FX", "Fv": "FY", "Fw": "FZ", "u0": "FX", "v0": "FY", "w0": "FZ", "tf_u": "FX", "tf_v": "FY",
+        "tf_w": "FZ"}
 
 
 def extent_shape(kind: str, nx: int, ny: int, nz: int) -> tuple:
