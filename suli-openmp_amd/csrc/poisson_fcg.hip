@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -923,7 +924,13 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
         fallbacks++;
       }
     } else if (!finite) {
-      suli_set_error("fcg_solve: non-finite residual in the inner iteration");
+      char msg[320];
+      snprintf(msg, sizeof msg,
+               "fcg_solve: non-finite residual in the inner iteration (pass %d, %d iterations, r.r %g, p.q %g, "
+               "p.r %g, r0.r0 %g, outer r.r %g, b.b %g)",
+               outer, f.ctl_h->it, rr_end, f.ctl_h->k1[0], f.ctl_h->k1[1], f.ctl_h->rin0[0], f.ctl_h->rout[0],
+               f.ctl_h->bb[0]);
+      suli_set_error(msg);
       return 1;
     }
     if (inner_total >= maxit_in) {

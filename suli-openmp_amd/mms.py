@@ -22,7 +22,9 @@ continuity:
 The sources are derived with sympy and evaluated at the staggered face
 positions of tf_u(nx+1,ny,nz), tf_v, tf_w; they enter convdiff like the
 reference's tf_u (6_convdiff.f90:156).  Starting from the exact fields, the
-solver runs to a fixed time with dt = 0.1 h (CFL 0.2); the errors of u, v,
+solver runs to a fixed time with dt = min(0.1 h, 0.2 h^2 rho/mu) (CFL 0.2,
+viscous number 0.2; the explicit viscous terms are unstable above 0.25 in
+two directions); the errors of u, v,
 w (faces), p (cells, mean removed) and the interface volume are reported per
 grid with the observed order log2(e(2h)/e(h)).
 
@@ -118,8 +120,10 @@ def run(n, precision, t_end=0.1, tol=1e-10):
         vol0 = s.level_set_ini()
         s.contorno(1)
         s.contorno(3)
-        dt = 0.1 * h
-        steps = max(1, int(round(t_end / dt)))
+        # advective CFL 0.2 (|u| <= 2) and, on fine grids, the explicit
+        # viscous limit (mu/rho) dt/h^2 <= 0.2 of the cross-derivative terms
+        dt = min(0.1 * h, 0.2 * h * h * RHO / MU)
+        steps = max(1, int(math.ceil(t_end / dt)))
         dt = t_end / steps
         t0 = time.time()
         iters = 0
