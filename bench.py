@@ -16,12 +16,23 @@ stages each with visco, convdiff+tempo, contorno, projection, posdin).
 After --warmup untimed steps, --steps steps are timed between two stream
 synchronisations.  value = nx*ny*nz*steps/seconds/1e6 (Mcell-steps/s).
 
-The JSON line also carries the roofline of the dominant kernel (algorithmic
-bytes per launch / its average HIP-event duration on the library's stream
-over the timed region) and a CPU baseline: the flang-built reference
-harness (oracle/_ref) timed on one host core on a bounded sample of the same
-workload (one level_set plus one RK stage from the GPU's post-warmup state;
-per-step time = level_set + 2 x stage).
+The last warm-up step is timed kernel by kernel (HIP events around every
+launch) to pick the dominant fine-grid kernel; during the timed steps only
+that kernel carries events, attached to its dispatches
+(hipExtLaunchKernelGGL), so the other launches run undisturbed.  The JSON
+line carries:
+- roofline: algorithmic bytes per launch of the dominant kernel / its average
+  duration over the timed steps, against 8 TB/s; traffic = HBM bytes per
+  launch from rocprof FETCH_SIZE + WRITE_SIZE (profiles/hbm_traffic_*.json,
+  made by --pmc-summary from counter runs of this script, scaled by a 256 MiB
+  device copy measured the same way);
+- poisson.comparison: ms per solve of one projection state with the plain
+  (Jacobi) FCG, the multigrid FCG and the learned preconditioner
+  (suli-openmp_amd/weights), all to a 1e-8 FP64 relative residual;
+- cpu_baseline: the flang-built reference harness (oracle/_ref) timed on one
+  host core on a bounded sample of the same workload (one level_set plus
+  one RK stage from the GPU's post-warmup state; per-step time = level_set
+  + 2 x stage).
 """
 import argparse
 import json

@@ -1,5 +1,8 @@
 // Host-side state of the library: one GPU, one stream, every field resident.
 #pragma once
+#include <hip/hip_ext.h>
+#include <hip/hip_runtime.h>
+
 #include <cmath>
 #include <string>
 #include <vector>
@@ -117,6 +120,30 @@ struct ProfScope {
     }
   }
 };
+
+// Times one kernel launch exactly: the events go into the launch itself
+// (hipExtLaunchKernelGGL records them at the dispatch's start and end), so
+// the dispatch gap before the kernel is not counted as ProfScope would.
+struct ProfKernel {
+  Ctx &c;
+  int id = -1;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfKernel(Ctx &cc, const char *name) : c(cc) {
+    if (c.prof.on && (c.prof.only.empty() || c.prof.only == name)) {
+      id = prof_id(c, name);
+      a = prof_event(c);
+      b = prof_event(c);
+    }
+  }
+  ~ProfKernel() {
+    if (id >= 0) {
+      c.prof.pending.push_back({id, a, b});
+      if (c.prof.pending.size() > 4096) prof_flush(c);
+    }
+  }
+};
+#define SULI_TIMED_LAUNCH(pk, kernel, grid, block, shmem, stream, ...) \
+  hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, (pk).a, (pk).b, 0, __VA_ARGS__)
 
 // Poisson solvers (poisson_ref.hip, poisson_fcg.hip)
 int poisson_ref_solve(Ctx &c, double dt, suli_poisson_stats *st);

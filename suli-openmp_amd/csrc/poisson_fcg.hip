@@ -709,13 +709,13 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
     for (int s = 0; s < nu1; s++) {
       {
-        ProfScope ps(c, gsn);
-        hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 0,
-                           s == 0 ? 1 : 0, active);
+        ProfKernel pk(c, gsn);
+        SULI_TIMED_LAUNCH(pk, k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x,
+                          0, s == 0 ? 1 : 0, active);
       }
-      ProfScope ps(c, gsn);
-      hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 1,
-                         0, active);
+      ProfKernel pk(c, gsn);
+      SULI_TIMED_LAUNCH(pk, k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 1,
+                        0, active);
     }
     ProfScope ps(c, l == 0 ? "mg_restrict_l0" : "mg_restrict");
     hipLaunchKernelGGL(k_mg_restrict, cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, lf.cx, lf.cy, lf.cz, lf.dg,
@@ -745,12 +745,14 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
       hipLaunchKernelGGL(k_mg_prolong, cells_full(lf.g), b2, 0, c.st, lf.g, lc.g, lc.x, lf.x, active);
     }
     for (int s = 0; s < nu2; s++) {
-      ProfScope ps(c, gsn);
-      hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 1,
-                         0, active);
-      ProfScope ps2(c, gsn);
-      hipLaunchKernelGGL(k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 0,
-                         0, active);
+      {
+        ProfKernel pk(c, gsn);
+        SULI_TIMED_LAUNCH(pk, k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x,
+                          1, 0, active);
+      }
+      ProfKernel pk(c, gsn);
+      SULI_TIMED_LAUNCH(pk, k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 0,
+                        0, active);
     }
   }
   {
@@ -886,10 +888,12 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
           zin = f.z32;
         }
         {
-          ProfScope ps(c, "fcg_spmv_dot");
           const RedSlot r1{c.partials, CTL_PTR(f.ctl, k1)};
-          hipLaunchKernelGGL(k_fcg1, gk, b2, 0, c.st, g, f.ax32, f.ay32, f.az32, zin, f.r32, f.p32, f.p32b, f.q32,
-                             f.ctl, use_pc ? 1 : 0, r1);
+          {
+            ProfKernel pk(c, "fcg_spmv_dot");
+            SULI_TIMED_LAUNCH(pk, k_fcg1, gk, b2, 0, c.st, g, f.ax32, f.ay32, f.az32, zin, (const float *)f.r32,
+                              (const float *)f.p32, f.p32b, f.q32, f.ctl, use_pc ? 1 : 0, r1);
+          }
           launch_finish<2>(c.st, gk, r1, EpiK1{f.ctl});
         }
         std::swap(f.p32, f.p32b);
