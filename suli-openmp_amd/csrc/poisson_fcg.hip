@@ -470,7 +470,12 @@ __global__ __launch_bounds__(256) void k_mg_prolong(Grid gf, Grid gc, const floa
 // one work-group, levels separated by barriers: the coarse end of the
 // hierarchy costs one launch instead of a dozen latency-bound ones.
 #define SULI_MG_MAXLV 12
-#define SULI_MG_SMALL 8192
+#ifndef SULI_MG_SMALL
+#define SULI_MG_SMALL 1024
+#endif
+#ifndef SULI_MG_COARSE_SWEEPS
+#define SULI_MG_COARSE_SWEEPS 24  // Gauss-Seidel sweeps on the coarsest level
+#endif
 struct MgSmall {
   Grid g[SULI_MG_MAXLV];
   float *cx[SULI_MG_MAXLV], *cy[SULI_MG_MAXLV], *cz[SULI_MG_MAXLV], *dg[SULI_MG_MAXLV];
@@ -733,7 +738,7 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
     m.s = S;
     m.L = L;
     m.nu = nu1;
-    m.coarse_sweeps = 24;
+    m.coarse_sweeps = SULI_MG_COARSE_SWEEPS;
     ProfScope ps(c, "mg_small_levels");
     hipLaunchKernelGGL(k_mg_small, dim3(1), dim3(1024), 0, c.st, m, active);
   }
