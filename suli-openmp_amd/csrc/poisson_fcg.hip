@@ -701,6 +701,9 @@ static int mg_setup(Ctx &c) {
 static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
   PoissonFCG &f = *c.fcg;
   const dim3 b2(SULI_BX, SULI_BY, 1);
+#ifndef SULI_MG_NU0
+#define SULI_MG_NU0 2  // sweeps before and after the coarse correction on the finest level
+#endif
   const int nu1 = 2, nu2 = 2;
   const int L = (int)f.lv.size();
   int S = 0;
@@ -712,7 +715,7 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
   for (int l = 0; l < S; l++) {
     MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
-    for (int s = 0; s < nu1; s++) {
+    for (int s = 0; s < (l == 0 ? SULI_MG_NU0 : nu1); s++) {
       {
         ProfKernel pk(c, gsn);
         SULI_TIMED_LAUNCH(pk, k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x,
@@ -749,7 +752,7 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
       ProfScope ps(c, l == 0 ? "mg_prolong_l0" : "mg_prolong");
       hipLaunchKernelGGL(k_mg_prolong, cells_full(lf.g), b2, 0, c.st, lf.g, lc.g, lc.x, lf.x, active);
     }
-    for (int s = 0; s < nu2; s++) {
+    for (int s = 0; s < (l == 0 ? SULI_MG_NU0 : nu2); s++) {
       {
         ProfKernel pk(c, gsn);
         SULI_TIMED_LAUNCH(pk, k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x,
