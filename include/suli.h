@@ -218,9 +218,10 @@ int suli_unet_loss_grad(const double *r, const double *rho, double *loss, float 
  * every kernel launch while enabled.  suli_prof_enable(1) clears the totals;
  * suli_prof_get returns kernel idx's name, launch count and summed time. */
 int suli_prof_enable(int on);
-/* Time only the scope of this name (null or "": all), so that the events
- * of the other launches do not slow a timed run; keeps the totals. */
-int suli_prof_select(const char *name);
+/* Time only the scope of this name (null or "": all), and of it only every
+ * every-th launch (events cost a few microseconds per launch), so that a
+ * timed run is not slowed; keeps the totals. */
+int suli_prof_select(const char *name, int every);
 int suli_prof_count(void);
 int suli_prof_get(int idx, char *name, int len, int64_t *launches, double *total_ms);
 

@@ -195,17 +195,21 @@ extern "C" int suli_prof_enable(int on) {
   prof_flush(c);
   c.prof.on = on != 0;
   c.prof.only.clear();
+  c.prof.every = 1;
+  c.prof.seen = 0;
   c.prof.names.clear();
   c.prof.total_ms.clear();
   c.prof.launches.clear();
   return 0;
 }
 
-extern "C" int suli_prof_select(const char *name) {
+extern "C" int suli_prof_select(const char *name, int every) {
   NEEDCTX();
   Ctx &c = *g_ctx;
   prof_flush(c);
   c.prof.only = name ? name : "";
+  c.prof.every = every > 1 ? every : 1;
+  c.prof.seen = 0;
   return 0;
 }
 

@@ -20,6 +20,9 @@ struct PoissonFCG;  // poisson_fcg.hip
 struct Prof {
   bool on = false;
   std::string only;  // when set, only the scope of this name is timed
+  int every = 1;     // and only every every-th instance of it
+  long long seen = 0;
+  bool sample() { return every <= 1 || (seen++ % every) == 0; }
   std::vector<std::string> names;
   std::vector<double> total_ms;
   std::vector<long long> launches;
@@ -105,7 +108,7 @@ struct ProfScope {
   int id = -1;
   hipEvent_t a = nullptr;
   ProfScope(Ctx &cc, const char *name) : c(cc) {
-    if (c.prof.on && (c.prof.only.empty() || c.prof.only == name)) {
+    if (c.prof.on && (c.prof.only.empty() || c.prof.only == name) && c.prof.sample()) {
       id = prof_id(c, name);
       a = prof_event(c);
       (void)hipEventRecord(a, c.st);
@@ -129,7 +132,7 @@ struct ProfKernel {
   int id = -1;
   hipEvent_t a = nullptr, b = nullptr;
   ProfKernel(Ctx &cc, const char *name) : c(cc) {
-    if (c.prof.on && (c.prof.only.empty() || c.prof.only == name)) {
+    if (c.prof.on && (c.prof.only.empty() || c.prof.only == name) && c.prof.sample()) {
       id = prof_id(c, name);
       a = prof_event(c);
       b = prof_event(c);

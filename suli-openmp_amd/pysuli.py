@@ -100,7 +100,7 @@ def lib():
             "suli_set_vol_ini": ([f64], i32), "suli_get_volumes": ([p(f64), p(f64)], i32),
             "suli_set_solver": ([i32, i32, f64, i32], i32),
             "suli_set_forcing": ([f64, f64, f64], i32), "suli_set_time": ([f64], i32),
-            "suli_prof_enable": ([i32], i32), "suli_prof_select": ([C.c_char_p], i32),
+            "suli_prof_enable": ([i32], i32), "suli_prof_select": ([C.c_char_p, i32], i32),
             "suli_prof_count": ([], i32), "suli_prof_get": ([i32, C.c_char_p, i32, p(C.c_int64), p(f64)], i32),
             "suli_unet_nparams": ([], C.c_int64), "suli_unet_configure": ([i32], i32), "suli_unet_levels": ([], i32),
             "suli_unet_init": ([C.c_uint64], i32),
