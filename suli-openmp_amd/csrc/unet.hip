@@ -263,6 +263,10 @@ __global__ void k_train_dLds(Dims d, const float *__restrict__ r, const float *_
   const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (v >= d.n()) return;
   const double rs = dots[0], ss = dots[1], rr = dots[2];
+  if (!(rr > 0.0) || !(ss > 0.0)) {  // a zero residual or output: no gradient
+    t[v] = 0.f;
+    return;
+  }
   const double a = -2.0 * rs / (rr * ss), b = 2.0 * rs * rs / (rr * ss * ss);
   t[v] = (float)(a * r[v] + b * s[v]);
 }
@@ -830,8 +834,14 @@ extern "C" int suli_unet_train(int iters, double lr, uint64_t seed, double *loss
   // the last min(iters, 32) losses are read back for the report
   const int nrep = std::min(iters, 32);
   std::vector<double> rep((size_t)nrep * 3, 0.0);
-  double *rep_d = nullptr;
-  HIPCHK(hipMalloc((void **)&rep_d, std::max<size_t>(1, rep.size()) * sizeof(double)));
+  struct DevBuf {  // freed on every return path
+    double *p = nullptr;
+    ~DevBuf() {
+      if (p) (void)hipFree(p);
+    }
+  } rep_buf;
+  HIPCHK(hipMalloc((void **)&rep_buf.p, std::max<size_t>(1, rep.size()) * sizeof(double)));
+  double *rep_d = rep_buf.p;
   for (int it = 0; it < iters; it++) {
     const int s = (int)(rng() % (uint64_t)nset);
     loss_and_dz(c, u, u.ds_r + (int64_t)s * n, u.ds_rho + (int64_t)s * n);
@@ -847,7 +857,6 @@ extern "C" int suli_unet_train(int iters, double lr, uint64_t seed, double *loss
   }
   if (nrep > 0) HIPCHK(hipMemcpyAsync(rep.data(), rep_d, rep.size() * sizeof(double), hipMemcpyDeviceToHost, c.st));
   HIPCHK(hipStreamSynchronize(c.st));
-  (void)hipFree(rep_d);
   HIPCHK(hipGetLastError());
   double m = 0;
   for (int q = 0; q < nrep; q++) m += 1.0 - rep[3 * q] * rep[3 * q] / (rep[3 * q + 2] * rep[3 * q + 1]);

@@ -102,6 +102,14 @@ module suli_gpu
     integer(c_int) function c_suli_sync() bind(C, name='suli_sync')
       import :: c_int
     end function
+    integer(c_int) function c_suli_set_time(t) bind(C, name='suli_set_time')
+      import :: c_int, c_double
+      real(c_double), value :: t
+    end function
+    integer(c_int) function c_suli_set_forcing(amp, period, phase) bind(C, name='suli_set_forcing')
+      import :: c_int, c_double
+      real(c_double), value :: amp, period, phase
+    end function
   end interface
 
   ! driver state mirroring module disc (0_modules.f90:9,39-41)
@@ -148,6 +156,8 @@ contains
   end subroutine visco
 
   subroutine convdiff()
+    ! the time of the step for a time-dependent body force (suli_set_forcing)
+    call check(c_suli_set_time(t), 'set_time')
     call check(c_suli_convdiff(), 'convdiff')
   end subroutine convdiff
 
