@@ -85,3 +85,30 @@ def scaled_relres(A, D, q, pcell):
     """||D^-1/2 (q - A p)|| / ||D^-1/2 q||, the solver's stopping measure."""
     s = 1.0 / np.sqrt(D)
     return float(np.linalg.norm(s * (q - A @ pcell)) / np.linalg.norm(s * q))
+
+
+def dambreak_diagnostics(p, u, v, w, ls, rho):
+    """Integral diagnostics of a dam-break state given in the reference
+    extents: water volume (phi > 0 cells, cell count times dV), the front
+    position as the reference writes it (4_plot.f90:364-375: scanning i from
+    nx down to 2, the first sign change of ls along x gives dx*(i-0.5) +
+    ls(i,j,k)), the kinetic energy sum 0.5 rho |u_c|^2 dV with the face
+    velocities averaged to the cell centres, and max |u|."""
+    nx, ny, nz = p.nx, p.ny, p.nz
+    dV = p.dx * p.dy * p.dz
+    front = None
+    for i in range(nx, 1, -1):  # Fortran index i, arrays 0-based
+        a, b = ls[i - 2], ls[i - 1]
+        hit = np.argwhere((a * b) <= 0.0)
+        if hit.size:
+            # the reference loops k outer, j inner: first (k, j) in that order
+            order = sorted(((kk, jj) for jj, kk in hit), key=lambda t: (t[0], t[1]))
+            kk, jj = order[0]
+            front = p.dx * (i - 0.5) + ls[i - 1, jj, kk]
+            break
+    uc = 0.5 * (u[1:-2, 1:-1, 1:-1] + u[2:-1, 1:-1, 1:-1])
+    vc = 0.5 * (v[1:-1, 1:-2, 1:-1] + v[1:-1, 2:-1, 1:-1])
+    wc = 0.5 * (w[1:-1, 1:-1, 1:-2] + w[1:-1, 1:-1, 2:-1])
+    ke = float(np.sum(0.5 * rho * (uc * uc + vc * vc + wc * wc)) * dV)
+    return {"volume": float(np.count_nonzero(ls > 0) * dV), "front": front, "ke": ke,
+            "umax": float(max(np.abs(u).max(), np.abs(v).max(), np.abs(w).max()))}
