@@ -69,8 +69,8 @@ BYTES_PER_CELL = {
     "fcg_residual64": 7 * 8,       # x, b, sqrt(D), ax, ay, az (FP64) -> r (FP64)
     "fcg_project": 8 + 8 + 4 + 4,  # r, sqrt(D) (FP64) -> r32, d
     "fcg_xupdate": 8 + 4 + 8,
-    "mg_gs_l0": 5 * 4 + 2,         # b, dg (half), c_x,c_y,c_z, x -> x (half)
-    "mg_restrict_l0": 7 * 4,
+    "mg_gs_l0": 16,                # b, dg (half the cells, FP32), c_x,c_y,c_z (BF16), x -> x (half)
+    "mg_restrict_l0": 4 + 4 + 4 + 3 * 2 + 1,  # b, x, dg (FP32), BF16 faces -> coarse b (1/8)
     "mg_prolong_l0": 2 * 4,
     "mg_scale": 3 * 4,
 }

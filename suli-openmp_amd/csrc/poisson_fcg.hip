@@ -373,6 +373,37 @@ __global__ void k_mg_c0(Grid g, const double *__restrict__ ax, const double *__r
   if (i <= nx && j <= ny && k <= nz) sD32[c] = (float)sD[c];
 }
 
+// face conductances are FP32 on the coarse levels and BF16 on the finest one
+// (half the bytes of the sweeps that dominate the V-cycle; the diagonal is
+// the sum of the rounded faces, so the operator stays consistent)
+__device__ __forceinline__ float mgc(float v) { return v; }
+__device__ __forceinline__ float mgc(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+__device__ __forceinline__ uint16_t to_bf16(float f) {  // round to nearest even (finite values)
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// BF16 faces of the finest level and the diagonal as their sum
+__global__ void k_mg_bf16(Grid g, const float *__restrict__ cx, const float *__restrict__ cy,
+                          const float *__restrict__ cz, uint16_t *__restrict__ hx, uint16_t *__restrict__ hy,
+                          uint16_t *__restrict__ hz) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (i > g.nx + 1 || j > g.ny + 1 || k > g.nz + 1) return;
+  const int64_t c = g.idx(i, j, k);
+  if (j <= g.ny && k <= g.nz) hx[c] = to_bf16(cx[c]);
+  if (i <= g.nx && k <= g.nz) hy[c] = to_bf16(cy[c]);
+  if (i <= g.nx && j <= g.ny) hz[c] = to_bf16(cz[c]);
+}
+__global__ void k_mg_diag_bf16(Grid g, const uint16_t *__restrict__ hx, const uint16_t *__restrict__ hy,
+                               const uint16_t *__restrict__ hz, float *__restrict__ dg) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (i > g.nx || j > g.ny || k > g.nz) return;
+  const int64_t c = g.idx(i, j, k);
+  dg[c] = mgc(hx[c]) + mgc(hx[c + 1]) + mgc(hy[c]) + mgc(hy[c + g.sx]) + mgc(hz[c]) + mgc(hz[c + g.sz]);
+}
 __global__ void k_mg_diag(Grid g, const float *__restrict__ cx, const float *__restrict__ cy,
                           const float *__restrict__ cz, float *__restrict__ dg) {
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
@@ -407,12 +438,13 @@ __global__ void k_mg_coarsen(Grid gf, Grid gc, const float *__restrict__ fx, con
   }
 }
 
-__device__ __forceinline__ float mg_Ax(const Grid &g, int64_t c, const float *__restrict__ cx,
-                                       const float *__restrict__ cy, const float *__restrict__ cz,
+template <typename CT>
+__device__ __forceinline__ float mg_Ax(const Grid &g, int64_t c, const CT *__restrict__ cx,
+                                       const CT *__restrict__ cy, const CT *__restrict__ cz,
                                        const float *__restrict__ x) {
   // sum_f c_f x_nb (wall faces have c = 0; their ghost neighbours hold 0)
-  return cx[c + 1] * x[c + 1] + cx[c] * x[c - 1] + cy[c + g.sx] * x[c + g.sx] + cy[c] * x[c - g.sx] +
-         cz[c + g.sz] * x[c + g.sz] + cz[c] * x[c - g.sz];
+  return mgc(cx[c + 1]) * x[c + 1] + mgc(cx[c]) * x[c - 1] + mgc(cy[c + g.sx]) * x[c + g.sx] +
+         mgc(cy[c]) * x[c - g.sx] + mgc(cz[c + g.sz]) * x[c + g.sz] + mgc(cz[c]) * x[c - g.sz];
 }
 
 // red-black Gauss-Seidel half sweep on the cells of one colour.  The 7-point
@@ -420,8 +452,9 @@ __device__ __forceinline__ float mg_Ax(const Grid &g, int64_t c, const float *__
 // reads only the other colour; with zero_x set that colour is taken as zero
 // (the first sweep from x = 0), and its stale values are overwritten by the
 // next half sweep before anything reads them.
-__global__ __launch_bounds__(256) void k_mg_gs(Grid g, const float *__restrict__ cx, const float *__restrict__ cy,
-                                               const float *__restrict__ cz, const float *__restrict__ dg,
+template <typename CT>
+__global__ __launch_bounds__(256) void k_mg_gs(Grid g, const CT *__restrict__ cx, const CT *__restrict__ cy,
+                                               const CT *__restrict__ cz, const float *__restrict__ dg,
                                                const float *__restrict__ b, float *__restrict__ x, int color,
                                                int zero_x, const int *active) {
   if (!*active) return;
@@ -436,8 +469,9 @@ __global__ __launch_bounds__(256) void k_mg_gs(Grid g, const float *__restrict__
 }
 
 // coarse rhs = sum over the 8 children of the fine residual b - A x; coarse x = 0
-__global__ __launch_bounds__(256) void k_mg_restrict(Grid gf, Grid gc, const float *__restrict__ cx,
-                                                     const float *__restrict__ cy, const float *__restrict__ cz,
+template <typename CT>
+__global__ __launch_bounds__(256) void k_mg_restrict(Grid gf, Grid gc, const CT *__restrict__ cx,
+                                                     const CT *__restrict__ cy, const CT *__restrict__ cz,
                                                      const float *__restrict__ dg, const float *__restrict__ b,
                                                      const float *__restrict__ x, float *__restrict__ bc,
                                                      float *__restrict__ xc, const int *active) {
@@ -582,6 +616,7 @@ __global__ void k_xinit(Grid g, const double *__restrict__ p, const double *__re
 struct MgLevel {
   Grid g;
   float *cx = nullptr, *cy = nullptr, *cz = nullptr, *dg = nullptr, *b = nullptr, *x = nullptr;
+  uint16_t *hx = nullptr, *hy = nullptr, *hz = nullptr;  // BF16 faces (finest level)
 };
 
 struct PoissonFCG {
@@ -631,6 +666,11 @@ int fcg_create(Ctx &c) {
   if (falloc(c, &l0.cx, n) || falloc(c, &l0.cy, n) || falloc(c, &l0.cz, n) || falloc(c, &l0.dg, n) ||
       falloc(c, &l0.b, n) || falloc(c, &l0.x, n))
     return 1;
+  for (uint16_t **h : {&l0.hx, &l0.hy, &l0.hz}) {
+    HIPCHK(hipMalloc((void **)h, (size_t)n * sizeof(uint16_t)));
+    HIPCHK(hipMemsetAsync(*h, 0, (size_t)n * sizeof(uint16_t), c.st));
+    c.allocs.push_back(*h);
+  }
   f->lv.push_back(l0);
   for (;;) {
     const Grid &gf = f->lv.back().g;
@@ -684,7 +724,8 @@ static int mg_setup(Ctx &c) {
     MgLevel &l0 = f.lv[0];
     hipLaunchKernelGGL(k_mg_c0, faces_full(c.g), b2, 0, c.st, c.g, c.ax, c.ay, c.az, c.D, l0.cx, l0.cy, l0.cz,
                        l0.dg, f.ax32, f.ay32, f.az32, c.sD, f.sD32);
-    hipLaunchKernelGGL(k_mg_diag, cells_full(c.g), b2, 0, c.st, c.g, l0.cx, l0.cy, l0.cz, l0.dg);
+    hipLaunchKernelGGL(k_mg_bf16, faces_full(c.g), b2, 0, c.st, c.g, l0.cx, l0.cy, l0.cz, l0.hx, l0.hy, l0.hz);
+    hipLaunchKernelGGL(k_mg_diag_bf16, cells_full(c.g), b2, 0, c.st, c.g, l0.hx, l0.hy, l0.hz, l0.dg);
     for (size_t q = 1; q < f.lv.size(); q++) {
       MgLevel &lf = f.lv[q - 1], &lc = f.lv[q];
       hipLaunchKernelGGL(k_mg_coarsen, faces_full(lc.g), b2, 0, c.st, lf.g, lc.g, lf.cx, lf.cy, lf.cz, lc.cx,
@@ -704,10 +745,25 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
 #ifndef SULI_MG_NU0
 #define SULI_MG_NU0 2  // sweeps before and after the coarse correction on the finest level
 #endif
-  const int nu1 = 2, nu2 = 2;
+#ifndef SULI_MG_NU
+#define SULI_MG_NU 2  // sweeps before and after the coarse correction on the coarser levels
+#endif
+  const int nu1 = SULI_MG_NU, nu2 = SULI_MG_NU;
   const int L = (int)f.lv.size();
   int S = 0;
   while (S < L - 1 && (int64_t)f.lv[S].g.nx * f.lv[S].g.ny * f.lv[S].g.nz > SULI_MG_SMALL) S++;
+  // one red-black half sweep of level lf (BF16 faces where the level has them)
+  auto gs = [&](MgLevel &lf, const char *name, int color, int zero_x) {
+    ProfKernel pk(c, name);
+    if (lf.hx)
+      SULI_TIMED_LAUNCH(pk, k_mg_gs<uint16_t>, half_cells(lf.g), b2, 0, c.st, lf.g, (const uint16_t *)lf.hx,
+                        (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)lf.dg, (const float *)lf.b,
+                        lf.x, color, zero_x, active);
+    else
+      SULI_TIMED_LAUNCH(pk, k_mg_gs<float>, half_cells(lf.g), b2, 0, c.st, lf.g, (const float *)lf.cx,
+                        (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.dg, (const float *)lf.b, lf.x,
+                        color, zero_x, active);
+  };
   {
     ProfScope ps(c, "mg_scale");
     hipLaunchKernelGGL(k_scale, cells_kloop(c.g, f.kz), b2, 0, c.st, c.g, f.sD32, r, f.lv[0].b, active);
@@ -716,18 +772,18 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
     MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
     for (int s = 0; s < (l == 0 ? SULI_MG_NU0 : nu1); s++) {
-      {
-        ProfKernel pk(c, gsn);
-        SULI_TIMED_LAUNCH(pk, k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x,
-                          0, s == 0 ? 1 : 0, active);
-      }
-      ProfKernel pk(c, gsn);
-      SULI_TIMED_LAUNCH(pk, k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 1,
-                        0, active);
+      gs(lf, gsn, 0, s == 0 ? 1 : 0);
+      gs(lf, gsn, 1, 0);
     }
     ProfScope ps(c, l == 0 ? "mg_restrict_l0" : "mg_restrict");
-    hipLaunchKernelGGL(k_mg_restrict, cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, lf.cx, lf.cy, lf.cz, lf.dg,
-                       lf.b, lf.x, lc.b, lc.x, active);
+    if (lf.hx)
+      hipLaunchKernelGGL((k_mg_restrict<uint16_t>), cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, (const uint16_t *)lf.hx,
+                         (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)lf.dg,
+                         (const float *)lf.b, (const float *)lf.x, lc.b, lc.x, active);
+    else
+      hipLaunchKernelGGL((k_mg_restrict<float>), cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, (const float *)lf.cx,
+                         (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.dg, (const float *)lf.b,
+                         (const float *)lf.x, lc.b, lc.x, active);
   }
   {
     MgSmall m;
@@ -753,14 +809,8 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
       hipLaunchKernelGGL(k_mg_prolong, cells_full(lf.g), b2, 0, c.st, lf.g, lc.g, lc.x, lf.x, active);
     }
     for (int s = 0; s < (l == 0 ? SULI_MG_NU0 : nu2); s++) {
-      {
-        ProfKernel pk(c, gsn);
-        SULI_TIMED_LAUNCH(pk, k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x,
-                          1, 0, active);
-      }
-      ProfKernel pk(c, gsn);
-      SULI_TIMED_LAUNCH(pk, k_mg_gs, half_cells(lf.g), b2, 0, c.st, lf.g, lf.cx, lf.cy, lf.cz, lf.dg, lf.b, lf.x, 0,
-                        0, active);
+      gs(lf, gsn, 1, 0);
+      gs(lf, gsn, 0, 0);
     }
   }
   {
