@@ -169,6 +169,21 @@ int suli_set_time(double t);
  * counterpart in the reference: its solver is fixed at compile time. */
 int suli_set_solver(int poisson, int precond, double tol_rel, int max_it);
 
+/* Multigrid tuning, no counterpart in the reference.  Levels below the
+ * finest with at most fuse_max_cells cells run all the half sweeps of one
+ * side of the V-cycle in one launch (LDS tiles with a halo; the same cell
+ * updates as one launch per half sweep; 0 turns it off; default 65536).
+ * coarse_sweeps: red-black Gauss-Seidel sweeps on the coarsest level
+ * (default 24).  A negative value keeps the current setting. */
+int suli_set_mg(int64_t fuse_max_cells, int coarse_sweeps);
+
+/* z = M r with the preconditioner of the current solver setting and the
+ * operator of the last FCG suli_graddin (its dt and density): r and z are
+ * scaled residuals in cell order (i fastest, nx*ny*nz values), computed in
+ * FP32.  For tests of the preconditioners; no counterpart in the
+ * reference. */
+int suli_precond_apply(const double *r, double *z, int64_t n);
+
 /* Learned preconditioner (SULI_PRECOND_LEARNED).  No counterpart in the
  * reference (9_grad.f90 is Jacobi-scaled CG).  An L-level 3-D convolutional
  * U-Net (channels 16/32/64/64.., 3x3x3 kernels, ReLU, average pooling, nearest

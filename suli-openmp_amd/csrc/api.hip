@@ -631,6 +631,61 @@ extern "C" int suli_set_solver(int poisson, int precond, double tol_rel, int max
   return 0;
 }
 
+extern "C" int suli_set_mg(int64_t fuse_max_cells, int coarse_sweeps) {
+  NEEDCTX();
+  if (fuse_max_cells >= 0) g_ctx->mg_fuse_max = fuse_max_cells;
+  if (coarse_sweeps >= 0) g_ctx->mg_coarse_sweeps = coarse_sweeps;
+  return 0;
+}
+
+extern "C" int suli_precond_apply(const double *r, double *z, int64_t n) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  const Grid &g = c.g;
+  if (n != (int64_t)g.nx * g.ny * g.nz) {
+    suli_set_error("suli_precond_apply: expects nx*ny*nz = " + std::to_string((int64_t)g.nx * g.ny * g.nz) +
+                   " values, got " + std::to_string(n));
+    return 1;
+  }
+  if (!c.fcg) {
+    suli_set_error("suli_precond_apply: no FCG solver");
+    return 1;
+  }
+  std::vector<float> fr((size_t)g.total, 0.f);
+  int64_t q = 0;
+  for (int k = 1; k <= g.nz; k++)
+    for (int j = 1; j <= g.ny; j++)
+      for (int i = 1; i <= g.nx; i++) fr[(size_t)g.idx(i, j, k)] = (float)r[q++];
+  float *dr = nullptr, *dz = nullptr;
+  const size_t bytes = (size_t)g.total * sizeof(float);
+  HIPCHK(hipMalloc((void **)&dr, bytes));
+  if (hipMalloc((void **)&dz, bytes) != hipSuccess) {
+    (void)hipFree(dr);
+    suli_set_error("suli_precond_apply: out of device memory");
+    return 1;
+  }
+  int rc = 0;
+  if (hipMemcpyAsync(dr, fr.data(), bytes, hipMemcpyHostToDevice, c.st) != hipSuccess ||
+      hipMemsetAsync(dz, 0, bytes, c.st) != hipSuccess) {
+    suli_set_error("suli_precond_apply: copy to the device failed");
+    rc = 1;
+  }
+  if (!rc) rc = fcg_precond_apply(c, dr, dz);
+  if (!rc && (hipMemcpyAsync(fr.data(), dz, bytes, hipMemcpyDeviceToHost, c.st) != hipSuccess ||
+              hipStreamSynchronize(c.st) != hipSuccess)) {
+    suli_set_error("suli_precond_apply: copy from the device failed");
+    rc = 1;
+  }
+  (void)hipFree(dr);
+  (void)hipFree(dz);
+  if (rc) return 1;
+  q = 0;
+  for (int k = 1; k <= g.nz; k++)
+    for (int j = 1; j <= g.ny; j++)
+      for (int i = 1; i <= g.nx; i++) z[q++] = fr[(size_t)g.idx(i, j, k)];
+  return 0;
+}
+
 extern "C" int suli_posdin(double dt) {
   NEEDCTX();
   Ctx &c = *g_ctx;

@@ -45,6 +45,10 @@ struct Ctx {
     if (force_amp != 0.0) q.gx = ph.gx + force_amp * std::sin(2.0 * M_PI * t_now / force_period + force_phase);
     return q;
   }
+  // multigrid levels below the finest with at most this many cells smooth in
+  // one fused launch per side of the V-cycle (0: every half sweep a launch)
+  int64_t mg_fuse_max = 65536;
+  int mg_coarse_sweeps = 24;  // red-black sweeps on the coarsest level
   int prec = SULI_PREC_FP64;
   size_t esz = 8;
   hipStream_t st = nullptr;
@@ -153,3 +157,5 @@ int poisson_ref_solve(Ctx &c, double dt, suli_poisson_stats *st);
 int fcg_create(Ctx &c);
 void fcg_destroy(Ctx &c);
 int fcg_solve(Ctx &c, double dt, suli_poisson_stats *st);
+// z = M r (device FP32 frames) with the operator of the last fcg_solve
+int fcg_precond_apply(Ctx &c, const float *r, float *z);

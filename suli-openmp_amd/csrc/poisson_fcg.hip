@@ -447,6 +447,14 @@ __device__ __forceinline__ float mg_Ax(const Grid &g, int64_t c, const CT *__res
          mgc(cy[c]) * x[c - g.sx] + mgc(cz[c + g.sz]) * x[c + g.sz] + mgc(cz[c]) * x[c - g.sz];
 }
 
+// smoothing sweeps (red and black half sweeps) on each side of the coarse correction
+#ifndef SULI_MG_NU0
+#define SULI_MG_NU0 2  // sweeps before and after the coarse correction on the finest level
+#endif
+#ifndef SULI_MG_NU
+#define SULI_MG_NU 2  // sweeps before and after the coarse correction on the coarser levels
+#endif
+
 // red-black Gauss-Seidel half sweep on the cells of one colour.  The 7-point
 // stencil couples a cell only to cells of the other colour, so a half sweep
 // reads only the other colour; with zero_x set that colour is taken as zero
@@ -500,21 +508,177 @@ __global__ __launch_bounds__(256) void k_mg_prolong(Grid gf, Grid gc, const floa
   x[gf.idx(i, j, k)] += xc[gc.idx((i + 1) >> 1, (j + 1) >> 1, (k + 1) >> 1)];
 }
 
+// ---- fused smoothing of the small multigrid levels ------------------------------
+// On a level of a few thousand to tens of thousands of cells each half sweep
+// is a latency-bound launch.  These kernels run all the half sweeps of one
+// side of the V-cycle in one launch: a work-group holds its tile of T^3 cells
+// and a halo of NH cells (one per half sweep) in LDS, repeats the half
+// sweeps on a region that shrinks by one cell per half sweep, and writes only
+// its tile.  The arithmetic of each cell update is k_mg_gs's, so the tile
+// ends with the values of the separate launches.  The pre-smoothing kernel
+// ends with the restriction of the residual to the next level, the
+// post-smoothing kernel starts with the prolongation from it.
+#define SULI_MG_FT 8                  // tile edge (cells); T^3 threads per work-group
+#define SULI_MG_FNH (2 * SULI_MG_NU)  // half sweeps per side of the V-cycle
+
+// sum_f c_f x_nb from the LDS tile (edge E), in mg_Ax's order
+template <int E>
+__device__ __forceinline__ float tile_Ax(int l, const float *fx, const float *fy, const float *fz, const float *sx) {
+  return fx[l + 1] * sx[l + 1] + fx[l] * sx[l - 1] + fy[l + E] * sx[l + E] + fy[l] * sx[l - E] +
+         fz[l + E * E] * sx[l + E * E] + fz[l] * sx[l - E * E];
+}
+
+// Load b, dg and the three face arrays over the tile and its halo (local
+// index a <-> cell i0 + a per direction); cells outside the level's frame
+// read as 0.  x is the prolongated iterate (post) or 0 (pre, x = 0 on entry).
+template <int E, int NT>
+__device__ __forceinline__ void mg_tile_load(const Grid &g, const Grid &gc, int i0, int j0, int k0,
+                                             const float *__restrict__ cx, const float *__restrict__ cy,
+                                             const float *__restrict__ cz, const float *__restrict__ dg,
+                                             const float *__restrict__ b, const float *__restrict__ x,
+                                             const float *__restrict__ xc, float *sx, float *sb, float *sd,
+                                             float *fx, float *fy, float *fz) {
+  // all loads of a thread are issued before its LDS stores (E^3 = NPT * NT)
+  constexpr int NPT = E * E * E / NT;
+  static_assert(NPT * NT == E * E * E, "LDS region must split evenly over the work-group");
+  float vx[NPT], vb[NPT], vd[NPT], v1[NPT], v2[NPT], v3[NPT];
+#pragma unroll
+  for (int q = 0; q < NPT; q++) {
+    const int t = threadIdx.x + q * NT;
+    const int i = i0 + t % E, j = j0 + (t / E) % E, k = k0 + t / (E * E);
+    vx[q] = vb[q] = vd[q] = v1[q] = v2[q] = v3[q] = 0.f;
+    if (i >= 0 && i <= g.nx + 1 && j >= 0 && j <= g.ny + 1 && k >= 0 && k <= g.nz + 1) {
+      const int64_t c = g.idx(i, j, k);
+      vb[q] = b[c];
+      vd[q] = dg[c];
+      v1[q] = cx[c];
+      v2[q] = cy[c];
+      v3[q] = cz[c];
+      if (x && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny && k >= 1 && k <= g.nz)
+        vx[q] = x[c] + xc[gc.idx((i + 1) >> 1, (j + 1) >> 1, (k + 1) >> 1)];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NPT; q++) {
+    const int t = threadIdx.x + q * NT;
+    sx[t] = vx[q];
+    sb[t] = vb[q];
+    sd[t] = vd[q];
+    fx[t] = v1[q];
+    fy[t] = v2[q];
+    fz[t] = v3[q];
+  }
+  __syncthreads();
+}
+
+// One half sweep of colour `color` over the local box [lo, lo + w)^3 (w even).
+template <int E>
+__device__ __forceinline__ void mg_tile_gs(const Grid &g, int i0, int j0, int k0, int lo, int w, int color,
+                                           bool zero_x, float *sx, const float *sb, const float *sd,
+                                           const float *fx, const float *fy, const float *fz) {
+  const int hw = w / 2;
+  for (int t = threadIdx.x; t < hw * w * w; t += blockDim.x) {
+    const int bb = lo + (t / hw) % w, cc = lo + t / (hw * w);
+    const int j = j0 + bb, k = k0 + cc;
+    const int a = lo + 2 * (t % hw) + ((color - (i0 + lo + j + k)) & 1);  // (i + j + k) & 1 == color
+    const int i = i0 + a;
+    if (i < 1 || i > g.nx || j < 1 || j > g.ny || k < 1 || k > g.nz) continue;
+    const int l = a + E * (bb + E * cc);
+    const float s = zero_x ? 0.f : tile_Ax<E>(l, fx, fy, fz, sx);
+    sx[l] = (sb[l] + s) / sd[l];
+  }
+  __syncthreads();
+}
+
+// Pre-smoothing from x = 0 (half sweeps red, black, ...), then x of the tile
+// to the level and the coarse rhs bc = sum of the 8 children's residuals
+// (coarse x = 0), as k_mg_restrict computes them.
+template <int T, int NH>
+__global__ __launch_bounds__(T *T *T) void k_mg_pre_fused(Grid g, Grid gc, const float *__restrict__ cx,
+                                                         const float *__restrict__ cy, const float *__restrict__ cz,
+                                                         const float *__restrict__ dg, const float *__restrict__ b,
+                                                         float *__restrict__ x, float *__restrict__ bc,
+                                                         float *__restrict__ xc, const int *active) {
+  if (!*active) return;
+  constexpr int E = T + 2 * NH, N = E * E * E;
+  __shared__ float sx[N], sb[N], sd[N], fx[N], fy[N], fz[N];
+  __shared__ float rt[T * T * T];
+  const int i0 = 1 + blockIdx.x * T - NH, j0 = 1 + blockIdx.y * T - NH, k0 = 1 + blockIdx.z * T - NH;
+  mg_tile_load<E, T * T * T>(g, gc, i0, j0, k0, cx, cy, cz, dg, b, nullptr, nullptr, sx, sb, sd, fx, fy, fz);
+  // half sweep h updates the box tile + (NH - h): after NH of them x is final on tile + 1
+#pragma unroll
+  for (int h = 0; h < NH; h++) mg_tile_gs<E>(g, i0, j0, k0, h, E - 2 * h, h & 1, h == 0, sx, sb, sd, fx, fy, fz);
+  const int t = threadIdx.x;
+  const int a = NH + t % T, bb = NH + (t / T) % T, cc = NH + t / (T * T);
+  const int i = i0 + a, j = j0 + bb, k = k0 + cc;
+  float r = 0.f;
+  if (i <= g.nx && j <= g.ny && k <= g.nz) {
+    const int l = a + E * (bb + E * cc);
+    x[g.idx(i, j, k)] = sx[l];
+    r = sb[l] - (sd[l] * sx[l] - tile_Ax<E>(l, fx, fy, fz, sx));
+  }
+  rt[t] = r;
+  __syncthreads();
+  constexpr int H = T / 2;
+  if (t < H * H * H) {
+    const int u = t % H, v = (t / H) % H, w = t / (H * H);
+    const int I = 1 + blockIdx.x * H + u, J = 1 + blockIdx.y * H + v, K = 1 + blockIdx.z * H + w;
+    if (I <= gc.nx && J <= gc.ny && K <= gc.nz) {
+      float s = 0.f;
+#pragma unroll
+      for (int d = 0; d < 8; d++)
+        s += rt[(2 * u + (d & 1)) + T * ((2 * v + ((d >> 1) & 1)) + T * (2 * w + (d >> 2)))];
+      const int64_t c = gc.idx(I, J, K);
+      bc[c] = s;
+      xc[c] = 0.f;
+    }
+  }
+}
+
+// x_out = post-smoothing (half sweeps black, red, ...) of x + x_coarse(parent).
+// x_out must not alias x: neighbouring work-groups read x over their halos.
+template <int T, int NH>
+__global__ __launch_bounds__(T *T *T) void k_mg_post_fused(Grid g, Grid gc, const float *__restrict__ cx,
+                                                          const float *__restrict__ cy, const float *__restrict__ cz,
+                                                          const float *__restrict__ dg, const float *__restrict__ b,
+                                                          const float *__restrict__ x, const float *__restrict__ xc,
+                                                          float *__restrict__ xo, const int *active) {
+  if (!*active) return;
+  constexpr int E = T + 2 * NH, N = E * E * E;
+  __shared__ float sx[N], sb[N], sd[N], fx[N], fy[N], fz[N];
+  const int i0 = 1 + blockIdx.x * T - NH, j0 = 1 + blockIdx.y * T - NH, k0 = 1 + blockIdx.z * T - NH;
+  mg_tile_load<E, T * T * T>(g, gc, i0, j0, k0, cx, cy, cz, dg, b, x, xc, sx, sb, sd, fx, fy, fz);
+  // half sweep h updates the box tile + (NH - 1 - h): after NH of them x is final on the tile
+#pragma unroll
+  for (int h = 0; h < NH; h++)
+    mg_tile_gs<E>(g, i0, j0, k0, h + 1, E - 2 - 2 * h, (h & 1) ^ 1, false, sx, sb, sd, fx, fy, fz);
+  const int t = threadIdx.x;
+  const int a = NH + t % T, bb = NH + (t / T) % T, cc = NH + t / (T * T);
+  const int i = i0 + a, j = j0 + bb, k = k0 + cc;
+  if (i <= g.nx && j <= g.ny && k <= g.nz) xo[g.idx(i, j, k)] = sx[a + E * (bb + E * cc)];
+}
+
 // V-cycle over the small levels s..L-1 (each at most SULI_MG_SMALL cells) in
 // one work-group, levels separated by barriers: the coarse end of the
-// hierarchy costs one launch instead of a dozen latency-bound ones.
+// hierarchy costs one launch instead of a dozen latency-bound ones.  When the
+// levels' frames fit (SULI_MG_SMALL_LDS floats for the six arrays of every
+// level) they are copied to LDS first and only level s's x goes back.
 #define SULI_MG_MAXLV 12
 #ifndef SULI_MG_SMALL
 #define SULI_MG_SMALL 1024
 #endif
-#ifndef SULI_MG_COARSE_SWEEPS
-#define SULI_MG_COARSE_SWEEPS 24  // Gauss-Seidel sweeps on the coarsest level
-#endif
+#define SULI_MG_SMALL_LDS 32768  // 128 KiB
 struct MgSmall {
   Grid g[SULI_MG_MAXLV];
   float *cx[SULI_MG_MAXLV], *cy[SULI_MG_MAXLV], *cz[SULI_MG_MAXLV], *dg[SULI_MG_MAXLV];
   float *b[SULI_MG_MAXLV], *x[SULI_MG_MAXLV];
+  int off[SULI_MG_MAXLV];  // LDS offset of level l's six frames (cx, cy, cz, dg, b, x)
   int s, L, nu, coarse_sweeps;
+};
+
+struct SmallLv {
+  const float *cx, *cy, *cz, *dg;
+  float *b, *x;
 };
 
 __device__ __forceinline__ void cell_of(const Grid &g, int t, int &i, int &j, int &k) {
@@ -523,26 +687,74 @@ __device__ __forceinline__ void cell_of(const Grid &g, int t, int &i, int &j, in
   k = 1 + t / (g.nx * g.ny);
 }
 
-__device__ void small_gs(const MgSmall &m, int l, int color, bool zero_x) {
-  const Grid &g = m.g[l];
-  const int n = g.nx * g.ny * g.nz;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) {
-    int i, j, k;
-    cell_of(g, t, i, j, k);
-    if (((i + j + k) & 1) != color) continue;
+// A thread's first cell pair on a small level: row position ti (cells 2ti+1
+// and 2ti+2) in row (j, k), computed once per level; further pairs (levels of
+// more than 2 * blockDim cells) are decomposed as they come.
+struct SmallCell {
+  int ti, j, k;
+};
+__device__ __forceinline__ SmallCell small_cell(const Grid &g) {
+  const int hx = (g.nx + 1) >> 1, t = threadIdx.x;
+  return SmallCell{t % hx, 1 + (t / hx) % g.ny, 1 + t / (hx * g.ny)};
+}
+
+// red-black half sweep of a small level (the cell of colour `color` in each
+// pair, placed as in k_mg_gs)
+__device__ __forceinline__ void small_gs(const Grid &g, const SmallLv &v, const SmallCell &sc, int color,
+                                         bool zero_x) {
+  const int hx = (g.nx + 1) >> 1, tot = hx * g.ny * g.nz;
+  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+    int ti = sc.ti, j = sc.j, k = sc.k;
+    if (t != (int)threadIdx.x) {
+      ti = t % hx;
+      j = 1 + (t / hx) % g.ny;
+      k = 1 + t / (hx * g.ny);
+    }
+    const int i = 1 + 2 * ti + ((1 + j + k + color) & 1);
+    if (i > g.nx) continue;
     const int64_t c = g.idx(i, j, k);
-    const float s = zero_x ? 0.f : mg_Ax(g, c, m.cx[l], m.cy[l], m.cz[l], m.x[l]);
-    m.x[l][c] = (m.b[l][c] + s) / m.dg[l][c];
+    const float s = zero_x ? 0.f : mg_Ax(g, c, v.cx, v.cy, v.cz, v.x);
+    v.x[c] = (v.b[c] + s) / v.dg[c];
   }
   __syncthreads();
 }
 
+template <bool LDS>
+__device__ __forceinline__ SmallLv small_lv(const MgSmall &m, int l, float *smem) {
+  if (LDS) {
+    float *p = smem + m.off[l];
+    const int64_t n = m.g[l].total;
+    return SmallLv{p, p + n, p + 2 * n, p + 3 * n, p + 4 * n, p + 5 * n};
+  }
+  return SmallLv{m.cx[l], m.cy[l], m.cz[l], m.dg[l], m.b[l], m.x[l]};
+}
+
+template <bool LDS>
 __global__ __launch_bounds__(1024) void k_mg_small(MgSmall m, const int *active) {
   if (!*active) return;
+  __shared__ float smem[LDS ? SULI_MG_SMALL_LDS : 1];
+  if (LDS) {
+    // whole frames: ghost faces are 0 and ghost x must read as 0
+    for (int l = m.s; l < m.L; l++) {
+      const int n = (int)m.g[l].total;
+      float *p = smem + m.off[l];
+      for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        p[t] = m.cx[l][t];
+        p[n + t] = m.cy[l][t];
+        p[2 * n + t] = m.cz[l][t];
+        p[3 * n + t] = m.dg[l][t];
+        p[4 * n + t] = l == m.s ? m.b[l][t] : 0.f;
+        p[5 * n + t] = 0.f;
+      }
+    }
+    __syncthreads();
+  }
   for (int l = m.s; l < m.L - 1; l++) {
+    const SmallLv v = small_lv<LDS>(m, l, smem), vc = small_lv<LDS>(m, l + 1, smem);
+    const SmallCell sc = small_cell(m.g[l]);
     for (int sw = 0; sw < m.nu; sw++) {
-      small_gs(m, l, 0, sw == 0);
-      small_gs(m, l, 1, false);
+      small_gs(m.g[l], v, sc, 0, sw == 0);
+      small_gs(m.g[l], v, sc, 1, false);
     }
     const Grid &gf = m.g[l], &gc = m.g[l + 1];
     const int nc = gc.nx * gc.ny * gc.nz;
@@ -552,29 +764,46 @@ __global__ __launch_bounds__(1024) void k_mg_small(MgSmall m, const int *active)
       float s = 0.f;
       for (int d = 0; d < 8; d++) {
         const int64_t c = gf.idx(2 * I - 1 + (d & 1), 2 * J - 1 + ((d >> 1) & 1), 2 * K - 1 + (d >> 2));
-        s += m.b[l][c] - (m.dg[l][c] * m.x[l][c] - mg_Ax(gf, c, m.cx[l], m.cy[l], m.cz[l], m.x[l]));
+        s += v.b[c] - (v.dg[c] * v.x[c] - mg_Ax(gf, c, v.cx, v.cy, v.cz, v.x));
       }
-      m.b[l + 1][gc.idx(I, J, K)] = s;
+      vc.b[gc.idx(I, J, K)] = s;
     }
     __syncthreads();
   }
-  const int lc = m.L - 1;
-  for (int sw = 0; sw < m.coarse_sweeps; sw++) {
-    small_gs(m, lc, 0, sw == 0);
-    small_gs(m, lc, 1, false);
+  {
+    const int lc = m.L - 1;
+    const SmallLv v = small_lv<LDS>(m, lc, smem);
+    const SmallCell sc = small_cell(m.g[lc]);
+    for (int sw = 0; sw < m.coarse_sweeps; sw++) {
+      small_gs(m.g[lc], v, sc, 0, sw == 0);
+      small_gs(m.g[lc], v, sc, 1, false);
+    }
   }
   for (int l = m.L - 2; l >= m.s; l--) {
+    const SmallLv v = small_lv<LDS>(m, l, smem), vc = small_lv<LDS>(m, l + 1, smem);
     const Grid &gf = m.g[l], &gc = m.g[l + 1];
     const int nf = gf.nx * gf.ny * gf.nz;
     for (int t = threadIdx.x; t < nf; t += blockDim.x) {
       int i, j, k;
       cell_of(gf, t, i, j, k);
-      m.x[l][gf.idx(i, j, k)] += m.x[l + 1][gc.idx((i + 1) >> 1, (j + 1) >> 1, (k + 1) >> 1)];
+      v.x[gf.idx(i, j, k)] += vc.x[gc.idx((i + 1) >> 1, (j + 1) >> 1, (k + 1) >> 1)];
     }
     __syncthreads();
+    const SmallCell sc = small_cell(gf);
     for (int sw = 0; sw < m.nu; sw++) {
-      small_gs(m, l, 1, false);
-      small_gs(m, l, 0, false);
+      small_gs(gf, v, sc, 1, false);
+      small_gs(gf, v, sc, 0, false);
+    }
+  }
+  if (LDS) {
+    const SmallLv v = small_lv<LDS>(m, m.s, smem);
+    const Grid &g = m.g[m.s];
+    const int n = g.nx * g.ny * g.nz;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      int i, j, k;
+      cell_of(g, t, i, j, k);
+      const int64_t c = g.idx(i, j, k);
+      m.x[m.s][c] = v.x[c];
     }
   }
 }
@@ -617,6 +846,7 @@ struct MgLevel {
   Grid g;
   float *cx = nullptr, *cy = nullptr, *cz = nullptr, *dg = nullptr, *b = nullptr, *x = nullptr;
   uint16_t *hx = nullptr, *hy = nullptr, *hz = nullptr;  // BF16 faces (finest level)
+  float *xo = nullptr;  // post-smoothed x of a fused level (the fused kernel cannot update x in place)
 };
 
 struct PoissonFCG {
@@ -625,6 +855,7 @@ struct PoissonFCG {
   float *r32 = nullptr, *d32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr, *z32 = nullptr;
   float *ax32 = nullptr, *ay32 = nullptr, *az32 = nullptr, *sD32 = nullptr;
   std::vector<MgLevel> lv;  // lv[0] shares the main frame
+  bool setup_done = false;  // operator copies and hierarchy built by a solve
   int kz = 1;               // grid.z of the k-looped reduction kernels
   UNetPrecond *unet = nullptr;
   float *pm[SULI_FCG_M] = {}, *qm[SULI_FCG_M] = {};  // FCG(m) directions and their images
@@ -684,7 +915,8 @@ int fcg_create(Ctx &c) {
     gc.dx = gf.dx * 2; gc.dy = gf.dy * 2; gc.dz = gf.dz * 2;
     l.g = gc;
     if (falloc(c, &l.cx, gc.total) || falloc(c, &l.cy, gc.total) || falloc(c, &l.cz, gc.total) ||
-        falloc(c, &l.dg, gc.total) || falloc(c, &l.b, gc.total) || falloc(c, &l.x, gc.total))
+        falloc(c, &l.dg, gc.total) || falloc(c, &l.b, gc.total) || falloc(c, &l.x, gc.total) ||
+        falloc(c, &l.xo, gc.total))
       return 1;
     f->lv.push_back(l);
   }
@@ -742,14 +974,9 @@ static int mg_setup(Ctx &c) {
 static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
   PoissonFCG &f = *c.fcg;
   const dim3 b2(SULI_BX, SULI_BY, 1);
-#ifndef SULI_MG_NU0
-#define SULI_MG_NU0 2  // sweeps before and after the coarse correction on the finest level
-#endif
-#ifndef SULI_MG_NU
-#define SULI_MG_NU 2  // sweeps before and after the coarse correction on the coarser levels
-#endif
   const int nu1 = SULI_MG_NU, nu2 = SULI_MG_NU;
   const int L = (int)f.lv.size();
+  const auto ncell = [&](int l) { return (int64_t)f.lv[l].g.nx * f.lv[l].g.ny * f.lv[l].g.nz; };
   int S = 0;
   while (S < L - 1 && (int64_t)f.lv[S].g.nx * f.lv[S].g.ny * f.lv[S].g.nz > SULI_MG_SMALL) S++;
   // one red-black half sweep of level lf (BF16 faces where the level has them)
@@ -768,8 +995,22 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
     ProfScope ps(c, "mg_scale");
     hipLaunchKernelGGL(k_scale, cells_kloop(c.g, f.kz), b2, 0, c.st, c.g, f.sD32, r, f.lv[0].b, active);
   }
+  // levels 1..S-1 of at most c.mg_fuse_max cells smooth in one launch per side
+  const auto fused = [&](int l) { return l >= 1 && l < S && ncell(l) <= c.mg_fuse_max; };
+  const dim3 bt(SULI_MG_FT * SULI_MG_FT * SULI_MG_FT);
+  const auto tiles = [](const Grid &g) {
+    return dim3((g.nx + SULI_MG_FT - 1) / SULI_MG_FT, (g.ny + SULI_MG_FT - 1) / SULI_MG_FT,
+                (g.nz + SULI_MG_FT - 1) / SULI_MG_FT);
+  };
   for (int l = 0; l < S; l++) {
     MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
+    if (fused(l)) {
+      ProfScope ps(c, "mg_pre_fused");
+      hipLaunchKernelGGL((k_mg_pre_fused<SULI_MG_FT, SULI_MG_FNH>), tiles(lf.g), bt, 0, c.st, lf.g, lc.g,
+                         (const float *)lf.cx, (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.dg,
+                         (const float *)lf.b, lf.x, lc.b, lc.x, active);
+      continue;
+    }
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
     for (int s = 0; s < (l == 0 ? SULI_MG_NU0 : nu1); s++) {
       gs(lf, gsn, 0, s == 0 ? 1 : 0);
@@ -797,16 +1038,32 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
     m.s = S;
     m.L = L;
     m.nu = nu1;
-    m.coarse_sweeps = SULI_MG_COARSE_SWEEPS;
+    m.coarse_sweeps = c.mg_coarse_sweeps;
+    int64_t off = 0;
+    for (int l = S; l < L && l - S < SULI_MG_MAXLV; l++) {
+      m.off[l] = (int)off;
+      off += 6 * f.lv[l].g.total;
+    }
     ProfScope ps(c, "mg_small_levels");
-    hipLaunchKernelGGL(k_mg_small, dim3(1), dim3(1024), 0, c.st, m, active);
+    if (off <= SULI_MG_SMALL_LDS)
+      hipLaunchKernelGGL(k_mg_small<true>, dim3(1), dim3(1024), 0, c.st, m, active);
+    else
+      hipLaunchKernelGGL(k_mg_small<false>, dim3(1), dim3(1024), 0, c.st, m, active);
   }
   for (int l = S - 1; l >= 0; l--) {
     MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
+    const float *xcoarse = fused(l + 1) ? lc.xo : lc.x;  // the next level's final x
+    if (fused(l)) {
+      ProfScope ps(c, "mg_post_fused");
+      hipLaunchKernelGGL((k_mg_post_fused<SULI_MG_FT, SULI_MG_FNH>), tiles(lf.g), bt, 0, c.st, lf.g, lc.g,
+                         (const float *)lf.cx, (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.dg,
+                         (const float *)lf.b, (const float *)lf.x, xcoarse, lf.xo, active);
+      continue;
+    }
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
     {
       ProfScope ps(c, l == 0 ? "mg_prolong_l0" : "mg_prolong");
-      hipLaunchKernelGGL(k_mg_prolong, cells_full(lf.g), b2, 0, c.st, lf.g, lc.g, lc.x, lf.x, active);
+      hipLaunchKernelGGL(k_mg_prolong, cells_full(lf.g), b2, 0, c.st, lf.g, lc.g, xcoarse, lf.x, active);
     }
     for (int s = 0; s < (l == 0 ? SULI_MG_NU0 : nu2); s++) {
       gs(lf, gsn, 1, 0);
@@ -879,6 +1136,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     launch_finish<1>(c.st, gk, rb);
   }
   if (mg_setup(c)) return 1;
+  f.setup_done = true;
   if (pc == SULI_PRECOND_LEARNED) {
     if (unet_prepare(c, f.unet)) return 1;
     for (int m = 0; m < SULI_FCG_M; m++)
@@ -1025,5 +1283,32 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     stt->res = res;
     stt->ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   }
+  return 0;
+}
+
+int fcg_precond_apply(Ctx &c, const float *r, float *z) {
+  PoissonFCG &f = *c.fcg;
+  if (!f.setup_done) {
+    suli_set_error("suli_precond_apply: no FCG solve has set up the operator yet");
+    return 1;
+  }
+  const int pc = c.P.precond;
+  if (pc == SULI_PRECOND_LEARNED && unet_prepare(c, f.unet)) return 1;
+  // the loop-control flag the kernels test: set it for this one application
+  FcgCtl *h = f.ctl_h;
+  HIPCHK(hipMemcpyAsync(h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
+  HIPCHK(hipStreamSynchronize(c.st));
+  const int saved = h->active;
+  h->active = 1;
+  int *active = (int *)((char *)f.ctl + offsetof(FcgCtl, active));
+  HIPCHK(hipMemcpyAsync(active, &h->active, sizeof(int), hipMemcpyHostToDevice, c.st));
+  if (pc == SULI_PRECOND_JACOBI)
+    HIPCHK(hipMemcpyAsync(z, r, (size_t)c.g.total * sizeof(float), hipMemcpyDeviceToDevice, c.st));
+  else
+    apply_precond(c, pc, r, z, active);
+  HIPCHK(hipGetLastError());
+  h->active = saved;
+  HIPCHK(hipMemcpyAsync(active, &h->active, sizeof(int), hipMemcpyHostToDevice, c.st));
+  HIPCHK(hipStreamSynchronize(c.st));
   return 0;
 }

@@ -99,6 +99,8 @@ def lib():
             "suli_step": ([f64, p(LsStats), p(PoissonStats)], i32), "suli_sync": ([], i32),
             "suli_set_vol_ini": ([f64], i32), "suli_get_volumes": ([p(f64), p(f64)], i32),
             "suli_set_solver": ([i32, i32, f64, i32], i32),
+            "suli_set_mg": ([C.c_int64, i32], i32),
+            "suli_precond_apply": ([p(f64), p(f64), C.c_int64], i32),
             "suli_set_forcing": ([f64, f64, f64], i32), "suli_set_time": ([f64], i32),
             "suli_prof_enable": ([i32], i32), "suli_prof_select": ([C.c_char_p, i32], i32),
             "suli_prof_count": ([], i32), "suli_prof_get": ([i32, C.c_char_p, i32, p(C.c_int64), p(f64)], i32),
@@ -243,6 +245,24 @@ class Solver:
 
     def set_solver(self, poisson, precond, tol_rel=0.0, max_it=0):
         _check(lib().suli_set_solver(poisson, precond, tol_rel, max_it), "suli_set_solver")
+
+    def set_mg(self, fuse_max_cells=-1, coarse_sweeps=-1):
+        """Multigrid tuning: fuse the half sweeps of levels of at most
+        fuse_max_cells cells (0: off); sweeps on the coarsest level; -1 keeps."""
+        _check(lib().suli_set_mg(int(fuse_max_cells), int(coarse_sweeps)), "suli_set_mg")
+
+    def precond_apply(self, r):
+        """z = M r with the current preconditioner and the operator of the last
+        FCG graddin; r as a [k, j, i] array of the scaled residual."""
+        r = np.ascontiguousarray(r, dtype=np.float64).ravel()
+        n = self.nx * self.ny * self.nz
+        if r.size != n:
+            raise SuliError(f"precond_apply: expected {n} cells")
+        z = np.empty(n, dtype=np.float64)
+        P = C.POINTER
+        _check(lib().suli_precond_apply(r.ctypes.data_as(P(C.c_double)), z.ctypes.data_as(P(C.c_double)), n),
+               "suli_precond_apply")
+        return z.reshape(self.nz, self.ny, self.nx)
 
     # ---- learned preconditioner -------------------------------------------
     def unet_nparams(self):
