@@ -631,10 +631,11 @@ extern "C" int suli_set_solver(int poisson, int precond, double tol_rel, int max
   return 0;
 }
 
-extern "C" int suli_set_mg(int64_t fuse_max_cells, int coarse_sweeps) {
+extern "C" int suli_set_mg(int64_t fuse_max_cells, int coarse_sweeps, int fuse_fine) {
   NEEDCTX();
   if (fuse_max_cells >= 0) g_ctx->mg_fuse_max = fuse_max_cells;
   if (coarse_sweeps >= 0) g_ctx->mg_coarse_sweeps = coarse_sweeps;
+  if (fuse_fine >= 0) g_ctx->mg_fuse_fine = fuse_fine ? 1 : 0;
   return 0;
 }
 

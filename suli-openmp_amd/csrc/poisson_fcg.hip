@@ -498,6 +498,104 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Grid gf, Grid gc, const CT 
   xc[cc] = 0.f;
 }
 
+// ---- half sweeps fused with their neighbours in the V-cycle ----------------------
+// A thread of these kernels owns the cell pair (2ti+1, 2ti+2) of row (j, k),
+// one red and one black cell; its cell of the colour being updated sits
+// where k_mg_gs puts it.
+
+// First pre-smoothing half sweep of the finest level with the input scaling:
+// b = s r on both cells of the pair (k_scale), x = b/dg on the red one (the
+// zero_x sweep of k_mg_gs).
+__global__ __launch_bounds__(256) void k_mg_gs_in(Grid g, const float *__restrict__ dg, const float *__restrict__ s,
+                                                  const float *__restrict__ r, float *__restrict__ b,
+                                                  float *__restrict__ x, const int *active) {
+  if (!*active) return;
+  const int ti = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  const int i1 = 1 + 2 * ti;
+  if (j > g.ny || k > g.nz || i1 > g.nx) return;
+  const int64_t c1 = g.idx(i1, j, k);
+  const bool two = i1 < g.nx;
+  const float b1 = s[c1] * r[c1], b2 = two ? s[c1 + 1] * r[c1 + 1] : 0.f;
+  b[c1] = b1;
+  if (two) b[c1 + 1] = b2;
+  if (((i1 + j + k) & 1) == 0)
+    x[c1] = (b1 + 0.f) / dg[c1];
+  else if (two)
+    x[c1 + 1] = (b2 + 0.f) / dg[c1 + 1];
+}
+
+// First post-smoothing half sweep (black) with the prolongation: the red
+// neighbours are read as x + x_coarse(parent), the values k_mg_prolong would
+// have stored.  Neither colour's prolongated value is stored: this sweep
+// overwrites the black cells and the next one the red cells without reading
+// their old values.
+template <typename CT>
+__global__ __launch_bounds__(256) void k_mg_gs_prolong(Grid g, Grid gc, const CT *__restrict__ cx,
+                                                       const CT *__restrict__ cy, const CT *__restrict__ cz,
+                                                       const float *__restrict__ dg, const float *__restrict__ b,
+                                                       float *__restrict__ x, const float *__restrict__ xc,
+                                                       const int *active) {
+  if (!*active) return;
+  const int ti = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (j > g.ny || k > g.nz) return;
+  const int i = 1 + 2 * ti + ((j + k) & 1);  // colour 1
+  if (i > g.nx) return;
+  const int64_t c = g.idx(i, j, k);
+  const auto X = [&](int64_t cn, int in, int jn, int kn) {
+    return x[cn] + xc[gc.idx((in + 1) >> 1, (jn + 1) >> 1, (kn + 1) >> 1)];
+  };
+  const float s = mgc(cx[c + 1]) * X(c + 1, i + 1, j, k) + mgc(cx[c]) * X(c - 1, i - 1, j, k) +
+                  mgc(cy[c + g.sx]) * X(c + g.sx, i, j + 1, k) + mgc(cy[c]) * X(c - g.sx, i, j - 1, k) +
+                  mgc(cz[c + g.sz]) * X(c + g.sz, i, j, k + 1) + mgc(cz[c]) * X(c - g.sz, i, j, k - 1);
+  x[c] = (b[c] + s) / dg[c];
+}
+
+// Last post-smoothing half sweep (red) of the finest level with the output
+// scaling z = s x on both cells of the pair (k_scale) and, when partials is
+// set and *first is 0, the block sums of z.q (k_zq).  x itself is not
+// stored: nothing reads it before the next V-cycle starts from x = 0.
+template <typename CT>
+__global__ __launch_bounds__(256) void k_mg_gs_out(Grid g, const CT *__restrict__ cx, const CT *__restrict__ cy,
+                                                   const CT *__restrict__ cz, const float *__restrict__ dg,
+                                                   const float *__restrict__ b, const float *__restrict__ x,
+                                                   const float *__restrict__ s, float *__restrict__ z,
+                                                   const float *__restrict__ q, const int *first,
+                                                   double *partials, const int *active) {
+  if (!*active) return;
+  const int ti = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  const int i1 = 1 + 2 * ti;
+  double acc[1] = {0.0};
+  const bool dot = partials && !*first;
+  if (j <= g.ny && k <= g.nz && i1 <= g.nx) {
+    const int i = i1 + ((1 + j + k) & 1);  // colour 0
+    const int64_t c1 = g.idx(i1, j, k);
+    const bool two = i1 < g.nx;
+    if (i <= g.nx) {
+      const int64_t c = c1 + (i - i1), cb = i == i1 ? c1 + 1 : c1;
+      const float xr = (b[c] + mg_Ax(g, c, cx, cy, cz, x)) / dg[c];
+      const float zr = s[c] * xr;
+      z[c] = zr;
+      if (dot) acc[0] += (double)zr * (double)q[c];
+      if (two) {
+        const float zb = s[cb] * x[cb];
+        z[cb] = zb;
+        if (dot) acc[0] += (double)zb * (double)q[cb];
+      }
+    } else {  // the pair's only cell is black
+      const float zb = s[c1] * x[c1];
+      z[c1] = zb;
+      if (dot) acc[0] += (double)zb * (double)q[c1];
+    }
+  }
+  if (!partials) return;
+  __shared__ double lds[SULI_LDS_RED];
+  block_reduce<1>(acc, lds);
+  block_partial<1>(acc, partials);
+}
+
 // x_fine += x_coarse(parent)
 __global__ __launch_bounds__(256) void k_mg_prolong(Grid gf, Grid gc, const float *__restrict__ xc,
                                                     float *__restrict__ x, const int *active) {
@@ -668,6 +766,12 @@ __global__ __launch_bounds__(T *T *T) void k_mg_post_fused(Grid g, Grid gc, cons
 #define SULI_MG_SMALL 1024
 #endif
 #define SULI_MG_SMALL_LDS 32768  // 128 KiB
+#ifndef SULI_MG_SMALL_THREADS
+#define SULI_MG_SMALL_THREADS 1024
+#endif
+#ifndef SULI_MG_MIN_CELLS
+#define SULI_MG_MIN_CELLS 512  // no level is made below one of at most this many cells
+#endif
 struct MgSmall {
   Grid g[SULI_MG_MAXLV];
   float *cx[SULI_MG_MAXLV], *cy[SULI_MG_MAXLV], *cz[SULI_MG_MAXLV], *dg[SULI_MG_MAXLV];
@@ -906,7 +1010,7 @@ int fcg_create(Ctx &c) {
   for (;;) {
     const Grid &gf = f->lv.back().g;
     if (gf.nx % 2 || gf.ny % 2 || gf.nz % 2 || gf.nx < 4 || gf.ny < 4 || gf.nz < 4) break;
-    if ((int64_t)gf.nx * gf.ny * gf.nz <= 512 || f->lv.size() >= SULI_MG_MAXLV) break;
+    if ((int64_t)gf.nx * gf.ny * gf.nz <= SULI_MG_MIN_CELLS || f->lv.size() >= SULI_MG_MAXLV) break;
     MgLevel l;
     Grid gc = gf;
     gc.nx = gf.nx / 2; gc.ny = gf.ny / 2; gc.nz = gf.nz / 2;
@@ -969,16 +1073,28 @@ static int mg_setup(Ctx &c) {
   return 0;
 }
 
-// One multigrid V-cycle: z = sqrt(D) V(sqrt(D) r).  Levels with more than
-// SULI_MG_SMALL cells run one launch per pass; the rest run in k_mg_small.
-static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
+// z.q of the preconditioned FCG beta, computed by the V-cycle's last kernel
+struct ZqOut {
+  const float *q;
+  const int *first;  // no sum on the first iteration of a pass
+  RedSlot rs;
+};
+
+// One multigrid V-cycle: z = sqrt(D) V(sqrt(D) r), and z.q when zq is set.
+// Levels with more than SULI_MG_SMALL cells run one launch per half sweep
+// (levels of at most c.mg_fuse_max cells below the finest: one launch per
+// side), the rest run in k_mg_small.  With c.mg_fuse_fine the input scaling,
+// the prolongations and the output scaling with z.q ride on the neighbouring
+// half sweeps.
+static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const ZqOut *zq = nullptr) {
   PoissonFCG &f = *c.fcg;
   const dim3 b2(SULI_BX, SULI_BY, 1);
   const int nu1 = SULI_MG_NU, nu2 = SULI_MG_NU;
   const int L = (int)f.lv.size();
   const auto ncell = [&](int l) { return (int64_t)f.lv[l].g.nx * f.lv[l].g.ny * f.lv[l].g.nz; };
   int S = 0;
-  while (S < L - 1 && (int64_t)f.lv[S].g.nx * f.lv[S].g.ny * f.lv[S].g.nz > SULI_MG_SMALL) S++;
+  while (S < L - 1 && ncell(S) > SULI_MG_SMALL) S++;
+  const bool ff = c.mg_fuse_fine && S >= 1;  // the finest level has launches of its own
   // one red-black half sweep of level lf (BF16 faces where the level has them)
   auto gs = [&](MgLevel &lf, const char *name, int color, int zero_x) {
     ProfKernel pk(c, name);
@@ -991,7 +1107,11 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
                         (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.dg, (const float *)lf.b, lf.x,
                         color, zero_x, active);
   };
-  {
+  if (ff) {
+    ProfScope ps(c, "mg_gs_l0_in");
+    hipLaunchKernelGGL(k_mg_gs_in, half_cells(c.g), b2, 0, c.st, c.g, (const float *)f.lv[0].dg,
+                       (const float *)f.sD32, r, f.lv[0].b, f.lv[0].x, active);
+  } else {
     ProfScope ps(c, "mg_scale");
     hipLaunchKernelGGL(k_scale, cells_kloop(c.g, f.kz), b2, 0, c.st, c.g, f.sD32, r, f.lv[0].b, active);
   }
@@ -1013,7 +1133,7 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
     }
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
     for (int s = 0; s < (l == 0 ? SULI_MG_NU0 : nu1); s++) {
-      gs(lf, gsn, 0, s == 0 ? 1 : 0);
+      if (!(ff && l == 0 && s == 0)) gs(lf, gsn, 0, s == 0 ? 1 : 0);
       gs(lf, gsn, 1, 0);
     }
     ProfScope ps(c, l == 0 ? "mg_restrict_l0" : "mg_restrict");
@@ -1046,9 +1166,9 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
     }
     ProfScope ps(c, "mg_small_levels");
     if (off <= SULI_MG_SMALL_LDS)
-      hipLaunchKernelGGL(k_mg_small<true>, dim3(1), dim3(1024), 0, c.st, m, active);
+      hipLaunchKernelGGL(k_mg_small<true>, dim3(1), dim3(SULI_MG_SMALL_THREADS), 0, c.st, m, active);
     else
-      hipLaunchKernelGGL(k_mg_small<false>, dim3(1), dim3(1024), 0, c.st, m, active);
+      hipLaunchKernelGGL(k_mg_small<false>, dim3(1), dim3(SULI_MG_SMALL_THREADS), 0, c.st, m, active);
   }
   for (int l = S - 1; l >= 0; l--) {
     MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
@@ -1061,18 +1181,52 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active) {
       continue;
     }
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
-    {
+    const int nu = l == 0 ? SULI_MG_NU0 : nu2;
+    if (c.mg_fuse_fine) {
+      ProfScope ps(c, l == 0 ? "mg_gs_l0_prolong" : "mg_gs_prolong");
+      if (lf.hx)
+        hipLaunchKernelGGL((k_mg_gs_prolong<uint16_t>), half_cells(lf.g), b2, 0, c.st, lf.g, lc.g,
+                           (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
+                           (const float *)lf.dg, (const float *)lf.b, lf.x, xcoarse, active);
+      else
+        hipLaunchKernelGGL((k_mg_gs_prolong<float>), half_cells(lf.g), b2, 0, c.st, lf.g, lc.g,
+                           (const float *)lf.cx, (const float *)lf.cy, (const float *)lf.cz,
+                           (const float *)lf.dg, (const float *)lf.b, lf.x, xcoarse, active);
+    } else {
       ProfScope ps(c, l == 0 ? "mg_prolong_l0" : "mg_prolong");
       hipLaunchKernelGGL(k_mg_prolong, cells_full(lf.g), b2, 0, c.st, lf.g, lc.g, xcoarse, lf.x, active);
     }
-    for (int s = 0; s < (l == 0 ? SULI_MG_NU0 : nu2); s++) {
-      gs(lf, gsn, 1, 0);
-      gs(lf, gsn, 0, 0);
+    for (int s = 0; s < nu; s++) {
+      if (!(c.mg_fuse_fine && s == 0)) gs(lf, gsn, 1, 0);
+      if (!(ff && l == 0 && s == nu - 1)) gs(lf, gsn, 0, 0);
     }
+  }
+  if (ff) {
+    MgLevel &l0 = f.lv[0];
+    ProfScope ps(c, "mg_gs_l0_out");
+    double *part = zq ? zq->rs.partials : nullptr;
+    const float *q = zq ? zq->q : nullptr;
+    const int *first = zq ? zq->first : active;
+    if (l0.hx)
+      hipLaunchKernelGGL((k_mg_gs_out<uint16_t>), half_cells(c.g), b2, 0, c.st, c.g, (const uint16_t *)l0.hx,
+                         (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const float *)l0.dg,
+                         (const float *)l0.b, (const float *)l0.x, (const float *)f.sD32, z, q, first, part, active);
+    else
+      hipLaunchKernelGGL((k_mg_gs_out<float>), half_cells(c.g), b2, 0, c.st, c.g, (const float *)l0.cx,
+                         (const float *)l0.cy, (const float *)l0.cz, (const float *)l0.dg, (const float *)l0.b,
+                         (const float *)l0.x, (const float *)f.sD32, z, q, first, part, active);
+    if (zq) launch_finish<1>(c.st, half_cells(c.g), zq->rs, EpiActive{f.ctl});
+    return;
   }
   {
     ProfScope ps(c, "mg_scale");
     hipLaunchKernelGGL(k_scale, cells_kloop(c.g, f.kz), b2, 0, c.st, c.g, f.sD32, f.lv[0].x, z, active);
+  }
+  if (zq) {
+    const dim3 gk = cells_kloop(c.g, f.kz);
+    ProfScope ps(c, "fcg_zq");
+    hipLaunchKernelGGL(k_zq, gk, b2, 0, c.st, c.g, z, zq->q, f.ctl, zq->rs);
+    launch_finish<1>(c.st, gk, zq->rs, EpiActive{f.ctl});
   }
 }
 
@@ -1146,7 +1300,10 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   double res0 = -1, res = 0, bnorm = 0;
   bool conv = false;
   const int max_outer = 12;
-  const double rel_in = 1e-4;
+#ifndef SULI_FCG_REL_IN
+#define SULI_FCG_REL_IN 1e-4  // inner (FP32) reduction a refinement pass aims for at most
+#endif
+  const double rel_in = SULI_FCG_REL_IN;
   const int maxit_in = c.P.max_it > 0 ? c.P.max_it : 10000;
   int *active = (int *)((char *)f.ctl + offsetof(FcgCtl, active));
   for (;;) {
@@ -1196,11 +1353,9 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
         }
         const float *zin = f.r32;
         if (use_pc) {
-          apply_precond(c, pc, f.r32, f.z32, active);
-          ProfScope ps(c, "fcg_zq");
-          const RedSlot rz{c.partials, CTL_PTR(f.ctl, zq)};
-          hipLaunchKernelGGL(k_zq, gk, b2, 0, c.st, g, f.z32, f.q32, f.ctl, rz);
-          launch_finish<1>(c.st, gk, rz, EpiActive{f.ctl});
+          const ZqOut zo{f.q32, (const int *)((char *)f.ctl + offsetof(FcgCtl, first)),
+                         RedSlot{c.partials, CTL_PTR(f.ctl, zq)}};
+          mg_vcycle(c, f.r32, f.z32, active, &zo);
           zin = f.z32;
         }
         {

@@ -99,7 +99,7 @@ def lib():
             "suli_step": ([f64, p(LsStats), p(PoissonStats)], i32), "suli_sync": ([], i32),
             "suli_set_vol_ini": ([f64], i32), "suli_get_volumes": ([p(f64), p(f64)], i32),
             "suli_set_solver": ([i32, i32, f64, i32], i32),
-            "suli_set_mg": ([C.c_int64, i32], i32),
+            "suli_set_mg": ([C.c_int64, i32, i32], i32),
             "suli_precond_apply": ([p(f64), p(f64), C.c_int64], i32),
             "suli_set_forcing": ([f64, f64, f64], i32), "suli_set_time": ([f64], i32),
             "suli_prof_enable": ([i32], i32), "suli_prof_select": ([C.c_char_p, i32], i32),
@@ -246,10 +246,11 @@ class Solver:
     def set_solver(self, poisson, precond, tol_rel=0.0, max_it=0):
         _check(lib().suli_set_solver(poisson, precond, tol_rel, max_it), "suli_set_solver")
 
-    def set_mg(self, fuse_max_cells=-1, coarse_sweeps=-1):
+    def set_mg(self, fuse_max_cells=-1, coarse_sweeps=-1, fuse_fine=-1):
         """Multigrid tuning: fuse the half sweeps of levels of at most
-        fuse_max_cells cells (0: off); sweeps on the coarsest level; -1 keeps."""
-        _check(lib().suli_set_mg(int(fuse_max_cells), int(coarse_sweeps)), "suli_set_mg")
+        fuse_max_cells cells (0: off); sweeps on the coarsest level; scaling,
+        prolongation and z.q inside the neighbouring half sweeps (1/0); -1 keeps."""
+        _check(lib().suli_set_mg(int(fuse_max_cells), int(coarse_sweeps), int(fuse_fine)), "suli_set_mg")
 
     def precond_apply(self, r):
         """z = M r with the current preconditioner and the operator of the last

@@ -64,14 +64,17 @@ def test_gmg_beats_jacobi_iterations():
 
 
 @pytest.mark.parametrize("grid", [(128, 64, 64), (96, 48, 40)])
-def test_fused_coarse_smoothing_matches_launch_per_sweep(grid):
-    """The fused pre/post-smoothing kernels of the small multigrid levels give
-    the V-cycle of one launch per half sweep (same cell updates; 96x48x40 has
-    partial tiles on the fused levels), and the same FCG iteration count."""
+def test_fused_multigrid_matches_launch_per_sweep(grid):
+    """The fused kernels of the V-cycle give the V-cycle of one launch per
+    half sweep and per scaling/prolongation: the LDS-tile smoothing of the
+    small levels (96x48x40 has partial tiles), and the scaling, prolongation
+    and z.q carried by the neighbouring half sweeps.  Same cell updates, so
+    the same z to rounding and the same FCG iteration count."""
     nx, ny, nz = grid
     h = 1.0 / nx
     sv = pysuli.Solver(nx, ny, nz, h, precision=pysuli.PREC_MIXED, poisson=pysuli.POISSON_FCG,
                        precond=pysuli.PRECOND_GMG, tol_rel=1e-8)
+    modes = [(0, 0), (65536, 0), (0, 1), (65536, 1), (1 << 40, 1)]  # (fuse_max_cells, fuse_fine)
     try:
         sv.init_dambreak(0.3, 0.4)
         sv.contorno(1)
@@ -79,26 +82,26 @@ def test_fused_coarse_smoothing_matches_launch_per_sweep(grid):
         sv.step(2e-4)
         r = np.random.default_rng(5).standard_normal((nz, ny, nx))
         z = {}
-        for fuse in (0, 65536, 1 << 40):
-            sv.set_mg(fuse_max_cells=fuse)
-            z[fuse] = sv.precond_apply(r)
-        ref = z[0]
+        for m in modes:
+            sv.set_mg(fuse_max_cells=m[0], fuse_fine=m[1])
+            z[m] = sv.precond_apply(r)
+        ref = z[(0, 0)]
         scale = np.abs(ref).max()
         assert np.isfinite(ref).all() and scale > 0
-        for fuse in (65536, 1 << 40):
-            d = np.abs(z[fuse] - ref).max() / scale
-            print(f"{grid} fuse {fuse}: max |z - z_unfused| / max |z| = {d:.2e}")
-            assert d <= 1e-6, d
+        for m in modes[1:]:
+            d = np.abs(z[m] - ref).max() / scale
+            print(f"{grid} mode {m}: max |z - z_separate| / max |z| = {d:.2e}")
+            assert d <= 1e-6, (m, d)
         # the preconditioner is symmetric positive: r.Mr > 0
         assert float((r * ref).sum()) > 0
         its = {}
         p0 = sv.download("prd1")
-        for fuse in (0, 65536):
-            sv.set_mg(fuse_max_cells=fuse)
-            sv.upload("prd1", p0)
-            its[fuse] = sv.graddin(2e-4)
+        for m in ((0, 0), (65536, 1)):
+            sv.set_mg(fuse_max_cells=m[0], fuse_fine=m[1])
+            sv.upload("prd1", 0 * p0)
+            its[m] = sv.graddin(2e-4)
         print(its)
-        assert its[0]["converged"] and its[65536]["converged"]
-        assert abs(its[0]["iters"] - its[65536]["iters"]) <= 1
+        assert its[(0, 0)]["converged"] and its[(65536, 1)]["converged"]
+        assert abs(its[(0, 0)]["iters"] - its[(65536, 1)]["iters"]) <= 1
     finally:
         sv.close()
