@@ -98,6 +98,16 @@ extern "C" int suli_init(const suli_params *p) {
     suli_set_error("suli_init: unknown preconditioner");
     return 1;
   }
+  {
+    // frame offsets are 32-bit (fidx): the largest frame field must stay below 2^31 elements
+    const int64_t sx = ((p->nx + 3 + 31) / 32) * 32;
+    const int64_t tot = 31 + sx * (p->ny + 3) * (int64_t)(p->nz + 3) + 64;
+    if (tot >= (int64_t(1) << 31)) {
+      suli_set_error("suli_init: grid too large: a field frame of " + std::to_string(tot) +
+                     " elements exceeds the 2^31 of the 32-bit frame offsets");
+      return 1;
+    }
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     suli_set_error("suli_init: no HIP device visible");
@@ -117,9 +127,9 @@ extern "C" int suli_init(const suli_params *p) {
   g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
   g.sx = ((p->nx + 3 + 31) / 32) * 32;
   g.sy = p->ny + 3;
-  g.sz = (int64_t)g.sx * g.sy;
+  g.sz = (fidx)g.sx * g.sy;
   g.off = 31;
-  g.total = g.off + g.sz * (p->nz + 3) + 64;
+  g.total = (int64_t)g.off + (int64_t)g.sz * (p->nz + 3) + 64;
   g.dx = p->dx; g.dy = p->dy; g.dz = p->dz;
   g.ccx0 = p->ccx0; g.ccxf = p->ccxf; g.ccy0 = p->ccy0; g.ccyf = p->ccyf; g.ccz0 = p->ccz0; g.cczf = p->cczf;
   Phys &ph = c->ph;

@@ -19,16 +19,23 @@
 
 #define SULI_LDS_RED 16  // max wavefronts per block in the reductions below
 
+// offset of an element in a frame field
+typedef uint32_t fidx;
+
 struct Grid {
   int nx, ny, nz;
   int sx, sy;        // pitches in i and j
-  int64_t sz;        // sx*sy (k stride)
-  int64_t off;       // offset of (0,0,0)
-  int64_t total;     // elements per frame field
+  uint32_t sz;       // sx*sy (k stride)
+  uint32_t off;      // offset of (0,0,0)
+  int64_t total;     // elements per frame field (below 2^31: checked at set-up)
   double dx, dy, dz;
   int ccx0, ccxf, ccy0, ccyf, ccz0, cczf;
-  __host__ __device__ __forceinline__ int64_t idx(int i, int j, int k) const {
-    return off + (int64_t)i + (int64_t)sx * ((int64_t)j + (int64_t)sy * (int64_t)k);
+  // Frame offsets are 32-bit unsigned: with a uniform base pointer the loads
+  // take the scalar-base + 32-bit-offset form, one VGPR per address instead
+  // of two.  Neighbour offsets may wrap below zero in intermediate sums; the
+  // final offset of every access lies in the frame.
+  __host__ __device__ __forceinline__ fidx idx(int i, int j, int k) const {
+    return off + (fidx)i + (fidx)sx * ((fidx)j + (fidx)sy * (fidx)k);
   }
 };
 
@@ -193,14 +200,14 @@ __device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5) {
 // the 1/11 formula on both sides.  The ghosts depend only on phi(1..3) and
 // phi(n-2..n), which always lie inside the thread's own window.
 template <typename T>
-__device__ __forceinline__ void window7(const T *__restrict__ f, int64_t c, int64_t st, int pos,
+__device__ __forceinline__ void window7(const T *__restrict__ f, fidx c, fidx st, int pos,
                                         int n, int ihs, T (&w)[7]) {
   if (ihs == 0) {
 #pragma unroll
     for (int m = 0; m < 7; m++) {
       int q = pos - 3 + m;
       int qs = q < 1 ? q + n - 1 : (q > n ? q - n + 1 : q);
-      w[m] = f[c + (int64_t)(qs - pos) * st];
+      w[m] = f[c + (fidx)(qs - pos) * st];
     }
     return;
   }
@@ -208,11 +215,11 @@ __device__ __forceinline__ void window7(const T *__restrict__ f, int64_t c, int6
   for (int m = 0; m < 7; m++) {
     int q = pos - 3 + m;
     int qc = q < 1 ? 1 : (q > n ? n : q);
-    w[m] = f[c + (int64_t)(qc - pos) * st];
+    w[m] = f[c + (fidx)(qc - pos) * st];
   }
   if (pos <= 3) {
-    const T p1 = f[c + (int64_t)(1 - pos) * st], p2 = f[c + (int64_t)(2 - pos) * st],
-            p3 = f[c + (int64_t)(3 - pos) * st];
+    const T p1 = f[c + (fidx)(1 - pos) * st], p2 = f[c + (fidx)(2 - pos) * st],
+            p3 = f[c + (fidx)(3 - pos) * st];
     T g0, gm1, gm2;
     if (ihs == 1) {
       g0 = T(1.) / T(5.) * (T(12.) * p1 - T(9.) * p2 + T(2.) * p3);
@@ -230,8 +237,8 @@ __device__ __forceinline__ void window7(const T *__restrict__ f, int64_t c, int6
     }
   }
   if (pos >= n - 2) {
-    const T pn = f[c + (int64_t)(n - pos) * st], pn1 = f[c + (int64_t)(n - 1 - pos) * st],
-            pn2 = f[c + (int64_t)(n - 2 - pos) * st];
+    const T pn = f[c + (fidx)(n - pos) * st], pn1 = f[c + (fidx)(n - 1 - pos) * st],
+            pn2 = f[c + (fidx)(n - 2 - pos) * st];
     const T g1 = T(1.) / T(11.) * (T(18.) * pn - T(9.) * pn1 + T(2.) * pn2);
     const T g2 = T(1.) / T(11.) * (T(18.) * g1 - T(9.) * pn + T(2.) * pn1);
     const T g3 = T(1.) / T(11.) * (T(18.) * g2 - T(9.) * g1 + T(2.) * pn);
@@ -259,7 +266,7 @@ __device__ __forceinline__ void weno_pm(const T (&w)[7], T h, T &dp, T &dn) {
 // all_ihs1 is set, ihs 1 in every direction (as mod_ls1 uses it).
 template <typename T>
 __device__ __forceinline__ void der_weno_cell(const T *__restrict__ f, const Grid &g, int i, int j,
-                                              int k, int64_t c, bool all_ihs1, T &ta, T &tb,
+                                              int k, fidx c, bool all_ihs1, T &ta, T &tb,
                                               T &tc, T &td, T &te, T &tf) {
   T w[7];
   const int ihx = all_ihs1 ? 1 : (g.ccx0 == 0 ? 0 : 2);

@@ -488,9 +488,10 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Grid gf, Grid gc, const CT 
             K = 1 + blockIdx.z;
   if (I > gc.nx || J > gc.ny) return;
   float s = 0.f;
+  const int64_t c0 = gf.idx(2 * I - 1, 2 * J - 1, 2 * K - 1);
 #pragma unroll
   for (int d = 0; d < 8; d++) {
-    const int64_t c = gf.idx(2 * I - 1 + (d & 1), 2 * J - 1 + ((d >> 1) & 1), 2 * K - 1 + (d >> 2));
+    const int64_t c = c0 + (d & 1) + ((d >> 1) & 1) * (int64_t)gf.sx + (d >> 2) * (int64_t)gf.sz;
     s += b[c] - (dg[c] * x[c] - mg_Ax(gf, c, cx, cy, cz, x));
   }
   const int64_t cc = gc.idx(I, J, K);
@@ -574,7 +575,7 @@ __global__ __launch_bounds__(256) void k_mg_gs_out(Grid g, const CT *__restrict_
     const int64_t c1 = g.idx(i1, j, k);
     const bool two = i1 < g.nx;
     if (i <= g.nx) {
-      const int64_t c = c1 + (i - i1), cb = i == i1 ? c1 + 1 : c1;
+      const int64_t c = c1 + (int64_t)(i - i1), cb = i == i1 ? c1 + 1 : c1;
       const float xr = (b[c] + mg_Ax(g, c, cx, cy, cz, x)) / dg[c];
       const float zr = s[c] * xr;
       z[c] = zr;
@@ -1014,8 +1015,8 @@ int fcg_create(Ctx &c) {
     MgLevel l;
     Grid gc = gf;
     gc.nx = gf.nx / 2; gc.ny = gf.ny / 2; gc.nz = gf.nz / 2;
-    gc.sx = gc.nx + 3; gc.sy = gc.ny + 3; gc.sz = (int64_t)gc.sx * gc.sy; gc.off = 0;
-    gc.total = gc.sz * (gc.nz + 3) + 8;
+    gc.sx = gc.nx + 3; gc.sy = gc.ny + 3; gc.sz = (fidx)gc.sx * gc.sy; gc.off = 0;
+    gc.total = (int64_t)gc.sz * (gc.nz + 3) + 8;
     gc.dx = gf.dx * 2; gc.dy = gf.dy * 2; gc.dz = gf.dz * 2;
     l.g = gc;
     if (falloc(c, &l.cx, gc.total) || falloc(c, &l.cy, gc.total) || falloc(c, &l.cz, gc.total) ||

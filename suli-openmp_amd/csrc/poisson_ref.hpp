@@ -24,10 +24,10 @@ __device__ __forceinline__ int mirror_idx(int q, int n, bool periodic) {
 // c_f at a face (matspri/matsprj/matsprk, :46-76): dt/(h*h)/rho_face with
 // rho_face = interp*_cf(rho) (end faces copied, 8_derivadas.f90:304-434).
 template <typename T>
-__device__ __forceinline__ double face_coef(const T *__restrict__ rho, int64_t cf, int64_t st, int f,
+__device__ __forceinline__ double face_coef(const T *__restrict__ rho, fidx cf, fidx st, int f,
                                             int n, double dth2) {
   const int fc = f < 2 ? 2 : (f > n ? n : f);
-  const int64_t cc = cf + (int64_t)(fc - f) * st;
+  const fidx cc = cf + (fidx)(fc - f) * st;
   const double rf = ((double)rho[cc] + (double)rho[cc - st]) * 0.5;
   return dth2 / rf;
 }
@@ -41,8 +41,8 @@ __global__ __launch_bounds__(256) void k_pD(Grid g, double dt, const T *__restri
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             k = 1 + blockIdx.z;
   if (i > g.nx || j > g.ny) return;
-  const int64_t c = g.idx(i, j, k);
-  const int64_t X = 1, Y = g.sx, Z = g.sz;
+  const fidx c = g.idx(i, j, k);
+  const fidx X = 1, Y = g.sx, Z = g.sz;
   const double mx = dt / (g.dx * g.dx), my = dt / (g.dy * g.dy), mz = dt / (g.dz * g.dz);
   const double d = face_coef<T>(rho, c + X, X, i + 1, g.nx, mx) + face_coef<T>(rho, c, X, i, g.nx, mx) +
                    face_coef<T>(rho, c + Y, Y, j + 1, g.ny, my) + face_coef<T>(rho, c, Y, j, g.ny, my) +
@@ -63,7 +63,7 @@ __global__ void k_mirror_ghosts(Grid g, double *__restrict__ a, double *__restri
   if (i >= 1 && i <= g.nx && j >= 1 && j <= g.ny && k >= 1 && k <= g.nz) return;
   const int mi = mirror_idx(i, g.nx, g.ccx0 == 0), mj = mirror_idx(j, g.ny, g.ccy0 == 0),
             mk = mirror_idx(k, g.nz, false);
-  const int64_t c = g.idx(i, j, k), s = g.idx(mi, mj, mk);
+  const fidx c = g.idx(i, j, k), s = g.idx(mi, mj, mk);
   a[c] = a[s];
   if (b) b[c] = b[s];
 }
@@ -77,8 +77,8 @@ __global__ __launch_bounds__(256) void k_pcoef(Grid g, double dt, const T *__res
             k = 1 + blockIdx.z;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx + 1 || j > ny + 1 || k > nz + 1) return;
-  const int64_t c = g.idx(i, j, k);
-  const int64_t X = 1, Y = g.sx, Z = g.sz;
+  const fidx c = g.idx(i, j, k);
+  const fidx X = 1, Y = g.sx, Z = g.sz;
   if (j <= ny && k <= nz)
     ax[c] = face_coef<T>(rho, c, X, i, nx, dt / (g.dx * g.dx)) / sqrt(D[c] * D[c - X]);
   if (i <= nx && k <= nz)

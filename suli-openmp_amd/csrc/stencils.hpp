@@ -21,7 +21,7 @@ __global__ __launch_bounds__(256) void k_ls_adv(Grid g, const T *__restrict__ ph
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             k = 1 + blockIdx.z;
   if (i > g.nx || j > g.ny) return;
-  const int64_t c = g.idx(i, j, k);
+  const fidx c = g.idx(i, j, k);
   T ta, tb, tc, td, te, tf;
   der_weno_cell<T>(phi, g, i, j, k, c, false, ta, tb, tc, td, te, tf);
   const T a = (u[c] + u[c + 1]) * T(0.5);
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_reinit_sign(Grid g, T dx1, const T *__r
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             k = 1 + blockIdx.z;
   if (i > g.nx || j > g.ny) return;
-  const int64_t c = g.idx(i, j, k);
+  const fidx c = g.idx(i, j, k);
   T ta, tb, tc, td, te, tf;
   der_weno_cell<T>(phi, g, i, j, k, c, false, ta, tb, tc, td, te, tf);
   const T lsv = phi[c];
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_reinit_stage(Grid g, const T *__restric
             k = 1 + blockIdx.z;
   double acc[2] = {0.0, 0.0};
   if (i <= g.nx && j <= g.ny) {
-    const int64_t c = g.idx(i, j, k);
+    const fidx c = g.idx(i, j, k);
     T ta, tb, tc, td, te, tf;
     der_weno_cell<T>(phi, g, i, j, k, c, false, ta, tb, tc, td, te, tf);
     const T lsv = phi[c];
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void k_heavi(Grid g, Phys ph, const T *__restr
             k = 1 + blockIdx.z;
   double acc[1] = {0.0};
   if (i <= g.nx && j <= g.ny) {
-    const int64_t c = g.idx(i, j, k);
+    const fidx c = g.idx(i, j, k);
     T ta, tb, tc, td, te, tf;
     der_weno_cell<T>(phi, g, i, j, k, c, false, ta, tb, tc, td, te, tf);
     if (!(tabs(ta) > tabs(td))) ta = td;
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_curv(Grid g, const T *__restrict__ nrx,
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             k = 1 + blockIdx.z;
   if (i > g.nx || j > g.ny) return;
-  const int64_t c = g.idx(i, j, k);
+  const fidx c = g.idx(i, j, k);
   T w[7], ta, tb, tc, td, te, tf;
   window7<T>(nrx, c, 1, i, g.nx, 1, w);
   weno_pm<T>(w, T(g.dx), ta, td);
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void k_masscorr(Grid g, T *__restrict__ phi,
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             k = 1 + blockIdx.z;
   if (i > g.nx || j > g.ny) return;
-  const int64_t c = g.idx(i, j, k);
+  const fidx c = g.idx(i, j, k);
   phi[c] = phi[c] - f * modls[c] / vol0;
 }
 
@@ -203,8 +203,8 @@ __global__ __launch_bounds__(256) void k_visco(Grid g, T c23, T cs, const T *__r
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             k = 1 + blockIdx.z;
   if (i > g.nx || j > g.ny) return;
-  const int64_t c = g.idx(i, j, k);
-  const int64_t X = 1, Y = g.sx, Z = g.sz;
+  const fidx c = g.idx(i, j, k);
+  const fidx X = 1, Y = g.sx, Z = g.sz;
   const T h2x = T(2.) * T(g.dx), h2y = T(2.) * T(g.dy), h2z = T(2.) * T(g.dz);
   // u at faces i (c) and i+1 (c+X)
   const T dudx = ((u[c + X] - u[c - X]) / h2x + (u[c + 2 * X] - u[c]) / h2x) * T(0.5);
@@ -249,50 +249,48 @@ struct MomArgs {
 };
 
 template <typename T>
-__device__ __forceinline__ T interp_face(const T *__restrict__ a, int64_t cf, int64_t st, int f,
+__device__ __forceinline__ T interp_face(const T *__restrict__ a, fidx cf, fidx st, int f,
                                          int n) {
   // interp*_cf at face f (8_derivadas.f90:304-434): average of cells fc-1, fc
   // with fc = clamp(f, 2, n); cf is the address of face f's cell (f).
   const int fc = f < 2 ? 2 : (f > n ? n : f);
-  const int64_t cc = cf + (int64_t)(fc - f) * st;
+  const fidx cc = cf + (fidx)(fc - f) * st;
   return (a[cc] + a[cc - st]) * T(0.5);
 }
 template <typename T>
-__device__ __forceinline__ T cell_clamp(const T *__restrict__ a, int64_t cf, int64_t st, int f,
+__device__ __forceinline__ T cell_clamp(const T *__restrict__ a, fidx cf, fidx st, int f,
                                         int n) {
   const int fc = f < 1 ? 1 : (f > n ? n : f);
-  return a[cf + (int64_t)(fc - f) * st];
+  return a[cf + (fidx)(fc - f) * st];
 }
 // face viscosity along d1 at (face f1 along d1), e.g. xnut(i,j,k) = ls_nux + nutx
 template <typename T>
 __device__ __forceinline__ T face_visc(const T *__restrict__ lsnu, const T *__restrict__ nut,
-                                       int64_t cf, int64_t st, int f, int n) {
+                                       fidx cf, fidx st, int f, int n) {
   return interp_face<T>(lsnu, cf, st, f, n) + interp_face<T>(nut, cf, st, f, n);
 }
 // edge value: interp along d2 of the face viscosity along d1, divided by h2
 template <typename T>
 __device__ __forceinline__ T edge_visc(const T *__restrict__ lsnu, const T *__restrict__ nut,
-                                       int64_t ce, int64_t st1, int f1, int n1, int64_t st2, int f2,
+                                       fidx ce, fidx st1, int f1, int n1, fidx st2, int f2,
                                        int n2, T h2) {
   const int fc = f2 < 2 ? 2 : (f2 > n2 ? n2 : f2);
-  const int64_t cc = ce + (int64_t)(fc - f2) * st2;
+  const fidx cc = ce + (fidx)(fc - f2) * st2;
   return ((face_visc<T>(lsnu, nut, cc, st1, f1, n1) + face_visc<T>(lsnu, nut, cc - st2, st1, f1, n1)) *
           T(0.5)) / h2;
 }
 template <typename T>
 __device__ __forceinline__ T surf_cell(const T *__restrict__ hd, const T *__restrict__ kurv,
-                                       const T *__restrict__ rho, int64_t cc) {
+                                       const T *__restrict__ rho, fidx cc) {
   return hd[cc] * kurv[cc] / rho[cc];
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_momentum(Grid g, Phys ph, MomArgs<T> A) {
-  const int i = blockIdx.x * SULI_BX + threadIdx.x, j = blockIdx.y * SULI_BY + threadIdx.y,
-            k = blockIdx.z;
+__device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, const MomArgs<T> &A, int i, int j,
+                                              int k) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  if (i > nx + 2 || j > ny + 2) return;
-  const int64_t c = g.idx(i, j, k);
-  const int64_t X = 1, Y = g.sx, Z = g.sz;
+  const fidx c = g.idx(i, j, k);
+  const fidx X = 1, Y = g.sx, Z = g.sz;
   const T dx = T(g.dx), dy = T(g.dy), dz = T(g.dz);
   const T *__restrict__ u = A.u;
   const T *__restrict__ v = A.v;
@@ -320,7 +318,7 @@ __global__ __launch_bounds__(256) void k_momentum(Grid g, Phys ph, MomArgs<T> A)
       const T dmac = edge_visc<T>(A.lsnu, A.nut, c, X, i, nx, Z, k, nz, dz);
       const T rhox = interp_face<T>(A.rho, c, X, i, nx);
       const int ic = i < 2 ? 2 : (i > nx ? nx : i);
-      const int64_t cc = c + (int64_t)(ic - i);
+      const fidx cc = c + (fidx)(ic - i);
       const T dhsdx = (surf_cell<T>(A.hsx, A.kurv, A.rho, cc) + surf_cell<T>(A.hsx, A.kurv, A.rho, cc - X)) * T(0.5);
       const bool pl = (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
       const T ub = pl ? uc : T(0);
@@ -365,7 +363,7 @@ __global__ __launch_bounds__(256) void k_momentum(Grid g, Phys ph, MomArgs<T> A)
       const T dmbc = edge_visc<T>(A.lsnu, A.nut, c, Y, j, ny, Z, k, nz, dz);
       const T rhoy = interp_face<T>(A.rho, c, Y, j, ny);
       const int jc = j < 2 ? 2 : (j > ny ? ny : j);
-      const int64_t cc = c + (int64_t)(jc - j) * Y;
+      const fidx cc = c + (fidx)(jc - j) * Y;
       const T dhsdy = (surf_cell<T>(A.hsy, A.kurv, A.rho, cc) + surf_cell<T>(A.hsy, A.kurv, A.rho, cc - Y)) * T(0.5);
       const bool pl = (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
       const T vb = pl ? vc : T(0);
@@ -410,7 +408,7 @@ __global__ __launch_bounds__(256) void k_momentum(Grid g, Phys ph, MomArgs<T> A)
       const T dmdm = cell_clamp<T>(A.nut, c - Z, Z, k - 1, nz) / dz;
       const T rhoz = interp_face<T>(A.rho, c, Z, k, nz);
       const int kc = k < 2 ? 2 : (k > nz ? nz : k);
-      const int64_t cc = c + (int64_t)(kc - k) * Z;
+      const fidx cc = c + (fidx)(kc - k) * Z;
       const T dhsdz = (surf_cell<T>(A.hsz, A.kurv, A.rho, cc) + surf_cell<T>(A.hsz, A.kurv, A.rho, cc - Z)) * T(0.5);
       const bool pl = (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
       const T wb = pl ? wc : T(0);
@@ -435,6 +433,14 @@ __global__ __launch_bounds__(256) void k_momentum(Grid g, Phys ph, MomArgs<T> A)
       A.wn[c] = w[c];
     }
   }
+}
+
+// All three momentum updates of one RK stage, one k-plane per block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_momentum(Grid g, Phys ph, MomArgs<T> A) {
+  const int i = blockIdx.x * SULI_BX + threadIdx.x, j = blockIdx.y * SULI_BY + threadIdx.y;
+  if (i > g.nx + 2 || j > g.ny + 2) return;
+  momentum_cell<T>(g, ph, A, i, j, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -600,7 +606,7 @@ __global__ __launch_bounds__(256) void k_posdin(Grid g, double dt, const double 
             k = 1 + blockIdx.z;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx + 1 || j > ny + 1) return;
-  const int64_t c = g.idx(i, j, k);
+  const fidx c = g.idx(i, j, k);
   const double a1 = dt / g.dx, a2 = dt / g.dy, a3 = dt / g.dz;
   if (j <= ny && k <= nz) {
     const double rx = (double)interp_face<T>(rho, c, 1, i, nx);
@@ -625,7 +631,7 @@ __global__ __launch_bounds__(256) void k_divmax(Grid g, const T *__restrict__ u,
             k = 1 + blockIdx.z;
   double acc[1] = {0.0};
   if (i <= g.nx && j <= g.ny) {
-    const int64_t c = g.idx(i, j, k);
+    const fidx c = g.idx(i, j, k);
     const double d = ((double)u[c + 1] - (double)u[c]) / g.dx + ((double)v[c + g.sx] - (double)v[c]) / g.dy +
                      ((double)w[c + g.sz] - (double)w[c]) / g.dz;
     acc[0] = d < 0 ? -d : d;
