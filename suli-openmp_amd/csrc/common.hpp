@@ -107,6 +107,16 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double *lds /* NV*
   }
 }
 
+// Multigrid face conductances: FP32 value of an FP32 or BF16 conductance, and
+// BF16 rounding to nearest even (finite values).
+__device__ __forceinline__ float mgc(float v) { return v; }
+__device__ __forceinline__ float mgc(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
 // Reduction slots: per-block partials (NV values per block, value-major) and
 // the NV results.  A producer kernel writes its block partials with
 // block_partial (no fence, no atomic); a one-block finish kernel launched

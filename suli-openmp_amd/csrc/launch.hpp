@@ -12,6 +12,13 @@ struct MomPtrs {
   void *Fu, *Fv, *Fw;
 };
 
+// outputs of the FCG face set-up (k_fcg_faces)
+struct FcgFaceOut {
+  double *ax, *ay, *az;
+  float *ax32, *ay32, *az32;
+  uint16_t *hx, *hy, *hz;
+};
+
 template <typename T>
 struct Launch {
   static void ls_adv(const Grid &g, hipStream_t s, const void *phi, const void *phin, const void *u,
@@ -39,10 +46,15 @@ struct Launch {
   static void cast_from_f64(hipStream_t s, const double *src, void *dst, int64_t n);
   static void cast_to_f64(hipStream_t s, const void *src, double *dst, int64_t n);
   // projection set-up shared by the Poisson solvers
+  // (ghosts of D and sqrt(D) mirrored unless mirror is false)
   static void pD(const Grid &g, hipStream_t s, double dt, const void *u, const void *v, const void *w,
-                 const void *rho, double *D, double *sD, double *q);
+                 const void *rho, double *D, double *sD, double *q, bool mirror = true);
   static void pcoef(const Grid &g, hipStream_t s, double dt, const void *rho, const double *D,
                     double *ax, double *ay, double *az);
+  // the face arrays of the FCG solve (k_fcg_faces, poisson_ref.hpp): pcoef's
+  // coefficients with their FP32 copies and the BF16 multigrid conductances
+  static void fcg_faces(const Grid &g, hipStream_t s, double dt, const void *rho, const double *D,
+                        const FcgFaceOut &o);
 };
 
 // grid helpers

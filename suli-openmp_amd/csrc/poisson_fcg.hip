@@ -84,23 +84,6 @@ __device__ __forceinline__ A apply_As(const Grid &g, int64_t c, int i, int j, in
   const int kz0 = 1 + blockIdx.z;
 
 // ---- outer (FP64) ---------------------------------------------------------
-// b_s = q/sqrt(D) norm
-__global__ __launch_bounds__(256) void k_bnorm(Grid g, const double *__restrict__ q, const double *__restrict__ sD,
-                                               double *__restrict__ b, RedSlot rs) {
-  CELL_IJK;
-  double acc[1] = {0.0};
-  if (i <= g.nx && j <= g.ny)
-    for (int k = kz0; k <= g.nz; k += gridDim.z) {
-      const int64_t c = g.idx(i, j, k);
-      const double bv = q[c] / sD[c];
-      b[c] = bv;
-      acc[0] += bv * bv;
-    }
-  __shared__ double lds[SULI_LDS_RED];
-  block_reduce<1>(acc, lds);
-  block_partial<1>(acc, rs.partials);
-}
-
 // r = b - A_s x in FP64; sums r.r, r.n, n.n
 __global__ __launch_bounds__(256) void k_res64(Grid g, const double *__restrict__ ax, const double *__restrict__ ay,
                                                const double *__restrict__ az, const double *__restrict__ x,
@@ -375,14 +358,8 @@ __global__ void k_mg_c0(Grid g, const double *__restrict__ ax, const double *__r
 
 // face conductances are FP32 on the coarse levels and BF16 on the finest one
 // (half the bytes of the sweeps that dominate the V-cycle; the diagonal is
-// the sum of the rounded faces, so the operator stays consistent)
-__device__ __forceinline__ float mgc(float v) { return v; }
-__device__ __forceinline__ float mgc(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
-__device__ __forceinline__ uint16_t to_bf16(float f) {  // round to nearest even (finite values)
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+// the sum of the rounded faces, so the operator stays consistent): mgc and
+// to_bf16 in common.hpp
 
 // BF16 faces of the finest level and the diagonal as their sum
 __global__ void k_mg_bf16(Grid g, const float *__restrict__ cx, const float *__restrict__ cy,
@@ -396,13 +373,16 @@ __global__ void k_mg_bf16(Grid g, const float *__restrict__ cx, const float *__r
   if (i <= g.nx && k <= g.nz) hy[c] = to_bf16(cy[c]);
   if (i <= g.nx && j <= g.ny) hz[c] = to_bf16(cz[c]);
 }
+// (and, when sD is given, its FP32 copy)
 __global__ void k_mg_diag_bf16(Grid g, const uint16_t *__restrict__ hx, const uint16_t *__restrict__ hy,
-                               const uint16_t *__restrict__ hz, float *__restrict__ dg) {
+                               const uint16_t *__restrict__ hz, float *__restrict__ dg,
+                               const double *__restrict__ sD = nullptr, float *__restrict__ sD32 = nullptr) {
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             k = 1 + blockIdx.z;
   if (i > g.nx || j > g.ny || k > g.nz) return;
   const int64_t c = g.idx(i, j, k);
   dg[c] = mgc(hx[c]) + mgc(hx[c + 1]) + mgc(hy[c]) + mgc(hy[c + g.sx]) + mgc(hz[c]) + mgc(hz[c + g.sz]);
+  if (sD) sD32[c] = (float)sD[c];
 }
 __global__ void k_mg_diag(Grid g, const float *__restrict__ cx, const float *__restrict__ cy,
                           const float *__restrict__ cz, float *__restrict__ dg) {
@@ -414,8 +394,9 @@ __global__ void k_mg_diag(Grid g, const float *__restrict__ cx, const float *__r
 }
 
 // coarse face conductance: half the sum of the four fine faces on it
-__global__ void k_mg_coarsen(Grid gf, Grid gc, const float *__restrict__ fx, const float *__restrict__ fy,
-                             const float *__restrict__ fz, float *__restrict__ cx, float *__restrict__ cy,
+template <typename CT>
+__global__ void k_mg_coarsen(Grid gf, Grid gc, const CT *__restrict__ fx, const CT *__restrict__ fy,
+                             const CT *__restrict__ fz, float *__restrict__ cx, float *__restrict__ cy,
                              float *__restrict__ cz) {
   const int I = 1 + blockIdx.x * SULI_BX + threadIdx.x, J = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             K = 1 + blockIdx.z;
@@ -423,18 +404,18 @@ __global__ void k_mg_coarsen(Grid gf, Grid gc, const float *__restrict__ fx, con
   const int64_t c = gc.idx(I, J, K);
   if (J <= gc.ny && K <= gc.nz) {
     const int fi = 2 * I - 1, fj = 2 * J - 1, fk = 2 * K - 1;
-    cx[c] = 0.5f * (fx[gf.idx(fi, fj, fk)] + fx[gf.idx(fi, fj + 1, fk)] + fx[gf.idx(fi, fj, fk + 1)] +
-                    fx[gf.idx(fi, fj + 1, fk + 1)]);
+    cx[c] = 0.5f * (mgc(fx[gf.idx(fi, fj, fk)]) + mgc(fx[gf.idx(fi, fj + 1, fk)]) +
+                    mgc(fx[gf.idx(fi, fj, fk + 1)]) + mgc(fx[gf.idx(fi, fj + 1, fk + 1)]));
   }
   if (I <= gc.nx && K <= gc.nz) {
     const int fi = 2 * I - 1, fj = 2 * J - 1, fk = 2 * K - 1;
-    cy[c] = 0.5f * (fy[gf.idx(fi, fj, fk)] + fy[gf.idx(fi + 1, fj, fk)] + fy[gf.idx(fi, fj, fk + 1)] +
-                    fy[gf.idx(fi + 1, fj, fk + 1)]);
+    cy[c] = 0.5f * (mgc(fy[gf.idx(fi, fj, fk)]) + mgc(fy[gf.idx(fi + 1, fj, fk)]) +
+                    mgc(fy[gf.idx(fi, fj, fk + 1)]) + mgc(fy[gf.idx(fi + 1, fj, fk + 1)]));
   }
   if (I <= gc.nx && J <= gc.ny) {
     const int fi = 2 * I - 1, fj = 2 * J - 1, fk = 2 * K - 1;
-    cz[c] = 0.5f * (fz[gf.idx(fi, fj, fk)] + fz[gf.idx(fi + 1, fj, fk)] + fz[gf.idx(fi, fj + 1, fk)] +
-                    fz[gf.idx(fi + 1, fj + 1, fk)]);
+    cz[c] = 0.5f * (mgc(fz[gf.idx(fi, fj, fk)]) + mgc(fz[gf.idx(fi + 1, fj, fk)]) +
+                    mgc(fz[gf.idx(fi, fj + 1, fk)]) + mgc(fz[gf.idx(fi + 1, fj + 1, fk)]));
   }
 }
 
@@ -935,13 +916,24 @@ __global__ void k_pout(Grid g, const double *__restrict__ x, const double *__res
   p[g.idx(i, j, k)] = x[s] / sD[s];
 }
 
-__global__ void k_xinit(Grid g, const double *__restrict__ p, const double *__restrict__ sD, double *__restrict__ x) {
+// initial iterate x = p sqrt(D) and b_s = q/sqrt(D) with the block sums of b_s.b_s
+__global__ __launch_bounds__(256) void k_xb(Grid g, const double *__restrict__ p, const double *__restrict__ q,
+                                            const double *__restrict__ sD, double *__restrict__ x,
+                                            double *__restrict__ b, RedSlot rs) {
   CELL_IJK;
-  if (i > g.nx || j > g.ny) return;
-  for (int k = kz0; k <= g.nz; k += gridDim.z) {
-    const int64_t c = g.idx(i, j, k);
-    x[c] = p[c] * sD[c];
-  }
+  double acc[1] = {0.0};
+  if (i <= g.nx && j <= g.ny)
+    for (int k = kz0; k <= g.nz; k += gridDim.z) {
+      const int64_t c = g.idx(i, j, k);
+      const double sd = sD[c];
+      x[c] = p[c] * sd;
+      const double bv = q[c] / sd;
+      b[c] = bv;
+      acc[0] += bv * bv;
+    }
+  __shared__ double lds[SULI_LDS_RED];
+  block_reduce<1>(acc, lds);
+  block_partial<1>(acc, rs.partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -1053,20 +1045,29 @@ static dim3 half_cells(const Grid &g) {
 }
 
 // Build the FP32 operator copies and the multigrid hierarchy for this dt/rho.
-static int mg_setup(Ctx &c) {
+// (level 0 and the FP32 operator copies here unless the fused set-up made them)
+static int mg_setup(Ctx &c, bool level0) {
   PoissonFCG &f = *c.fcg;
   const dim3 b2(SULI_BX, SULI_BY, 1);
   {
     ProfScope ps(c, "mg_setup");
     MgLevel &l0 = f.lv[0];
-    hipLaunchKernelGGL(k_mg_c0, faces_full(c.g), b2, 0, c.st, c.g, c.ax, c.ay, c.az, c.D, l0.cx, l0.cy, l0.cz,
-                       l0.dg, f.ax32, f.ay32, f.az32, c.sD, f.sD32);
-    hipLaunchKernelGGL(k_mg_bf16, faces_full(c.g), b2, 0, c.st, c.g, l0.cx, l0.cy, l0.cz, l0.hx, l0.hy, l0.hz);
-    hipLaunchKernelGGL(k_mg_diag_bf16, cells_full(c.g), b2, 0, c.st, c.g, l0.hx, l0.hy, l0.hz, l0.dg);
+    if (level0) {
+      hipLaunchKernelGGL(k_mg_c0, faces_full(c.g), b2, 0, c.st, c.g, c.ax, c.ay, c.az, c.D, l0.cx, l0.cy, l0.cz,
+                         l0.dg, f.ax32, f.ay32, f.az32, c.sD, f.sD32);
+      hipLaunchKernelGGL(k_mg_bf16, faces_full(c.g), b2, 0, c.st, c.g, l0.cx, l0.cy, l0.cz, l0.hx, l0.hy, l0.hz);
+      hipLaunchKernelGGL(k_mg_diag_bf16, cells_full(c.g), b2, 0, c.st, c.g, l0.hx, l0.hy, l0.hz, l0.dg);
+    }
     for (size_t q = 1; q < f.lv.size(); q++) {
       MgLevel &lf = f.lv[q - 1], &lc = f.lv[q];
-      hipLaunchKernelGGL(k_mg_coarsen, faces_full(lc.g), b2, 0, c.st, lf.g, lc.g, lf.cx, lf.cy, lf.cz, lc.cx,
-                         lc.cy, lc.cz);
+      // the finest level coarsens from its BF16 faces (the operator its sweeps use)
+      if (lf.hx)
+        hipLaunchKernelGGL(k_mg_coarsen<uint16_t>, faces_full(lc.g), b2, 0, c.st, lf.g, lc.g,
+                           (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, lc.cx, lc.cy,
+                           lc.cz);
+      else
+        hipLaunchKernelGGL(k_mg_coarsen<float>, faces_full(lc.g), b2, 0, c.st, lf.g, lc.g, (const float *)lf.cx,
+                           (const float *)lf.cy, (const float *)lf.cz, lc.cx, lc.cy, lc.cz);
       hipLaunchKernelGGL(k_mg_diag, cells_full(lc.g), b2, 0, c.st, lc.g, lc.cx, lc.cy, lc.cz, lc.dg);
     }
   }
@@ -1283,14 +1284,23 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   // FP64 set-up shared with the reference CG
   {
     ProfScope ps(c, "poisson_setup");
-    DISPATCH(c, Launch<TT>::pD(g, c.st, dt, c.u, c.v, c.w, c.rho, c.D, c.sD, c.q));
-    DISPATCH(c, Launch<TT>::pcoef(g, c.st, dt, c.rho, c.D, c.ax, c.ay, c.az));
-    hipLaunchKernelGGL(k_xinit, gk, b2, 0, c.st, g, c.prd1, c.sD, f.x);
     const RedSlot rb{c.partials, CTL_PTR(f.ctl, bb)};
-    hipLaunchKernelGGL(k_bnorm, gk, b2, 0, c.st, g, c.q, c.sD, f.b, rb);
+    // (the fused face pass reads mirrored cells itself: no ghost fill)
+    DISPATCH(c, Launch<TT>::pD(g, c.st, dt, c.u, c.v, c.w, c.rho, c.D, c.sD, c.q, !c.mg_fuse_fine));
+    if (c.mg_fuse_fine) {
+      // the faces in one pass with the finest multigrid level's BF16 faces
+      MgLevel &l0 = f.lv[0];
+      const FcgFaceOut fo{c.ax, c.ay, c.az, f.ax32, f.ay32, f.az32, l0.hx, l0.hy, l0.hz};
+      DISPATCH(c, Launch<TT>::fcg_faces(g, c.st, dt, c.rho, c.D, fo));
+      hipLaunchKernelGGL(k_mg_diag_bf16, cells_full(c.g), b2, 0, c.st, c.g, l0.hx, l0.hy, l0.hz, l0.dg, c.sD,
+                         f.sD32);
+    } else {
+      DISPATCH(c, Launch<TT>::pcoef(g, c.st, dt, c.rho, c.D, c.ax, c.ay, c.az));
+    }
+    hipLaunchKernelGGL(k_xb, gk, b2, 0, c.st, g, c.prd1, c.q, c.sD, f.x, f.b, rb);
     launch_finish<1>(c.st, gk, rb);
   }
-  if (mg_setup(c)) return 1;
+  if (mg_setup(c, !c.mg_fuse_fine)) return 1;
   f.setup_done = true;
   if (pc == SULI_PRECOND_LEARNED) {
     if (unet_prepare(c, f.unet)) return 1;

@@ -87,6 +87,36 @@ __global__ __launch_bounds__(256) void k_pcoef(Grid g, double dt, const T *__res
     az[c] = face_coef<T>(rho, c, Z, k, nz, dt / (g.dz * g.dz)) / sqrt(D[c] * D[c - Z]);
 }
 
+// ---- faces of the mixed-precision FCG set-up, in one pass ---------------------
+// Faces: the scaled coefficients a_f of
+// k_pcoef, their FP32 copies, and the finest multigrid level's conductances
+// c_f rounded to BF16 (zero on walls); the diagonal of that level follows
+// from the rounded faces (k_mg_diag_bf16).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fcg_faces(Grid g, double dt, const T *__restrict__ rho,
+                                                   const double *__restrict__ D, FcgFaceOut o) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i > nx + 1 || j > ny + 1 || k > nz + 1) return;
+  const fidx c = g.idx(i, j, k);
+  const fidx X = 1, Y = g.sx, Z = g.sz;
+  // D on both sides of the face; a side outside the domain reads the mirrored
+  // interior cell (the ghost value k_mirror_ghosts would have stored)
+  const auto face = [&](fidx st, int f, int n, bool per, double h, double *a, float *a32, uint16_t *hf) {
+    const double fc = face_coef<T>(rho, c, st, f, n, dt / (h * h));
+    const double dlo = f == 1 ? D[c + (fidx)(mirror_idx(0, n, per) - 1) * st] : D[c - st];
+    const double dhi = f == n + 1 ? D[c + (fidx)(mirror_idx(n + 1, n, per) - (n + 1)) * st] : D[c];
+    const double av = fc / sqrt(dhi * dlo);
+    a[c] = av;
+    a32[c] = (float)av;
+    hf[c] = to_bf16((f == 1 || f == n + 1) ? 0.f : (float)fc);
+  };
+  if (j <= ny && k <= nz) face(X, i, nx, g.ccx0 == 0, g.dx, o.ax, o.ax32, o.hx);
+  if (i <= nx && k <= nz) face(Y, j, ny, g.ccy0 == 0, g.dy, o.ay, o.ay32, o.hy);
+  if (i <= nx && j <= ny) face(Z, k, nz, false, g.dz, o.az, o.az32, o.hz);
+}
+
 // Loop control of the reference CG, kept in device memory so that the host
 // can enqueue iterations without waiting for each residual.
 struct CGCtl {

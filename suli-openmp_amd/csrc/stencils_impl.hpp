@@ -152,12 +152,20 @@ void Launch<T>::cast_to_f64(hipStream_t s, const void *src, double *dst, int64_t
 
 template <typename T>
 void Launch<T>::pD(const Grid &g, hipStream_t s, double dt, const void *u, const void *v, const void *w,
-                   const void *rho, double *D, double *sD, double *q) {
+                   const void *rho, double *D, double *sD, double *q, bool mirror) {
   SULI_LAUNCH(k_pD<T>, cell_grid(g), blk2(), s, g, dt, (const T *)u, (const T *)v, (const T *)w,
               (const T *)rho, D, sD, q);
+  if (!mirror) return;
   dim3 b(64, 4, 1);
   dim3 gr((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4, g.nz + 2);
   SULI_LAUNCH(k_mirror_ghosts<0>, gr, b, s, g, D, sD);
+}
+
+template <typename T>
+void Launch<T>::fcg_faces(const Grid &g, hipStream_t s, double dt, const void *rho, const double *D,
+                          const FcgFaceOut &o) {
+  dim3 gf((g.nx + 1 + SULI_BX - 1) / SULI_BX, (g.ny + 1 + SULI_BY - 1) / SULI_BY, g.nz + 1);
+  SULI_LAUNCH(k_fcg_faces<T>, gf, blk2(), s, g, dt, (const T *)rho, D, o);
 }
 
 template <typename T>
