@@ -69,11 +69,11 @@ BYTES_PER_CELL = {
     "fcg_residual64": 7 * 8,       # x, b, sqrt(D), ax, ay, az (FP64) -> r (FP64)
     "fcg_project": 8 + 8 + 4 + 4,  # r, sqrt(D) (FP64) -> r32, d
     "fcg_xupdate": 8 + 4 + 8,
-    "mg_gs_l0": 16,                # b, dg (half the cells, FP32), c_x,c_y,c_z (BF16), x -> x (half)
+    "mg_gs_l0": 14,                # b (half the cells), x, c_x,c_y,c_z (BF16; the diagonal is their sum) -> x (half)
     "mg_gs_l0_in": 4 + 4 + 2 + 4 + 2,  # r, sqrt(D) (FP32), dg (half) -> b, x (half)
-    "mg_gs_l0_prolong": 16 + 1,    # a half sweep + the coarse x (1/8 of the cells)
-    "mg_gs_l0_out": 4 + 6 + 2 + 2 + 3 * 4,  # x, faces, b, dg (half), sqrt(D), q -> z
-    "mg_restrict_l0": 4 + 4 + 4 + 3 * 2 + 1,  # b, x, dg (FP32), BF16 faces -> coarse b (1/8)
+    "mg_gs_l0_prolong": 14 + 1,    # a half sweep + the coarse x (1/8 of the cells)
+    "mg_gs_l0_out": 4 + 6 + 2 + 3 * 4,  # x, faces, b (half), sqrt(D), q -> z
+    "mg_restrict_l0": 4 + 4 + 3 * 2 + 1,  # b, x (FP32), BF16 faces -> coarse b (1/8)
     "mg_prolong_l0": 2 * 4,
     "mg_scale": 3 * 4,
 }

@@ -436,6 +436,15 @@ __device__ __forceinline__ float mg_Ax(const Grid &g, int64_t c, const CT *__res
 #define SULI_MG_NU 2  // sweeps before and after the coarse correction on the coarser levels
 #endif
 
+// The diagonal as the sum of the six face conductances, in k_mg_diag's order
+// (the stored dg, bit for bit): the sweeps read the six faces anyway, so
+// reading dg as well would only add bytes.
+template <typename CT>
+__device__ __forceinline__ float mg_dg(const Grid &g, int64_t c, const CT *__restrict__ cx,
+                                       const CT *__restrict__ cy, const CT *__restrict__ cz) {
+  return mgc(cx[c]) + mgc(cx[c + 1]) + mgc(cy[c]) + mgc(cy[c + g.sx]) + mgc(cz[c]) + mgc(cz[c + g.sz]);
+}
+
 // red-black Gauss-Seidel half sweep on the cells of one colour.  The 7-point
 // stencil couples a cell only to cells of the other colour, so a half sweep
 // reads only the other colour; with zero_x set that colour is taken as zero
@@ -443,9 +452,8 @@ __device__ __forceinline__ float mg_Ax(const Grid &g, int64_t c, const CT *__res
 // next half sweep before anything reads them.
 template <typename CT>
 __global__ __launch_bounds__(256) void k_mg_gs(Grid g, const CT *__restrict__ cx, const CT *__restrict__ cy,
-                                               const CT *__restrict__ cz, const float *__restrict__ dg,
-                                               const float *__restrict__ b, float *__restrict__ x, int color,
-                                               int zero_x, const int *active) {
+                                               const CT *__restrict__ cz, const float *__restrict__ b,
+                                               float *__restrict__ x, int color, int zero_x, const int *active) {
   if (!*active) return;
   const int ti = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             k = 1 + blockIdx.z;
@@ -454,16 +462,16 @@ __global__ __launch_bounds__(256) void k_mg_gs(Grid g, const CT *__restrict__ cx
   if (i > g.nx) return;
   const int64_t c = g.idx(i, j, k);
   const float s = zero_x ? 0.f : mg_Ax(g, c, cx, cy, cz, x);
-  x[c] = (b[c] + s) / dg[c];
+  x[c] = (b[c] + s) / mg_dg(g, c, cx, cy, cz);
 }
 
 // coarse rhs = sum over the 8 children of the fine residual b - A x; coarse x = 0
 template <typename CT>
 __global__ __launch_bounds__(256) void k_mg_restrict(Grid gf, Grid gc, const CT *__restrict__ cx,
                                                      const CT *__restrict__ cy, const CT *__restrict__ cz,
-                                                     const float *__restrict__ dg, const float *__restrict__ b,
-                                                     const float *__restrict__ x, float *__restrict__ bc,
-                                                     float *__restrict__ xc, const int *active) {
+                                                     const float *__restrict__ b, const float *__restrict__ x,
+                                                     float *__restrict__ bc, float *__restrict__ xc,
+                                                     const int *active) {
   if (!*active) return;
   const int I = 1 + blockIdx.x * SULI_BX + threadIdx.x, J = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             K = 1 + blockIdx.z;
@@ -473,7 +481,7 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Grid gf, Grid gc, const CT 
 #pragma unroll
   for (int d = 0; d < 8; d++) {
     const int64_t c = c0 + (d & 1) + ((d >> 1) & 1) * (int64_t)gf.sx + (d >> 2) * (int64_t)gf.sz;
-    s += b[c] - (dg[c] * x[c] - mg_Ax(gf, c, cx, cy, cz, x));
+    s += b[c] - (mg_dg(gf, c, cx, cy, cz) * x[c] - mg_Ax(gf, c, cx, cy, cz, x));
   }
   const int64_t cc = gc.idx(I, J, K);
   bc[cc] = s;
@@ -515,9 +523,8 @@ __global__ __launch_bounds__(256) void k_mg_gs_in(Grid g, const float *__restric
 template <typename CT>
 __global__ __launch_bounds__(256) void k_mg_gs_prolong(Grid g, Grid gc, const CT *__restrict__ cx,
                                                        const CT *__restrict__ cy, const CT *__restrict__ cz,
-                                                       const float *__restrict__ dg, const float *__restrict__ b,
-                                                       float *__restrict__ x, const float *__restrict__ xc,
-                                                       const int *active) {
+                                                       const float *__restrict__ b, float *__restrict__ x,
+                                                       const float *__restrict__ xc, const int *active) {
   if (!*active) return;
   const int ti = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             k = 1 + blockIdx.z;
@@ -531,7 +538,7 @@ __global__ __launch_bounds__(256) void k_mg_gs_prolong(Grid g, Grid gc, const CT
   const float s = mgc(cx[c + 1]) * X(c + 1, i + 1, j, k) + mgc(cx[c]) * X(c - 1, i - 1, j, k) +
                   mgc(cy[c + g.sx]) * X(c + g.sx, i, j + 1, k) + mgc(cy[c]) * X(c - g.sx, i, j - 1, k) +
                   mgc(cz[c + g.sz]) * X(c + g.sz, i, j, k + 1) + mgc(cz[c]) * X(c - g.sz, i, j, k - 1);
-  x[c] = (b[c] + s) / dg[c];
+  x[c] = (b[c] + s) / mg_dg(g, c, cx, cy, cz);
 }
 
 // Last post-smoothing half sweep (red) of the finest level with the output
@@ -540,8 +547,8 @@ __global__ __launch_bounds__(256) void k_mg_gs_prolong(Grid g, Grid gc, const CT
 // stored: nothing reads it before the next V-cycle starts from x = 0.
 template <typename CT>
 __global__ __launch_bounds__(256) void k_mg_gs_out(Grid g, const CT *__restrict__ cx, const CT *__restrict__ cy,
-                                                   const CT *__restrict__ cz, const float *__restrict__ dg,
-                                                   const float *__restrict__ b, const float *__restrict__ x,
+                                                   const CT *__restrict__ cz, const float *__restrict__ b,
+                                                   const float *__restrict__ x,
                                                    const float *__restrict__ s, float *__restrict__ z,
                                                    const float *__restrict__ q, const int *first,
                                                    double *partials, const int *active) {
@@ -557,7 +564,7 @@ __global__ __launch_bounds__(256) void k_mg_gs_out(Grid g, const CT *__restrict_
     const bool two = i1 < g.nx;
     if (i <= g.nx) {
       const int64_t c = c1 + (int64_t)(i - i1), cb = i == i1 ? c1 + 1 : c1;
-      const float xr = (b[c] + mg_Ax(g, c, cx, cy, cz, x)) / dg[c];
+      const float xr = (b[c] + mg_Ax(g, c, cx, cy, cz, x)) / mg_dg(g, c, cx, cy, cz);
       const float zr = s[c] * xr;
       z[c] = zr;
       if (dot) acc[0] += (double)zr * (double)q[c];
@@ -1102,12 +1109,12 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     ProfKernel pk(c, name);
     if (lf.hx)
       SULI_TIMED_LAUNCH(pk, k_mg_gs<uint16_t>, half_cells(lf.g), b2, 0, c.st, lf.g, (const uint16_t *)lf.hx,
-                        (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)lf.dg, (const float *)lf.b,
-                        lf.x, color, zero_x, active);
+                        (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)lf.b, lf.x, color,
+                        zero_x, active);
     else
       SULI_TIMED_LAUNCH(pk, k_mg_gs<float>, half_cells(lf.g), b2, 0, c.st, lf.g, (const float *)lf.cx,
-                        (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.dg, (const float *)lf.b, lf.x,
-                        color, zero_x, active);
+                        (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.b, lf.x, color, zero_x,
+                        active);
   };
   if (ff) {
     ProfScope ps(c, "mg_gs_l0_in");
@@ -1141,12 +1148,12 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     ProfScope ps(c, l == 0 ? "mg_restrict_l0" : "mg_restrict");
     if (lf.hx)
       hipLaunchKernelGGL((k_mg_restrict<uint16_t>), cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, (const uint16_t *)lf.hx,
-                         (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)lf.dg,
-                         (const float *)lf.b, (const float *)lf.x, lc.b, lc.x, active);
+                         (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)lf.b,
+                         (const float *)lf.x, lc.b, lc.x, active);
     else
       hipLaunchKernelGGL((k_mg_restrict<float>), cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, (const float *)lf.cx,
-                         (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.dg, (const float *)lf.b,
-                         (const float *)lf.x, lc.b, lc.x, active);
+                         (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.b, (const float *)lf.x,
+                         lc.b, lc.x, active);
   }
   {
     MgSmall m;
@@ -1189,11 +1196,11 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
       if (lf.hx)
         hipLaunchKernelGGL((k_mg_gs_prolong<uint16_t>), half_cells(lf.g), b2, 0, c.st, lf.g, lc.g,
                            (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
-                           (const float *)lf.dg, (const float *)lf.b, lf.x, xcoarse, active);
+                           (const float *)lf.b, lf.x, xcoarse, active);
       else
         hipLaunchKernelGGL((k_mg_gs_prolong<float>), half_cells(lf.g), b2, 0, c.st, lf.g, lc.g,
                            (const float *)lf.cx, (const float *)lf.cy, (const float *)lf.cz,
-                           (const float *)lf.dg, (const float *)lf.b, lf.x, xcoarse, active);
+                           (const float *)lf.b, lf.x, xcoarse, active);
     } else {
       ProfScope ps(c, l == 0 ? "mg_prolong_l0" : "mg_prolong");
       hipLaunchKernelGGL(k_mg_prolong, cells_full(lf.g), b2, 0, c.st, lf.g, lc.g, xcoarse, lf.x, active);
@@ -1211,12 +1218,12 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     const int *first = zq ? zq->first : active;
     if (l0.hx)
       hipLaunchKernelGGL((k_mg_gs_out<uint16_t>), half_cells(c.g), b2, 0, c.st, c.g, (const uint16_t *)l0.hx,
-                         (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const float *)l0.dg,
-                         (const float *)l0.b, (const float *)l0.x, (const float *)f.sD32, z, q, first, part, active);
+                         (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const float *)l0.b,
+                         (const float *)l0.x, (const float *)f.sD32, z, q, first, part, active);
     else
       hipLaunchKernelGGL((k_mg_gs_out<float>), half_cells(c.g), b2, 0, c.st, c.g, (const float *)l0.cx,
-                         (const float *)l0.cy, (const float *)l0.cz, (const float *)l0.dg, (const float *)l0.b,
-                         (const float *)l0.x, (const float *)f.sD32, z, q, first, part, active);
+                         (const float *)l0.cy, (const float *)l0.cz, (const float *)l0.b, (const float *)l0.x,
+                         (const float *)f.sD32, z, q, first, part, active);
     if (zq) launch_finish<1>(c.st, half_cells(c.g), zq->rs, EpiActive{f.ctl});
     return;
   }
