@@ -69,10 +69,11 @@ BYTES_PER_CELL = {
     "fcg_residual64": 7 * 8,       # x, b, sqrt(D), ax, ay, az (FP64) -> r (FP64)
     "fcg_project": 8 + 8 + 4 + 4,  # r, sqrt(D) (FP64) -> r32, d
     "fcg_xupdate": 8 + 4 + 8,
-    "mg_gs_l0": 14,                # b (half the cells), x, c_x,c_y,c_z (BF16; the diagonal is their sum) -> x (half)
-    "mg_gs_l0_in": 4 + 4 + 2 + 4 + 2,  # r, sqrt(D) (FP32), dg (half) -> b, x (half)
-    "mg_gs_l0_prolong": 14 + 1,    # a half sweep + the coarse x (1/8 of the cells)
-    "mg_gs_l0_out": 4 + 6 + 2 + 3 * 4,  # x, faces, b (half), sqrt(D), q -> z
+    "mg_gs_l0": 2 + 2 + 6 + 2,     # x of the other colour, b of this one (FP32, half the cells each),
+                                   # c_x,c_y,c_z (BF16; the diagonal is their sum) -> x of this colour
+    "mg_gs_l0_in": 4 + 4 + 2 + 4 + 2,  # r, sqrt(D) (FP32), red dg (half) -> b (both colours), red x
+    "mg_gs_l0_prolong": 12 + 1,    # a half sweep + the coarse x (1/8 of the cells)
+    "mg_gs_l0_out": 2 + 6 + 2 + 3 * 4,  # black x, faces, red b, sqrt(D), q -> z
     "mg_restrict_l0": 4 + 4 + 3 * 2 + 1,  # b, x (FP32), BF16 faces -> coarse b (1/8)
     "mg_prolong_l0": 2 * 4,
     "mg_scale": 3 * 4,
@@ -264,8 +265,8 @@ def hbm_traffic(kernel):
 
 
 # timing scope -> kernel of that scope (the finest-grid dispatches of it)
-PMC_KERNELS = {"mg_gs_l0": "k_mg_gs", "mg_gs_l0_in": "k_mg_gs_in", "mg_gs_l0_prolong": "k_mg_gs_prolong",
-               "mg_gs_l0_out": "k_mg_gs_out", "mg_restrict_l0": "k_mg_restrict", "mg_prolong_l0": "k_mg_prolong",
+PMC_KERNELS = {"mg_gs_l0": "k_mg0_gs", "mg_gs_l0_in": "k_mg0_in", "mg_gs_l0_prolong": "k_mg0_prolong",
+               "mg_gs_l0_out": "k_mg0_out", "mg_restrict_l0": "k_mg0_restrict", "mg_prolong_l0": "k_mg_prolong",
                "mg_scale": "k_scale", "fcg_spmv_dot": "k_fcg1", "fcg_update": "k_fcg2", "fcg_zq": "k_zq",
                "fcg_residual64": "k_res64", "fcg_project": "k_r32", "fcg_xupdate": "k_xupd",
                "convdiff_tempo": "k_momentum", "ls_advect_rk3": "k_ls_adv", "reinit_rk3": "k_reinit_stage",

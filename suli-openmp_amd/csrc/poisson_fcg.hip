@@ -374,15 +374,19 @@ __global__ void k_mg_bf16(Grid g, const float *__restrict__ cx, const float *__r
   if (i <= g.nx && j <= g.ny) hz[c] = to_bf16(cz[c]);
 }
 // (and, when sD is given, its FP32 copy)
+// (and, when given, the FP32 copy of sqrt(D) and the red cells' diagonal in
+// the colour-split layout of the finest level)
 __global__ void k_mg_diag_bf16(Grid g, const uint16_t *__restrict__ hx, const uint16_t *__restrict__ hy,
-                               const uint16_t *__restrict__ hz, float *__restrict__ dg,
-                               const double *__restrict__ sD = nullptr, float *__restrict__ sD32 = nullptr) {
+                               const uint16_t *__restrict__ hz, float *__restrict__ dg, const double *__restrict__ sD,
+                               float *__restrict__ sD32, Grid h, float *__restrict__ dred) {
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             k = 1 + blockIdx.z;
   if (i > g.nx || j > g.ny || k > g.nz) return;
   const int64_t c = g.idx(i, j, k);
-  dg[c] = mgc(hx[c]) + mgc(hx[c + 1]) + mgc(hy[c]) + mgc(hy[c + g.sx]) + mgc(hz[c]) + mgc(hz[c + g.sz]);
+  const float d = mgc(hx[c]) + mgc(hx[c + 1]) + mgc(hy[c]) + mgc(hy[c + g.sx]) + mgc(hz[c]) + mgc(hz[c + g.sz]);
+  dg[c] = d;
   if (sD) sD32[c] = (float)sD[c];
+  if (dred && ((i + j + k) & 1) == 0) dred[h.idx((i - 1) >> 1, j, k)] = d;
 }
 __global__ void k_mg_diag(Grid g, const float *__restrict__ cx, const float *__restrict__ cy,
                           const float *__restrict__ cz, float *__restrict__ dg) {
@@ -583,6 +587,162 @@ __global__ __launch_bounds__(256) void k_mg_gs_out(Grid g, const CT *__restrict_
   __shared__ double lds[SULI_LDS_RED];
   block_reduce<1>(acc, lds);
   block_partial<1>(acc, partials);
+}
+
+// ---- the finest level with its iterate and right-hand side split by colour ---------
+// A half sweep reads the other colour's x and writes its own colour's x and
+// reads its b: in the frame layout each of those touches every cache line of
+// the array, twice the bytes the colour needs.  With the fused kernels the
+// finest level keeps x, b (and the diagonal used by the first sweep) in two
+// compact arrays, one per colour: the cell of colour col of pair p (cells
+// 2p+1, 2p+2) of row (j, k) sits at h.idx(p, j, k) of array col, where h is a
+// frame of (nx+1)/2 pairs with one ghost pair/row/plane on each side (all
+// zero).  The faces stay in the frame layout.  Seen from a cell of pair p
+// that is the second of its pair (sec = 1), the x-neighbours of the other
+// colour are pairs p and p+1; for the first (sec = 0) pairs p-1 and p; the
+// y- and z-neighbours are pair p of the neighbouring rows.
+
+// first pre-smoothing half sweep from x = 0, with the input scaling b = s r
+__global__ __launch_bounds__(256) void k_mg0_in(Grid g, Grid h, const float *__restrict__ dred,
+                                                const float *__restrict__ s, const float *__restrict__ r,
+                                                float *__restrict__ bred, float *__restrict__ bblack,
+                                                float *__restrict__ xred, const int *active) {
+  if (!*active) return;
+  const int p = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  const int i1 = 1 + 2 * p;
+  if (j > g.ny || k > g.nz || i1 > g.nx) return;
+  const int64_t c1 = g.idx(i1, j, k), q = h.idx(p, j, k);
+  const bool two = i1 < g.nx;
+  const float b1 = s[c1] * r[c1], b2 = two ? s[c1 + 1] * r[c1 + 1] : 0.f;
+  if (((i1 + j + k) & 1) == 0) {  // the first cell is red
+    bred[q] = b1;
+    if (two) bblack[q] = b2;
+    xred[q] = (b1 + 0.f) / dred[q];
+  } else {
+    bblack[q] = b1;
+    if (two) {
+      bred[q] = b2;
+      xred[q] = (b2 + 0.f) / dred[q];
+    }
+  }
+}
+
+// half sweep of colour `color` (x of the colour from b and the other colour's x)
+__global__ __launch_bounds__(256) void k_mg0_gs(Grid g, Grid h, const uint16_t *__restrict__ cx,
+                                                const uint16_t *__restrict__ cy, const uint16_t *__restrict__ cz,
+                                                const float *__restrict__ bcol, const float *__restrict__ xo,
+                                                float *__restrict__ xcol, int color, const int *active) {
+  if (!*active) return;
+  const int p = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (j > g.ny || k > g.nz) return;
+  const int sec = (1 + j + k + color) & 1;
+  const int i = 1 + 2 * p + sec;
+  if (i > g.nx) return;
+  const int64_t c = g.idx(i, j, k), q = h.idx(p, j, k);
+  const float s = mgc(cx[c + 1]) * xo[q + sec] + mgc(cx[c]) * xo[q - 1 + sec] + mgc(cy[c + g.sx]) * xo[q + h.sx] +
+                  mgc(cy[c]) * xo[q - h.sx] + mgc(cz[c + g.sz]) * xo[q + h.sz] + mgc(cz[c]) * xo[q - h.sz];
+  xcol[q] = (bcol[q] + s) / mg_dg(g, c, cx, cy, cz);
+}
+
+// first post-smoothing half sweep (black) with the prolongation folded into
+// the red neighbours it reads (k_mg_gs_prolong)
+__global__ __launch_bounds__(256) void k_mg0_prolong(Grid g, Grid h, Grid gc, const uint16_t *__restrict__ cx,
+                                                     const uint16_t *__restrict__ cy,
+                                                     const uint16_t *__restrict__ cz, const float *__restrict__ bblack,
+                                                     const float *__restrict__ xred, float *__restrict__ xblack,
+                                                     const float *__restrict__ xc, const int *active) {
+  if (!*active) return;
+  const int p = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  if (j > g.ny || k > g.nz) return;
+  const int sec = (j + k) & 1;  // colour 1
+  const int i = 1 + 2 * p + sec;
+  if (i > g.nx) return;
+  const int64_t c = g.idx(i, j, k), q = h.idx(p, j, k);
+  const auto X = [&](int64_t qn, int in, int jn, int kn) {
+    return xred[qn] + xc[gc.idx((in + 1) >> 1, (jn + 1) >> 1, (kn + 1) >> 1)];
+  };
+  const float s = mgc(cx[c + 1]) * X(q + sec, i + 1, j, k) + mgc(cx[c]) * X(q - 1 + sec, i - 1, j, k) +
+                  mgc(cy[c + g.sx]) * X(q + h.sx, i, j + 1, k) + mgc(cy[c]) * X(q - h.sx, i, j - 1, k) +
+                  mgc(cz[c + g.sz]) * X(q + h.sz, i, j, k + 1) + mgc(cz[c]) * X(q - h.sz, i, j, k - 1);
+  xblack[q] = (bblack[q] + s) / mg_dg(g, c, cx, cy, cz);
+}
+
+// last post-smoothing half sweep (red) with the output scaling z = s x on both
+// cells of the pair (frame layout) and the block sums of z.q (k_mg_gs_out)
+__global__ __launch_bounds__(256) void k_mg0_out(Grid g, Grid h, const uint16_t *__restrict__ cx,
+                                                 const uint16_t *__restrict__ cy, const uint16_t *__restrict__ cz,
+                                                 const float *__restrict__ bred, const float *__restrict__ xblack,
+                                                 const float *__restrict__ s, float *__restrict__ z,
+                                                 const float *__restrict__ q, const int *first, double *partials,
+                                                 const int *active) {
+  if (!*active) return;
+  const int p = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  const int i1 = 1 + 2 * p;
+  double acc[1] = {0.0};
+  const bool dot = partials && !*first;
+  if (j <= g.ny && k <= g.nz && i1 <= g.nx) {
+    const int sec = (1 + j + k) & 1;  // colour 0
+    const int i = i1 + sec;
+    const int64_t c1 = g.idx(i1, j, k), qh = h.idx(p, j, k);
+    const bool two = i1 < g.nx;
+    if (i <= g.nx) {
+      const int64_t c = c1 + sec, cb = sec ? c1 : c1 + 1;
+      const float sx = mgc(cx[c + 1]) * xblack[qh + sec] + mgc(cx[c]) * xblack[qh - 1 + sec] +
+                       mgc(cy[c + g.sx]) * xblack[qh + h.sx] + mgc(cy[c]) * xblack[qh - h.sx] +
+                       mgc(cz[c + g.sz]) * xblack[qh + h.sz] + mgc(cz[c]) * xblack[qh - h.sz];
+      const float xr = (bred[qh] + sx) / mg_dg(g, c, cx, cy, cz);
+      const float zr = s[c] * xr;
+      z[c] = zr;
+      if (dot) acc[0] += (double)zr * (double)q[c];
+      if (two) {
+        const float zb = s[cb] * xblack[qh];
+        z[cb] = zb;
+        if (dot) acc[0] += (double)zb * (double)q[cb];
+      }
+    } else {  // the pair's only cell is black
+      const float zb = s[c1] * xblack[qh];
+      z[c1] = zb;
+      if (dot) acc[0] += (double)zb * (double)q[c1];
+    }
+  }
+  if (!partials) return;
+  __shared__ double lds[SULI_LDS_RED];
+  block_reduce<1>(acc, lds);
+  block_partial<1>(acc, partials);
+}
+
+// restriction of the residual to level 1 (k_mg_restrict on the split arrays)
+__global__ __launch_bounds__(256) void k_mg0_restrict(Grid gf, Grid h, Grid gc, const uint16_t *__restrict__ cx,
+                                                      const uint16_t *__restrict__ cy,
+                                                      const uint16_t *__restrict__ cz, const float *__restrict__ b0,
+                                                      const float *__restrict__ b1, const float *__restrict__ x0,
+                                                      const float *__restrict__ x1, float *__restrict__ bc,
+                                                      float *__restrict__ xc, const int *active) {
+  if (!*active) return;
+  const int I = 1 + blockIdx.x * SULI_BX + threadIdx.x, J = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            K = 1 + blockIdx.z;
+  if (I > gc.nx || J > gc.ny) return;
+  float s = 0.f;
+  const int64_t c0 = gf.idx(2 * I - 1, 2 * J - 1, 2 * K - 1), q0 = h.idx(I - 1, 2 * J - 1, 2 * K - 1);
+#pragma unroll
+  for (int d = 0; d < 8; d++) {
+    const int di = d & 1, dj = (d >> 1) & 1, dk = d >> 2;
+    const int64_t c = c0 + di + dj * (int64_t)gf.sx + dk * (int64_t)gf.sz;
+    const int64_t q = q0 + dj * (int64_t)h.sx + dk * (int64_t)h.sz;
+    const bool red = ((di + dj + dk + 2 * I - 1 + 2 * J - 1 + 2 * K - 1) & 1) == 0;
+    const float *xs = red ? x0 : x1, *xo = red ? x1 : x0;
+    const float bv = red ? b0[q] : b1[q];
+    const float ax = mgc(cx[c + 1]) * xo[q + di] + mgc(cx[c]) * xo[q - 1 + di] + mgc(cy[c + gf.sx]) * xo[q + h.sx] +
+                     mgc(cy[c]) * xo[q - h.sx] + mgc(cz[c + gf.sz]) * xo[q + h.sz] + mgc(cz[c]) * xo[q - h.sz];
+    s += bv - (mg_dg(gf, c, cx, cy, cz) * xs[q] - ax);
+  }
+  const int64_t cc = gc.idx(I, J, K);
+  bc[cc] = s;
+  xc[cc] = 0.f;
 }
 
 // x_fine += x_coarse(parent)
@@ -959,6 +1119,9 @@ struct PoissonFCG {
   float *r32 = nullptr, *d32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr, *z32 = nullptr;
   float *ax32 = nullptr, *ay32 = nullptr, *az32 = nullptr, *sD32 = nullptr;
   std::vector<MgLevel> lv;  // lv[0] shares the main frame
+  // finest level split by colour (fused kernels): pair frame, x and b per colour, red diagonal
+  Grid hg{};
+  float *x0c = nullptr, *x1c = nullptr, *b0c = nullptr, *b1c = nullptr, *d0c = nullptr;
   bool setup_done = false;  // operator copies and hierarchy built by a solve
   int kz = 1;               // grid.z of the k-looped reduction kernels
   UNetPrecond *unet = nullptr;
@@ -1007,6 +1170,19 @@ int fcg_create(Ctx &c) {
     c.allocs.push_back(*h);
   }
   f->lv.push_back(l0);
+  {
+    Grid h = g;
+    h.nx = (g.nx + 1) / 2;
+    h.sx = ((h.nx + 2 + 31) / 32) * 32;
+    h.sy = g.ny + 2;
+    h.sz = (fidx)h.sx * h.sy;
+    h.off = 32;
+    h.total = (int64_t)h.off + (int64_t)h.sz * (g.nz + 2) + 32;
+    f->hg = h;
+    if (falloc(c, &f->x0c, h.total) || falloc(c, &f->x1c, h.total) || falloc(c, &f->b0c, h.total) ||
+        falloc(c, &f->b1c, h.total) || falloc(c, &f->d0c, h.total))
+      return 1;
+  }
   for (;;) {
     const Grid &gf = f->lv.back().g;
     if (gf.nx % 2 || gf.ny % 2 || gf.nz % 2 || gf.nx < 4 || gf.ny < 4 || gf.nz < 4) break;
@@ -1063,7 +1239,8 @@ static int mg_setup(Ctx &c, bool level0) {
       hipLaunchKernelGGL(k_mg_c0, faces_full(c.g), b2, 0, c.st, c.g, c.ax, c.ay, c.az, c.D, l0.cx, l0.cy, l0.cz,
                          l0.dg, f.ax32, f.ay32, f.az32, c.sD, f.sD32);
       hipLaunchKernelGGL(k_mg_bf16, faces_full(c.g), b2, 0, c.st, c.g, l0.cx, l0.cy, l0.cz, l0.hx, l0.hy, l0.hz);
-      hipLaunchKernelGGL(k_mg_diag_bf16, cells_full(c.g), b2, 0, c.st, c.g, l0.hx, l0.hy, l0.hz, l0.dg);
+      hipLaunchKernelGGL(k_mg_diag_bf16, cells_full(c.g), b2, 0, c.st, c.g, l0.hx, l0.hy, l0.hz, l0.dg,
+                         (const double *)nullptr, (float *)nullptr, f.hg, f.d0c);
     }
     for (size_t q = 1; q < f.lv.size(); q++) {
       MgLevel &lf = f.lv[q - 1], &lc = f.lv[q];
@@ -1107,6 +1284,12 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
   // one red-black half sweep of level lf (BF16 faces where the level has them)
   auto gs = [&](MgLevel &lf, const char *name, int color, int zero_x) {
     ProfKernel pk(c, name);
+    if (ff && &lf == &f.lv[0]) {  // finest level, colour-split arrays (never from x = 0 here)
+      SULI_TIMED_LAUNCH(pk, k_mg0_gs, half_cells(lf.g), b2, 0, c.st, lf.g, f.hg, (const uint16_t *)lf.hx,
+                        (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)(color ? f.b1c : f.b0c),
+                        (const float *)(color ? f.x0c : f.x1c), color ? f.x1c : f.x0c, color, active);
+      return;
+    }
     if (lf.hx)
       SULI_TIMED_LAUNCH(pk, k_mg_gs<uint16_t>, half_cells(lf.g), b2, 0, c.st, lf.g, (const uint16_t *)lf.hx,
                         (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)lf.b, lf.x, color,
@@ -1118,8 +1301,8 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
   };
   if (ff) {
     ProfScope ps(c, "mg_gs_l0_in");
-    hipLaunchKernelGGL(k_mg_gs_in, half_cells(c.g), b2, 0, c.st, c.g, (const float *)f.lv[0].dg,
-                       (const float *)f.sD32, r, f.lv[0].b, f.lv[0].x, active);
+    hipLaunchKernelGGL(k_mg0_in, half_cells(c.g), b2, 0, c.st, c.g, f.hg, (const float *)f.d0c,
+                       (const float *)f.sD32, r, f.b0c, f.b1c, f.x0c, active);
   } else {
     ProfScope ps(c, "mg_scale");
     hipLaunchKernelGGL(k_scale, cells_kloop(c.g, f.kz), b2, 0, c.st, c.g, f.sD32, r, f.lv[0].b, active);
@@ -1146,7 +1329,11 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
       gs(lf, gsn, 1, 0);
     }
     ProfScope ps(c, l == 0 ? "mg_restrict_l0" : "mg_restrict");
-    if (lf.hx)
+    if (ff && l == 0)
+      hipLaunchKernelGGL(k_mg0_restrict, cells_full(lc.g), b2, 0, c.st, lf.g, f.hg, lc.g, (const uint16_t *)lf.hx,
+                         (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)f.b0c, (const float *)f.b1c,
+                         (const float *)f.x0c, (const float *)f.x1c, lc.b, lc.x, active);
+    else if (lf.hx)
       hipLaunchKernelGGL((k_mg_restrict<uint16_t>), cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, (const uint16_t *)lf.hx,
                          (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)lf.b,
                          (const float *)lf.x, lc.b, lc.x, active);
@@ -1193,7 +1380,11 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     const int nu = l == 0 ? SULI_MG_NU0 : nu2;
     if (c.mg_fuse_fine) {
       ProfScope ps(c, l == 0 ? "mg_gs_l0_prolong" : "mg_gs_prolong");
-      if (lf.hx)
+      if (l == 0)  // (c.mg_fuse_fine and S >= 1: ff)
+        hipLaunchKernelGGL(k_mg0_prolong, half_cells(lf.g), b2, 0, c.st, lf.g, f.hg, lc.g, (const uint16_t *)lf.hx,
+                           (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)f.b1c,
+                           (const float *)f.x0c, f.x1c, xcoarse, active);
+      else if (lf.hx)
         hipLaunchKernelGGL((k_mg_gs_prolong<uint16_t>), half_cells(lf.g), b2, 0, c.st, lf.g, lc.g,
                            (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
                            (const float *)lf.b, lf.x, xcoarse, active);
@@ -1216,14 +1407,9 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     double *part = zq ? zq->rs.partials : nullptr;
     const float *q = zq ? zq->q : nullptr;
     const int *first = zq ? zq->first : active;
-    if (l0.hx)
-      hipLaunchKernelGGL((k_mg_gs_out<uint16_t>), half_cells(c.g), b2, 0, c.st, c.g, (const uint16_t *)l0.hx,
-                         (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const float *)l0.b,
-                         (const float *)l0.x, (const float *)f.sD32, z, q, first, part, active);
-    else
-      hipLaunchKernelGGL((k_mg_gs_out<float>), half_cells(c.g), b2, 0, c.st, c.g, (const float *)l0.cx,
-                         (const float *)l0.cy, (const float *)l0.cz, (const float *)l0.b, (const float *)l0.x,
-                         (const float *)f.sD32, z, q, first, part, active);
+    hipLaunchKernelGGL(k_mg0_out, half_cells(c.g), b2, 0, c.st, c.g, f.hg, (const uint16_t *)l0.hx,
+                       (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const float *)f.b0c, (const float *)f.x1c,
+                       (const float *)f.sD32, z, q, first, part, active);
     if (zq) launch_finish<1>(c.st, half_cells(c.g), zq->rs, EpiActive{f.ctl});
     return;
   }
@@ -1299,8 +1485,8 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       MgLevel &l0 = f.lv[0];
       const FcgFaceOut fo{c.ax, c.ay, c.az, f.ax32, f.ay32, f.az32, l0.hx, l0.hy, l0.hz};
       DISPATCH(c, Launch<TT>::fcg_faces(g, c.st, dt, c.rho, c.D, fo));
-      hipLaunchKernelGGL(k_mg_diag_bf16, cells_full(c.g), b2, 0, c.st, c.g, l0.hx, l0.hy, l0.hz, l0.dg, c.sD,
-                         f.sD32);
+      hipLaunchKernelGGL(k_mg_diag_bf16, cells_full(c.g), b2, 0, c.st, c.g, l0.hx, l0.hy, l0.hz, l0.dg,
+                         (const double *)c.sD, f.sD32, f.hg, f.d0c);
     } else {
       DISPATCH(c, Launch<TT>::pcoef(g, c.st, dt, c.rho, c.D, c.ax, c.ay, c.az));
     }
