@@ -47,6 +47,7 @@ struct FcgCtl {
   int active;        // inner loop running
   int it;            // inner iterations of this pass
   int first;         // next K1 is the first of the pass (beta = 0)
+  int done;          // the last outer residual met the tolerance (set with the pass's arming)
   // FCG(m) with the learned preconditioner: the last SULI_FCG_M directions
   int nk;                    // directions held (0..SULI_FCG_M)
   double zqm[SULI_FCG_M];    // z.q_j of the current z
@@ -126,18 +127,28 @@ __global__ __launch_bounds__(256) void k_r32(Grid g, const double *__restrict__ 
 }
 
 // loop-control epilogues of the reduction finishes (run by one thread)
-struct EpiR32 {  // arm the inner loop: stop when r.r <= rel_in^2 r0.r0
+// Arm the inner loop of a refinement pass: stop when r.r <= rin^2 r0.r0 with
+// rin = max(rel_in, min(0.3 tol ||b|| / ||r_outer||, 1)) (no further than the
+// overall tolerance needs), from the outer residual of this pass (rout) and
+// ||b|| (bb) already on the device; when that residual meets the tolerance
+// the pass does not run (done).
+struct EpiR32 {
   FcgCtl *ctl;
-  double rel_in;
+  double rel_in, tol;
   int maxit;
   __device__ bool enabled() const { return true; }
   __device__ void operator()(const double *res) const {
-    ctl->thr_in = res[0] * rel_in * rel_in;
+    const double rr = ctl->rout[0], bb = ctl->bb[0];
+    const bool conv = !(rr > tol * tol * bb);
+    const double want = rr > 0 ? 0.3 * tol * sqrt(bb) / sqrt(rr) : 1.0;
+    const double rin = fmax(rel_in, fmin(want, 1.0));
+    ctl->thr_in = res[0] * rin * rin;
     ctl->maxit_in = maxit;
     ctl->it = 0;
     ctl->first = 1;
     ctl->nk = 0;
-    ctl->active = res[0] > 0 ? 1 : 0;
+    ctl->done = conv ? 1 : 0;
+    ctl->active = (res[0] > 0 && !conv) ? 1 : 0;
   }
 };
 struct EpiActive {  // plain result, only while the inner loop runs
@@ -433,6 +444,9 @@ __device__ __forceinline__ float mg_Ax(const Grid &g, int64_t c, const CT *__res
 }
 
 // smoothing sweeps (red and black half sweeps) on each side of the coarse correction
+#ifndef SULI_MG_NU1
+#define SULI_MG_NU1 SULI_MG_NU  // the same on level 1 when it has launches of its own
+#endif
 #ifndef SULI_MG_NU0
 #define SULI_MG_NU0 2  // sweeps before and after the coarse correction on the finest level
 #endif
@@ -1324,7 +1338,7 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
       continue;
     }
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
-    for (int s = 0; s < (l == 0 ? SULI_MG_NU0 : nu1); s++) {
+    for (int s = 0; s < (l == 0 ? SULI_MG_NU0 : (l == 1 ? SULI_MG_NU1 : nu1)); s++) {
       if (!(ff && l == 0 && s == 0)) gs(lf, gsn, 0, s == 0 ? 1 : 0);
       gs(lf, gsn, 1, 0);
     }
@@ -1377,7 +1391,7 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
       continue;
     }
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
-    const int nu = l == 0 ? SULI_MG_NU0 : nu2;
+    const int nu = l == 0 ? SULI_MG_NU0 : (l == 1 ? SULI_MG_NU1 : nu2);
     if (c.mg_fuse_fine) {
       ProfScope ps(c, l == 0 ? "mg_gs_l0_prolong" : "mg_gs_prolong");
       if (l == 0)  // (c.mg_fuse_fine and S >= 1: ff)
@@ -1510,6 +1524,17 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   const double rel_in = SULI_FCG_REL_IN;
   const int maxit_in = c.P.max_it > 0 ? c.P.max_it : 10000;
   int *active = (int *)((char *)f.ctl + offsetof(FcgCtl, active));
+  // Host round trips: the residual of a pass is read back before the pass is
+  // enqueued only when the pass might not be needed.  When the previous pass
+  // (or, for the first pass, the previous solve) leaves the residual clearly
+  // above the tolerance, the residual, the arming of the next pass and its
+  // first iterations go out together; the device then decides (EpiR32) and
+  // a pass that turns out unnecessary is skipped by its own kernels.  When the
+  // solve is expected to end, the output pressure is computed with the
+  // residual check so that the last read-back finds it done.
+  bool speculate = pc != SULI_PRECOND_LEARNED && f.pass_iters[0] > 0;
+  const dim3 pg((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4, g.nz + 2);
+  bool pout_done = false;
   for (;;) {
     {
       ProfScope ps(c, "fcg_residual64");
@@ -1517,28 +1542,33 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       hipLaunchKernelGGL(k_res64, gk, b2, 0, c.st, g, c.ax, c.ay, c.az, f.x, f.b, c.sD, f.r, rr);
       launch_finish<3>(c.st, gk, rr);
     }
-    HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
-    HIPCHK(hipStreamSynchronize(c.st));
-    bnorm = std::sqrt(f.ctl_h->bb[0]);
-    const double rn = std::sqrt(f.ctl_h->rout[0]);
-    res = bnorm > 0 ? rn / bnorm : 0.0;
-    if (res0 < 0) res0 = res;
-    if (!(res > tol) || outer >= max_outer) {
-      conv = !(res > tol);
-      break;
+    const bool spec = speculate && outer < max_outer;
+    if (!spec) {
+      {
+        ProfScope ps(c, "poisson_final");  // in case this residual ends the solve
+        hipLaunchKernelGGL(k_pout, pg, dim3(64, 4, 1), 0, c.st, g, f.x, c.sD, c.prd1);
+      }
+      HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
+      HIPCHK(hipStreamSynchronize(c.st));
+      bnorm = std::sqrt(f.ctl_h->bb[0]);
+      const double rn = std::sqrt(f.ctl_h->rout[0]);
+      res = bnorm > 0 ? rn / bnorm : 0.0;
+      if (res0 < 0) res0 = res;
+      if (!(res > tol) || outer >= max_outer) {
+        conv = !(res > tol);
+        pout_done = true;
+        break;
+      }
     }
     outer++;
-    // inner target: rel_in of this pass, but no further than the overall tolerance needs
-    const double want = 0.3 * tol * bnorm / rn;
-    const double rin = std::max(rel_in, std::min(want, 1.0));
     const bool use_pc = pc != SULI_PRECOND_JACOBI;
     const int pass_max = pc == SULI_PRECOND_LEARNED ? std::min(kLearnedPassCap, maxit_in - inner_total)
                                                     : maxit_in - inner_total;
     {
       ProfScope ps(c, "fcg_project");
       const RedSlot r3{c.partials, CTL_PTR(f.ctl, rin0)};
-      hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rin, pass_max, r3);
-      launch_finish<1>(c.st, gk, r3, EpiR32{f.ctl, rin, pass_max});
+      hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rel_in, pass_max, r3);
+      launch_finish<1>(c.st, gk, r3, EpiR32{f.ctl, rel_in, tol, pass_max});
     }
     unet_capture(c, f.unet, f.r32, active, 0);
     // Preconditioned passes take few iterations and consecutive solves of a
@@ -1586,7 +1616,24 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       if (use_pc) chunk = 2;
       else if (chunk < 64) chunk *= 2;
     }
+    if (spec) {  // this pass's outer residual, read with its first chunk
+      bnorm = std::sqrt(f.ctl_h->bb[0]);
+      res = bnorm > 0 ? std::sqrt(f.ctl_h->rout[0]) / bnorm : 0.0;
+      if (res0 < 0) res0 = res;
+      if (f.ctl_h->done) {  // it met the tolerance: the pass did not run
+        outer--;
+        conv = true;
+        break;
+      }
+    }
     inner_total += f.ctl_h->it;
+    // the next pass is enqueued with this one's residual check when the FP32
+    // reduction of this pass leaves the residual clearly above the tolerance
+    {
+      const double r0 = f.ctl_h->rin0[0], r1 = f.ctl_h->k2[0];
+      const double pred = (r0 > 0 && r1 >= 0) ? res * std::sqrt(r1 / r0) : 0.0;
+      speculate = pc != SULI_PRECOND_LEARNED && pred > 3.0 * tol;
+    }
     if (use_pc) f.pass_iters[pidx] = f.ctl_h->it;
     const double rr_end = f.ctl_h->k2[0];
     const bool finite = rr_end == rr_end && rr_end < 1e300;
@@ -1624,13 +1671,13 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       break;
     }
   }
-  {
+  if (!pout_done) {
     ProfScope ps(c, "poisson_final");
-    const dim3 pg((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4, g.nz + 2);
     hipLaunchKernelGGL(k_pout, pg, dim3(64, 4, 1), 0, c.st, g, f.x, c.sD, c.prd1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c.st));
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(c.st));
   auto t1 = std::chrono::steady_clock::now();
   if (stt) {
     std::memset(stt, 0, sizeof(*stt));
