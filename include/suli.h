@@ -205,6 +205,18 @@ int suli_precond_apply(const double *r, double *z, int64_t n);
  * weights (suli_unet_init or suli_unet_set_weights follows). */
 int suli_unet_configure(int levels);
 int suli_unet_levels(void);
+/* Where the network enters the multigrid hierarchy (default 0).  0: the
+ * network is the whole preconditioner, z = M(r_s) on the fine grid.  e >= 1:
+ * two-grid form: the V-cycle of SULI_PRECOND_GMG smooths levels 0..e-1 as
+ * usual and the network replaces the coarse solve of level e (the grid
+ * halved e times): on that level's Jacobi-scaled problem it maps
+ * (b~/rms(b~), normalised log of the block-mean density) to a correction,
+ * scaled by the energy-optimal step.  Training samples are then that
+ * level's problems (right-hand side, density and face conductances),
+ * captured in multigrid and learned solves alike.  The grid dimensions must
+ * be multiples of 2^(e+L-1).  Changing the level empties the training set. */
+int suli_unet_set_entry(int level);
+int suli_unet_entry(void);
 int64_t suli_unet_nparams(void);
 /* He-uniform initial weights from a seed; clears the Adam moments. */
 int suli_unet_init(uint64_t seed);

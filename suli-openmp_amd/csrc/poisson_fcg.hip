@@ -1141,6 +1141,7 @@ struct PoissonFCG {
   UNetPrecond *unet = nullptr;
   float *pm[SULI_FCG_M] = {}, *qm[SULI_FCG_M] = {};  // FCG(m) directions and their images
   int pass_iters[4] = {0, 0, 0, 0};                   // inner iterations of the last solve's passes
+  int cur_iter = 0;  // inner iteration of the preconditioner application being enqueued
 };
 
 static int falloc(Ctx &c, float **p, int64_t n) {
@@ -1285,8 +1286,13 @@ struct ZqOut {
 // (levels of at most c.mg_fuse_max cells below the finest: one launch per
 // side), the rest run in k_mg_small.  With c.mg_fuse_fine the input scaling,
 // the prolongations and the output scaling with z.q ride on the neighbouring
-// half sweeps.
-static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const ZqOut *zq = nullptr) {
+// half sweeps.  With net_level = e (1 <= e < number of levels) the levels
+// below e are not visited: the network solves level e's problem
+// (unet_coarse_solve) in place of the coarse cycle.  While the network's
+// training capture is armed at entry level e, level e's problem is appended
+// to the set after its restriction (multigrid and learned solves alike).
+static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const ZqOut *zq = nullptr,
+                      int net_level = 0) {
   PoissonFCG &f = *c.fcg;
   const dim3 b2(SULI_BX, SULI_BY, 1);
   const int nu1 = SULI_MG_NU, nu2 = SULI_MG_NU;
@@ -1294,6 +1300,14 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
   const auto ncell = [&](int l) { return (int64_t)f.lv[l].g.nx * f.lv[l].g.ny * f.lv[l].g.nz; };
   int S = 0;
   while (S < L - 1 && ncell(S) > SULI_MG_SMALL) S++;
+  if (net_level > 0) S = net_level;
+  const int cap_level = unet_capture_armed(f.unet) ? unet_entry(f.unet) : 0;
+  const auto capture = [&](int l) {  // level l's right-hand side is complete
+    if (cap_level > 0 && l == cap_level) {
+      MgLevel &le = f.lv[l];
+      unet_capture_level(c, f.unet, le.g, le.cx, le.cy, le.cz, le.dg, le.b, active, f.cur_iter);
+    }
+  };
   const bool ff = c.mg_fuse_fine && S >= 1;  // the finest level has launches of its own
   // one red-black half sweep of level lf (BF16 faces where the level has them)
   auto gs = [&](MgLevel &lf, const char *name, int color, int zero_x) {
@@ -1335,6 +1349,7 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
       hipLaunchKernelGGL((k_mg_pre_fused<SULI_MG_FT, SULI_MG_FNH>), tiles(lf.g), bt, 0, c.st, lf.g, lc.g,
                          (const float *)lf.cx, (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.dg,
                          (const float *)lf.b, lf.x, lc.b, lc.x, active);
+      capture(l + 1);
       continue;
     }
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
@@ -1355,8 +1370,12 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
       hipLaunchKernelGGL((k_mg_restrict<float>), cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, (const float *)lf.cx,
                          (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.b, (const float *)lf.x,
                          lc.b, lc.x, active);
+    capture(l + 1);
   }
-  {
+  if (net_level > 0) {
+    MgLevel &le = f.lv[S];
+    unet_coarse_solve(c, f.unet, le.g, le.cx, le.cy, le.cz, le.dg, le.b, le.x, active);
+  } else {
     MgSmall m;
     std::memset(&m, 0, sizeof(m));
     for (int l = S; l < L && l - S < SULI_MG_MAXLV; l++) {
@@ -1442,7 +1461,9 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
 // Apply the preconditioner selected for this solve: z = M r.
 static void apply_precond(Ctx &c, int pc, const float *r, float *z, const int *active) {
   if (pc == SULI_PRECOND_LEARNED) {
-    unet_apply(c, c.fcg->unet, r, z, active);
+    const int e = unet_entry(c.fcg->unet);
+    if (e > 0) mg_vcycle(c, r, z, active, nullptr, e);
+    else unet_apply(c, c.fcg->unet, r, z, active);
     return;
   }
   mg_vcycle(c, r, z, active);
@@ -1457,7 +1478,7 @@ static constexpr int kLearnedPassCap = 64;
 static void fcgm_iteration(Ctx &c, PoissonFCG &f, int slot, dim3 gk, const int *active) {
   const Grid &g = c.g;
   const dim3 b2(SULI_BX, SULI_BY, 1);
-  unet_apply(c, f.unet, f.r32, f.z32, active);
+  apply_precond(c, SULI_PRECOND_LEARNED, f.r32, f.z32, active);
   {
     ProfScope ps(c, "fcgm_spmv_dots");
     const RedSlot rz{c.partials, CTL_PTR(f.ctl, zqm)};
@@ -1509,11 +1530,19 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   }
   if (mg_setup(c, !c.mg_fuse_fine)) return 1;
   f.setup_done = true;
+  const int entry = unet_entry(f.unet);
+  if (entry > 0 && entry >= (int)f.lv.size() && (pc == SULI_PRECOND_LEARNED || unet_capture_armed(f.unet))) {
+    suli_set_error("learned preconditioner: entry level " + std::to_string(entry) + " but the multigrid hierarchy has " +
+                   std::to_string(f.lv.size()) + " levels");
+    return 1;
+  }
   if (pc == SULI_PRECOND_LEARNED) {
     if (unet_prepare(c, f.unet)) return 1;
     for (int m = 0; m < SULI_FCG_M; m++)
       if (!f.pm[m] && (falloc(c, &f.pm[m], g.total) || falloc(c, &f.qm[m], g.total))) return 1;
   }
+  if (entry > 0 && (pc == SULI_PRECOND_LEARNED || unet_capture_armed(f.unet)) && unet_level_density(c, f.unet))
+    return 1;
   int outer = 0, inner_total = 0, fallbacks = 0;
   double res0 = -1, res = 0, bnorm = 0;
   bool conv = false;
@@ -1581,6 +1610,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     for (;;) {
       for (int n = 0; n < chunk; n++, iter++) {
         if (iter > 0) unet_capture(c, f.unet, f.r32, active, iter);
+        f.cur_iter = iter;
         if (pc == SULI_PRECOND_LEARNED) {
           fcgm_iteration(c, f, iter % SULI_FCG_M, gk, active);
           continue;
@@ -1699,7 +1729,11 @@ int fcg_precond_apply(Ctx &c, const float *r, float *z) {
     return 1;
   }
   const int pc = c.P.precond;
-  if (pc == SULI_PRECOND_LEARNED && unet_prepare(c, f.unet)) return 1;
+  if (pc == SULI_PRECOND_LEARNED && unet_entry(f.unet) >= (int)f.lv.size()) {
+    suli_set_error("suli_precond_apply: the network's entry level is not a level of the multigrid hierarchy");
+    return 1;
+  }
+  if (pc == SULI_PRECOND_LEARNED && (unet_prepare(c, f.unet) || unet_level_density(c, f.unet))) return 1;
   // the loop-control flag the kernels test: set it for this one application
   FcgCtl *h = f.ctl_h;
   HIPCHK(hipMemcpyAsync(h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));

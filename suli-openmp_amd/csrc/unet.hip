@@ -104,6 +104,14 @@ struct UNetPrecond {
   int ds_max = 0, ds_cap = 0, ds_every = 0;  // armed while ds_max > 0
   int *ds_cnt = nullptr;                       // samples held (device counter)
   float *ds_r = nullptr, *ds_rho = nullptr;
+  // the network as the coarse solver of multigrid level `entry` (0: the
+  // network is the whole preconditioner): block-mean density of that level
+  // for the current solve, and the captured samples' face conductances
+  int entry = 0;
+  float *rho_e = nullptr;
+  int64_t rho_e_cap = 0;
+  int ds_entry = 0;         // entry level the held samples were captured at
+  float *ds_f = nullptr;    // per sample: x, y, z faces of the level (compact)
   double lr1 = 0.0, lr2 = 0.0;  // log-density normalisation: (log rho - lr1)/lr2
   int step = 0;
 };
@@ -183,6 +191,136 @@ __global__ void k_capture(Grid g, Dims d, const float *__restrict__ r, const T *
 __global__ void k_capture_bump(int *cnt, int max, const int *__restrict__ active) {
   if (active && !*active) return;
   if (threadIdx.x == 0 && *cnt < max) *cnt += 1;
+}
+
+// ---- the network as the coarse solver of a multigrid level -----------------
+// Level e of the V-cycle (cells 1..n of its own frame, face conductances cx
+// (face i between cells i-1 and i, zero on the walls), diagonal dg = their
+// sum) holds the problem A x = b.  The network sees it Jacobi-scaled,
+// b~ = b/sqrt(dg), with the block-mean density of the level's cells, and its
+// output z~ maps back to y = z~/sqrt(dg); the correction is x = alpha y with
+// the energy-optimal alpha = (y.b)/(y.A y).
+
+// compact block-mean density of level e: mean of rho over the 2^e-cube of
+// fine cells under each level-e cell
+template <typename T>
+__global__ void k_rho_block(Grid g, Dims d, int e, const T *__restrict__ rho, float *__restrict__ rho_e) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= d.n()) return;
+  const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+  const int m = 1 << e;
+  double s = 0.0;
+  for (int c = 0; c < m; c++)
+    for (int b = 0; b < m; b++)
+      for (int a = 0; a < m; a++) s += (double)rho[g.idx(1 + i * m + a, 1 + j * m + b, 1 + k * m + c)];
+  rho_e[v] = (float)(s / (double)(m * m * m));
+}
+
+// sum over the level's cells of b^2/dg -> partials (512-block grid-stride)
+__global__ __launch_bounds__(256) void k_cs_sumsq(Grid ge, Dims d, const float *__restrict__ b,
+                                                  const float *__restrict__ dg, double *__restrict__ part,
+                                                  const int *__restrict__ active) {
+  if (active && !*active) return;
+  double acc[1] = {0.0};
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < d.n(); v += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+    const fidx c = ge.idx(i + 1, j + 1, k + 1);
+    const double bv = b[c];
+    acc[0] += bv * bv / (double)dg[c];
+  }
+  __shared__ double lds[SULI_LDS_RED];
+  block_reduce<1>(acc, lds);
+  block_partial<1>(acc, part);
+}
+
+// x0 = [b~/rms(b~), (log rho_e - lr1)/lr2]
+__global__ void k_cs_prep(Grid ge, Dims d, const float *__restrict__ b, const float *__restrict__ dg,
+                          const float *__restrict__ rho_e, const double *__restrict__ bb, double lr1, double lr2,
+                          float *__restrict__ x0, const int *__restrict__ active) {
+  if (active && !*active) return;
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= d.n()) return;
+  const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+  const fidx c = ge.idx(i + 1, j + 1, k + 1);
+  const double rms = sqrt(bb[0] / (double)d.n());
+  x0[2 * v] = rms > 0 ? (float)(b[c] / sqrt((double)dg[c]) / rms) : 0.f;
+  x0[2 * v + 1] = (float)((log((double)rho_e[v]) - lr1) / lr2);
+}
+
+// y = z~/sqrt(dg) into the level's frame
+__global__ void k_cs_scatter(Grid ge, Dims d, const float *__restrict__ zc, const float *__restrict__ dg,
+                             float *__restrict__ x, const int *__restrict__ active) {
+  if (active && !*active) return;
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= d.n()) return;
+  const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+  const fidx c = ge.idx(i + 1, j + 1, k + 1);
+  x[c] = zc[v] / sqrtf(dg[c]);
+}
+
+// y.b and y.(A y) -> partials (2 per block); A y = dg y - sum_f c_f y_nb
+__global__ __launch_bounds__(256) void k_cs_dots(Grid ge, Dims d, const float *__restrict__ cx,
+                                                 const float *__restrict__ cy, const float *__restrict__ cz,
+                                                 const float *__restrict__ dg, const float *__restrict__ b,
+                                                 const float *__restrict__ y, double *__restrict__ part,
+                                                 const int *__restrict__ active) {
+  if (active && !*active) return;
+  double acc[2] = {0.0, 0.0};
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < d.n(); v += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+    const fidx c = ge.idx(i + 1, j + 1, k + 1);
+    const float nb = cx[c + 1] * y[c + 1] + cx[c] * y[c - 1] + cy[c + ge.sx] * y[c + ge.sx] + cy[c] * y[c - ge.sx] +
+                     cz[c + ge.sz] * y[c + ge.sz] + cz[c] * y[c - ge.sz];
+    const double yv = y[c], ay = (double)dg[c] * yv - (double)nb;
+    acc[0] += yv * (double)b[c];
+    acc[1] += yv * ay;
+  }
+  __shared__ double lds[2 * SULI_LDS_RED];
+  block_reduce<2>(acc, lds);
+  block_partial<2>(acc, part);
+}
+
+// x = alpha y, alpha = (y.b)/(y.A y) (0 when y.A y is not positive)
+__global__ void k_cs_scale(Grid ge, Dims d, float *__restrict__ x, const double *__restrict__ dots,
+                           const int *__restrict__ active) {
+  if (active && !*active) return;
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= d.n()) return;
+  const double yay = dots[1];
+  const float alpha = yay > 0.0 ? (float)(dots[0] / yay) : 0.f;
+  const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+  const fidx c = ge.idx(i + 1, j + 1, k + 1);
+  x[c] = alpha * x[c];
+}
+
+// training sample of a level's problem: b~ = b/sqrt(dg), rho_e and the face
+// conductances in compact order (x faces i + (nx+1)(j + ny k), i = 0..nx;
+// y and z alike), sample *cnt of the set
+__global__ void k_capture_level(Grid ge, Dims d, const float *__restrict__ b, const float *__restrict__ dg,
+                                const float *__restrict__ cx, const float *__restrict__ cy,
+                                const float *__restrict__ cz, const float *__restrict__ rho_e,
+                                float *__restrict__ rs0, float *__restrict__ rhos0, float *__restrict__ fs0,
+                                const int *__restrict__ cnt, int max, const int *__restrict__ active) {
+  if (active && !*active) return;
+  const int sidx = *cnt;
+  if (sidx >= max) return;
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= d.n()) return;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int64_t nfx = (int64_t)(nx + 1) * ny * nz, nfy = (int64_t)nx * (ny + 1) * nz,
+                nfz = (int64_t)nx * ny * (nz + 1);
+  float *rs = rs0 + (int64_t)sidx * d.n(), *rhos = rhos0 + (int64_t)sidx * d.n();
+  float *fx = fs0 + (int64_t)sidx * (nfx + nfy + nfz), *fy = fx + nfx, *fz = fy + nfy;
+  const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((int64_t)nx * ny));
+  const fidx c = ge.idx(i + 1, j + 1, k + 1);
+  rs[v] = b[c] / sqrtf(dg[c]);
+  rhos[v] = rho_e[v];
+  fx[i + (int64_t)(nx + 1) * (j + (int64_t)ny * k)] = cx[c];
+  if (i == nx - 1) fx[nx + (int64_t)(nx + 1) * (j + (int64_t)ny * k)] = cx[c + 1];
+  fy[i + (int64_t)nx * (j + (int64_t)(ny + 1) * k)] = cy[c];
+  if (j == ny - 1) fy[i + (int64_t)nx * (ny + (int64_t)(ny + 1) * k)] = cy[c + ge.sx];
+  fz[i + (int64_t)nx * (j + (int64_t)ny * k)] = cz[c];
+  if (k == nz - 1) fz[i + (int64_t)nx * (j + (int64_t)ny * nz)] = cz[c + ge.sz];
 }
 
 // Inverse face densities of the scaled operator on the compact grid: face
@@ -277,7 +415,7 @@ __global__ void k_train_dLds(Dims d, const float *__restrict__ r, const float *_
 namespace {
 
 constexpr int kRedPart = 1 << 16;  // reduction partials; results follow
-constexpr int kResNorm = kRedPart, kResDots = kRedPart + 8;
+constexpr int kResNorm = kRedPart, kResDots = kRedPart + 8, kResCs = kRedPart + 16, kResCsDots = kRedPart + 24;
 
 inline Dims level_dims(Dims d0, int l) { return Dims{d0.nx >> l, d0.ny >> l, d0.nz >> l}; }
 
@@ -499,27 +637,39 @@ void refresh_bf16(Ctx &c, UNetPrecond &u) {
                          a.lay[q].ci, u.wf[q], u.wt[q]);
 }
 
+inline int64_t nfaces(Dims d) {
+  return (int64_t)(d.nx + 1) * d.ny * d.nz + (int64_t)d.nx * (d.ny + 1) * d.nz + (int64_t)d.nx * d.ny * (d.nz + 1);
+}
+
 // Loss terms and dL/dz for a compact (r, rho) pair at dims u.d0: dots
 // (r.s, s.s, r.r) with s = A_s z land at red[kResDots..+2], dL/dz in u.dz.
-void loss_and_dz(Ctx &c, UNetPrecond &u, const float *r, const float *rho) {
+// A_s is the scaled operator of the face conductances `faces` (x, y, z
+// faces in compact order, a multigrid level's captured operator) or, when
+// faces is null, of the projection on rho at the context's grid spacing.
+void loss_and_dz(Ctx &c, UNetPrecond &u, const float *r, const float *rho, const float *faces = nullptr) {
   hipStream_t st = c.st;
   const Dims d = u.d0;
   const int64_t n = d.n();
-  const int64_t nf = (int64_t)(d.nx + 1) * d.ny * d.nz + (int64_t)d.nx * (d.ny + 1) * d.nz +
-                     (int64_t)d.nx * d.ny * (d.nz + 1);
-  const float mx = (float)(1.0 / (c.g.dx * c.g.dx)), my = (float)(1.0 / (c.g.dy * c.g.dy)),
-              mz = (float)(1.0 / (c.g.dz * c.g.dz));
-  hipLaunchKernelGGL(k_train_faces, g1(nf), dim3(256), 0, st, d, mx, my, mz, rho, u.cx, u.cy, u.cz);
+  const float *cx = u.cx, *cy = u.cy, *cz = u.cz;
+  if (faces) {
+    cx = faces;
+    cy = cx + (int64_t)(d.nx + 1) * d.ny * d.nz;
+    cz = cy + (int64_t)d.nx * (d.ny + 1) * d.nz;
+  } else {
+    const float mx = (float)(1.0 / (c.g.dx * c.g.dx)), my = (float)(1.0 / (c.g.dy * c.g.dy)),
+                mz = (float)(1.0 / (c.g.dz * c.g.dz));
+    hipLaunchKernelGGL(k_train_faces, g1(nfaces(d)), dim3(256), 0, st, d, mx, my, mz, rho, u.cx, u.cy, u.cz);
+  }
   // |r|^2 for the input normalisation
   hipLaunchKernelGGL(k_train_dots, dim3(512), dim3(256), 0, st, d, r, r, u.red);
   finish_sum(st, u.red, 512, u.red + kResNorm, 3);
   hipLaunchKernelGGL(k_prep_compact, g1(n), dim3(256), 0, st, d, r, rho, u.red + kResNorm + 2, u.lr1, u.lr2, u.x0);
   forward(c, u, nullptr);
-  hipLaunchKernelGGL(k_train_As, g1(n), dim3(256), 0, st, d, u.cx, u.cy, u.cz, u.z, u.s);
+  hipLaunchKernelGGL(k_train_As, g1(n), dim3(256), 0, st, d, cx, cy, cz, u.z, u.s);
   hipLaunchKernelGGL(k_train_dots, dim3(512), dim3(256), 0, st, d, r, u.s, u.red);
   finish_sum(st, u.red, 512, u.red + kResDots, 3);
   hipLaunchKernelGGL(k_train_dLds, g1(n), dim3(256), 0, st, d, r, u.s, u.red + kResDots, u.t);
-  hipLaunchKernelGGL(k_train_As, g1(n), dim3(256), 0, st, d, u.cx, u.cy, u.cz, u.t, u.dz);
+  hipLaunchKernelGGL(k_train_As, g1(n), dim3(256), 0, st, d, cx, cy, cz, u.t, u.dz);
 }
 
 // (re)build the parameter storage for an L-level network
@@ -569,18 +719,79 @@ void unet_destroy(Ctx &c, UNetPrecond *u) {
 
 bool unet_ready(const UNetPrecond *u) { return u && u->ready; }
 
+// grid of the network: the fine grid (entry 0) or multigrid level `entry`
+static Dims entry_dims(const Grid &g, int e) { return Dims{g.nx >> e, g.ny >> e, g.nz >> e}; }
+static bool entry_fits(const Grid &g, int e, int L) {
+  const int m = 1 << (e + L - 1);
+  return g.nx % m == 0 && g.ny % m == 0 && g.nz % m == 0;
+}
+static std::string entry_msg(const UNetPrecond &u) {
+  return "nx, ny and nz must be multiples of 2^(entry+levels-1) = " + std::to_string(1 << (u.entry + u.a.L - 1)) +
+         " (entry level " + std::to_string(u.entry) + ", " + std::to_string(u.a.L) + " levels)";
+}
+
 int unet_prepare(Ctx &c, UNetPrecond *u) {
   if (!u || !u->ready) {
     suli_set_error("learned preconditioner selected but no weights are loaded (suli_unet_init or "
                    "suli_unet_set_weights)");
     return 1;
   }
-  if (!dims_ok(c.g, u->a.L)) {
-    suli_set_error("learned preconditioner: nx, ny and nz must be multiples of 2^(levels-1) = " +
-                   std::to_string(1 << (u->a.L - 1)));
+  if (!entry_fits(c.g, u->entry, u->a.L)) {
+    suli_set_error("learned preconditioner: " + entry_msg(*u));
     return 1;
   }
-  return alloc_acts(c, *u, Dims{c.g.nx, c.g.ny, c.g.nz});
+  return alloc_acts(c, *u, entry_dims(c.g, u->entry));
+}
+
+int unet_entry(const UNetPrecond *u) { return u ? u->entry : 0; }
+
+bool unet_capture_armed(const UNetPrecond *u) { return u && u->ds_max > 0; }
+
+int unet_level_density(Ctx &c, UNetPrecond *u) {
+  if (!u || u->entry <= 0) return 0;
+  const Dims d = entry_dims(c.g, u->entry);
+  if (d.n() > u->rho_e_cap) {
+    if (alloc_f(c, &u->rho_e, d.n())) return 1;
+    u->rho_e_cap = d.n();
+  }
+  if (c.prec == SULI_PREC_FP64)
+    hipLaunchKernelGGL((k_rho_block<double>), g1(d.n()), dim3(256), 0, c.st, c.g, d, u->entry,
+                       (const double *)c.rho, u->rho_e);
+  else
+    hipLaunchKernelGGL((k_rho_block<float>), g1(d.n()), dim3(256), 0, c.st, c.g, d, u->entry,
+                       (const float *)c.rho, u->rho_e);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+void unet_coarse_solve(Ctx &c, UNetPrecond *u, const Grid &ge, const float *cx, const float *cy, const float *cz,
+                       const float *dg, const float *b, float *x, const int *active) {
+  const Dims d{ge.nx, ge.ny, ge.nz};
+  const int64_t n = d.n();
+  ProfScope ps(c, "unet_coarse_solve");
+  hipLaunchKernelGGL(k_cs_sumsq, dim3(512), dim3(256), 0, c.st, ge, d, b, dg, u->red, active);
+  finish_sum(c.st, u->red, 512, u->red + kResCs, 1);
+  hipLaunchKernelGGL(k_cs_prep, g1(n), dim3(256), 0, c.st, ge, d, b, dg, (const float *)u->rho_e,
+                     (const double *)(u->red + kResCs), u->lr1, u->lr2, u->x0, active);
+  forward(c, *u, active);
+  hipLaunchKernelGGL(k_cs_scatter, g1(n), dim3(256), 0, c.st, ge, d, (const float *)u->z, dg, x, active);
+  hipLaunchKernelGGL(k_cs_dots, dim3(512), dim3(256), 0, c.st, ge, d, cx, cy, cz, dg, b, (const float *)x, u->red,
+                     active);
+  hipLaunchKernelGGL((k_finish<2, false, EpiNone>), dim3(1), dim3(256), 0, c.st, u->red, 512, u->red + kResCsDots,
+                     EpiNone());
+  hipLaunchKernelGGL(k_cs_scale, g1(n), dim3(256), 0, c.st, ge, d, x, (const double *)(u->red + kResCsDots),
+                     active);
+}
+
+void unet_capture_level(Ctx &c, UNetPrecond *u, const Grid &ge, const float *cx, const float *cy, const float *cz,
+                        const float *dg, const float *b, const int *active, int iter) {
+  if (!u || u->ds_max <= 0 || u->ds_entry <= 0 || !u->rho_e) return;
+  if (iter > 0 && (u->ds_every <= 0 || iter % u->ds_every != 0)) return;
+  const Dims d{ge.nx, ge.ny, ge.nz};
+  if (d.nx != u->dsd.nx || d.ny != u->dsd.ny || d.nz != u->dsd.nz) return;
+  hipLaunchKernelGGL(k_capture_level, g1(d.n()), dim3(256), 0, c.st, ge, d, b, dg, cx, cy, cz,
+                     (const float *)u->rho_e, u->ds_r, u->ds_rho, u->ds_f, u->ds_cnt, u->ds_max, active);
+  hipLaunchKernelGGL(k_capture_bump, dim3(1), dim3(64), 0, c.st, u->ds_cnt, u->ds_max, active);
 }
 
 void unet_apply(Ctx &c, UNetPrecond *u, const float *r, float *z, const int *active) {
@@ -604,7 +815,7 @@ void unet_apply(Ctx &c, UNetPrecond *u, const float *r, float *z, const int *act
 }
 
 void unet_capture(Ctx &c, UNetPrecond *u, const float *r, const int *active, int iter) {
-  if (!u || u->ds_max <= 0) return;
+  if (!u || u->ds_max <= 0 || u->ds_entry != 0) return;
   if (iter > 0 && (u->ds_every <= 0 || iter % u->ds_every != 0)) return;
   const Grid &g = c.g;
   const Dims d{g.nx, g.ny, g.nz};
@@ -718,8 +929,8 @@ extern "C" int suli_unet_capture(int max_samples, int every) {
   NEEDU();
   Ctx &c = *g_ctx;
   UNetPrecond &u = *fcg_unet(c);
-  if (!dims_ok(c.g, u.a.L)) {
-    suli_set_error("suli_unet_capture: nx, ny and nz must be multiples of 2^(levels-1)");
+  if (!entry_fits(c.g, u.entry, u.a.L)) {
+    suli_set_error("suli_unet_capture: " + entry_msg(u));
     return 1;
   }
   if (max_samples < 0 || every < 0) {
@@ -730,8 +941,8 @@ extern "C" int suli_unet_capture(int max_samples, int every) {
     u.ds_max = 0;
     return 0;
   }
-  const Dims d{c.g.nx, c.g.ny, c.g.nz};
-  const int64_t n = d.n();
+  const Dims d = entry_dims(c.g, u.entry);
+  const int64_t n = d.n(), nf = u.entry > 0 ? nfaces(d) : 0;
   if (!u.ds_cnt) {
     HIPCHK(hipMalloc((void **)&u.ds_cnt, sizeof(int)));
     c.allocs.push_back(u.ds_cnt);
@@ -741,22 +952,49 @@ extern "C" int suli_unet_capture(int max_samples, int every) {
     int held = 0;
     if (dataset_count(c, u, &held)) return 1;
     held = std::min(held, u.ds_cap);
-    float *a = nullptr, *b = nullptr;
+    float *a = nullptr, *b = nullptr, *f = nullptr;
     if (alloc_f(c, &a, (int64_t)max_samples * n) || alloc_f(c, &b, (int64_t)max_samples * n)) return 1;
+    if (nf > 0 && alloc_f(c, &f, (int64_t)max_samples * nf)) return 1;
     if (held > 0) {
       HIPCHK(hipMemcpyAsync(a, u.ds_r, (size_t)held * n * sizeof(float), hipMemcpyDeviceToDevice, c.st));
       HIPCHK(hipMemcpyAsync(b, u.ds_rho, (size_t)held * n * sizeof(float), hipMemcpyDeviceToDevice, c.st));
+      if (nf > 0) HIPCHK(hipMemcpyAsync(f, u.ds_f, (size_t)held * nf * sizeof(float), hipMemcpyDeviceToDevice, c.st));
       HIPCHK(hipStreamSynchronize(c.st));
     }
     // the old buffers stay in the context's allocation list until suli_finalize
     u.ds_r = a;
     u.ds_rho = b;
+    u.ds_f = f;
     u.ds_cap = max_samples;
   }
   u.dsd = d;
+  u.ds_entry = u.entry;
   u.ds_max = max_samples;
   u.ds_every = every;
   return 0;
+}
+
+extern "C" int suli_unet_set_entry(int level) {
+  NEEDU();
+  Ctx &c = *g_ctx;
+  UNetPrecond &u = *fcg_unet(c);
+  if (level < 0 || level > 4) {
+    suli_set_error("suli_unet_set_entry: level must be in 0..4");
+    return 1;
+  }
+  if (level != u.entry) {  // the held samples belong to the old level: drop them
+    u.entry = level;
+    u.ds_max = 0;
+    u.ds_cap = 0;
+    if (u.ds_cnt) HIPCHK(hipMemsetAsync(u.ds_cnt, 0, sizeof(int), c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+  }
+  return 0;
+}
+
+extern "C" int suli_unet_entry(void) {
+  if (!g_ctx || !g_ctx->fcg) return -1;
+  return fcg_unet(*g_ctx)->entry;
 }
 
 extern "C" int suli_unet_clear(void) {
@@ -828,7 +1066,7 @@ extern "C" int suli_unet_train(int iters, double lr, uint64_t seed, double *loss
     return 1;
   }
   if (alloc_acts(c, u, u.dsd)) return 1;
-  const int64_t n = (int64_t)u.dsd.nx * u.dsd.ny * u.dsd.nz;
+  const int64_t n = (int64_t)u.dsd.nx * u.dsd.ny * u.dsd.nz, nf = u.ds_entry > 0 ? nfaces(u.dsd) : 0;
   std::mt19937_64 rng(seed);
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   // the last min(iters, 32) losses are read back for the report
@@ -844,7 +1082,7 @@ extern "C" int suli_unet_train(int iters, double lr, uint64_t seed, double *loss
   double *rep_d = rep_buf.p;
   for (int it = 0; it < iters; it++) {
     const int s = (int)(rng() % (uint64_t)nset);
-    loss_and_dz(c, u, u.ds_r + (int64_t)s * n, u.ds_rho + (int64_t)s * n);
+    loss_and_dz(c, u, u.ds_r + (int64_t)s * n, u.ds_rho + (int64_t)s * n, nf > 0 ? u.ds_f + (int64_t)s * nf : nullptr);
     backward(c, u);
     u.step++;
     const double c1 = 1.0 - std::pow((double)b1, u.step), c2 = 1.0 - std::pow((double)b2, u.step);

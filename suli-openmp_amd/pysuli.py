@@ -105,6 +105,7 @@ def lib():
             "suli_prof_enable": ([i32], i32), "suli_prof_select": ([C.c_char_p, i32], i32),
             "suli_prof_count": ([], i32), "suli_prof_get": ([i32, C.c_char_p, i32, p(C.c_int64), p(f64)], i32),
             "suli_unet_nparams": ([], C.c_int64), "suli_unet_configure": ([i32], i32), "suli_unet_levels": ([], i32),
+            "suli_unet_set_entry": ([i32], i32), "suli_unet_entry": ([], i32),
             "suli_unet_init": ([C.c_uint64], i32),
             "suli_unet_set_weights": ([p(C.c_float), C.c_int64], i32),
             "suli_unet_get_weights": ([p(C.c_float), C.c_int64], i32),
@@ -274,6 +275,12 @@ class Solver:
 
     def unet_levels(self):
         return int(lib().suli_unet_levels())
+
+    def unet_set_entry(self, level):
+        _check(lib().suli_unet_set_entry(level), "suli_unet_set_entry")
+
+    def unet_entry(self):
+        return int(lib().suli_unet_entry())
 
     def unet_init(self, seed):
         _check(lib().suli_unet_init(seed), "suli_unet_init")

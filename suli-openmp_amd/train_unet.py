@@ -13,7 +13,17 @@ the seed; water/air 1000/1.2 kg/m^3; dt = 0.2 h / (2 sqrt(g H))):
    that the network learns the residuals of its own iterations (a solve that
    stalls falls back to multigrid, whose iterations are captured as well).
 
+With --entry e >= 1 the network is the coarse solver of multigrid level e
+(two-grid form, see suli_unet_set_entry): the samples are then level e's
+problems (right-hand side, block density and face conductances) met by the
+V-cycle in both phases, and the loss is taken on that level's operator.
+
     python3 train_unet.py --nx 128 --levels 6 --out weights/unet_dambreak_L6.npy
+    python3 train_unet.py --nx 256 --entry 1 --levels 5 --out weights/unet_dambreak_e1_L5.npy
+
+--eval-seed adds a held-out column geometry: after training, one projection
+of that tank is solved with Jacobi, multigrid and the learned preconditioner
+and the times are written to the log.
 
 Writes the FP32 weight vector (.npy) and a JSON log beside it.
 """
@@ -78,17 +88,44 @@ class Tank:
         return out
 
 
+def evaluate(tank, seed, warm_steps, repeats=3):
+    """ms per solve of one projection of a held-out tank with each preconditioner."""
+    s = tank.s
+    tank.reset(seed)
+    tank.steps(warm_steps, S.PRECOND_GMG)
+    dt = tank.dt
+    s.level_set(dt)
+    s.contorno(3)
+    s.visco()
+    s.convdiff()
+    s.tempo(1, 0.5 * dt)
+    s.contorno(2)
+    p0 = s.download("prd1")
+    out = {}
+    for name, pc in (("jacobi", S.PRECOND_JACOBI), ("gmg", S.PRECOND_GMG), ("learned", S.PRECOND_LEARNED)):
+        s.set_solver(S.POISSON_FCG, pc)
+        runs = []
+        for _ in range(repeats):
+            s.upload("prd1", p0)
+            runs.append(s.graddin(0.5 * dt))
+        ms = sorted(r["ms"] for r in runs)
+        out[name] = {"ms": ms[len(ms) // 2], "iters": runs[-1]["iters"], "fallbacks": runs[-1]["fallbacks"],
+                     "res": runs[-1]["res"]}
+    return out
+
+
 def train(args):
     t0 = time.time()
     tank = Tank(args.nx)
     s = tank.s
     s.unet_configure(args.levels)
+    s.unet_set_entry(args.entry)
     if args.init:
         s.unet_set_weights(np.load(args.init))
     else:
         s.unet_init(args.seed)
-    log = {"nx": args.nx, "ny": tank.ny, "nz": tank.nz, "levels": args.levels, "nparams": s.unet_nparams(),
-           "rounds": []}
+    log = {"nx": args.nx, "ny": tank.ny, "nz": tank.nz, "levels": args.levels, "entry": args.entry,
+           "nparams": s.unet_nparams(), "rounds": []}
     seeds = list(range(args.seed, args.seed + args.seeds))
     # 1. multigrid trajectories
     s.unet_capture(args.capacity, 1)
@@ -119,6 +156,9 @@ def train(args):
               f"loss {loss:.4f} {rec['seconds']:.1f}s", flush=True)
         if rec["samples"] >= args.capacity:  # keep the newest trajectories: restart the set
             s.unet_clear()
+    if args.eval_seed is not None:
+        log["eval"] = {"seed": args.eval_seed, **evaluate(tank, args.eval_seed, args.warm_steps)}
+        print("eval", json.dumps(log["eval"]), flush=True)
     w = s.unet_get_weights()
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, w)
@@ -133,6 +173,8 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--nx", type=int, default=128, help="grid nx (ny = nz = nx/2)")
     ap.add_argument("--levels", type=int, default=6)
+    ap.add_argument("--entry", type=int, default=0, help="multigrid level the network solves (0: whole grid)")
+    ap.add_argument("--eval-seed", type=int, default=None, help="held-out geometry timed after training")
     ap.add_argument("--seeds", type=int, default=4, help="number of column geometries")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--warm-steps", type=int, default=8, help="multigrid steps captured per geometry")
