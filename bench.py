@@ -28,8 +28,9 @@ line carries:
   made by --pmc-summary from counter runs of this script, scaled by a 256 MiB
   device copy measured the same way);
 - poisson.comparison: ms per solve of one projection state with the plain
-  (Jacobi) FCG, the multigrid FCG and the learned preconditioner
-  (suli-openmp_amd/weights), all to a 1e-8 FP64 relative residual;
+  (Jacobi) FCG, the multigrid FCG and each shipped learned preconditioner
+  (suli-openmp_amd/weights; the default one as "learned"), all to a 1e-8
+  FP64 relative residual;
 - cpu_baseline: the flang-built reference harness (oracle/_ref) timed on one
   host core on a bounded sample of the same workload (one level_set plus
   one RK stage from the GPU's post-warmup state; per-step time = level_set
@@ -199,24 +200,14 @@ def _cpu_model():
     return "unknown"
 
 
-WEIGHTS_DIR = os.path.join(ROOT, "suli-openmp_amd", "weights")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_256x128x128.json")
-
-
-def learned_weights():
-    """(levels, weights) of the shipped learned preconditioner, or None."""
-    import numpy as np
-    best = None
-    for f in sorted(os.listdir(WEIGHTS_DIR)) if os.path.isdir(WEIGHTS_DIR) else []:
-        if f.endswith(".npy") and os.path.exists(os.path.join(WEIGHTS_DIR, f[:-4] + ".json")):
-            meta = json.load(open(os.path.join(WEIGHTS_DIR, f[:-4] + ".json")))
-            best = (int(meta["levels"]), np.load(os.path.join(WEIGHTS_DIR, f)), f)
-    return best
 
 
 def poisson_comparison(sv, dt, repeats):
     """ms per solve of the same projection (the first RK stage of the next
-    step, warm-started from the current pressure) with each preconditioner."""
+    step, warm-started from the current pressure) with each preconditioner:
+    Jacobi (plain), multigrid, and every shipped learned preconditioner (the
+    default one as "learned", all of them under "learned_by_entry")."""
     import pysuli
     sv.level_set(dt)
     sv.contorno(3)
@@ -226,30 +217,39 @@ def poisson_comparison(sv, dt, repeats):
     sv.contorno(2)
     p0 = sv.download("prd1")
     out = {}
-    cases = [("jacobi", pysuli.PRECOND_JACOBI), ("gmg", pysuli.PRECOND_GMG)]
-    lw = learned_weights()
-    if lw is not None:
-        levels, w, fname = lw
-        sv.unet_configure(levels)
-        sv.unet_set_weights(w)
-        cases.append(("learned", pysuli.PRECOND_LEARNED))
-    for name, pc in cases:
+    cases = [("jacobi", pysuli.PRECOND_JACOBI, None), ("gmg", pysuli.PRECOND_GMG, None)]
+    shipped = [m for m in pysuli.shipped_learned() if sv.nx % (1 << (m["entry"] + m["levels"] - 1)) == 0
+               and sv.ny % (1 << (m["entry"] + m["levels"] - 1)) == 0]
+    cases += [(m["name"], pysuli.PRECOND_LEARNED, m) for m in shipped]
+    for name, pc, meta in cases:
+        if meta is not None:
+            pysuli.use_learned(sv, meta)
         sv.set_solver(pysuli.POISSON_FCG, pc)
         runs = []
         for _ in range(repeats):
             sv.upload("prd1", p0)
             runs.append(sv.graddin(0.5 * dt))
         ms = sorted(r["ms"] for r in runs)
-        out[name] = {"ms_per_solve": round(ms[len(ms) // 2], 3), "ms_min": round(ms[0], 3),
-                     "iters": runs[-1]["iters"], "outer": runs[-1]["outer"], "fallbacks": runs[-1]["fallbacks"],
-                     "converged": runs[-1]["converged"], "relres": runs[-1]["res"]}
+        rec = {"ms_per_solve": round(ms[len(ms) // 2], 3), "ms_min": round(ms[0], 3),
+               "iters": runs[-1]["iters"], "outer": runs[-1]["outer"], "fallbacks": runs[-1]["fallbacks"],
+               "converged": runs[-1]["converged"], "relres": runs[-1]["res"]}
+        if meta is None:
+            out[name] = rec
+            continue
+        rec.update(entry=meta["entry"], levels=meta["levels"], weights=f"suli-openmp_amd/weights/{name}.npy",
+                   plain_over=round(out["jacobi"]["ms_per_solve"] / rec["ms_per_solve"], 3))
+        out.setdefault("learned_by_entry", {})[name] = rec
+        if meta["default"] or "learned" not in out:
+            out["learned"] = rec
     if "learned" in out:
-        out["learned"]["weights"] = f"suli-openmp_amd/weights/{lw[2]} ({lw[0]} levels)"
-        out["plain_over_learned"] = round(out["jacobi"]["ms_per_solve"] / out["learned"]["ms_per_solve"], 3)
+        out["plain_over_learned"] = out["learned"]["plain_over"]
+        out["gmg_over_learned"] = round(out["gmg"]["ms_per_solve"] / out["learned"]["ms_per_solve"], 3)
     out["plain_over_gmg"] = round(out["jacobi"]["ms_per_solve"] / out["gmg"]["ms_per_solve"], 3)
     out["note"] = ("one first-stage projection of the 256x128x128 state after the timed steps, warm-started "
                    f"from the current pressure, median of {repeats} solves, FP64 relative residual <= 1e-8; "
-                   "plain = Jacobi-FCG (the reference's symmetric scaling)")
+                   "plain = Jacobi-FCG (the reference's symmetric scaling); learned = the default shipped "
+                   "network; entry = the multigrid level a network solves (0 = the whole grid); the entry >= 1 "
+                   "networks were trained on the column geometries of seeds 1-6, not on this one")
     return out
 
 
@@ -380,6 +380,11 @@ def main():
     sv = pysuli.Solver(nx, ny, nz, h, precision=pysuli.PREC_MIXED, poisson=pysuli.POISSON_FCG, precond=pc,
                        tol_rel=1e-8)
     L = pysuli.lib()
+    if args.precond == "learned":  # the default shipped network
+        shipped = pysuli.shipped_learned()
+        if not shipped:
+            sys.exit("bench.py: --precond learned needs the weights in suli-openmp_amd/weights")
+        pysuli.use_learned(sv, shipped[0])
     vol_ini = sv.init_dambreak(width, height)
     sv.contorno(1)
     progress(f"{nx}x{ny}x{nz} initialised; {args.warmup} warm-up steps")

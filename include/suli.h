@@ -217,6 +217,11 @@ int suli_unet_levels(void);
  * be multiples of 2^(e+L-1).  Changing the level empties the training set. */
 int suli_unet_set_entry(int level);
 int suli_unet_entry(void);
+/* Directions FCG keeps with the learned preconditioner: 4 (default: each new
+ * direction A-orthogonalised against the last four, FCG(4)) or 1 (FCG(1),
+ * the multigrid iteration; used only with entry levels >= 1, where the
+ * network is one part of a V-cycle). */
+int suli_unet_set_fcg_m(int m);
 int64_t suli_unet_nparams(void);
 /* He-uniform initial weights from a seed; clears the Adam moments. */
 int suli_unet_init(uint64_t seed);
@@ -244,6 +249,16 @@ int suli_unet_train(int iters, double lr, uint64_t seed, double *loss);
  * output z (may be null) for one sample given on the host in cell order
  * (i fastest, nx*ny*nz values): r is the scaled residual, rho the density. */
 int suli_unet_loss_grad(const double *r, const double *rho, double *loss, float *grad, float *z);
+/* The held training set, for inspection and tests: its grid (the network's
+ * grid at the entry level it was captured at) and sample idx: r (the scaled
+ * right-hand side), rho (density) and, for entry >= 1, the level's face
+ * conductances (x faces (nx+1)*ny*nz in order i + (nx+1)*(j + ny*k), then y
+ * and z faces alike; zero on the walls).  Any pointer may be null. */
+int suli_unet_dataset_dims(int *nx, int *ny, int *nz, int *entry);
+int suli_unet_dataset_get(int idx, float *r, float *rho, float *faces);
+/* Loss, weight gradient and network output of held sample idx, through the
+ * operator that training uses for it. */
+int suli_unet_sample_loss_grad(int idx, double *loss, float *grad, float *z);
 
 /* Per-kernel GPU timing: HIP events recorded on the library's stream around
  * every kernel launch while enabled.  suli_prof_enable(1) clears the totals;

@@ -256,7 +256,57 @@ def face_coefs(rho, dx, dy, dz):
 def apply_As(z, rho, dx, dy, dz):
     """A_s z = z - sum_f c_f/sqrt(D_P D_Q) z_Q, a wall face coupling the cell
     to itself (Neumann ghost)."""
-    cx, cy, cz = face_coefs(rho, dx, dy, dz)
+    return apply_As_faces(z, *face_coefs(rho, dx, dy, dz))
+
+
+def mg_faces(rho, dt, dx, dy, dz, level):
+    """Face conductances of multigrid level `level` of the FCG preconditioner
+    (no counterpart in the reference): on the finest grid dt/(h^2 rho_f) with
+    the face densities of face_coefs (the graddin coefficients), rounded to
+    FP32 and then BF16, zero on the wall faces (the Neumann operator without
+    the ghost coupling); each coarser face is half the sum, in FP32, of the
+    four fine faces on it.  rho as [k, j, i]; returns (cx, cy, cz) with n+1
+    faces along their own axis."""
+    rho = np.asarray(rho, dtype=np.float64)
+
+    def fine(axis, h):
+        a = np.moveaxis(rho, axis, 0)
+        n = a.shape[0]
+        f = np.zeros((n + 1,) + a.shape[1:])
+        f[1:n] = (dt / (h * h)) / ((a[1:] + a[:-1]) * 0.5)
+        return np.moveaxis(bf16(f), 0, axis).astype(np.float32)
+
+    c = [fine(2, dx), fine(1, dy), fine(0, dz)]
+    h = np.float32(0.5)
+    for _ in range(level):
+        cx, cy, cz = c
+        # coarse face (K, J, I) of each direction: fine faces at 2I (2J, 2K)
+        # along the face normal, the 2x2 fine faces over it summed in the
+        # order (a, b), (a+1, b), (a, b+1), (a+1, b+1) of the two tangential
+        # indices (x faces: j then k; y faces: i then k; z faces: i then j)
+        x = cx[:, :, ::2]
+        cx = h * (((x[0::2, 0::2] + x[0::2, 1::2]) + x[1::2, 0::2]) + x[1::2, 1::2])
+        y = cy[:, ::2, :]
+        cy = h * (((y[0::2, :, 0::2] + y[0::2, :, 1::2]) + y[1::2, :, 0::2]) + y[1::2, :, 1::2])
+        zf = cz[::2]
+        cz = h * (((zf[:, 0::2, 0::2] + zf[:, 0::2, 1::2]) + zf[:, 1::2, 0::2]) + zf[:, 1::2, 1::2])
+        c = [cx, cy, cz]
+    return tuple(np.asarray(a, dtype=np.float64) for a in c)
+
+
+def block_mean(rho, level):
+    """Mean of rho over the 2^level-cubes of cells (the density input of the
+    network on multigrid level `level`)."""
+    m = 1 << level
+    nz, ny, nx = rho.shape
+    return np.asarray(rho, dtype=np.float64).reshape(nz // m, m, ny // m, m, nx // m, m).mean(axis=(1, 3, 5))
+
+
+def apply_As_faces(z, cx, cy, cz):
+    """A_s z = z - sum_f c_f/sqrt(D_P D_Q) z_Q for face conductances cx, cy,
+    cz (n+1 faces along their own axis, face f between cells f-1 and f),
+    D = the sum of a cell's six faces, a wall face coupling the cell to itself
+    (Neumann ghost)."""
     D = cx[:, :, 1:] + cx[:, :, :-1] + cy[:, 1:] + cy[:, :-1] + cz[1:] + cz[:-1]
     s = np.asarray(z, dtype=np.float64).copy()
 
@@ -279,19 +329,23 @@ def apply_As(z, rho, dx, dy, dz):
     return s
 
 
-def loss_and_grad(wvec, r, rho, rho_f1, rho_f2, dx, dy, dz, bf=False):
-    """L = 1 - (r.s)^2/(|r|^2 |s|^2), s = A_s z(r, rho); returns (L, dL/dw, z)."""
+def loss_and_grad(wvec, r, rho, rho_f1, rho_f2, dx, dy, dz, bf=False, faces=None):
+    """L = 1 - (r.s)^2/(|r|^2 |s|^2), s = A_s z(r, rho); returns (L, dL/dw, z).
+    A_s is the scaled projection operator of rho, or that of the face
+    conductances `faces` = (cx, cy, cz) when given (a multigrid level)."""
     r64 = f32(r) if bf else np.asarray(r, dtype=np.float64)
     rho64 = f32(rho) if bf else np.asarray(rho, dtype=np.float64)
     x0 = inputs(r, rho, rho_f1, rho_f2, bf)
     z, A = forward(wvec, x0, bf)
-    s = apply_As(z, rho64, dx, dy, dz)
+    if faces is None:
+        faces = face_coefs(rho64, dx, dy, dz)
+    s = apply_As_faces(z, *faces)
     rs, ss, rr = float(np.sum(r64 * s)), float(np.sum(s * s)), float(np.sum(r64 * r64))
     L = 1.0 - rs * rs / (rr * ss)
     t = -2.0 * rs / (rr * ss) * r64 + 2.0 * rs * rs / (rr * ss * ss) * s
     if bf:
         t = f32(t)
-    dz_ = apply_As(t, rho64, dx, dy, dz)
+    dz_ = apply_As_faces(t, *faces)
     if bf:
         dz_ = f32(dz_)
     return L, backward(dz_, A, bf), z

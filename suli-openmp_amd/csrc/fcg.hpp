@@ -21,6 +21,10 @@ void unet_capture(Ctx &c, UNetPrecond *u, const float *r, const int *active, int
 // preconditioner (unet_apply); e >= 1 = the V-cycle smooths levels 0..e-1
 // and the network replaces the coarse solve of level e (unet_coarse_solve).
 int unet_entry(const UNetPrecond *u);
+// directions kept by FCG with the learned preconditioner: 4 (FCG(4), each new
+// direction A-orthogonalised against the last four) or 1 (FCG(1), as with
+// multigrid; entry levels >= 1 only)
+int unet_fcg_m(const UNetPrecond *u);
 bool unet_capture_armed(const UNetPrecond *u);
 // block-mean density of the entry level for this solve (entry >= 1)
 int unet_level_density(Ctx &c, UNetPrecond *u);

@@ -1290,7 +1290,9 @@ struct ZqOut {
 // below e are not visited: the network solves level e's problem
 // (unet_coarse_solve) in place of the coarse cycle.  While the network's
 // training capture is armed at entry level e, level e's problem is appended
-// to the set after its restriction (multigrid and learned solves alike).
+// to the set after its restriction (multigrid and learned solves alike; the
+// levels above e then run their own launches even where they would fit the
+// single-work-group kernel).
 static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const ZqOut *zq = nullptr,
                       int net_level = 0) {
   PoissonFCG &f = *c.fcg;
@@ -1300,8 +1302,9 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
   const auto ncell = [&](int l) { return (int64_t)f.lv[l].g.nx * f.lv[l].g.ny * f.lv[l].g.nz; };
   int S = 0;
   while (S < L - 1 && ncell(S) > SULI_MG_SMALL) S++;
-  if (net_level > 0) S = net_level;
   const int cap_level = unet_capture_armed(f.unet) ? unet_entry(f.unet) : 0;
+  if (net_level > 0) S = net_level;
+  else if (cap_level > S) S = cap_level;  // level cap_level's right-hand side is built by launches
   const auto capture = [&](int l) {  // level l's right-hand side is complete
     if (cap_level > 0 && l == cap_level) {
       MgLevel &le = f.lv[l];
@@ -1543,6 +1546,9 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   }
   if (entry > 0 && (pc == SULI_PRECOND_LEARNED || unet_capture_armed(f.unet)) && unet_level_density(c, f.unet))
     return 1;
+  // FCG(1) with the network as a multigrid level's coarse solver (the
+  // multigrid iteration with that level's solve replaced), when selected
+  bool learned_m1 = pc == SULI_PRECOND_LEARNED && entry > 0 && unet_fcg_m(f.unet) == 1;
   int outer = 0, inner_total = 0, fallbacks = 0;
   double res0 = -1, res = 0, bnorm = 0;
   bool conv = false;
@@ -1611,7 +1617,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       for (int n = 0; n < chunk; n++, iter++) {
         if (iter > 0) unet_capture(c, f.unet, f.r32, active, iter);
         f.cur_iter = iter;
-        if (pc == SULI_PRECOND_LEARNED) {
+        if (pc == SULI_PRECOND_LEARNED && !learned_m1) {
           fcgm_iteration(c, f, iter % SULI_FCG_M, gk, active);
           continue;
         }
@@ -1619,7 +1625,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
         if (use_pc) {
           const ZqOut zo{f.q32, (const int *)((char *)f.ctl + offsetof(FcgCtl, first)),
                          RedSlot{c.partials, CTL_PTR(f.ctl, zq)}};
-          mg_vcycle(c, f.r32, f.z32, active, &zo);
+          mg_vcycle(c, f.r32, f.z32, active, &zo, pc == SULI_PRECOND_LEARNED ? entry : 0);
           zin = f.z32;
         }
         {

@@ -110,6 +110,7 @@ struct UNetPrecond {
   int entry = 0;
   float *rho_e = nullptr;
   int64_t rho_e_cap = 0;
+  int fcg_m = 4;            // directions FCG keeps for this preconditioner (1 or 4)
   int ds_entry = 0;         // entry level the held samples were captured at
   float *ds_f = nullptr;    // per sample: x, y, z faces of the level (compact)
   double lr1 = 0.0, lr2 = 0.0;  // log-density normalisation: (log rho - lr1)/lr2
@@ -745,6 +746,8 @@ int unet_prepare(Ctx &c, UNetPrecond *u) {
 
 int unet_entry(const UNetPrecond *u) { return u ? u->entry : 0; }
 
+int unet_fcg_m(const UNetPrecond *u) { return u ? u->fcg_m : 4; }
+
 bool unet_capture_armed(const UNetPrecond *u) { return u && u->ds_max > 0; }
 
 int unet_level_density(Ctx &c, UNetPrecond *u) {
@@ -992,6 +995,16 @@ extern "C" int suli_unet_set_entry(int level) {
   return 0;
 }
 
+extern "C" int suli_unet_set_fcg_m(int m) {
+  NEEDU();
+  if (m != 1 && m != 4) {
+    suli_set_error("suli_unet_set_fcg_m: m must be 1 or 4");
+    return 1;
+  }
+  fcg_unet(*g_ctx)->fcg_m = m;
+  return 0;
+}
+
 extern "C" int suli_unet_entry(void) {
   if (!g_ctx || !g_ctx->fcg) return -1;
   return fcg_unet(*g_ctx)->entry;
@@ -1036,6 +1049,63 @@ extern "C" int suli_unet_loss_grad(const double *r, const double *rho, double *l
   HIPCHK(hipMemcpyAsync(u.tmpf, hr.data(), n * sizeof(float), hipMemcpyHostToDevice, c.st));
   HIPCHK(hipMemcpyAsync(u.rho1, hrho.data(), n * sizeof(float), hipMemcpyHostToDevice, c.st));
   loss_and_dz(c, u, u.tmpf, u.rho1);
+  backward(c, u);
+  double dots[3];
+  HIPCHK(hipMemcpyAsync(dots, u.red + kResDots, 3 * sizeof(double), hipMemcpyDeviceToHost, c.st));
+  if (grad) HIPCHK(hipMemcpyAsync(grad, u.g, (size_t)u.nparam * sizeof(float), hipMemcpyDeviceToHost, c.st));
+  if (z_out) HIPCHK(hipMemcpyAsync(z_out, u.z, n * sizeof(float), hipMemcpyDeviceToHost, c.st));
+  HIPCHK(hipStreamSynchronize(c.st));
+  HIPCHK(hipGetLastError());
+  if (loss) *loss = 1.0 - dots[0] * dots[0] / (dots[2] * dots[1]);
+  return 0;
+}
+
+extern "C" int suli_unet_dataset_get(int idx, float *r, float *rho, float *faces) {
+  NEEDU();
+  Ctx &c = *g_ctx;
+  UNetPrecond &u = *fcg_unet(c);
+  int nset = 0;
+  if (dataset_count(c, u, &nset)) return 1;
+  if (idx < 0 || idx >= nset) {
+    suli_set_error("suli_unet_dataset_get: sample " + std::to_string(idx) + " of " + std::to_string(nset));
+    return 1;
+  }
+  const int64_t n = u.dsd.n(), nf = u.ds_entry > 0 ? nfaces(u.dsd) : 0;
+  if (r) HIPCHK(hipMemcpyAsync(r, u.ds_r + idx * n, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c.st));
+  if (rho) HIPCHK(hipMemcpyAsync(rho, u.ds_rho + idx * n, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c.st));
+  if (faces && nf > 0)
+    HIPCHK(hipMemcpyAsync(faces, u.ds_f + idx * nf, (size_t)nf * sizeof(float), hipMemcpyDeviceToHost, c.st));
+  HIPCHK(hipStreamSynchronize(c.st));
+  return 0;
+}
+
+extern "C" int suli_unet_dataset_dims(int *nx, int *ny, int *nz, int *entry) {
+  NEEDU();
+  UNetPrecond &u = *fcg_unet(*g_ctx);
+  *nx = u.dsd.nx;
+  *ny = u.dsd.ny;
+  *nz = u.dsd.nz;
+  *entry = u.ds_entry;
+  return 0;
+}
+
+extern "C" int suli_unet_sample_loss_grad(int idx, double *loss, float *grad, float *z_out) {
+  NEEDU();
+  Ctx &c = *g_ctx;
+  UNetPrecond &u = *fcg_unet(c);
+  if (!u.ready) {
+    suli_set_error("suli_unet_sample_loss_grad: no weights loaded (suli_unet_init or suli_unet_set_weights)");
+    return 1;
+  }
+  int nset = 0;
+  if (dataset_count(c, u, &nset)) return 1;
+  if (idx < 0 || idx >= nset) {
+    suli_set_error("suli_unet_sample_loss_grad: sample " + std::to_string(idx) + " of " + std::to_string(nset));
+    return 1;
+  }
+  if (alloc_acts(c, u, u.dsd)) return 1;
+  const int64_t n = u.dsd.n(), nf = u.ds_entry > 0 ? nfaces(u.dsd) : 0;
+  loss_and_dz(c, u, u.ds_r + idx * n, u.ds_rho + idx * n, nf > 0 ? u.ds_f + idx * nf : nullptr);
   backward(c, u);
   double dots[3];
   HIPCHK(hipMemcpyAsync(dots, u.red + kResDots, 3 * sizeof(double), hipMemcpyDeviceToHost, c.st));

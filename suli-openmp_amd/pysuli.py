@@ -105,13 +105,16 @@ def lib():
             "suli_prof_enable": ([i32], i32), "suli_prof_select": ([C.c_char_p, i32], i32),
             "suli_prof_count": ([], i32), "suli_prof_get": ([i32, C.c_char_p, i32, p(C.c_int64), p(f64)], i32),
             "suli_unet_nparams": ([], C.c_int64), "suli_unet_configure": ([i32], i32), "suli_unet_levels": ([], i32),
-            "suli_unet_set_entry": ([i32], i32), "suli_unet_entry": ([], i32),
+            "suli_unet_set_entry": ([i32], i32), "suli_unet_entry": ([], i32), "suli_unet_set_fcg_m": ([i32], i32),
             "suli_unet_init": ([C.c_uint64], i32),
             "suli_unet_set_weights": ([p(C.c_float), C.c_int64], i32),
             "suli_unet_get_weights": ([p(C.c_float), C.c_int64], i32),
             "suli_unet_capture": ([i32, i32], i32), "suli_unet_clear": ([], i32), "suli_unet_dataset_size": ([], i32),
             "suli_unet_train": ([i32, f64, C.c_uint64, p(f64)], i32),
             "suli_unet_loss_grad": ([p(f64), p(f64), p(f64), p(C.c_float), p(C.c_float)], i32),
+            "suli_unet_dataset_dims": ([p(i32), p(i32), p(i32), p(i32)], i32),
+            "suli_unet_dataset_get": ([i32, p(C.c_float), p(C.c_float), p(C.c_float)], i32),
+            "suli_unet_sample_loss_grad": ([i32, p(f64), p(C.c_float), p(C.c_float)], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -282,6 +285,9 @@ class Solver:
     def unet_entry(self):
         return int(lib().suli_unet_entry())
 
+    def unet_set_fcg_m(self, m):
+        _check(lib().suli_unet_set_fcg_m(m), "suli_unet_set_fcg_m")
+
     def unet_init(self, seed):
         _check(lib().suli_unet_init(seed), "suli_unet_init")
 
@@ -326,6 +332,42 @@ class Solver:
                                          z.ctypes.data_as(P(C.c_float))), "suli_unet_loss_grad")
         return loss.value, g, z.reshape(self.nz, self.ny, self.nx)
 
+    def unet_dataset_dims(self):
+        """(nx, ny, nz, entry) of the held training set."""
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().suli_unet_dataset_dims(*[C.byref(x) for x in v]), "suli_unet_dataset_dims")
+        return tuple(x.value for x in v)
+
+    def unet_dataset_get(self, idx):
+        """Sample idx: (r, rho, faces) with r, rho as [k, j, i] arrays and
+        faces = (cx, cy, cz), each with n+1 faces along its own axis, or None
+        for samples of the whole grid."""
+        nx, ny, nz, entry = self.unet_dataset_dims()
+        n = nx * ny * nz
+        r = np.empty(n, dtype=np.float32)
+        rho = np.empty(n, dtype=np.float32)
+        nfx, nfy, nfz = (nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)
+        f = np.empty(nfx + nfy + nfz if entry > 0 else 1, dtype=np.float32)
+        P = C.POINTER
+        _check(lib().suli_unet_dataset_get(idx, r.ctypes.data_as(P(C.c_float)), rho.ctypes.data_as(P(C.c_float)),
+                                           f.ctypes.data_as(P(C.c_float))), "suli_unet_dataset_get")
+        faces = None
+        if entry > 0:
+            faces = (f[:nfx].reshape(nz, ny, nx + 1), f[nfx:nfx + nfy].reshape(nz, ny + 1, nx),
+                     f[nfx + nfy:].reshape(nz + 1, ny, nx))
+        return r.reshape(nz, ny, nx), rho.reshape(nz, ny, nx), faces
+
+    def unet_sample_loss_grad(self, idx):
+        """(loss, gradient, z) of held sample idx through its training operator."""
+        nx, ny, nz, _ = self.unet_dataset_dims()
+        loss = C.c_double()
+        g = np.empty(self.unet_nparams(), dtype=np.float32)
+        z = np.empty(nx * ny * nz, dtype=np.float32)
+        P = C.POINTER
+        _check(lib().suli_unet_sample_loss_grad(idx, C.byref(loss), g.ctypes.data_as(P(C.c_float)),
+                                                z.ctypes.data_as(P(C.c_float))), "suli_unet_sample_loss_grad")
+        return loss.value, g, z.reshape(nz, ny, nx)
+
 
 def load_state(solver: Solver, state, names=None):
     """Upload an oracle State (reference layout) into the GPU solver."""
@@ -335,3 +377,33 @@ def load_state(solver: Solver, state, names=None):
         solver.upload(n, state.arr(n))
     if state.scal.get("vol_ini", 0) > 0:
         solver.set_vol_ini(state.scal["vol_ini"])
+
+
+WEIGHTS_DIR = os.path.join(HERE, "weights")
+
+
+def shipped_learned(weights_dir=WEIGHTS_DIR):
+    """The learned preconditioners shipped in weights/: one dict per
+    <name>.json that has its <name>.npy beside it, with the network depth
+    (levels), the multigrid level it solves (entry, 0 = the whole grid) and
+    whether it is the default one; the default comes first."""
+    import json
+    out = []
+    for f in sorted(os.listdir(weights_dir)) if os.path.isdir(weights_dir) else []:
+        if not f.endswith(".json"):
+            continue
+        npy = os.path.join(weights_dir, f[:-5] + ".npy")
+        if not os.path.exists(npy):
+            continue
+        meta = json.load(open(os.path.join(weights_dir, f)))
+        out.append({"name": f[:-5], "levels": int(meta["levels"]), "entry": int(meta.get("entry", 0)),
+                    "default": bool(meta.get("default", False)), "path": npy})
+    out.sort(key=lambda m: (not m["default"], m["entry"], m["name"]))
+    return out
+
+
+def use_learned(solver, item):
+    """Load a shipped learned preconditioner (an item of shipped_learned)."""
+    solver.unet_configure(item["levels"])
+    solver.unet_set_entry(item["entry"])
+    solver.unet_set_weights(np.load(item["path"]))

@@ -7,7 +7,7 @@ gx(t) = 0.05 g sin(2 pi t / 1.2 s + phase), the phase drawn from the seed
 with splitmix64; density ratio 1000 (rho_water 1000, rho_air from the
 ratio); no-slip walls; Smagorinsky Cs = 0.1; mixed precision; dt = 0.2 h /
 sqrt(g depth).  After the steps it solves the projection of the next stage
-with each preconditioner (the learned one from suli-openmp_amd/weights,
+with each preconditioner (the learned ones from suli-openmp_amd/weights,
 trained on dam breaks: unseen geometry and forcing) and reports ms per
 solve, iterations and fallbacks.
 
@@ -65,16 +65,14 @@ def run(nx, steps, ratio, seed=0, depth=0.4, precond="gmg"):
         s.tempo(1, 0.5 * dt)
         s.contorno(2)
         p0 = s.download("prd1")
-        cases = [("jacobi", S.PRECOND_JACOBI), ("gmg", S.PRECOND_GMG)]
-        wdir = os.path.join(HERE, "weights")
-        meta = os.path.join(wdir, "unet_dambreak_L6.json")
-        if os.path.exists(meta) and nx % 32 == 0:
-            levels = json.load(open(meta))["levels"]
-            s.unet_configure(levels)
-            s.unet_set_weights(np.load(os.path.join(wdir, "unet_dambreak_L6.npy")))
-            cases.append(("learned", S.PRECOND_LEARNED))
+        cases = [("jacobi", S.PRECOND_JACOBI, None), ("gmg", S.PRECOND_GMG, None)]
+        for m in S.shipped_learned():
+            if nx % (1 << (m["entry"] + m["levels"])) == 0:
+                cases.append((m["name"], S.PRECOND_LEARNED, m))
         out["poisson"] = {}
-        for name, c in cases:
+        for name, c, meta in cases:
+            if meta is not None:
+                S.use_learned(s, meta)
             s.upload("prd1", p0)
             s.set_solver(S.POISSON_FCG, c)
             st = s.graddin(0.5 * dt)

@@ -49,3 +49,21 @@ def test_bad_grid_rejected():
     p = pysuli.Params()
     p.nx, p.ny, p.nz = 2, 8, 8
     assert pysuli.lib().suli_init(C.byref(p)) != 0
+
+
+def test_shipped_learned_weights():
+    """Every shipped network has the parameter count of its architecture, one
+    of them is the default and comes first, and its grid constraint fits the
+    benchmark grid."""
+    import numpy as np
+    import unet_ref as U
+
+    shipped = pysuli.shipped_learned()
+    assert shipped and shipped[0]["default"] and sum(m["default"] for m in shipped) == 1
+    for m in shipped:
+        w = np.load(m["path"])
+        assert w.dtype == np.float32 and w.size == U.nparams(m["levels"]), m["name"]
+        assert np.all(np.isfinite(w))
+        assert 0 <= m["entry"] <= 4 and 2 <= m["levels"] <= 6
+    d = shipped[0]
+    assert 128 % (1 << (d["entry"] + d["levels"] - 1)) == 0

@@ -157,3 +157,92 @@ def test_gradient_matches_finite_differences_other_depths(L):
         eps = 1e-7
         fd = (U.loss(w + eps * d, *args) - U.loss(w - eps * d, *args)) / (2 * eps)
         assert np.isclose(fd, g @ d, rtol=1e-5, atol=1e-9), (fd, g @ d)
+
+
+# ---- the network as a multigrid level's coarse solver -------------------------
+
+def _dense_As_faces(cx, cy, cz):
+    """The matrix of apply_As_faces, column by column."""
+    nz, ny, nx = cx.shape[0], cx.shape[1], cx.shape[2] - 1
+    n = nx * ny * nz
+    M = np.empty((n, n))
+    for q in range(n):
+        e = np.zeros(n)
+        e[q] = 1.0
+        M[:, q] = U.apply_As_faces(e.reshape(nz, ny, nx), cx, cy, cz).ravel()
+    return M
+
+
+def test_multigrid_faces_closed_forms():
+    """Uniform density: every interior face of level l is 2^l times the
+    BF16-rounded fine conductance, wall faces are zero.  Density layered along
+    one axis: a coarse face normal to it equals twice the fine face at twice
+    its index (so the normal index is the fine face 2I, not 2I-1)."""
+    nz, ny, nx = 8, 8, 16
+    dt, h = 1e-3, 1 / 16
+    c0 = U.bf16(dt / (h * h) / 700.0)
+    rho = np.full((nz, ny, nx), 700.0)
+    for lev in range(3):
+        cx, cy, cz = U.mg_faces(rho, dt, h, h, h, lev)
+        for c, ax in ((cx, 2), (cy, 1), (cz, 0)):
+            c = np.moveaxis(c, ax, 0)
+            assert np.all(c[0] == 0) and np.all(c[-1] == 0)
+            np.testing.assert_allclose(c[1:-1], (2.0 ** lev) * c0, rtol=1e-6)
+    g = np.random.default_rng(0)
+    for ax in range(3):
+        prof = 1.2 + 998.8 * g.random(rho.shape[ax])
+        shp = [1, 1, 1]
+        shp[ax] = -1
+        lay = np.broadcast_to(prof.reshape(shp), rho.shape).copy()
+        f0 = U.mg_faces(lay, dt, h, h, h, 0)[2 - ax]
+        f1 = U.mg_faces(lay, dt, h, h, h, 1)[2 - ax]
+        f0n, f1n = np.moveaxis(f0, ax, 0), np.moveaxis(f1, ax, 0)
+        np.testing.assert_allclose(f1n, 2.0 * f0n[::2, ::2, ::2], rtol=1e-6)
+
+
+def test_multigrid_level_operator_invariants():
+    """On every level the scaled operator of the coarsened faces is symmetric,
+    positive semi-definite with the single null vector sqrt(D) (Neumann
+    walls), so the network's coarse problem is the multigrid one."""
+    g = np.random.default_rng(1)
+    nz, ny, nx = 4, 4, 8
+    rho = np.where(g.random((nz, ny, nx)) < 0.4, 1000.0, 1.2)
+    for lev in range(2):
+        cx, cy, cz = U.mg_faces(rho, 2e-4, 1 / 8, 1 / 8, 1 / 8, lev)
+        M = _dense_As_faces(cx, cy, cz)
+        np.testing.assert_allclose(M, M.T, atol=1e-12)
+        ev = np.linalg.eigvalsh(0.5 * (M + M.T))
+        assert ev[0] > -1e-12 and ev[1] > 1e-8
+        D = cx[:, :, 1:] + cx[:, :, :-1] + cy[:, 1:] + cy[:, :-1] + cz[1:] + cz[:-1]
+        assert np.abs(M @ np.sqrt(D).ravel()).max() < 1e-12 * np.sqrt(D).max()
+        nz, ny, nx = nz // 2, ny // 2, nx // 2
+
+
+def test_block_mean_of_linear_density():
+    k, j, i = np.meshgrid(np.arange(8), np.arange(4), np.arange(8), indexing="ij")
+    rho = 3.0 + 2.0 * i + 0.5 * j - 1.5 * k
+    for lev in (1, 2):
+        m = 1 << lev
+        K, J, I = np.meshgrid(np.arange(8 // m), np.arange(4 // m), np.arange(8 // m), indexing="ij")
+        c = lambda a: a * m + 0.5 * (m - 1)  # noqa: E731  (block centre)
+        np.testing.assert_allclose(U.block_mean(rho, lev), 3.0 + 2.0 * c(I) + 0.5 * c(J) - 1.5 * c(K))
+
+
+def test_face_loss_gradient_matches_finite_differences():
+    """The loss through a multigrid level's operator (faces given) has the
+    gradient of its finite differences."""
+    g = np.random.default_rng(2)
+    nz, ny, nx = 4, 4, 8
+    rho = np.where(g.random((2 * nz, 2 * ny, 2 * nx)) < 0.5, 1000.0, 1.2)
+    faces = U.mg_faces(rho, 2e-4, 1 / 16, 1 / 16, 1 / 16, 1)
+    rhoe = U.block_mean(rho, 1)
+    r = g.standard_normal((nz, ny, nx))
+    w = 0.2 * g.standard_normal(U.nparams(2))
+    L0, grad, _ = U.loss_and_grad(w, r, rhoe, 1.2, 1000.0, 0, 0, 0, faces=faces)
+    for q in g.choice(w.size, 12, replace=False):
+        e = np.zeros_like(w)
+        e[q] = 1e-6
+        lp = U.loss_and_grad(w + e, r, rhoe, 1.2, 1000.0, 0, 0, 0, faces=faces)[0]
+        lm = U.loss_and_grad(w - e, r, rhoe, 1.2, 1000.0, 0, 0, 0, faces=faces)[0]
+        fd = (lp - lm) / 2e-6
+        assert abs(fd - grad[q]) <= 1e-5 * max(1.0, abs(fd)), (q, fd, grad[q])
