@@ -110,6 +110,7 @@ struct UNetPrecond {
   int entry = 0;
   float *rho_e = nullptr;
   int64_t rho_e_cap = 0;
+  double *rho_h[2] = {nullptr, nullptr};  // intermediate halvings (levels 1..e-1)
   int fcg_m = 4;            // directions FCG keeps for this preconditioner (1 or 4)
   int ds_entry = 0;         // entry level the held samples were captured at
   float *ds_f = nullptr;    // per sample: x, y, z faces of the level (compact)
@@ -202,19 +203,29 @@ __global__ void k_capture_bump(int *cnt, int max, const int *__restrict__ active
 // output z~ maps back to y = z~/sqrt(dg); the correction is x = alpha y with
 // the energy-optimal alpha = (y.b)/(y.A y).
 
-// compact block-mean density of level e: mean of rho over the 2^e-cube of
-// fine cells under each level-e cell
-template <typename T>
-__global__ void k_rho_block(Grid g, Dims d, int e, const T *__restrict__ rho, float *__restrict__ rho_e) {
+// Compact block-mean density of level e, the mean of rho over the 2^e-cube
+// of fine cells under each level-e cell, built as e halvings (each the mean
+// of 8 children; the first reads the frame): every pass is one parallel
+// sweep over its level.
+template <typename T, typename TO>
+__global__ void k_rho_halve_frame(Grid g, Dims d, const T *__restrict__ rho, TO *__restrict__ out) {
   const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (v >= d.n()) return;
   const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
-  const int m = 1 << e;
   double s = 0.0;
-  for (int c = 0; c < m; c++)
-    for (int b = 0; b < m; b++)
-      for (int a = 0; a < m; a++) s += (double)rho[g.idx(1 + i * m + a, 1 + j * m + b, 1 + k * m + c)];
-  rho_e[v] = (float)(s / (double)(m * m * m));
+  for (int c = 0; c < 8; c++) s += (double)rho[g.idx(1 + 2 * i + (c & 1), 1 + 2 * j + ((c >> 1) & 1), 1 + 2 * k + (c >> 2))];
+  out[v] = (TO)(0.125 * s);
+}
+template <typename TO>
+__global__ void k_rho_halve(Dims d, const double *__restrict__ in, TO *__restrict__ out) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= d.n()) return;
+  const int i = (int)(v % d.nx), j = (int)((v / d.nx) % d.ny), k = (int)(v / ((int64_t)d.nx * d.ny));
+  const int64_t fx = 2 * d.nx, fy = 2 * d.ny;
+  double s = 0.0;
+  for (int c = 0; c < 8; c++)
+    s += in[(2 * i + (c & 1)) + fx * ((2 * j + ((c >> 1) & 1)) + fy * (int64_t)(2 * k + (c >> 2)))];
+  out[v] = (TO)(0.125 * s);
 }
 
 // sum over the level's cells of b^2/dg -> partials (512-block grid-stride)
@@ -420,13 +431,19 @@ constexpr int kResNorm = kRedPart, kResDots = kRedPart + 8, kResCs = kRedPart + 
 
 inline Dims level_dims(Dims d0, int l) { return Dims{d0.nx >> l, d0.ny >> l, d0.nz >> l}; }
 
+// Grids of fewer voxels than this have too few 64x4x2 tiles to fill the
+// chip: their convolutions take one block per 16-voxel row group (taps split
+// over the block's wavefronts) and the FP32 first and last layers one
+// 64-thread block per 64 voxels.
+constexpr int64_t kSmallGrid = (int64_t)1 << 18;
+
 template <int CIN, int COUT, int FLAGS>
 void conv_t(hipStream_t st, Dims d, const uint16_t *in, const uint16_t *w, const float *bias, const uint16_t *gate,
             uint16_t *out, const int *active) {
-  if constexpr (CIN == 16 && COUT == 16) {
+  if (CIN == 16 && COUT == 16 && d.n() >= kSmallGrid) {
     const dim3 grid((d.nx + C16_TX - 1) / C16_TX, (d.ny + C16_TY - 1) / C16_TY, (d.nz + C16_TZ - 1) / C16_TZ);
     hipLaunchKernelGGL((k_conv16<FLAGS>), grid, dim3(256), 0, st, d, in, w, bias, gate, out, active);
-  } else if (d.n() < (int64_t)1 << 18) {  // coarse grids: split the taps over the block's wavefronts
+  } else if (d.n() < kSmallGrid) {  // coarse grids: split the taps over the block's wavefronts
     hipLaunchKernelGGL((k_conv_mfma_split<CIN, COUT, FLAGS>), dim3((d.nx + 15) / 16, d.ny, d.nz), dim3(256), 0, st, d,
                        in, w, bias, gate, out, active);
   } else {
@@ -540,8 +557,13 @@ void forward(Ctx &c, UNetPrecond &u, const int *active) {
   const Dims d0 = u.d0;
   const int q0 = a.ea(0);
   const dim3 tiles((d0.nx + C16_TX - 1) / C16_TX, (d0.ny + C16_TY - 1) / C16_TY, (d0.nz + C16_TZ - 1) / C16_TZ);
-  hipLaunchKernelGGL(k_conv_in_tile, tiles, dim3(256), 0, st, d0, u.x0, W + u.woff[q0], W + u.boff[q0], u.A[0],
-                     active);
+  const bool small = d0.n() < kSmallGrid;
+  if (small)
+    hipLaunchKernelGGL(k_conv_in, g1(d0.n(), 64), dim3(64), 0, st, d0, u.x0, W + u.woff[q0], W + u.boff[q0], u.A[0],
+                       active);
+  else
+    hipLaunchKernelGGL(k_conv_in_tile, tiles, dim3(256), 0, st, d0, u.x0, W + u.woff[q0], W + u.boff[q0], u.A[0],
+                       active);
   auto fc = [&](int q, Dims d, const uint16_t *in, uint16_t *out) {
     conv(FWD, a.lay[q].ci, a.lay[q].co, st, d, in, u.wf[q], W + u.boff[q], nullptr, out, active);
   };
@@ -566,8 +588,12 @@ void forward(Ctx &c, UNetPrecond &u, const int *active) {
     prev = u.D[l];
   }
   const int qo = a.out();
-  hipLaunchKernelGGL(k_conv_out_tile, tiles, dim3(256), 0, st, d0, u.D[0], W + u.woff[qo], W + u.boff[qo], u.z,
-                     active);
+  if (small)
+    hipLaunchKernelGGL(k_conv_out, g1(d0.n(), 64), dim3(64), 0, st, d0, u.D[0], W + u.woff[qo], W + u.boff[qo], u.z,
+                       active);
+  else
+    hipLaunchKernelGGL(k_conv_out_tile, tiles, dim3(256), 0, st, d0, u.D[0], W + u.woff[qo], W + u.boff[qo], u.z,
+                       active);
 }
 
 // backward pass from u.dz; gradients into u.g (overwritten)
@@ -752,17 +778,43 @@ bool unet_capture_armed(const UNetPrecond *u) { return u && u->ds_max > 0; }
 
 int unet_level_density(Ctx &c, UNetPrecond *u) {
   if (!u || u->entry <= 0) return 0;
-  const Dims d = entry_dims(c.g, u->entry);
+  const int e = u->entry;
+  const Dims d = entry_dims(c.g, e);
   if (d.n() > u->rho_e_cap) {
+    // (level-1 sized intermediates serve every entry level of this grid)
+    const int64_t n1 = entry_dims(c.g, 1).n();
     if (alloc_f(c, &u->rho_e, d.n())) return 1;
+    for (double *&h : u->rho_h) {
+      HIPCHK(hipMalloc((void **)&h, (size_t)n1 * sizeof(double)));
+      c.allocs.push_back(h);
+    }
     u->rho_e_cap = d.n();
   }
-  if (c.prec == SULI_PREC_FP64)
-    hipLaunchKernelGGL((k_rho_block<double>), g1(d.n()), dim3(256), 0, c.st, c.g, d, u->entry,
-                       (const double *)c.rho, u->rho_e);
-  else
-    hipLaunchKernelGGL((k_rho_block<float>), g1(d.n()), dim3(256), 0, c.st, c.g, d, u->entry,
-                       (const float *)c.rho, u->rho_e);
+  // level 1 from the frame, then halvings; the last pass writes rho_e (FP32)
+  const Dims d1 = entry_dims(c.g, 1);
+  if (e == 1) {
+    if (c.prec == SULI_PREC_FP64)
+      hipLaunchKernelGGL((k_rho_halve_frame<double, float>), g1(d1.n()), dim3(256), 0, c.st, c.g, d1,
+                         (const double *)c.rho, u->rho_e);
+    else
+      hipLaunchKernelGGL((k_rho_halve_frame<float, float>), g1(d1.n()), dim3(256), 0, c.st, c.g, d1,
+                         (const float *)c.rho, u->rho_e);
+  } else {
+    if (c.prec == SULI_PREC_FP64)
+      hipLaunchKernelGGL((k_rho_halve_frame<double, double>), g1(d1.n()), dim3(256), 0, c.st, c.g, d1,
+                         (const double *)c.rho, u->rho_h[0]);
+    else
+      hipLaunchKernelGGL((k_rho_halve_frame<float, double>), g1(d1.n()), dim3(256), 0, c.st, c.g, d1,
+                         (const float *)c.rho, u->rho_h[0]);
+    for (int l = 2; l <= e; l++) {
+      const Dims dl = entry_dims(c.g, l);
+      const double *in = u->rho_h[(l - 2) % 2];  // level l-1
+      if (l == e)
+        hipLaunchKernelGGL((k_rho_halve<float>), g1(dl.n()), dim3(256), 0, c.st, dl, in, u->rho_e);
+      else
+        hipLaunchKernelGGL((k_rho_halve<double>), g1(dl.n()), dim3(256), 0, c.st, dl, in, u->rho_h[(l - 1) % 2]);
+    }
+  }
   HIPCHK(hipGetLastError());
   return 0;
 }

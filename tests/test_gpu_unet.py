@@ -36,7 +36,9 @@ def solver(nx, ny, nz, **kw):
     return S.Solver(nx, ny, nz, 1.0 / nx, **kw)
 
 
-@pytest.mark.parametrize("L,dims", [(3, (32, 16, 16)), (3, (20, 12, 8)), (5, (32, 16, 16)), (2, (12, 8, 8))])
+# grids below 2^18 voxels take the row-group kernels, 64^3 the LDS-tiled ones
+@pytest.mark.parametrize("L,dims", [(3, (32, 16, 16)), (3, (20, 12, 8)), (5, (32, 16, 16)), (2, (12, 8, 8)),
+                                    (2, (64, 64, 64))])
 def test_network_and_gradient_match_oracle(L, dims):
     nx, ny, nz = dims
     print(f"unet parity L={L} dims={dims}", flush=True)
