@@ -260,6 +260,26 @@ int suli_unet_dataset_get(int idx, float *r, float *rho, float *faces);
  * operator that training uses for it. */
 int suli_unet_sample_loss_grad(int idx, double *loss, float *grad, float *z);
 
+/* Field output (plot_f, 4_plot.f90:260-363): the one FP32 record of an
+ * arquivos/campos_NNNNN file, built on the device from the resident fields:
+ * x11, y11, z11, uaux, vaux, waux ((nx+1)(ny+1)(nz+1) node values each:
+ * coordinates and node-averaged velocities), ls and kaux (nx*ny*nz; kaux = 0,
+ * no obstacles) and, for t_plot = 1, prdaux, vorti, vortj, vortk (nx*ny*nz),
+ * xnuta ((nx+1)*ny*nz), ynuta (nx*(ny+1)*nz), znuta (nx*ny*(nz+1)), each in
+ * Fortran order.  suli_frame_values gives the number of values,
+ * suli_frame_record copies them to host memory, suli_write_frame writes
+ * the file as the reference does (Fortran unformatted sequential: a 4-byte
+ * little-endian byte count before and after the values; records beyond
+ * 2^31-1 bytes are refused).  mms_t = 0 output only. */
+int suli_frame_values(int t_plot, int64_t *n);
+int suli_frame_record(int t_plot, float *buf, int64_t n);
+int suli_write_frame(const char *path, int t_plot);
+/* The ParaView index of visu (12_paraview_3D.f90:1-179) for the records
+ * campos_<first>..campos_<last> of an nx*ny*nz grid: the same grids,
+ * attributes, dimensions and byte offsets, as well-formed XML.  Needs no
+ * library context. */
+int suli_write_xdmf(const char *path, int nx, int ny, int nz, int first, int last, int t_plot, double dt_frame);
+
 /* Per-kernel GPU timing: HIP events recorded on the library's stream around
  * every kernel launch while enabled.  suli_prof_enable(1) clears the totals;
  * suli_prof_get returns kernel idx's name, launch count and summed time. */

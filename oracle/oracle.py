@@ -284,7 +284,8 @@ def ref_build(p: Params, quiet: bool = True) -> str:
     """Build (if needed) the reference harness for this configuration."""
     tag = ref_tag(p)
     exe = os.path.join(REF_DIR, tag, "harness")
-    if os.path.exists(exe):
+    srcs = [os.path.join(HERE, "ref", f) for f in ("harness.f90", "build_ref.sh")]
+    if os.path.exists(exe) and os.path.getmtime(exe) >= max(os.path.getmtime(f) for f in srcs):
         return exe
     args = [os.path.join(HERE, "ref", "build_ref.sh"), tag, str(p.nx), str(p.ny), str(p.nz),
             repr(float(p.dx)), repr(float(p.dy)), repr(float(p.dz)),
@@ -353,3 +354,72 @@ def random_state(p: Params, seed: int = 0, vel: float = 0.05, width: float = 0.4
     s.scal["dt"] = 1e-3
     s.run("ls_ini")
     return s
+
+
+# -- field output ------------------------------------------------------------------
+def plot_record(s: State, t_plot: int = 1) -> np.ndarray:
+    """The FP32 record that plot_f writes to arquivos/campos_NNNNN
+    (4_plot.f90:260-363; 4_plot.f90:186-187 for t_plot = 0 and :211-213 for
+    t_plot = 1, mms_t = 0, no obstacles), as a flat array: the FP64 values of
+    the reference in its evaluation order, rounded once to FP32."""
+    p = s.p
+    nx, ny, nz = p.nx, p.ny, p.nz
+
+    def axis(n, h):  # x1(0) = -dx, x1(i) = x1(i-1) + dx (4_plot.f90:264-266,275-277)
+        t = np.empty(n + 2)
+        x = -h
+        t[0] = x
+        for i in range(1, n + 2):
+            x = x + h
+            t[i] = x
+        return t
+
+    shp = (nx + 1, ny + 1, nz + 1)
+    X = np.broadcast_to(axis(nx, p.dx)[1:, None, None], shp)
+    Y = np.broadcast_to(axis(ny, p.dy)[None, 1:, None], shp)
+    Z = np.broadcast_to(axis(nz, p.dz)[None, None, 1:], shp)
+    u, v, w = s.u, s.v, s.w  # reference indices from 0
+    a, b = slice(1, None), slice(0, -1)  # node index i and i-1 over i = 1..n+1
+    U = u[1:nx + 2, 0:ny + 2, 0:nz + 2]
+    V = v[0:nx + 2, 1:ny + 2, 0:nz + 2]
+    W = w[0:nx + 2, 0:ny + 2, 1:nz + 2]
+    # (4_plot.f90:269-272)
+    uaux = (((U[:, a, a] + U[:, b, a]) + U[:, a, b]) + U[:, b, b]) * 0.25
+    vaux = (((V[a, :, a] + V[b, :, a]) + V[a, :, b]) + V[b, :, b]) * 0.25
+    waux = (((W[a, a, :] + W[a, b, :]) + W[b, a, :]) + W[b, b, :]) * 0.25
+    parts = [X, Y, Z, uaux, vaux, waux, s.ls, np.zeros((nx, ny, nz))]
+    if t_plot == 1:
+        c = (slice(1, nx + 1), slice(1, ny + 1), slice(1, nz + 1))
+        sh = lambda arr, di, dj, dk: arr[1 + di:nx + 1 + di, 1 + dj:ny + 1 + dj, 1 + dk:nz + 1 + dk]  # noqa: E731
+        # (4_plot.f90:190-201)
+        dudy = (sh(u, 0, 1, 0) - sh(u, 0, 0, 0)) / p.dy
+        dudz = (sh(u, 0, 0, 1) - sh(u, 0, 0, 0)) / p.dz
+        dvdx = (sh(v, 1, 0, 0) - sh(v, 0, 0, 0)) / p.dx
+        dvdz = (sh(v, 0, 0, 1) - sh(v, 0, 0, 0)) / p.dz
+        dwdx = (sh(w, 1, 0, 0) - sh(w, 0, 0, 0)) / p.dx
+        dwdy = (sh(w, 0, 1, 0) - sh(w, 0, 0, 0)) / p.dy
+
+        def interp_cf(arr, axis):  # 8_derivadas.f90:306-323 and its y, z versions
+            m = np.moveaxis(arr, axis, 0)
+            mid = (m[1:] + m[:-1]) * 0.5
+            return np.moveaxis(np.concatenate([mid[:1], mid, mid[-1:]], axis=0), 0, axis)
+
+        nut = s.nut
+        parts += [s.prd1[c], dwdy - dvdz, dudz - dwdx, dvdx - dudy,
+                  interp_cf(nut, 0), interp_cf(nut, 1), interp_cf(nut, 2)]
+    return np.concatenate([np.asarray(x, dtype=np.float64).ravel(order="F").astype(np.float32) for x in parts])
+
+
+def xdmf_layout(text: str) -> list:
+    """(Seek, Dimensions) of every binary DataItem of an XDMF index, in order,
+    from the reference's visu output (list-directed lines that its compiler
+    may break inside a tag) or from a well-formed file."""
+    import re
+    flat = re.sub(r"\s+", " ", text.replace("\n ", ""))
+    out = []
+    for m in re.finditer(r"<DataItem[^>]*Format=\"Binary\"[^>]*>", flat):
+        tag = m.group(0)
+        seek = int(re.search(r"Seek=\"\s*(\d+)\s*\"", tag).group(1))
+        dims = tuple(int(t) for t in re.search(r"Dimensions=\"([^\"]*)\"", tag).group(1).split())
+        out.append((seek, dims))
+    return out

@@ -9,7 +9,8 @@
 !
 ! CMD: ls_ini | level_set | heaviside | mod_ls1 | reinic | der_weno | weno1 ARG(ihs)
 !      visco | convdiff | tempo ARG(tt) | contorno ARG(nlock) | graddin | posdin
-!      stage ARG(tt) | step ARG(nsteps) | bench
+!      stage ARG(tt) | step ARG(nsteps) | bench | plot | visu
+! (plot and visu write under arquivos/ of the working directory)
 !
 ! State file: int32 nx,ny,nz; 10 float64 scalars (dt, t, vol_ini, vol_ins,
 ! rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1); then the arrays listed in
@@ -21,6 +22,7 @@ program harness
   use smag
   use ls_param
   use obst
+  use tempo_mod
   implicit none
   character(len=512) :: fin, fout, cmd, arg
   integer :: iarg, nsteps, k, i, j
@@ -166,6 +168,19 @@ program harness
     write(*,'(A,ES16.8,A,ES16.8)') 'REF_BENCH stage1_seconds=', real(c1-ls1,8)/real(crate,8), &
       ' graddin_seconds=', real(g1-g0,8)/real(crate,8)
     flush(6)
+  case ('plot')
+    ! one frame of plot_f (4_plot.f90:260-363): the file counter starts at 10
+    ! as plot_i leaves it (4_plot.f90:131) and `it` is a frame step, so the
+    ! record goes to arquivos/campos_00011 of the working directory
+    cont = 10
+    it = ceiling(dt_frame/dt)
+    agora = 0
+    agora1 = 0
+    prev = 0.
+    call plot_f()
+  case ('visu')
+    ! the ParaView index arquivos/visu.xdmf (12_paraview_3D.f90:1-179)
+    call visu()
   case default
     write(*,*) 'unknown command ', trim(cmd)
     stop 2

@@ -52,6 +52,8 @@ static void free_all(Ctx &c) {
   c.allocs.clear();
   if (c.ctl_h) (void)hipHostFree(c.ctl_h);
   if (c.res_h) (void)hipHostFree(c.res_h);
+  if (c.out_host) (void)hipHostFree(c.out_host);
+  if (c.out_dev) (void)hipFree(c.out_dev);
   if (c.st) (void)hipStreamDestroy(c.st);
 }
 

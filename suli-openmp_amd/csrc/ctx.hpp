@@ -74,6 +74,9 @@ struct Ctx {
   bool convdiff_pending = false;
   double vol_ini = 0, vol_ins = 0;
   PoissonFCG *fcg = nullptr;
+  // field output: the record on the device, its pinned host copy, coordinate tables
+  float *out_dev = nullptr, *out_host = nullptr, *out_tab = nullptr;
+  int64_t out_cap = 0, out_host_cap = 0;
   std::vector<void *> allocs;
   Prof prof;
 

@@ -55,6 +55,10 @@ struct Launch {
   // coefficients with their FP32 copies and the BF16 multigrid conductances
   static void fcg_faces(const Grid &g, hipStream_t s, double dt, const void *rho, const double *D,
                         const FcgFaceOut &o);
+  // the record of a field-output file (k_frame_record)
+  static void frame_record(const Grid &g, hipStream_t s, int t_plot, const void *u, const void *v, const void *w,
+                           const void *ls, const double *p, const void *nut, const float *xs, const float *ys,
+                           const float *zs, float *out);
 };
 
 // grid helpers

@@ -640,3 +640,73 @@ __global__ __launch_bounds__(256) void k_divmax(Grid g, const T *__restrict__ u,
   block_reduce<1, true>(acc, lds);
   block_partial<1>(acc, rs.partials);
 }
+
+// Field output of plot_f (4_plot.f90:266-213 of the t_plot branches): the one
+// record of an arquivos/campos_NNNNN file, FP32, in the reference's order
+// and extents (Fortran order within each array):
+//   x11, y11, z11, uaux, vaux, waux        (nx+1, ny+1, nz+1) nodes
+//   ls, kaux                               (nx, ny, nz)
+//   t_plot = 1 adds prdaux, vorti, vortj, vortk (nx, ny, nz),
+//                   xnuta (nx+1, ny, nz), ynuta (nx, ny+1, nz), znuta (nx, ny, nz+1).
+// Node velocities average the four faces around the node's edge
+// (4_plot.f90:269-272), vorticity takes forward differences (:190-201), the
+// face viscosities interp*_cf(nut) with the end faces copied (:206-208,
+// 8_derivadas.f90:306-323); kaux is 0 (no obstacles); the coordinates come
+// from per-axis tables (the reference's running sums, rounded to FP32).
+// One thread per node; every value is computed in T and rounded once.
+template <typename T>
+__global__ __launch_bounds__(256) void k_frame_record(Grid g, int t_plot, const T *__restrict__ u,
+                                                      const T *__restrict__ v, const T *__restrict__ w,
+                                                      const T *__restrict__ ls, const double *__restrict__ p,
+                                                      const T *__restrict__ nut, const float *__restrict__ xs,
+                                                      const float *__restrict__ ys, const float *__restrict__ zs,
+                                                      float *__restrict__ out) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i > nx + 1 || j > ny + 1 || k > nz + 1) return;
+  const int64_t n1 = (int64_t)(nx + 1) * (ny + 1) * (nz + 1), n = (int64_t)nx * ny * nz;
+  const int64_t f1 = (i - 1) + (int64_t)(nx + 1) * ((j - 1) + (int64_t)(ny + 1) * (k - 1));
+  const fidx c = g.idx(i, j, k);
+  const fidx X = 1, Y = g.sx, Z = g.sz;
+  const T q = (T)0.25;
+  out[f1] = xs[i];
+  out[n1 + f1] = ys[j];
+  out[2 * n1 + f1] = zs[k];
+  out[3 * n1 + f1] = (float)((((u[c] + u[c - Y]) + u[c - Z]) + u[c - Y - Z]) * q);
+  out[4 * n1 + f1] = (float)((((v[c] + v[c - X]) + v[c - Z]) + v[c - X - Z]) * q);
+  out[5 * n1 + f1] = (float)((((w[c] + w[c - Y]) + w[c - X]) + w[c - X - Y]) * q);
+  float *o = out + 6 * n1;
+  if (i <= nx && j <= ny && k <= nz) {
+    const int64_t f = (i - 1) + (int64_t)nx * ((j - 1) + (int64_t)ny * (k - 1));
+    o[f] = (float)ls[c];
+    o[n + f] = 0.f;
+    if (t_plot == 1) {
+      o[2 * n + f] = (float)p[c];
+      const T dudy = (u[c + Y] - u[c]) / (T)g.dy, dudz = (u[c + Z] - u[c]) / (T)g.dz;
+      const T dvdx = (v[c + X] - v[c]) / (T)g.dx, dvdz = (v[c + Z] - v[c]) / (T)g.dz;
+      const T dwdx = (w[c + X] - w[c]) / (T)g.dx, dwdy = (w[c + Y] - w[c]) / (T)g.dy;
+      o[3 * n + f] = (float)(dwdy - dvdz);
+      o[4 * n + f] = (float)(dudz - dwdx);
+      o[5 * n + f] = (float)(dvdx - dudy);
+    }
+  }
+  if (t_plot != 1) return;
+  const T h = (T)0.5;
+  float *ox = o + 6 * n, *oy = ox + (int64_t)(nx + 1) * ny * nz, *oz = oy + (int64_t)nx * (ny + 1) * nz;
+  if (j <= ny && k <= nz) {  // xnuta(i), i = 1..nx+1: the face between cells ia-1 and ia
+    const int ia = i < 2 ? 2 : (i > nx ? nx : i);
+    const fidx ca = g.idx(ia, j, k);
+    ox[(i - 1) + (int64_t)(nx + 1) * ((j - 1) + (int64_t)ny * (k - 1))] = (float)((nut[ca] + nut[ca - X]) * h);
+  }
+  if (i <= nx && k <= nz) {
+    const int ja = j < 2 ? 2 : (j > ny ? ny : j);
+    const fidx ca = g.idx(i, ja, k);
+    oy[(i - 1) + (int64_t)nx * ((j - 1) + (int64_t)(ny + 1) * (k - 1))] = (float)((nut[ca] + nut[ca - Y]) * h);
+  }
+  if (i <= nx && j <= ny) {
+    const int ka = k < 2 ? 2 : (k > nz ? nz : k);
+    const fidx ca = g.idx(i, j, ka);
+    oz[(i - 1) + (int64_t)nx * ((j - 1) + (int64_t)ny * (k - 1))] = (float)((nut[ca] + nut[ca - Z]) * h);
+  }
+}

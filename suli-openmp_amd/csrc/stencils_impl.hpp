@@ -174,3 +174,12 @@ void Launch<T>::pcoef(const Grid &g, hipStream_t s, double dt, const void *rho, 
   dim3 grid((g.nx + 1 + SULI_BX - 1) / SULI_BX, (g.ny + 1 + SULI_BY - 1) / SULI_BY, g.nz + 1);
   SULI_LAUNCH(k_pcoef<T>, grid, blk2(), s, g, dt, (const T *)rho, D, ax, ay, az);
 }
+
+template <typename T>
+void Launch<T>::frame_record(const Grid &g, hipStream_t s, int t_plot, const void *u, const void *v, const void *w,
+                             const void *ls, const double *p, const void *nut, const float *xs, const float *ys,
+                             const float *zs, float *out) {
+  const dim3 grid((g.nx + 1 + SULI_BX - 1) / SULI_BX, (g.ny + 1 + SULI_BY - 1) / SULI_BY, g.nz + 1);
+  SULI_LAUNCH(k_frame_record<T>, grid, blk2(), s, g, t_plot, (const T *)u, (const T *)v, (const T *)w,
+              (const T *)ls, p, (const T *)nut, xs, ys, zs, out);
+}

@@ -7,10 +7,12 @@
 ! Tank 1.0 x 0.5 x 0.5 m scaled to the grid (h = 1/NX), dam break of a water
 ! column WIDTH x HEIGHT in the x = 0 corner, water/air 1000/1.2 kg/m^3, no-slip
 ! walls, Smagorinsky Cs = 0.1, mixed precision, FCG projection to 1e-8 with
-! PRECOND 0 Jacobi, 1 multigrid (default), 2 learned.  Every FRAME steps the
-! velocities, pressure and level set are copied back into the host arrays
-! u, v, w, prd1, ls with the reference extents (0_modules.f90:92-122), the
-! frame cadence of plot_f (4_plot.f90:260).
+! PRECOND 0 Jacobi, 1 multigrid (default), 2 learned.  With FRAME > 0 the
+! field output of the reference is written: arquivos/campos_00010 at the
+! start (plot_i), one record every FRAME steps (plot_f, 4_plot.f90:260) and
+! arquivos/visu.xdmf at the end (visu); every frame the velocities, pressure
+! and level set are also copied back into the host arrays u, v, w, prd1, ls
+! with the reference extents (0_modules.f90:92-122).
 program principal_gpu
   use suli_gpu
   implicit none
@@ -67,6 +69,7 @@ program principal_gpu
 
   call contorno(1)
   call contorno(3)
+  if (frame > 0) call plot_i()
 
   call system_clock(c0, crate)
   do it = 1, ts
@@ -94,6 +97,7 @@ program principal_gpu
       (ls_st%vol_ins - ls_st%vol_ini) / ls_st%vol_ini, ' maxdiv=', div, ' poisson_iters=', pois_st%iters
     if (frame > 0) then
       if (mod(it, frame) == 0) then
+        call plot_f()
         call download(SULI_U, u)
         call download(SULI_V, v)
         call download(SULI_W, w)
@@ -108,5 +112,6 @@ program principal_gpu
   write(*,'(A,F10.3,A,F10.2,A,F8.2)') 'seconds=', real(c1 - c0, 8) / real(crate, 8), &
     ' Mcell-steps/s=', real(nx, 8) * ny * nz * ts / (real(c1 - c0, 8) / real(crate, 8)) / 1.0d6, &
     ' poisson_iters/solve=', real(pois_iters, 8) / max(pois_solves, 1_8)
+  if (frame > 0) call visu(frame * dt)
   call check(c_suli_finalize(), 'finalize')
 end program principal_gpu

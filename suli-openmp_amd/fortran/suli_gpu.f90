@@ -110,6 +110,18 @@ module suli_gpu
       import :: c_int, c_double
       real(c_double), value :: amp, period, phase
     end function
+    integer(c_int) function c_suli_write_frame(path, t_plot) bind(C, name='suli_write_frame')
+      import :: c_int, c_char
+      character(kind=c_char), dimension(*) :: path
+      integer(c_int), value :: t_plot
+    end function
+    integer(c_int) function c_suli_write_xdmf(path, nx, ny, nz, first, last, t_plot, dt_frame) &
+        bind(C, name='suli_write_xdmf')
+      import :: c_int, c_char, c_double
+      character(kind=c_char), dimension(*) :: path
+      integer(c_int), value :: nx, ny, nz, first, last, t_plot
+      real(c_double), value :: dt_frame
+    end function
   end interface
 
   ! driver state mirroring module disc (0_modules.f90:9,39-41)
@@ -120,6 +132,9 @@ module suli_gpu
   type(suli_ls_stats) :: ls_st
   type(suli_poisson_stats) :: pois_st
   integer(8) :: pois_iters = 0, pois_solves = 0
+  ! field output (module tempo, 0_modules.f90:155; t_plot, 0_modules.f90:22)
+  integer(8) :: cont = 10
+  integer :: t_plot = 1
 
 contains
 
@@ -180,6 +195,32 @@ contains
     real(8), intent(out) :: div
     call check(c_suli_est_div(div), 'est')
   end subroutine est
+
+  ! field output of plot_i / plot_f (4_plot.f90:131,184-215,260-363): the
+  ! record arquivos/campos_NNNNN of the current fields, NNNNN = cont; plot_f
+  ! advances the counter first, as the reference does
+  subroutine plot_i()
+    call write_campos()
+  end subroutine plot_i
+
+  subroutine plot_f()
+    cont = cont + 1
+    call write_campos()
+  end subroutine plot_f
+
+  subroutine write_campos()
+    character(len=32) :: name
+    call execute_command_line('mkdir -p arquivos')
+    write(name, '(A,I5.5,A)') 'arquivos/campos_', cont, c_null_char
+    call check(c_suli_write_frame(name, int(t_plot, c_int)), 'plot')
+  end subroutine write_campos
+
+  ! visu (12_paraview_3D.f90:1-179): the ParaView index of campos_00010..cont
+  subroutine visu(dt_frame)
+    real(8), intent(in) :: dt_frame
+    call check(c_suli_write_xdmf('arquivos/visu.xdmf'//c_null_char, prm%nx, prm%ny, prm%nz, 10_c_int, &
+                                 int(cont, c_int), int(t_plot, c_int), dt_frame), 'visu')
+  end subroutine visu
 
   ! host copy of a field in the reference extents, for output
   subroutine download(field, a)

@@ -115,6 +115,10 @@ def lib():
             "suli_unet_dataset_dims": ([p(i32), p(i32), p(i32), p(i32)], i32),
             "suli_unet_dataset_get": ([i32, p(C.c_float), p(C.c_float), p(C.c_float)], i32),
             "suli_unet_sample_loss_grad": ([i32, p(f64), p(C.c_float), p(C.c_float)], i32),
+            "suli_frame_values": ([i32, p(C.c_int64)], i32),
+            "suli_frame_record": ([i32, p(C.c_float), C.c_int64], i32),
+            "suli_write_frame": ([C.c_char_p, i32], i32),
+            "suli_write_xdmf": ([C.c_char_p, i32, i32, i32, i32, i32, i32, f64], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -332,6 +336,18 @@ class Solver:
                                          z.ctypes.data_as(P(C.c_float))), "suli_unet_loss_grad")
         return loss.value, g, z.reshape(self.nz, self.ny, self.nx)
 
+    def frame_record(self, t_plot=1):
+        """The FP32 record of a field-output file (plot_f) as a flat array."""
+        n = C.c_int64()
+        _check(lib().suli_frame_values(t_plot, C.byref(n)), "suli_frame_values")
+        buf = np.empty(n.value, dtype=np.float32)
+        _check(lib().suli_frame_record(t_plot, buf.ctypes.data_as(C.POINTER(C.c_float)), buf.size),
+               "suli_frame_record")
+        return buf
+
+    def write_frame(self, path, t_plot=1):
+        _check(lib().suli_write_frame(os.fsencode(path), t_plot), "suli_write_frame")
+
     def unet_dataset_dims(self):
         """(nx, ny, nz, entry) of the held training set."""
         v = [C.c_int() for _ in range(4)]
@@ -367,6 +383,11 @@ class Solver:
         _check(lib().suli_unet_sample_loss_grad(idx, C.byref(loss), g.ctypes.data_as(P(C.c_float)),
                                                 z.ctypes.data_as(P(C.c_float))), "suli_unet_sample_loss_grad")
         return loss.value, g, z.reshape(nz, ny, nx)
+
+
+def write_xdmf(path, nx, ny, nz, first, last, t_plot=1, dt_frame=1.0):
+    """The ParaView index of the records campos_<first>..campos_<last>."""
+    _check(lib().suli_write_xdmf(os.fsencode(path), nx, ny, nz, first, last, t_plot, dt_frame), "suli_write_xdmf")
 
 
 def load_state(solver: Solver, state, names=None):

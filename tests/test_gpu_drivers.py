@@ -15,13 +15,13 @@ pytestmark = pytest.mark.gpu
 PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "suli-openmp_amd")
 
 
-def test_fortran_driver_runs_the_time_loop():
+def test_fortran_driver_runs_the_time_loop(tmp_path):
     exe = os.path.join(PKG, "principal_gpu")
     if not os.path.exists(exe):
         pytest.skip("principal_gpu not built (flang absent)")
     print("fortran driver 64x32x32, 20 steps", flush=True)
     out = subprocess.run([exe, "64", "32", "32", "20", "1", "0.25", "0.25", "10"], capture_output=True, text=True,
-                         timeout=240)
+                         timeout=240, cwd=str(tmp_path))
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
     steps = re.findall(r"step\s+(\d+) t=\s*(\S+) vol_err=\s*(\S+) maxdiv=\s*(\S+) poisson_iters=\s*(\d+)", out.stdout)
     assert len(steps) == 20
@@ -30,6 +30,20 @@ def test_fortran_driver_runs_the_time_loop():
     frames = re.findall(r"frame\s+(\d+) max\|u\|=\s*(\S+)", out.stdout)
     assert [int(f[0]) for f in frames] == [10, 20] and all(0 < float(f[1]) < 5 for f in frames)
     assert "Mcell-steps/s=" in out.stdout
+    # field output: the initial record and one per frame, each one Fortran
+    # record of the t_plot = 1 layout, and the ParaView index over them
+    nx, ny, nz = 64, 32, 32
+    n1, n = (nx + 1) * (ny + 1) * (nz + 1), nx * ny * nz
+    nval = 6 * n1 + 6 * n + (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1)
+    for q in (10, 11, 12):
+        raw = open(tmp_path / "arquivos" / f"campos_{q:05d}", "rb").read()
+        assert len(raw) == 4 * nval + 8
+        assert np.frombuffer(raw[:4], "<i4")[0] == 4 * nval == np.frombuffer(raw[-4:], "<i4")[0]
+        assert np.all(np.isfinite(np.frombuffer(raw[4:-4], "<f4")))
+    import xml.etree.ElementTree as ET
+    text = open(tmp_path / "arquivos" / "visu.xdmf").read()
+    grids = ET.fromstring(text.split("\n", 2)[2]).find("Domain").find("Grid").findall("Grid")
+    assert [g.get("Name") for g in grids] == ["campos_00010", "campos_00011", "campos_00012"]
 
 
 def test_sloshing_forcing_drives_the_tank():
