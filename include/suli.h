@@ -280,6 +280,22 @@ int suli_write_frame(const char *path, int t_plot);
  * library context. */
 int suli_write_xdmf(const char *path, int nx, int ny, int nz, int first, int last, int t_plot, double dt_frame);
 
+/* Checkpoint and restart (restart_salva / restart_ini,
+ * 11_diversos.f90:232-329).  suli_checkpoint_save/load: binary, the device
+ * state of the run as it is (fields with their ghosts in the run's
+ * precision, volumes, time and forcing), so that a run resumed from it
+ * continues bit for bit; loads into a context of the same grid and
+ * precision only.  suli_restart_write/read: the reference's own restart
+ * file dados/arquivorestart, list-directed text in restart_salva's order and
+ * extents; items outside this library's model (obstacle indices, prescribed
+ * boundary planes, d_max, d_min, ls_m, tf_u/v/w) are written as zeros and
+ * skipped on reading, as are items it rebuilds (prd0, prd, ub, vb, wb,
+ * xnut, ynut, znut). */
+int suli_checkpoint_save(const char *path);
+int suli_checkpoint_load(const char *path);
+int suli_restart_write(const char *path);
+int suli_restart_read(const char *path);
+
 /* Per-kernel GPU timing: HIP events recorded on the library's stream around
  * every kernel launch while enabled.  suli_prof_enable(1) clears the totals;
  * suli_prof_get returns kernel idx's name, launch count and summed time. */

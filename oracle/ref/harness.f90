@@ -9,8 +9,9 @@
 !
 ! CMD: ls_ini | level_set | heaviside | mod_ls1 | reinic | der_weno | weno1 ARG(ihs)
 !      visco | convdiff | tempo ARG(tt) | contorno ARG(nlock) | graddin | posdin
-!      stage ARG(tt) | step ARG(nsteps) | bench | plot | visu
-! (plot and visu write under arquivos/ of the working directory)
+!      stage ARG(tt) | step ARG(nsteps) | bench | plot | visu | restart_salva | restart_ini
+! (plot and visu write under arquivos/, the restart commands use dados/, of
+! the working directory)
 !
 ! State file: int32 nx,ny,nz; 10 float64 scalars (dt, t, vol_ini, vol_ins,
 ! rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1); then the arrays listed in
@@ -181,6 +182,12 @@ program harness
   case ('visu')
     ! the ParaView index arquivos/visu.xdmf (12_paraview_3D.f90:1-179)
     call visu()
+  case ('restart_salva')
+    ! dados/arquivorestart of the working directory (11_diversos.f90:303-329)
+    call restart_salva()
+  case ('restart_ini')
+    ! reads dados/arquivorestart (11_diversos.f90:232-301)
+    call restart_ini()
   case default
     write(*,*) 'unknown command ', trim(cmd)
     stop 2

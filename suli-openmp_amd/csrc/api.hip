@@ -768,3 +768,285 @@ extern "C" int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps) {
   }
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Checkpoint / restart (restart_salva, restart_ini: 11_diversos.f90:232-329)
+
+// Fields of a binary checkpoint: everything the time step carries from one
+// step to the next or reads before recomputing it.
+static const int kCkptFields[] = {SULI_U,      SULI_V,      SULI_W,      SULI_PRD1,   SULI_RHO,  SULI_LS_NU,
+                                  SULI_LS,     SULI_MOD_LS, SULI_KURV,   SULI_HS,     SULI_HSX,  SULI_HSY,
+                                  SULI_HSZ,    SULI_DDLSDX, SULI_DDLSDY, SULI_DDLSDZ, SULI_NUT,  SULI_U0,
+                                  SULI_V0,     SULI_W0,     SULI_TFU,    SULI_TFV,    SULI_TFW};
+static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '1'};
+
+struct CkptHeader {
+  char magic[8];
+  int32_t nx, ny, nz, prec;
+  int64_t frame_total;
+  int32_t nfields, tf_on;
+  double vol_ini, vol_ins, t_now, force_amp, force_period, force_phase;
+};
+
+// Binary checkpoint: the device frames of the state fields as they are (the
+// precision of the run, ghosts included) and the scalars of the run, so that
+// a run resumed from it continues bit for bit.  It loads into a context of
+// the same grid and precision only.
+extern "C" int suli_checkpoint_save(const char *path) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  if (!path) {
+    suli_set_error("suli_checkpoint_save: null path");
+    return 1;
+  }
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.nx = c.g.nx; h.ny = c.g.ny; h.nz = c.g.nz; h.prec = c.prec;
+  h.frame_total = c.g.total;
+  h.nfields = (int32_t)(sizeof(kCkptFields) / sizeof(int));
+  h.tf_on = c.tf_on ? 1 : 0;
+  h.vol_ini = c.vol_ini; h.vol_ins = c.vol_ins; h.t_now = c.t_now;
+  h.force_amp = c.force_amp; h.force_period = c.force_period; h.force_phase = c.force_phase;
+  HIPCHK(hipStreamSynchronize(c.st));
+  FILE *f = std::fopen(path, "wb");
+  if (!f) {
+    suli_set_error(std::string("suli_checkpoint_save: cannot open ") + path);
+    return 1;
+  }
+  bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+  std::vector<char> buf;
+  for (int q = 0; q < h.nfields && ok; q++) {
+    FieldRef r;
+    field_ref(c, kCkptFields[q], r);
+    const size_t bytes = (size_t)c.g.total * (r.f64 ? 8 : c.esz);
+    buf.resize(bytes);
+    if (hipMemcpy(buf.data(), r.dev, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+      std::fclose(f);
+      suli_set_error("suli_checkpoint_save: device copy failed");
+      return 1;
+    }
+    const int32_t id = kCkptFields[q];
+    ok = std::fwrite(&id, 4, 1, f) == 1 && std::fwrite(buf.data(), 1, bytes, f) == bytes;
+  }
+  if (std::fclose(f) != 0 || !ok) {
+    suli_set_error(std::string("suli_checkpoint_save: write failed on ") + path);
+    return 1;
+  }
+  return 0;
+}
+
+extern "C" int suli_checkpoint_load(const char *path) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  FILE *f = path ? std::fopen(path, "rb") : nullptr;
+  if (!f) {
+    suli_set_error(std::string("suli_checkpoint_load: cannot open ") + (path ? path : "(null)"));
+    return 1;
+  }
+  CkptHeader h{};
+  std::string err;
+  if (std::fread(&h, sizeof h, 1, f) != 1 || std::memcmp(h.magic, kCkptMagic, 8) != 0) err = "not a checkpoint";
+  else if (h.nx != c.g.nx || h.ny != c.g.ny || h.nz != c.g.nz || h.prec != c.prec || h.frame_total != c.g.total)
+    err = "grid " + std::to_string(h.nx) + "x" + std::to_string(h.ny) + "x" + std::to_string(h.nz) + " precision " +
+          std::to_string(h.prec) + " differs from this context's";
+  std::vector<char> buf;
+  for (int q = 0; q < h.nfields && err.empty(); q++) {
+    int32_t id = -1;
+    FieldRef r;
+    if (std::fread(&id, 4, 1, f) != 1 || !field_ref(c, id, r) || r.derived) {
+      err = "bad field record " + std::to_string(q);
+      break;
+    }
+    const size_t bytes = (size_t)c.g.total * (r.f64 ? 8 : c.esz);
+    buf.resize(bytes);
+    if (std::fread(buf.data(), 1, bytes, f) != bytes) {
+      err = "truncated at field " + std::to_string(id);
+      break;
+    }
+    if (hipMemcpy(r.dev, buf.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) err = "device copy failed";
+  }
+  std::fclose(f);
+  if (!err.empty()) {
+    suli_set_error(std::string("suli_checkpoint_load: ") + path + ": " + err);
+    return 1;
+  }
+  c.tf_on = h.tf_on != 0;
+  c.vol_ini = h.vol_ini; c.vol_ins = h.vol_ins; c.t_now = h.t_now;
+  c.force_amp = h.force_amp; c.force_period = h.force_period; c.force_phase = h.force_phase;
+  c.convdiff_pending = false;
+  return 0;
+}
+
+// Reference restart file (dados/arquivorestart): list-directed text of the
+// arrays and scalars of restart_salva (11_diversos.f90:319-321), in that
+// order, with the reference extents (0_modules.f90).  Items this library
+// does not model are written as the reference holds them in its
+// configuration here and skipped on reading: the obstacle indices ku, kv, kw
+// (0, no obstacles), the prescribed-boundary planes bxx0..bzzf1 (0, no
+// inflow), d_max, d_min, ls_m (0), the friction planes ub, vb, wb (rebuilt by
+// contorno), prd0 and prd (copies of prd1), the face viscosities (rebuilt by
+// visco) and tf_u, tf_v, tf_w (zeroed by restart_ini when mms_t = 0,
+// 11_diversos.f90:268-273).
+namespace {
+struct RItem {
+  int field;          // >= 0: a library field; -1: zeros; -2: scalar
+  int n0, n1, n2;     // extents (values written); scalar: 1
+  const char *name;
+};
+std::vector<RItem> restart_items(const Grid &g) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  std::vector<RItem> v = {
+      {SULI_U, nx + 3, ny + 2, nz + 2, "u"},     {SULI_V, nx + 2, ny + 3, nz + 2, "v"},
+      {SULI_W, nx + 2, ny + 2, nz + 3, "w"},     {-1, nx + 3, ny + 2, 1, "ku"},
+      {-1, nx + 2, ny + 3, 1, "kv"},             {-1, nx + 2, ny + 2, 1, "kw"},
+      {-3, nx + 2, ny + 2, nz + 2, "prd0"},      {SULI_PRD1, nx + 2, ny + 2, nz + 2, "prd1"},
+      {-3, nx + 2, ny + 2, nz + 2, "prd"},       {SULI_RHO, nx, ny, nz, "rho"},
+      {SULI_LS_NU, nx, ny, nz, "ls_nu"},         {SULI_LS, nx, ny, nz, "ls"},
+      {SULI_MOD_LS, nx, ny, nz, "mod_ls"}};
+  const int planes[24][2] = {{ny + 2, nz + 2}, {ny + 2, nz + 2}, {ny + 3, nz + 2}, {ny + 2, nz + 3},   // bxx0 bxx1 bxy0 bxz0
+                             {ny + 2, nz + 2}, {ny + 2, nz + 2}, {ny + 3, nz + 2}, {ny + 2, nz + 3},   // bxxf bxxf1 bxyf bxzf
+                             {nx + 3, nz + 2}, {nx + 2, nz + 2}, {nx + 2, nz + 2}, {nx + 2, nz + 3},   // byx0 byy0 byy1 byz0
+                             {nx + 3, nz + 2}, {nx + 2, nz + 2}, {nx + 2, nz + 2}, {nx + 2, nz + 3},   // byxf byyf byyf1 byzf
+                             {nx + 3, ny + 2}, {nx + 2, ny + 3}, {nx + 2, ny + 2}, {nx + 2, ny + 2},   // bzx0 bzy0 bzz0 bzz1
+                             {nx + 3, ny + 2}, {nx + 2, ny + 3}, {nx + 2, ny + 2}, {nx + 2, ny + 2}};  // bzxf bzyf bzzf bzzf1
+  for (auto &p : planes) v.push_back({-1, p[0], p[1], 1, "bc plane"});
+  v.push_back({-1, nx + 3, ny + 2, nz + 2, "ub"});
+  v.push_back({-1, nx + 2, ny + 3, nz + 2, "vb"});
+  v.push_back({-1, nx + 2, ny + 2, nz + 3, "wb"});
+  v.push_back({-1, 1, 1, 1, "d_max"});
+  v.push_back({-1, 1, 1, 1, "d_min"});
+  v.push_back({-1, 1, 1, 1, "ls_m"});
+  v.push_back({SULI_HSX, nx, ny, nz, "hsx"});
+  v.push_back({SULI_HSY, nx, ny, nz, "hsy"});
+  v.push_back({SULI_HSZ, nx, ny, nz, "hsz"});
+  v.push_back({SULI_HS, nx, ny, nz, "hs"});
+  v.push_back({SULI_KURV, nx, ny, nz, "kurv"});
+  v.push_back({-4, nx + 1, ny, nz, "xnut"});
+  v.push_back({-4, nx, ny + 1, nz, "ynut"});
+  v.push_back({-4, nx, ny, nz + 1, "znut"});
+  v.push_back({-1, nx + 1, ny, nz, "tf_u"});
+  v.push_back({-1, nx, ny + 1, nz, "tf_v"});
+  v.push_back({-1, nx, ny, nz + 1, "tf_w"});
+  v.push_back({-2, 1, 1, 1, "vol_ini"});
+  v.push_back({-2, 1, 1, 1, "vol_ins"});
+  return v;
+}
+}  // namespace
+
+extern "C" int suli_restart_write(const char *path) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  FILE *f = path ? std::fopen(path, "w") : nullptr;
+  if (!f) {
+    suli_set_error(std::string("suli_restart_write: cannot open ") + (path ? path : "(null)"));
+    return 1;
+  }
+  std::vector<double> host;
+  bool ok = true;
+  int col = 0;
+  auto put = [&](double x) {
+    ok = ok && std::fprintf(f, col == 3 ? " %.17g\n" : " %.17g", x) > 0;
+    col = (col + 1) % 4;
+  };
+  for (const RItem &it : restart_items(c.g)) {
+    const int64_t n = (int64_t)it.n0 * it.n1 * it.n2;
+    int field = it.field;
+    if (field == -3) field = SULI_PRD1;  // prd0, prd: copies of prd1
+    if (field == -4) field = it.name[0] == 'x' ? SULI_XNUT : (it.name[0] == 'y' ? SULI_YNUT : SULI_ZNUT);
+    if (field >= 0) {
+      host.resize((size_t)n);
+      if (suli_download(field, host.data(), n)) {
+        std::fclose(f);
+        return 1;
+      }
+      for (double x : host) put(x);
+    } else if (field == -2) {
+      put(std::strcmp(it.name, "vol_ini") == 0 ? c.vol_ini : c.vol_ins);
+    } else {
+      for (int64_t q = 0; q < n; q++) put(0.0);
+    }
+  }
+  if (col) ok = ok && std::fputc('\n', f) != EOF;
+  if (std::fclose(f) != 0 || !ok) {
+    suli_set_error(std::string("suli_restart_write: write failed on ") + path);
+    return 1;
+  }
+  return 0;
+}
+
+// list-directed values: separators are blanks, commas and line ends; "r*c"
+// stands for r copies of c
+namespace {
+struct LdReader {
+  FILE *f;
+  double rep_val = 0;
+  int64_t rep_left = 0;
+  bool next(double *x) {
+    if (rep_left > 0) {
+      rep_left--;
+      *x = rep_val;
+      return true;
+    }
+    char tok[128];
+    int n = 0, ch;
+    while ((ch = std::fgetc(f)) != EOF && (ch == ' ' || ch == '\n' || ch == '\r' || ch == '\t' || ch == ','));
+    if (ch == EOF) return false;
+    while (ch != EOF && !(ch == ' ' || ch == '\n' || ch == '\r' || ch == '\t' || ch == ',') && n < 127) {
+      tok[n++] = (char)ch;
+      ch = std::fgetc(f);
+    }
+    tok[n] = 0;
+    for (int q = 0; q < n; q++)
+      if (tok[q] == 'd' || tok[q] == 'D') tok[q] = 'e';  // Fortran D exponents
+    char *star = std::strchr(tok, '*');
+    char *end = nullptr;
+    if (star) {
+      *star = 0;
+      const long long r = std::strtoll(tok, &end, 10);
+      const double v = std::strtod(star + 1, &end);
+      if (r < 1 || end == star + 1) return false;
+      rep_val = v;
+      rep_left = r - 1;
+      *x = v;
+      return true;
+    }
+    *x = std::strtod(tok, &end);
+    return end != tok && *end == 0;
+  }
+};
+}  // namespace
+
+extern "C" int suli_restart_read(const char *path) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  FILE *f = path ? std::fopen(path, "r") : nullptr;
+  if (!f) {
+    suli_set_error(std::string("suli_restart_read: cannot open ") + (path ? path : "(null)"));
+    return 1;
+  }
+  LdReader rd{f};
+  std::vector<double> host;
+  double vol[2] = {0, 0};
+  int nvol = 0;
+  for (const RItem &it : restart_items(c.g)) {
+    const int64_t n = (int64_t)it.n0 * it.n1 * it.n2;
+    host.resize((size_t)n);
+    for (int64_t q = 0; q < n; q++)
+      if (!rd.next(&host[(size_t)q])) {
+        std::fclose(f);
+        suli_set_error(std::string("suli_restart_read: ") + path + ": file ends or holds a non-number in " +
+                       it.name + " (value " + std::to_string(q + 1) + " of " + std::to_string(n) + ")");
+        return 1;
+      }
+    if (it.field >= 0 && suli_upload(it.field, host.data(), n)) {
+      std::fclose(f);
+      return 1;
+    }
+    if (it.field == -2) vol[nvol++] = host[0];
+  }
+  std::fclose(f);
+  c.vol_ini = vol[0];
+  c.vol_ins = vol[1];
+  c.convdiff_pending = false;
+  return 0;
+}

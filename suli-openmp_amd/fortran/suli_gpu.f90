@@ -115,6 +115,14 @@ module suli_gpu
       character(kind=c_char), dimension(*) :: path
       integer(c_int), value :: t_plot
     end function
+    integer(c_int) function c_suli_restart_write(path) bind(C, name='suli_restart_write')
+      import :: c_int, c_char
+      character(kind=c_char), dimension(*) :: path
+    end function
+    integer(c_int) function c_suli_restart_read(path) bind(C, name='suli_restart_read')
+      import :: c_int, c_char
+      character(kind=c_char), dimension(*) :: path
+    end function
     integer(c_int) function c_suli_write_xdmf(path, nx, ny, nz, first, last, t_plot, dt_frame) &
         bind(C, name='suli_write_xdmf')
       import :: c_int, c_char, c_double
@@ -221,6 +229,21 @@ contains
     call check(c_suli_write_xdmf('arquivos/visu.xdmf'//c_null_char, prm%nx, prm%ny, prm%nz, 10_c_int, &
                                  int(cont, c_int), int(t_plot, c_int), dt_frame), 'visu')
   end subroutine visu
+
+  ! restart_salva / restart_ini (11_diversos.f90:232-329): the reference's
+  ! restart file dados/arquivorestart, written from and read into the GPU state
+  subroutine restart_salva()
+    call execute_command_line('mkdir -p dados')
+    call check(c_suli_restart_write('dados/arquivorestart'//c_null_char), 'restart_salva')
+    write(*,*) 'Salvou restart'
+  end subroutine restart_salva
+
+  subroutine restart_ini()
+    call check(c_suli_restart_read('dados/arquivorestart'//c_null_char), 'restart_ini')
+    it = 0
+    t = 0.
+    cont = 10
+  end subroutine restart_ini
 
   ! host copy of a field in the reference extents, for output
   subroutine download(field, a)

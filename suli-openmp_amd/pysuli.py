@@ -118,6 +118,8 @@ def lib():
             "suli_frame_values": ([i32, p(C.c_int64)], i32),
             "suli_frame_record": ([i32, p(C.c_float), C.c_int64], i32),
             "suli_write_frame": ([C.c_char_p, i32], i32),
+            "suli_checkpoint_save": ([C.c_char_p], i32), "suli_checkpoint_load": ([C.c_char_p], i32),
+            "suli_restart_write": ([C.c_char_p], i32), "suli_restart_read": ([C.c_char_p], i32),
             "suli_write_xdmf": ([C.c_char_p, i32, i32, i32, i32, i32, i32, f64], i32),
         }
         for name, (args, res) in sig.items():
@@ -347,6 +349,18 @@ class Solver:
 
     def write_frame(self, path, t_plot=1):
         _check(lib().suli_write_frame(os.fsencode(path), t_plot), "suli_write_frame")
+
+    def checkpoint_save(self, path):
+        _check(lib().suli_checkpoint_save(os.fsencode(path)), "suli_checkpoint_save")
+
+    def checkpoint_load(self, path):
+        _check(lib().suli_checkpoint_load(os.fsencode(path)), "suli_checkpoint_load")
+
+    def restart_write(self, path):
+        _check(lib().suli_restart_write(os.fsencode(path)), "suli_restart_write")
+
+    def restart_read(self, path):
+        _check(lib().suli_restart_read(os.fsencode(path)), "suli_restart_read")
 
     def unet_dataset_dims(self):
         """(nx, ny, nz, entry) of the held training set."""
