@@ -82,6 +82,28 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// Work-group order of the grid stencils.  The dispatcher places
+// work-group b on XCD b mod 8 and each XCD has its own L2; in grid order the
+// block rows of one XCD are every fourth row, so the rows above and below a
+// block (the j-neighbours of its stencil) sit in other XCDs' caches.  The
+// blocks are renumbered so that XCD x owns a contiguous band of block rows
+// through all planes, walked plane by plane: its j- and k-neighbours are its
+// own.  (Identity when the block rows do not split into 8 bands.)
+__device__ __forceinline__ void xcd_band_block(int &bx, int &by, int &bz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  if (gy % 8u) {
+    bx = blockIdx.x;
+    by = blockIdx.y;
+    bz = blockIdx.z;
+    return;
+  }
+  const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned band = gy / 8u, t = b / 8u;
+  bx = (int)(t % gx);
+  by = (int)((b % 8u) * band + (t / gx) % band);
+  bz = (int)(t / (gx * band));
+}
+
 // Sum (or max) over the block of nv values per thread; result valid in thread 0.
 template <int NV, bool MAX = false>
 __device__ __forceinline__ void block_reduce(double (&v)[NV], double *lds /* NV*SULI_LDS_RED */) {

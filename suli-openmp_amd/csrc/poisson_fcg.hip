@@ -80,9 +80,11 @@ __device__ __forceinline__ A apply_As(const Grid &g, int64_t c, int i, int j, in
          (A)ay[c] * (A)v[js] - (A)az[c + g.sz] * (A)v[kt] - (A)az[c] * (A)v[kb];
 }
 
-#define CELL_IJK                                                                             \
-  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y; \
-  const int kz0 = 1 + blockIdx.z;
+#define CELL_IJK                                                                        \
+  int bx_, by_, bz_;                                                                    \
+  xcd_band_block(bx_, by_, bz_);                                                         \
+  const int i = 1 + bx_ * SULI_BX + threadIdx.x, j = 1 + by_ * SULI_BY + threadIdx.y; \
+  const int kz0 = 1 + bz_;
 
 // ---- outer (FP64) ---------------------------------------------------------
 // r = b - A_s x in FP64; sums r.r, r.n, n.n
@@ -622,8 +624,9 @@ __global__ __launch_bounds__(256) void k_mg0_in(Grid g, Grid h, const float *__r
                                                 float *__restrict__ bred, float *__restrict__ bblack,
                                                 float *__restrict__ xred, const int *active) {
   if (!*active) return;
-  const int p = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
-            k = 1 + blockIdx.z;
+  int bx, by, bz;
+  xcd_band_block(bx, by, bz);
+  const int p = bx * SULI_BX + threadIdx.x, j = 1 + by * SULI_BY + threadIdx.y, k = 1 + bz;
   const int i1 = 1 + 2 * p;
   if (j > g.ny || k > g.nz || i1 > g.nx) return;
   const int64_t c1 = g.idx(i1, j, k), q = h.idx(p, j, k);
@@ -648,8 +651,9 @@ __global__ __launch_bounds__(256) void k_mg0_gs(Grid g, Grid h, const uint16_t *
                                                 const float *__restrict__ bcol, const float *__restrict__ xo,
                                                 float *__restrict__ xcol, int color, const int *active) {
   if (!*active) return;
-  const int p = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
-            k = 1 + blockIdx.z;
+  int bx, by, bz;
+  xcd_band_block(bx, by, bz);
+  const int p = bx * SULI_BX + threadIdx.x, j = 1 + by * SULI_BY + threadIdx.y, k = 1 + bz;
   if (j > g.ny || k > g.nz) return;
   const int sec = (1 + j + k + color) & 1;
   const int i = 1 + 2 * p + sec;
@@ -668,8 +672,9 @@ __global__ __launch_bounds__(256) void k_mg0_prolong(Grid g, Grid h, Grid gc, co
                                                      const float *__restrict__ xred, float *__restrict__ xblack,
                                                      const float *__restrict__ xc, const int *active) {
   if (!*active) return;
-  const int p = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
-            k = 1 + blockIdx.z;
+  int bx, by, bz;
+  xcd_band_block(bx, by, bz);
+  const int p = bx * SULI_BX + threadIdx.x, j = 1 + by * SULI_BY + threadIdx.y, k = 1 + bz;
   if (j > g.ny || k > g.nz) return;
   const int sec = (j + k) & 1;  // colour 1
   const int i = 1 + 2 * p + sec;
@@ -693,8 +698,9 @@ __global__ __launch_bounds__(256) void k_mg0_out(Grid g, Grid h, const uint16_t 
                                                  const float *__restrict__ q, const int *first, double *partials,
                                                  const int *active) {
   if (!*active) return;
-  const int p = blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
-            k = 1 + blockIdx.z;
+  int bx, by, bz;
+  xcd_band_block(bx, by, bz);
+  const int p = bx * SULI_BX + threadIdx.x, j = 1 + by * SULI_BY + threadIdx.y, k = 1 + bz;
   const int i1 = 1 + 2 * p;
   double acc[1] = {0.0};
   const bool dot = partials && !*first;
@@ -737,8 +743,9 @@ __global__ __launch_bounds__(256) void k_mg0_restrict(Grid gf, Grid h, Grid gc, 
                                                       const float *__restrict__ x1, float *__restrict__ bc,
                                                       float *__restrict__ xc, const int *active) {
   if (!*active) return;
-  const int I = 1 + blockIdx.x * SULI_BX + threadIdx.x, J = 1 + blockIdx.y * SULI_BY + threadIdx.y,
-            K = 1 + blockIdx.z;
+  int bx, by, bz;
+  xcd_band_block(bx, by, bz);
+  const int I = 1 + bx * SULI_BX + threadIdx.x, J = 1 + by * SULI_BY + threadIdx.y, K = 1 + bz;
   if (I > gc.nx || J > gc.ny) return;
   float s = 0.f;
   const int64_t c0 = gf.idx(2 * I - 1, 2 * J - 1, 2 * K - 1), q0 = h.idx(I - 1, 2 * J - 1, 2 * K - 1);
