@@ -167,22 +167,31 @@ struct EpiNone {
   __device__ void operator()(const double *) const {}
 };
 
-// Sum (or max) of the nb block partials of NV values, in block order; thread 0
-// stores result[0..NV) and runs the epilogue (loop control of the solvers).
+// Sum (or max) of the nb block partials of NV values in a fixed order; thread
+// 0 stores result[0..NV) and runs the epilogue (loop control of the solvers).
+// The partials were written by blocks on every XCD and come from memory: a
+// block of SULI_FINISH_THREADS threads with all of a thread's loads issued
+// before its adds keeps the serial load latency to one or two round trips.
+#define SULI_FINISH_THREADS 1024
 template <int NV, bool MAX, typename Epi>
-__global__ __launch_bounds__(256) void k_finish(const double *__restrict__ partials, int nb,
-                                                double *__restrict__ result, Epi epi) {
+__global__ __launch_bounds__(SULI_FINISH_THREADS) void k_finish(const double *__restrict__ partials, int nb,
+                                                                double *__restrict__ result, Epi epi) {
   if (!epi.enabled()) return;
   __shared__ double lds[NV * SULI_LDS_RED];
   double acc[NV];
+  const double init = MAX ? -1.0e300 : 0.0;
+  const int bd = blockDim.x;
 #pragma unroll
   for (int q = 0; q < NV; q++) {
-    double a = MAX ? -1.0e300 : 0.0;
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-      const double x = partials[(size_t)q * nb + t];
-      a = MAX ? (x > a ? x : a) : a + x;
+    const double *pq = partials + (size_t)q * nb;
+    double a0 = init, a1 = init;
+    for (int t = threadIdx.x; t < nb; t += 2 * bd) {
+      const double x0 = pq[t];
+      const double x1 = t + bd < nb ? pq[t + bd] : init;
+      a0 = MAX ? (x0 > a0 ? x0 : a0) : a0 + x0;
+      a1 = MAX ? (x1 > a1 ? x1 : a1) : a1 + x1;
     }
-    acc[q] = a;
+    acc[q] = MAX ? (a1 > a0 ? a1 : a0) : a0 + a1;
   }
   block_reduce<NV, MAX>(acc, lds);
   if (threadIdx.x == 0) {
@@ -195,7 +204,8 @@ __global__ __launch_bounds__(256) void k_finish(const double *__restrict__ parti
 template <int NV, bool MAX = false, typename Epi = EpiNone>
 inline void launch_finish(hipStream_t s, dim3 producer_grid, RedSlot rs, Epi epi = Epi()) {
   const int nb = (int)(producer_grid.x * producer_grid.y * producer_grid.z);
-  hipLaunchKernelGGL((k_finish<NV, MAX, Epi>), dim3(1), dim3(256), 0, s, rs.partials, nb, rs.result, epi);
+  hipLaunchKernelGGL((k_finish<NV, MAX, Epi>), dim3(1), dim3(SULI_FINISH_THREADS), 0, s, rs.partials, nb, rs.result,
+                     epi);
 }
 
 // ---------------------------------------------------------------------------

@@ -494,8 +494,12 @@ void wgrad(UNetPrecond &u, hipStream_t st, Dims d, const uint16_t *dy, const uin
 }
 
 void finish_sum(hipStream_t st, double *part, int nb, double *res, int nv) {
-  if (nv == 1) hipLaunchKernelGGL((k_finish<1, false, EpiNone>), dim3(1), dim3(256), 0, st, part, nb, res, EpiNone());
-  else hipLaunchKernelGGL((k_finish<3, false, EpiNone>), dim3(1), dim3(256), 0, st, part, nb, res, EpiNone());
+  if (nv == 1)
+    hipLaunchKernelGGL((k_finish<1, false, EpiNone>), dim3(1), dim3(SULI_FINISH_THREADS), 0, st, part, nb, res,
+                       EpiNone());
+  else
+    hipLaunchKernelGGL((k_finish<3, false, EpiNone>), dim3(1), dim3(SULI_FINISH_THREADS), 0, st, part, nb, res,
+                       EpiNone());
 }
 
 int alloc_f(Ctx &c, float **p, int64_t n) {
@@ -832,8 +836,8 @@ void unet_coarse_solve(Ctx &c, UNetPrecond *u, const Grid &ge, const float *cx, 
   hipLaunchKernelGGL(k_cs_scatter, g1(n), dim3(256), 0, c.st, ge, d, (const float *)u->z, dg, x, active);
   hipLaunchKernelGGL(k_cs_dots, dim3(512), dim3(256), 0, c.st, ge, d, cx, cy, cz, dg, b, (const float *)x, u->red,
                      active);
-  hipLaunchKernelGGL((k_finish<2, false, EpiNone>), dim3(1), dim3(256), 0, c.st, u->red, 512, u->red + kResCsDots,
-                     EpiNone());
+  hipLaunchKernelGGL((k_finish<2, false, EpiNone>), dim3(1), dim3(SULI_FINISH_THREADS), 0, c.st, u->red, 512,
+                     u->red + kResCsDots, EpiNone());
   hipLaunchKernelGGL(k_cs_scale, g1(n), dim3(256), 0, c.st, ge, d, x, (const double *)(u->red + kResCsDots),
                      active);
 }
