@@ -260,6 +260,15 @@ int suli_unet_dataset_get(int idx, float *r, float *rho, float *faces);
  * operator that training uses for it. */
 int suli_unet_sample_loss_grad(int idx, double *loss, float *grad, float *z);
 
+/* Initial guess of the FCG projection (SULI_POISSON_FCG; the reference CG
+ * always starts from the current pressure, 9_grad.f90:121).  1 (default):
+ * the projection of RK stage 1 (the stage set by the last suli_tempo)
+ * starts from the linear extrapolation 2 p2(n) - p2(n-1) of the last two
+ * stage-2 pressures, once they are known for this dt; 0: from the current
+ * pressure, as the reference.  Uploading prd1 restarts the history.  No
+ * counterpart in the reference. */
+int suli_set_guess(int mode);
+
 /* Field output (plot_f, 4_plot.f90:260-363): the one FP32 record of an
  * arquivos/campos_NNNNN file, built on the device from the resident fields:
  * x11, y11, z11, uaux, vaux, waux ((nx+1)(ny+1)(nz+1) node values each:

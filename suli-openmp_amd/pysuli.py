@@ -119,6 +119,7 @@ def lib():
             "suli_frame_record": ([i32, p(C.c_float), C.c_int64], i32),
             "suli_write_frame": ([C.c_char_p, i32], i32),
             "suli_checkpoint_save": ([C.c_char_p], i32), "suli_checkpoint_load": ([C.c_char_p], i32),
+            "suli_set_guess": ([i32], i32),
             "suli_restart_write": ([C.c_char_p], i32), "suli_restart_read": ([C.c_char_p], i32),
             "suli_write_xdmf": ([C.c_char_p, i32, i32, i32, i32, i32, i32, f64], i32),
         }
@@ -349,6 +350,9 @@ class Solver:
 
     def write_frame(self, path, t_plot=1):
         _check(lib().suli_write_frame(os.fsencode(path), t_plot), "suli_write_frame")
+
+    def set_guess(self, mode):
+        _check(lib().suli_set_guess(mode), "suli_set_guess")
 
     def checkpoint_save(self, path):
         _check(lib().suli_checkpoint_save(os.fsencode(path)), "suli_checkpoint_save")
