@@ -262,9 +262,10 @@ int suli_unet_sample_loss_grad(int idx, double *loss, float *grad, float *z);
 
 /* Initial guess of the FCG projection (SULI_POISSON_FCG; the reference CG
  * always starts from the current pressure, 9_grad.f90:121).  1 (default):
- * the projection of RK stage 1 (the stage set by the last suli_tempo)
- * starts from the linear extrapolation 2 p2(n) - p2(n-1) of the last two
- * stage-2 pressures, once they are known for this dt; 0: from the current
+ * once the stage-2 pressures of the last steps (same step size) are known,
+ * a projection of RK stage 1 or 2 (the stage set by the last suli_tempo)
+ * starts from their extrapolation to this step, 3 p2(n) - 3 p2(n-1) +
+ * p2(n-2) (from two: 2 p2(n) - p2(n-1), stage 1 only); 0: from the current
  * pressure, as the reference.  Uploading prd1 restarts the history.  No
  * counterpart in the reference. */
 int suli_set_guess(int mode);

@@ -377,7 +377,7 @@ extern "C" int suli_upload(int field, const double *host, int64_t n) {
   if (host_to_frame(c, r.dev, r.f64, fr)) return 1;
   if (field == SULI_U || field == SULI_V || field == SULI_W) c.convdiff_pending = false;
   if (field == SULI_TFU || field == SULI_TFV || field == SULI_TFW) c.tf_on = true;
-  if (field == SULI_PRD1) c.p2_count = 0;  // a new pressure: no history to extrapolate
+  if (field == SULI_PRD1) c.p2n = 0;  // a new pressure: no history to extrapolate
   return 0;
 }
 
@@ -634,7 +634,7 @@ extern "C" int suli_set_guess(int mode) {
     return 1;
   }
   g_ctx->guess = mode;
-  g_ctx->p2_count = 0;
+  g_ctx->p2n = 0;
   return 0;
 }
 
@@ -799,8 +799,8 @@ struct CkptHeader {
   int64_t frame_total;
   int32_t nfields, tf_on;
   double vol_ini, vol_ins, t_now, force_amp, force_period, force_phase;
-  int32_t guess, stage, p2_count, has_p2;
-  double dt1_prev;
+  int32_t guess, stage, p2n, pad;
+  double p2dt;
 };
 
 // Binary checkpoint: the device frames of the state fields as they are (the
@@ -822,8 +822,8 @@ extern "C" int suli_checkpoint_save(const char *path) {
   h.tf_on = c.tf_on ? 1 : 0;
   h.vol_ini = c.vol_ini; h.vol_ins = c.vol_ins; h.t_now = c.t_now;
   h.force_amp = c.force_amp; h.force_period = c.force_period; h.force_phase = c.force_phase;
-  h.guess = c.guess; h.stage = c.stage; h.p2_count = c.p2_count; h.has_p2 = c.p2_prev ? 1 : 0;
-  h.dt1_prev = c.dt1_prev;
+  h.guess = c.guess; h.stage = c.stage; h.p2n = c.p2n; h.pad = 0;
+  h.p2dt = c.p2dt;
   HIPCHK(hipStreamSynchronize(c.st));
   FILE *f = std::fopen(path, "wb");
   if (!f) {
@@ -845,10 +845,10 @@ extern "C" int suli_checkpoint_save(const char *path) {
     const int32_t id = kCkptFields[q];
     ok = std::fwrite(&id, 4, 1, f) == 1 && std::fwrite(buf.data(), 1, bytes, f) == bytes;
   }
-  if (ok && h.has_p2) {  // the stage-2 pressure of the previous step (the extrapolation's history)
+  for (int q = 0; q < h.p2n && ok; q++) {  // the stage-2 pressures of the last steps (the guess's history)
     const size_t bytes = (size_t)c.g.total * 8;
     buf.resize(bytes);
-    if (hipMemcpy(buf.data(), c.p2_prev, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpy(buf.data(), c.p2h[q], bytes, hipMemcpyDeviceToHost) != hipSuccess) {
       std::fclose(f);
       suli_set_error("suli_checkpoint_save: device copy failed");
       return 1;
@@ -892,15 +892,15 @@ extern "C" int suli_checkpoint_load(const char *path) {
     }
     if (hipMemcpy(r.dev, buf.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) err = "device copy failed";
   }
-  if (err.empty() && h.has_p2) {
+  for (int q = 0; q < h.p2n && q < 3 && err.empty(); q++) {
     const size_t bytes = (size_t)c.g.total * 8;
     buf.resize(bytes);
-    if (!c.p2_prev) {
-      if (hipMalloc((void **)&c.p2_prev, bytes) != hipSuccess) err = "device allocation failed";
-      else c.allocs.push_back(c.p2_prev);
+    if (!c.p2h[q]) {
+      if (hipMalloc((void **)&c.p2h[q], bytes) != hipSuccess) err = "device allocation failed";
+      else c.allocs.push_back(c.p2h[q]);
     }
     if (err.empty() && std::fread(buf.data(), 1, bytes, f) != bytes) err = "truncated at the pressure history";
-    if (err.empty() && hipMemcpy(c.p2_prev, buf.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+    if (err.empty() && hipMemcpy(c.p2h[q], buf.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
       err = "device copy failed";
   }
   std::fclose(f);
@@ -911,7 +911,7 @@ extern "C" int suli_checkpoint_load(const char *path) {
   c.tf_on = h.tf_on != 0;
   c.vol_ini = h.vol_ini; c.vol_ins = h.vol_ins; c.t_now = h.t_now;
   c.force_amp = h.force_amp; c.force_period = h.force_period; c.force_phase = h.force_phase;
-  c.guess = h.guess; c.stage = h.stage; c.p2_count = h.has_p2 ? h.p2_count : 0; c.dt1_prev = h.dt1_prev;
+  c.guess = h.guess; c.stage = h.stage; c.p2n = std::min(h.p2n, 3); c.p2dt = h.p2dt;
   c.convdiff_pending = false;
   return 0;
 }
@@ -1087,6 +1087,6 @@ extern "C" int suli_restart_read(const char *path) {
   c.vol_ini = vol[0];
   c.vol_ins = vol[1];
   c.convdiff_pending = false;
-  c.p2_count = 0;
+  c.p2n = 0;
   return 0;
 }

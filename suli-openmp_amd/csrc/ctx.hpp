@@ -75,12 +75,11 @@ struct Ctx {
   double vol_ini = 0, vol_ins = 0;
   PoissonFCG *fcg = nullptr;
   // initial guess of the FCG projection: the RK stage set by the last tempo
-  // call; with guess = 1 a stage-1 solve starts from the linear extrapolation
-  // 2 p2(n) - p2(n-1) of the stage-2 pressures (p2_prev holds p2(n-1), valid
-  // once p2_count >= 2 stage-2 solves of the same dt have completed)
-  int guess = 1, stage = 0, p2_count = 0;
-  double dt1_prev = 0.0;
-  double *p2_prev = nullptr;
+  // call; the last three stage-2 pressures p2h[0] (newest) .. p2h[2], p2n of
+  // them valid, all of the base step p2dt (guess = 1: extrapolate from them)
+  int guess = 1, stage = 0, p2n = 0;
+  double p2dt = 0.0;
+  double *p2h[3] = {nullptr, nullptr, nullptr};
   // field output: the record on the device, its pinned host copy, coordinate tables
   float *out_dev = nullptr, *out_host = nullptr, *out_tab = nullptr;
   int64_t out_cap = 0, out_host_cap = 0;
