@@ -1161,7 +1161,9 @@ struct PoissonFCG {
   int kz = 1;               // grid.z of the k-looped reduction kernels
   UNetPrecond *unet = nullptr;
   float *pm[SULI_FCG_M] = {}, *qm[SULI_FCG_M] = {};  // FCG(m) directions and their images
-  int pass_iters[4] = {0, 0, 0, 0};                   // inner iterations of the last solve's passes
+  // inner iterations of the passes of the last solve of each RK stage (0:
+  // solves outside a stage): the two stages' solves differ
+  int pass_iters[3][4] = {};
   int cur_iter = 0;  // inner iteration of the preconditioner application being enqueued
 };
 
@@ -1602,7 +1604,8 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   // a pass that turns out unnecessary is skipped by its own kernels.  When the
   // solve is expected to end, the output pressure is computed with the
   // residual check so that the last read-back finds it done.
-  bool speculate = pc != SULI_PRECOND_LEARNED && f.pass_iters[0] > 0;
+  int *pits = f.pass_iters[c.stage == 1 || c.stage == 2 ? c.stage : 0];
+  bool speculate = pc != SULI_PRECOND_LEARNED && pits[0] > 0;
   const dim3 pg((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4, g.nz + 2);
   bool pout_done = false;
   for (;;) {
@@ -1646,7 +1649,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     // time, so that few iterations are enqueued past the end (those still
     // cost their launches).  Jacobi passes take hundreds: chunks of 16..64.
     const int pidx = std::min(outer - 1, 3);
-    int chunk = use_pc ? (f.pass_iters[pidx] > 0 ? f.pass_iters[pidx] : 4) : 16;
+    int chunk = use_pc ? (pits[pidx] > 0 ? pits[pidx] : 4) : 16;
     int iter = 0;
     for (;;) {
       for (int n = 0; n < chunk; n++, iter++) {
@@ -1705,7 +1708,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       const double pred = (r0 > 0 && r1 >= 0) ? res * std::sqrt(r1 / r0) : 0.0;
       speculate = pc != SULI_PRECOND_LEARNED && pred > 3.0 * tol;
     }
-    if (use_pc) f.pass_iters[pidx] = f.ctl_h->it;
+    if (use_pc) pits[pidx] = f.ctl_h->it;
     const double rr_end = f.ctl_h->k2[0];
     const bool finite = rr_end == rr_end && rr_end < 1e300;
     if (finite) {
