@@ -96,7 +96,21 @@ enum {
                       the manufactured-solution forcing; zero until uploaded */
   SULI_TFV = 27,
   SULI_TFW = 28,
-  SULI_NFIELDS = 29
+  SULI_FU0 = 29,   /* fu0(nx+1,ny,nz): forces kept by the RK3 and AB2 schemes */
+  SULI_FV0 = 30,
+  SULI_FW0 = 31,
+  SULI_FU1 = 32,   /* fu1(nx+1,ny,nz): stage-2 forces of the RK3 scheme */
+  SULI_FV1 = 33,
+  SULI_FW1 = 34,
+  SULI_NFIELDS = 35
+};
+
+/* Time-integration schemes of suli_tempo (t_tempo, 0_modules.f90:24). */
+enum {
+  SULI_TIME_EULER = 0, /* one stage, u += dt F                           */
+  SULI_TIME_RK2 = 1,   /* stages dt/2, dt (default)                      */
+  SULI_TIME_RK3 = 2,   /* stages dt/2, dt, dt                            */
+  SULI_TIME_AB2 = 3    /* one stage, Adams-Bashforth 2 (Euler first)     */
 };
 
 typedef struct {
@@ -151,10 +165,28 @@ int suli_est_div(double *maxdiv);                  /* est, 4_plot.f90:500       
  * them implicit: the forces are fused into suli_tempo's update). */
 int suli_convdiff_forces(void);
 
-/* One RK stage (visco .. contorno(1)) and one whole time step
- * (1_principal.f90:74-95) with the RK2 stage steps dt/2, dt. */
+/* One stage (visco .. contorno(1)) and one whole time step
+ * (1_principal.f90:74-95) of the selected time scheme, stage steps a_dt(tt)
+ * of 2_iniciais.f90:85-102 (RK2: dt/2, dt; RK3: dt/2, dt, dt; Euler and
+ * AB2: dt).  suli_step fills ps[0 .. stages-1], one entry per stage
+ * (suli_time_stages: at most 3). */
 int suli_stage(int tt, double dt_base, suli_poisson_stats *st);
 int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps);
+
+/* Select the time scheme of later suli_tempo / suli_stage / suli_step calls
+ * (t_tempo of 0_modules.f90:24, a compile-time parameter in the reference;
+ * SULI_TIME_RK2 until set).  tempo (11_diversos.f90:10-228) per scheme:
+ *   EULER  u = u + dt F
+ *   RK2    tt 1: u0 = u, u = u + dt F; tt 2: u = u0 + dt F
+ *   RK3    tt 1: u0 = u, fu0 = F, u = u + dt F;
+ *          tt 2: fu1 = F, u = u0 - dt fu0 + 2 dt F;
+ *          tt 3: u = u0 + dt/6 (fu0 + 4 fu1 + F)
+ *   AB2    u = u + dt/2 (3 F - fu0), fu0 = F; the first suli_tempo after
+ *          this call is the Euler update (the reference's it = 1) and keeps F
+ *          in fu0.
+ * Selecting a scheme restarts the AB2 history. */
+int suli_set_time_scheme(int t_tempo);
+int suli_time_stages(void); /* ntt of the selected scheme (1, 2 or 3), -1 without a context */
 
 /* Time-dependent horizontal body force (sloshing tanks): convdiff uses
  * gx(t) = gx + amplitude*sin(2 pi t/period + phase) in place of the constant

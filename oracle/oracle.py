@@ -31,6 +31,7 @@ ARRAYS = [
     ("Fu", "FX"), ("Fv", "FY"), ("Fw", "FZ"),
     ("u0", "FX"), ("v0", "FY"), ("w0", "FZ"),
     ("ub", "U"), ("vb", "V"), ("wb", "W"),
+    ("fu0", "FX"), ("fv0", "FY"), ("fw0", "FZ"), ("fu1", "FX"), ("fv1", "FY"), ("fw1", "FZ"),
 ]
 SCALARS = ["dt", "t", "vol_ini", "vol_ins", "rho_f1", "rho_f2", "mi_f1", "mi_f2", "sigma", "alpha1"]
 
@@ -75,6 +76,7 @@ class Params:
     mi_f2: float = 1.0e-3
     sigma: float = 0.0728
     alpha1: float = 1.5
+    t_tempo: int = 1  # 0 Euler, 1 RK2, 2 RK3, 3 AB2 (0_modules.f90:24)
 
     @property
     def dx1(self) -> float:
@@ -96,7 +98,7 @@ class _OrState(C.Structure):
         ("rho_f1", C.c_double), ("rho_f2", C.c_double), ("mi_f1", C.c_double), ("mi_f2", C.c_double),
         ("sigma", C.c_double), ("alpha1", C.c_double),
         ("dt", C.c_double), ("t", C.c_double), ("vol_ini", C.c_double), ("vol_ins", C.c_double),
-        ("dt1", C.c_double), ("a_dt", C.c_double * 3), ("tt", C.c_int),
+        ("dt1", C.c_double), ("a_dt", C.c_double * 3), ("tt", C.c_int), ("t_tempo", C.c_int), ("it", C.c_int),
         ("cg_iters", C.c_int), ("cg_res2", C.c_double), ("reinit_iters", C.c_int),
         ("mass_corr_iters", C.c_int),
     ] + [(name, C.POINTER(C.c_double)) for name, _ in ARRAYS] + [
@@ -199,7 +201,7 @@ class State:
         p = self.p
         st = _OrState()
         for f in ["nx", "ny", "nz", "dx", "dy", "dz", "ccx0", "ccxf", "ccy0", "ccyf", "ccz0", "cczf",
-                  "m_turb", "csmag", "gx", "gz", "chezy"]:
+                  "m_turb", "csmag", "gx", "gz", "chezy", "t_tempo"]:
             setattr(st, f, getattr(p, f))
         for k in ["rho_f1", "rho_f2", "mi_f1", "mi_f2", "sigma", "alpha1", "dt", "t", "vol_ini", "vol_ins"]:
             setattr(st, k, self.scal[k])
@@ -216,10 +218,12 @@ class State:
         self.diag = dict(cg_iters=st.cg_iters, cg_res2=st.cg_res2, reinit_iters=st.reinit_iters,
                          mass_corr_iters=st.mass_corr_iters)
 
-    def run(self, cmd: str, arg: int = 0, tt: int = 1) -> "State":
-        """Run one oracle routine (same command names as the reference harness)."""
+    def run(self, cmd: str, arg: int = 0, tt: int = 1, it: int = 1) -> "State":
+        """Run one oracle routine (same command names as the reference harness);
+        `it` is the step counter that tempo sees (AB2 takes Euler at it = 1)."""
         st = self._struct()
         st.tt = tt
+        st.it = it
         L = lib()
         if cmd == "ls_ini":
             L.or_level_set_ini_tail(C.byref(st))
@@ -249,6 +253,7 @@ class State:
             st.dt = self.scal["dt"]
         elif cmd == "step":
             for it in range(1, max(arg, 1) + 1):
+                st.it = it
                 st.t = it * st.dt  # 1_principal.f90:65
                 L.or_step(C.byref(st), st.dt)
         else:
@@ -277,7 +282,7 @@ REF_DIR = os.path.join(HERE, "_ref")
 
 def ref_tag(p: Params) -> str:
     return (f"g{p.nx}x{p.ny}x{p.nz}_h{p.dx:g}_{p.dy:g}_{p.dz:g}_bc{p.ccx0}{p.ccxf}{p.ccy0}{p.ccyf}{p.ccz0}{p.cczf}"
-            f"_t{p.m_turb}_cs{p.csmag:g}_g{p.gx:g}_{p.gz:g}")
+            f"_t{p.m_turb}_cs{p.csmag:g}_g{p.gx:g}_{p.gz:g}" + (f"_tt{p.t_tempo}" if p.t_tempo != 1 else ""))
 
 
 def ref_build(p: Params, quiet: bool = True) -> str:
@@ -290,7 +295,7 @@ def ref_build(p: Params, quiet: bool = True) -> str:
     args = [os.path.join(HERE, "ref", "build_ref.sh"), tag, str(p.nx), str(p.ny), str(p.nz),
             repr(float(p.dx)), repr(float(p.dy)), repr(float(p.dz)),
             str(p.ccx0), str(p.ccxf), str(p.ccy0), str(p.ccyf), str(p.ccz0), str(p.cczf),
-            str(p.m_turb), repr(float(p.csmag)), repr(float(p.gx)), repr(float(p.gz))]
+            str(p.m_turb), repr(float(p.csmag)), repr(float(p.gx)), repr(float(p.gz)), str(p.t_tempo)]
     subprocess.run(args, check=True, stdout=subprocess.DEVNULL if quiet else None)
     return exe
 

@@ -44,17 +44,23 @@ program harness
 
   call io_state(fin, 0)
 
-  ! RK3-TVD coefficients and RK2 stage steps as set up by
-  ! 14_level_set.f90:34-44 and 2_iniciais.f90:432-437.
+  ! RK3-TVD coefficients and the stage steps of the time scheme t_tempo as
+  ! set up by 14_level_set.f90:34-44 and 2_iniciais.f90:85-102.
   adtl(1) = 1.;     bdtl(1) = 0.;     gdtl(1) = 1.
   adtl(2) = 3./4.;  bdtl(2) = 1./4.;  gdtl(2) = 1./4.
   adtl(3) = 1./3.;  bdtl(3) = 2./3.;  gdtl(3) = 2./3.
   ls_m = 0.
-  ntt = 2
   dt_base = dt
-  a_dt(1) = 0.5*dt_base
-  a_dt(2) = 1.0*dt_base
-  a_dt(3) = 1.0*dt_base
+  if ((t_tempo == 0) .or. (t_tempo == 3)) then
+    ntt = 1
+    a_dt = 1.0*dt_base
+  else
+    ntt = 2
+    if (t_tempo == 2) ntt = 3
+    a_dt(1) = 0.5*dt_base
+    a_dt(2) = 1.0*dt_base
+    a_dt(3) = 1.0*dt_base
+  endif
   tt = 1
   it = 1
 
@@ -214,7 +220,8 @@ contains
       rho_f1 = sc(5); rho_f2 = sc(6); mi_f1 = sc(7); mi_f2 = sc(8)
       sigma = sc(9); alpha1 = sc(10)
       read(21) u, v, w, prd1, rho, ls_nu, ls, mod_ls, kurv, hs, hsx, hsy, hsz, &
-               ddlsdx, ddlsdy, ddlsdz, nut, xnut, ynut, znut, Fu, Fv, Fw, u0, v0, w0, ub, vb, wb
+               ddlsdx, ddlsdy, ddlsdz, nut, xnut, ynut, znut, Fu, Fv, Fw, u0, v0, w0, ub, vb, wb, &
+               fu0, fv0, fw0, fu1, fv1, fw1
       close(21)
     else
       open(unit=22, file=trim(fname), access='stream', form='unformatted', status='replace')
@@ -223,7 +230,8 @@ contains
       write(22) hdr
       write(22) sc
       write(22) u, v, w, prd1, rho, ls_nu, ls, mod_ls, kurv, hs, hsx, hsy, hsz, &
-                ddlsdx, ddlsdy, ddlsdz, nut, xnut, ynut, znut, Fu, Fv, Fw, u0, v0, w0, ub, vb, wb
+                ddlsdx, ddlsdy, ddlsdz, nut, xnut, ynut, znut, Fu, Fv, Fw, u0, v0, w0, ub, vb, wb, &
+               fu0, fv0, fw0, fu1, fv1, fw1
       close(22)
     endif
   end subroutine io_state

@@ -38,8 +38,10 @@ int64_t or_state_len(int nx, int ny, int nz) {
   int64_t P = (int64_t)(nx + 2) * (ny + 2) * (nz + 2);
   int64_t C = (int64_t)nx * ny * nz;
   int64_t FX = (int64_t)(nx + 1) * ny * nz, FY = (int64_t)nx * (ny + 1) * nz, FZ = (int64_t)nx * ny * (nz + 1);
-  /* u v w prd1 | 12 cell arrays | nut | xnut ynut znut | Fu Fv Fw | u0 v0 w0 | ub vb wb */
-  return U + V + W + P + 12 * C + C + FX + FY + FZ + FX + FY + FZ + FX + FY + FZ + U + V + W;
+  /* u v w prd1 | 12 cell arrays | nut | xnut ynut znut | Fu Fv Fw | u0 v0 w0 | ub vb wb |
+     fu0 fv0 fw0 | fu1 fv1 fw1 */
+  return U + V + W + P + 12 * C + C + FX + FY + FZ + FX + FY + FZ + FX + FY + FZ + U + V + W +
+         2 * (FX + FY + FZ);
 }
 
 void or_bind(or_state *s, double *b) {
@@ -58,6 +60,8 @@ void or_bind(or_state *s, double *b) {
   s->Fu = b; b += FX; s->Fv = b; b += FY; s->Fw = b; b += FZ;
   s->u0 = b; b += FX; s->v0 = b; b += FY; s->w0 = b; b += FZ;
   s->ub = b; b += U; s->vb = b; b += V; s->wb = b; b += W;
+  s->fu0 = b; b += FX; s->fv0 = b; b += FY; s->fw0 = b; b += FZ;
+  s->fu1 = b; b += FX; s->fv1 = b; b += FY; s->fw1 = b; b += FZ;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -671,40 +675,66 @@ void or_convdiff(or_state *s) {
   free(ama); free(bmb); free(dmd);
 }
 
-/* tempo, RK2 branch, 11_diversos.f90:54-111 */
+/* tempo, 11_diversos.f90:10-228: the update of one face of one component
+ * (q the velocity, q0 the start-of-step copy, F the forces, f0/f1 the kept
+ * forces fu0/fu1), for every t_tempo scheme, in the reference's order of
+ * evaluation (left to right). */
+static void tempo_face(const or_state *s, double *q, double *q0, const double *F, double *f0, double *f1) {
+  const double dt = s->dt;
+  if (s->t_tempo == 0 || (s->t_tempo == 3 && s->it == 1)) { /* :19-52 */
+    *q = *q + dt * *F;
+    if (s->t_tempo == 3) *f0 = *F;
+  } else if (s->t_tempo == 1) { /* :53-111 */
+    const double aux1 = dt;
+    if (s->tt == 1) {
+      *q0 = *q;
+      *q = *q + aux1 * *F;
+    } else if (s->tt == 2) {
+      *q = *q0 + aux1 * *F;
+    }
+  } else if (s->t_tempo == 2) { /* :112-198 */
+    if (s->tt == 1) {
+      const double aux1 = dt;
+      *q0 = *q;
+      *f0 = *F;
+      *q = *q + aux1 * *F;
+    } else if (s->tt == 2) {
+      const double aux1 = 2. * dt;
+      *f1 = *F;
+      *q = *q0 - dt * *f0 + aux1 * *F;
+    } else if (s->tt == 3) {
+      const double aux1 = dt / 6.;
+      *q = *q0 + aux1 * (*f0 + 4. * *f1 + *F);
+    }
+  } else if (s->t_tempo == 3) { /* it /= 1, :199-226 */
+    *q = *q + dt * 0.5 * (3 * *F - *f0);
+    *f0 = *F;
+  }
+}
+
 void or_tempo(or_state *s) {
   DIMS;
-  const double aux1 = s->dt;
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
-      for (int i = 1; i <= nx + 1; i++) {
-        if (s->tt == 1) {
-          s->u0[IFX(i, j, k)] = s->u[IU(i, j, k)];
-          s->u[IU(i, j, k)] = s->u[IU(i, j, k)] + aux1 * s->Fu[IFX(i, j, k)];
-        } else {
-          s->u[IU(i, j, k)] = s->u0[IFX(i, j, k)] + aux1 * s->Fu[IFX(i, j, k)];
-        }
-      }
+      for (int i = 1; i <= nx + 1; i++)
+        tempo_face(s, &s->u[IU(i, j, k)], &s->u0[IFX(i, j, k)], &s->Fu[IFX(i, j, k)], &s->fu0[IFX(i, j, k)],
+                   &s->fu1[IFX(i, j, k)]);
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny + 1; j++)
-      for (int i = 1; i <= nx; i++) {
-        if (s->tt == 1) {
-          s->v0[IFY(i, j, k)] = s->v[IV(i, j, k)];
-          s->v[IV(i, j, k)] = s->v[IV(i, j, k)] + aux1 * s->Fv[IFY(i, j, k)];
-        } else {
-          s->v[IV(i, j, k)] = s->v0[IFY(i, j, k)] + aux1 * s->Fv[IFY(i, j, k)];
-        }
-      }
+      for (int i = 1; i <= nx; i++)
+        tempo_face(s, &s->v[IV(i, j, k)], &s->v0[IFY(i, j, k)], &s->Fv[IFY(i, j, k)], &s->fv0[IFY(i, j, k)],
+                   &s->fv1[IFY(i, j, k)]);
   for (int k = 1; k <= nz + 1; k++)
     for (int j = 1; j <= ny; j++)
-      for (int i = 1; i <= nx; i++) {
-        if (s->tt == 1) {
-          s->w0[IFZ(i, j, k)] = s->w[IW(i, j, k)];
-          s->w[IW(i, j, k)] = s->w[IW(i, j, k)] + aux1 * s->Fw[IFZ(i, j, k)];
-        } else {
-          s->w[IW(i, j, k)] = s->w0[IFZ(i, j, k)] + aux1 * s->Fw[IFZ(i, j, k)];
-        }
-      }
+      for (int i = 1; i <= nx; i++)
+        tempo_face(s, &s->w[IW(i, j, k)], &s->w0[IFZ(i, j, k)], &s->Fw[IFZ(i, j, k)], &s->fw0[IFZ(i, j, k)],
+                   &s->fw1[IFZ(i, j, k)]);
+}
+
+/* number of stages and their steps a_dt, 2_iniciais.f90:85-102 */
+static int or_ntt(int t_tempo) { return t_tempo == 2 ? 3 : t_tempo == 1 ? 2 : 1; }
+static double or_a_dt(int t_tempo, int tt, double dt) {
+  return (or_ntt(t_tempo) > 1 && tt == 1) ? 0.5 * dt : 1.0 * dt;
 }
 
 /* contorno, 3_contorno.f90:29-443 with prd_corr (:872-950, every dp ≡ 0);
@@ -1010,7 +1040,7 @@ double or_est_div(const or_state *s) {
 /* one RK2 stage of 1_principal.f90:77-95 */
 void or_stage(or_state *s, int tt, double dt_base) {
   s->tt = tt;
-  s->dt = (tt == 1) ? 0.5 * dt_base : 1.0 * dt_base;
+  s->dt = or_a_dt(s->t_tempo, tt, dt_base);
   or_visco(s);
   or_convdiff(s);
   or_tempo(s);
@@ -1025,6 +1055,6 @@ void or_step(or_state *s, double dt_base) {
   s->dt = dt_base;
   or_level_set(s);
   or_contorno(s, 3);
-  for (int tt = 1; tt <= 2; tt++) or_stage(s, tt, dt_base);
+  for (int tt = 1; tt <= or_ntt(s->t_tempo); tt++) or_stage(s, tt, dt_base);
   s->dt = dt_base;
 }

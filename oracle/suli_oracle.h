@@ -30,6 +30,8 @@ typedef struct {
   double dt1;          /* pseudo time step of the reinitialisation, 0.1*max(h) */
   double a_dt[3];
   int tt;
+  int t_tempo;         /* time scheme, 0_modules.f90:24: 0 Euler, 1 RK2, 2 RK3, 3 AB2 */
+  int it;              /* step counter of 1_principal.f90:64 (AB2 takes Euler at it = 1) */
   /* diagnostics written by the oracle */
   int cg_iters;        /* iterations of the last graddin call */
   double cg_res2;      /* final sum of squared scaled residuals */
@@ -39,6 +41,7 @@ typedef struct {
   double *u, *v, *w, *prd1, *rho, *ls_nu, *ls, *mod_ls, *kurv, *hs, *hsx, *hsy, *hsz;
   double *ddlsdx, *ddlsdy, *ddlsdz, *nut, *xnut, *ynut, *znut, *Fu, *Fv, *Fw;
   double *u0, *v0, *w0, *ub, *vb, *wb;
+  double *fu0, *fv0, *fw0, *fu1, *fv1, *fw1;  /* kept forces of RK3 / AB2 */
   /* optional momentum sources (manufactured solutions); NULL means zero */
   double *tf_u, *tf_v, *tf_w;
 } or_state;

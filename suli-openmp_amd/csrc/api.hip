@@ -146,7 +146,8 @@ extern "C" int suli_init(const suli_params *p) {
   void **tf[] = {&c->u, &c->v, &c->w, &c->un, &c->vn, &c->wn, &c->u0, &c->v0, &c->w0, &c->ls, &c->lsA,
                  &c->lsB, &c->S, &c->hs, &c->hsx, &c->hsy, &c->hsz, &c->rho, &c->lsnu, &c->modls,
                  &c->kurv, &c->nut, &c->nrx, &c->nry, &c->nrz, &c->ddx, &c->ddy, &c->ddz, &c->Fu,
-                 &c->Fv, &c->Fw, &c->tfu, &c->tfv, &c->tfw};
+                 &c->Fv, &c->Fw, &c->tfu, &c->tfv, &c->tfw, &c->fu0, &c->fv0, &c->fw0, &c->fu1,
+                 &c->fv1, &c->fw1};
   for (void **q : tf)
     if (dalloc(*c, q, fb)) return 1;
   double **td[] = {&c->prd1, &c->D, &c->sD, &c->q, &c->ax, &c->ay, &c->az, &c->e, &c->r, &c->pp, &c->mp};
@@ -314,6 +315,12 @@ static bool field_ref(Ctx &c, int f, FieldRef &r) {
     case SULI_TFU: r = {K_FX, c.tfu, false, false}; return true;
     case SULI_TFV: r = {K_FY, c.tfv, false, false}; return true;
     case SULI_TFW: r = {K_FZ, c.tfw, false, false}; return true;
+    case SULI_FU0: r = {K_FX, c.fu0, false, false}; return true;
+    case SULI_FV0: r = {K_FY, c.fv0, false, false}; return true;
+    case SULI_FW0: r = {K_FZ, c.fw0, false, false}; return true;
+    case SULI_FU1: r = {K_FX, c.fu1, false, false}; return true;
+    case SULI_FV1: r = {K_FY, c.fv1, false, false}; return true;
+    case SULI_FW1: r = {K_FZ, c.fw1, false, false}; return true;
     default: return false;
   }
 }
@@ -573,6 +580,7 @@ static MomPtrs mom_ptrs(Ctx &c) {
   m.tfw = c.tf_on ? c.tfw : nullptr;
   m.un = c.un; m.vn = c.vn; m.wn = c.wn; m.u0o = c.u0; m.v0o = c.v0; m.w0o = c.w0;
   m.Fu = nullptr; m.Fv = nullptr; m.Fw = nullptr;
+  m.fu0 = m.fv0 = m.fw0 = m.fu1 = m.fv1 = m.fw1 = nullptr;
   return m;
 }
 
@@ -582,9 +590,13 @@ extern "C" int suli_convdiff_forces(void) {
   MomPtrs m = mom_ptrs(c);
   m.u0 = c.u; m.v0 = c.v; m.w0 = c.w;  // stage-2 form: writes only the scratch swap buffers
   m.Fu = c.Fu; m.Fv = c.Fv; m.Fw = c.Fw;
-  { ProfScope ps_(c, "convdiff_tempo"); DISPATCH(c, Launch<TT>::momentum(c.g, c.phys_now(), c.st, m, 0.0, 2)); }
+  { ProfScope ps_(c, "convdiff_tempo"); DISPATCH(c, Launch<TT>::momentum(c.g, c.phys_now(), c.st, m, 0.0, MOM_FROM_U0)); }
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+static int time_stages(int t_tempo) {  // ntt, 2_iniciais.f90:85-102
+  return t_tempo == SULI_TIME_RK3 ? 3 : t_tempo == SULI_TIME_RK2 ? 2 : 1;
 }
 
 extern "C" int suli_tempo(int tt, double dt) {
@@ -594,19 +606,56 @@ extern "C" int suli_tempo(int tt, double dt) {
     suli_set_error("suli_tempo: needs a preceding suli_convdiff (the forces are fused into the update)");
     return 1;
   }
-  if (tt != 1 && tt != 2) {
-    suli_set_error("suli_tempo: RK2 stage must be 1 or 2");
+  const int ntt = time_stages(c.t_tempo);
+  if (tt < 1 || tt > ntt) {
+    suli_set_error("suli_tempo: stage " + std::to_string(tt) + " outside 1.." + std::to_string(ntt) +
+                   " of the selected time scheme");
     return 1;
   }
   MomPtrs m = mom_ptrs(c);
-  { ProfScope ps_(c, "convdiff_tempo"); DISPATCH(c, Launch<TT>::momentum(c.g, c.phys_now(), c.st, m, dt, tt)); }
+  int mode = MOM_EULER;
+  switch (c.t_tempo) {  // 11_diversos.f90:19-227
+    case SULI_TIME_RK2: mode = tt == 1 ? MOM_RK_FIRST : MOM_FROM_U0; break;
+    case SULI_TIME_RK3:
+      mode = tt == 1 ? MOM_RK_FIRST : tt == 2 ? MOM_RK3_2 : MOM_RK3_3;
+      m.fu0 = c.fu0; m.fv0 = c.fv0; m.fw0 = c.fw0;
+      m.fu1 = c.fu1; m.fv1 = c.fv1; m.fw1 = c.fw1;
+      break;
+    case SULI_TIME_AB2:
+      mode = c.ab2_started ? MOM_AB2 : MOM_EULER;
+      m.fu0 = c.fu0; m.fv0 = c.fv0; m.fw0 = c.fw0;
+      break;
+    default: break;
+  }
+  { ProfScope ps_(c, "convdiff_tempo"); DISPATCH(c, Launch<TT>::momentum(c.g, c.phys_now(), c.st, m, dt, mode)); }
   HIPCHK(hipGetLastError());
   std::swap(c.u, c.un);
   std::swap(c.v, c.vn);
   std::swap(c.w, c.wn);
   c.convdiff_pending = false;
-  c.stage = tt;
+  if (c.t_tempo == SULI_TIME_AB2) c.ab2_started = true;
+  // kind of stage for the projection's initial guess: first stage of a
+  // multi-stage step (dt/2), last stage of the step (dt), other
+  c.stage = tt == ntt ? 2 : tt == 1 ? 1 : 0;
   return 0;
+}
+
+extern "C" int suli_set_time_scheme(int t_tempo) {
+  NEEDCTX();
+  if (t_tempo < SULI_TIME_EULER || t_tempo > SULI_TIME_AB2) {
+    suli_set_error("suli_set_time_scheme: t_tempo must be 0 (Euler), 1 (RK2), 2 (RK3) or 3 (AB2)");
+    return 1;
+  }
+  Ctx &c = *g_ctx;
+  c.t_tempo = t_tempo;
+  c.ab2_started = false;
+  c.p2n = 0;  // the pressure history belongs to the previous scheme's stages
+  return 0;
+}
+
+extern "C" int suli_time_stages(void) {
+  if (!g_ctx) return -1;
+  return time_stages(g_ctx->t_tempo);
 }
 
 extern "C" int suli_contorno(int nlock) {
@@ -735,7 +784,8 @@ extern "C" int suli_est_div(double *maxdiv) {
 
 extern "C" int suli_stage(int tt, double dt_base, suli_poisson_stats *st) {
   NEEDCTX();
-  const double dt = tt == 1 ? 0.5 * dt_base : 1.0 * dt_base;  // a_dt, 2_iniciais.f90:432-437
+  // a_dt(tt), 2_iniciais.f90:85-102: dt/2 for the first of several stages
+  const double dt = tt == 1 && time_stages(g_ctx->t_tempo) > 1 ? 0.5 * dt_base : 1.0 * dt_base;
   int rc;
   if ((rc = suli_visco())) return rc;
   if ((rc = suli_convdiff())) return rc;
@@ -772,12 +822,11 @@ extern "C" int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps) {
   int rc;
   if ((rc = suli_level_set(dt, ls))) return rc;
   if ((rc = suli_contorno(3))) return rc;
-  suli_poisson_stats a{}, b{};
-  if ((rc = suli_stage(1, dt, &a))) return rc;
-  if ((rc = suli_stage(2, dt, &b))) return rc;
-  if (ps) {
-    ps[0] = a;
-    ps[1] = b;
+  const int ntt = time_stages(g_ctx->t_tempo);
+  for (int tt = 1; tt <= ntt; tt++) {
+    suli_poisson_stats a{};
+    if ((rc = suli_stage(tt, dt, &a))) return rc;
+    if (ps) ps[tt - 1] = a;
   }
   return 0;
 }
@@ -790,8 +839,9 @@ extern "C" int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps) {
 static const int kCkptFields[] = {SULI_U,      SULI_V,      SULI_W,      SULI_PRD1,   SULI_RHO,  SULI_LS_NU,
                                   SULI_LS,     SULI_MOD_LS, SULI_KURV,   SULI_HS,     SULI_HSX,  SULI_HSY,
                                   SULI_HSZ,    SULI_DDLSDX, SULI_DDLSDY, SULI_DDLSDZ, SULI_NUT,  SULI_U0,
-                                  SULI_V0,     SULI_W0,     SULI_TFU,    SULI_TFV,    SULI_TFW};
-static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '1'};
+                                  SULI_V0,     SULI_W0,     SULI_TFU,    SULI_TFV,    SULI_TFW,  SULI_FU0,
+                                  SULI_FV0,    SULI_FW0};
+static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '2'};
 
 struct CkptHeader {
   char magic[8];
@@ -799,8 +849,9 @@ struct CkptHeader {
   int64_t frame_total;
   int32_t nfields, tf_on;
   double vol_ini, vol_ins, t_now, force_amp, force_period, force_phase;
-  int32_t guess, stage, p2n, pad;
+  int32_t guess, stage, p2n, t_tempo;
   double p2dt;
+  int32_t ab2_started, pad;
 };
 
 // Binary checkpoint: the device frames of the state fields as they are (the
@@ -822,8 +873,9 @@ extern "C" int suli_checkpoint_save(const char *path) {
   h.tf_on = c.tf_on ? 1 : 0;
   h.vol_ini = c.vol_ini; h.vol_ins = c.vol_ins; h.t_now = c.t_now;
   h.force_amp = c.force_amp; h.force_period = c.force_period; h.force_phase = c.force_phase;
-  h.guess = c.guess; h.stage = c.stage; h.p2n = c.p2n; h.pad = 0;
+  h.guess = c.guess; h.stage = c.stage; h.p2n = c.p2n; h.t_tempo = c.t_tempo;
   h.p2dt = c.p2dt;
+  h.ab2_started = c.ab2_started ? 1 : 0; h.pad = 0;
   HIPCHK(hipStreamSynchronize(c.st));
   FILE *f = std::fopen(path, "wb");
   if (!f) {
@@ -873,6 +925,7 @@ extern "C" int suli_checkpoint_load(const char *path) {
   CkptHeader h{};
   std::string err;
   if (std::fread(&h, sizeof h, 1, f) != 1 || std::memcmp(h.magic, kCkptMagic, 8) != 0) err = "not a checkpoint";
+  else if (h.t_tempo < SULI_TIME_EULER || h.t_tempo > SULI_TIME_AB2) err = "unknown time scheme";
   else if (h.nx != c.g.nx || h.ny != c.g.ny || h.nz != c.g.nz || h.prec != c.prec || h.frame_total != c.g.total)
     err = "grid " + std::to_string(h.nx) + "x" + std::to_string(h.ny) + "x" + std::to_string(h.nz) + " precision " +
           std::to_string(h.prec) + " differs from this context's";
@@ -912,6 +965,7 @@ extern "C" int suli_checkpoint_load(const char *path) {
   c.vol_ini = h.vol_ini; c.vol_ins = h.vol_ins; c.t_now = h.t_now;
   c.force_amp = h.force_amp; c.force_period = h.force_period; c.force_phase = h.force_phase;
   c.guess = h.guess; c.stage = h.stage; c.p2n = std::min(h.p2n, 3); c.p2dt = h.p2dt;
+  c.t_tempo = h.t_tempo; c.ab2_started = h.ab2_started != 0;
   c.convdiff_pending = false;
   return 0;
 }
@@ -1088,5 +1142,6 @@ extern "C" int suli_restart_read(const char *path) {
   c.vol_ins = vol[1];
   c.convdiff_pending = false;
   c.p2n = 0;
+  c.ab2_started = false;  // the reference restarts its step counter: it = 1 takes the Euler update
   return 0;
 }

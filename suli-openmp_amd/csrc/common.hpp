@@ -22,6 +22,16 @@
 // offset of an element in a frame field
 typedef uint32_t fidx;
 
+// Stage updates of tempo (11_diversos.f90:10-228) done by the momentum kernel.
+enum MomUpdate {
+  MOM_RK_FIRST = 1,  // RK2/RK3 tt=1: u0 = u; [fu0 = F]; u = u + dt F
+  MOM_FROM_U0 = 2,   // RK2 tt=2: u = u0 + dt F
+  MOM_EULER = 3,     // Euler and the first AB2 step: u = u + dt F; [fu0 = F]
+  MOM_RK3_2 = 4,     // fu1 = F; u = u0 - dt fu0 + (2 dt) F
+  MOM_RK3_3 = 5,     // u = u0 + (dt/6) (fu0 + 4 fu1 + F)
+  MOM_AB2 = 6        // u = u + dt 0.5 (3 F - fu0); fu0 = F
+};
+
 struct Grid {
   int nx, ny, nz;
   int sx, sy;        // pitches in i and j

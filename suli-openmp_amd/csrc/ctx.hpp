@@ -62,6 +62,10 @@ struct Ctx {
   void *modls = nullptr, *kurv = nullptr, *nut = nullptr, *nrx = nullptr, *nry = nullptr, *nrz = nullptr;
   void *ddx = nullptr, *ddy = nullptr, *ddz = nullptr, *Fu = nullptr, *Fv = nullptr, *Fw = nullptr;
   void *tfu = nullptr, *tfv = nullptr, *tfw = nullptr;  // momentum sources tf_u, tf_v, tf_w
+  // kept forces of the RK3 and AB2 schemes (fu0, fu1 of 11_diversos.f90)
+  void *fu0 = nullptr, *fv0 = nullptr, *fw0 = nullptr, *fu1 = nullptr, *fv1 = nullptr, *fw1 = nullptr;
+  int t_tempo = SULI_TIME_RK2;  // time scheme
+  bool ab2_started = false;     // fu0 holds the forces of the last AB2 step
   bool tf_on = false;                                    // set by the first upload of a source
   double *prd1 = nullptr;
   // projection (FP64)
@@ -74,9 +78,11 @@ struct Ctx {
   bool convdiff_pending = false;
   double vol_ini = 0, vol_ins = 0;
   PoissonFCG *fcg = nullptr;
-  // initial guess of the FCG projection: the RK stage set by the last tempo
-  // call; the last three stage-2 pressures p2h[0] (newest) .. p2h[2], p2n of
-  // them valid, all of the base step p2dt (guess = 1: extrapolate from them)
+  // initial guess of the FCG projection: the kind of stage set by the last
+  // tempo call (1: first stage of a multi-stage step, step dt/2; 2: last
+  // stage, step dt; 0: other); the last three last-stage pressures p2h[0]
+  // (newest) .. p2h[2], p2n of them valid, all of the base step p2dt
+  // (guess = 1: extrapolate from them)
   int guess = 1, stage = 0, p2n = 0;
   double p2dt = 0.0;
   double *p2h[3] = {nullptr, nullptr, nullptr};

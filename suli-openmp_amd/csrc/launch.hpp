@@ -10,6 +10,7 @@ struct MomPtrs {
   const void *tfu, *tfv, *tfw;
   void *un, *vn, *wn, *u0o, *v0o, *w0o;
   void *Fu, *Fv, *Fw;
+  void *fu0, *fv0, *fw0, *fu1, *fv1, *fw1;
 };
 
 // outputs of the FCG face set-up (k_fcg_faces)
@@ -35,8 +36,9 @@ struct Launch {
   static void masscorr(const Grid &g, hipStream_t s, void *phi, const void *modls, double f, double vol0);
   static void visco(const Grid &g, const Phys &ph, hipStream_t s, const void *u, const void *v,
                     const void *w, const void *rho, void *nut);
+  // mode: MomUpdate (stencils.hpp)
   static void momentum(const Grid &g, const Phys &ph, hipStream_t s, const MomPtrs &m, double dt,
-                       int stage);
+                       int mode);
   static void contorno(const Grid &g, hipStream_t s, int nlock, void *u, void *v, void *w, double *p);
   static void posdin(const Grid &g, hipStream_t s, double dt, const double *p, const void *rho,
                      void *u, void *v, void *w);

@@ -226,8 +226,8 @@ __global__ __launch_bounds__(256) void k_visco(Grid g, T c23, T cs, const T *__r
 
 // ---------------------------------------------------------------------------
 // Momentum: convdiff (6_convdiff.f90:14-189) with first-order upwind
-// advection classico der=1 (7_advectivo.f90:33-113), fused with the RK2
-// stage update tempo (11_diversos.f90:54-111).  The face viscosities
+// advection classico der=1 (7_advectivo.f90:33-113), fused with the stage
+// update of tempo (11_diversos.f90:10-228, every t_tempo scheme).  The face viscosities
 // xnut/ynut/znut (5_viscosidade.f90:178-211) and their edge averages
 // (6_convdiff.f90:118-145) are rebuilt on the fly from the cell arrays nut
 // and ls_nu with the reference's end-face copies.
@@ -244,9 +244,42 @@ struct MomArgs {
   const T *tfu, *tfv, *tfw;  // optional source terms (may be null)
   T *un, *vn, *wn, *u0o, *v0o, *w0o;
   T *Fu, *Fv, *Fw;           // optional force output (may be null)
+  T *fu0, *fv0, *fw0;        // kept forces of RK3 stage 1 / AB2 (null when unused)
+  T *fu1, *fv1, *fw1;        // RK3 stage-2 forces (null when unused)
   T dt;
-  int stage;                 // 1 or 2
+  int mode;                  // MomUpdate
 };
+
+// The velocity update of one stage of tempo (11_diversos.f90:10-228), per
+// face: uc the velocity the forces F were evaluated at, u0 the start-of-step
+// copy, fa/fb the kept forces (fu0/fu1).  Fortran evaluation order, so that
+// the FP64 instantiation rounds as the reference does.
+template <typename T>
+__device__ __forceinline__ void stage_update(int mode, T dt, fidx c, T uc, T F, const T *__restrict__ u0,
+                                             T *__restrict__ u0o, T *__restrict__ un, T *fa, T *fb) {
+  switch (mode) {
+    case MOM_RK_FIRST:
+      u0o[c] = uc;
+      if (fa) fa[c] = F;
+      un[c] = uc + dt * F;
+      break;
+    case MOM_FROM_U0: un[c] = u0[c] + dt * F; break;
+    case MOM_EULER:
+      if (fa) fa[c] = F;
+      un[c] = uc + dt * F;
+      break;
+    case MOM_RK3_2:
+      fb[c] = F;
+      un[c] = u0[c] - dt * fa[c] + (T(2.) * dt) * F;
+      break;
+    case MOM_RK3_3: un[c] = u0[c] + (dt / T(6.)) * (fa[c] + T(4.) * fb[c] + F); break;
+    default: {  // MOM_AB2
+      const T f0 = fa[c];
+      fa[c] = F;
+      un[c] = uc + dt * T(0.5) * (T(3.) * F - f0);
+    }
+  }
+}
 
 template <typename T>
 __device__ __forceinline__ T interp_face(const T *__restrict__ a, fidx cf, fidx st, int f,
@@ -332,12 +365,7 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
                       rhox +
                   T(ph.gx) - sigma * dhsdx + tf - fr * sqrt(ub * ub) * uc;
       if (A.Fu) A.Fu[c] = F;
-      if (A.stage == 1) {
-        A.u0o[c] = uc;
-        A.un[c] = uc + A.dt * F;
-      } else {
-        A.un[c] = A.u0[c] + A.dt * F;
-      }
+      stage_update<T>(A.mode, A.dt, c, uc, F, A.u0, A.u0o, A.un, A.fu0, A.fu1);
     } else {
       A.un[c] = u[c];
     }
@@ -377,12 +405,7 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
                       rhoy -
                   sigma * dhsdy + tf - fr * sqrt(vb * vb) * vc;
       if (A.Fv) A.Fv[c] = F;
-      if (A.stage == 1) {
-        A.v0o[c] = vc;
-        A.vn[c] = vc + A.dt * F;
-      } else {
-        A.vn[c] = A.v0[c] + A.dt * F;
-      }
+      stage_update<T>(A.mode, A.dt, c, vc, F, A.v0, A.v0o, A.vn, A.fv0, A.fv1);
     } else {
       A.vn[c] = v[c];
     }
@@ -423,12 +446,7 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
                       rhoz -
                   (wc - T(0.)) * epis_z - T(ph.gz) - sigma * dhsdz + tf - fr * sqrt(wb * wb) * wc;
       if (A.Fw) A.Fw[c] = F;
-      if (A.stage == 1) {
-        A.w0o[c] = wc;
-        A.wn[c] = wc + A.dt * F;
-      } else {
-        A.wn[c] = A.w0[c] + A.dt * F;
-      }
+      stage_update<T>(A.mode, A.dt, c, wc, F, A.w0, A.w0o, A.wn, A.fw0, A.fw1);
     } else {
       A.wn[c] = w[c];
     }

@@ -61,7 +61,7 @@ void Launch<T>::visco(const Grid &g, const Phys &ph, hipStream_t s, const void *
 
 template <typename T>
 void Launch<T>::momentum(const Grid &g, const Phys &ph, hipStream_t s, const MomPtrs &m, double dt,
-                         int stage) {
+                         int mode) {
   MomArgs<T> a;
   a.u = (const T *)m.u; a.v = (const T *)m.v; a.w = (const T *)m.w;
   a.u0 = (const T *)m.u0; a.v0 = (const T *)m.v0; a.w0 = (const T *)m.w0;
@@ -72,8 +72,10 @@ void Launch<T>::momentum(const Grid &g, const Phys &ph, hipStream_t s, const Mom
   a.un = (T *)m.un; a.vn = (T *)m.vn; a.wn = (T *)m.wn;
   a.u0o = (T *)m.u0o; a.v0o = (T *)m.v0o; a.w0o = (T *)m.w0o;
   a.Fu = (T *)m.Fu; a.Fv = (T *)m.Fv; a.Fw = (T *)m.Fw;
+  a.fu0 = (T *)m.fu0; a.fv0 = (T *)m.fv0; a.fw0 = (T *)m.fw0;
+  a.fu1 = (T *)m.fu1; a.fv1 = (T *)m.fv1; a.fw1 = (T *)m.fw1;
   a.dt = (T)dt;
-  a.stage = stage;
+  a.mode = mode;
   dim3 grid((g.nx + 3 + SULI_BX - 1) / SULI_BX, (g.ny + 3 + SULI_BY - 1) / SULI_BY, g.nz + 3);
   SULI_LAUNCH(k_momentum<T>, grid, blk2(), s, g, ph, a);
 }

@@ -2,7 +2,7 @@
 ! PROGRAM PNH (1_principal.f90:64-108) with its CALL lines unchanged; the
 ! subroutines come from module suli_gpu and run on the GPU.
 !
-!   principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME]]]
+!   principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME [T_TEMPO]]]]
 !
 ! Tank 1.0 x 0.5 x 0.5 m scaled to the grid (h = 1/NX), dam break of a water
 ! column WIDTH x HEIGHT in the x = 0 corner, water/air 1000/1.2 kg/m^3, no-slip
@@ -12,7 +12,8 @@
 ! start (plot_i), one record every FRAME steps (plot_f, 4_plot.f90:260) and
 ! arquivos/visu.xdmf at the end (visu); every frame the velocities, pressure
 ! and level set are also copied back into the host arrays u, v, w, prd1, ls
-! with the reference extents (0_modules.f90:92-122).
+! with the reference extents (0_modules.f90:92-122).  T_TEMPO selects the
+! time scheme (0_modules.f90:24): 0 Euler, 1 RK2 (default), 2 RK3, 3 AB2.
 program principal_gpu
   use suli_gpu
   implicit none
@@ -24,7 +25,7 @@ program principal_gpu
 
   nargs = command_argument_count()
   if (nargs < 4) then
-    write(*,*) 'usage: principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME]]]'
+    write(*,*) 'usage: principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME [T_TEMPO]]]]'
     stop 2
   endif
   call get_command_argument(1, arg); read(arg, *) nx
@@ -45,6 +46,9 @@ program principal_gpu
   if (nargs >= 8) then
     call get_command_argument(8, arg); read(arg, *) frame
   endif
+  if (nargs >= 9) then
+    call get_command_argument(9, arg); read(arg, *) t_tempo
+  endif
   h = 1.0d0 / nx
 
   ! iniciais (2_iniciais.f90:10-106) and level_set_ini (14_level_set.f90:10-219)
@@ -60,10 +64,7 @@ program principal_gpu
   call check(c_suli_init(prm), 'init')
   call check(c_suli_init_dambreak(width, height, vol_ini), 'init_dambreak')
   dt = 0.2d0 * h / (2.0d0 * sqrt(prm%gz * height))
-  ntt = 2
-  a_dt(1) = 0.5d0 * dt
-  a_dt(2) = 1.0d0 * dt
-  a_dt(3) = 1.0d0 * dt
+  call tempo_ini()
   allocate(u(0:nx+2,0:ny+1,0:nz+1), v(0:nx+1,0:ny+2,0:nz+1), w(0:nx+1,0:ny+1,0:nz+2))
   allocate(prd1(0:nx+1,0:ny+1,0:nz+1), ls(nx,ny,nz))
 

@@ -102,6 +102,10 @@ module suli_gpu
     integer(c_int) function c_suli_sync() bind(C, name='suli_sync')
       import :: c_int
     end function
+    integer(c_int) function c_suli_set_time_scheme(t_tempo) bind(C, name='suli_set_time_scheme')
+      import :: c_int
+      integer(c_int), value :: t_tempo
+    end function
     integer(c_int) function c_suli_set_time(t) bind(C, name='suli_set_time')
       import :: c_int, c_double
       real(c_double), value :: t
@@ -143,6 +147,8 @@ module suli_gpu
   ! field output (module tempo, 0_modules.f90:155; t_plot, 0_modules.f90:22)
   integer(8) :: cont = 10
   integer :: t_plot = 1
+  ! time scheme (0_modules.f90:24): 0 Euler, 1 RK2, 2 RK3, 3 AB2
+  integer :: t_tempo = 1
 
 contains
 
@@ -183,6 +189,22 @@ contains
     call check(c_suli_set_time(t), 'set_time')
     call check(c_suli_convdiff(), 'convdiff')
   end subroutine convdiff
+
+  ! The time-scheme part of iniciais (2_iniciais.f90:85-102): stage count
+  ! ntt and stage steps a_dt from dt, and the scheme of the library's tempo.
+  subroutine tempo_ini()
+    if ((t_tempo == 0) .or. (t_tempo == 3)) then
+      ntt = 1
+      a_dt = 1.0d0 * dt
+    else
+      ntt = 2
+      if (t_tempo == 2) ntt = 3
+      a_dt(1) = 0.5d0 * dt
+      a_dt(2) = 1.0d0 * dt
+      a_dt(3) = 1.0d0 * dt
+    endif
+    call check(c_suli_set_time_scheme(int(t_tempo, c_int)), 'set_time_scheme')
+  end subroutine tempo_ini
 
   subroutine tempo()
     call check(c_suli_tempo(int(tt, c_int), dt), 'tempo')

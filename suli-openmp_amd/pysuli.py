@@ -23,11 +23,12 @@ PRECOND_JACOBI, PRECOND_GMG, PRECOND_LEARNED = 0, 1, 2
 
 FIELDS = ["u", "v", "w", "prd1", "rho", "ls_nu", "ls", "mod_ls", "kurv", "hs", "hsx", "hsy", "hsz",
           "ddlsdx", "ddlsdy", "ddlsdz", "nut", "xnut", "ynut", "znut", "Fu", "Fv", "Fw", "u0", "v0", "w0",
-          "tf_u", "tf_v", "tf_w"]
+          "tf_u", "tf_v", "tf_w", "fu0", "fv0", "fw0", "fu1", "fv1", "fw1"]
 FIELD_ID = {n: q for q, n in enumerate(FIELDS)}
 KIND = {"u": "U", "v": "V", "w": "W", "prd1": "P", "xnut": "FX", "ynut": "FY", "znut": "FZ",
         "Fu": "FX", "Fv": "FY", "Fw": "FZ", "u0": "FX", "v0": "FY", "w0": "FZ", "tf_u": "FX", "tf_v": "FY",
-        "tf_w": "FZ"}
+        "tf_w": "FZ", "fu0": "FX", "fv0": "FY", "fw0": "FZ", "fu1": "FX", "fv1": "FY", "fw1": "FZ"}
+TIME_EULER, TIME_RK2, TIME_RK3, TIME_AB2 = 0, 1, 2, 3  # t_tempo, 0_modules.f90:24
 
 
 def extent_shape(kind: str, nx: int, ny: int, nz: int) -> tuple:
@@ -120,6 +121,7 @@ def lib():
             "suli_write_frame": ([C.c_char_p, i32], i32),
             "suli_checkpoint_save": ([C.c_char_p], i32), "suli_checkpoint_load": ([C.c_char_p], i32),
             "suli_set_guess": ([i32], i32),
+            "suli_set_time_scheme": ([i32], i32), "suli_time_stages": ([], i32),
             "suli_restart_write": ([C.c_char_p], i32), "suli_restart_read": ([C.c_char_p], i32),
             "suli_write_xdmf": ([C.c_char_p, i32, i32, i32, i32, i32, i32, f64], i32),
         }
@@ -241,9 +243,9 @@ class Solver:
 
     def step(self, dt):
         ls = LsStats()
-        ps = (PoissonStats * 2)()
+        ps = (PoissonStats * 3)()
         _check(lib().suli_step(dt, C.byref(ls), ps), "suli_step")
-        return ls.as_dict(), [ps[0].as_dict(), ps[1].as_dict()]
+        return ls.as_dict(), [ps[q].as_dict() for q in range(lib().suli_time_stages())]
 
     def sync(self):
         _check(lib().suli_sync(), "suli_sync")
@@ -353,6 +355,13 @@ class Solver:
 
     def set_guess(self, mode):
         _check(lib().suli_set_guess(mode), "suli_set_guess")
+
+    def set_time_scheme(self, t_tempo):
+        """Euler, RK2, RK3 or AB2 (t_tempo 0..3, 11_diversos.f90:10-228)."""
+        _check(lib().suli_set_time_scheme(t_tempo), "suli_set_time_scheme")
+
+    def time_stages(self):
+        return lib().suli_time_stages()
 
     def checkpoint_save(self, path):
         _check(lib().suli_checkpoint_save(os.fsencode(path)), "suli_checkpoint_save")

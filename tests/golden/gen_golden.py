@@ -6,7 +6,11 @@ set, seeded smooth random velocities and pressure), runs one reference
 subroutine through the harness, and stores the input state and every array
 the call changed in tests/golden/<case>.npz (float64, Fortran order).
 
-    python tests/golden/gen_golden.py
+    python tests/golden/gen_golden.py            # RK2 cases (g8_*, per_*)
+    python tests/golden/gen_golden.py schemes    # Euler, RK3, AB2 (g8tt*_*)
+
+The scheme cases use reference builds with t_tempo = 0, 2, 3
+(0_modules.f90:24) and start from nonzero kept forces fu0, fu1.
 """
 import os
 import sys
@@ -23,6 +27,8 @@ CASES = {
     "g8": O.Params(),
     "per": O.Params(nx=12, ny=8, nz=6, dx=1 / 12, dy=1 / 12, dz=1 / 12, ccx0=0, ccxf=0, cczf=1),
 }
+# time schemes other than RK2: t_tempo -> stages (ntt, 2_iniciais.f90:85-102)
+SCHEMES = {0: 1, 2: 3, 3: 1}
 COMMANDS = [("ls_ini", None), ("heaviside", None), ("mod_ls1", None), ("reinic", None), ("level_set", None),
             ("visco", None), ("convdiff", None), ("tempo", 1), ("tempo", 2), ("contorno", 1),
             ("contorno", 2), ("graddin", None), ("posdin", None), ("stage", 1), ("step", 2)]
@@ -37,6 +43,38 @@ def base_state(p, seed):
     s.v0[...] = 0.9 * s.v[1:-1, 1:-1, 1:-1]
     s.w0[...] = 0.9 * s.w[1:-1, 1:-1, 1:-1]
     return s
+
+
+def _save(path, p, s_in, s_out):
+    data = {"__scal_in": np.array([s_in.scal[k] for k in O.SCALARS]),
+            "__scal_out": np.array([s_out.scal[k] for k in O.SCALARS]),
+            "__params": np.array([p.nx, p.ny, p.nz, p.ccx0, p.ccxf, p.ccy0, p.ccyf, p.ccz0, p.cczf]),
+            "__h": np.array([p.dx, p.dy, p.dz]), "__tt": np.array([p.t_tempo])}
+    for name, _ in O.ARRAYS:
+        a, b = s_in.arr(name), s_out.arr(name)
+        data["in_" + name] = a
+        if not np.array_equal(a, b):
+            data["out_" + name] = b
+    np.savez_compressed(path, **data)
+    return sorted(k[4:] for k in data if k.startswith("out_"))
+
+
+def schemes():
+    import dataclasses
+    for tt_scheme, ntt in SCHEMES.items():
+        p = dataclasses.replace(O.Params(), t_tempo=tt_scheme)
+        O.ref_build(p)
+        cmds = [("tempo", t) for t in range(1, ntt + 1)] + [("stage", t) for t in range(1, ntt + 1)] + [("step", 2)]
+        for q, (cmd, arg) in enumerate(cmds):
+            s_in = base_state(p, seed=300 + 10 * tt_scheme + q)
+            rng = np.random.default_rng(1000 + q)
+            for n in ["fu0", "fv0", "fw0", "fu1", "fv1", "fw1"]:
+                s_in.arr(n)[...] = 0.1 * rng.standard_normal(s_in.arr(n).shape)
+            s_out = s_in.copy()
+            O.ref_run(s_out, cmd, arg)
+            tag = f"{cmd}{arg}"
+            changed = _save(os.path.join(HERE, f"g8tt{tt_scheme}_{tag}.npz"), p, s_in, s_out)
+            print(f"g8tt{tt_scheme}", tag, "changed:", changed)
 
 
 def main():
@@ -81,4 +119,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["schemes"]:
+        schemes()
+    else:
+        main()

@@ -26,9 +26,15 @@ def consistent_state(p, seed=1, **kw):
 
 
 def gpu_from_state(state, **kw):
+    """GPU solver with the state's parameters and fields; for a time scheme
+    other than RK2, the scheme and its kept forces fu0, fu1 as well."""
     import pysuli
     sv = pysuli.Solver.from_oracle_params(state.p, **kw)
     pysuli.load_state(sv, state)
+    if state.p.t_tempo != 1:
+        sv.set_time_scheme(state.p.t_tempo)
+        for n in ["fu0", "fv0", "fw0", "fu1", "fv1", "fw1"]:
+            sv.upload(n, state.arr(n))
     return sv
 
 

@@ -46,6 +46,25 @@ def test_fortran_driver_runs_the_time_loop(tmp_path):
     assert [g.get("Name") for g in grids] == ["campos_00010", "campos_00011", "campos_00012"]
 
 
+@pytest.mark.parametrize("t_tempo", [0, 2, 3])
+def test_fortran_driver_time_schemes(tmp_path, t_tempo):
+    """The driver with t_tempo = 0 (Euler), 2 (RK3), 3 (AB2): ntt stages per
+    step (2_iniciais.f90:85-102), volume and divergence as with RK2."""
+    exe = os.path.join(PKG, "principal_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("principal_gpu not built (flang absent)")
+    print(f"fortran driver 64x32x32, 10 steps, t_tempo {t_tempo}", flush=True)
+    out = subprocess.run([exe, "64", "32", "32", "10", "1", "0.25", "0.25", "0", str(t_tempo)],
+                         capture_output=True, text=True, timeout=240, cwd=str(tmp_path))
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    steps = re.findall(r"step\s+(\d+) t=\s*(\S+) vol_err=\s*(\S+) maxdiv=\s*(\S+) poisson_iters=\s*(\d+)", out.stdout)
+    assert len(steps) == 10
+    for it, t, vol_err, maxdiv, iters in steps:
+        assert abs(float(vol_err)) < 1e-3 and float(maxdiv) < 1e-3 and 0 < int(iters) < 50
+    m = re.search(r"poisson_iters/solve=\s*(\S+)", out.stdout)
+    assert m, out.stdout[-500:]
+
+
 def test_sloshing_forcing_drives_the_tank():
     """A flat free surface at rest stays at rest without forcing and is set
     in motion by gx(t); the volume is conserved either way."""

@@ -28,8 +28,9 @@ def fixture_state():
     p = O.Params(nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, ccx0=ccx0, ccxf=ccxf, ccy0=ccy0, ccyf=ccyf,
                  ccz0=ccz0, cczf=cczf)
     s = O.State(p)
-    for name, _ in O.ARRAYS:
-        s.arr(name)[...] = d["in_" + name]
+    for name, _ in O.ARRAYS:  # the fixture predates the RK3/AB2 arrays: those stay zero
+        if "in_" + name in d:
+            s.arr(name)[...] = d["in_" + name]
     s.scal = dict(zip(O.SCALARS, [float(x) for x in d["__scal_in"]]))
     return s, d
 
