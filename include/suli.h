@@ -48,7 +48,7 @@ typedef struct {
   /* wall codes (0_modules.f90:53-61): 0 periodic, 1 free slip, 2 no slip,
      4 outflow (x = xf only) */
   int32_t ccx0, ccxf, ccy0, ccyf, ccz0, cczf;
-  /* model switches: m_turb 0 none, 1 Smagorinsky (0_modules.f90:31) */
+  /* model switches: m_turb 0 none, 1 Smagorinsky, 2 directional Smagorinsky (0_modules.f90:31) */
   int32_t m_turb;
   /* fluid and model constants (14_level_set.f90:25-30, 0_modules.f90:141-143,181) */
   double rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1;
@@ -187,6 +187,17 @@ int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps);
  * Selecting a scheme restarts the AB2 history. */
 int suli_set_time_scheme(int t_tempo);
 int suli_time_stages(void); /* ntt of the selected scheme (1, 2 or 3), -1 without a context */
+
+/* Select the advective term of later convdiff calls (adv_type and der of
+ * 0_modules.f90:26-27, compile-time parameters in the reference; 1, 1 until
+ * set): adv_type 1 classico with der 1 upwind (7_advectivo.f90:33-113),
+ * 2 central (:116-173) or 3 second-order upwind (:176-291); adv_type 2
+ * rotacional (:296-440) and 3 antissim (:443-653), der 1 only (the
+ * reference stops on the other pairs).  The turbulence model is
+ * suli_params.m_turb: 0 none, 1 Smagorinsky, 2 directional Smagorinsky
+ * (5_viscosidade.f90:104-170: face viscosities xnut, ynut, znut stored and
+ * downloadable, nut left unchanged). */
+int suli_set_advection(int adv_type, int der);
 
 /* Time-dependent horizontal body force (sloshing tanks): convdiff uses
  * gx(t) = gx + amplitude*sin(2 pi t/period + phase) in place of the constant

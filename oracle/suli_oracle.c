@@ -502,6 +502,41 @@ void or_visco(or_state *s) {
     free(dudx_i); free(dudy_i); free(dudz_i); free(dvdx_i); free(dvdy_i); free(dvdz_i);
     free(dwdx_i); free(dwdy_i); free(dwdz_i);
     free(dudx); free(dudy); free(dudz); free(dvdx); free(dvdy); free(dvdz); free(dwdx); free(dwdy); free(dwdz);
+  } else if (s->m_turb == 2) {
+    /* directional Smagorinsky, 5_viscosidade.f90:104-170: face viscosities
+     * from the strain at each face; nut is left as it is */
+    double *dudx_i = alloc_n(FX), *dudy_i = alloc_n(FX), *dudz_i = alloc_n(FX);
+    double *dvdx_i = alloc_n(FY), *dvdy_i = alloc_n(FY), *dvdz_i = alloc_n(FY);
+    double *dwdx_i = alloc_n(FZ), *dwdy_i = alloc_n(FZ), *dwdz_i = alloc_n(FZ);
+    derivax(s->u, nx1, ny, nz, s->dx, dudx_i); derivax(s->v, nx, ny1, nz, s->dx, dvdx_i); derivax(s->w, nx, ny, nz1, s->dx, dwdx_i);
+    derivay(s->u, nx1, ny, nz, s->dy, dudy_i); derivay(s->v, nx, ny1, nz, s->dy, dvdy_i); derivay(s->w, nx, ny, nz1, s->dy, dwdy_i);
+    derivaz(s->u, nx1, ny, nz, s->dz, dudz_i); derivaz(s->v, nx, ny1, nz, s->dz, dvdz_i); derivaz(s->w, nx, ny, nz1, s->dz, dwdz_i);
+    const int64_t EXY = (int64_t)nx1 * ny1 * nz, EXZ = (int64_t)nx1 * ny * nz1, EYZ = (int64_t)nx * ny1 * nz1;
+    double *ea = alloc_n(EXY), *eb = alloc_n(EXZ), *ec = alloc_n(EYZ);
+    double *dvdx_x = alloc_n(FX), *dwdx_x = alloc_n(FX), *dudy_y = alloc_n(FY), *dwdy_y = alloc_n(FY);
+    double *dudz_z = alloc_n(FZ), *dvdz_z = alloc_n(FZ);
+    interpx_cf(dvdx_i, nx, ny1, nz, ea); interpy_fc(ea, nx1, ny1, nz, dvdx_x);
+    interpx_cf(dwdx_i, nx, ny, nz1, eb); interpz_fc(eb, nx1, ny, nz1, dwdx_x);
+    interpy_cf(dudy_i, nx1, ny, nz, ea); interpx_fc(ea, nx1, ny1, nz, dudy_y);
+    interpy_cf(dwdy_i, nx, ny, nz1, ec); interpz_fc(ec, nx, ny1, nz1, dwdy_y);
+    interpz_cf(dudz_i, nx1, ny, nz, eb); interpx_fc(eb, nx1, ny, nz1, dudz_z);
+    interpz_cf(dvdz_i, nx, ny1, nz, ec); interpy_fc(ec, nx, ny1, nz1, dvdz_z);
+    const double cs = s->csmag;
+    for (int64_t c = 0; c < FX; c++) {
+      const double p1 = dudx_i[c] + 0.5 * (dudy_i[c] + dvdx_x[c] + dudz_i[c] + dwdx_x[c]);
+      xnut_a[c] = s->dx * s->dx * cs * cs * sqrt(2. * p1 * p1);
+    }
+    for (int64_t c = 0; c < FY; c++) {
+      const double p1 = dvdy_i[c] + 0.5 * (dvdx_i[c] + dudy_y[c] + dvdz_i[c] + dwdy_y[c]);
+      ynut_a[c] = s->dy * s->dy * cs * cs * sqrt(2. * p1 * p1);
+    }
+    for (int64_t c = 0; c < FZ; c++) {
+      const double p1 = dwdz_i[c] + 0.5 * (dwdy_i[c] + dvdz_z[c] + dudz_z[c] + dwdx_i[c]);
+      znut_a[c] = s->dz * s->dz * cs * cs * sqrt(2. * p1 * p1);
+    }
+    free(dudx_i); free(dudy_i); free(dudz_i); free(dvdx_i); free(dvdy_i); free(dvdz_i);
+    free(dwdx_i); free(dwdy_i); free(dwdz_i); free(ea); free(eb); free(ec);
+    free(dvdx_x); free(dwdx_x); free(dudy_y); free(dwdy_y); free(dudz_z); free(dvdz_z);
   }
   double *lx = alloc_n(FX), *ly = alloc_n(FY), *lz = alloc_n(FZ);
   interpx_cf(s->ls_nu, nx, ny, nz, lx);
@@ -514,10 +549,221 @@ void or_visco(or_state *s) {
 }
 
 /* classico, der = 1 (first-order upwind), 7_advectivo.f90:33-113 */
+/* second-order one-sided derivatives of a ghosted array, 8_derivadas.f90:83-290:
+ * u2n backward (central at the first face), u2p forward (central at the last) */
+static double d_u2n(const double *a, int64_t c, int64_t st, int f, double h) {
+  if (f >= 2) return 0.5 * (a[c - 2 * st] - 4 * a[c - st] + 3 * a[c]) / h;
+  return 0.5 * (a[c + st] - a[c - st]) / h;
+}
+static double d_u2p(const double *a, int64_t c, int64_t st, int f, int n, double h) {
+  if (f <= n - 1) return 0.5 * (-a[c + 2 * st] + 4 * a[c + st] - 3 * a[c]) / h;
+  return 0.5 * (a[c + st] - a[c - st]) / h;
+}
+
+/* transverse advecting velocities of der = 2, 3 (7_advectivo.f90:129-145,
+ * 176-192): face -> centre -> face interpolation, end faces copied */
+static void adv_interp(const or_state *s, double *bma, double *dma, double *amb, double *dmb, double *amd,
+                       double *bmd) {
+  DIMS;
+  const int nx1 = nx + 1, ny1 = ny + 1, nz1 = nz + 1;
+  double *aux = alloc_n((int64_t)nx * ny * nz);
+  double *us = alloc_n((int64_t)nx1 * ny * nz), *vs = alloc_n((int64_t)nx * ny1 * nz),
+         *ws = alloc_n((int64_t)nx * ny * nz1);
+  for (int k = 1; k <= nz; k++) /* the sections u(1:nx1,1:ny,1:nz), v(1:nx,1:ny1,1:nz), w(1:nx,1:ny,1:nz1) */
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx1; i++) us[IFX(i, j, k)] = s->u[IU(i, j, k)];
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny1; j++)
+      for (int i = 1; i <= nx; i++) vs[IFY(i, j, k)] = s->v[IV(i, j, k)];
+  for (int k = 1; k <= nz1; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) ws[IFZ(i, j, k)] = s->w[IW(i, j, k)];
+  interpy_fc(vs, nx, ny1, nz, aux); interpx_cf(aux, nx, ny, nz, bma);
+  interpz_fc(ws, nx, ny, nz1, aux); interpx_cf(aux, nx, ny, nz, dma);
+  interpx_fc(us, nx1, ny, nz, aux); interpy_cf(aux, nx, ny, nz, amb);
+  interpz_fc(ws, nx, ny, nz1, aux); interpy_cf(aux, nx, ny, nz, dmb);
+  interpx_fc(us, nx1, ny, nz, aux); interpz_cf(aux, nx, ny, nz, amd);
+  interpy_fc(vs, nx, ny1, nz, aux); interpz_cf(aux, nx, ny, nz, bmd);
+  free(aux); free(us); free(vs); free(ws);
+}
+
+/* classico der = 2 (central, 7_advectivo.f90:116-173) and der = 3 (upwind
+ * second order, :176-291) */
+static void classico_der23(const or_state *s, double *uint_, double *vint_, double *wint_) {
+  DIMS;
+  const double *u = s->u, *v = s->v, *w = s->w;
+  const double dx = s->dx, dy = s->dy, dz = s->dz;
+  const int nx1 = nx + 1, ny1 = ny + 1, nz1 = nz + 1;
+  const int64_t FX = (int64_t)nx1 * ny * nz, FY = (int64_t)nx * ny1 * nz, FZ = (int64_t)nx * ny * nz1;
+  double *bma = alloc_n(FX), *dma = alloc_n(FX), *amb = alloc_n(FY), *dmb = alloc_n(FY), *amd = alloc_n(FZ),
+         *bmd = alloc_n(FZ);
+  adv_interp(s, bma, dma, amb, dmb, amd, bmd);
+  const int64_t uX = 1, uY = nx + 3, uZ = (int64_t)(nx + 3) * (ny + 2);
+  const int64_t vX = 1, vY = nx + 2, vZ = (int64_t)(nx + 2) * (ny + 3);
+  const int64_t wX = 1, wY = nx + 2, wZ = (int64_t)(nx + 2) * (ny + 2);
+  if (s->der == 2) {
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx1; i++) {
+          const int64_t c = IU(i, j, k);
+          const double dudx = (u[c + uX] - u[c - uX]) / (2. * dx), dudy = (u[c + uY] - u[c - uY]) / (2. * dy),
+                       dudz = (u[c + uZ] - u[c - uZ]) / (2. * dz);
+          uint_[IFX(i, j, k)] = u[c] * dudx + bma[IFX(i, j, k)] * dudy + dma[IFX(i, j, k)] * dudz;
+        }
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny1; j++)
+        for (int i = 1; i <= nx; i++) {
+          const int64_t c = IV(i, j, k);
+          const double dvdx = (v[c + vX] - v[c - vX]) / (2. * dx), dvdy = (v[c + vY] - v[c - vY]) / (2. * dy),
+                       dvdz = (v[c + vZ] - v[c - vZ]) / (2. * dz);
+          vint_[IFY(i, j, k)] = amb[IFY(i, j, k)] * dvdx + v[c] * dvdy + dmb[IFY(i, j, k)] * dvdz;
+        }
+    for (int k = 1; k <= nz1; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+          const int64_t c = IW(i, j, k);
+          const double dwdx = (w[c + wX] - w[c - wX]) / (2. * dx), dwdy = (w[c + wY] - w[c - wY]) / (2. * dy),
+                       dwdz = (w[c + wZ] - w[c - wZ]) / (2. * dz);
+          wint_[IFZ(i, j, k)] = amd[IFZ(i, j, k)] * dwdx + bmd[IFZ(i, j, k)] * dwdy + w[c] * dwdz;
+        }
+  } else {
+#define UPW2(q, c, st, f, n, h, a) (fmax_f(a, 0.) * d_u2n(q, c, st, f, h) + fmin_f(a, 0.) * d_u2p(q, c, st, f, n, h))
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx1; i++) {
+          const int64_t c = IU(i, j, k);
+          const double dudx = UPW2(u, c, uX, i, nx1, dx, u[c]);
+          const double dudy = UPW2(u, c, uY, j, ny, dy, bma[IFX(i, j, k)]);
+          const double dudz = UPW2(u, c, uZ, k, nz, dz, dma[IFX(i, j, k)]);
+          uint_[IFX(i, j, k)] = dudx + dudy + dudz;
+        }
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny1; j++)
+        for (int i = 1; i <= nx; i++) {
+          const int64_t c = IV(i, j, k);
+          const double dvdx = UPW2(v, c, vX, i, nx, dx, amb[IFY(i, j, k)]);
+          const double dvdy = UPW2(v, c, vY, j, ny1, dy, v[c]);
+          const double dvdz = UPW2(v, c, vZ, k, nz, dz, dmb[IFY(i, j, k)]);
+          vint_[IFY(i, j, k)] = dvdx + dvdy + dvdz;
+        }
+    for (int k = 1; k <= nz1; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+          const int64_t c = IW(i, j, k);
+          const double dwdx = UPW2(w, c, wX, i, nx, dx, amd[IFZ(i, j, k)]);
+          const double dwdy = UPW2(w, c, wY, j, ny, dy, bmd[IFZ(i, j, k)]);
+          const double dwdz = UPW2(w, c, wZ, k, nz1, dz, w[c]);
+          wint_[IFZ(i, j, k)] = dwdx + dwdy + dwdz;
+        }
+#undef UPW2
+  }
+  free(bma); free(dma); free(amb); free(dmb); free(amd); free(bmd);
+}
+
+/* The advecting velocities of rotacional / antissim (7_advectivo.f90:334-423,
+ * 471-600) at a face of each component: the component itself along its own
+ * direction, four-face averages of the others, each in the reference's order. */
+static double av_bma(const or_state *s, int i, int j, int k) {
+  DIMS; const double *v = s->v;
+  return (v[IV(i, j, k)] + v[IV(i - 1, j, k)] + v[IV(i, j + 1, k)] + v[IV(i - 1, j + 1, k)]) * 0.25;
+}
+static double av_dma(const or_state *s, int i, int j, int k) {
+  DIMS; const double *w = s->w;
+  return (w[IW(i, j, k)] + w[IW(i - 1, j, k)] + w[IW(i, j, k + 1)] + w[IW(i - 1, j, k + 1)]) * 0.25;
+}
+static double av_amb(const or_state *s, int i, int j, int k) {
+  DIMS; const double *u = s->u;
+  return (u[IU(i, j, k)] + u[IU(i + 1, j, k)] + u[IU(i, j - 1, k)] + u[IU(i + 1, j - 1, k)]) * 0.25;
+}
+static double av_dmb(const or_state *s, int i, int j, int k) {
+  DIMS; const double *w = s->w;
+  return (w[IW(i, j, k)] + w[IW(i, j - 1, k)] + w[IW(i, j, k + 1)] + w[IW(i, j - 1, k + 1)]) * 0.25;
+}
+static double av_amd(const or_state *s, int i, int j, int k) {
+  DIMS; const double *u = s->u;
+  return (u[IU(i, j, k)] + u[IU(i + 1, j, k)] + u[IU(i, j, k - 1)] + u[IU(i + 1, j, k - 1)]) * 0.25;
+}
+static double av_bmd(const or_state *s, int i, int j, int k) {
+  DIMS; const double *v = s->v;
+  return (v[IV(i, j, k)] + v[IV(i, j + 1, k)] + v[IV(i, j, k - 1)] + v[IV(i, j + 1, k - 1)]) * 0.25;
+}
+
+/* One direction of the flux-form upwind term for face (i,j,k) of component q
+ * (ghosted array, stride st): rotacional (:356-360,...) when anti = 0, the
+ * antisymmetric average with the advective form (:512-523,...) when anti = 1.
+ * Ap, Am, Ac are the advecting velocities along this direction at the faces
+ * behind, ahead and here; for antissim the caller has already copied the
+ * end entries (ap(0) = ap(1), an(n+2) = an(n+1)). */
+static double flux_term(const double *q, int64_t c, int64_t st, double Am, double Ac, double Ap, double h,
+                        int anti) {
+  const double apc = fmax_f(Ac, 0.), apm = fmax_f(Am, 0.), anp = fmin_f(Ap, 0.), anc = fmin_f(Ac, 0.);
+  if (!anti) return (apc * q[c] - apm * q[c - st]) / h + (anp * q[c + st] - anc * q[c]) / h;
+  return ((apc * q[c] - apm * q[c - st]) / h + (anp * q[c + st] - anc * q[c]) / h + apc * (q[c] - q[c - st]) / h +
+          anc * (q[c + st] - q[c]) / h) *
+         0.5;
+}
+
+/* rotacional (adv_type = 2, der = 1), 7_advectivo.f90:296-440, and antissim
+ * (adv_type = 3, der = 1), :443-653.  rotacional's w loop reads dn at
+ * k = nz+2, an entry the reference never sets (it assigns dp there,
+ * :419): this restatement uses min(w(nz+2), 0), the evident intent, so
+ * Fw on the plane k = nz+1 is parity unpinned for adv_type = 2. */
+static void flux_form(const or_state *s, double *uint_, double *vint_, double *wint_, int anti) {
+  DIMS;
+  const double *u = s->u, *v = s->v, *w = s->w;
+  const double dx = s->dx, dy = s->dy, dz = s->dz;
+  const int64_t uX = 1, uY = nx + 3, uZ = (int64_t)(nx + 3) * (ny + 2);
+  const int64_t vX = 1, vY = nx + 2, vZ = (int64_t)(nx + 2) * (ny + 3);
+  const int64_t wX = 1, wY = nx + 2, wZ = (int64_t)(nx + 2) * (ny + 2);
+  /* antissim copies the first entry of ap, bp, dp from the second and the
+   * last entry of the normal direction's an from the one before */
+#define LO(f) (anti && (f) < 1 ? 1 : (f))
+#define HI(f, n) (anti && (f) > (n) ? (n) : (f))
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx + 1; i++) {
+        const int64_t c = IU(i, j, k);
+        const double tx = flux_term(u, c, uX, u[IU(LO(i - 1), j, k)], u[c], u[IU(HI(i + 1, nx + 1), j, k)], dx, anti);
+        const double ty = flux_term(u, c, uY, av_bma(s, i, LO(j - 1), k), av_bma(s, i, j, k), av_bma(s, i, j + 1, k),
+                                    dy, anti);
+        const double tz = flux_term(u, c, uZ, av_dma(s, i, j, LO(k - 1)), av_dma(s, i, j, k), av_dma(s, i, j, k + 1),
+                                    dz, anti);
+        uint_[IFX(i, j, k)] = tx + ty + tz;
+      }
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny + 1; j++)
+      for (int i = 1; i <= nx; i++) {
+        const int64_t c = IV(i, j, k);
+        const double tx = flux_term(v, c, vX, av_amb(s, LO(i - 1), j, k), av_amb(s, i, j, k), av_amb(s, i + 1, j, k),
+                                    dx, anti);
+        const double ty = flux_term(v, c, vY, v[IV(i, LO(j - 1), k)], v[c], v[IV(i, HI(j + 1, ny + 1), k)], dy, anti);
+        const double tz = flux_term(v, c, vZ, av_dmb(s, i, j, LO(k - 1)), av_dmb(s, i, j, k), av_dmb(s, i, j, k + 1),
+                                    dz, anti);
+        vint_[IFY(i, j, k)] = tx + ty + tz;
+      }
+  for (int k = 1; k <= nz + 1; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        const int64_t c = IW(i, j, k);
+        const double tx = flux_term(w, c, wX, av_amd(s, LO(i - 1), j, k), av_amd(s, i, j, k), av_amd(s, i + 1, j, k),
+                                    dx, anti);
+        const double ty = flux_term(w, c, wY, av_bmd(s, i, LO(j - 1), k), av_bmd(s, i, j, k), av_bmd(s, i, j + 1, k),
+                                    dy, anti);
+        const double tz = flux_term(w, c, wZ, w[IW(i, j, LO(k - 1))], w[c], w[IW(i, j, HI(k + 1, nz + 1))], dz, anti);
+        wint_[IFZ(i, j, k)] = tx + ty + tz;
+      }
+#undef LO
+#undef HI
+}
+
 void or_classico(const or_state *s, double *uint_, double *vint_, double *wint_) {
   DIMS;
   const double *u = s->u, *v = s->v, *w = s->w;
   const double dx = s->dx, dy = s->dy, dz = s->dz;
+  if (s->der == 2 || s->der == 3) {
+    classico_der23(s, uint_, vint_, wint_);
+    return;
+  }
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx + 1; i++) {
@@ -559,7 +805,7 @@ void or_classico(const or_state *s, double *uint_, double *vint_, double *wint_)
       }
 }
 
-/* convdiff, 6_convdiff.f90:14-189 (adv_type = 1, no sponge layer) */
+/* convdiff, 6_convdiff.f90:14-189 (adv_type 1, 2, 3 at :70-82; no sponge layer) */
 void or_convdiff(or_state *s) {
   DIMS;
   const int nx1 = nx + 1, ny1 = ny + 1, nz1 = nz + 1;
@@ -568,7 +814,8 @@ void or_convdiff(or_state *s) {
   const double dx = s->dx, dy = s->dy, dz = s->dz;
   const double *u = s->u, *v = s->v, *w = s->w;
   double *uint_ = alloc_n(FX), *vint_ = alloc_n(FY), *wint_ = alloc_n(FZ);
-  or_classico(s, uint_, vint_, wint_);
+  if (s->adv_type == 2 || s->adv_type == 3) flux_form(s, uint_, vint_, wint_, s->adv_type == 3);
+  else or_classico(s, uint_, vint_, wint_);
   double *rhox = alloc_n(FX), *rhoy = alloc_n(FY), *rhoz = alloc_n(FZ);
   interpx_cf(s->rho, nx, ny, nz, rhox);
   interpy_cf(s->rho, nx, ny, nz, rhoy);

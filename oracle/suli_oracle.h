@@ -21,7 +21,7 @@ typedef struct {
   int nx, ny, nz;
   double dx, dy, dz;
   int ccx0, ccxf, ccy0, ccyf, ccz0, cczf;
-  int m_turb;          /* 0 or 1 */
+  int m_turb;          /* 0 none, 1 Smagorinsky, 2 directional Smagorinsky */
   double csmag, gx, gz, chezy;
   /* fluid properties (14_level_set.f90:25-30) */
   double rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1;
@@ -32,6 +32,8 @@ typedef struct {
   int tt;
   int t_tempo;         /* time scheme, 0_modules.f90:24: 0 Euler, 1 RK2, 2 RK3, 3 AB2 */
   int it;              /* step counter of 1_principal.f90:64 (AB2 takes Euler at it = 1) */
+  int der;             /* advection differences, 0_modules.f90:26: 1 upwind, 2 central, 3 upwind 2nd order */
+  int adv_type;        /* advection form, 0_modules.f90:27: 1 classic, 2 rotational, 3 antisymmetric */
   /* diagnostics written by the oracle */
   int cg_iters;        /* iterations of the last graddin call */
   double cg_res2;      /* final sum of squared scaled residuals */

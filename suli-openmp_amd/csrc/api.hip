@@ -92,6 +92,10 @@ extern "C" int suli_init(const suli_params *p) {
     suli_set_error("suli_init: unknown precision");
     return 1;
   }
+  if (p->m_turb < 0 || p->m_turb > 2) {
+    suli_set_error("suli_init: m_turb must be 0 (none), 1 (Smagorinsky) or 2 (directional Smagorinsky)");
+    return 1;
+  }
   if (p->poisson != SULI_POISSON_REF && p->poisson != SULI_POISSON_FCG) {
     suli_set_error("suli_init: unknown poisson solver");
     return 1;
@@ -147,7 +151,7 @@ extern "C" int suli_init(const suli_params *p) {
                  &c->lsB, &c->S, &c->hs, &c->hsx, &c->hsy, &c->hsz, &c->rho, &c->lsnu, &c->modls,
                  &c->kurv, &c->nut, &c->nrx, &c->nry, &c->nrz, &c->ddx, &c->ddy, &c->ddz, &c->Fu,
                  &c->Fv, &c->Fw, &c->tfu, &c->tfv, &c->tfw, &c->fu0, &c->fv0, &c->fw0, &c->fu1,
-                 &c->fv1, &c->fw1};
+                 &c->fv1, &c->fw1, &c->xnf, &c->ynf, &c->znf};
   for (void **q : tf)
     if (dalloc(*c, q, fb)) return 1;
   double **td[] = {&c->prd1, &c->D, &c->sD, &c->q, &c->ax, &c->ay, &c->az, &c->e, &c->r, &c->pp, &c->mp};
@@ -303,9 +307,15 @@ static bool field_ref(Ctx &c, int f, FieldRef &r) {
     case SULI_DDLSDY: r = {K_C, c.ddy, false, false}; return true;
     case SULI_DDLSDZ: r = {K_C, c.ddz, false, false}; return true;
     case SULI_NUT: r = {K_C, c.nut, false, false}; return true;
-    case SULI_XNUT: r = {K_FX, nullptr, false, true}; return true;
-    case SULI_YNUT: r = {K_FY, nullptr, false, true}; return true;
-    case SULI_ZNUT: r = {K_FZ, nullptr, false, true}; return true;
+    case SULI_XNUT:  // stored by the directional model, else rebuilt from nut and ls_nu
+      if (c.ph.m_turb == 2) r = {K_FX, c.xnf, false, false}; else r = {K_FX, nullptr, false, true};
+      return true;
+    case SULI_YNUT:
+      if (c.ph.m_turb == 2) r = {K_FY, c.ynf, false, false}; else r = {K_FY, nullptr, false, true};
+      return true;
+    case SULI_ZNUT:
+      if (c.ph.m_turb == 2) r = {K_FZ, c.znf, false, false}; else r = {K_FZ, nullptr, false, true};
+      return true;
     case SULI_FU: r = {K_FX, c.Fu, false, false}; return true;
     case SULI_FV: r = {K_FY, c.Fv, false, false}; return true;
     case SULI_FW: r = {K_FZ, c.Fw, false, false}; return true;
@@ -559,6 +569,9 @@ extern "C" int suli_visco(void) {
   if (c.ph.m_turb == 1) {
     { ProfScope ps_(c, "visco"); DISPATCH(c, Launch<TT>::visco(c.g, c.ph, c.st, c.u, c.v, c.w, c.rho, c.nut)); }
     HIPCHK(hipGetLastError());
+  } else if (c.ph.m_turb == 2) {
+    { ProfScope ps_(c, "visco"); DISPATCH(c, Launch<TT>::visco_dir(c.g, c.ph, c.st, c.u, c.v, c.w, c.lsnu, c.xnf, c.ynf, c.znf)); }
+    HIPCHK(hipGetLastError());
   }
   return 0;
 }
@@ -581,6 +594,9 @@ static MomPtrs mom_ptrs(Ctx &c) {
   m.un = c.un; m.vn = c.vn; m.wn = c.wn; m.u0o = c.u0; m.v0o = c.v0; m.w0o = c.w0;
   m.Fu = nullptr; m.Fv = nullptr; m.Fw = nullptr;
   m.fu0 = m.fv0 = m.fw0 = m.fu1 = m.fv1 = m.fw1 = nullptr;
+  const bool dir = c.ph.m_turb == 2;
+  m.xnf = dir ? c.xnf : nullptr; m.ynf = dir ? c.ynf : nullptr; m.znf = dir ? c.znf : nullptr;
+  m.adv = c.adv;
   return m;
 }
 
@@ -650,6 +666,17 @@ extern "C" int suli_set_time_scheme(int t_tempo) {
   c.t_tempo = t_tempo;
   c.ab2_started = false;
   c.p2n = 0;  // the pressure history belongs to the previous scheme's stages
+  return 0;
+}
+
+extern "C" int suli_set_advection(int adv_type, int der) {
+  NEEDCTX();
+  const bool ok = (adv_type == 1 && der >= 1 && der <= 3) || ((adv_type == 2 || adv_type == 3) && der == 1);
+  if (!ok) {  // the reference stops on the other pairs (7_advectivo.f90:426-429,649-652)
+    suli_set_error("suli_set_advection: adv_type 1 takes der 1, 2 or 3; adv_type 2 and 3 take der 1");
+    return 1;
+  }
+  g_ctx->adv = 10 * adv_type + der;
   return 0;
 }
 
@@ -851,7 +878,7 @@ struct CkptHeader {
   double vol_ini, vol_ins, t_now, force_amp, force_period, force_phase;
   int32_t guess, stage, p2n, t_tempo;
   double p2dt;
-  int32_t ab2_started, pad;
+  int32_t ab2_started, adv;
 };
 
 // Binary checkpoint: the device frames of the state fields as they are (the
@@ -875,7 +902,7 @@ extern "C" int suli_checkpoint_save(const char *path) {
   h.force_amp = c.force_amp; h.force_period = c.force_period; h.force_phase = c.force_phase;
   h.guess = c.guess; h.stage = c.stage; h.p2n = c.p2n; h.t_tempo = c.t_tempo;
   h.p2dt = c.p2dt;
-  h.ab2_started = c.ab2_started ? 1 : 0; h.pad = 0;
+  h.ab2_started = c.ab2_started ? 1 : 0; h.adv = c.adv;
   HIPCHK(hipStreamSynchronize(c.st));
   FILE *f = std::fopen(path, "wb");
   if (!f) {
@@ -965,7 +992,7 @@ extern "C" int suli_checkpoint_load(const char *path) {
   c.vol_ini = h.vol_ini; c.vol_ins = h.vol_ins; c.t_now = h.t_now;
   c.force_amp = h.force_amp; c.force_period = h.force_period; c.force_phase = h.force_phase;
   c.guess = h.guess; c.stage = h.stage; c.p2n = std::min(h.p2n, 3); c.p2dt = h.p2dt;
-  c.t_tempo = h.t_tempo; c.ab2_started = h.ab2_started != 0;
+  c.t_tempo = h.t_tempo; c.ab2_started = h.ab2_started != 0; c.adv = h.adv;
   c.convdiff_pending = false;
   return 0;
 }

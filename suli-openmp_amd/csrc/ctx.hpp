@@ -65,6 +65,9 @@ struct Ctx {
   // kept forces of the RK3 and AB2 schemes (fu0, fu1 of 11_diversos.f90)
   void *fu0 = nullptr, *fv0 = nullptr, *fw0 = nullptr, *fu1 = nullptr, *fv1 = nullptr, *fw1 = nullptr;
   int t_tempo = SULI_TIME_RK2;  // time scheme
+  int adv = 11;                 // advection: 10*adv_type + der (6_convdiff.f90:70-82)
+  // face viscosities xnut, ynut, znut of the directional model (m_turb = 2)
+  void *xnf = nullptr, *ynf = nullptr, *znf = nullptr;
   bool ab2_started = false;     // fu0 holds the forces of the last AB2 step
   bool tf_on = false;                                    // set by the first upload of a source
   double *prd1 = nullptr;

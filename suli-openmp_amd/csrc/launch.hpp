@@ -11,6 +11,8 @@ struct MomPtrs {
   void *un, *vn, *wn, *u0o, *v0o, *w0o;
   void *Fu, *Fv, *Fw;
   void *fu0, *fv0, *fw0, *fu1, *fv1, *fw1;
+  const void *xnf, *ynf, *znf;  // face viscosities of the directional model, else null
+  int adv;                      // 10*adv_type + der (11, 12, 13, 21, 31)
 };
 
 // outputs of the FCG face set-up (k_fcg_faces)
@@ -40,6 +42,9 @@ struct Launch {
   static void momentum(const Grid &g, const Phys &ph, hipStream_t s, const MomPtrs &m, double dt,
                        int mode);
   static void contorno(const Grid &g, hipStream_t s, int nlock, void *u, void *v, void *w, double *p);
+  // directional Smagorinsky face viscosities (m_turb = 2)
+  static void visco_dir(const Grid &g, const Phys &ph, hipStream_t s, const void *u, const void *v, const void *w,
+                        const void *lsnu, void *xn, void *yn, void *zn);
   static void posdin(const Grid &g, hipStream_t s, double dt, const double *p, const void *rho,
                      void *u, void *v, void *w);
   static void divmax(const Grid &g, hipStream_t s, const void *u, const void *v, const void *w,

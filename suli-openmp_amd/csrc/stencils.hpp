@@ -246,6 +246,7 @@ struct MomArgs {
   T *Fu, *Fv, *Fw;           // optional force output (may be null)
   T *fu0, *fv0, *fw0;        // kept forces of RK3 stage 1 / AB2 (null when unused)
   T *fu1, *fv1, *fw1;        // RK3 stage-2 forces (null when unused)
+  const T *xnf, *ynf, *znf;  // face viscosities of the directional model (null: rebuilt from nut, ls_nu)
   T dt;
   int mode;                  // MomUpdate
 };
@@ -312,13 +313,218 @@ __device__ __forceinline__ T edge_visc(const T *__restrict__ lsnu, const T *__re
   return ((face_visc<T>(lsnu, nut, cc, st1, f1, n1) + face_visc<T>(lsnu, nut, cc - st2, st1, f1, n1)) *
           T(0.5)) / h2;
 }
+// edge viscosity from stored face viscosities fa (FACES: the directional
+// model) or rebuilt from the cell arrays
+template <typename T, bool FACES>
+__device__ __forceinline__ T evisc(const T *__restrict__ fa, const T *__restrict__ lsnu, const T *__restrict__ nut,
+                                   fidx ce, fidx st1, int f1, int n1, fidx st2, int f2, int n2, T h2) {
+  if constexpr (!FACES) return edge_visc<T>(lsnu, nut, ce, st1, f1, n1, st2, f2, n2, h2);
+  const int fc = f2 < 2 ? 2 : (f2 > n2 ? n2 : f2);
+  const fidx cc = ce + (fidx)(fc - f2) * st2;
+  return ((fa[cc] + fa[cc - st2]) * T(0.5)) / h2;
+}
 template <typename T>
 __device__ __forceinline__ T surf_cell(const T *__restrict__ hd, const T *__restrict__ kurv,
                                        const T *__restrict__ rho, fidx cc) {
   return hd[cc] * kurv[cc] / rho[cc];
 }
 
+// ---------------------------------------------------------------------------
+// Advective terms of convdiff (6_convdiff.f90:70-82), one face of one
+// velocity component each.  ADV = 10*adv_type + der: 11 classico upwind
+// (7_advectivo.f90:33-113), 12 classico central (:116-173), 13 classico
+// upwind second order (:176-291), 21 rotacional (:296-440), 31 antissim
+// (:443-653).  Each sum in the reference's order.
+
+// second-order one-sided differences of 8_derivadas.f90:83-290 along stride
+// st at face f of n: backward (central at f = 1), forward (central at f = n)
 template <typename T>
+__device__ __forceinline__ T upw2(const T *__restrict__ q, fidx c, fidx st, int f, int n, T h, T a) {
+  const T qc = q[c], qm = q[c - st], qp = q[c + st];
+  const T dn = f >= 2 ? T(0.5) * (q[c - 2 * st] - T(4.) * qm + T(3.) * qc) / h : T(0.5) * (qp - qm) / h;
+  const T dp = f <= n - 1 ? T(0.5) * (-q[c + 2 * st] + T(4.) * qp - T(3.) * qc) / h : T(0.5) * (qp - qm) / h;
+  return tmax(a, T(0)) * dn + tmin(a, T(0)) * dp;
+}
+// flux-form upwind term along one direction (rotacional), or its average
+// with the advective form (antissim); am, ac, ap: advecting velocity at the
+// faces behind, here, ahead
+template <typename T, bool ANTI>
+__device__ __forceinline__ T flux1(const T *__restrict__ q, fidx c, fidx st, T am, T ac, T ap, T h) {
+  const T qc = q[c], qm = q[c - st], qp = q[c + st];
+  const T apc = tmax(ac, T(0)), apm = tmax(am, T(0)), anp = tmin(ap, T(0)), anc = tmin(ac, T(0));
+  if (!ANTI) return (apc * qc - apm * qm) / h + (anp * qp - anc * qc) / h;
+  return ((apc * qc - apm * qm) / h + (anp * qp - anc * qc) / h + apc * (qc - qm) / h + anc * (qp - qc) / h) *
+         T(0.5);
+}
+template <typename T>
+__device__ __forceinline__ int clamp2(int f, int n) { return f < 2 ? 2 : (f > n ? n : f); }
+
+template <typename T, int ADV>
+__device__ __forceinline__ T adv_u(const Grid &g, const T *__restrict__ u, const T *__restrict__ v,
+                                   const T *__restrict__ w, fidx c, int i, int j, int k, T dx, T dy, T dz) {
+  const fidx X = 1, Y = g.sx, Z = g.sz;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const T uc = u[c];
+  if constexpr (ADV == 11) {
+    const T bma_a = (v[c] + v[c - X] + v[c + Y] + v[c - X + Y]) * T(0.25);
+    const T dma_a = (w[c] + w[c - X] + w[c + Z] + w[c - X + Z]) * T(0.25);
+    T a1 = tmax(uc, T(0)), a2 = tmin(uc, T(0));
+    const T dudx = a1 * (uc - u[c - X]) / dx + a2 * (u[c + X] - uc) / dx;
+    a1 = tmax(bma_a, T(0)); a2 = tmin(bma_a, T(0));
+    const T dudy = a1 * (uc - u[c - Y]) / dy + a2 * (u[c + Y] - uc) / dy;
+    a1 = tmax(dma_a, T(0)); a2 = tmin(dma_a, T(0));
+    const T dudz = a1 * (uc - u[c - Z]) / dz + a2 * (u[c + Z] - uc) / dz;
+    const T uint_ = dudx + dudy + dudz;
+    return uint_;
+  } else if constexpr (ADV == 12 || ADV == 13) {
+    const fidx cc = c + (fidx)(clamp2<T>(i, nx) - i);  // interpx_cf of the cell averages
+    const T bma = ((v[cc] + v[cc + Y]) * T(0.5) + (v[cc - X] + v[cc - X + Y]) * T(0.5)) * T(0.5);
+    const T dma = ((w[cc] + w[cc + Z]) * T(0.5) + (w[cc - X] + w[cc - X + Z]) * T(0.5)) * T(0.5);
+    if constexpr (ADV == 12)
+      return uc * ((u[c + X] - u[c - X]) / (T(2.) * dx)) + bma * ((u[c + Y] - u[c - Y]) / (T(2.) * dy)) +
+             dma * ((u[c + Z] - u[c - Z]) / (T(2.) * dz));
+    else
+      return upw2<T>(u, c, X, i, nx + 1, dx, uc) + upw2<T>(u, c, Y, j, ny, dy, bma) + upw2<T>(u, c, Z, k, nz, dz, dma);
+  } else {
+    constexpr bool A = ADV == 31;
+    auto bma = [&](fidx e) { return (v[e] + v[e - X] + v[e + Y] + v[e - X + Y]) * T(0.25); };
+    auto dma = [&](fidx e) { return (w[e] + w[e - X] + w[e + Z] + w[e - X + Z]) * T(0.25); };
+    const T tx = flux1<T, A>(u, c, X, A && i == 1 ? uc : u[c - X], uc, A && i == nx + 1 ? uc : u[c + X], dx);
+    const T bc = bma(c), dc = dma(c);
+    const T ty = flux1<T, A>(u, c, Y, A && j == 1 ? bc : bma(c - Y), bc, bma(c + Y), dy);
+    const T tz = flux1<T, A>(u, c, Z, A && k == 1 ? dc : dma(c - Z), dc, dma(c + Z), dz);
+    return tx + ty + tz;
+  }
+}
+
+template <typename T, int ADV>
+__device__ __forceinline__ T adv_v(const Grid &g, const T *__restrict__ u, const T *__restrict__ v,
+                                   const T *__restrict__ w, fidx c, int i, int j, int k, T dx, T dy, T dz) {
+  const fidx X = 1, Y = g.sx, Z = g.sz;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const T vc = v[c];
+  if constexpr (ADV == 11) {
+    const T amb_a = (u[c] + u[c + X] + u[c - Y] + u[c + X - Y]) * T(0.25);
+    const T dmb_a = (w[c] + w[c - Y] + w[c + Z] + w[c - Y + Z]) * T(0.25);
+    T a1 = tmax(amb_a, T(0)), a2 = tmin(amb_a, T(0));
+    const T dvdx = a1 * (vc - v[c - X]) / dx + a2 * (v[c + X] - vc) / dx;
+    a1 = tmax(vc, T(0)); a2 = tmin(vc, T(0));
+    const T dvdy = a1 * (vc - v[c - Y]) / dy + a2 * (v[c + Y] - vc) / dy;
+    a1 = tmax(dmb_a, T(0)); a2 = tmin(dmb_a, T(0));
+    const T dvdz = a1 * (vc - v[c - Z]) / dz + a2 * (v[c + Z] - vc) / dz;
+    const T vint_ = dvdx + dvdy + dvdz;
+    return vint_;
+  } else if constexpr (ADV == 12 || ADV == 13) {
+    const fidx cc = c + (fidx)(clamp2<T>(j, ny) - j) * Y;  // interpy_cf of the cell averages
+    const T amb = ((u[cc] + u[cc + X]) * T(0.5) + (u[cc - Y] + u[cc - Y + X]) * T(0.5)) * T(0.5);
+    const T dmb = ((w[cc] + w[cc + Z]) * T(0.5) + (w[cc - Y] + w[cc - Y + Z]) * T(0.5)) * T(0.5);
+    if constexpr (ADV == 12)
+      return amb * ((v[c + X] - v[c - X]) / (T(2.) * dx)) + vc * ((v[c + Y] - v[c - Y]) / (T(2.) * dy)) +
+             dmb * ((v[c + Z] - v[c - Z]) / (T(2.) * dz));
+    else
+      return upw2<T>(v, c, X, i, nx, dx, amb) + upw2<T>(v, c, Y, j, ny + 1, dy, vc) + upw2<T>(v, c, Z, k, nz, dz, dmb);
+  } else {
+    constexpr bool A = ADV == 31;
+    auto amb = [&](fidx e) { return (u[e] + u[e + X] + u[e - Y] + u[e + X - Y]) * T(0.25); };
+    auto dmb = [&](fidx e) { return (w[e] + w[e - Y] + w[e + Z] + w[e - Y + Z]) * T(0.25); };
+    const T ac = amb(c), dc = dmb(c);
+    const T tx = flux1<T, A>(v, c, X, A && i == 1 ? ac : amb(c - X), ac, amb(c + X), dx);
+    const T ty = flux1<T, A>(v, c, Y, A && j == 1 ? vc : v[c - Y], vc, A && j == ny + 1 ? vc : v[c + Y], dy);
+    const T tz = flux1<T, A>(v, c, Z, A && k == 1 ? dc : dmb(c - Z), dc, dmb(c + Z), dz);
+    return tx + ty + tz;
+  }
+}
+
+template <typename T, int ADV>
+__device__ __forceinline__ T adv_w(const Grid &g, const T *__restrict__ u, const T *__restrict__ v,
+                                   const T *__restrict__ w, fidx c, int i, int j, int k, T dx, T dy, T dz) {
+  const fidx X = 1, Y = g.sx, Z = g.sz;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const T wc = w[c];
+  if constexpr (ADV == 11) {
+    const T amd_a = (u[c] + u[c + X] + u[c - Z] + u[c + X - Z]) * T(0.25);
+    const T bmd_a = (v[c] + v[c + Y] + v[c - Z] + v[c + Y - Z]) * T(0.25);
+    T a1 = tmax(amd_a, T(0)), a2 = tmin(amd_a, T(0));
+    const T dwdx = a1 * (wc - w[c - X]) / dx + a2 * (w[c + X] - wc) / dx;
+    a1 = tmax(bmd_a, T(0)); a2 = tmin(bmd_a, T(0));
+    const T dwdy = a1 * (wc - w[c - Y]) / dy + a2 * (w[c + Y] - wc) / dy;
+    a1 = tmax(wc, T(0)); a2 = tmin(wc, T(0));
+    const T dwdz = a1 * (wc - w[c - Z]) / dz + a2 * (w[c + Z] - wc) / dz;
+    const T wint_ = dwdx + dwdy + dwdz;
+    return wint_;
+  } else if constexpr (ADV == 12 || ADV == 13) {
+    const fidx cc = c + (fidx)(clamp2<T>(k, nz) - k) * Z;  // interpz_cf of the cell averages
+    const T amd = ((u[cc] + u[cc + X]) * T(0.5) + (u[cc - Z] + u[cc - Z + X]) * T(0.5)) * T(0.5);
+    const T bmd = ((v[cc] + v[cc + Y]) * T(0.5) + (v[cc - Z] + v[cc - Z + Y]) * T(0.5)) * T(0.5);
+    if constexpr (ADV == 12)
+      return amd * ((w[c + X] - w[c - X]) / (T(2.) * dx)) + bmd * ((w[c + Y] - w[c - Y]) / (T(2.) * dy)) +
+             wc * ((w[c + Z] - w[c - Z]) / (T(2.) * dz));
+    else
+      return upw2<T>(w, c, X, i, nx, dx, amd) + upw2<T>(w, c, Y, j, ny, dy, bmd) + upw2<T>(w, c, Z, k, nz + 1, dz, wc);
+  } else {
+    // rotacional's dn at k = nz+2 is never set by the reference (it assigns
+    // dp there); min(w(nz+2), 0) is used, as for every other entry
+    constexpr bool A = ADV == 31;
+    auto amd = [&](fidx e) { return (u[e] + u[e + X] + u[e - Z] + u[e + X - Z]) * T(0.25); };
+    auto bmd = [&](fidx e) { return (v[e] + v[e + Y] + v[e - Z] + v[e + Y - Z]) * T(0.25); };
+    const T ac = amd(c), bc = bmd(c);
+    const T tx = flux1<T, A>(w, c, X, A && i == 1 ? ac : amd(c - X), ac, amd(c + X), dx);
+    const T ty = flux1<T, A>(w, c, Y, A && j == 1 ? bc : bmd(c - Y), bc, bmd(c + Y), dy);
+    const T tz = flux1<T, A>(w, c, Z, A && k == 1 ? wc : w[c - Z], wc, A && k == nz + 1 ? wc : w[c + Z], dz);
+    return tx + ty + tz;
+  }
+}
+
+// Directional Smagorinsky model (m_turb = 2, 5_viscosidade.f90:104-170,
+// 178-211): the face viscosities xnut, ynut, znut from the strain at each
+// face (central differences of the velocities, 8_derivadas.f90:1-68, moved
+// to the face by the interpolations of :104-131) plus ls_nu interpolated to
+// the face; nut is left unchanged.  One thread per (i,j,k) of
+// [1,nx+1]x[1,ny+1]x[1,nz+1], writing each face kind it owns.
+template <typename T>
+__global__ __launch_bounds__(256) void k_visco_dir(Grid g, T cs, const T *__restrict__ u, const T *__restrict__ v,
+                                                   const T *__restrict__ w, const T *__restrict__ lsnu,
+                                                   T *__restrict__ xn, T *__restrict__ yn, T *__restrict__ zn) {
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
+            k = 1 + blockIdx.z;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i > nx + 1 || j > ny + 1) return;
+  const fidx X = 1, Y = g.sx, Z = g.sz;
+  const fidx c = g.idx(i, j, k);
+  const T dx = (T)g.dx, dy = (T)g.dy, dz = (T)g.dz;
+  auto cdx = [&](const T *q, fidx e) { return (q[e + X] - q[e - X]) / (T(2.) * dx); };
+  auto cdy = [&](const T *q, fidx e) { return (q[e + Y] - q[e - Y]) / (T(2.) * dy); };
+  auto cdz = [&](const T *q, fidx e) { return (q[e + Z] - q[e - Z]) / (T(2.) * dz); };
+  if (j <= ny && k <= nz) {  // u face
+    const fidx ci = c + (fidx)(clamp2<T>(i, nx) - i);  // interpx_cf column
+    auto dvdx_a = [&](fidx e) { return (cdx(v, e) + cdx(v, e - X)) * T(0.5); };
+    auto dwdx_a = [&](fidx e) { return (cdx(w, e) + cdx(w, e - X)) * T(0.5); };
+    const T dvdx_x = (dvdx_a(ci) + dvdx_a(ci + Y)) * T(0.5);
+    const T dwdx_x = (dwdx_a(ci) + dwdx_a(ci + Z)) * T(0.5);
+    const T p1 = cdx(u, c) + T(0.5) * (cdy(u, c) + dvdx_x + cdz(u, c) + dwdx_x);
+    xn[c] = interp_face<T>(lsnu, c, X, i, nx) + dx * dx * cs * cs * sqrt(T(2.) * p1 * p1);
+  }
+  if (i <= nx && k <= nz) {  // v face
+    const fidx cj = c + (fidx)(clamp2<T>(j, ny) - j) * Y;
+    auto dudy_a = [&](fidx e) { return (cdy(u, e) + cdy(u, e - Y)) * T(0.5); };
+    auto dwdy_a = [&](fidx e) { return (cdy(w, e) + cdy(w, e - Y)) * T(0.5); };
+    const T dudy_y = (dudy_a(cj) + dudy_a(cj + X)) * T(0.5);
+    const T dwdy_y = (dwdy_a(cj) + dwdy_a(cj + Z)) * T(0.5);
+    const T p1 = cdy(v, c) + T(0.5) * (cdx(v, c) + dudy_y + cdz(v, c) + dwdy_y);
+    yn[c] = interp_face<T>(lsnu, c, Y, j, ny) + dy * dy * cs * cs * sqrt(T(2.) * p1 * p1);
+  }
+  if (i <= nx && j <= ny) {  // w face
+    const fidx ck = c + (fidx)(clamp2<T>(k, nz) - k) * Z;
+    auto dvdz_a = [&](fidx e) { return (cdz(v, e) + cdz(v, e - Z)) * T(0.5); };
+    auto dudz_a = [&](fidx e) { return (cdz(u, e) + cdz(u, e - Z)) * T(0.5); };
+    const T dvdz_z = (dvdz_a(ck) + dvdz_a(ck + Y)) * T(0.5);
+    const T dudz_z = (dudz_a(ck) + dudz_a(ck + X)) * T(0.5);
+    const T p1 = cdz(w, c) + T(0.5) * (cdy(w, c) + dvdz_z + dudz_z + cdx(w, c));
+    zn[c] = interp_face<T>(lsnu, c, Z, k, nz) + dz * dz * cs * cs * sqrt(T(2.) * p1 * p1);
+  }
+}
+
+template <typename T, int ADV, bool FACES>
 __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, const MomArgs<T> &A, int i, int j,
                                               int k) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -334,21 +540,13 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
   if (i <= nx + 2 && j <= ny + 1 && k <= nz + 1) {
     if (i >= 1 && i <= nx + 1 && j >= 1 && j <= ny && k >= 1 && k <= nz) {
       const T uc = u[c];
-      const T bma_a = (v[c] + v[c - X] + v[c + Y] + v[c - X + Y]) * T(0.25);
-      const T dma_a = (w[c] + w[c - X] + w[c + Z] + w[c - X + Z]) * T(0.25);
-      T a1 = tmax(uc, T(0)), a2 = tmin(uc, T(0));
-      const T dudx = a1 * (uc - u[c - X]) / dx + a2 * (u[c + X] - uc) / dx;
-      a1 = tmax(bma_a, T(0)); a2 = tmin(bma_a, T(0));
-      const T dudy = a1 * (uc - u[c - Y]) / dy + a2 * (u[c + Y] - uc) / dy;
-      a1 = tmax(dma_a, T(0)); a2 = tmin(dma_a, T(0));
-      const T dudz = a1 * (uc - u[c - Z]) / dz + a2 * (u[c + Z] - uc) / dz;
-      const T uint_ = dudx + dudy + dudz;
+      const T uint_ = adv_u<T, ADV>(g, u, v, w, c, i, j, k, dx, dy, dz);
       const T amai = cell_clamp<T>(A.nut, c, X, i, nx) / dx;
       const T amam = cell_clamp<T>(A.nut, c - X, X, i - 1, nx) / dx;
-      const T bmap = edge_visc<T>(A.lsnu, A.nut, c + Y, X, i, nx, Y, j + 1, ny, dy);
-      const T bmac = edge_visc<T>(A.lsnu, A.nut, c, X, i, nx, Y, j, ny, dy);
-      const T dmap = edge_visc<T>(A.lsnu, A.nut, c + Z, X, i, nx, Z, k + 1, nz, dz);
-      const T dmac = edge_visc<T>(A.lsnu, A.nut, c, X, i, nx, Z, k, nz, dz);
+      const T bmap = evisc<T, FACES>(A.xnf, A.lsnu, A.nut, c + Y, X, i, nx, Y, j + 1, ny, dy);
+      const T bmac = evisc<T, FACES>(A.xnf, A.lsnu, A.nut, c, X, i, nx, Y, j, ny, dy);
+      const T dmap = evisc<T, FACES>(A.xnf, A.lsnu, A.nut, c + Z, X, i, nx, Z, k + 1, nz, dz);
+      const T dmac = evisc<T, FACES>(A.xnf, A.lsnu, A.nut, c, X, i, nx, Z, k, nz, dz);
       const T rhox = interp_face<T>(A.rho, c, X, i, nx);
       const int ic = i < 2 ? 2 : (i > nx ? nx : i);
       const fidx cc = c + (fidx)(ic - i);
@@ -374,21 +572,13 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
   if (i <= nx + 1 && j <= ny + 2 && k <= nz + 1) {
     if (i >= 1 && i <= nx && j >= 1 && j <= ny + 1 && k >= 1 && k <= nz) {
       const T vc = v[c];
-      const T amb_a = (u[c] + u[c + X] + u[c - Y] + u[c + X - Y]) * T(0.25);
-      const T dmb_a = (w[c] + w[c - Y] + w[c + Z] + w[c - Y + Z]) * T(0.25);
-      T a1 = tmax(amb_a, T(0)), a2 = tmin(amb_a, T(0));
-      const T dvdx = a1 * (vc - v[c - X]) / dx + a2 * (v[c + X] - vc) / dx;
-      a1 = tmax(vc, T(0)); a2 = tmin(vc, T(0));
-      const T dvdy = a1 * (vc - v[c - Y]) / dy + a2 * (v[c + Y] - vc) / dy;
-      a1 = tmax(dmb_a, T(0)); a2 = tmin(dmb_a, T(0));
-      const T dvdz = a1 * (vc - v[c - Z]) / dz + a2 * (v[c + Z] - vc) / dz;
-      const T vint_ = dvdx + dvdy + dvdz;
-      const T ambp = edge_visc<T>(A.lsnu, A.nut, c + X, Y, j, ny, X, i + 1, nx, dx);
-      const T ambc = edge_visc<T>(A.lsnu, A.nut, c, Y, j, ny, X, i, nx, dx);
+      const T vint_ = adv_v<T, ADV>(g, u, v, w, c, i, j, k, dx, dy, dz);
+      const T ambp = evisc<T, FACES>(A.ynf, A.lsnu, A.nut, c + X, Y, j, ny, X, i + 1, nx, dx);
+      const T ambc = evisc<T, FACES>(A.ynf, A.lsnu, A.nut, c, Y, j, ny, X, i, nx, dx);
       const T bmbj = cell_clamp<T>(A.nut, c, Y, j, ny) / dy;
       const T bmbm = cell_clamp<T>(A.nut, c - Y, Y, j - 1, ny) / dy;
-      const T dmbp = edge_visc<T>(A.lsnu, A.nut, c + Z, Y, j, ny, Z, k + 1, nz, dz);
-      const T dmbc = edge_visc<T>(A.lsnu, A.nut, c, Y, j, ny, Z, k, nz, dz);
+      const T dmbp = evisc<T, FACES>(A.ynf, A.lsnu, A.nut, c + Z, Y, j, ny, Z, k + 1, nz, dz);
+      const T dmbc = evisc<T, FACES>(A.ynf, A.lsnu, A.nut, c, Y, j, ny, Z, k, nz, dz);
       const T rhoy = interp_face<T>(A.rho, c, Y, j, ny);
       const int jc = j < 2 ? 2 : (j > ny ? ny : j);
       const fidx cc = c + (fidx)(jc - j) * Y;
@@ -414,19 +604,11 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
   if (i <= nx + 1 && j <= ny + 1 && k <= nz + 2) {
     if (i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz + 1) {
       const T wc = w[c];
-      const T amd_a = (u[c] + u[c + X] + u[c - Z] + u[c + X - Z]) * T(0.25);
-      const T bmd_a = (v[c] + v[c + Y] + v[c - Z] + v[c + Y - Z]) * T(0.25);
-      T a1 = tmax(amd_a, T(0)), a2 = tmin(amd_a, T(0));
-      const T dwdx = a1 * (wc - w[c - X]) / dx + a2 * (w[c + X] - wc) / dx;
-      a1 = tmax(bmd_a, T(0)); a2 = tmin(bmd_a, T(0));
-      const T dwdy = a1 * (wc - w[c - Y]) / dy + a2 * (w[c + Y] - wc) / dy;
-      a1 = tmax(wc, T(0)); a2 = tmin(wc, T(0));
-      const T dwdz = a1 * (wc - w[c - Z]) / dz + a2 * (w[c + Z] - wc) / dz;
-      const T wint_ = dwdx + dwdy + dwdz;
-      const T amdp = edge_visc<T>(A.lsnu, A.nut, c + X, Z, k, nz, X, i + 1, nx, dx);
-      const T amdc = edge_visc<T>(A.lsnu, A.nut, c, Z, k, nz, X, i, nx, dx);
-      const T bmdp = edge_visc<T>(A.lsnu, A.nut, c + Y, Z, k, nz, Y, j + 1, ny, dy);
-      const T bmdc = edge_visc<T>(A.lsnu, A.nut, c, Z, k, nz, Y, j, ny, dy);
+      const T wint_ = adv_w<T, ADV>(g, u, v, w, c, i, j, k, dx, dy, dz);
+      const T amdp = evisc<T, FACES>(A.znf, A.lsnu, A.nut, c + X, Z, k, nz, X, i + 1, nx, dx);
+      const T amdc = evisc<T, FACES>(A.znf, A.lsnu, A.nut, c, Z, k, nz, X, i, nx, dx);
+      const T bmdp = evisc<T, FACES>(A.znf, A.lsnu, A.nut, c + Y, Z, k, nz, Y, j + 1, ny, dy);
+      const T bmdc = evisc<T, FACES>(A.znf, A.lsnu, A.nut, c, Z, k, nz, Y, j, ny, dy);
       const T dmdk = cell_clamp<T>(A.nut, c, Z, k, nz) / dz;
       const T dmdm = cell_clamp<T>(A.nut, c - Z, Z, k - 1, nz) / dz;
       const T rhoz = interp_face<T>(A.rho, c, Z, k, nz);
@@ -454,11 +636,11 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
 }
 
 // All three momentum updates of one RK stage, one k-plane per block.
-template <typename T>
+template <typename T, int ADV, bool FACES>
 __global__ __launch_bounds__(256) void k_momentum(Grid g, Phys ph, MomArgs<T> A) {
   const int i = blockIdx.x * SULI_BX + threadIdx.x, j = blockIdx.y * SULI_BY + threadIdx.y;
   if (i > g.nx + 2 || j > g.ny + 2) return;
-  momentum_cell<T>(g, ph, A, i, j, blockIdx.z);
+  momentum_cell<T, ADV, FACES>(g, ph, A, i, j, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------

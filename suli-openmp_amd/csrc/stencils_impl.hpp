@@ -74,10 +74,30 @@ void Launch<T>::momentum(const Grid &g, const Phys &ph, hipStream_t s, const Mom
   a.Fu = (T *)m.Fu; a.Fv = (T *)m.Fv; a.Fw = (T *)m.Fw;
   a.fu0 = (T *)m.fu0; a.fv0 = (T *)m.fv0; a.fw0 = (T *)m.fw0;
   a.fu1 = (T *)m.fu1; a.fv1 = (T *)m.fv1; a.fw1 = (T *)m.fw1;
+  a.xnf = (const T *)m.xnf; a.ynf = (const T *)m.ynf; a.znf = (const T *)m.znf;
   a.dt = (T)dt;
   a.mode = mode;
   dim3 grid((g.nx + 3 + SULI_BX - 1) / SULI_BX, (g.ny + 3 + SULI_BY - 1) / SULI_BY, g.nz + 3);
-  SULI_LAUNCH(k_momentum<T>, grid, blk2(), s, g, ph, a);
+  const bool faces = m.xnf != nullptr;  // directional model: stored face viscosities
+#define SULI_MOM(ADV)                                                                        \
+  (faces ? (void)(k_momentum<T, ADV, true><<<grid, blk2(), 0, s>>>(g, ph, a))                 \
+         : (void)(k_momentum<T, ADV, false><<<grid, blk2(), 0, s>>>(g, ph, a)))
+  switch (m.adv) {
+    case 12: SULI_MOM(12); break;
+    case 13: SULI_MOM(13); break;
+    case 21: SULI_MOM(21); break;
+    case 31: SULI_MOM(31); break;
+    default: SULI_MOM(11); break;
+  }
+#undef SULI_MOM
+}
+
+template <typename T>
+void Launch<T>::visco_dir(const Grid &g, const Phys &ph, hipStream_t s, const void *u, const void *v, const void *w,
+                          const void *lsnu, void *xn, void *yn, void *zn) {
+  dim3 grid((g.nx + 1 + SULI_BX - 1) / SULI_BX, (g.ny + 1 + SULI_BY - 1) / SULI_BY, g.nz + 1);
+  SULI_LAUNCH(k_visco_dir<T>, grid, blk2(), s, g, (T)ph.csmag, (const T *)u, (const T *)v, (const T *)w,
+              (const T *)lsnu, (T *)xn, (T *)yn, (T *)zn);
 }
 
 template <typename T>

@@ -106,6 +106,10 @@ module suli_gpu
       import :: c_int
       integer(c_int), value :: t_tempo
     end function
+    integer(c_int) function c_suli_set_advection(adv_type, der) bind(C, name='suli_set_advection')
+      import :: c_int
+      integer(c_int), value :: adv_type, der
+    end function
     integer(c_int) function c_suli_set_time(t) bind(C, name='suli_set_time')
       import :: c_int, c_double
       real(c_double), value :: t
@@ -149,6 +153,9 @@ module suli_gpu
   integer :: t_plot = 1
   ! time scheme (0_modules.f90:24): 0 Euler, 1 RK2, 2 RK3, 3 AB2
   integer :: t_tempo = 1
+  ! advection (0_modules.f90:26-27): der 1 upwind, 2 central, 3 upwind 2nd
+  ! order; adv_type 1 classic, 2 rotational, 3 antisymmetric
+  integer :: der = 1, adv_type = 1
 
 contains
 
@@ -191,7 +198,8 @@ contains
   end subroutine convdiff
 
   ! The time-scheme part of iniciais (2_iniciais.f90:85-102): stage count
-  ! ntt and stage steps a_dt from dt, and the scheme of the library's tempo.
+  ! ntt and stage steps a_dt from dt, and the scheme of the library's tempo;
+  ! with it the advection scheme of the library's convdiff.
   subroutine tempo_ini()
     if ((t_tempo == 0) .or. (t_tempo == 3)) then
       ntt = 1
@@ -204,6 +212,7 @@ contains
       a_dt(3) = 1.0d0 * dt
     endif
     call check(c_suli_set_time_scheme(int(t_tempo, c_int)), 'set_time_scheme')
+    call check(c_suli_set_advection(int(adv_type, c_int), int(der, c_int)), 'set_advection')
   end subroutine tempo_ini
 
   subroutine tempo()

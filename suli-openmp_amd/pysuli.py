@@ -122,6 +122,7 @@ def lib():
             "suli_checkpoint_save": ([C.c_char_p], i32), "suli_checkpoint_load": ([C.c_char_p], i32),
             "suli_set_guess": ([i32], i32),
             "suli_set_time_scheme": ([i32], i32), "suli_time_stages": ([], i32),
+            "suli_set_advection": ([i32, i32], i32),
             "suli_restart_write": ([C.c_char_p], i32), "suli_restart_read": ([C.c_char_p], i32),
             "suli_write_xdmf": ([C.c_char_p, i32, i32, i32, i32, i32, i32, f64], i32),
         }
@@ -362,6 +363,10 @@ class Solver:
 
     def time_stages(self):
         return lib().suli_time_stages()
+
+    def set_advection(self, adv_type, der):
+        """classico (adv_type 1, der 1..3), rotacional (2, 1), antissim (3, 1)."""
+        _check(lib().suli_set_advection(adv_type, der), "suli_set_advection")
 
     def checkpoint_save(self, path):
         _check(lib().suli_checkpoint_save(os.fsencode(path)), "suli_checkpoint_save")

@@ -8,6 +8,7 @@ the call changed in tests/golden/<case>.npz (float64, Fortran order).
 
     python tests/golden/gen_golden.py            # RK2 cases (g8_*, per_*)
     python tests/golden/gen_golden.py schemes    # Euler, RK3, AB2 (g8tt*_*)
+    python tests/golden/gen_golden.py variants   # advection forms, directional LES
 
 The scheme cases use reference builds with t_tempo = 0, 2, 3
 (0_modules.f90:24) and start from nonzero kept forces fu0, fu1.
@@ -49,7 +50,8 @@ def _save(path, p, s_in, s_out):
     data = {"__scal_in": np.array([s_in.scal[k] for k in O.SCALARS]),
             "__scal_out": np.array([s_out.scal[k] for k in O.SCALARS]),
             "__params": np.array([p.nx, p.ny, p.nz, p.ccx0, p.ccxf, p.ccy0, p.ccyf, p.ccz0, p.cczf]),
-            "__h": np.array([p.dx, p.dy, p.dz]), "__tt": np.array([p.t_tempo])}
+            "__h": np.array([p.dx, p.dy, p.dz]), "__tt": np.array([p.t_tempo]),
+            "__model": np.array([p.der, p.adv_type, p.m_turb])}
     for name, _ in O.ARRAYS:
         a, b = s_in.arr(name), s_out.arr(name)
         data["in_" + name] = a
@@ -75,6 +77,30 @@ def schemes():
             tag = f"{cmd}{arg}"
             changed = _save(os.path.join(HERE, f"g8tt{tt_scheme}_{tag}.npz"), p, s_in, s_out)
             print(f"g8tt{tt_scheme}", tag, "changed:", changed)
+
+
+# advection (der, adv_type of 0_modules.f90:26-27) and turbulence-model
+# variants: case name -> Params overrides
+VARIANTS = {"g8adv12": dict(der=2), "g8adv13": dict(der=3), "g8adv21": dict(adv_type=2),
+            "g8adv31": dict(adv_type=3), "g8les2": dict(m_turb=2)}
+
+
+def variants():
+    import dataclasses
+    for case, kw in VARIANTS.items():
+        p = dataclasses.replace(O.Params(), **kw)
+        O.ref_build(p)
+        cmds = [("visco", None), ("stage", 1)] if "les" in case else [("convdiff", None), ("stage", 1)]
+        for q, (cmd, arg) in enumerate(cmds):
+            s_in = base_state(p, seed=500 + q)
+            if cmd == "convdiff":
+                for n in ["Fu", "Fv", "Fw"]:
+                    s_in.arr(n)[...] = 0.0
+            s_out = s_in.copy()
+            O.ref_run(s_out, cmd, arg)
+            tag = cmd + ("" if arg is None else str(arg))
+            changed = _save(os.path.join(HERE, f"{case}_{tag}.npz"), p, s_in, s_out)
+            print(case, tag, "changed:", changed)
 
 
 def main():
@@ -121,5 +147,7 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["schemes"]:
         schemes()
+    elif sys.argv[1:] == ["variants"]:
+        variants()
     else:
         main()

@@ -31,6 +31,8 @@ def gpu_from_state(state, **kw):
     import pysuli
     sv = pysuli.Solver.from_oracle_params(state.p, **kw)
     pysuli.load_state(sv, state)
+    if (state.p.der, state.p.adv_type) != (1, 1):
+        sv.set_advection(state.p.adv_type, state.p.der)
     if state.p.t_tempo != 1:
         sv.set_time_scheme(state.p.t_tempo)
         for n in ["fu0", "fv0", "fw0", "fu1", "fv1", "fw1"]:

@@ -23,7 +23,9 @@ def _params_from(d):
     nx, ny, nz, ccx0, ccxf, ccy0, ccyf, ccz0, cczf = [int(x) for x in d["__params"]]
     dx, dy, dz = [float(x) for x in d["__h"]]
     return O.Params(nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, ccx0=ccx0, ccxf=ccxf, ccy0=ccy0, ccyf=ccyf,
-                    ccz0=ccz0, cczf=cczf, t_tempo=int(d["__tt"][0]) if "__tt" in d else 1)
+                    ccz0=ccz0, cczf=cczf, t_tempo=int(d["__tt"][0]) if "__tt" in d else 1,
+                    **(dict(zip(("der", "adv_type", "m_turb"), [int(x) for x in d["__model"]])) if "__model" in d
+                       else {}))
 
 
 @pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p)[:-4] for p in GOLD])
@@ -46,6 +48,11 @@ def test_golden_bitexact(path):
         want = d["out_" + name] if ("out_" + name) in d else (d["in_" + name] if ("in_" + name) in d else
                                                                np.zeros_like(got))
         n_changed += ("out_" + name) in d
+        if name == "Fw" and p.adv_type == 2:
+            # rotacional reads dn(:,:,nz+2), which the reference never sets
+            # (7_advectivo.f90:419 assigns dp there): the top face plane of Fw
+            # is unpinned, every other value must match
+            got, want = got[:, :, :-1], want[:, :, :-1]
         assert np.array_equal(got, want), f"{name}: max diff {np.max(np.abs(got - want)):.3e}"
     assert n_changed > 0
     for q, k in enumerate(O.SCALARS):
