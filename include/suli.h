@@ -199,6 +199,20 @@ int suli_time_stages(void); /* ntt of the selected scheme (1, 2 or 3), -1 withou
  * downloadable, nut left unchanged). */
 int suli_set_advection(int adv_type, int der);
 
+/* Bottom obstacle (obst_t != 0 in the reference, 0_modules.f90:29,76-79):
+ * the columns ku(0:nx+2,0:ny+1), kv(0:nx+1,0:ny+2), kw(0:nx+1,0:ny+1), Fortran
+ * order, as obstaculo leaves them (3_contorno.f90:741-760: 0 <= ku, kv <= nz,
+ * 0 <= kw <= nz+1).  contorno(1), contorno(2) zero the velocities of each
+ * column up to k = ku(i,j) and take the friction plane of the column at
+ * ku(i,j)+1 instead of the wall planes (3_contorno.f90:266-291); contorno(3)
+ * fills the level set of the cells below kw (:445-490) and recomputes the
+ * heaviside fields; the field output marks those cells in kaux; the restart
+ * file carries the columns.  All three null removes the obstacle.  The
+ * reference reads the level set outside its array where an obstacle column
+ * on the first or last row or column stands beside a lower ghost column;
+ * the library reads the ghost cell there. */
+int suli_set_obstacle(const int32_t *ku, const int32_t *kv, const int32_t *kw);
+
 /* Time-dependent horizontal body force (sloshing tanks): convdiff uses
  * gx(t) = gx + amplitude*sin(2 pi t/period + phase) in place of the constant
  * gx of 6_convdiff.f90:155, with t the time of the current step.

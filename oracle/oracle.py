@@ -32,6 +32,7 @@ ARRAYS = [
     ("u0", "FX"), ("v0", "FY"), ("w0", "FZ"),
     ("ub", "U"), ("vb", "V"), ("wb", "W"),
     ("fu0", "FX"), ("fv0", "FY"), ("fw0", "FZ"), ("fu1", "FX"), ("fv1", "FY"), ("fw1", "FZ"),
+    ("ku", "KU"), ("kv", "KV"), ("kw", "KW"),
 ]
 SCALARS = ["dt", "t", "vol_ini", "vol_ins", "rho_f1", "rho_f2", "mi_f1", "mi_f2", "sigma", "alpha1"]
 
@@ -47,6 +48,7 @@ def extent_shape(kind: str, nx: int, ny: int, nz: int) -> tuple:
         "FX": (nx + 1, ny, nz),
         "FY": (nx, ny + 1, nz),
         "FZ": (nx, ny, nz + 1),
+        "KU": (nx + 3, ny + 2), "KV": (nx + 2, ny + 3), "KW": (nx + 2, ny + 2),  # obstacle columns
     }[kind]
 
 
@@ -79,6 +81,7 @@ class Params:
     t_tempo: int = 1  # 0 Euler, 1 RK2, 2 RK3, 3 AB2 (0_modules.f90:24)
     der: int = 1       # advection differences: 1 upwind, 2 central, 3 upwind 2nd order (0_modules.f90:26)
     adv_type: int = 1  # advection form: 1 classic, 2 rotational, 3 antisymmetric (0_modules.f90:27)
+    obst_t: int = 0    # nonzero: the state's ku, kv, kw hold a bottom obstacle (0_modules.f90:29)
 
     @property
     def dx1(self) -> float:
@@ -101,7 +104,7 @@ class _OrState(C.Structure):
         ("sigma", C.c_double), ("alpha1", C.c_double),
         ("dt", C.c_double), ("t", C.c_double), ("vol_ini", C.c_double), ("vol_ins", C.c_double),
         ("dt1", C.c_double), ("a_dt", C.c_double * 3), ("tt", C.c_int), ("t_tempo", C.c_int), ("it", C.c_int),
-        ("der", C.c_int), ("adv_type", C.c_int),
+        ("der", C.c_int), ("adv_type", C.c_int), ("obst_t", C.c_int),
         ("cg_iters", C.c_int), ("cg_res2", C.c_double), ("reinit_iters", C.c_int),
         ("mass_corr_iters", C.c_int),
     ] + [(name, C.POINTER(C.c_double)) for name, _ in ARRAYS] + [
@@ -204,7 +207,7 @@ class State:
         p = self.p
         st = _OrState()
         for f in ["nx", "ny", "nz", "dx", "dy", "dz", "ccx0", "ccxf", "ccy0", "ccyf", "ccz0", "cczf",
-                  "m_turb", "csmag", "gx", "gz", "chezy", "t_tempo", "der", "adv_type"]:
+                  "m_turb", "csmag", "gx", "gz", "chezy", "t_tempo", "der", "adv_type", "obst_t"]:
             setattr(st, f, getattr(p, f))
         for k in ["rho_f1", "rho_f2", "mi_f1", "mi_f2", "sigma", "alpha1", "dt", "t", "vol_ini", "vol_ins"]:
             setattr(st, k, self.scal[k])
@@ -286,7 +289,8 @@ REF_DIR = os.path.join(HERE, "_ref")
 def ref_tag(p: Params) -> str:
     return (f"g{p.nx}x{p.ny}x{p.nz}_h{p.dx:g}_{p.dy:g}_{p.dz:g}_bc{p.ccx0}{p.ccxf}{p.ccy0}{p.ccyf}{p.ccz0}{p.cczf}"
             f"_t{p.m_turb}_cs{p.csmag:g}_g{p.gx:g}_{p.gz:g}" + (f"_tt{p.t_tempo}" if p.t_tempo != 1 else "")
-            + (f"_adv{p.adv_type}{p.der}" if (p.der, p.adv_type) != (1, 1) else ""))
+            + (f"_adv{p.adv_type}{p.der}" if (p.der, p.adv_type) != (1, 1) else "")
+            + (f"_ob{p.obst_t}" if p.obst_t else ""))
 
 
 def ref_build(p: Params, quiet: bool = True) -> str:
@@ -300,7 +304,7 @@ def ref_build(p: Params, quiet: bool = True) -> str:
             repr(float(p.dx)), repr(float(p.dy)), repr(float(p.dz)),
             str(p.ccx0), str(p.ccxf), str(p.ccy0), str(p.ccyf), str(p.ccz0), str(p.cczf),
             str(p.m_turb), repr(float(p.csmag)), repr(float(p.gx)), repr(float(p.gz)), str(p.t_tempo),
-            str(p.der), str(p.adv_type)]
+            str(p.der), str(p.adv_type), str(p.obst_t)]
     subprocess.run(args, check=True, stdout=subprocess.DEVNULL if quiet else None)
     return exe
 

@@ -13,6 +13,7 @@
 #   0_modules.f90:20     uinicial = 0 (fluid at rest)
 #   0_modules.f90:24     t_tempo (optional 18th argument, default 1: RK2)
 #   0_modules.f90:26-27  der, adv_type (optional 19th and 20th, default 1, 1)
+#   0_modules.f90:29     obst_t (optional 21st, default 0; the harness loads ku, kv, kw)
 #   0_modules.f90:31     m_turb
 #   0_modules.f90:53-61  boundary codes ccx0..cczf
 #   0_modules.f90:143    gx, gz
@@ -23,15 +24,15 @@
 # 1_principal.f90 (the PROGRAM) is left out; harness.f90 replaces it and
 # calls the reference subroutines in the order of 1_principal.f90:64-108.
 #
-# usage: build_ref.sh TAG NX NY NZ DX DY DZ CCX0 CCXF CCY0 CCYF CCZ0 CCZF MTURB CSMAG GX GZ [TTEMPO [DER ADV_TYPE]]
+# usage: build_ref.sh TAG NX NY NZ DX DY DZ CCX0 CCXF CCY0 CCYF CCZ0 CCZF MTURB CSMAG GX GZ [TTEMPO [DER ADV_TYPE [OBST_T]]]
 set -euo pipefail
-if [ $# -ne 17 ] && [ $# -ne 18 ] && [ $# -ne 20 ]; then
-  echo "usage: $0 TAG NX NY NZ DX DY DZ CCX0 CCXF CCY0 CCYF CCZ0 CCZF MTURB CSMAG GX GZ [TTEMPO [DER ADV_TYPE]]" >&2
+if [ $# -ne 17 ] && [ $# -ne 18 ] && [ $# -ne 20 ] && [ $# -ne 21 ]; then
+  echo "usage: $0 TAG NX NY NZ DX DY DZ CCX0 CCXF CCY0 CCYF CCZ0 CCZF MTURB CSMAG GX GZ [TTEMPO [DER ADV_TYPE [OBST_T]]]" >&2
   exit 2
 fi
 TAG=$1; NX=$2; NY=$3; NZ=$4; DX=$5; DY=$6; DZ=$7
 CCX0=$8; CCXF=$9; CCY0=${10}; CCYF=${11}; CCZ0=${12}; CCZF=${13}
-MTURB=${14}; CSMAG=${15}; GX=${16}; GZ=${17}; TTEMPO=${18:-1}; DER=${19:-1}; ADVT=${20:-1}
+MTURB=${14}; CSMAG=${15}; GX=${16}; GZ=${17}; TTEMPO=${18:-1}; DER=${19:-1}; ADVT=${20:-1}; OBST=${21:-0}
 
 HERE=$(cd "$(dirname "$0")" && pwd)
 REF=${SULI_REFERENCE:-/root/reference}/SuLi-IC
@@ -58,6 +59,7 @@ edit "$M" "s/^real(8),parameter :: uinicial = 0.31/real(8),parameter :: uinicial
 edit "$M" "s/^integer,parameter :: t_tempo = 1 /integer,parameter :: t_tempo = $TTEMPO /" "t_tempo = $TTEMPO "
 edit "$M" "s/^integer,parameter :: der = 1 /integer,parameter :: der = $DER /" "der = $DER "
 edit "$M" "s/^integer,parameter :: adv_type = 1 /integer,parameter :: adv_type = $ADVT /" "adv_type = $ADVT "
+edit "$M" "s/^integer,parameter :: obst_t = 0 /integer,parameter :: obst_t = $OBST /" "obst_t = $OBST "
 edit "$M" "s/^integer,parameter :: m_turb = 0/integer,parameter :: m_turb = $MTURB/" "m_turb = $MTURB"
 edit "$M" "s/^integer,parameter :: ccx0=0/integer,parameter :: ccx0=$CCX0/" "ccx0=$CCX0"
 edit "$M" "s/^integer,parameter :: ccxf=0/integer,parameter :: ccxf=$CCXF/" "ccxf=$CCXF"

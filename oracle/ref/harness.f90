@@ -206,6 +206,7 @@ contains
   subroutine io_state(fname, mode)
     character(len=*), intent(in) :: fname
     integer, intent(in) :: mode
+    real(8) :: dku(0:nx1+1,0:ny+1), dkv(0:nx+1,0:ny1+1), dkw(0:nx+1,0:ny+1)  ! ku, kv, kw as float64
     integer :: hdr(3)
     real(8) :: sc(10)
     if (mode == 0) then
@@ -221,7 +222,8 @@ contains
       sigma = sc(9); alpha1 = sc(10)
       read(21) u, v, w, prd1, rho, ls_nu, ls, mod_ls, kurv, hs, hsx, hsy, hsz, &
                ddlsdx, ddlsdy, ddlsdz, nut, xnut, ynut, znut, Fu, Fv, Fw, u0, v0, w0, ub, vb, wb, &
-               fu0, fv0, fw0, fu1, fv1, fw1
+               fu0, fv0, fw0, fu1, fv1, fw1, dku, dkv, dkw
+      ku = nint(dku); kv = nint(dkv); kw = nint(dkw)
       close(21)
     else
       open(unit=22, file=trim(fname), access='stream', form='unformatted', status='replace')
@@ -229,9 +231,10 @@ contains
       sc = (/ dt, t, vol_ini, vol_ins, rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1 /)
       write(22) hdr
       write(22) sc
+      dku = ku; dkv = kv; dkw = kw
       write(22) u, v, w, prd1, rho, ls_nu, ls, mod_ls, kurv, hs, hsx, hsy, hsz, &
                 ddlsdx, ddlsdy, ddlsdz, nut, xnut, ynut, znut, Fu, Fv, Fw, u0, v0, w0, ub, vb, wb, &
-               fu0, fv0, fw0, fu1, fv1, fw1
+                fu0, fv0, fw0, fu1, fv1, fw1, dku, dkv, dkw
       close(22)
     endif
   end subroutine io_state

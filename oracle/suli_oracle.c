@@ -39,9 +39,10 @@ int64_t or_state_len(int nx, int ny, int nz) {
   int64_t C = (int64_t)nx * ny * nz;
   int64_t FX = (int64_t)(nx + 1) * ny * nz, FY = (int64_t)nx * (ny + 1) * nz, FZ = (int64_t)nx * ny * (nz + 1);
   /* u v w prd1 | 12 cell arrays | nut | xnut ynut znut | Fu Fv Fw | u0 v0 w0 | ub vb wb |
-     fu0 fv0 fw0 | fu1 fv1 fw1 */
+     fu0 fv0 fw0 | fu1 fv1 fw1 | ku kv kw */
   return U + V + W + P + 12 * C + C + FX + FY + FZ + FX + FY + FZ + FX + FY + FZ + U + V + W +
-         2 * (FX + FY + FZ);
+         2 * (FX + FY + FZ) + (int64_t)(nx + 3) * (ny + 2) + (int64_t)(nx + 2) * (ny + 3) +
+         (int64_t)(nx + 2) * (ny + 2);
 }
 
 void or_bind(or_state *s, double *b) {
@@ -62,6 +63,9 @@ void or_bind(or_state *s, double *b) {
   s->ub = b; b += U; s->vb = b; b += V; s->wb = b; b += W;
   s->fu0 = b; b += FX; s->fv0 = b; b += FY; s->fw0 = b; b += FZ;
   s->fu1 = b; b += FX; s->fv1 = b; b += FY; s->fw1 = b; b += FZ;
+  s->ku = b; b += (int64_t)(nx + 3) * (ny + 2);
+  s->kv = b; b += (int64_t)(nx + 2) * (ny + 3);
+  s->kw = b; b += (int64_t)(nx + 2) * (ny + 2);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -986,7 +990,9 @@ static double or_a_dt(int t_tempo, int tt, double dt) {
 
 /* contorno, 3_contorno.f90:29-443 with prd_corr (:872-950, every dp ≡ 0);
  * wall codes 0 (periodic), 1 (free slip), 2 (no slip), 4 (outflow at xf);
- * obst_t = 0, so nlock = 3 does nothing. */
+ * a bottom obstacle (obst_t != 0) zeroes the velocities of its columns and
+ * moves the friction planes on top of it (:266-291); nlock = 3 fills the
+ * level set inside it (:445-490). */
 void or_contorno(or_state *s, int nlock) {
   DIMS;
   const int nx1 = nx + 1, ny1 = ny + 1, nz1 = nz + 1;
@@ -1082,18 +1088,34 @@ void or_contorno(or_state *s, int nlock) {
       for (j = 0; j <= ny1 + 1; j++) for (i = 0; i <= nx + 1; i++) v[IV(i, j, nz + 1)] = (s->cczf == 1) ? v[IV(i, j, nz)] : -v[IV(i, j, nz)];
       for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, nz1 + 1)] = (s->cczf == 1) ? -w[IW(i, j, nz1 - 1)] : w[IW(i, j, nz1 - 1)];
     }
-    /* friction planes, obst_t = 0, :247-264 */
-    if (s->ccz0 == 2) {
+    if (s->obst_t != 0) { /* bottom obstacle, :266-291 */
+#define KU(i, j) ((int)s->ku[(i) + (int64_t)(nx + 3) * (j)])
+#define KV(i, j) ((int)s->kv[(i) + (int64_t)(nx + 2) * (j)])
+#define KW(i, j) ((int)s->kw[(i) + (int64_t)(nx + 2) * (j)])
+      for (j = 0; j <= ny + 1; j++)
+        for (i = 0; i <= nx + 1; i++) {
+          for (k = 0; k <= KU(i, j); k++) u[IU(i, j, k)] = 0.;
+          for (k = 0; k <= KV(i, j); k++) v[IV(i, j, k)] = 0.;
+          for (k = 0; k <= KW(i, j); k++) w[IW(i, j, k)] = 0.;
+          s->ub[IU(i, j, KU(i, j) + 1)] = u[IU(i, j, KU(i, j) + 1)];
+          s->vb[IV(i, j, KV(i, j) + 1)] = v[IV(i, j, KV(i, j) + 1)];
+          s->wb[IW(i, j, KW(i, j) + 1)] = w[IW(i, j, KW(i, j) + 1)];
+        }
+      for (j = 0; j <= ny + 1; j++)
+        for (k = 0; k <= KU(nx1 + 1, j); k++) u[IU(nx1 + 1, j, k)] = 0.;
+      for (i = 0; i <= nx + 1; i++)
+        for (k = 0; k <= KV(i, ny1 + 1); k++) v[IV(i, ny1 + 1, k)] = 0.;
+    } else if (s->ccz0 == 2) { /* friction planes, obst_t = 0, :247-264 */
       for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx1 + 1; i++) s->ub[IU(i, j, 2)] = u[IU(i, j, 2)];
       for (j = 0; j <= ny1 + 1; j++) for (i = 0; i <= nx + 1; i++) s->vb[IV(i, j, 2)] = v[IV(i, j, 2)];
       for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) s->wb[IW(i, j, 2)] = w[IW(i, j, 2)];
     }
-    if (s->ccy0 == 2) {
+    if (s->ccy0 == 2 && s->obst_t == 0) {
       for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx1 + 1; i++) s->ub[IU(i, 2, k)] = u[IU(i, 2, k)];
       for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) s->vb[IV(i, 2, k)] = v[IV(i, 2, k)];
       for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) s->wb[IW(i, 2, k)] = w[IW(i, 2, k)];
     }
-    if (s->ccyf == 2) {
+    if (s->ccyf == 2 && s->obst_t == 0) {
       for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx1 + 1; i++) s->ub[IU(i, ny1 - 1, k)] = u[IU(i, ny1 - 1, k)];
       for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) s->vb[IV(i, ny1 - 1, k)] = v[IV(i, ny1 - 1, k)];
       for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) s->wb[IW(i, ny1 - 1, k)] = w[IW(i, ny1 - 1, k)];
@@ -1128,6 +1150,28 @@ void or_contorno(or_state *s, int nlock) {
     if (s->ccz0 == 1 || s->ccz0 == 2) for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, 1)] = 0. + 0.;
     if (s->cczf == 1 || s->cczf == 2) for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, nz1)] = 0. + 0.;
   }
+  if (nlock == 3 && s->obst_t != 0) {
+    /* level set inside the obstacle from its neighbours, :445-490; every
+     * value read lies outside the obstacle, so the loop order is immaterial */
+    double *ls = s->ls;
+    for (j = 1; j <= ny; j++)
+      for (i = 1; i <= nx; i++)
+        for (k = 1; k <= KW(i, j) - 1; k++) {
+          if (KW(i - 1, j) < k && KW(i, j - 1) < k) ls[IC(i, j, k)] = (ls[IC(i - 1, j, k)] + ls[IC(i, j - 1, k)]) * 0.5;
+          else if (KW(i + 1, j) < k && KW(i, j - 1) < k) ls[IC(i, j, k)] = (ls[IC(i + 1, j, k)] + ls[IC(i, j - 1, k)]) * 0.5;
+          else if (KW(i + 1, j) < k && KW(i, j + 1) < k) ls[IC(i, j, k)] = (ls[IC(i + 1, j, k)] + ls[IC(i, j + 1, k)]) * 0.5;
+          else if (KW(i - 1, j) < k && KW(i, j + 1) < k) ls[IC(i, j, k)] = (ls[IC(i - 1, j, k)] + ls[IC(i, j + 1, k)]) * 0.5;
+          else if (KW(i - 1, j) < k && KW(i, j - 1) >= k && KW(i, j + 1) >= k) ls[IC(i, j, k)] = ls[IC(i - 1, j, k)];
+          else if (KW(i, j - 1) < k && KW(i - 1, j) >= k && KW(i + 1, j) >= k) ls[IC(i, j, k)] = ls[IC(i, j - 1, k)];
+          else if (KW(i + 1, j) < k && KW(i, j - 1) >= k && KW(i, j + 1) >= k) ls[IC(i, j, k)] = ls[IC(i + 1, j, k)];
+          else if (KW(i, j + 1) < k && KW(i - 1, j) >= k && KW(i + 1, j) >= k) ls[IC(i, j, k)] = ls[IC(i, j + 1, k)];
+          else if (k == KW(i, j) - 1 && k < nz) ls[IC(i, j, k)] = ls[IC(i, j, k + 1)];
+        }
+    or_heaviside(s);
+  }
+#undef KU
+#undef KV
+#undef KW
 }
 
 /* graddin, 9_grad.f90:10-278: symmetrically scaled CG on the

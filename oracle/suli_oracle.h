@@ -34,6 +34,7 @@ typedef struct {
   int it;              /* step counter of 1_principal.f90:64 (AB2 takes Euler at it = 1) */
   int der;             /* advection differences, 0_modules.f90:26: 1 upwind, 2 central, 3 upwind 2nd order */
   int adv_type;        /* advection form, 0_modules.f90:27: 1 classic, 2 rotational, 3 antisymmetric */
+  int obst_t;          /* 0: no bottom obstacle; else ku, kv, kw below hold its columns (0_modules.f90:29,76-79) */
   /* diagnostics written by the oracle */
   int cg_iters;        /* iterations of the last graddin call */
   double cg_res2;      /* final sum of squared scaled residuals */
@@ -44,6 +45,9 @@ typedef struct {
   double *ddlsdx, *ddlsdy, *ddlsdz, *nut, *xnut, *ynut, *znut, *Fu, *Fv, *Fw;
   double *u0, *v0, *w0, *ub, *vb, *wb;
   double *fu0, *fv0, *fw0, *fu1, *fv1, *fw1;  /* kept forces of RK3 / AB2 */
+  /* obstacle columns ku(0:nx+2,0:ny+1), kv(0:nx+1,0:ny+2), kw(0:nx+1,0:ny+1)
+   * (integer values held as float64): velocities zeroed up to index k = ku */
+  double *ku, *kv, *kw;
   /* optional momentum sources (manufactured solutions); NULL means zero */
   double *tf_u, *tf_v, *tf_w;
 } or_state;

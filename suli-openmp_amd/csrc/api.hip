@@ -597,6 +597,7 @@ static MomPtrs mom_ptrs(Ctx &c) {
   const bool dir = c.ph.m_turb == 2;
   m.xnf = dir ? c.xnf : nullptr; m.ynf = dir ? c.ynf : nullptr; m.znf = dir ? c.znf : nullptr;
   m.adv = c.adv;
+  m.ku = c.obst ? c.ku_d : nullptr; m.kv = c.obst ? c.kv_d : nullptr; m.kw = c.obst ? c.kw_d : nullptr;
   return m;
 }
 
@@ -680,6 +681,56 @@ extern "C" int suli_set_advection(int adv_type, int der) {
   return 0;
 }
 
+extern "C" int suli_set_obstacle(const int32_t *ku, const int32_t *kv, const int32_t *kw) {
+  NEEDCTX();
+  Ctx &c = *g_ctx;
+  const Grid &g = c.g;
+  if (!ku && !kv && !kw) {
+    c.obst = false;
+    c.ku_h.clear(); c.kv_h.clear(); c.kw_h.clear();
+    return 0;
+  }
+  if (!ku || !kv || !kw) {
+    suli_set_error("suli_set_obstacle: give all of ku, kv, kw (or none to remove the obstacle)");
+    return 1;
+  }
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  struct M { const int32_t *a; int n0, n1, top; const char *name; } ms[3] = {
+      {ku, nx + 3, ny + 2, nz, "ku"}, {kv, nx + 2, ny + 3, nz, "kv"}, {kw, nx + 2, ny + 2, nz + 1, "kw"}};
+  const size_t plane = (size_t)g.sx * g.sy;
+  std::vector<int> h[3];
+  for (int q = 0; q < 3; q++) {
+    h[q].assign(plane, 0);
+    for (int j = 0; j < ms[q].n1; j++)
+      for (int i = 0; i < ms[q].n0; i++) {
+        const int32_t x = ms[q].a[i + (int64_t)ms[q].n0 * j];
+        if (x < 0 || x > ms[q].top) {
+          suli_set_error(std::string("suli_set_obstacle: ") + ms[q].name + "(" + std::to_string(i) + "," +
+                         std::to_string(j) + ") = " + std::to_string(x) + " outside 0.." + std::to_string(ms[q].top));
+          return 1;
+        }
+        h[q][(size_t)i + (size_t)g.sx * j] = x;
+      }
+  }
+  int **d[3] = {&c.ku_d, &c.kv_d, &c.kw_d};
+  for (int q = 0; q < 3; q++) {
+    if (!*d[q]) {
+      if (hipMalloc((void **)d[q], plane * sizeof(int)) != hipSuccess) {
+        suli_set_error("suli_set_obstacle: out of device memory");
+        return 1;
+      }
+      c.allocs.push_back(*d[q]);
+    }
+    HIPCHK(hipMemcpyAsync(*d[q], h[q].data(), plane * sizeof(int), hipMemcpyHostToDevice, c.st));
+  }
+  HIPCHK(hipStreamSynchronize(c.st));
+  c.ku_h.assign(ku, ku + (size_t)(nx + 3) * (ny + 2));
+  c.kv_h.assign(kv, kv + (size_t)(nx + 2) * (ny + 3));
+  c.kw_h.assign(kw, kw + (size_t)(nx + 2) * (ny + 2));
+  c.obst = true;
+  return 0;
+}
+
 extern "C" int suli_time_stages(void) {
   if (!g_ctx) return -1;
   return time_stages(g_ctx->t_tempo);
@@ -688,7 +739,20 @@ extern "C" int suli_time_stages(void) {
 extern "C" int suli_contorno(int nlock) {
   NEEDCTX();
   Ctx &c = *g_ctx;
-  { ProfScope ps_(c, "contorno"); DISPATCH(c, Launch<TT>::contorno(c.g, c.st, nlock, c.u, c.v, c.w, c.prd1)); }
+  if (nlock == 3) {  // only a bottom obstacle acts here (3_contorno.f90:445-490)
+    if (!c.obst) return 0;
+    { ProfScope ps_(c, "contorno"); DISPATCH(c, Launch<TT>::obst_ls(c.g, c.st, c.ls, c.kw_d)); }
+    // heaviside (14_level_set.f90:611); mod_ls is not recomputed there, so
+    // the fused mod_ls output goes to the free reinitialisation buffer
+    { ProfScope ps_(c, "heaviside_normals");
+      DISPATCH(c, Launch<TT>::heavi(c.g, c.ph, c.st, c.ls, c.hs, c.hsx, c.hsy, c.hsz, c.rho, c.lsnu, c.S, c.nrx,
+                                    c.nry, c.nrz, c.slot(RS_VOL, 0))); }
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+  { ProfScope ps_(c, "contorno");
+    DISPATCH(c, Launch<TT>::contorno(c.g, c.st, nlock, c.u, c.v, c.w, c.prd1, c.obst ? c.ku_d : nullptr,
+                                     c.obst ? c.kv_d : nullptr, c.obst ? c.kw_d : nullptr)); }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1082,6 +1146,9 @@ extern "C" int suli_restart_write(const char *path) {
       for (double x : host) put(x);
     } else if (field == -2) {
       put(std::strcmp(it.name, "vol_ini") == 0 ? c.vol_ini : c.vol_ins);
+    } else if (c.obst && it.name[0] == 'k' && it.name[2] == 0) {  // ku, kv, kw
+      const std::vector<int32_t> &m = it.name[1] == 'u' ? c.ku_h : (it.name[1] == 'v' ? c.kv_h : c.kw_h);
+      for (int32_t x : m) put((double)x);
     } else {
       for (int64_t q = 0; q < n; q++) put(0.0);
     }
@@ -1148,6 +1215,8 @@ extern "C" int suli_restart_read(const char *path) {
   std::vector<double> host;
   double vol[2] = {0, 0};
   int nvol = 0;
+  std::vector<int32_t> km[3];
+  bool any_k = false;
   for (const RItem &it : restart_items(c.g)) {
     const int64_t n = (int64_t)it.n0 * it.n1 * it.n2;
     host.resize((size_t)n);
@@ -1163,8 +1232,20 @@ extern "C" int suli_restart_read(const char *path) {
       return 1;
     }
     if (it.field == -2) vol[nvol++] = host[0];
+    if (it.field == -1 && it.name[0] == 'k' && it.name[2] == 0) {  // ku, kv, kw
+      std::vector<int32_t> &m = it.name[1] == 'u' ? km[0] : (it.name[1] == 'v' ? km[1] : km[2]);
+      m.resize((size_t)n);
+      for (int64_t q = 0; q < n; q++) {
+        m[(size_t)q] = (int32_t)std::lround(host[(size_t)q]);
+        any_k = any_k || m[(size_t)q] != 0;
+      }
+    }
   }
   std::fclose(f);
+  // the file's obstacle columns replace the context's (restart_ini reads
+  // ku, kv, kw, 11_diversos.f90:243); zero columns leave a context without
+  // an obstacle as it is
+  if ((c.obst || any_k) && suli_set_obstacle(km[0].data(), km[1].data(), km[2].data())) return 1;
   c.vol_ini = vol[0];
   c.vol_ins = vol[1];
   c.convdiff_pending = false;

@@ -66,6 +66,11 @@ struct Ctx {
   void *fu0 = nullptr, *fv0 = nullptr, *fw0 = nullptr, *fu1 = nullptr, *fv1 = nullptr, *fw1 = nullptr;
   int t_tempo = SULI_TIME_RK2;  // time scheme
   int adv = 11;                 // advection: 10*adv_type + der (6_convdiff.f90:70-82)
+  // bottom obstacle: columns ku, kv, kw on the device (plane index i + sx*j)
+  // and as given (reference extents, for the restart file)
+  bool obst = false;
+  int *ku_d = nullptr, *kv_d = nullptr, *kw_d = nullptr;
+  std::vector<int32_t> ku_h, kv_h, kw_h;
   // face viscosities xnut, ynut, znut of the directional model (m_turb = 2)
   void *xnf = nullptr, *ynf = nullptr, *znf = nullptr;
   bool ab2_started = false;     // fu0 holds the forces of the last AB2 step

@@ -13,6 +13,7 @@ struct MomPtrs {
   void *fu0, *fv0, *fw0, *fu1, *fv1, *fw1;
   const void *xnf, *ynf, *znf;  // face viscosities of the directional model, else null
   int adv;                      // 10*adv_type + der (11, 12, 13, 21, 31)
+  const int *ku, *kv, *kw;      // bottom-obstacle columns (plane layout), else null
 };
 
 // outputs of the FCG face set-up (k_fcg_faces)
@@ -41,7 +42,9 @@ struct Launch {
   // mode: MomUpdate (stencils.hpp)
   static void momentum(const Grid &g, const Phys &ph, hipStream_t s, const MomPtrs &m, double dt,
                        int mode);
-  static void contorno(const Grid &g, hipStream_t s, int nlock, void *u, void *v, void *w, double *p);
+  static void contorno(const Grid &g, hipStream_t s, int nlock, void *u, void *v, void *w, double *p,
+                       const int *ku = nullptr, const int *kv = nullptr, const int *kw = nullptr);
+  static void obst_ls(const Grid &g, hipStream_t s, void *ls, const int *kw);
   // directional Smagorinsky face viscosities (m_turb = 2)
   static void visco_dir(const Grid &g, const Phys &ph, hipStream_t s, const void *u, const void *v, const void *w,
                         const void *lsnu, void *xn, void *yn, void *zn);
@@ -64,8 +67,8 @@ struct Launch {
                         const FcgFaceOut &o);
   // the record of a field-output file (k_frame_record)
   static void frame_record(const Grid &g, hipStream_t s, int t_plot, const void *u, const void *v, const void *w,
-                           const void *ls, const double *p, const void *nut, const float *xs, const float *ys,
-                           const float *zs, float *out);
+                           const void *ls, const double *p, const void *nut, const int *kw, const float *xs,
+                           const float *ys, const float *zs, float *out);
 };
 
 // grid helpers

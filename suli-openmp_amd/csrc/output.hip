@@ -74,7 +74,8 @@ int frame_build(Ctx &c, int t_plot, int64_t *nval) {
     HIPCHK(hipStreamSynchronize(c.st));
   }
   const float *xs = c.out_tab, *ys = xs + g.nx + 2, *zs = ys + g.ny + 2;
-  DISPATCH(c, Launch<TT>::frame_record(g, c.st, t_plot, c.u, c.v, c.w, c.ls, c.prd1, c.nut, xs, ys, zs, c.out_dev));
+  DISPATCH(c, Launch<TT>::frame_record(g, c.st, t_plot, c.u, c.v, c.w, c.ls, c.prd1, c.nut, c.obst ? c.kw_d : nullptr, xs,
+                                               ys, zs, c.out_dev));
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -232,6 +232,9 @@ __global__ __launch_bounds__(256) void k_visco(Grid g, T c23, T cs, const T *__r
 // (6_convdiff.f90:118-145) are rebuilt on the fly from the cell arrays nut
 // and ls_nu with the reference's end-face copies.
 //
+// With a bottom obstacle the friction plane of each column is the face just
+// above it, k = ku(i,j)+1 (3_contorno.f90:273-275), and the wall planes are
+// not used.
 // The friction term -(gz/C^2)*|ub|*u (6_convdiff.f90:156,168,183) reads ub
 // from u itself: contorno copies u into ub on the planes k=2, j=2, j=ny
 // (3_contorno.f90:247-264) and contorno(1) always runs between the last
@@ -247,6 +250,7 @@ struct MomArgs {
   T *fu0, *fv0, *fw0;        // kept forces of RK3 stage 1 / AB2 (null when unused)
   T *fu1, *fv1, *fw1;        // RK3 stage-2 forces (null when unused)
   const T *xnf, *ynf, *znf;  // face viscosities of the directional model (null: rebuilt from nut, ls_nu)
+  const int *ku, *kv, *kw;   // bottom-obstacle columns, plane index i + sx*j (null: no obstacle)
   T dt;
   int mode;                  // MomUpdate
 };
@@ -551,7 +555,8 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
       const int ic = i < 2 ? 2 : (i > nx ? nx : i);
       const fidx cc = c + (fidx)(ic - i);
       const T dhsdx = (surf_cell<T>(A.hsx, A.kurv, A.rho, cc) + surf_cell<T>(A.hsx, A.kurv, A.rho, cc - X)) * T(0.5);
-      const bool pl = (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
+      const bool pl = A.ku ? k == A.ku[i + g.sx * j] + 1
+                           : (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
       const T ub = pl ? uc : T(0);
       const T tf = A.tfu ? A.tfu[c] : T(0);
       const T F = -uint_ +
@@ -583,7 +588,8 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
       const int jc = j < 2 ? 2 : (j > ny ? ny : j);
       const fidx cc = c + (fidx)(jc - j) * Y;
       const T dhsdy = (surf_cell<T>(A.hsy, A.kurv, A.rho, cc) + surf_cell<T>(A.hsy, A.kurv, A.rho, cc - Y)) * T(0.5);
-      const bool pl = (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
+      const bool pl = A.kv ? k == A.kv[i + g.sx * j] + 1
+                           : (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
       const T vb = pl ? vc : T(0);
       const T tf = A.tfv ? A.tfv[c] : T(0);
       const T F = -vint_ +
@@ -615,7 +621,8 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
       const int kc = k < 2 ? 2 : (k > nz ? nz : k);
       const fidx cc = c + (fidx)(kc - k) * Z;
       const T dhsdz = (surf_cell<T>(A.hsz, A.kurv, A.rho, cc) + surf_cell<T>(A.hsz, A.kurv, A.rho, cc - Z)) * T(0.5);
-      const bool pl = (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
+      const bool pl = A.kw ? k == A.kw[i + g.sx * j] + 1
+                           : (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
       const T wb = pl ? wc : T(0);
       const T tf = A.tfw ? A.tfw[c] : T(0);
       const T epis_z = T(0);
@@ -649,6 +656,56 @@ __global__ __launch_bounds__(256) void k_momentum(Grid g, Phys ph, MomArgs<T> A)
 // 1 free slip, 2 no slip, 4 outflow (x = xf only).  The reference fills the
 // ghost planes in the order y walls, x walls, z walls (pressure: x, y, z;
 // plain copies commute), so each phase is one launch over its ghost planes.
+// Bottom obstacle in contorno(1), contorno(2) (3_contorno.f90:266-291):
+// the velocities of each column are zeroed up to k = ku(i,j) (kv, kw),
+// for i, j in [0, n+1] and the extra face planes u(nx+2, :), v(:, ny+2).
+// One thread per column.
+template <typename T>
+__global__ void k_bc_obst(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
+                          const int *__restrict__ ku, const int *__restrict__ kv, const int *__restrict__ kw) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int nx = g.nx, ny = g.ny;
+  if (i > nx + 2 || j > ny + 2) return;
+  const int q = i + g.sx * j;
+  if (j <= ny + 1) {  // u: i in [0, nx+2]
+    for (int k = 0; k <= ku[q]; k++) u[g.idx(i, j, k)] = T(0);
+  }
+  if (i <= nx + 1) {  // v: j in [0, ny+2]
+    for (int k = 0; k <= kv[q]; k++) v[g.idx(i, j, k)] = T(0);
+  }
+  if (i <= nx + 1 && j <= ny + 1) {
+    for (int k = 0; k <= kw[q]; k++) w[g.idx(i, j, k)] = T(0);
+  }
+}
+
+// contorno(3) with a bottom obstacle (3_contorno.f90:445-486): the level set
+// of the cells below kw(i,j) from the neighbouring columns, in the
+// reference's order of cases.  Every value read belongs to a cell outside
+// the obstacle (a neighbour column lower than k, or the cell above), so the
+// update is in place without races.
+template <typename T>
+__global__ void k_obst_ls(Grid g, T *__restrict__ ls, const int *__restrict__ kw) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.nx || j > g.ny) return;
+  const int sx = g.sx;
+  const int q = i + sx * j;
+  if (k > kw[q] - 1) return;
+  const fidx c = g.idx(i, j, k), X = 1, Y = g.sx;
+  const int wm = kw[q - 1], wp = kw[q + 1], sm = kw[q - sx], sp = kw[q + sx];
+  T val = ls[c];
+  if (wm < k && sm < k) val = (ls[c - X] + ls[c - Y]) * T(0.5);
+  else if (wp < k && sm < k) val = (ls[c + X] + ls[c - Y]) * T(0.5);
+  else if (wp < k && sp < k) val = (ls[c + X] + ls[c + Y]) * T(0.5);
+  else if (wm < k && sp < k) val = (ls[c - X] + ls[c + Y]) * T(0.5);
+  else if (wm < k && sm >= k && sp >= k) val = ls[c - X];
+  else if (sm < k && wm >= k && wp >= k) val = ls[c - Y];
+  else if (wp < k && sm >= k && sp >= k) val = ls[c + X];
+  else if (sp < k && wm >= k && wp >= k) val = ls[c + Y];
+  else if (k == kw[q] - 1 && k < g.nz) val = ls[c + g.sz];
+  ls[c] = val;
+}
+
 template <typename T>
 __global__ void k_bc_y(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
                        double *__restrict__ p) {
@@ -851,14 +908,16 @@ __global__ __launch_bounds__(256) void k_divmax(Grid g, const T *__restrict__ u,
 // Node velocities average the four faces around the node's edge
 // (4_plot.f90:269-272), vorticity takes forward differences (:190-201), the
 // face viscosities interp*_cf(nut) with the end faces copied (:206-208,
-// 8_derivadas.f90:306-323); kaux is 0 (no obstacles); the coordinates come
+// 8_derivadas.f90:306-323); kaux marks the cells below kw(i,j) of a bottom
+// obstacle (0 without one); the coordinates come
 // from per-axis tables (the reference's running sums, rounded to FP32).
 // One thread per node; every value is computed in T and rounded once.
 template <typename T>
 __global__ __launch_bounds__(256) void k_frame_record(Grid g, int t_plot, const T *__restrict__ u,
                                                       const T *__restrict__ v, const T *__restrict__ w,
                                                       const T *__restrict__ ls, const double *__restrict__ p,
-                                                      const T *__restrict__ nut, const float *__restrict__ xs,
+                                                      const T *__restrict__ nut, const int *__restrict__ kw,
+                                                      const float *__restrict__ xs,
                                                       const float *__restrict__ ys, const float *__restrict__ zs,
                                                       float *__restrict__ out) {
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
@@ -880,7 +939,7 @@ __global__ __launch_bounds__(256) void k_frame_record(Grid g, int t_plot, const 
   if (i <= nx && j <= ny && k <= nz) {
     const int64_t f = (i - 1) + (int64_t)nx * ((j - 1) + (int64_t)ny * (k - 1));
     o[f] = (float)ls[c];
-    o[n + f] = 0.f;
+    o[n + f] = (kw && k <= kw[i + g.sx * j] - 1) ? 1.f : 0.f;  // kaux, 4_plot.f90:285-297
     if (t_plot == 1) {
       o[2 * n + f] = (float)p[c];
       const T dudy = (u[c + Y] - u[c]) / (T)g.dy, dudz = (u[c + Z] - u[c]) / (T)g.dz;

@@ -75,6 +75,7 @@ void Launch<T>::momentum(const Grid &g, const Phys &ph, hipStream_t s, const Mom
   a.fu0 = (T *)m.fu0; a.fv0 = (T *)m.fv0; a.fw0 = (T *)m.fw0;
   a.fu1 = (T *)m.fu1; a.fv1 = (T *)m.fv1; a.fw1 = (T *)m.fw1;
   a.xnf = (const T *)m.xnf; a.ynf = (const T *)m.ynf; a.znf = (const T *)m.znf;
+  a.ku = m.ku; a.kv = m.kv; a.kw = m.kw;
   a.dt = (T)dt;
   a.mode = mode;
   dim3 grid((g.nx + 3 + SULI_BX - 1) / SULI_BX, (g.ny + 3 + SULI_BY - 1) / SULI_BY, g.nz + 3);
@@ -93,6 +94,11 @@ void Launch<T>::momentum(const Grid &g, const Phys &ph, hipStream_t s, const Mom
 }
 
 template <typename T>
+void Launch<T>::obst_ls(const Grid &g, hipStream_t s, void *ls, const int *kw) {
+  SULI_LAUNCH(k_obst_ls<T>, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), dim3(64, 4, 1), s, g, (T *)ls, kw);
+}
+
+template <typename T>
 void Launch<T>::visco_dir(const Grid &g, const Phys &ph, hipStream_t s, const void *u, const void *v, const void *w,
                           const void *lsnu, void *xn, void *yn, void *zn) {
   dim3 grid((g.nx + 1 + SULI_BX - 1) / SULI_BX, (g.ny + 1 + SULI_BY - 1) / SULI_BY, g.nz + 1);
@@ -101,13 +107,15 @@ void Launch<T>::visco_dir(const Grid &g, const Phys &ph, hipStream_t s, const vo
 }
 
 template <typename T>
-void Launch<T>::contorno(const Grid &g, hipStream_t s, int nlock, void *u, void *v, void *w, double *p) {
+void Launch<T>::contorno(const Grid &g, hipStream_t s, int nlock, void *u, void *v, void *w, double *p,
+                         const int *ku, const int *kv, const int *kw) {
   if (nlock != 1 && nlock != 2) return;
   const dim3 b(64, 4, 1);
   auto gr = [&](int a, int c) { return dim3((a + 63) / 64, (c + 3) / 4, 1); };
   SULI_LAUNCH(k_bc_y<T>, gr(g.nx + 3, g.nz + 3), b, s, g, (T *)u, (T *)v, (T *)w, p);
   SULI_LAUNCH(k_bc_x<T>, gr(g.ny + 3, g.nz + 3), b, s, g, (T *)u, (T *)v, (T *)w, p);
   SULI_LAUNCH(k_bc_z<T>, gr(g.nx + 3, g.ny + 3), b, s, g, (T *)u, (T *)v, (T *)w, p);
+  if (ku) SULI_LAUNCH(k_bc_obst<T>, gr(g.nx + 3, g.ny + 3), b, s, g, (T *)u, (T *)v, (T *)w, ku, kv, kw);
   if (nlock == 2) {
     SULI_LAUNCH(k_bcn_y<T>, gr(g.nx + 2, g.nz + 2), b, s, g, (T *)v);
     SULI_LAUNCH(k_bcn_x<T>, gr(g.ny + 3, g.nz + 3), b, s, g, (T *)u, (T *)v, (T *)w);
@@ -199,9 +207,9 @@ void Launch<T>::pcoef(const Grid &g, hipStream_t s, double dt, const void *rho, 
 
 template <typename T>
 void Launch<T>::frame_record(const Grid &g, hipStream_t s, int t_plot, const void *u, const void *v, const void *w,
-                             const void *ls, const double *p, const void *nut, const float *xs, const float *ys,
-                             const float *zs, float *out) {
+                             const void *ls, const double *p, const void *nut, const int *kw, const float *xs,
+                             const float *ys, const float *zs, float *out) {
   const dim3 grid((g.nx + 1 + SULI_BX - 1) / SULI_BX, (g.ny + 1 + SULI_BY - 1) / SULI_BY, g.nz + 1);
   SULI_LAUNCH(k_frame_record<T>, grid, blk2(), s, g, t_plot, (const T *)u, (const T *)v, (const T *)w,
-              (const T *)ls, p, (const T *)nut, xs, ys, zs, out);
+              (const T *)ls, p, (const T *)nut, kw, xs, ys, zs, out);
 }

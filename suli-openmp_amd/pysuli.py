@@ -123,6 +123,7 @@ def lib():
             "suli_set_guess": ([i32], i32),
             "suli_set_time_scheme": ([i32], i32), "suli_time_stages": ([], i32),
             "suli_set_advection": ([i32, i32], i32),
+            "suli_set_obstacle": ([C.c_void_p, C.c_void_p, C.c_void_p], i32),
             "suli_restart_write": ([C.c_char_p], i32), "suli_restart_read": ([C.c_char_p], i32),
             "suli_write_xdmf": ([C.c_char_p, i32, i32, i32, i32, i32, i32, f64], i32),
         }
@@ -363,6 +364,20 @@ class Solver:
 
     def time_stages(self):
         return lib().suli_time_stages()
+
+    def set_obstacle(self, ku, kv, kw):
+        """Bottom-obstacle columns ku (nx+3, ny+2), kv (nx+2, ny+3), kw (nx+2, ny+2)
+        in the reference's index order; None for all three removes it."""
+        if ku is None and kv is None and kw is None:
+            _check(lib().suli_set_obstacle(None, None, None), "suli_set_obstacle")
+            return
+        shapes = [(self.nx + 3, self.ny + 2), (self.nx + 2, self.ny + 3), (self.nx + 2, self.ny + 2)]
+        arrs = []
+        for a, shp in zip((ku, kv, kw), shapes):
+            a = np.asarray(a)
+            assert a.shape == shp, (a.shape, shp)
+            arrs.append(np.ascontiguousarray(np.rint(a).astype(np.int32).ravel(order="F")))
+        _check(lib().suli_set_obstacle(*[a.ctypes.data for a in arrs]), "suli_set_obstacle")
 
     def set_advection(self, adv_type, der):
         """classico (adv_type 1, der 1..3), rotacional (2, 1), antissim (3, 1)."""

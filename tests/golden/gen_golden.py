@@ -9,6 +9,7 @@ the call changed in tests/golden/<case>.npz (float64, Fortran order).
     python tests/golden/gen_golden.py            # RK2 cases (g8_*, per_*)
     python tests/golden/gen_golden.py schemes    # Euler, RK3, AB2 (g8tt*_*)
     python tests/golden/gen_golden.py variants   # advection forms, directional LES
+    python tests/golden/gen_golden.py obstacle   # bottom obstacle columns (g8obst_*)
 
 The scheme cases use reference builds with t_tempo = 0, 2, 3
 (0_modules.f90:24) and start from nonzero kept forces fu0, fu1.
@@ -51,7 +52,7 @@ def _save(path, p, s_in, s_out):
             "__scal_out": np.array([s_out.scal[k] for k in O.SCALARS]),
             "__params": np.array([p.nx, p.ny, p.nz, p.ccx0, p.ccxf, p.ccy0, p.ccyf, p.ccz0, p.cczf]),
             "__h": np.array([p.dx, p.dy, p.dz]), "__tt": np.array([p.t_tempo]),
-            "__model": np.array([p.der, p.adv_type, p.m_turb])}
+            "__model": np.array([p.der, p.adv_type, p.m_turb, p.obst_t])}
     for name, _ in O.ARRAYS:
         a, b = s_in.arr(name), s_out.arr(name)
         data["in_" + name] = a
@@ -103,6 +104,35 @@ def variants():
             print(case, tag, "changed:", changed)
 
 
+def obstacle_masks(p, seed=0):
+    """Columns of a bottom obstacle (ku, kv, kw of 0_modules.f90:76-79): a
+    seeded bump one to three cells high in the middle of the tank, clear of
+    the side walls (contorno(3) reads ls(i, j-1, k) beside a lower column,
+    outside the array for an obstacle on the first or last row), with ku,
+    kv <= nz and kw <= nz + 1 as obstaculo clamps them."""
+    rng = np.random.default_rng(seed)
+    def bump(nxe, nye, top):
+        m = np.zeros((nxe, nye))
+        m[nxe // 3: 2 * nxe // 3 + 1, 2: nye - 2] = rng.integers(1, top + 1, size=(2 * nxe // 3 + 1 - nxe // 3, nye - 4))
+        return m
+    return bump(p.nx + 3, p.ny + 2, 2), bump(p.nx + 2, p.ny + 3, 2), bump(p.nx + 2, p.ny + 2, 3)
+
+
+def obstacle():
+    import dataclasses
+    p = dataclasses.replace(O.Params(), obst_t=1)
+    O.ref_build(p)
+    cmds = [("contorno", 1), ("contorno", 2), ("contorno", 3), ("stage", 1), ("step", 2)]
+    for q, (cmd, arg) in enumerate(cmds):
+        s_in = base_state(p, seed=700 + q)
+        s_in.ku[...], s_in.kv[...], s_in.kw[...] = obstacle_masks(p, seed=q)
+        s_out = s_in.copy()
+        O.ref_run(s_out, cmd, arg)
+        tag = f"{cmd}{arg}"
+        changed = _save(os.path.join(HERE, f"g8obst_{tag}.npz"), p, s_in, s_out)
+        print("g8obst", tag, "changed:", changed)
+
+
 def main():
     for case, p in CASES.items():
         O.ref_build(p)
@@ -149,5 +179,7 @@ if __name__ == "__main__":
         schemes()
     elif sys.argv[1:] == ["variants"]:
         variants()
+    elif sys.argv[1:] == ["obstacle"]:
+        obstacle()
     else:
         main()

@@ -25,12 +25,33 @@ def consistent_state(p, seed=1, **kw):
     return s
 
 
+def obstacle_state(p, seed=1, **kw):
+    """consistent_state with a seeded bottom obstacle one to three cells high
+    in the middle of the tank, clear of the side walls (ku, kv <= nz,
+    kw <= nz+1 as 3_contorno.f90:741-760 clamps them); p.obst_t must be
+    nonzero."""
+    assert p.obst_t
+    s = O.random_state(p, seed=seed, **kw)
+    rng = np.random.default_rng(seed)
+    for name in ("ku", "kv", "kw"):
+        a = s.arr(name)
+        n0, n1 = a.shape
+        top = 3 if name == "kw" else 2
+        a[...] = 0.0
+        a[n0 // 3: 2 * n0 // 3 + 1, 2: n1 - 2] = rng.integers(1, top + 1, size=(2 * n0 // 3 + 1 - n0 // 3, n1 - 4))
+    s.run("visco")
+    s.run("contorno", 1)
+    return s
+
+
 def gpu_from_state(state, **kw):
     """GPU solver with the state's parameters and fields; for a time scheme
     other than RK2, the scheme and its kept forces fu0, fu1 as well."""
     import pysuli
     sv = pysuli.Solver.from_oracle_params(state.p, **kw)
     pysuli.load_state(sv, state)
+    if state.p.obst_t:
+        sv.set_obstacle(state.ku, state.kv, state.kw)
     if (state.p.der, state.p.adv_type) != (1, 1):
         sv.set_advection(state.p.adv_type, state.p.der)
     if state.p.t_tempo != 1:

@@ -24,7 +24,7 @@ def _params_from(d):
     dx, dy, dz = [float(x) for x in d["__h"]]
     return O.Params(nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, ccx0=ccx0, ccxf=ccxf, ccy0=ccy0, ccyf=ccyf,
                     ccz0=ccz0, cczf=cczf, t_tempo=int(d["__tt"][0]) if "__tt" in d else 1,
-                    **(dict(zip(("der", "adv_type", "m_turb"), [int(x) for x in d["__model"]])) if "__model" in d
+                    **(dict(zip(("der", "adv_type", "m_turb", "obst_t"), [int(x) for x in d["__model"]])) if "__model" in d
                        else {}))
 
 
