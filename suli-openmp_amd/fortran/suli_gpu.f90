@@ -106,6 +106,10 @@ module suli_gpu
       import :: c_int
       integer(c_int), value :: t_tempo
     end function
+    integer(c_int) function c_suli_set_obstacle(ku, kv, kw) bind(C, name='suli_set_obstacle')
+      import :: c_int, c_int32_t
+      integer(c_int32_t), intent(in) :: ku(*), kv(*), kw(*)
+    end function
     integer(c_int) function c_suli_set_advection(adv_type, der) bind(C, name='suli_set_advection')
       import :: c_int
       integer(c_int), value :: adv_type, der
