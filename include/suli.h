@@ -201,8 +201,8 @@ int suli_set_advection(int adv_type, int der);
 
 /* Bottom obstacle (obst_t != 0 in the reference, 0_modules.f90:29,76-79):
  * the columns ku(0:nx+2,0:ny+1), kv(0:nx+1,0:ny+2), kw(0:nx+1,0:ny+1), Fortran
- * order, as obstaculo leaves them (3_contorno.f90:741-760: 0 <= ku, kv <= nz,
- * 0 <= kw <= nz+1).  contorno(1), contorno(2) zero the velocities of each
+ * order, as obstaculo leaves them (3_contorno.f90:741-760: -1 <= ku, kv <= nz,
+ * -1 <= kw <= nz+1).  contorno(1), contorno(2) zero the velocities of each
  * column up to k = ku(i,j) and take the friction plane of the column at
  * ku(i,j)+1 instead of the wall planes (3_contorno.f90:266-291); contorno(3)
  * fills the level set of the cells below kw (:445-490) and recomputes the
@@ -212,6 +212,16 @@ int suli_set_advection(int adv_type, int der);
  * on the first or last row or column stands beside a lower ghost column;
  * the library reads the ghost cell there. */
 int suli_set_obstacle(const int32_t *ku, const int32_t *kv, const int32_t *kw);
+
+/* obstaculo (3_contorno.f90:494-760) on the host, no context needed: the
+ * columns ku, kv, kw (extents as suli_set_obstacle) of the reference's
+ * obstacle geometry obst_t = 1 (sinusoidal dunes), 2 (Yue et al. dune),
+ * 3 (Gaussian bump), 4 (Beji and Battjes bar), 5, 6 (Delft 1980 channels),
+ * 7 (hole and hump) or 10 (Koshizuka / Kleefsman box) on an nx*ny*nz grid of
+ * spacing dx, dy, dz.  8 and 9 read unset heights in the reference and are
+ * refused.  Entries may be -1 (a column cut below the floor). */
+int suli_obstacle_columns(int obst_t, int nx, int ny, int nz, double dx, double dy, double dz, int32_t *ku,
+                          int32_t *kv, int32_t *kw);
 
 /* Time-dependent horizontal body force (sloshing tanks): convdiff uses
  * gx(t) = gx + amplitude*sin(2 pi t/period + phase) in place of the constant

@@ -10,6 +10,7 @@
 ! CMD: ls_ini | level_set | heaviside | mod_ls1 | reinic | der_weno | weno1 ARG(ihs)
 !      visco | convdiff | tempo ARG(tt) | contorno ARG(nlock) | graddin | posdin
 !      stage ARG(tt) | step ARG(nsteps) | bench | plot | visu | restart_salva | restart_ini
+!      obstaculo
 ! (plot and visu write under arquivos/, the restart commands use dados/, of
 ! the working directory)
 !
@@ -191,6 +192,9 @@ program harness
   case ('restart_salva')
     ! dados/arquivorestart of the working directory (11_diversos.f90:303-329)
     call restart_salva()
+  case ('obstaculo')
+    ! the obstacle columns ku, kv, kw of obst_t (3_contorno.f90:494-760)
+    call obstaculo()
   case ('restart_ini')
     ! reads dados/arquivorestart (11_diversos.f90:232-301)
     call restart_ini()

@@ -704,9 +704,9 @@ extern "C" int suli_set_obstacle(const int32_t *ku, const int32_t *kv, const int
     for (int j = 0; j < ms[q].n1; j++)
       for (int i = 0; i < ms[q].n0; i++) {
         const int32_t x = ms[q].a[i + (int64_t)ms[q].n0 * j];
-        if (x < 0 || x > ms[q].top) {
+        if (x < -1 || x > ms[q].top) {
           suli_set_error(std::string("suli_set_obstacle: ") + ms[q].name + "(" + std::to_string(i) + "," +
-                         std::to_string(j) + ") = " + std::to_string(x) + " outside 0.." + std::to_string(ms[q].top));
+                         std::to_string(j) + ") = " + std::to_string(x) + " outside -1.." + std::to_string(ms[q].top));
           return 1;
         }
         h[q][(size_t)i + (size_t)g.sx * j] = x;

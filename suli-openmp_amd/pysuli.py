@@ -124,6 +124,7 @@ def lib():
             "suli_set_time_scheme": ([i32], i32), "suli_time_stages": ([], i32),
             "suli_set_advection": ([i32, i32], i32),
             "suli_set_obstacle": ([C.c_void_p, C.c_void_p, C.c_void_p], i32),
+            "suli_obstacle_columns": ([i32, i32, i32, i32, f64, f64, f64, C.c_void_p, C.c_void_p, C.c_void_p], i32),
             "suli_restart_write": ([C.c_char_p], i32), "suli_restart_read": ([C.c_char_p], i32),
             "suli_write_xdmf": ([C.c_char_p, i32, i32, i32, i32, i32, i32, f64], i32),
         }
@@ -445,6 +446,16 @@ def load_state(solver: Solver, state, names=None):
         solver.upload(n, state.arr(n))
     if state.scal.get("vol_ini", 0) > 0:
         solver.set_vol_ini(state.scal["vol_ini"])
+
+
+def obstacle_columns(obst_t, nx, ny, nz, dx, dy, dz):
+    """ku (nx+3, ny+2), kv (nx+2, ny+3), kw (nx+2, ny+2) of the reference's
+    obstacle geometry obst_t (obstaculo, 3_contorno.f90:494-760)."""
+    shapes = [(nx + 3, ny + 2), (nx + 2, ny + 3), (nx + 2, ny + 2)]
+    out = [np.zeros(int(np.prod(s)), dtype=np.int32) for s in shapes]
+    _check(lib().suli_obstacle_columns(obst_t, nx, ny, nz, dx, dy, dz, *[a.ctypes.data for a in out]),
+           "suli_obstacle_columns")
+    return [a.reshape(s, order="F") for a, s in zip(out, shapes)]
 
 
 WEIGHTS_DIR = os.path.join(HERE, "weights")

@@ -10,6 +10,7 @@ the call changed in tests/golden/<case>.npz (float64, Fortran order).
     python tests/golden/gen_golden.py schemes    # Euler, RK3, AB2 (g8tt*_*)
     python tests/golden/gen_golden.py variants   # advection forms, directional LES
     python tests/golden/gen_golden.py obstacle   # bottom obstacle columns (g8obst_*)
+    python tests/golden/gen_golden.py geometry   # obstaculo's columns per obst_t (obstaculo_*)
 
 The scheme cases use reference builds with t_tempo = 0, 2, 3
 (0_modules.f90:24) and start from nonzero kept forces fu0, fu1.
@@ -133,6 +134,22 @@ def obstacle():
         print("g8obst", tag, "changed:", changed)
 
 
+# grid of the obstacle-geometry vectors: 3 m x 0.6 m x 0.6 m
+GEOM = dict(nx=60, ny=6, nz=12, dx=0.05, dy=0.1, dz=0.05)
+
+
+def geometry():
+    import dataclasses
+    for t in (1, 2, 3, 4, 5, 6, 7, 10):
+        p = dataclasses.replace(O.Params(**GEOM), obst_t=t)
+        O.ref_build(p)
+        s = O.State(p)
+        O.ref_run(s, "obstaculo")
+        np.savez_compressed(os.path.join(HERE, f"obstaculo_{t}.npz"), ku=s.ku, kv=s.kv, kw=s.kw,
+                            grid=np.array([p.nx, p.ny, p.nz]), h=np.array([p.dx, p.dy, p.dz]))
+        print("obst_t", t, "ku range", s.ku.min(), s.ku.max(), "kw range", s.kw.min(), s.kw.max())
+
+
 def main():
     for case, p in CASES.items():
         O.ref_build(p)
@@ -181,5 +198,7 @@ if __name__ == "__main__":
         variants()
     elif sys.argv[1:] == ["obstacle"]:
         obstacle()
+    elif sys.argv[1:] == ["geometry"]:
+        geometry()
     else:
         main()
