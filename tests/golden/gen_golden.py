@@ -145,7 +145,7 @@ def geometry():
         O.ref_build(p)
         s = O.State(p)
         O.ref_run(s, "obstaculo")
-        np.savez_compressed(os.path.join(HERE, f"obstaculo_{t}.npz"), ku=s.ku, kv=s.kv, kw=s.kw,
+        np.savez_compressed(os.path.join(HERE, "obstaculo", f"obstaculo_{t}.npz"), ku=s.ku, kv=s.kv, kw=s.kw,
                             grid=np.array([p.nx, p.ny, p.nz]), h=np.array([p.dx, p.dy, p.dz]))
         print("obst_t", t, "ku range", s.ku.min(), s.ku.max(), "kw range", s.kw.min(), s.kw.max())
 

@@ -1,7 +1,7 @@
 """obstaculo's obstacle columns (3_contorno.f90:494-760), computed on the
 host by the library (suli_obstacle_columns, no GPU needed), against the
 columns reference builds with obst_t = 1..7, 10 wrote
-(tests/golden/obstaculo_*.npz, tests/golden/gen_golden.py geometry):
+(tests/golden/obstaculo/*.npz, tests/golden/gen_golden.py geometry):
 integers, so bit for bit."""
 import glob
 import os
@@ -11,7 +11,7 @@ import pytest
 
 import pysuli
 
-GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "obstaculo_*.npz")))
+GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "obstaculo", "obstaculo_*.npz")))
 
 
 @pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p)[:-4] for p in GOLD])
