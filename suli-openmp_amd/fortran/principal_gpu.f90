@@ -2,7 +2,7 @@
 ! PROGRAM PNH (1_principal.f90:64-108) with its CALL lines unchanged; the
 ! subroutines come from module suli_gpu and run on the GPU.
 !
-!   principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME [T_TEMPO]]]]
+!   principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME [T_TEMPO [OBST_T]]]]]
 !
 ! Tank 1.0 x 0.5 x 0.5 m scaled to the grid (h = 1/NX), dam break of a water
 ! column WIDTH x HEIGHT in the x = 0 corner, water/air 1000/1.2 kg/m^3, no-slip
@@ -14,10 +14,13 @@
 ! and level set are also copied back into the host arrays u, v, w, prd1, ls
 ! with the reference extents (0_modules.f90:92-122).  T_TEMPO selects the
 ! time scheme (0_modules.f90:24): 0 Euler, 1 RK2 (default), 2 RK3, 3 AB2.
+! OBST_T > 0 puts the reference's bottom obstacle of that type (obstaculo,
+! 3_contorno.f90:494-760) into the tank.
 program principal_gpu
   use suli_gpu
   implicit none
-  integer :: nx, ny, nz, ts, precond, frame, nargs
+  integer :: nx, ny, nz, ts, precond, frame, nargs, obst_t
+  integer(c_int32_t), allocatable :: ku(:,:), kv(:,:), kw(:,:)
   real(8) :: width, height, h, div, vol_ini
   character(len=64) :: arg
   integer(8) :: c0, c1, crate
@@ -25,7 +28,7 @@ program principal_gpu
 
   nargs = command_argument_count()
   if (nargs < 4) then
-    write(*,*) 'usage: principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME [T_TEMPO]]]]'
+    write(*,*) 'usage: principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME [T_TEMPO [OBST_T]]]]]'
     stop 2
   endif
   call get_command_argument(1, arg); read(arg, *) nx
@@ -49,6 +52,10 @@ program principal_gpu
   if (nargs >= 9) then
     call get_command_argument(9, arg); read(arg, *) t_tempo
   endif
+  obst_t = 0
+  if (nargs >= 10) then
+    call get_command_argument(10, arg); read(arg, *) obst_t
+  endif
   h = 1.0d0 / nx
 
   ! iniciais (2_iniciais.f90:10-106) and level_set_ini (14_level_set.f90:10-219)
@@ -63,6 +70,12 @@ program principal_gpu
   prm%tol_abs = 0.0d0; prm%tol_rel = 1.0d-8; prm%device = 0
   call check(c_suli_init(prm), 'init')
   call check(c_suli_init_dambreak(width, height, vol_ini), 'init_dambreak')
+  if (obst_t > 0) then  ! obstaculo (2_iniciais.f90:49) and the library's columns
+    allocate(ku(0:nx+2,0:ny+1), kv(0:nx+1,0:ny+2), kw(0:nx+1,0:ny+1))
+    call check(c_suli_obstacle_columns(int(obst_t, c_int), int(nx, c_int), int(ny, c_int), int(nz, c_int), &
+                                       h, h, h, ku, kv, kw), 'obstacle_columns')
+    call check(c_suli_set_obstacle(ku, kv, kw), 'set_obstacle')
+  endif
   dt = 0.2d0 * h / (2.0d0 * sqrt(prm%gz * height))
   call tempo_ini()
   allocate(u(0:nx+2,0:ny+1,0:nz+1), v(0:nx+1,0:ny+2,0:nz+1), w(0:nx+1,0:ny+1,0:nz+2))

@@ -110,6 +110,13 @@ module suli_gpu
       import :: c_int, c_int32_t
       integer(c_int32_t), intent(in) :: ku(*), kv(*), kw(*)
     end function
+    integer(c_int) function c_suli_obstacle_columns(obst_t, nx, ny, nz, dx, dy, dz, ku, kv, kw) &
+        bind(C, name='suli_obstacle_columns')
+      import :: c_int, c_int32_t, c_double
+      integer(c_int), value :: obst_t, nx, ny, nz
+      real(c_double), value :: dx, dy, dz
+      integer(c_int32_t) :: ku(*), kv(*), kw(*)
+    end function
     integer(c_int) function c_suli_set_advection(adv_type, der) bind(C, name='suli_set_advection')
       import :: c_int
       integer(c_int), value :: adv_type, der

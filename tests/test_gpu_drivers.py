@@ -65,6 +65,25 @@ def test_fortran_driver_time_schemes(tmp_path, t_tempo):
     assert m, out.stdout[-500:]
 
 
+def test_fortran_driver_obstacle(tmp_path):
+    """The driver with the reference's obstacle type 7 (a hump rising at
+    x = 0.88 m, 3_contorno.f90:639-652) from suli_obstacle_columns: the dam
+    break runs over it.  The projection does not see the obstacle (the
+    reference zeroes the velocities of its columns afterwards, 3_contorno.f90:
+    266-291), so the divergence next to it is not small: only bounded."""
+    exe = os.path.join(PKG, "principal_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("principal_gpu not built (flang absent)")
+    print("fortran driver 64x32x32, 10 steps, obst_t 7", flush=True)
+    out = subprocess.run([exe, "64", "32", "32", "10", "1", "0.25", "0.25", "0", "1", "7"],
+                         capture_output=True, text=True, timeout=240, cwd=str(tmp_path))
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    steps = re.findall(r"step\s+(\d+) t=\s*(\S+) vol_err=\s*(\S+) maxdiv=\s*(\S+) poisson_iters=\s*(\d+)", out.stdout)
+    assert len(steps) == 10
+    for it, t, vol_err, maxdiv, iters in steps:
+        assert abs(float(vol_err)) < 1e-2 and float(maxdiv) < 10.0 and 0 < int(iters) < 50
+
+
 def test_sloshing_forcing_drives_the_tank():
     """A flat free surface at rest stays at rest without forcing and is set
     in motion by gx(t); the volume is conserved either way."""
