@@ -161,6 +161,7 @@ int suli_graddin(double dt, suli_poisson_stats *st);/* graddin, 9_grad.f90:10   
 int suli_posdin(double dt);                        /* posdin, 10_pos.f90:1             */
 int suli_est_div(double *maxdiv);                  /* est, 4_plot.f90:500              */
 
+
 /* Fu, Fv, Fw of convdiff materialised for inspection (suli_convdiff leaves
  * them implicit: the forces are fused into suli_tempo's update). */
 int suli_convdiff_forces(void);
@@ -360,14 +361,20 @@ int suli_write_xdmf(const char *path, int nx, int ny, int nz, int first, int las
 /* Checkpoint and restart (restart_salva / restart_ini,
  * 11_diversos.f90:232-329).  suli_checkpoint_save/load: binary, the device
  * state of the run as it is (fields with their ghosts in the run's
- * precision, volumes, time and forcing), so that a run resumed from it
+ * precision, volumes, time, forcing, time scheme, the projection's guess
+ * history and the bottom-obstacle columns), so that a run resumed from it
  * continues bit for bit; loads into a context of the same grid and
- * precision only.  suli_restart_write/read: the reference's own restart
- * file dados/arquivorestart, list-directed text in restart_salva's order and
- * extents; items outside this library's model (obstacle indices, prescribed
- * boundary planes, d_max, d_min, ls_m, tf_u/v/w) are written as zeros and
- * skipped on reading, as are items it rebuilds (prd0, prd, ub, vb, wb,
- * xnut, ynut, znut). */
+ * precision only, and the file's obstacle (or its absence) replaces the
+ * context's.  suli_restart_write/read: the reference's own restart file
+ * dados/arquivorestart, list-directed text in restart_salva's order and
+ * extents.  The obstacle columns ku, kv, kw are written and read back:
+ * reading a file replaces the context's obstacle with the file's columns
+ * (a context with an obstacle that reads all-zero columns keeps the
+ * obstacle switched on, every column at k = 0, its friction planes at
+ * k = 1 as the reference's).  Items outside this library's model
+ * (prescribed boundary planes, d_max, d_min, ls_m, tf_u/v/w) are written
+ * as zeros and skipped on reading, as are items it rebuilds (prd0, prd, ub,
+ * vb, wb, xnut, ynut, znut). */
 int suli_checkpoint_save(const char *path);
 int suli_checkpoint_load(const char *path);
 int suli_restart_write(const char *path);

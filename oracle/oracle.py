@@ -106,7 +106,7 @@ class _OrState(C.Structure):
         ("dt1", C.c_double), ("a_dt", C.c_double * 3), ("tt", C.c_int), ("t_tempo", C.c_int), ("it", C.c_int),
         ("der", C.c_int), ("adv_type", C.c_int), ("obst_t", C.c_int),
         ("cg_iters", C.c_int), ("cg_res2", C.c_double), ("reinit_iters", C.c_int),
-        ("mass_corr_iters", C.c_int),
+        ("mass_corr_iters", C.c_int), ("cg_max_it", C.c_int),
     ] + [(name, C.POINTER(C.c_double)) for name, _ in ARRAYS] + [
         ("tf_u", C.POINTER(C.c_double)), ("tf_v", C.POINTER(C.c_double)), ("tf_w", C.POINTER(C.c_double)),
     ]
@@ -169,6 +169,7 @@ class State:
                          sigma=p.sigma, alpha1=p.alpha1)
         self.tf = None  # optional (tf_u, tf_v, tf_w) source arrays
         self.diag = {}
+        self.cg_max_it = 0  # graddin iteration cap (0: the reference's 10000)
 
     def __getattr__(self, name):
         offs = self.__dict__.get("offsets")
@@ -185,6 +186,7 @@ class State:
         s.buf[:] = self.buf
         s.scal = dict(self.scal)
         s.tf = None if self.tf is None else tuple(a.copy() for a in self.tf)
+        s.cg_max_it = self.cg_max_it
         return s
 
     # -- reference harness state files --------------------------------------
@@ -212,6 +214,7 @@ class State:
         for k in ["rho_f1", "rho_f2", "mi_f1", "mi_f2", "sigma", "alpha1", "dt", "t", "vol_ini", "vol_ins"]:
             setattr(st, k, self.scal[k])
         st.dt1 = p.dt1
+        st.cg_max_it = self.cg_max_it
         lib().or_bind(C.byref(st), _dptr(self.buf))
         if self.tf is not None:
             self._tf_keep = [np.ascontiguousarray(a, dtype=np.float64).ravel(order="F") for a in self.tf]

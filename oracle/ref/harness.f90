@@ -10,7 +10,7 @@
 ! CMD: ls_ini | level_set | heaviside | mod_ls1 | reinic | der_weno | weno1 ARG(ihs)
 !      visco | convdiff | tempo ARG(tt) | contorno ARG(nlock) | graddin | posdin
 !      stage ARG(tt) | step ARG(nsteps) | bench | plot | visu | restart_salva | restart_ini
-!      obstaculo
+!      obstaculo | est
 ! (plot and visu write under arquivos/, the restart commands use dados/, of
 ! the working directory)
 !
@@ -192,6 +192,13 @@ program harness
   case ('restart_salva')
     ! dados/arquivorestart of the working directory (11_diversos.f90:303-329)
     call restart_salva()
+  case ('est')
+    ! est (4_plot.f90:500-525): the divergence field goes to kurv and its
+    ! largest magnitude to the vol_ins slot of the state file (est itself
+    ! logs the value to unit 100002 of the working directory)
+    call est(s1)
+    kurv = s1
+    vol_ins = maxval(abs(s1))
   case ('obstaculo')
     ! the obstacle columns ku, kv, kw of obst_t (3_contorno.f90:494-760)
     call obstaculo()

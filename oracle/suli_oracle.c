@@ -1177,7 +1177,9 @@ void or_contorno(or_state *s, int nlock) {
 /* graddin, 9_grad.f90:10-278: symmetrically scaled CG on the
  * variable-density pressure equation, warm-started from prd1, stopped when
  * the sum of squared scaled residuals drops to 1e-4/(nx*ny*nz) or after
- * 10000 iterations. */
+ * 10000 iterations (or after cg_max_it, when set: a fixed-iteration run for
+ * comparisons where the reference operator is not symmetric and its CG does
+ * not stop). */
 void or_graddin(or_state *s) {
   DIMS;
   const int nx1 = nx + 1, ny1 = ny + 1, nz1 = nz + 1;
@@ -1245,7 +1247,8 @@ void or_graddin(or_state *s) {
   memcpy(pp, r, sizeof(double) * (size_t)G);
   int cont = 0;
   const double thr = 0.0001 / (nx * ny * nz);
-  while (fabs(alfamupr) > thr && cont < 10000) {
+  const int cap = s->cg_max_it > 0 ? s->cg_max_it : 10000;
+  while (fabs(alfamupr) > thr && cont < cap) {
     cont = cont + 1;
     double alfadipr = 0., betamupr = 0.;
     alfamupr = 0.;

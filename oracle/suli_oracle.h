@@ -40,6 +40,7 @@ typedef struct {
   double cg_res2;      /* final sum of squared scaled residuals */
   int reinit_iters;    /* iterations of the last reinitialisation */
   int mass_corr_iters; /* volume-correction passes of the last level_set call */
+  int cg_max_it;       /* iteration cap of graddin (0: the reference's 10000, 9_grad.f90:175) */
   /* arrays (pointers into the bound buffer) */
   double *u, *v, *w, *prd1, *rho, *ls_nu, *ls, *mod_ls, *kurv, *hs, *hsx, *hsy, *hsz;
   double *ddlsdx, *ddlsdy, *ddlsdz, *nut, *xnut, *ynut, *znut, *Fu, *Fv, *Fw;

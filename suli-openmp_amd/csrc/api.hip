@@ -932,7 +932,7 @@ static const int kCkptFields[] = {SULI_U,      SULI_V,      SULI_W,      SULI_PR
                                   SULI_HSZ,    SULI_DDLSDX, SULI_DDLSDY, SULI_DDLSDZ, SULI_NUT,  SULI_U0,
                                   SULI_V0,     SULI_W0,     SULI_TFU,    SULI_TFV,    SULI_TFW,  SULI_FU0,
                                   SULI_FV0,    SULI_FW0};
-static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '2'};
+static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '3'};
 
 struct CkptHeader {
   char magic[8];
@@ -943,6 +943,7 @@ struct CkptHeader {
   int32_t guess, stage, p2n, t_tempo;
   double p2dt;
   int32_t ab2_started, adv;
+  int32_t obst, pad_;  // 1: the bottom-obstacle columns ku, kv, kw follow the pressure history
 };
 
 // Binary checkpoint: the device frames of the state fields as they are (the
@@ -967,6 +968,7 @@ extern "C" int suli_checkpoint_save(const char *path) {
   h.guess = c.guess; h.stage = c.stage; h.p2n = c.p2n; h.t_tempo = c.t_tempo;
   h.p2dt = c.p2dt;
   h.ab2_started = c.ab2_started ? 1 : 0; h.adv = c.adv;
+  h.obst = c.obst ? 1 : 0;
   HIPCHK(hipStreamSynchronize(c.st));
   FILE *f = std::fopen(path, "wb");
   if (!f) {
@@ -998,6 +1000,9 @@ extern "C" int suli_checkpoint_save(const char *path) {
     }
     ok = std::fwrite(buf.data(), 1, bytes, f) == bytes;
   }
+  if (h.obst && ok)  // the obstacle columns in the reference extents, as suli_set_obstacle took them
+    for (const std::vector<int32_t> *k : {&c.ku_h, &c.kv_h, &c.kw_h})
+      ok = ok && std::fwrite(k->data(), 4, k->size(), f) == k->size();
   if (std::fclose(f) != 0 || !ok) {
     suli_set_error(std::string("suli_checkpoint_save: write failed on ") + path);
     return 1;
@@ -1047,11 +1052,22 @@ extern "C" int suli_checkpoint_load(const char *path) {
     if (err.empty() && hipMemcpy(c.p2h[q], buf.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
       err = "device copy failed";
   }
+  const int nx = c.g.nx, ny = c.g.ny;
+  std::vector<int32_t> kcol[3];
+  const size_t kn[3] = {(size_t)(nx + 3) * (ny + 2), (size_t)(nx + 2) * (ny + 3), (size_t)(nx + 2) * (ny + 2)};
+  for (int q = 0; q < 3 && h.obst && err.empty(); q++) {
+    kcol[q].resize(kn[q]);
+    if (std::fread(kcol[q].data(), 4, kn[q], f) != kn[q]) err = "truncated at the obstacle columns";
+  }
   std::fclose(f);
   if (!err.empty()) {
     suli_set_error(std::string("suli_checkpoint_load: ") + path + ": " + err);
     return 1;
   }
+  // the checkpoint's obstacle replaces the context's (none: the obstacle is removed)
+  if (h.obst ? suli_set_obstacle(kcol[0].data(), kcol[1].data(), kcol[2].data())
+             : suli_set_obstacle(nullptr, nullptr, nullptr))
+    return 1;
   c.tf_on = h.tf_on != 0;
   c.vol_ini = h.vol_ini; c.vol_ins = h.vol_ins; c.t_now = h.t_now;
   c.force_amp = h.force_amp; c.force_period = h.force_period; c.force_phase = h.force_phase;
@@ -1063,14 +1079,15 @@ extern "C" int suli_checkpoint_load(const char *path) {
 
 // Reference restart file (dados/arquivorestart): list-directed text of the
 // arrays and scalars of restart_salva (11_diversos.f90:319-321), in that
-// order, with the reference extents (0_modules.f90).  Items this library
-// does not model are written as the reference holds them in its
-// configuration here and skipped on reading: the obstacle indices ku, kv, kw
-// (0, no obstacles), the prescribed-boundary planes bxx0..bzzf1 (0, no
-// inflow), d_max, d_min, ls_m (0), the friction planes ub, vb, wb (rebuilt by
-// contorno), prd0 and prd (copies of prd1), the face viscosities (rebuilt by
-// visco) and tf_u, tf_v, tf_w (zeroed by restart_ini when mms_t = 0,
-// 11_diversos.f90:268-273).
+// order, with the reference extents (0_modules.f90).  The obstacle columns
+// ku, kv, kw are written from the context's obstacle (zeros without one)
+// and read back into it through suli_set_obstacle.  Items this library does
+// not model are written as the reference holds them in its configuration
+// here and skipped on reading: the prescribed-boundary planes bxx0..bzzf1
+// (0, no inflow), d_max, d_min, ls_m (0), the friction planes ub, vb, wb
+// (rebuilt by contorno), prd0 and prd (copies of prd1), the face
+// viscosities (rebuilt by visco) and tf_u, tf_v, tf_w (zeroed by
+// restart_ini when mms_t = 0, 11_diversos.f90:268-273).
 namespace {
 struct RItem {
   int field;          // >= 0: a library field; -1: zeros; -2: scalar

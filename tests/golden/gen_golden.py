@@ -10,7 +10,10 @@ the call changed in tests/golden/<case>.npz (float64, Fortran order).
     python tests/golden/gen_golden.py schemes    # Euler, RK3, AB2 (g8tt*_*)
     python tests/golden/gen_golden.py variants   # advection forms, directional LES
     python tests/golden/gen_golden.py obstacle   # bottom obstacle columns (g8obst_*)
+    python tests/golden/gen_golden.py obstacle_neg  # with columns below the bed, -1 and 0 (g8obstneg_*)
     python tests/golden/gen_golden.py geometry   # obstaculo's columns per obst_t (obstaculo_*)
+    python tests/golden/gen_golden.py est        # est's divergence and its maximum (est/est_*)
+    python tests/golden/gen_golden.py periodic   # graddin on the x-periodic grid (per_graddin)
 
 The scheme cases use reference builds with t_tempo = 0, 2, 3
 (0_modules.f90:24) and start from nonzero kept forces fu0, fu1.
@@ -105,7 +108,7 @@ def variants():
             print(case, tag, "changed:", changed)
 
 
-def obstacle_masks(p, seed=0):
+def obstacle_masks(p, seed=0, neg=False):
     """Columns of a bottom obstacle (ku, kv, kw of 0_modules.f90:76-79): a
     seeded bump one to three cells high in the middle of the tank, clear of
     the side walls (contorno(3) reads ls(i, j-1, k) beside a lower column,
@@ -116,22 +119,32 @@ def obstacle_masks(p, seed=0):
         m = np.zeros((nxe, nye))
         m[nxe // 3: 2 * nxe // 3 + 1, 2: nye - 2] = rng.integers(1, top + 1, size=(2 * nxe // 3 + 1 - nxe // 3, nye - 4))
         return m
-    return bump(p.nx + 3, p.ny + 2, 2), bump(p.nx + 2, p.ny + 3, 2), bump(p.nx + 2, p.ny + 2, 3)
+    ms = [bump(p.nx + 3, p.ny + 2, 2), bump(p.nx + 2, p.ny + 3, 2), bump(p.nx + 2, p.ny + 2, 3)]
+    if neg:
+        # obstaculo's columns below the bottom (3_contorno.f90:741-760 gives
+        # -1 where the bed lies under z = 0): ku = kv = -1 and kw = 0 on the
+        # row after the bump, ku = kv = kw = -1 on the row before it
+        for q, m in enumerate(ms):
+            n0, n1 = m.shape
+            m[2 * n0 // 3 + 1, 2: n1 - 2] = 0 if q == 2 else -1
+            m[n0 // 3 - 1, 2: n1 - 2] = -1
+    return tuple(ms)
 
 
-def obstacle():
+def obstacle(neg=False):
     import dataclasses
     p = dataclasses.replace(O.Params(), obst_t=1)
     O.ref_build(p)
     cmds = [("contorno", 1), ("contorno", 2), ("contorno", 3), ("stage", 1), ("step", 2)]
+    case = "g8obstneg" if neg else "g8obst"
     for q, (cmd, arg) in enumerate(cmds):
-        s_in = base_state(p, seed=700 + q)
-        s_in.ku[...], s_in.kv[...], s_in.kw[...] = obstacle_masks(p, seed=q)
+        s_in = base_state(p, seed=(750 if neg else 700) + q)
+        s_in.ku[...], s_in.kv[...], s_in.kw[...] = obstacle_masks(p, seed=q, neg=neg)
         s_out = s_in.copy()
         O.ref_run(s_out, cmd, arg)
         tag = f"{cmd}{arg}"
-        changed = _save(os.path.join(HERE, f"g8obst_{tag}.npz"), p, s_in, s_out)
-        print("g8obst", tag, "changed:", changed)
+        changed = _save(os.path.join(HERE, f"{case}_{tag}.npz"), p, s_in, s_out)
+        print(case, tag, "changed:", changed)
 
 
 # grid of the obstacle-geometry vectors: 3 m x 0.6 m x 0.6 m
@@ -148,6 +161,47 @@ def geometry():
         np.savez_compressed(os.path.join(HERE, "obstaculo", f"obstaculo_{t}.npz"), ku=s.ku, kv=s.kv, kw=s.kw,
                             grid=np.array([p.nx, p.ny, p.nz]), h=np.array([p.dx, p.dy, p.dz]))
         print("obst_t", t, "ku range", s.ku.min(), s.ku.max(), "kw range", s.kw.min(), s.kw.max())
+
+
+def est():
+    """est (4_plot.f90:500-525) on three states per grid: seeded velocities,
+    the same after a reference projection (graddin + posdin, divergence at
+    the solver's residual level), and one face velocity raised at a wall
+    cell.  Stored: the velocities, the divergence field and its maximum."""
+    os.makedirs(os.path.join(HERE, "est"), exist_ok=True)
+    for case, p in CASES.items():
+        O.ref_build(p)
+        for q in range(3):
+            s = base_state(p, seed=900 + q)
+            if q == 1:
+                s.run("contorno", 2)
+                O.ref_run(s, "graddin")
+                O.ref_run(s, "posdin")
+            if q == 2:
+                s.u[1, 1, 1] += 7.0
+            out = s.copy()
+            O.ref_run(out, "est")
+            np.savez_compressed(os.path.join(HERE, "est", f"est_{case}_{q}.npz"), u=s.u, v=s.v, w=s.w,
+                                div=out.kurv, divmax=np.array([out.scal["vol_ins"]]),
+                                grid=np.array([p.nx, p.ny, p.nz]), h=np.array([p.dx, p.dy, p.dz]))
+            print("est", case, q, "max |div|", out.scal["vol_ins"])
+
+
+def periodic():
+    """graddin on the x-periodic grid: the reference's operator is not
+    symmetric across the periodic faces (the face density is copied at both
+    ends, 8_derivadas.f90:318-323), its CG does not reach the stopping test
+    and runs the full 10000 iterations (9_grad.f90:175); the fixture pins the
+    oracle's recurrence over all of them.  Same input as main() would build."""
+    p = CASES["per"]
+    O.ref_build(p)
+    q = COMMANDS.index(("graddin", None))
+    s_in = base_state(p, seed=100 + q)
+    s_in.run("contorno", 2)
+    s_out = s_in.copy()
+    O.ref_run(s_out, "graddin")
+    changed = _save(os.path.join(HERE, "per_graddin.npz"), p, s_in, s_out)
+    print("per graddin changed:", changed)
 
 
 def main():
@@ -198,7 +252,13 @@ if __name__ == "__main__":
         variants()
     elif sys.argv[1:] == ["obstacle"]:
         obstacle()
+    elif sys.argv[1:] == ["obstacle_neg"]:
+        obstacle(neg=True)
     elif sys.argv[1:] == ["geometry"]:
         geometry()
+    elif sys.argv[1:] == ["est"]:
+        est()
+    elif sys.argv[1:] == ["periodic"]:
+        periodic()
     else:
         main()

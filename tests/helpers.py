@@ -28,8 +28,10 @@ def consistent_state(p, seed=1, **kw):
 def obstacle_state(p, seed=1, **kw):
     """consistent_state with a seeded bottom obstacle one to three cells high
     in the middle of the tank, clear of the side walls (ku, kv <= nz,
-    kw <= nz+1 as 3_contorno.f90:741-760 clamps them); p.obst_t must be
-    nonzero."""
+    kw <= nz+1 as 3_contorno.f90:741-760 clamps them), and columns below the
+    bed beside it as obstaculo leaves them where the bed dips under z = 0:
+    ku = kv = -1 with kw = 0 on the row after the bump, all three -1 on the
+    row before it; p.obst_t must be nonzero."""
     assert p.obst_t
     s = O.random_state(p, seed=seed, **kw)
     rng = np.random.default_rng(seed)
@@ -39,6 +41,8 @@ def obstacle_state(p, seed=1, **kw):
         top = 3 if name == "kw" else 2
         a[...] = 0.0
         a[n0 // 3: 2 * n0 // 3 + 1, 2: n1 - 2] = rng.integers(1, top + 1, size=(2 * n0 // 3 + 1 - n0 // 3, n1 - 4))
+        a[2 * n0 // 3 + 1, 2: n1 - 2] = 0 if name == "kw" else -1
+        a[n0 // 3 - 1, 2: n1 - 2] = -1
     s.run("visco")
     s.run("contorno", 1)
     return s

@@ -24,13 +24,33 @@ def _tank(prec):
     return p, sv
 
 
+def _columns(p, seed=3):
+    """Bottom-obstacle columns for the tank: a seeded bump one to three cells
+    high in the middle, and a strip of -1 columns (obstaculo's value for a
+    column without obstacle, 3_contorno.f90:741-760) beside it."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for n0, n1, top in ((p.nx + 3, p.ny + 2, 2), (p.nx + 2, p.ny + 3, 2), (p.nx + 2, p.ny + 2, 3)):
+        k = np.zeros((n0, n1), dtype=np.int32)
+        k[n0 // 3: 2 * n0 // 3, 2: n1 - 2] = rng.integers(1, top + 1, size=(2 * n0 // 3 - n0 // 3, n1 - 4))
+        k[2 * n0 // 3: 2 * n0 // 3 + 2, 2: n1 - 2] = -1
+        out.append(k)
+    return out
+
+
+@pytest.mark.parametrize("obst", [False, True], ids=["plain", "obstacle"])
 @pytest.mark.parametrize("prec", [pysuli.PREC_FP64, pysuli.PREC_MIXED])
-def test_checkpoint_resumes_bit_for_bit(tmp_path, prec):
-    print(f"checkpoint resume 32x16x16 prec={prec}", flush=True)
+def test_checkpoint_resumes_bit_for_bit(tmp_path, prec, obst):
+    """Resume in a fresh context: with an obstacle, the checkpoint alone
+    installs the columns (the fresh context never sees suli_set_obstacle);
+    without one, loading removes the obstacle the fresh context was given."""
+    print(f"checkpoint resume 32x16x16 prec={prec} obstacle={obst}", flush=True)
     dt = 1e-3
     path = str(tmp_path / "run.ckpt")
     p, sv = _tank(prec)
     try:
+        if obst:
+            sv.set_obstacle(*_columns(p))
         sv.init_dambreak(0.3, 0.3)
         sv.contorno(1)
         for _ in range(3):
@@ -44,6 +64,8 @@ def test_checkpoint_resumes_bit_for_bit(tmp_path, prec):
         sv.close()
     p, sv = _tank(prec)
     try:
+        if not obst:  # a checkpoint without an obstacle removes the context's
+            sv.set_obstacle(*_columns(p, seed=5))
         sv.checkpoint_load(path)
         for _ in range(3):
             sv.step(dt)

@@ -105,14 +105,27 @@ def test_contorno(grid, nlock):
 
 
 def test_graddin(grid):
-    if grid.ccx0 == 0:
-        pytest.skip("periodic-x projection: the reference operator is not symmetric across the periodic "
-                    "faces (rho_face copied at both ends) and its CG does not converge")
     st = consistent_state(grid, seed=8)
     st.run("contorno", 2)
     sv = gpu_from_state(st)
-    info = sv.graddin(1e-3)
     st.scal["dt"] = 1e-3
+    if grid.ccx0 == 0:
+        # The reference's periodic-x operator is not symmetric (the face
+        # density is copied at both ends, 8_derivadas.f90:318-323), so its CG
+        # never meets the stopping test and runs 10000 iterations, whose
+        # iterate is round-off dominated.  Both solvers follow the same
+        # recurrence for a fixed 20 iterations instead.
+        import pysuli
+        sv.set_solver(pysuli.POISSON_REF, pysuli.PRECOND_JACOBI, 0.0, 20)
+        st.cg_max_it = 20
+        info = sv.graddin(1e-3)
+        st.run("graddin")
+        assert info["iters"] == st.diag["cg_iters"] == 20, (info, st.diag)
+        errs = _cmp(sv, st, ["prd1"], 1e-10, "graddin, periodic x, 20 iterations")
+        print("periodic graddin, 20 iterations: relative error", errs)
+        sv.close()
+        return
+    info = sv.graddin(1e-3)
     st.run("graddin")
     assert abs(info["iters"] - st.diag["cg_iters"]) <= 1, (info, st.diag)
     tol = 1e-10 if info["iters"] == st.diag["cg_iters"] else 1e-4

@@ -49,6 +49,37 @@ def test_fcg_true_residual(precision, precond, tol):
     assert np.max(np.abs(dd)) <= 1e-5 * np.max(np.abs(x - x.mean())), np.max(np.abs(dd))
 
 
+@pytest.mark.parametrize("precision", [pysuli.PREC_FP64, pysuli.PREC_MIXED])
+@pytest.mark.parametrize("grid", [(64, 32, 32), (128, 64, 64)])
+def test_fcg_true_residual_large(grid, precision):
+    """The FP64 residual check on grids where the block order of the
+    finest-level and Krylov kernels is the XCD-band one (block rows split
+    into 8 bands), the finest level is split by colour and the small levels
+    run fused: the residual recomputed here from the downloaded fields meets
+    the tolerance.  Cold start from p = 0 and the warm start of the state."""
+    nx, ny, nz = grid
+    p = O.Params(nx=nx, ny=ny, nz=nz, dx=1 / nx, dy=1 / nx, dz=1 / nx)
+    st = _stage_state(p, seed=23)
+    for cold in (False, True):
+        if cold:
+            st.prd1[...] = 0.0
+        print(f"FCG-GMG residual {grid} precision {precision} cold {cold}", flush=True)
+        sv = gpu_from_state(st, precision=precision, poisson=pysuli.POISSON_FCG, precond=pysuli.PRECOND_GMG,
+                            tol_rel=1e-8)
+        try:
+            info = sv.graddin(1e-3)
+            pres = sv.download("prd1")
+            rho = sv.download("rho")
+            u, v, w = sv.download("u"), sv.download("v"), sv.download("w")
+        finally:
+            sv.close()
+        A, D, q = assemble_projection(p, rho, u, v, w, 1e-3)
+        rr = scaled_relres(A, D, q, pres[1:-1, 1:-1, 1:-1].ravel(order="F"))
+        print(f"  {info} host relres {rr:.3e}")
+        assert info["converged"] == 1 and info["fallbacks"] == 0, info
+        assert rr <= 1.5e-8, rr
+
+
 def test_gmg_beats_jacobi_iterations():
     p = O.Params(nx=64, ny=32, nz=32, dx=1 / 64, dy=1 / 64, dz=1 / 64)
     st = _stage_state(p, seed=22)

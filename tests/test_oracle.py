@@ -307,3 +307,110 @@ def test_cg_summation_order_sensitivity():
     d = np.max(np.abs(e / sD - po)) / np.max(np.abs(po))
     assert it == a.diag["cg_iters"]
     assert 1e-12 < d < 2e-6, d
+
+
+# ---- est (4_plot.f90:500-525) -------------------------------------------------
+EST = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "est", "*.npz")))
+
+
+@pytest.mark.parametrize("path", EST, ids=[os.path.basename(p)[:-4] for p in EST])
+def test_est_golden(path):
+    """The oracle's max |div u| equals the reference est's maximum bit for
+    bit (seeded velocities, a projected field, one raised wall face)."""
+    d = np.load(path)
+    nx, ny, nz = [int(x) for x in d["grid"]]
+    dx, dy, dz = [float(x) for x in d["h"]]
+    per = os.path.basename(path).startswith("est_per")
+    p = O.Params(nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, **(dict(ccx0=0, ccxf=0, cczf=1) if per else {}))
+    s = O.State(p)
+    s.u[...], s.v[...], s.w[...] = d["u"], d["v"], d["w"]
+    assert s.est_div() == float(d["divmax"][0])
+    assert float(np.max(np.abs(d["div"]))) == float(d["divmax"][0])
+
+
+def test_est_closed_forms():
+    """u = a x, v = b y, w = c z has div = a + b + c in every cell; raising
+    one u face by delta moves the divergence of the two cells beside it by
+    -+delta/dx and leaves every other cell alone.  Unequal spacings catch a
+    swapped dx/dy/dz, a + b + c = 0 makes the raised face the maximum."""
+    p = O.Params(nx=6, ny=5, nz=4, dx=0.125, dy=0.25, dz=0.5)
+    s = O.State(p)
+    xf = (np.arange(p.nx + 3) - 1.0) * p.dx  # u(i) sits at x = (i-1) dx
+    yf = (np.arange(p.ny + 3) - 1.0) * p.dy
+    zf = (np.arange(p.nz + 3) - 1.0) * p.dz
+    a, b, c = 3.0, 1.0, 0.5
+    s.u[...] = a * xf[:, None, None]
+    s.v[...] = b * yf[None, :, None]
+    s.w[...] = c * zf[None, None, :]
+    assert s.est_div() == a + b + c
+    s.u[...] = 1.0 * xf[:, None, None]
+    s.v[...] = 1.0 * yf[None, :, None]
+    s.w[...] = -2.0 * zf[None, None, :]
+    assert s.est_div() == 0.0
+    delta = 0.375
+    s.u[4, 2, 3] += delta  # the face between cells i = 3 and i = 4 of row (j, k) = (2, 3)
+    assert s.est_div() == delta / p.dx
+    s.u[4, 2, 3] -= delta
+    s.w[2, 2, p.nz + 1] -= delta  # top wall face of cell (2, 2, nz)
+    assert s.est_div() == delta / p.dz
+
+
+# ---- dam-break initial level set ----------------------------------------------
+def _segment_distance(px, pz, ax, az, bx, bz):
+    """Euclidean distance from points (px, pz) to the segment a-b."""
+    ex, ez = bx - ax, bz - az
+    t = np.clip(((px - ax) * ex + (pz - az) * ez) / (ex * ex + ez * ez), 0.0, 1.0)
+    return np.hypot(px - (ax + t * ex), pz - (az + t * ez))
+
+
+@pytest.mark.parametrize("width,height", [(0.25, 0.25), (0.3125, 0.40625), (0.2, 0.45)])
+def test_dambreak_phi_distance(width, height):
+    """The initial level set is the signed distance to the column's free
+    faces: the distance from each cell centre to the two segments x = width,
+    0 <= z <= height and z = height, 0 <= x <= width (the faces on walls are
+    not interfaces), positive inside the column, the same on every y row.
+    A swapped width/height, a centre offset of one cell or a Manhattan
+    corner distance fails here."""
+    p = O.Params(nx=32, ny=5, nz=24, dx=1 / 32, dy=0.1, dz=1 / 48)
+    phi = O.dambreak_phi(p, width, height)
+    x = (np.arange(1, p.nx + 1) - 0.5) * p.dx
+    z = (np.arange(1, p.nz + 1) - 0.5) * p.dz
+    X, Z = np.meshgrid(x, z, indexing="ij")
+    d = np.minimum(_segment_distance(X, Z, width, 0.0, width, height),
+                   _segment_distance(X, Z, 0.0, height, width, height))
+    inside = (X < width) & (Z < height)
+    exact = np.where(inside, d, -d)
+    for j in range(p.ny):
+        assert np.allclose(phi[:, j, :], exact, rtol=0, atol=1e-14)
+    # positive exactly at the centres inside the column, zero on the free
+    # faces (a centre on z = height when height/dz is a half integer)
+    assert np.array_equal(np.sign(phi[:, 0, :]), np.sign(exact))
+    assert np.any(inside) and np.any(~inside)
+
+
+def test_dambreak_phi_eikonal():
+    """|grad phi| = 1 by central differences wherever the stencil stays in one
+    smooth region (off the inner bisector x - width = z - height and off
+    the outer corner's edges), and the outside corner distance is the
+    Euclidean one."""
+    p = O.Params(nx=64, ny=2, nz=64, dx=1 / 64, dy=0.5, dz=1 / 64)
+    width, height = 0.5, 0.5
+    phi = O.dambreak_phi(p, width, height)[:, 0, :]
+    h = p.dx
+    gx = (phi[2:, 1:-1] - phi[:-2, 1:-1]) / (2 * h)
+    gz = (phi[1:-1, 2:] - phi[1:-1, :-2]) / (2 * h)
+    g = np.hypot(gx, gz)
+    x = (np.arange(2, p.nx) - 0.5) * h
+    z = (np.arange(2, p.nz) - 0.5) * h
+    X, Z = np.meshgrid(x, z, indexing="ij")
+    smooth = (np.abs((X - width) - (Z - height)) > 2 * h) & (np.abs(X - width) > 2 * h) & (np.abs(Z - height) > 2 * h)
+    corner = (X > width) & (Z > height)
+    # linear pieces (inside, beside the faces): central differences are exact
+    assert np.allclose(g[smooth & ~corner], 1.0, atol=1e-12)
+    # outside corner, phi = -r: |central difference| = 1 + O(h^2/r^2)
+    r = np.hypot(X - width, Z - height)
+    assert np.all(np.abs(g[smooth & corner] - 1.0) <= (h / r[smooth & corner]) ** 2)
+    # outside corner: phi = -sqrt((x-w)^2 + (z-h)^2)
+    i, k = 60, 50  # 0-based, both beyond the column
+    xc, zc = (i + 0.5) * h, (k + 0.5) * h
+    assert phi[i, k] == pytest.approx(-np.hypot(xc - width, zc - height), abs=1e-15)
