@@ -161,6 +161,23 @@ int suli_graddin(double dt, suli_poisson_stats *st);/* graddin, 9_grad.f90:10   
 int suli_posdin(double dt);                        /* posdin, 10_pos.f90:1             */
 int suli_est_div(double *maxdiv);                  /* est, 4_plot.f90:500              */
 
+/* No counterpart in the reference: a variant of suli_est_div for callers
+ * that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  max |div u| of est (4_plot.f90:500-525)
+ * over the caller's DEVICE arrays in the reference extents and Fortran
+ * order, FP64: u(0:nx+2,0:ny+1,0:nz+1), v(0:nx+1,0:ny+2,0:nz+1),
+ * w(0:nx+1,0:ny+1,0:nz+2); the result is written to the device memory
+ * *div_max.  Needs no suli_init; enqueued on `stream` (null: the null
+ * stream) and returns without synchronising.  Allocates nothing: the block
+ * partials go to a static device buffer of 2048 doubles per GPU (16 KiB,
+ * held until the process ends), shared by every call on that GPU, so calls
+ * that overlap on one GPU race.  The first call on a device (or suli_init
+ * on it) looks up the buffer's address: make one of them before capturing
+ * a graph.  Grids: each extent >= 1, ny*nz <= 2^30, velocity arrays of at
+ * most 2^36 elements; others are rejected with an error. */
+int suli_est_div_noalloc(const double *u, const double *v, const double *w, int nx, int ny, int nz,
+                         double dx, double dy, double dz, double *div_max, void *stream);
+
 
 /* Fu, Fv, Fw of convdiff materialised for inspection (suli_convdiff leaves
  * them implicit: the forces are fused into suli_tempo's update). */

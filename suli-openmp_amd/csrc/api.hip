@@ -126,6 +126,7 @@ extern "C" int suli_init(const suli_params *p) {
   if (c->P.tol_rel <= 0) c->P.tol_rel = 1e-8;
   g_ctx = c;
   HIPCHK(hipSetDevice(p->device));
+  if (suli_noalloc_setup(nullptr)) return 1;  // this device's scratch of suli_est_div_noalloc
   HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   c->prec = p->precision;
   c->esz = c->prec == SULI_PREC_FP64 ? 8 : 4;

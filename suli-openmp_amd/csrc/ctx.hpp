@@ -105,6 +105,9 @@ struct Ctx {
 
 extern Ctx *g_ctx;
 void suli_set_error(const std::string &m);
+// address of the current device's scratch buffer of suli_est_div_noalloc
+// (est_noalloc.hip), looked up once per device
+int suli_noalloc_setup(double **out);
 
 #define HIPCHK(x)                                                                    \
   do {                                                                               \

@@ -96,6 +96,7 @@ def lib():
             "suli_visco": ([], i32), "suli_convdiff": ([], i32), "suli_tempo": ([i32, f64], i32),
             "suli_graddin": ([f64, p(PoissonStats)], i32), "suli_posdin": ([f64], i32),
             "suli_est_div": ([p(f64)], i32), "suli_convdiff_forces": ([], i32),
+            "suli_est_div_noalloc": ([C.c_void_p] * 3 + [i32] * 3 + [f64] * 3 + [C.c_void_p, C.c_void_p], i32),
             "suli_stage": ([i32, f64, p(PoissonStats)], i32),
             "suli_step": ([f64, p(LsStats), p(PoissonStats)], i32), "suli_sync": ([], i32),
             "suli_set_vol_ini": ([f64], i32), "suli_get_volumes": ([p(f64), p(f64)], i32),
