@@ -12,29 +12,35 @@ refinement to a 1e-8 FP64 relative residual, geometric-multigrid
 preconditioner.  Synthetic data: the tank is initialised in place.
 
 A "step" is one full time step of 1_principal.f90:74-95 (level set, two RK2
-stages each with visco, convdiff+tempo, contorno, projection, posdin).
-After --warmup untimed steps, --steps steps are timed between two stream
-synchronisations.  value = nx*ny*nz*steps/seconds/1e6 (Mcell-steps/s).
+stages each with visco, convdiff+tempo, contorno, projection, posdin).  The
+run has --total steps (default 1000, 0.2 s of the collapse): --advance
+untimed steps (default 500: the column is falling and the surge runs along
+the floor), --warmup steps of which the last is timed kernel by kernel,
+then --steps timed steps between two stream synchronisations, then the rest
+of the run untimed.  value = nx*ny*nz*steps/seconds/1e6 (Mcell-steps/s).
 
-The last warm-up step is timed kernel by kernel (HIP events around every
-launch) to pick the dominant fine-grid kernel; during the timed steps only
-every 7th launch of that kernel carries events, attached to its dispatches
-(hipExtLaunchKernelGGL; an event pair costs a few microseconds of dispatch
-time), so the other launches run undisturbed.  The JSON
-line carries:
+During the timed steps only every 7th launch of the dominant kernel carries
+events, attached to its dispatches (hipExtLaunchKernelGGL; an event pair
+costs a few microseconds of dispatch time), so the other launches run
+undisturbed.  The JSON line carries:
 - roofline: algorithmic bytes per launch of the dominant kernel / its average
   duration over the timed steps, against 8 TB/s; traffic = HBM bytes per
   launch from rocprof FETCH_SIZE + WRITE_SIZE (profiles/hbm_traffic_*.json,
   made by --pmc-summary from counter runs of this script, scaled by a 256 MiB
   device copy measured the same way);
-- poisson.comparison: ms per solve of one projection state with the plain
-  (Jacobi) FCG, the multigrid FCG and each shipped learned preconditioner
-  (suli-openmp_amd/weights; the default one as "learned"), all to a 1e-8
-  FP64 relative residual;
-- cpu_baseline: the flang-built reference harness (oracle/_ref) timed on one
-  host core on a bounded sample of the same workload (one level_set plus
-  one RK stage from the GPU's post-warmup state; per-step time = level_set
-  + 2 x stage).
+- poisson: GPU ms per solve of the timed steps, and per 100-step window of
+  the whole run; comparison = the projection benchmark set (both RK stages
+  of steps 1, 10, 100, 500, 1000, each warm- and cold-started) solved with
+  the plain (Jacobi) FCG, the multigrid FCG and the default learned
+  preconditioner, and every shipped network on the step-500 state, all to a
+  1e-8 FP64 relative residual;
+- diagnostics: interface volume drift after the timed steps and at the end
+  of the run, max |div u|;
+- cpu_baseline: the flang-built reference harness (-O3, oracle/_ref) timed on
+  one host core on a bounded sample of the same workload (one level_set plus
+  one RK stage from the GPU's state after the timed steps; per-step time =
+  level_set + 2 x stage), with the reference's own CG stopping rule and with
+  the relative 1e-8 rule, with the reference's CG iteration counts.
 """
 import argparse
 import json
@@ -112,14 +118,51 @@ def prof_table(L):
     return out
 
 
-def cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, budget_s):
-    """Time the flang-built reference on a bounded sample of the workload."""
-    import numpy as np
+def _run_harness(exe, fin, budget_s, label):
+    """Run a reference harness 'bench' on one pinned core; its output, or
+    what it printed before the budget ran out, and the wall time."""
+    def _big_stack():  # the reference keeps large automatic arrays on the stack
+        import resource
+        hard = resource.getrlimit(resource.RLIMIT_STACK)[1]
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+    logf = fin + f".{label}.out"
+    t0 = time.time()
+    killed = False
+    with open(logf, "w") as fo:
+        proc = subprocess.Popen(["taskset", "-c", "0", exe, fin, "-", "bench"], stdout=fo, stderr=subprocess.STDOUT,
+                                text=True, env=dict(os.environ, OMP_NUM_THREADS="1"), preexec_fn=_big_stack,
+                                cwd=os.path.dirname(fin))
+        while True:
+            try:
+                proc.wait(timeout=15)
+                break
+            except subprocess.TimeoutExpired:
+                el = time.time() - t0
+                progress(f"CPU baseline ({label}, reference harness, 1 core) running {el:.0f}s")
+                if el > budget_s:
+                    proc.kill()
+                    proc.wait()
+                    killed = True
+                    break
+    out = open(logf).read()
+    os.remove(logf)
+    return out, time.time() - t0, killed, proc.returncode
+
+
+def cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, budget_s, step_label):
+    """Time the flang-built reference (-O3, as its Makefile:7) on a bounded
+    sample of the workload: one level_set plus one RK stage from the GPU's
+    state after the timed steps, once with the reference's own stopping rule
+    (sum r_s^2 <= 1e-4/N, 9_grad.f90:175) and once with the relative rule
+    the GPU solves to (||r_s|| <= 1e-8 ||b_s||).  Per step = level_set + 2
+    stages (the second stage's solve is taken as long as the first's)."""
+    import re
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     p = O.Params(nx=nx, ny=ny, nz=nz, dx=h, dy=h, dz=h)
-    exe = os.path.join(O.REF_DIR, O.ref_tag(p), "harness")
-    if not os.path.exists(exe):
+    exes = {v: os.path.join(O.REF_DIR, O.ref_tag(p) + "_" + v, "harness") for v in ("bench", "bench_rel")}
+    if not os.path.exists(exes["bench"]):
         return None
     st = O.State(p)
     for n in ["u", "v", "w", "prd1", "ls", "nut"]:
@@ -129,60 +172,62 @@ def cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, budget_s):
     fin = os.path.join(tmp, "state.bin")
     st.to_file(fin)
     del st
-    t0 = time.time()
-    def _big_stack():  # the reference keeps large automatic arrays on the stack
-        import resource
-        hard = resource.getrlimit(resource.RLIMIT_STACK)[1]
-        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
-
-    logf = os.path.join(tmp, "harness.out")
+    runs = {}
     try:
-        with open(logf, "w") as fo:
-            proc = subprocess.Popen(["taskset", "-c", "0", exe, fin, "-", "bench"], stdout=fo,
-                                    stderr=subprocess.STDOUT, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"),
-                                    preexec_fn=_big_stack)
-            while True:
-                try:
-                    proc.wait(timeout=15)
-                    break
-                except subprocess.TimeoutExpired:
-                    el = time.time() - t0
-                    progress(f"CPU baseline (reference harness, 1 core) running {el:.0f}s")
-                    if el > budget_s:
-                        proc.kill()
-                        proc.wait()
-                        break
-        out = err = open(logf).read()
+        for v, share in (("bench", 0.4), ("bench_rel", 0.6)):
+            if not os.path.exists(exes[v]):
+                continue
+            out, wall, killed, rc = _run_harness(exes[v], fin, share * budget_s, v)
+            vals = {k: float(x) for k, x in re.findall(r"(\w+)=\s*([-+0-9.Ee]+)", out)}
+            its = [int(x) for x in re.findall(r"REF_CG_ITERS\s+(\d+)", out)]
+            runs[v] = dict(vals=vals, cg_iters=its[0] if its else None, wall=wall, killed=killed, rc=rc,
+                           tail=out.strip()[-200:])
     finally:
-        for f in (fin, logf):
-            if os.path.exists(f):
-                os.remove(f)
+        if os.path.exists(fin):
+            os.remove(fin)
         os.rmdir(tmp)
-    wall = time.time() - t0
-    import re
-    vals = {k: float(v) for k, v in re.findall(r"(\w+)=\s*([-+0-9.Ee]+)", out)}
     info = {"unit": "Mcell-steps/s", "cores": 1, "kind": "reference", "cpu_model": _cpu_model(),
-            "host_nproc": os.cpu_count(), "wall_s": round(wall, 1)}
-    if "level_set_seconds" not in vals:
-        info.update(value=None, sample=(f"reference level_set at {nx}x{ny}x{nz} did not finish within {budget_s:.0f}s "
-                                        f"(rc {proc.returncode}: {(err or '').strip()[-200:]})"))
+            "host_nproc": os.cpu_count(), "build": "flang -O3 -fdefault-real-8 -fdefault-double-8 (Makefile:7)"}
+
+    def per_step(r):
+        vals = r["vals"]
+        if "level_set_seconds" not in vals:
+            return None, None
+        ls_s = vals["level_set_seconds"]
+        if "stage1_seconds" in vals:
+            return ls_s + 2 * vals["stage1_seconds"], False
+        stage_min = max(r["wall"] - ls_s - 5.0, 0.0)  # still running when stopped: a lower bound
+        return ls_s + 2 * stage_min, True
+
+    a = runs.get("bench")
+    step_s, bound = per_step(a)
+    if step_s is None:
+        info.update(value=None, sample=(f"reference level_set at {nx}x{ny}x{nz} did not finish within "
+                                        f"{0.4 * budget_s:.0f}s (rc {a['rc']}: {a['tail']})"))
         return info
-    ls_s = vals["level_set_seconds"]
-    if "stage1_seconds" not in vals:
-        # the stage (with its CG) ran past the budget: the step takes at least
-        # level_set + 2 x (budget left for the stage), so the value is an upper bound
-        stage_min = max(wall - ls_s - 5.0, 0.0)
-        step_min = ls_s + 2 * stage_min
-        info.update(value=nx * ny * nz / step_min / 1e6, bound="upper",
-                    sample=(f"flang -O2 reference harness, 1 core, {nx}x{ny}x{nz} from the GPU post-warmup state: "
-                            f"level_set {ls_s:.1f}s; RK stage 1 still running after {stage_min:.0f}s (stopped); "
-                            f"step >= level_set + 2 x {stage_min:.0f}s = {step_min:.0f}s, so the value is an upper bound"))
-        return info
-    step_s = ls_s + 2 * vals["stage1_seconds"]
+    v = a["vals"]
     info.update(value=nx * ny * nz / step_s / 1e6,
-                sample=(f"flang -O2 reference harness, 1 core: one level_set ({ls_s:.1f}s) + one RK stage "
-                        f"({vals['stage1_seconds']:.1f}s, of which graddin {vals['graddin_seconds']:.1f}s) at "
-                        f"{nx}x{ny}x{nz} from the GPU post-warmup state; step = level_set + 2 stages = {step_s:.1f}s"))
+                sample=(f"flang -O3 reference harness, 1 core (taskset), {nx}x{ny}x{nz} from the GPU state after "
+                        f"{step_label}: one level_set ({v['level_set_seconds']:.1f}s) + one RK stage "
+                        + (f"({v['stage1_seconds']:.1f}s, of which graddin {v['graddin_seconds']:.1f}s in "
+                           f"{a['cg_iters']} CG iterations to the reference's sum r^2 <= 1e-4/N)"
+                           if not bound else "(stopped at the budget)")
+                        + f"; step = level_set + 2 stages = {step_s:.1f}s"),
+                reference_cg_iters=a["cg_iters"], level_set_s=v.get("level_set_seconds"),
+                stage_s=v.get("stage1_seconds"), graddin_s=v.get("graddin_seconds"))
+    if bound:
+        info["bound"] = "upper"
+    r = runs.get("bench_rel")
+    if r is not None:
+        srel, brel = per_step(r)
+        vr = r["vals"]
+        info["relative_1e-8"] = {
+            "value": None if srel is None else nx * ny * nz / srel / 1e6,
+            "bound": "upper" if brel else None,
+            "cg_iters": r["cg_iters"], "stage_s": vr.get("stage1_seconds"), "graddin_s": vr.get("graddin_seconds"),
+            "graddin_ms_per_solve": None if "graddin_seconds" not in vr else round(1e3 * vr["graddin_seconds"], 1),
+            "note": "the same sample with the reference's CG stopped at ||r_s|| <= 1e-8 ||b_s|| (the GPU's rule; "
+                    "build variant bench_rel of oracle/ref/build_ref.sh)"}
     return info
 
 
@@ -203,53 +248,124 @@ def _cpu_model():
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_256x128x128.json")
 
 
-def poisson_comparison(sv, dt, repeats):
-    """ms per solve of the same projection (the first RK stage of the next
-    step, warm-started from the current pressure) with each preconditioner:
-    Jacobi (plain), multigrid, and every shipped learned preconditioner (the
-    default one as "learned", all of them under "learned_by_entry")."""
-    import pysuli
+SNAP_FIELDS = ("u", "v", "w", "rho", "prd1")
+
+
+def step_capturing(sv, dt, t_new):
+    """One time step as suli_step runs it (1_principal.f90:74-95), call by
+    call, keeping each RK stage's projection input (u*, rho, the pressure it
+    starts from) before its solve.  Returns the solves' stats and the two
+    snapshots."""
+    sv.set_time(t_new)
     sv.level_set(dt)
     sv.contorno(3)
-    sv.visco()
-    sv.convdiff()
-    sv.tempo(1, 0.5 * dt)
-    sv.contorno(2)
-    p0 = sv.download("prd1")
-    out = {}
-    cases = [("jacobi", pysuli.PRECOND_JACOBI, None), ("gmg", pysuli.PRECOND_GMG, None)]
+    stats, snaps = [], []
+    for tt in (1, 2):
+        dts = 0.5 * dt if tt == 1 else dt
+        sv.visco()
+        sv.convdiff()
+        sv.tempo(tt, dts)
+        sv.contorno(2)
+        snaps.append({"stage": tt, "dt": dts, **{n: sv.download(n) for n in SNAP_FIELDS}})
+        stats.append(sv.graddin(dts))
+        sv.posdin(dts)
+        sv.contorno(1)
+    return stats, snaps
+
+
+def _solve_record(runs):
+    ms = sorted(r["ms"] for r in runs)
+    r = runs[-1]
+    return {"ms_per_solve": round(ms[len(ms) // 2], 3), "ms_min": round(ms[0], 3), "iters": r["iters"],
+            "outer": r["outer"], "fallbacks": r["fallbacks"], "converged": r["converged"], "relres": r["res"]}
+
+
+def _load_snapshot(sv, snap):
+    for n in SNAP_FIELDS:
+        sv.upload(n, snap[n])
+
+
+def _solve_snapshot(sv, snap, cold, repeats):
+    """Solves of a loaded snapshot (the solve writes only the pressure)."""
+    runs = []
+    for _ in range(repeats):
+        sv.upload("prd1", 0.0 * snap["prd1"] if cold else snap["prd1"])
+        runs.append(sv.graddin(snap["dt"]))
+    return _solve_record(runs)
+
+
+def poisson_comparison(sv, snaps, repeats):
+    """ms per solve of the projection benchmark set: the (u*, rho, p) of both
+    RK stages captured at steps 1, 10, 100, 500 and 1000 of the run, each
+    solved warm (from the pressure the reference would start from,
+    9_grad.f90:121) and cold (from p = 0), to a 1e-8 FP64 relative residual,
+    with plain Jacobi-FCG, multigrid FCG and the default learned
+    preconditioner; every shipped learned preconditioner on the step-500
+    stage-1 warm state.  The solver's own extrapolated guess is off here
+    (suli_set_guess(0)); ms = GPU time between stream events around each
+    solve, median of `repeats`."""
+    import pysuli
+    sv.set_guess(0)
     shipped = [m for m in pysuli.shipped_learned() if sv.nx % (1 << (m["entry"] + m["levels"] - 1)) == 0
                and sv.ny % (1 << (m["entry"] + m["levels"] - 1)) == 0]
-    cases += [(m["name"], pysuli.PRECOND_LEARNED, m) for m in shipped]
-    for name, pc, meta in cases:
-        if meta is not None:
-            pysuli.use_learned(sv, meta)
-        sv.set_solver(pysuli.POISSON_FCG, pc)
-        runs = []
-        for _ in range(repeats):
-            sv.upload("prd1", p0)
-            runs.append(sv.graddin(0.5 * dt))
-        ms = sorted(r["ms"] for r in runs)
-        rec = {"ms_per_solve": round(ms[len(ms) // 2], 3), "ms_min": round(ms[0], 3),
-               "iters": runs[-1]["iters"], "outer": runs[-1]["outer"], "fallbacks": runs[-1]["fallbacks"],
-               "converged": runs[-1]["converged"], "relres": runs[-1]["res"]}
-        if meta is None:
-            out[name] = rec
-            continue
-        rec.update(entry=meta["entry"], levels=meta["levels"], weights=f"suli-openmp_amd/weights/{name}.npy",
-                   plain_over=round(out["jacobi"]["ms_per_solve"] / rec["ms_per_solve"], 3))
-        out.setdefault("learned_by_entry", {})[name] = rec
-        if meta["default"] or "learned" not in out:
-            out["learned"] = rec
+    default = next((m for m in shipped if m["default"]), shipped[0] if shipped else None)
+    cases = [("jacobi", pysuli.PRECOND_JACOBI, None), ("gmg", pysuli.PRECOND_GMG, None)]
+    if default is not None:
+        cases.append(("learned", pysuli.PRECOND_LEARNED, default))
+    table = []
+    totals = {name: {"ms": 0.0, "iters": 0, "fallbacks": 0, "solves": 0, "all_converged": True}
+              for name, _, _ in cases}
+    for snap in snaps:
+        _load_snapshot(sv, snap)
+        for cold in (False, True):
+            row = {"step": snap["step"], "stage": snap["stage"], "start": "cold" if cold else "warm"}
+            for name, pc, meta in cases:
+                if meta is not None:
+                    pysuli.use_learned(sv, meta)
+                sv.set_solver(pysuli.POISSON_FCG, pc)
+                rec = _solve_snapshot(sv, snap, cold, repeats)
+                row[name] = {k: rec[k] for k in ("ms_per_solve", "iters", "outer", "fallbacks", "converged")}
+                t = totals[name]
+                t["ms"] += rec["ms_per_solve"]
+                t["iters"] += rec["iters"]
+                t["fallbacks"] += rec["fallbacks"]
+                t["solves"] += 1
+                t["all_converged"] &= bool(rec["converged"])
+            table.append(row)
+            progress(f"Poisson set: step {row['step']} stage {row['stage']} {row['start']}: "
+                     + ", ".join(f"{n} {row[n]['ms_per_solve']:.2f} ms/{row[n]['iters']} it" for n, _, _ in cases))
+    out = {}
+    for name, t in totals.items():
+        n = max(t["solves"], 1)
+        out[name] = {"ms_per_solve": round(t["ms"] / n, 3), "iters_per_solve": round(t["iters"] / n, 2),
+                     "fallbacks": t["fallbacks"], "solves": t["solves"], "all_converged": t["all_converged"]}
     if "learned" in out:
-        out["plain_over_learned"] = out["learned"]["plain_over"]
+        out["learned"].update(weights=f"suli-openmp_amd/weights/{default['name']}.npy", entry=default["entry"],
+                              levels=default["levels"])
+        out["plain_over_learned"] = round(out["jacobi"]["ms_per_solve"] / out["learned"]["ms_per_solve"], 3)
         out["gmg_over_learned"] = round(out["gmg"]["ms_per_solve"] / out["learned"]["ms_per_solve"], 3)
     out["plain_over_gmg"] = round(out["jacobi"]["ms_per_solve"] / out["gmg"]["ms_per_solve"], 3)
-    out["note"] = ("one first-stage projection of the 256x128x128 state after the timed steps, warm-started "
-                   f"from the current pressure, median of {repeats} solves, FP64 relative residual <= 1e-8; "
-                   "plain = Jacobi-FCG (the reference's symmetric scaling); learned = the default shipped "
-                   "network; entry = the multigrid level a network solves (0 = the whole grid); the entry >= 1 "
-                   "networks were trained on the column geometries of seeds 1-6, not on this one")
+    # every shipped network on one state
+    ref = next((s for s in snaps if s["step"] == 500 and s["stage"] == 1), snaps[len(snaps) // 2])
+    _load_snapshot(sv, ref)
+    sv.set_solver(pysuli.POISSON_FCG, pysuli.PRECOND_JACOBI)
+    plain = _solve_snapshot(sv, ref, False, repeats)
+    by_entry = {}
+    for m in shipped:
+        pysuli.use_learned(sv, m)
+        sv.set_solver(pysuli.POISSON_FCG, pysuli.PRECOND_LEARNED)
+        rec = _solve_snapshot(sv, ref, False, repeats)
+        rec.update(entry=m["entry"], levels=m["levels"], weights=f"suli-openmp_amd/weights/{m['name']}.npy",
+                   plain_over=round(plain["ms_per_solve"] / rec["ms_per_solve"], 3))
+        by_entry[m["name"]] = rec
+    out["learned_by_entry"] = {"state": f"step {ref['step']} stage {ref['stage']} warm", "jacobi": plain, **by_entry}
+    out["set"] = table
+    out["note"] = (f"projection benchmark set: steps {sorted({s['step'] for s in snaps})} x both RK stages, warm "
+                   "(from the current pressure) and cold (p = 0); ms_per_solve = mean over the set of the median "
+                   f"of {repeats} solves (GPU time), FP64 relative residual <= 1e-8; plain = Jacobi-FCG (the "
+                   "reference's symmetric scaling); learned = the default shipped network; entry = the multigrid "
+                   "level a network solves (0 = the whole grid); the entry >= 1 networks were trained on the "
+                   "column geometries of seeds 1-6, not on this one")
     return out
 
 
@@ -345,16 +461,39 @@ def pmc_calibrate():
     print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 5}))
 
 
+def _windows(per_step, width=100):
+    """Solve statistics per window of `width` steps: iterations and GPU ms
+    per solve, fallbacks, largest final residual."""
+    out = []
+    for w0 in range(1, max(per_step) + 1, width):
+        ss = [p for it in range(w0, w0 + width) for p in per_step.get(it, [])]
+        if not ss:
+            continue
+        out.append({"steps": f"{w0}-{min(w0 + width - 1, max(per_step))}", "solves": len(ss),
+                    "iters_per_solve": round(sum(p["iters"] for p in ss) / len(ss), 2),
+                    "ms_per_solve": round(sum(p["ms"] for p in ss) / len(ss), 3),
+                    "outer_per_solve": round(sum(p["outer"] for p in ss) / len(ss), 2),
+                    "fallbacks": sum(p["fallbacks"] for p in ss),
+                    "max_relres": max(p["res"] for p in ss)})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--advance", type=int, default=500,
+                    help="untimed steps before the warm-up: the timed window starts in the collapse of the column")
+    ap.add_argument("--total", type=int, default=1000,
+                    help="steps of the whole run (the steps after the timed window run untimed)")
+    ap.add_argument("--snap-steps", default="1,10,100,500,1000",
+                    help="steps whose two projections join the Poisson benchmark set")
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precond", default="gmg", choices=["jacobi", "gmg", "learned"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-budget", type=float, default=180.0)
+    ap.add_argument("--cpu-budget", type=float, default=240.0)
     ap.add_argument("--no-poisson", action="store_true", help="skip the preconditioner comparison")
     ap.add_argument("--poisson-repeats", type=int, default=3)
     ap.add_argument("--pmc-calibrate", action="store_true",
@@ -390,14 +529,50 @@ def main():
         pysuli.use_learned(sv, shipped[0])
     vol_ini = sv.init_dambreak(width, height)
     sv.contorno(1)
-    progress(f"{nx}x{ny}x{nz} initialised; {args.warmup} warm-up steps")
-    for _ in range(max(args.warmup - 1, 0)):
-        sv.step(dt)
+    ncell = nx * ny * nz
+    prof_step = args.advance + args.warmup           # the step timed kernel by kernel
+    first, last = prof_step + 1, prof_step + args.steps  # the timed window
+    total = max(args.total, last)
+    snap_steps = sorted({int(s) for s in args.snap_steps.split(",") if s.strip()
+                         if 1 <= int(s) <= total and not prof_step <= int(s) <= last})
+    progress(f"{nx}x{ny}x{nz} initialised; steps 1-{prof_step - 1} untimed (snapshots at {snap_steps}), "
+             f"step {prof_step} profiled, steps {first}-{last} timed, run to step {total}")
+    per_step = {}
+    snaps = []
+    t = 0.0
+    run_wall, run_steps = 0.0, 0
+
+    def untimed(it_from, it_to):
+        nonlocal t, run_wall, run_steps
+        seg0, nseg = time.perf_counter(), 0
+        for it in range(it_from, it_to + 1):
+            t = t + dt
+            if it in snap_steps:
+                sv.sync()
+                run_wall += time.perf_counter() - seg0
+                run_steps += nseg
+                ps, ss = step_capturing(sv, dt, t)
+                for s in ss:
+                    s["step"] = it
+                snaps.extend(ss)
+                seg0, nseg = time.perf_counter(), 0
+            else:
+                _, ps = sv.step(dt)
+                nseg += 1
+            per_step[it] = ps
+            if it % 100 == 0:
+                progress(f"step {it}: {ps[0]['iters']}/{ps[1]['iters']} FCG iterations")
+        sv.sync()
+        run_wall += time.perf_counter() - seg0
+        run_steps += nseg
+
+    untimed(1, prof_step - 1)
     # one untimed step with every kernel timed: the per-kernel table and the
     # choice of the dominant kernel (events around every launch slow a step,
     # so the timed steps below record events around that kernel only)
     L.suli_prof_enable(1)
-    sv.step(dt)
+    t = t + dt
+    _, per_step[prof_step] = sv.step(dt)
     sv.sync()
     prof = prof_table(L)
     known = {k: v for k, v in prof.items() if k in BYTES_PER_CELL}
@@ -408,16 +583,18 @@ def main():
     L.suli_prof_select(dname.encode(), 7)
     stats = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for it in range(first, last + 1):
         ls, ps = sv.step(dt)
         stats.append((ls, ps))
+        per_step[it] = ps
     sv.sync()
     t1 = time.perf_counter()
-    progress(f"timed {args.steps} steps in {t1 - t0:.3f}s")
+    for _ in range(first, last + 1):
+        t = t + dt
+    progress(f"timed steps {first}-{last} in {t1 - t0:.3f}s")
     dcount, dms = prof_table(L).get(dname, (0, 0.0))
     L.suli_prof_enable(0)
     secs = t1 - t0
-    ncell = nx * ny * nz
     value = ncell * args.steps / secs / 1e6
     solves = [p for _, ps in stats for p in ps]
     pois_ms = sum(p["ms"] for p in solves) / len(solves)
@@ -434,18 +611,24 @@ def main():
                 "sampling": "HIP events on every 7th launch over the timed steps",
                 "note": "largest fine-grid kernel by total time; coarse multigrid levels (mg_gs, mg_restrict, "
                         "mg_prolong on levels >= 1) are latency-bound and reported under kernels"}
-    vol_now = sv.volumes()[1]
-    maxdiv = sv.est_div()
+    vol_window = sv.volumes()[1]
+    div_window = sv.est_div()
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, args.cpu_budget)
+        cpu = cpu_baseline(sv, nx, ny, nz, h, dt, vol_ini, args.cpu_budget, f"step {last} (t = {t:.4f} s)")
+    untimed(last + 1, total)
+    vol_end = sv.volumes()[1]
+    div_end = sv.est_div()
     if not args.no_poisson:
-        progress("preconditioner comparison")
-    comparison = None if args.no_poisson else poisson_comparison(sv, dt, args.poisson_repeats)
+        progress(f"Poisson benchmark set: {len(snaps)} projections x warm/cold")
+    comparison = None if args.no_poisson or not snaps else poisson_comparison(sv, snaps, args.poisson_repeats)
     sv.close()
-    step_ms = sum(t for _, t in prof.values())
-    kernels = {k: {"launches": c, "ms_total": round(t, 3), "share_of_timed_kernels": round(t / step_ms, 4)}
-               for k, (c, t) in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    step_ms = sum(tk for _, tk in prof.values())
+    kernels = {}
+    for k, (c, tk) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+        kernels[k] = {"launches": c, "ms_total": round(tk, 3), "share_of_timed_kernels": round(tk / step_ms, 4)}
+        if k in BYTES_PER_CELL and tk > 0:
+            kernels[k]["algorithmic_gbs"] = round(BYTES_PER_CELL[k] * ncell * c / (tk * 1e-3) / 1e9, 1)
     line = {
         "metric": "Mcell-steps/s at 256x128x128; Poisson ms/solve to 1e-8, learned vs plain",
         "value": round(value, 3),
@@ -459,19 +642,30 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 (fp64 pressure, residuals, dots, integrals)",
         "data": "synthetic (seeded dam-break tank initialised in place)",
-        "config": {"workload": f"dam break {nx}x{ny}x{nz}, h=1/{nx}, column {width:.4f} x {height:.4f} m (seed {args.seed})",
+        "config": {"workload": (f"dam break {nx}x{ny}x{nz}, h=1/{nx}, column {width:.4f} x {height:.4f} m "
+                                f"(seed {args.seed}); timed steps {first}-{last} of a {total}-step run "
+                                f"(t = {(first - 1) * dt:.4f}-{last * dt:.4f} s of {total * dt:.4f} s)"),
                    "grid": [nx, ny, nz], "dt": dt, "poisson": f"FCG+FP64 refinement to 1e-8, {args.precond}",
-                   "parallelism": "single GPU"},
+                   "timed_steps": [first, last], "run_steps": total, "parallelism": "single GPU"},
         "poisson": {"ms_per_solve": round(pois_ms, 3), "iters_per_solve": round(pois_it, 2),
                     "outer_per_solve": round(sum(p["outer"] for p in solves) / len(solves), 2),
                     "fallbacks": sum(p["fallbacks"] for p in solves),
                     "all_converged": all(p["converged"] for p in solves),
                     "max_relres": max(p["res"] for p in solves), "precond": args.precond,
+                    "ms_note": "GPU time of each solve of the timed steps (stream events at its start and after "
+                               "its last kernel)",
+                    "run_windows": _windows(per_step),
                     "comparison": comparison},
-        "diagnostics": {"volume_drift": (vol_now - vol_ini) / vol_ini, "max_div": maxdiv},
+        "diagnostics": {"volume_drift_after_timed": (vol_window - vol_ini) / vol_ini,
+                        "max_div_after_timed": div_window,
+                        "volume_drift": (vol_end - vol_ini) / vol_ini, "volume_drift_step": total,
+                        "max_div": div_end,
+                        "untimed_steps_ms_per_step": round(1e3 * run_wall / max(run_steps, 1), 3),
+                        "untimed_steps": run_steps},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "kernels": {"note": "one untimed step after the warm-up, every launch timed with HIP events", **kernels},
+        "kernels": {"note": f"step {prof_step}, every launch timed with HIP events; algorithmic_gbs from the "
+                            "algorithmic bytes per cell of the fine-grid kernels", **kernels},
     }
     print(json.dumps(line))
 

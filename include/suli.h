@@ -120,7 +120,8 @@ typedef struct {
   int32_t converged;      /* 1 if the stopping rule was met               */
   double res0;            /* initial residual (REF: sum of squares; FCG: relative 2-norm) */
   double res;             /* final residual, same measure                 */
-  double ms;              /* wall time of the solve                       */
+  double ms;              /* GPU time of the solve: stream events at its
+                             start and after its last kernel             */
 } suli_poisson_stats;
 
 typedef struct {

@@ -296,9 +296,11 @@ def ref_tag(p: Params) -> str:
             + (f"_ob{p.obst_t}" if p.obst_t else ""))
 
 
-def ref_build(p: Params, quiet: bool = True) -> str:
-    """Build (if needed) the reference harness for this configuration."""
-    tag = ref_tag(p)
+def ref_build(p: Params, quiet: bool = True, variant: str = "") -> str:
+    """Build (if needed) the reference harness for this configuration;
+    variant "bench" / "bench_rel" are the CPU-baseline builds of
+    build_ref.sh (REF_VARIANT)."""
+    tag = ref_tag(p) + (f"_{variant}" if variant else "")
     exe = os.path.join(REF_DIR, tag, "harness")
     srcs = [os.path.join(HERE, "ref", f) for f in ("harness.f90", "build_ref.sh")]
     if os.path.exists(exe) and os.path.getmtime(exe) >= max(os.path.getmtime(f) for f in srcs):
@@ -308,7 +310,8 @@ def ref_build(p: Params, quiet: bool = True) -> str:
             str(p.ccx0), str(p.ccxf), str(p.ccy0), str(p.ccyf), str(p.ccz0), str(p.cczf),
             str(p.m_turb), repr(float(p.csmag)), repr(float(p.gx)), repr(float(p.gz)), str(p.t_tempo),
             str(p.der), str(p.adv_type), str(p.obst_t)]
-    subprocess.run(args, check=True, stdout=subprocess.DEVNULL if quiet else None)
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL if quiet else None,
+                   env=dict(os.environ, REF_VARIANT=variant))
     return exe
 
 

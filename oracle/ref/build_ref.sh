@@ -24,6 +24,16 @@
 # 1_principal.f90 (the PROGRAM) is left out; harness.f90 replaces it and
 # calls the reference subroutines in the order of 1_principal.f90:64-108.
 #
+# REF_VARIANT (environment) selects a CPU-baseline build, with the tag
+# suffix _<variant>:
+#   bench      -O3 as the reference's Makefile:7, and graddin prints its CG
+#              iteration count and final residual after its loop
+#              (9_grad.f90:267, a write statement; nothing else changes)
+#   bench_rel  as bench, with the stopping rule of 9_grad.f90:175 replaced by
+#              the relative one sqrt(sum r_s^2) <= 1e-8 ||b_s|| (b_s = q/sqrt(D),
+#              the scaled right-hand side; computed once before the loop)
+# Without it the build is -O2 (the parity harnesses).
+#
 # usage: build_ref.sh TAG NX NY NZ DX DY DZ CCX0 CCXF CCY0 CCYF CCZ0 CCZF MTURB CSMAG GX GZ [TTEMPO [DER ADV_TYPE [OBST_T]]]
 set -euo pipefail
 if [ $# -ne 17 ] && [ $# -ne 18 ] && [ $# -ne 20 ] && [ $# -ne 21 ]; then
@@ -78,12 +88,28 @@ if grep -n -i "^[[:space:]]*use tempo[[:space:]]*$" "$OUT"/src/*.f90; then
   echo "unrenamed 'use tempo' left" >&2; exit 4
 fi
 
-FFLAGS="-O2 -fdefault-real-8 -fdefault-double-8 -mcmodel=medium"
+OPT=-O2
+VARIANT=${REF_VARIANT:-}
+if [ -n "$VARIANT" ]; then
+  G=$OUT/src/9_grad.f90
+  OPT=-O3
+  case "$VARIANT" in
+    bench|bench_rel) ;;
+    *) echo "unknown REF_VARIANT $VARIANT" >&2; exit 2 ;;
+  esac
+  edit "$G" "/^[[:space:]]*! Desnormaliz/i\\		write(*,'(A,I0,A,ES16.8)') 'REF_CG_ITERS ', cont, ' res2=', abs(alfamupr)" "REF_CG_ITERS"
+  if [ "$VARIANT" = bench_rel ]; then
+    edit "$G" "s/^\\([[:space:]]*\\)real(8) :: aux1, aux2, aux3\$/\\1real(8) :: aux1, aux2, aux3, tolrel2/" "aux3, tolrel2"
+    edit "$G" "/^[[:space:]]*do while ((abs(alfamupr) > (0.0001\\/(nx\\*ny\\*nz)))/i\\		tolrel2 = 1e-16 * sum((matqpr/sqrt(matdpr(1:nx,1:ny,1:nz)))**2)" "tolrel2 = 1e-16"
+    edit "$G" "s/(abs(alfamupr) > (0.0001\\/(nx\\*ny\\*nz)))/(abs(alfamupr) > tolrel2)/" "(abs(alfamupr) > tolrel2)"
+  fi
+fi
+FFLAGS="$OPT -fdefault-real-8 -fdefault-double-8 -mcmodel=medium"
 cd "$OUT/obj"
 for f in 0_modules 2_iniciais 3_contorno 4_plot 5_viscosidade 6_convdiff 7_advectivo \
          8_derivadas 9_grad 10_pos 11_diversos 12_paraview_3D 13_mms 14_level_set; do
   "$FC" $FFLAGS -c "$OUT/src/$f.f90" -o "$f.o"
 done
 "$FC" $FFLAGS -c "$HERE/harness.f90" -o harness.o
-"$FC" -O2 -mcmodel=medium -o "$OUT/harness" ./*.o
-echo "built $OUT/harness (nx=$NX ny=$NY nz=$NZ)"
+"$FC" $OPT -mcmodel=medium -o "$OUT/harness" ./*.o
+echo "built $OUT/harness (nx=$NX ny=$NY nz=$NZ${VARIANT:+, variant $VARIANT})"

@@ -48,6 +48,8 @@ static void free_all(Ctx &c) {
   prof_flush(c);
   for (hipEvent_t e : c.prof.pool) (void)hipEventDestroy(e);
   c.prof.pool.clear();
+  for (hipEvent_t &e : c.solve_ev)
+    if (e) (void)hipEventDestroy(e), e = nullptr;
   for (void *p : c.allocs) (void)hipFree(p);
   c.allocs.clear();
   if (c.ctl_h) (void)hipHostFree(c.ctl_h);
@@ -55,6 +57,15 @@ static void free_all(Ctx &c) {
   if (c.out_host) (void)hipHostFree(c.out_host);
   if (c.out_dev) (void)hipFree(c.out_dev);
   if (c.st) (void)hipStreamDestroy(c.st);
+}
+
+int solve_elapsed_ms(Ctx &c, double *ms) {
+  HIPCHK(hipEventRecord(c.solve_ev[1], c.st));
+  HIPCHK(hipEventSynchronize(c.solve_ev[1]));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, c.solve_ev[0], c.solve_ev[1]));
+  *ms = t;
+  return 0;
 }
 
 extern "C" int suli_finalize(void) {
@@ -164,6 +175,8 @@ extern "C" int suli_init(const suli_params *p) {
   if (dalloc(*c, (void **)&c->res, RS_NSLOT * sizeof(double))) return 1;
   HIPCHK(hipHostMalloc((void **)&c->ctl_h, sizeof(CGCtl), hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void **)&c->res_h, RS_NSLOT * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipEventCreate(&c->solve_ev[0]));
+  HIPCHK(hipEventCreate(&c->solve_ev[1]));
   // density and viscosity start as the light fluid until a level set is set
   if (fcg_create(*c)) return 1;
   HIPCHK(hipStreamSynchronize(c->st));

@@ -99,6 +99,9 @@ struct Ctx {
   int64_t out_cap = 0, out_host_cap = 0;
   std::vector<void *> allocs;
   Prof prof;
+  // GPU time of a projection solve: events recorded on the stream at the
+  // solve's start and after its last kernel (suli_poisson_stats.ms)
+  hipEvent_t solve_ev[2] = {nullptr, nullptr};
 
   RedSlot slot(int off, int ctr) const { (void)ctr; return RedSlot{partials, res + off}; }
 };
@@ -182,6 +185,8 @@ struct ProfKernel {
 
 // Poisson solvers (poisson_ref.hip, poisson_fcg.hip)
 int poisson_ref_solve(Ctx &c, double dt, suli_poisson_stats *st);
+// milliseconds between the two solve events (the second recorded and waited for here)
+int solve_elapsed_ms(Ctx &c, double *ms);
 int fcg_create(Ctx &c);
 void fcg_destroy(Ctx &c);
 int fcg_solve(Ctx &c, double dt, suli_poisson_stats *st);
