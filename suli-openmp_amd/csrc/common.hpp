@@ -75,9 +75,10 @@ __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
 // ---------------------------------------------------------------------------
 // Deterministic block reductions in FP64.
 //
-// A kernel that produces a global sum writes one partial per block; the last
-// block to finish (counted with a global atomic) adds the partials in block
-// order and stores the result, so the sum is the same on every run.
+// A kernel that produces a global sum writes one partial per block (no
+// atomics); a one-work-group finish kernel (k_finish below) then adds the
+// partials in block order and stores the result, so the sum is the same on
+// every run.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

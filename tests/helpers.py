@@ -145,3 +145,41 @@ def dambreak_diagnostics(p, u, v, w, ls, rho):
     ke = float(np.sum(0.5 * rho * (uc * uc + vc * vc + wc * wc)) * dV)
     return {"volume": float(np.count_nonzero(ls > 0) * dV), "front": front, "ke": ke,
             "umax": float(max(np.abs(u).max(), np.abs(v).max(), np.abs(w).max()))}
+
+
+def stencil_relres(p, rho, u, v, w, dt, pres):
+    """||D^-1/2 (q - A p)|| / ||D^-1/2 q|| of the projection system of
+    graddin (9_grad.f90:46-151, non-periodic walls) by array stencils,
+    without assembling a matrix: for grids too large for the sparse
+    assembly of assemble_projection.  pres holds the ghosted pressure
+    (0:nx+1, 0:ny+1, 0:nz+1); only its interior is used (walls are Neumann:
+    the wall faces carry no flux)."""
+    rho = np.asarray(rho, dtype=np.float64)
+
+    def face_rho(axis):
+        a = np.moveaxis(rho, axis, 0)
+        n = a.shape[0]
+        f = np.empty((n + 1,) + a.shape[1:])
+        f[1:n] = 0.5 * (a[1:] + a[:-1])
+        f[0] = f[1]
+        f[n] = f[n - 1]
+        return np.moveaxis(f, 0, axis)
+
+    pc = np.asarray(pres, dtype=np.float64)[1:-1, 1:-1, 1:-1]
+    q = ((u[1:-2, 1:-1, 1:-1] - u[2:-1, 1:-1, 1:-1]) / p.dx + (v[1:-1, 1:-2, 1:-1] - v[1:-1, 2:-1, 1:-1]) / p.dy
+         + (w[1:-1, 1:-1, 1:-2] - w[1:-1, 1:-1, 2:-1]) / p.dz)
+    Ap = np.zeros_like(pc)
+    D = np.zeros_like(pc)
+    for axis, h in ((0, p.dx), (1, p.dy), (2, p.dz)):
+        c = dt / h**2 / face_rho(axis)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[axis], hi[axis] = slice(0, -1), slice(1, None)
+        D += c[tuple(hi)] + c[tuple(lo)]
+        cin = np.take(c, np.arange(1, c.shape[axis] - 1), axis=axis)  # interior faces
+        flux = cin * (pc[tuple(lo)] - pc[tuple(hi)])  # c_f (p_low - p_high)
+        Ap[tuple(lo)] += flux
+        Ap[tuple(hi)] -= flux
+        del c, cin, flux
+    s = 1.0 / np.sqrt(D)
+    return float(np.linalg.norm(s * (q - Ap)) / np.linalg.norm(s * q))

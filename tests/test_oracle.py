@@ -414,3 +414,17 @@ def test_dambreak_phi_eikonal():
     i, k = 60, 50  # 0-based, both beyond the column
     xc, zc = (i + 0.5) * h, (k + 0.5) * h
     assert phi[i, k] == pytest.approx(-np.hypot(xc - width, zc - height), abs=1e-15)
+
+
+def test_stencil_relres_matches_assembly():
+    """The matrix-free residual of the large-grid tests equals the one of
+    the sparse assembly (itself checked against the dense operator above)."""
+    from helpers import assemble_projection, scaled_relres, stencil_relres
+    p = O.Params(nx=12, ny=9, nz=7, dx=1 / 12, dy=1 / 10, dz=1 / 8)
+    s = O.random_state(p, seed=4)
+    rng = np.random.default_rng(1)
+    pres = rng.standard_normal(s.prd1.shape)
+    A, D, q = assemble_projection(p, s.rho, s.u, s.v, s.w, 1e-3)
+    want = scaled_relres(A, D, q, pres[1:-1, 1:-1, 1:-1].ravel(order="F"))
+    got = stencil_relres(p, s.rho, s.u, s.v, s.w, 1e-3, pres)
+    assert got == pytest.approx(want, rel=1e-12)
