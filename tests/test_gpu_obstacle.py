@@ -8,7 +8,7 @@ output's kaux and the restart file's columns.
 
 Tolerances as elsewhere: boundary conditions exact, routines 1e-12
 relative (FP64), steps 2e-5 (1e-4 if a reference CG solve stops one
-iteration apart), mixed precision 1e-3.
+iteration apart), mixed precision 2e-4 (10x the measured 2e-5).
 """
 import dataclasses
 
@@ -115,7 +115,8 @@ def test_steps_obstacle_mixed():
         st.run("step", 3)
         vmax = max(float(np.max(np.abs(st.arr(n)))) for n in VEL)
         ev = max(float(np.max(np.abs(sv.download(n) - st.arr(n)))) for n in VEL) / vmax
-        assert ev <= 1e-3, ev
+        print(f"mixed-precision velocity error {ev:.2e}", flush=True)
+        assert ev <= 2e-4, ev  # measured 2.0e-5: about 10x
     finally:
         sv.close()
 

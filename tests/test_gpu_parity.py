@@ -4,7 +4,8 @@ oracle (itself bit-exact with the flang-built reference) on the same inputs.
 FP64 path: per-routine tolerance 1e-12 relative to the field's max norm
 (the kernels evaluate the reference's expressions in its order without FMA;
 the remaining differences come from the GPU's sin/cos/sqrt rounding and the
-order of the FP64 reductions).  Mixed path (FP32 stencils): 1e-4 relative.
+order of the FP64 reductions).  Mixed path (FP32 stencils, three steps):
+level set 3e-6, pressure 1e-5, velocity 1e-4 of the largest speed.
 """
 import numpy as np
 import pytest
@@ -202,11 +203,14 @@ def test_mixed_precision_step():
     for _ in range(3):
         sv.step(1e-3)
         st.run("step", 1)
-    errs = _cmp(sv, st, ["ls", "hs", "rho"], 1e-5, "mixed 3 steps (level set)")
-    errs.update(_cmp(sv, st, ["prd1"], 1e-3, "mixed 3 steps (pressure)"))
+    # measured: level set 2.5e-7, pressure 1.1e-6, velocity 8.2e-6 (of the
+    # largest speed); the bars are about 10x those (the pressure and the
+    # velocity differ from the oracle mostly by where the reference CG stops)
+    errs = _cmp(sv, st, ["ls", "hs", "rho"], 3e-6, "mixed 3 steps (level set)")
+    errs.update(_cmp(sv, st, ["prd1"], 1e-5, "mixed 3 steps (pressure)"))
     errs["velocity"] = _vel_err(sv, st)
-    assert errs["velocity"] <= 1e-3, errs
     print("mixed-precision relative errors after 3 steps:", errs)
+    assert errs["velocity"] <= 1e-4, errs
     sv.close()
 
 

@@ -6,7 +6,7 @@ scheme bit for bit (tests/golden/g8tt*_*.npz).
 FP64 tolerances as for RK2: 1e-12 relative for one tempo call (the forces
 differ from the oracle's by the GPU's sqrt rounding only); whole steps to
 the reference CG's stopping point, 2e-5 (1e-4 if a solve stops one CG
-iteration apart).  Mixed precision: 1e-3.
+iteration apart).  Mixed precision: 4e-4 (10x the largest measured, 3.5e-5).
 """
 import dataclasses
 
@@ -139,7 +139,8 @@ def test_step_scheme_mixed_fcg(scheme):
         st.run("step", 3)
         vmax = max(float(np.max(np.abs(st.arr(n)))) for n in VEL)
         ev = max(float(np.max(np.abs(sv.download(n) - st.arr(n)))) for n in VEL) / vmax
-        assert ev <= 1e-3, ev
+        print(f"mixed-precision velocity error {ev:.2e}", flush=True)
+        assert ev <= 4e-4, ev  # measured 1e-5 to 3.5e-5: about 10x
         _cmp(sv, st, ["ls", "rho"], 1e-5, f"mixed scheme {scheme}")
     finally:
         sv.close()

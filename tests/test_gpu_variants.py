@@ -6,7 +6,7 @@ antissim (7_advectivo.f90:116-653), m_turb = 2 (5_viscosidade.f90:104-170).
 
 FP64 tolerances as for the default scheme: 1e-12 relative for one routine,
 2e-5 (1e-4 when a reference CG solve stops one iteration apart) for whole
-stages and steps; mixed precision 1e-3.
+stages and steps; mixed precision 3e-4 (10x the measured 2.5e-5).
 """
 import dataclasses
 
@@ -113,7 +113,8 @@ def test_steps_variant_mixed(name):
         st.run("step", 3)
         vmax = max(float(np.max(np.abs(st.arr(n)))) for n in VEL)
         ev = max(float(np.max(np.abs(sv.download(n) - st.arr(n)))) for n in VEL) / vmax
-        assert ev <= 1e-3, ev
+        print(f"mixed-precision velocity error {ev:.2e}", flush=True)
+        assert ev <= 3e-4, ev  # measured 2.0e-5 to 2.5e-5: about 10x
     finally:
         sv.close()
 
