@@ -134,6 +134,12 @@ __global__ __launch_bounds__(256) void k_r32(Grid g, const double *__restrict__ 
 // overall tolerance needs), from the outer residual of this pass (rout) and
 // ||b|| (bb) already on the device; when that residual meets the tolerance
 // the pass does not run (done).
+#ifndef SULI_FCG_ONEPASS_WANT
+#define SULI_FCG_ONEPASS_WANT 0.03  // one pass when it needs at least this fraction of the usual reduction
+#endif
+#ifndef SULI_FCG_ONEPASS_FLOOR
+#define SULI_FCG_ONEPASS_FLOOR 0.1  // and then aims for at most this fraction of it
+#endif
 struct EpiR32 {
   FcgCtl *ctl;
   double rel_in, tol;
@@ -146,7 +152,7 @@ struct EpiR32 {
     // one pass of at most 10x the usual reduction when that ends the solve
     // (a warm start within 1e-3 of the tolerance's residual); cold starts
     // take passes of rel_in, which cost fewer iterations in FP32
-    const double floor = want >= 0.03 * rel_in ? 0.1 * rel_in : rel_in;
+    const double floor = want >= SULI_FCG_ONEPASS_WANT * rel_in ? SULI_FCG_ONEPASS_FLOOR * rel_in : rel_in;
     const double rin = fmax(floor, fmin(want, 1.0));
     ctl->thr_in = res[0] * rin * rin;
     ctl->maxit_in = maxit;
@@ -964,6 +970,7 @@ struct MgSmall {
   int s, L, nu, coarse_sweeps;
 };
 
+
 struct SmallLv {
   const float *cx, *cy, *cz, *dg;
   float *b, *x;
@@ -974,6 +981,7 @@ __device__ __forceinline__ void cell_of(const Grid &g, int t, int &i, int &j, in
   j = 1 + (t / g.nx) % g.ny;
   k = 1 + t / (g.nx * g.ny);
 }
+
 
 // A thread's first cell pair on a small level: row position ti (cells 2ti+1
 // and 2ti+2) in row (j, k), computed once per level; further pairs (levels of
@@ -1005,6 +1013,66 @@ __device__ __forceinline__ void small_gs(const Grid &g, const SmallLv &v, const 
     v.x[c] = (v.b[c] + s) / v.dg[c];
   }
   __syncthreads();
+}
+
+// A cell of a small level held in registers for its sweeps: its frame
+// offset, six face conductances, the reciprocal of its diagonal and its
+// right-hand side, which do not change while the level is smoothed; a half
+// sweep then reads only the neighbours' x (in k_mg_gs's order of terms).
+struct RegCell {
+  uint32_t c;
+  float f[6], inv, b;
+  bool ok;
+};
+__device__ __forceinline__ RegCell reg_cell(const Grid &g, const SmallLv &v, int t, int color) {
+  RegCell r;
+  const int hx = (g.nx + 1) >> 1, tot = hx * g.ny * g.nz;
+  const int ti = t % hx, j = 1 + (t / hx) % g.ny, k = 1 + t / (hx * g.ny);
+  const int i = 1 + 2 * ti + ((1 + j + k + color) & 1);
+  r.ok = t < tot && i <= g.nx;
+  r.c = r.ok ? g.idx(i, j, k) : g.idx(1, 1, 1);
+  const uint32_t c = r.c;
+  r.f[0] = v.cx[c + 1];
+  r.f[1] = v.cx[c];
+  r.f[2] = v.cy[c + g.sx];
+  r.f[3] = v.cy[c];
+  r.f[4] = v.cz[c + g.sz];
+  r.f[5] = v.cz[c];
+  r.inv = 1.f / v.dg[c];
+  r.b = v.b[c];
+  return r;
+}
+__device__ __forceinline__ void reg_update(const Grid &g, float *x, const RegCell &r, bool zero_x) {
+  if (!r.ok) return;
+  const uint32_t c = r.c;
+  const float s = zero_x ? 0.f
+                         : r.f[0] * x[c + 1] + r.f[1] * x[c - 1] + r.f[2] * x[c + g.sx] + r.f[3] * x[c - g.sx] +
+                               r.f[4] * x[c + g.sz] + r.f[5] * x[c - g.sz];
+  x[c] = (r.b + s) * r.inv;
+}
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// nu red-black sweeps of a small level in LDS with register-held cells:
+// one pair of cells per thread (the level has at most blockDim pairs)
+__device__ __forceinline__ void small_sweeps_reg(const Grid &g, const SmallLv &v, int nu, bool from_zero,
+                                                 bool black_first) {
+  const RegCell r0 = reg_cell(g, v, threadIdx.x, 0), r1 = reg_cell(g, v, threadIdx.x, 1);
+  for (int sw = 0; sw < nu; sw++) {
+    if (!black_first) {
+      reg_update(g, v.x, r0, from_zero && sw == 0);
+      __syncthreads();
+      reg_update(g, v.x, r1, false);
+    } else {
+      reg_update(g, v.x, r1, false);
+      __syncthreads();
+      reg_update(g, v.x, r0, false);
+    }
+    __syncthreads();
+  }
 }
 
 template <bool LDS>
@@ -1039,10 +1107,14 @@ __global__ __launch_bounds__(1024) void k_mg_small(MgSmall m, const int *active)
   }
   for (int l = m.s; l < m.L - 1; l++) {
     const SmallLv v = small_lv<LDS>(m, l, smem), vc = small_lv<LDS>(m, l + 1, smem);
-    const SmallCell sc = small_cell(m.g[l]);
-    for (int sw = 0; sw < m.nu; sw++) {
-      small_gs(m.g[l], v, sc, 0, sw == 0);
-      small_gs(m.g[l], v, sc, 1, false);
+    if (LDS && ((m.g[l].nx + 1) >> 1) * m.g[l].ny * m.g[l].nz <= (int)blockDim.x) {
+      small_sweeps_reg(m.g[l], v, m.nu, true, false);
+    } else {
+      const SmallCell sc = small_cell(m.g[l]);
+      for (int sw = 0; sw < m.nu; sw++) {
+        small_gs(m.g[l], v, sc, 0, sw == 0);
+        small_gs(m.g[l], v, sc, 1, false);
+      }
     }
     const Grid &gf = m.g[l], &gc = m.g[l + 1];
     const int nc = gc.nx * gc.ny * gc.nz;
@@ -1061,10 +1133,34 @@ __global__ __launch_bounds__(1024) void k_mg_small(MgSmall m, const int *active)
   {
     const int lc = m.L - 1;
     const SmallLv v = small_lv<LDS>(m, lc, smem);
-    const SmallCell sc = small_cell(m.g[lc]);
-    for (int sw = 0; sw < m.coarse_sweeps; sw++) {
-      small_gs(m.g[lc], v, sc, 0, sw == 0);
-      small_gs(m.g[lc], v, sc, 1, false);
+    const Grid &g = m.g[lc];
+    if (LDS && ((g.nx + 1) >> 1) * g.ny * g.nz <= 128) {
+      // the coarsest level's sweeps on one wavefront, up to two pairs of
+      // cells per lane held in registers: within a wavefront the half sweeps
+      // need no work-group barrier (the lanes' LDS accesses are ordered by
+      // wavefront-scope fences)
+      if (threadIdx.x < 64) {
+        RegCell r[2][2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+          r[q][0] = reg_cell(g, v, threadIdx.x + 64 * q, 0);
+          r[q][1] = reg_cell(g, v, threadIdx.x + 64 * q, 1);
+        }
+        for (int sw = 0; sw < m.coarse_sweeps; sw++)
+#pragma unroll
+          for (int color = 0; color < 2; color++) {
+#pragma unroll
+            for (int q = 0; q < 2; q++) reg_update(g, v.x, r[q][color], color == 0 && sw == 0);
+            wave_lds_sync();
+          }
+      }
+      __syncthreads();
+    } else {
+      const SmallCell sc = small_cell(g);
+      for (int sw = 0; sw < m.coarse_sweeps; sw++) {
+        small_gs(g, v, sc, 0, sw == 0);
+        small_gs(g, v, sc, 1, false);
+      }
     }
   }
   for (int l = m.L - 2; l >= m.s; l--) {
@@ -1077,10 +1173,14 @@ __global__ __launch_bounds__(1024) void k_mg_small(MgSmall m, const int *active)
       v.x[gf.idx(i, j, k)] += vc.x[gc.idx((i + 1) >> 1, (j + 1) >> 1, (k + 1) >> 1)];
     }
     __syncthreads();
-    const SmallCell sc = small_cell(gf);
-    for (int sw = 0; sw < m.nu; sw++) {
-      small_gs(gf, v, sc, 1, false);
-      small_gs(gf, v, sc, 0, false);
+    if (LDS && ((gf.nx + 1) >> 1) * gf.ny * gf.nz <= (int)blockDim.x) {
+      small_sweeps_reg(gf, v, m.nu, false, true);
+    } else {
+      const SmallCell sc = small_cell(gf);
+      for (int sw = 0; sw < m.nu; sw++) {
+        small_gs(gf, v, sc, 1, false);
+        small_gs(gf, v, sc, 0, false);
+      }
     }
   }
   if (LDS) {

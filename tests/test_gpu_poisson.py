@@ -168,3 +168,4 @@ def test_extrapolated_guess_saves_iterations():
     for q in (2, 3):
         a, b = res[1][q], res[0][q]
         assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
+
