@@ -116,12 +116,14 @@ extern "C" int suli_init(const suli_params *p) {
     return 1;
   }
   {
-    // frame offsets are 32-bit (fidx): the largest frame field must stay below 2^31 elements
+    // frame offsets are 32-bit (fidx) and the stencils read through 32-bit
+    // byte offsets (Fld, common.hpp): a frame field must stay below 2^29
+    // elements (2^32 bytes in FP64)
     const int64_t sx = ((p->nx + 3 + 31) / 32) * 32;
     const int64_t tot = 31 + sx * (p->ny + 3) * (int64_t)(p->nz + 3) + 64;
-    if (tot >= (int64_t(1) << 31)) {
+    if (tot >= (int64_t(1) << 29)) {
       suli_set_error("suli_init: grid too large: a field frame of " + std::to_string(tot) +
-                     " elements exceeds the 2^31 of the 32-bit frame offsets");
+                     " elements exceeds the 2^29 of the 32-bit byte offsets of the stencils");
       return 1;
     }
   }

@@ -37,7 +37,7 @@ struct Grid {
   int sx, sy;        // pitches in i and j
   uint32_t sz;       // sx*sy (k stride)
   uint32_t off;      // offset of (0,0,0)
-  int64_t total;     // elements per frame field (below 2^31: checked at set-up)
+  int64_t total;     // elements per frame field (below 2^29: checked at set-up)
   double dx, dy, dz;
   int ccx0, ccxf, ccy0, ccyf, ccz0, cczf;
   // Frame offsets are 32-bit unsigned: with a uniform base pointer the loads
@@ -46,6 +46,18 @@ struct Grid {
   // final offset of every access lies in the frame.
   __host__ __device__ __forceinline__ fidx idx(int i, int j, int k) const {
     return off + (fidx)i + (fidx)sx * ((fidx)j + (fidx)sy * (fidx)k);
+  }
+};
+
+// A frame field read through 32-bit byte offsets: base + zext(c * sizeof(T)).
+// Frame fields hold fewer than 2^29 elements (checked at set-up), so the byte
+// offset fits in 32 bits, and the load takes the scalar-base + 32-bit-offset
+// form (one VGPR per address instead of a 64-bit address pair).
+template <typename T>
+struct Fld {
+  const T *__restrict__ p;
+  __device__ __forceinline__ T operator[](fidx c) const {
+    return *(const T *)((const char *)p + (uint32_t)(c * (uint32_t)sizeof(T)));
   }
 };
 
@@ -255,12 +267,13 @@ __device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5) {
 template <typename T>
 __device__ __forceinline__ void window7(const T *__restrict__ f, fidx c, fidx st, int pos,
                                         int n, int ihs, T (&w)[7]) {
+  const Fld<T> F{f};  // 32-bit byte offsets from the field's base
   if (ihs == 0) {
 #pragma unroll
     for (int m = 0; m < 7; m++) {
       int q = pos - 3 + m;
       int qs = q < 1 ? q + n - 1 : (q > n ? q - n + 1 : q);
-      w[m] = f[c + (fidx)(qs - pos) * st];
+      w[m] = F[c + (fidx)(qs - pos) * st];
     }
     return;
   }
@@ -268,11 +281,11 @@ __device__ __forceinline__ void window7(const T *__restrict__ f, fidx c, fidx st
   for (int m = 0; m < 7; m++) {
     int q = pos - 3 + m;
     int qc = q < 1 ? 1 : (q > n ? n : q);
-    w[m] = f[c + (fidx)(qc - pos) * st];
+    w[m] = F[c + (fidx)(qc - pos) * st];
   }
   if (pos <= 3) {
-    const T p1 = f[c + (fidx)(1 - pos) * st], p2 = f[c + (fidx)(2 - pos) * st],
-            p3 = f[c + (fidx)(3 - pos) * st];
+    const T p1 = F[c + (fidx)(1 - pos) * st], p2 = F[c + (fidx)(2 - pos) * st],
+            p3 = F[c + (fidx)(3 - pos) * st];
     T g0, gm1, gm2;
     if (ihs == 1) {
       g0 = T(1.) / T(5.) * (T(12.) * p1 - T(9.) * p2 + T(2.) * p3);
@@ -290,8 +303,8 @@ __device__ __forceinline__ void window7(const T *__restrict__ f, fidx c, fidx st
     }
   }
   if (pos >= n - 2) {
-    const T pn = f[c + (fidx)(n - pos) * st], pn1 = f[c + (fidx)(n - 1 - pos) * st],
-            pn2 = f[c + (fidx)(n - 2 - pos) * st];
+    const T pn = F[c + (fidx)(n - pos) * st], pn1 = F[c + (fidx)(n - 1 - pos) * st],
+            pn2 = F[c + (fidx)(n - 2 - pos) * st];
     const T g1 = T(1.) / T(11.) * (T(18.) * pn - T(9.) * pn1 + T(2.) * pn2);
     const T g2 = T(1.) / T(11.) * (T(18.) * g1 - T(9.) * pn + T(2.) * pn1);
     const T g3 = T(1.) / T(11.) * (T(18.) * g2 - T(9.) * g1 + T(2.) * pn);

@@ -286,50 +286,52 @@ __device__ __forceinline__ void stage_update(int mode, T dt, fidx c, T uc, T F, 
   }
 }
 
-template <typename T>
-__device__ __forceinline__ T interp_face(const T *__restrict__ a, fidx cf, fidx st, int f,
+// IN: the caller guarantees 2 <= f <= n (interior), so the clamps of the
+// end faces and cells are no-ops and left out (the same values are read)
+template <typename T, bool IN = false, typename P = const T *>
+__device__ __forceinline__ T interp_face(P a, fidx cf, fidx st, int f,
                                          int n) {
   // interp*_cf at face f (8_derivadas.f90:304-434): average of cells fc-1, fc
   // with fc = clamp(f, 2, n); cf is the address of face f's cell (f).
-  const int fc = f < 2 ? 2 : (f > n ? n : f);
+  const int fc = IN ? f : (f < 2 ? 2 : (f > n ? n : f));
   const fidx cc = cf + (fidx)(fc - f) * st;
   return (a[cc] + a[cc - st]) * T(0.5);
 }
-template <typename T>
-__device__ __forceinline__ T cell_clamp(const T *__restrict__ a, fidx cf, fidx st, int f,
+template <typename T, bool IN = false, typename P = const T *>
+__device__ __forceinline__ T cell_clamp(P a, fidx cf, fidx st, int f,
                                         int n) {
-  const int fc = f < 1 ? 1 : (f > n ? n : f);
+  const int fc = IN ? f : (f < 1 ? 1 : (f > n ? n : f));
   return a[cf + (fidx)(fc - f) * st];
 }
 // face viscosity along d1 at (face f1 along d1), e.g. xnut(i,j,k) = ls_nux + nutx
-template <typename T>
-__device__ __forceinline__ T face_visc(const T *__restrict__ lsnu, const T *__restrict__ nut,
+template <typename T, bool IN = false, typename P = const T *>
+__device__ __forceinline__ T face_visc(P lsnu, P nut,
                                        fidx cf, fidx st, int f, int n) {
-  return interp_face<T>(lsnu, cf, st, f, n) + interp_face<T>(nut, cf, st, f, n);
+  return interp_face<T, IN, P>(lsnu, cf, st, f, n) + interp_face<T, IN, P>(nut, cf, st, f, n);
 }
 // edge value: interp along d2 of the face viscosity along d1, divided by h2
-template <typename T>
-__device__ __forceinline__ T edge_visc(const T *__restrict__ lsnu, const T *__restrict__ nut,
+// (IN1: no clamp along d1, IN2: none along d2)
+template <typename T, bool IN1 = false, bool IN2 = false, typename P = const T *>
+__device__ __forceinline__ T edge_visc(P lsnu, P nut,
                                        fidx ce, fidx st1, int f1, int n1, fidx st2, int f2,
                                        int n2, T h2) {
-  const int fc = f2 < 2 ? 2 : (f2 > n2 ? n2 : f2);
+  const int fc = IN2 ? f2 : (f2 < 2 ? 2 : (f2 > n2 ? n2 : f2));
   const fidx cc = ce + (fidx)(fc - f2) * st2;
-  return ((face_visc<T>(lsnu, nut, cc, st1, f1, n1) + face_visc<T>(lsnu, nut, cc - st2, st1, f1, n1)) *
+  return ((face_visc<T, IN1, P>(lsnu, nut, cc, st1, f1, n1) + face_visc<T, IN1, P>(lsnu, nut, cc - st2, st1, f1, n1)) *
           T(0.5)) / h2;
 }
 // edge viscosity from stored face viscosities fa (FACES: the directional
 // model) or rebuilt from the cell arrays
-template <typename T, bool FACES>
-__device__ __forceinline__ T evisc(const T *__restrict__ fa, const T *__restrict__ lsnu, const T *__restrict__ nut,
+template <typename T, bool FACES, bool IN1 = false, bool IN2 = false, typename P = const T *>
+__device__ __forceinline__ T evisc(P fa, P lsnu, P nut,
                                    fidx ce, fidx st1, int f1, int n1, fidx st2, int f2, int n2, T h2) {
-  if constexpr (!FACES) return edge_visc<T>(lsnu, nut, ce, st1, f1, n1, st2, f2, n2, h2);
-  const int fc = f2 < 2 ? 2 : (f2 > n2 ? n2 : f2);
+  if constexpr (!FACES) return edge_visc<T, IN1, IN2, P>(lsnu, nut, ce, st1, f1, n1, st2, f2, n2, h2);
+  const int fc = IN2 ? f2 : (f2 < 2 ? 2 : (f2 > n2 ? n2 : f2));
   const fidx cc = ce + (fidx)(fc - f2) * st2;
   return ((fa[cc] + fa[cc - st2]) * T(0.5)) / h2;
 }
-template <typename T>
-__device__ __forceinline__ T surf_cell(const T *__restrict__ hd, const T *__restrict__ kurv,
-                                       const T *__restrict__ rho, fidx cc) {
+template <typename T, typename P = const T *>
+__device__ __forceinline__ T surf_cell(P hd, P kurv, P rho, fidx cc) {
   return hd[cc] * kurv[cc] / rho[cc];
 }
 
@@ -342,8 +344,8 @@ __device__ __forceinline__ T surf_cell(const T *__restrict__ hd, const T *__rest
 
 // second-order one-sided differences of 8_derivadas.f90:83-290 along stride
 // st at face f of n: backward (central at f = 1), forward (central at f = n)
-template <typename T>
-__device__ __forceinline__ T upw2(const T *__restrict__ q, fidx c, fidx st, int f, int n, T h, T a) {
+template <typename T, typename P>
+__device__ __forceinline__ T upw2(P q, fidx c, fidx st, int f, int n, T h, T a) {
   const T qc = q[c], qm = q[c - st], qp = q[c + st];
   const T dn = f >= 2 ? T(0.5) * (q[c - 2 * st] - T(4.) * qm + T(3.) * qc) / h : T(0.5) * (qp - qm) / h;
   const T dp = f <= n - 1 ? T(0.5) * (-q[c + 2 * st] + T(4.) * qp - T(3.) * qc) / h : T(0.5) * (qp - qm) / h;
@@ -352,8 +354,8 @@ __device__ __forceinline__ T upw2(const T *__restrict__ q, fidx c, fidx st, int 
 // flux-form upwind term along one direction (rotacional), or its average
 // with the advective form (antissim); am, ac, ap: advecting velocity at the
 // faces behind, here, ahead
-template <typename T, bool ANTI>
-__device__ __forceinline__ T flux1(const T *__restrict__ q, fidx c, fidx st, T am, T ac, T ap, T h) {
+template <typename T, bool ANTI, typename P>
+__device__ __forceinline__ T flux1(P q, fidx c, fidx st, T am, T ac, T ap, T h) {
   const T qc = q[c], qm = q[c - st], qp = q[c + st];
   const T apc = tmax(ac, T(0)), apm = tmax(am, T(0)), anp = tmin(ap, T(0)), anc = tmin(ac, T(0));
   if (!ANTI) return (apc * qc - apm * qm) / h + (anp * qp - anc * qc) / h;
@@ -363,9 +365,9 @@ __device__ __forceinline__ T flux1(const T *__restrict__ q, fidx c, fidx st, T a
 template <typename T>
 __device__ __forceinline__ int clamp2(int f, int n) { return f < 2 ? 2 : (f > n ? n : f); }
 
-template <typename T, int ADV>
-__device__ __forceinline__ T adv_u(const Grid &g, const T *__restrict__ u, const T *__restrict__ v,
-                                   const T *__restrict__ w, fidx c, int i, int j, int k, T dx, T dy, T dz) {
+template <typename T, int ADV, typename P>
+__device__ __forceinline__ T adv_u(const Grid &g, P u, P v,
+                                   P w, fidx c, int i, int j, int k, T dx, T dy, T dz) {
   const fidx X = 1, Y = g.sx, Z = g.sz;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const T uc = u[c];
@@ -401,9 +403,9 @@ __device__ __forceinline__ T adv_u(const Grid &g, const T *__restrict__ u, const
   }
 }
 
-template <typename T, int ADV>
-__device__ __forceinline__ T adv_v(const Grid &g, const T *__restrict__ u, const T *__restrict__ v,
-                                   const T *__restrict__ w, fidx c, int i, int j, int k, T dx, T dy, T dz) {
+template <typename T, int ADV, typename P>
+__device__ __forceinline__ T adv_v(const Grid &g, P u, P v,
+                                   P w, fidx c, int i, int j, int k, T dx, T dy, T dz) {
   const fidx X = 1, Y = g.sx, Z = g.sz;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const T vc = v[c];
@@ -439,9 +441,9 @@ __device__ __forceinline__ T adv_v(const Grid &g, const T *__restrict__ u, const
   }
 }
 
-template <typename T, int ADV>
-__device__ __forceinline__ T adv_w(const Grid &g, const T *__restrict__ u, const T *__restrict__ v,
-                                   const T *__restrict__ w, fidx c, int i, int j, int k, T dx, T dy, T dz) {
+template <typename T, int ADV, typename P>
+__device__ __forceinline__ T adv_w(const Grid &g, P u, P v,
+                                   P w, fidx c, int i, int j, int k, T dx, T dy, T dz) {
   const fidx X = 1, Y = g.sx, Z = g.sz;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const T wc = w[c];
@@ -528,16 +530,19 @@ __global__ __launch_bounds__(256) void k_visco_dir(Grid g, T cs, const T *__rest
   }
 }
 
-template <typename T, int ADV, bool FACES>
+template <typename T, int ADV, bool FACES, bool INX = false, bool INYZ = false>
 __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, const MomArgs<T> &A, int i, int j,
                                               int k) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const fidx c = g.idx(i, j, k);
   const fidx X = 1, Y = g.sx, Z = g.sz;
   const T dx = T(g.dx), dy = T(g.dy), dz = T(g.dz);
-  const T *__restrict__ u = A.u;
-  const T *__restrict__ v = A.v;
-  const T *__restrict__ w = A.w;
+  // (raw pointers: reading these through 32-bit byte offsets, as the WENO
+  // windows do, measured 2 % slower here)
+  const T *__restrict__ u = A.u, *__restrict__ v = A.v, *__restrict__ w = A.w;
+  const T *__restrict__ nut = A.nut, *__restrict__ lsnu = A.lsnu, *__restrict__ rho = A.rho;
+  const T *__restrict__ hsx = A.hsx, *__restrict__ hsy = A.hsy, *__restrict__ hsz = A.hsz, *__restrict__ kurv = A.kurv;
+  const T *__restrict__ xnf = A.xnf, *__restrict__ ynf = A.ynf, *__restrict__ znf = A.znf;
   const T fr = T(ph.gz / (ph.chezy * ph.chezy));
   const T sigma = T(ph.sigma);
   // ---- u faces: i in [1,nx+1], j in [1,ny], k in [1,nz]
@@ -545,16 +550,16 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
     if (i >= 1 && i <= nx + 1 && j >= 1 && j <= ny && k >= 1 && k <= nz) {
       const T uc = u[c];
       const T uint_ = adv_u<T, ADV>(g, u, v, w, c, i, j, k, dx, dy, dz);
-      const T amai = cell_clamp<T>(A.nut, c, X, i, nx) / dx;
-      const T amam = cell_clamp<T>(A.nut, c - X, X, i - 1, nx) / dx;
-      const T bmap = evisc<T, FACES>(A.xnf, A.lsnu, A.nut, c + Y, X, i, nx, Y, j + 1, ny, dy);
-      const T bmac = evisc<T, FACES>(A.xnf, A.lsnu, A.nut, c, X, i, nx, Y, j, ny, dy);
-      const T dmap = evisc<T, FACES>(A.xnf, A.lsnu, A.nut, c + Z, X, i, nx, Z, k + 1, nz, dz);
-      const T dmac = evisc<T, FACES>(A.xnf, A.lsnu, A.nut, c, X, i, nx, Z, k, nz, dz);
-      const T rhox = interp_face<T>(A.rho, c, X, i, nx);
-      const int ic = i < 2 ? 2 : (i > nx ? nx : i);
+      const T amai = cell_clamp<T, INX>(nut, c, X, i, nx) / dx;
+      const T amam = cell_clamp<T, INX>(nut, c - X, X, i - 1, nx) / dx;
+      const T bmap = evisc<T, FACES, INX, INYZ>(xnf, lsnu, nut, c + Y, X, i, nx, Y, j + 1, ny, dy);
+      const T bmac = evisc<T, FACES, INX, INYZ>(xnf, lsnu, nut, c, X, i, nx, Y, j, ny, dy);
+      const T dmap = evisc<T, FACES, INX, INYZ>(xnf, lsnu, nut, c + Z, X, i, nx, Z, k + 1, nz, dz);
+      const T dmac = evisc<T, FACES, INX, INYZ>(xnf, lsnu, nut, c, X, i, nx, Z, k, nz, dz);
+      const T rhox = interp_face<T, INX>(rho, c, X, i, nx);
+      const int ic = INX ? i : (i < 2 ? 2 : (i > nx ? nx : i));
       const fidx cc = c + (fidx)(ic - i);
-      const T dhsdx = (surf_cell<T>(A.hsx, A.kurv, A.rho, cc) + surf_cell<T>(A.hsx, A.kurv, A.rho, cc - X)) * T(0.5);
+      const T dhsdx = (surf_cell<T>(hsx, kurv, rho, cc) + surf_cell<T>(hsx, kurv, rho, cc - X)) * T(0.5);
       const bool pl = A.ku ? k == A.ku[i + g.sx * j] + 1
                            : (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
       const T ub = pl ? uc : T(0);
@@ -578,16 +583,16 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
     if (i >= 1 && i <= nx && j >= 1 && j <= ny + 1 && k >= 1 && k <= nz) {
       const T vc = v[c];
       const T vint_ = adv_v<T, ADV>(g, u, v, w, c, i, j, k, dx, dy, dz);
-      const T ambp = evisc<T, FACES>(A.ynf, A.lsnu, A.nut, c + X, Y, j, ny, X, i + 1, nx, dx);
-      const T ambc = evisc<T, FACES>(A.ynf, A.lsnu, A.nut, c, Y, j, ny, X, i, nx, dx);
-      const T bmbj = cell_clamp<T>(A.nut, c, Y, j, ny) / dy;
-      const T bmbm = cell_clamp<T>(A.nut, c - Y, Y, j - 1, ny) / dy;
-      const T dmbp = evisc<T, FACES>(A.ynf, A.lsnu, A.nut, c + Z, Y, j, ny, Z, k + 1, nz, dz);
-      const T dmbc = evisc<T, FACES>(A.ynf, A.lsnu, A.nut, c, Y, j, ny, Z, k, nz, dz);
-      const T rhoy = interp_face<T>(A.rho, c, Y, j, ny);
-      const int jc = j < 2 ? 2 : (j > ny ? ny : j);
+      const T ambp = evisc<T, FACES, INYZ, INX>(ynf, lsnu, nut, c + X, Y, j, ny, X, i + 1, nx, dx);
+      const T ambc = evisc<T, FACES, INYZ, INX>(ynf, lsnu, nut, c, Y, j, ny, X, i, nx, dx);
+      const T bmbj = cell_clamp<T, INYZ>(nut, c, Y, j, ny) / dy;
+      const T bmbm = cell_clamp<T, INYZ>(nut, c - Y, Y, j - 1, ny) / dy;
+      const T dmbp = evisc<T, FACES, INYZ, INYZ>(ynf, lsnu, nut, c + Z, Y, j, ny, Z, k + 1, nz, dz);
+      const T dmbc = evisc<T, FACES, INYZ, INYZ>(ynf, lsnu, nut, c, Y, j, ny, Z, k, nz, dz);
+      const T rhoy = interp_face<T, INYZ>(rho, c, Y, j, ny);
+      const int jc = INYZ ? j : (j < 2 ? 2 : (j > ny ? ny : j));
       const fidx cc = c + (fidx)(jc - j) * Y;
-      const T dhsdy = (surf_cell<T>(A.hsy, A.kurv, A.rho, cc) + surf_cell<T>(A.hsy, A.kurv, A.rho, cc - Y)) * T(0.5);
+      const T dhsdy = (surf_cell<T>(hsy, kurv, rho, cc) + surf_cell<T>(hsy, kurv, rho, cc - Y)) * T(0.5);
       const bool pl = A.kv ? k == A.kv[i + g.sx * j] + 1
                            : (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
       const T vb = pl ? vc : T(0);
@@ -611,16 +616,16 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
     if (i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz + 1) {
       const T wc = w[c];
       const T wint_ = adv_w<T, ADV>(g, u, v, w, c, i, j, k, dx, dy, dz);
-      const T amdp = evisc<T, FACES>(A.znf, A.lsnu, A.nut, c + X, Z, k, nz, X, i + 1, nx, dx);
-      const T amdc = evisc<T, FACES>(A.znf, A.lsnu, A.nut, c, Z, k, nz, X, i, nx, dx);
-      const T bmdp = evisc<T, FACES>(A.znf, A.lsnu, A.nut, c + Y, Z, k, nz, Y, j + 1, ny, dy);
-      const T bmdc = evisc<T, FACES>(A.znf, A.lsnu, A.nut, c, Z, k, nz, Y, j, ny, dy);
-      const T dmdk = cell_clamp<T>(A.nut, c, Z, k, nz) / dz;
-      const T dmdm = cell_clamp<T>(A.nut, c - Z, Z, k - 1, nz) / dz;
-      const T rhoz = interp_face<T>(A.rho, c, Z, k, nz);
-      const int kc = k < 2 ? 2 : (k > nz ? nz : k);
+      const T amdp = evisc<T, FACES, INYZ, INX>(znf, lsnu, nut, c + X, Z, k, nz, X, i + 1, nx, dx);
+      const T amdc = evisc<T, FACES, INYZ, INX>(znf, lsnu, nut, c, Z, k, nz, X, i, nx, dx);
+      const T bmdp = evisc<T, FACES, INYZ, INYZ>(znf, lsnu, nut, c + Y, Z, k, nz, Y, j + 1, ny, dy);
+      const T bmdc = evisc<T, FACES, INYZ, INYZ>(znf, lsnu, nut, c, Z, k, nz, Y, j, ny, dy);
+      const T dmdk = cell_clamp<T, INYZ>(nut, c, Z, k, nz) / dz;
+      const T dmdm = cell_clamp<T, INYZ>(nut, c - Z, Z, k - 1, nz) / dz;
+      const T rhoz = interp_face<T, INYZ>(rho, c, Z, k, nz);
+      const int kc = INYZ ? k : (k < 2 ? 2 : (k > nz ? nz : k));
       const fidx cc = c + (fidx)(kc - k) * Z;
-      const T dhsdz = (surf_cell<T>(A.hsz, A.kurv, A.rho, cc) + surf_cell<T>(A.hsz, A.kurv, A.rho, cc - Z)) * T(0.5);
+      const T dhsdz = (surf_cell<T>(hsz, kurv, rho, cc) + surf_cell<T>(hsz, kurv, rho, cc - Z)) * T(0.5);
       const bool pl = A.kw ? k == A.kw[i + g.sx * j] + 1
                            : (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
       const T wb = pl ? wc : T(0);
@@ -645,12 +650,51 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
 // All three momentum updates of one RK stage, one k-plane per block.
 // waves_per_eu(6): 80 VGPRs instead of 82 lifts the occupancy from 5 to 6
 // waves per SIMD without spills (8 would spill)
+//
+// Two launches per stage: the rows of the inner planes and rows,
+// 2 <= j <= ny-1, 2 <= k <= nz-1, over the whole i range (k_momentum_in),
+// where none of the end-face and end-cell clamps along j and k acts, so
+// they are compiled out (the repeated neighbour loads of the viscosity
+// edges then share addresses the compiler can merge); and the remaining
+// rows of the frame (k_momentum), which hold every clamp.  The same values
+// are computed either way.
+//
+// Remaining rows in thread order: the planes k in {0, 1, nz, nz+1, nz+2} of
+// the frame, then for 2 <= k <= nz-1 the rows j in {0, 1, ny, ny+1, ny+2}.
+__host__ __device__ __forceinline__ int64_t mom_shell_cells(const Grid &g) {
+  const int64_t W = g.nx + 3, H = g.ny + 3;
+  return 5 * W * H + (int64_t)(g.nz - 2) * 5 * W;
+}
+__device__ __forceinline__ void mom_shell_cell(const Grid &g, int t, int &i, int &j, int &k) {
+  const int W = g.nx + 3, H = g.ny + 3, nZ = 5 * W * H;
+  const auto side = [](int q, int n) { return q < 2 ? q : n + (q - 2); };  // 0, 1, n, n+1, n+2
+  if (t < nZ) {
+    k = side(t / (W * H), g.nz);
+    const int r = t % (W * H);
+    j = r / W;
+    i = r % W;
+  } else {
+    const int t2 = t - nZ, r = t2 % (5 * W);
+    k = 2 + t2 / (5 * W);
+    j = side(r / W, g.ny);
+    i = r % W;
+  }
+}
 template <typename T, int ADV, bool FACES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_momentum(Grid g, Phys ph,
                                                                                           MomArgs<T> A) {
-  const int i = blockIdx.x * SULI_BX + threadIdx.x, j = blockIdx.y * SULI_BY + threadIdx.y;
-  if (i > g.nx + 2 || j > g.ny + 2) return;
-  momentum_cell<T, ADV, FACES>(g, ph, A, i, j, blockIdx.z);
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= mom_shell_cells(g)) return;
+  int i, j, k;
+  mom_shell_cell(g, t, i, j, k);
+  momentum_cell<T, ADV, FACES>(g, ph, A, i, j, k);
+}
+template <typename T, int ADV, bool FACES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_momentum_in(Grid g, Phys ph,
+                                                                                             MomArgs<T> A) {
+  const int i = blockIdx.x * SULI_BX + threadIdx.x, j = 2 + blockIdx.y * SULI_BY + threadIdx.y;
+  if (i > g.nx + 2 || j > g.ny - 1) return;
+  momentum_cell<T, ADV, FACES, false, true>(g, ph, A, i, j, 2 + blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------

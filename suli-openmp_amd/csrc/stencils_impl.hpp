@@ -78,11 +78,15 @@ void Launch<T>::momentum(const Grid &g, const Phys &ph, hipStream_t s, const Mom
   a.ku = m.ku; a.kv = m.kv; a.kw = m.kw;
   a.dt = (T)dt;
   a.mode = mode;
-  dim3 grid((g.nx + 3 + SULI_BX - 1) / SULI_BX, (g.ny + 3 + SULI_BY - 1) / SULI_BY, g.nz + 3);
+  // the inner rows (grid gi, no clamps along j and k) and the other rows (grid)
+  const dim3 grid((unsigned)((mom_shell_cells(g) + 255) / 256));
+  const dim3 gi((g.nx + 3 + SULI_BX - 1) / SULI_BX, (g.ny - 2 + SULI_BY - 1) / SULI_BY, g.nz - 2);
   const bool faces = m.xnf != nullptr;  // directional model: stored face viscosities
 #define SULI_MOM(ADV)                                                                        \
-  (faces ? (void)(k_momentum<T, ADV, true><<<grid, blk2(), 0, s>>>(g, ph, a))                 \
-         : (void)(k_momentum<T, ADV, false><<<grid, blk2(), 0, s>>>(g, ph, a)))
+  (faces ? ((void)(k_momentum_in<T, ADV, true><<<gi, blk2(), 0, s>>>(g, ph, a)),              \
+            (void)(k_momentum<T, ADV, true><<<grid, 256, 0, s>>>(g, ph, a)))                  \
+         : ((void)(k_momentum_in<T, ADV, false><<<gi, blk2(), 0, s>>>(g, ph, a)),             \
+            (void)(k_momentum<T, ADV, false><<<grid, 256, 0, s>>>(g, ph, a))))
   switch (m.adv) {
     case 12: SULI_MOM(12); break;
     case 13: SULI_MOM(13); break;
