@@ -385,7 +385,7 @@ PMC_KERNELS = {"mg_gs_l0": "k_mg0_gs", "mg_gs_l0_in": "k_mg0_in", "mg_gs_l0_prol
                "mg_gs_l0_out": "k_mg0_out", "mg_restrict_l0": "k_mg0_restrict", "mg_prolong_l0": "k_mg_prolong",
                "mg_scale": "k_scale", "fcg_spmv_dot": "k_fcg1", "fcg_update": "k_fcg2", "fcg_zq": "k_zq",
                "fcg_residual64": "k_res64", "fcg_project": "k_r32", "fcg_xupdate": "k_xupd",
-               "convdiff_tempo": "k_momentum", "ls_advect_rk3": "k_ls_adv", "reinit_rk3": "k_reinit_stage",
+               "convdiff_tempo": ("k_momentum_in", "k_momentum"), "ls_advect_rk3": "k_ls_adv", "reinit_rk3": "k_reinit_stage",
                "reinit_sign": "k_reinit_sign", "heaviside_normals": "k_heavi", "curvature": "k_curv",
                "visco": "k_visco", "posdin": "k_posdin", "est_div": "k_divmax"}
 
@@ -424,8 +424,8 @@ def pmc_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, ncell):
     fetch, write = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
     res = {"factor_fetch": ff, "factor_write": fw, "calibration": f"256 MiB torch copy, {nf}/{nw} dispatches",
            "cells": ncell, "kernels": {}}
-    for scope, kern in PMC_KERNELS.items():
-        def per_launch(rows, counter):
+    for scope, kerns in PMC_KERNELS.items():
+        def per_launch(rows, counter, kern):
             ks = [(g, c.get(counter)) for name, g, c in rows.values() if _kernel_ident(name) == kern]
             ks = [(g, v) for g, v in ks if v is not None]
             if not ks:
@@ -436,10 +436,21 @@ def pmc_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, ncell):
             # once and move almost nothing: only those that ran are averaged
             v = [x for x in v if x > 0.25 * max(v)]
             return sum(v) / len(v), len(v)
-        fk, n1 = per_launch(fetch, "FETCH_SIZE")
-        wk, n2 = per_launch(write, "WRITE_SIZE")
-        if fk is None or wk is None:
+        # a scope of several kernels (the momentum's inner rows and boundary
+        # rows): the bytes of one launch of each, summed
+        kerns = kerns if isinstance(kerns, tuple) else (kerns,)
+        fk = wk = 0.0
+        n1 = n2 = 0
+        for kern in kerns:
+            f1, c1 = per_launch(fetch, "FETCH_SIZE", kern)
+            w1, c2 = per_launch(write, "WRITE_SIZE", kern)
+            if f1 is None or w1 is None:
+                fk = None
+                break
+            fk, wk, n1, n2 = fk + f1, wk + w1, n1 + c1, n2 + c2
+        if fk is None:
             continue
+        kern = "+".join(kerns)
         b = (fk * ff + wk * fw) * 1024
         res["kernels"][scope] = {"kernel": kern, "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
                                  "bytes_per_launch": round(b), "bytes_per_cell": round(b / ncell, 2),
