@@ -46,7 +46,9 @@ typedef struct {
   int32_t nx, ny, nz;
   double dx, dy, dz;
   /* wall codes (0_modules.f90:53-61): 0 periodic, 1 free slip, 2 no slip,
-     4 outflow (x = xf only) */
+     3 prescribed velocity (ghost and wall-normal faces from the boundary
+     planes SULI_BXX0 .. SULI_BZZF1), 4 (x walls only) outflow at xf and at
+     x0 the inflow of 3_contorno.f90:143-146 (v from bxy0, w copied) */
   int32_t ccx0, ccxf, ccy0, ccyf, ccz0, cczf;
   /* model switches: m_turb 0 none, 1 Smagorinsky, 2 directional Smagorinsky (0_modules.f90:31) */
   int32_t m_turb;
@@ -102,7 +104,15 @@ enum {
   SULI_FU1 = 32,   /* fu1(nx+1,ny,nz): stage-2 forces of the RK3 scheme */
   SULI_FV1 = 33,
   SULI_FW1 = 34,
-  SULI_NFIELDS = 35
+  SULI_NFIELDS = 35,
+  /* boundary planes of the prescribed-velocity walls (wall code 3; bxy0 also
+   * feeds code 4 at x = 0), 0_modules.f90:96-120, each in its module extents
+   * from index 0, e.g. bxx0(0:ny+1,0:nz+1), bxy0(0:ny+2,0:nz+1),
+   * byx0(0:nx+2,0:nz+1), bzz0(0:nx+1,0:ny+1); zero until uploaded or set by
+   * the manufactured solution (suli_mms_bc).  suli_field_len gives the count. */
+  SULI_BXX0 = 64, SULI_BXX1, SULI_BXY0, SULI_BXZ0, SULI_BXXF, SULI_BXXF1, SULI_BXYF, SULI_BXZF,
+  SULI_BYX0, SULI_BYY0, SULI_BYY1, SULI_BYZ0, SULI_BYXF, SULI_BYYF, SULI_BYYF1, SULI_BYZF,
+  SULI_BZX0, SULI_BZY0, SULI_BZZ0, SULI_BZZ1, SULI_BZXF, SULI_BZYF, SULI_BZZF, SULI_BZZF1
 };
 
 /* Time-integration schemes of suli_tempo (t_tempo, 0_modules.f90:24). */
@@ -249,6 +259,39 @@ int suli_obstacle_columns(int obst_t, int nx, int ny, int nz, double dx, double 
  * suli_step advances it by dt before the step. */
 int suli_set_forcing(double amplitude, double period, double phase);
 int suli_set_time(double t);
+
+/* The manufactured solution of Weng (2009) (13_mms.f90), computed on the
+ * device at the time of suli_set_time / suli_step (the reference's t).
+ * suli_mms_set: mms_t of 0_modules.f90:37 (0 off, 1 steady, 2 unsteady) and
+ * the constants a, h0, coef of module mms_m (0_modules.f90:208-212); needs
+ * h0 > |a|.  In this mode suli_stage runs no projection (1_principal.f90:89-93),
+ * suli_step computes the sources first (steady: at its first step,
+ * unsteady: every step; 1_principal.f90:70-71), the level set of the
+ * unsteady case corrects no volume (14_level_set.f90:257), and contorno(1|2)
+ * set the boundary planes first (3_contorno.f90:30).
+ *   suli_mms_i         mms_i (13_mms.f90:12-212): level set h - z, the
+ *                      interior-face velocities and (unsteady) the pressure;
+ *                      follow with suli_level_set_ini (level_set_ini's tail)
+ *   suli_mms_bc        mms_bc (:468-816): the 24 boundary planes (steady:
+ *                      only at t = 0); in the unsteady case bzy0 and bzyf
+ *                      take the formula of w, as in the reference
+ *   suli_termo_fonte1  termo_fonte1 (:819-1064) and termo_fonte2
+ *   suli_termo_fonte2  (:1070-1322): the momentum sources tf_u, tf_v, tf_w,
+ *                      exact derivatives of the manufactured fields; the
+ *                      reference's hand-derived ones agree for a = 0 and
+ *                      differ for a != 0
+ *   suli_mms           mms (:218-465): err[0..3] = RMS error of u, v, w over
+ *                      the faces whose interpolated level set is positive and
+ *                      of the pressure over the cells with positive level set
+ *                      (its squared errors kept between calls as the
+ *                      reference's erro_p); the steady case has no exact
+ *                      pressure: err[3] = NaN */
+int suli_mms_set(int mms_t, double a, double h0, double coef);
+int suli_mms_i(void);
+int suli_mms_bc(void);
+int suli_termo_fonte1(void);
+int suli_termo_fonte2(void);
+int suli_mms(double *err);
 
 /* Switch the pressure solver of later suli_graddin calls (the fields stay
  * as they are).  tol_rel <= 0 or max_it <= 0 keeps the current value.  No

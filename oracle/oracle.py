@@ -82,6 +82,10 @@ class Params:
     der: int = 1       # advection differences: 1 upwind, 2 central, 3 upwind 2nd order (0_modules.f90:26)
     adv_type: int = 1  # advection form: 1 classic, 2 rotational, 3 antisymmetric (0_modules.f90:27)
     obst_t: int = 0    # nonzero: the state's ku, kv, kw hold a bottom obstacle (0_modules.f90:29)
+    mms_t: int = 0     # manufactured solution: 0 none, 1 steady, 2 unsteady (0_modules.f90:37)
+    mms_a: float = 0.0     # its surface amplitude a, mean depth h0 and pressure scale coef
+    mms_h0: float = 2.0    # (module mms_m, 0_modules.f90:208-212)
+    mms_coef: float = 0.0
 
     @property
     def dx1(self) -> float:
@@ -109,6 +113,7 @@ class _OrState(C.Structure):
         ("mass_corr_iters", C.c_int), ("cg_max_it", C.c_int),
     ] + [(name, C.POINTER(C.c_double)) for name, _ in ARRAYS] + [
         ("tf_u", C.POINTER(C.c_double)), ("tf_v", C.POINTER(C.c_double)), ("tf_w", C.POINTER(C.c_double)),
+        ("mms_t", C.c_int), ("bp", C.POINTER(C.c_double) * 24),
     ]
 
 
@@ -133,6 +138,8 @@ def lib():
         L.or_est_div.restype = C.c_double
         L.or_stage.argtypes = [P, C.c_int, C.c_double]
         L.or_stage.restype = None
+        L.or_stage_noproj.argtypes = [P, C.c_int, C.c_double]
+        L.or_stage_noproj.restype = None
         L.or_step.argtypes = [P, C.c_double]
         L.or_step.restype = None
         L.or_state_len.argtypes = [C.c_int, C.c_int, C.c_int]
@@ -168,6 +175,7 @@ class State:
         self.scal.update(rho_f1=p.rho_f1, rho_f2=p.rho_f2, mi_f1=p.mi_f1, mi_f2=p.mi_f2,
                          sigma=p.sigma, alpha1=p.alpha1)
         self.tf = None  # optional (tf_u, tf_v, tf_w) source arrays
+        self.planes = None  # optional boundary planes of wall code 3 (name -> array, PLANE_NAMES)
         self.diag = {}
         self.cg_max_it = 0  # graddin iteration cap (0: the reference's 10000)
 
@@ -186,6 +194,7 @@ class State:
         s.buf[:] = self.buf
         s.scal = dict(self.scal)
         s.tf = None if self.tf is None else tuple(a.copy() for a in self.tf)
+        s.planes = None if self.planes is None else {k: a.copy() for k, a in self.planes.items()}
         s.cg_max_it = self.cg_max_it
         return s
 
@@ -219,6 +228,15 @@ class State:
         if self.tf is not None:
             self._tf_keep = [np.ascontiguousarray(a, dtype=np.float64).ravel(order="F") for a in self.tf]
             st.tf_u, st.tf_v, st.tf_w = [_dptr(a) for a in self._tf_keep]
+        st.mms_t = p.mms_t
+        if self.planes is not None:
+            self._bp_keep = [np.ascontiguousarray(self.planes[n], dtype=np.float64).ravel(order="F")
+                             for n in PLANE_NAMES]
+            for q, a in enumerate(self._bp_keep):
+                assert a.size == int(np.prod(plane_shape(p, PLANE_NAMES[q])))
+                st.bp[q] = _dptr(a)
+        elif any(c == 3 for c in (p.ccx0, p.ccxf, p.ccy0, p.ccyf, p.ccz0, p.cczf)) or p.ccx0 == 4:
+            raise ValueError("wall code 3 (and 4 at x = 0) needs the boundary planes (State.planes)")
         return st
 
     def _pull(self, st: _OrState) -> None:
@@ -257,6 +275,9 @@ class State:
             L.or_graddin(C.byref(st))
         elif cmd == "posdin":
             L.or_posdin(C.byref(st))
+        elif cmd == "stage_noproj":  # the stage of the manufactured-solution mode
+            L.or_stage_noproj(C.byref(st), tt, self.scal["dt"])
+            st.dt = self.scal["dt"]
         elif cmd == "stage":  # like the harness, leave dt at its base value afterwards
             L.or_stage(C.byref(st), arg, self.scal["dt"])
             st.dt = self.scal["dt"]
@@ -293,7 +314,8 @@ def ref_tag(p: Params) -> str:
     return (f"g{p.nx}x{p.ny}x{p.nz}_h{p.dx:g}_{p.dy:g}_{p.dz:g}_bc{p.ccx0}{p.ccxf}{p.ccy0}{p.ccyf}{p.ccz0}{p.cczf}"
             f"_t{p.m_turb}_cs{p.csmag:g}_g{p.gx:g}_{p.gz:g}" + (f"_tt{p.t_tempo}" if p.t_tempo != 1 else "")
             + (f"_adv{p.adv_type}{p.der}" if (p.der, p.adv_type) != (1, 1) else "")
-            + (f"_ob{p.obst_t}" if p.obst_t else ""))
+            + (f"_ob{p.obst_t}" if p.obst_t else "")
+            + (f"_mms{p.mms_t}_a{p.mms_a:g}_h{p.mms_h0:g}_c{p.mms_coef:g}" if p.mms_t else ""))
 
 
 def ref_build(p: Params, quiet: bool = True, variant: str = "") -> str:
@@ -310,8 +332,10 @@ def ref_build(p: Params, quiet: bool = True, variant: str = "") -> str:
             str(p.ccx0), str(p.ccxf), str(p.ccy0), str(p.ccyf), str(p.ccz0), str(p.cczf),
             str(p.m_turb), repr(float(p.csmag)), repr(float(p.gx)), repr(float(p.gz)), str(p.t_tempo),
             str(p.der), str(p.adv_type), str(p.obst_t)]
-    subprocess.run(args, check=True, stdout=subprocess.DEVNULL if quiet else None,
-                   env=dict(os.environ, REF_VARIANT=variant))
+    env = dict(os.environ, REF_VARIANT=variant)
+    if p.mms_t:
+        env["REF_MMS"] = f"{p.mms_t} {float(p.mms_a)!r} {float(p.mms_h0)!r} {float(p.mms_coef)!r}"
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL if quiet else None, env=env)
     return exe
 
 
@@ -332,6 +356,57 @@ def ref_run(state: State, cmd: str, arg: int | None = None, workdir: str | None 
     os.remove(fin)
     os.remove(fout)
     return res.stdout
+
+
+# boundary planes of the prescribed-velocity walls (wall code 3), in the order
+# of their declaration (0_modules.f90:96-120), with their extents as
+# (first-index length, second-index length); each starts at index 0
+PLANE_NAMES = ("bxx0", "bxx1", "bxy0", "bxz0", "bxxf", "bxxf1", "bxyf", "bxzf",
+               "byx0", "byy0", "byy1", "byz0", "byxf", "byyf", "byyf1", "byzf",
+               "bzx0", "bzy0", "bzz0", "bzz1", "bzxf", "bzyf", "bzzf", "bzzf1")
+
+
+def plane_shape(p: Params, name: str) -> tuple:
+    nx, ny, nz = p.nx, p.ny, p.nz
+    wall, comp = name[1], name[2]
+    # the two in-plane axes of the wall, then the extra face of the component
+    ax = {"x": ("y", "z"), "y": ("x", "z"), "z": ("x", "y")}[wall]
+    n = {"x": nx, "y": ny, "z": nz}
+    return tuple(n[a] + (3 if a == comp else 2) for a in ax)
+
+
+def ref_run_mms(state: State, cmd: str, arg: int | None = None) -> tuple:
+    """Run an mms command of the reference harness (mms_src, mms_run) on a copy
+    of `state` (in place); returns its stdout and the arrays of OUT.mms
+    (tf_u, tf_v, tf_w, tf_px, tf_py, tf_pz, the boundary planes, erro_u,
+    erro_v, erro_w, erro_p), each in its module extents."""
+    p = state.p
+    exe = ref_build(p)
+    wd = os.path.join(REF_DIR, ref_tag(p))
+    os.makedirs(os.path.join(wd, "dados"), exist_ok=True)  # mms_i opens dados/erros_mms.txt
+    fin = os.path.join(wd, f"in_{os.getpid()}.bin")
+    fout = os.path.join(wd, f"out_{os.getpid()}.bin")
+    state.to_file(fin)
+    argv = [exe, fin, fout, cmd] + ([str(arg)] if arg is not None else [])
+    res = subprocess.run(argv, check=True, capture_output=True, text=True, cwd=wd)
+    state.from_file(fout)
+    raw = np.fromfile(fout + ".mms", dtype="<f8")
+    for f in (fin, fout, fout + ".mms"):
+        os.remove(f)
+    nx, ny, nz = p.nx, p.ny, p.nz
+    shapes = [("tf_u", (nx + 1, ny, nz)), ("tf_v", (nx, ny + 1, nz)), ("tf_w", (nx, ny, nz + 1)),
+              ("tf_px", (nx + 1, ny, nz)), ("tf_py", (nx, ny + 1, nz)), ("tf_pz", (nx, ny, nz + 1))]
+    shapes += [(n, plane_shape(p, n)) for n in PLANE_NAMES]
+    shapes += [("erro_u", (nx + 1, ny, nz)), ("erro_v", (nx, ny + 1, nz)), ("erro_w", (nx, ny, nz + 1)),
+               ("erro_p", (nx, ny, nz))]
+    out, q = {}, 0
+    for n, sh in shapes:
+        m = int(np.prod(sh))
+        out[n] = raw[q:q + m].reshape(sh, order="F")
+        q += m
+    if q != raw.size:
+        raise RuntimeError(f"mms output holds {raw.size} values, expected {q}")
+    return res.stdout, out
 
 
 # -- synthetic states -------------------------------------------------------------
@@ -443,3 +518,241 @@ def xdmf_layout(text: str) -> list:
         dims = tuple(int(t) for t in re.search(r"Dimensions=\"([^\"]*)\"", tag).group(1).split())
         out.append((seek, dims))
     return out
+
+
+# -- the manufactured solution of Weng (2009), 13_mms.f90 -------------------------
+# The solution is a free-surface flow over a depth h(x, y[, t]): level set
+# phi = h - z, velocities built from sin/cos products of x, y and z/h.
+# mms_t = 1 (steady, 13_mms.f90:38-111):
+#   h = a sin x sin y + h0
+#   u = sin x cos y sin(pi z/(2h)) - cos x sin y (cos(2 pi z/h) - 1)
+#   v = -cos x sin y sin(pi z/(2h)) + sin x cos y (cos(2 pi z/h) - 1)
+#   w = -a (sin^2 x cos^2 y - cos^2 x sin^2 y) (sin(2 pi z/h) - (2 pi z/h) cos(2 pi z/h) + 2 pi) / (2 pi)
+# mms_t = 2 (unsteady, :114-207), with X = x - pi, Y = y - pi:
+#   h = a cos(X/2) cos(Y/2) cos t + h0,  c = cos(pi/2 (1 - z/h))
+#   u = sin Y cos^2(X/2) c sin t,  v = -sin X cos^2(Y/2) c sin t,  w = -a cos(X/2) cos(Y/2) c sin t
+#   p = -coef cos X cos Y cos(pi z/(2h))
+def _weng_xy(p: Params, x, y):
+    return (x - np.pi, y - np.pi) if p.mms_t == 2 else (x, y)
+
+
+def weng_h(p: Params, x, y, t: float):
+    X, Y = _weng_xy(p, x, y)
+    a, h0 = p.mms_a, p.mms_h0
+    if p.mms_t == 1:
+        return a * np.sin(X) * np.sin(Y) + h0
+    return a * np.cos(X * 0.5) * np.cos(Y * 0.5) * np.cos(t) + h0
+
+
+def weng_vel(p: Params, comp: str, x, y, z, t: float):
+    """Velocity component comp ('u', 'v', 'w') of the manufactured solution at
+    the points (x, y, z), time t."""
+    X, Y = _weng_xy(p, x, y)
+    h = weng_h(p, x, y, t)
+    a = p.mms_a
+    if p.mms_t == 1:
+        s4, c2 = np.sin(np.pi * 0.5 * z / h), np.cos(np.pi * 2.0 * z / h) - 1.0
+        if comp == "u":
+            return np.sin(X) * np.cos(Y) * s4 - np.cos(X) * np.sin(Y) * c2
+        if comp == "v":
+            return -np.cos(X) * np.sin(Y) * s4 + np.sin(X) * np.cos(Y) * c2
+        r = 2.0 * np.pi * z / h
+        return (-a * (np.sin(X) ** 2 * np.cos(Y) ** 2 - np.cos(X) ** 2 * np.sin(Y) ** 2)
+                * (np.sin(r) - r * np.cos(r) + 2.0 * np.pi) / (2.0 * np.pi))
+    c = np.cos(0.5 * np.pi * (1.0 - z / h)) * np.sin(t)
+    if comp == "u":
+        return np.sin(Y) * np.cos(X * 0.5) * np.cos(X * 0.5) * c
+    if comp == "v":
+        return -np.sin(X) * np.cos(Y * 0.5) * np.cos(Y * 0.5) * c
+    return -a * np.cos(X * 0.5) * np.cos(Y * 0.5) * c
+
+
+def weng_p(p: Params, x, y, z, t: float):
+    X, Y = _weng_xy(p, x, y)
+    return -p.mms_coef * np.cos(X) * np.cos(Y) * np.cos(0.5 * np.pi * z / weng_h(p, x, y, t))
+
+
+def _grid(p: Params, off, rng):
+    """Coordinates ((i - ox) dx, ...) of the index ranges rng = ((i0, i1), ...)."""
+    ax = [(np.arange(r0, r1 + 1) - o) * h for (r0, r1), o, h in zip(rng, off, (p.dx, p.dy, p.dz))]
+    return np.meshgrid(*ax, indexing="ij")
+
+
+def weng_init(st: State, t: float = 0.0) -> None:
+    """mms_i (13_mms.f90:12-212): the level set and the velocities at the
+    interior faces (u(1:nx+1,1:ny,1:nz), ...), for mms_t = 2 also the pressure
+    of the interior cells; ghosts are left to contorno."""
+    p = st.p
+    nx, ny, nz = p.nx, p.ny, p.nz
+    X, Y, Z = _grid(p, (0.5, 0.5, 0.5), ((1, nx), (1, ny), (1, nz)))
+    st.ls[...] = weng_h(p, X, Y, t) - Z
+    st.u[1:nx + 2, 1:ny + 1, 1:nz + 1] = weng_vel(p, "u", *_grid(p, (1.0, 0.5, 0.5), ((1, nx + 1), (1, ny), (1, nz))), t)
+    st.v[1:nx + 1, 1:ny + 2, 1:nz + 1] = weng_vel(p, "v", *_grid(p, (0.5, 1.0, 0.5), ((1, nx), (1, ny + 1), (1, nz))), t)
+    st.w[1:nx + 1, 1:ny + 1, 1:nz + 2] = weng_vel(p, "w", *_grid(p, (0.5, 0.5, 1.0), ((1, nx), (1, ny), (1, nz + 1))), t)
+    if p.mms_t == 2:
+        st.prd1[1:nx + 1, 1:ny + 1, 1:nz + 1] = weng_p(p, X, Y, Z, t)
+
+
+# mms_bc (13_mms.f90:468-816): per plane, the component whose formula it takes
+# and the index of the plane along the wall normal (the reference evaluates
+# the formula at the plane's own staggered position).  In the unsteady case
+# bzy0 and bzyf take the formula of w at the positions of v (:777-781, :795-799).
+_WENG_PLANES = {
+    # name: (wall axis, component, normal index, extent lower bounds)
+    "bxx0": ("x", "u", 0), "bxx1": ("x", "u", 1), "bxxf": ("x", "u", "n1"), "bxxf1": ("x", "u", "n1+1"),
+    "bxy0": ("x", "v", 0), "bxz0": ("x", "w", 0), "bxyf": ("x", "v", "n+1"), "bxzf": ("x", "w", "n+1"),
+    "byy0": ("y", "v", 0), "byy1": ("y", "v", 1), "byyf": ("y", "v", "n1"), "byyf1": ("y", "v", "n1+1"),
+    "byx0": ("y", "u", 0), "byz0": ("y", "w", 0), "byxf": ("y", "u", "n+1"), "byzf": ("y", "w", "n+1"),
+    "bzz0": ("z", "w", 0), "bzz1": ("z", "w", 1), "bzzf": ("z", "w", "n1"), "bzzf1": ("z", "w", "n1+1"),
+    "bzx0": ("z", "u", 0), "bzy0": ("z", "v", 0), "bzxf": ("z", "u", "n+1"), "bzyf": ("z", "v", "n+1"),
+}
+
+
+def weng_planes(p: Params, t: float, planes: dict | None = None) -> dict:
+    """The 24 boundary planes of mms_bc at time t, each in its module extents
+    (entries the reference's loops leave out stay as they were in `planes`,
+    zero if new).  For mms_t = 1 the planes are set only at t = 0."""
+    out = {n: np.zeros(plane_shape(p, n)) for n in PLANE_NAMES} if planes is None else planes
+    if p.mms_t == 1 and t != 0.0:
+        return out
+    n = {"x": p.nx, "y": p.ny, "z": p.nz}
+    for name, (wall, comp, norm) in _WENG_PLANES.items():
+        nn = n[wall]
+        q = {0: 0, 1: 1, "n1": nn + 1, "n1+1": nn + 2, "n+1": nn + 1}[norm]
+        # position offsets of the component's staggered grid: its own axis at
+        # the face (index - 1), the others at cell centres (index - 0.5)
+        face_axis = {"u": "x", "v": "y", "w": "z"}[comp]
+        off = {ax: (1.0 if ax == face_axis else 0.5) for ax in "xyz"}
+        coord = {}
+        ax_in = [a for a in "xyz" if a != wall]
+        # the loops run over 0..n+1 of both in-plane axes (13_mms.f90:485-486, ...)
+        g = np.meshgrid(*[np.arange(0, n[a] + 2) for a in ax_in], indexing="ij")
+        for a, gi in zip(ax_in, g):
+            coord[a] = (gi - off[a]) * {"x": p.dx, "y": p.dy, "z": p.dz}[a]
+        coord[wall] = (q - off[wall]) * {"x": p.dx, "y": p.dy, "z": p.dz}[wall] + 0.0 * g[0]
+        f = "w" if (p.mms_t == 2 and name in ("bzy0", "bzyf")) else comp
+        val = weng_vel(p, f, coord["x"], coord["y"], coord["z"], t)
+        out[name][:val.shape[0], :val.shape[1]] = val
+    return out
+
+
+_WENG_SRC = {}
+
+
+def weng_sources(p: Params, t: float) -> tuple:
+    """Exact momentum sources of the manufactured solution at time t on the
+    face grids of u, v, w (tf_u(nx+1,ny,nz), ...): [dq/dt +] u.grad q -
+    nu (lap q + d/dq_axis (div u)-like terms) with nu = 0.00100798/998, the
+    viscous form of convdiff (6_convdiff.f90:150-186):
+        tf_u = [u_t] + u u_x + v u_y + w u_z - nu (2 u_xx + u_yy + u_zz + v_xy + w_xz)
+    and likewise for v, w (termo_fonte1, termo_fonte2: 13_mms.f90:819-1322).
+    Derived symbolically (sympy) from the fields of weng_vel; the reference's
+    hand-derived expressions agree with these for a = 0 (its setting,
+    0_modules.f90:208) and deviate for a != 0."""
+    key = (p.mms_t, float(p.mms_a), float(p.mms_h0))
+    if key not in _WENG_SRC:
+        import sympy as sp
+        x, y, z, t_ = sp.symbols("x y z t")
+        a, h0 = sp.Float(p.mms_a), sp.Float(p.mms_h0)
+        if p.mms_t == 1:
+            X, Y = x, y
+            h = a * sp.sin(X) * sp.sin(Y) + h0
+            u = sp.sin(X) * sp.cos(Y) * sp.sin(sp.pi / 2 * z / h) - sp.cos(X) * sp.sin(Y) * (sp.cos(2 * sp.pi * z / h) - 1)
+            v = -sp.cos(X) * sp.sin(Y) * sp.sin(sp.pi / 2 * z / h) + sp.sin(X) * sp.cos(Y) * (sp.cos(2 * sp.pi * z / h) - 1)
+            r = 2 * sp.pi * z / h
+            w = -a * (sp.sin(X) ** 2 * sp.cos(Y) ** 2 - sp.cos(X) ** 2 * sp.sin(Y) ** 2) * (
+                sp.sin(r) - r * sp.cos(r) + 2 * sp.pi) / (2 * sp.pi)
+        else:
+            X, Y = x - sp.pi, y - sp.pi
+            h = a * sp.cos(X / 2) * sp.cos(Y / 2) * sp.cos(t_) + h0
+            c = sp.cos(sp.pi / 2 * (1 - z / h)) * sp.sin(t_)
+            u = sp.sin(Y) * sp.cos(X / 2) ** 2 * c
+            v = -sp.sin(X) * sp.cos(Y / 2) ** 2 * c
+            w = -a * sp.cos(X / 2) * sp.cos(Y / 2) * c
+        nu = sp.Float(0.00100798 / 998.0)
+        V = (u, v, w)
+        xs = (x, y, z)
+        fns = []
+        for d, q in enumerate(V):
+            adv = sum(V[e] * sp.diff(q, xs[e]) for e in range(3))
+            visc = sum(sp.diff(q, xs[e], 2) for e in range(3)) + sum(sp.diff(V[e], xs[d], xs[e]) for e in range(3))
+            src = adv - nu * visc + (sp.diff(q, t_) if p.mms_t == 2 else 0)
+            fns.append(sp.lambdify((x, y, z, t_), src, "numpy"))
+        _WENG_SRC[key] = fns
+    fu, fv, fw = _WENG_SRC[key]
+    nx, ny, nz = p.nx, p.ny, p.nz
+    one = lambda f, g: f(*g, t) * np.ones_like(g[0])  # noqa: E731
+    return (one(fu, _grid(p, (1.0, 0.5, 0.5), ((1, nx + 1), (1, ny), (1, nz)))),
+            one(fv, _grid(p, (0.5, 1.0, 0.5), ((1, nx), (1, ny + 1), (1, nz)))),
+            one(fw, _grid(p, (0.5, 0.5, 1.0), ((1, nx), (1, ny), (1, nz + 1)))))
+
+
+def _interp_cf(a, axis):
+    """Centre to face average with the end faces copied (8_derivadas.f90:304-434)."""
+    m = np.moveaxis(a, axis, 0)
+    mid = (m[1:] + m[:-1]) * 0.5
+    return np.moveaxis(np.concatenate([mid[:1], mid, mid[-1:]], axis=0), 0, axis)
+
+
+def weng_errors(st: State, t: float, erro_p: np.ndarray) -> tuple:
+    """mms() (13_mms.f90:218-465): RMS errors of u, v, w over the faces whose
+    interpolated level set is positive and of the pressure over the positive
+    cells.  erro_p (nx,ny,nz) is updated in place as the reference's module
+    array: only the wet cells are rewritten, so a cell that dries keeps its
+    last value and still enters the sum.  For mms_t = 1 the reference's exact
+    pressure is unset; the pressure error is returned as NaN."""
+    p = st.p
+    nx, ny, nz = p.nx, p.ny, p.nz
+    ls = st.ls
+    res = []
+    for comp, ax, arr, off, rng in (
+            ("u", 0, st.u[1:nx + 2, 1:ny + 1, 1:nz + 1], (1.0, 0.5, 0.5), ((1, nx + 1), (1, ny), (1, nz))),
+            ("v", 1, st.v[1:nx + 1, 1:ny + 2, 1:nz + 1], (0.5, 1.0, 0.5), ((1, nx), (1, ny + 1), (1, nz))),
+            ("w", 2, st.w[1:nx + 1, 1:ny + 1, 1:nz + 2], (0.5, 0.5, 1.0), ((1, nx), (1, ny), (1, nz + 1)))):
+        wet = _interp_cf(ls, ax) > 0.0
+        e = np.where(wet, (weng_vel(p, comp, *_grid(p, off, rng), t) - arr) ** 2, 0.0)
+        res.append(np.sqrt(e.sum() / wet.sum()))
+    wet = ls > 0.0
+    if p.mms_t == 2:
+        pm = weng_p(p, *_grid(p, (0.5, 0.5, 0.5), ((1, nx), (1, ny), (1, nz))), t)
+        erro_p[wet] = ((pm - st.prd1[1:nx + 1, 1:ny + 1, 1:nz + 1]) ** 2)[wet]
+        res.append(np.sqrt(erro_p.sum() / wet.sum()))
+    else:
+        res.append(float("nan"))
+    return tuple(float(r) for r in res)
+
+
+def weng_run(p: Params, dt: float, nsteps: int) -> tuple:
+    """The manufactured-solution run of 1_principal.f90:39-108 with mms_t > 0
+    from t = 0: mms_i and level_set_ini's tail (14_level_set.f90:178-218),
+    contorno(1), contorno(3), then per step t = it dt, the sources (mms_t = 1:
+    once, at it = 1; mms_t = 2: every step), level_set, contorno(3) and the
+    stages without the projection (visco, convdiff, tempo, contorno(2));
+    mms() after each step.  Returns the final state and the errors (u, v, w,
+    p) at t = 0 and after every step."""
+    st = State(p)
+    st.scal["dt"] = dt
+    st.scal["t"] = 0.0
+    weng_init(st, 0.0)
+    st.planes = weng_planes(p, 0.0)
+    st.run("ls_ini")
+    st.run("contorno", 1)
+    st.run("contorno", 3)
+    erro_p = np.zeros((p.nx, p.ny, p.nz))
+    errs = [weng_errors(st, 0.0, erro_p)]
+    for it in range(1, nsteps + 1):
+        t = it * dt
+        st.scal["t"] = t
+        if (p.mms_t == 1 and it == 1) or p.mms_t == 2:
+            st.tf = weng_sources(p, t)
+        st.run("level_set")
+        st.run("contorno", 3)
+        for tt in range(1, or_ntt(p.t_tempo) + 1):
+            st.planes = weng_planes(p, t, st.planes)
+            st.run("stage_noproj", tt=tt, it=it)
+        errs.append(weng_errors(st, t, erro_p))
+    return st, errs
+
+
+def or_ntt(t_tempo: int) -> int:
+    return 3 if t_tempo == 2 else 2 if t_tempo == 1 else 1

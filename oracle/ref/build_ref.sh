@@ -34,6 +34,10 @@
 #              the scaled right-hand side; computed once before the loop)
 # Without it the build is -O2 (the parity harnesses).
 #
+# REF_MMS="MMS_T A H0 COEF" (environment) builds the manufactured-solution
+# configuration: 0_modules.f90:37 mms_t and the module mms_m constants
+# a, h0, coef (0_modules.f90:208-212).
+#
 # usage: build_ref.sh TAG NX NY NZ DX DY DZ CCX0 CCXF CCY0 CCYF CCZ0 CCZF MTURB CSMAG GX GZ [TTEMPO [DER ADV_TYPE [OBST_T]]]
 set -euo pipefail
 if [ $# -ne 17 ] && [ $# -ne 18 ] && [ $# -ne 20 ] && [ $# -ne 21 ]; then
@@ -79,6 +83,13 @@ edit "$M" "s/^integer,parameter :: ccz0=2/integer,parameter :: ccz0=$CCZ0/" "ccz
 edit "$M" "s/^integer,parameter :: cczf=1/integer,parameter :: cczf=$CCZF/" "cczf=$CCZF"
 edit "$M" "s/^real(8), parameter :: gx = 9.80665 \* sin(atan(decliv)) , gz = 9.80665 \* cos(atan(decliv))/real(8), parameter :: gx = $GX , gz = $GZ/" "gx = $GX , gz = $GZ"
 edit "$M" "s/^real(8), parameter :: csmag=0.13/real(8), parameter :: csmag=$CSMAG/" "csmag=$CSMAG"
+if [ -n "${REF_MMS:-}" ]; then
+  read -r MMST MMSA MMSH0 MMSC <<<"$REF_MMS"
+  edit "$M" "s/^integer,parameter :: mms_t = 0 /integer,parameter :: mms_t = $MMST /" "mms_t = $MMST "
+  edit "$M" "s/^real(8), parameter :: a  = 0\.!5\$/real(8), parameter :: a  = $MMSA/" "a  = $MMSA\$"
+  edit "$M" "s/^real(8), parameter :: h0 = 2\.\$/real(8), parameter :: h0 = $MMSH0/" "h0 = $MMSH0\$"
+  edit "$M" "s/^real(8), parameter :: coef = 0\. ! 1\.\$/real(8), parameter :: coef = $MMSC/" "coef = $MMSC\$"
+fi
 edit "$M" "s/^module tempo\$/module tempo_mod/" "^module tempo_mod"
 edit "$M" "s/^end module tempo\$/end module tempo_mod/" "^end module tempo_mod"
 for f in 2_iniciais 4_plot 5_viscosidade 11_diversos; do

@@ -10,9 +10,14 @@
 ! CMD: ls_ini | level_set | heaviside | mod_ls1 | reinic | der_weno | weno1 ARG(ihs)
 !      visco | convdiff | tempo ARG(tt) | contorno ARG(nlock) | graddin | posdin
 !      stage ARG(tt) | step ARG(nsteps) | bench | plot | visu | restart_salva | restart_ini
-!      obstaculo | est
+!      obstaculo | est | mms_src | mms_run ARG(nsteps)
 ! (plot and visu write under arquivos/, the restart commands use dados/, of
 ! the working directory)
+!
+! The mms commands (builds with mms_t > 0, 13_mms.f90) also write OUT.mms:
+! tf_u, tf_v, tf_w, tf_px, tf_py, tf_pz, the 24 boundary planes in the order
+! of their declaration (0_modules.f90:96-120), erro_u, erro_v, erro_w,
+! erro_p, each in Fortran order with its module extents.
 !
 ! State file: int32 nx,ny,nz; 10 float64 scalars (dt, t, vol_ini, vol_ins,
 ! rho_f1, rho_f2, mi_f1, mi_f2, sigma, alpha1); then the arrays listed in
@@ -25,6 +30,7 @@ program harness
   use ls_param
   use obst
   use tempo_mod
+  use mms_m
   implicit none
   character(len=512) :: fin, fout, cmd, arg
   integer :: iarg, nsteps, k, i, j
@@ -202,6 +208,60 @@ program harness
   case ('obstaculo')
     ! the obstacle columns ku, kv, kw of obst_t (3_contorno.f90:494-760)
     call obstaculo()
+  case ('mms_src')
+    ! the boundary planes (mms_bc, 13_mms.f90:468-816; for mms_t = 1 only at
+    ! t = 0) and the momentum sources of mms_t (termo_fonte1 / termo_fonte2,
+    ! :819-1322) at the time t of the state file
+    call mms_bc()
+    if (mms_t == 1) then
+      call termo_fonte1()
+    else
+      call termo_fonte2()
+    endif
+    call io_mms(trim(fout)//'.mms')
+  case ('mms_run')
+    ! the manufactured-solution run of 1_principal.f90:39-108 with mms_t > 0
+    ! from t = 0: the initial fields of mms_i (level_set_ini with tipo = 6,
+    ! 14_level_set.f90:178-218), contorno(1), contorno(3), mms(); then per
+    ! step the sources, level_set, contorno(3), the stages without the
+    ! projection (visco, convdiff, tempo, contorno(2)) and mms().  The errors
+    ! of mms() are printed per step (MMS_ERR it t erro_u1 erro_v1 erro_w1
+    ! erro_p1, recomputed here from the module arrays as mms() does).
+    nsteps = max(iarg, 0)
+    t = 0.
+    it = 0
+    call mms_i()
+    call heaviside()
+    vol_ini = 0.
+    do k = 1, nz
+    do j = 1, ny
+    do i = 1, nx
+      vol_ini = vol_ini + (1.-(1.-hs(i,j,k)))*dx*dy*dz
+    enddo
+    enddo
+    enddo
+    call mod_ls1()
+    call contorno(1)
+    call contorno(3)
+    call mms()
+    call mms_print()
+    do it = 1, nsteps
+      t = it * dt
+      if ((mms_t == 1) .and. (it == 1)) call termo_fonte1()
+      if (mms_t == 2) call termo_fonte2()
+      call level_set()
+      call contorno(3)
+      do tt = 1, ntt
+        dt = a_dt(tt)
+        call visco()
+        call convdiff()
+        call tempo()
+        call contorno(2)
+      enddo
+      call mms()
+      call mms_print()
+    enddo
+    call io_mms(trim(fout)//'.mms')
   case ('restart_ini')
     ! reads dados/arquivorestart (11_diversos.f90:232-301)
     call restart_ini()
@@ -213,6 +273,33 @@ program harness
   if (trim(fout) /= '-') call io_state(fout, 1)
 
 contains
+
+  subroutine mms_print()
+    ! the four errors of mms() (13_mms.f90:441-444) from its module arrays
+    real(8) :: e(4)
+    integer :: nxc, nyc, nzc, npc
+    real(8), dimension(nx1,ny,nz) :: lsx
+    real(8), dimension(nx,ny1,nz) :: lsy
+    real(8), dimension(nx,ny,nz1) :: lsz
+    call interpx_cf(ls,nx,ny,nz,lsx)
+    call interpy_cf(ls,nx,ny,nz,lsy)
+    call interpz_cf(ls,nx,ny,nz,lsz)
+    nxc = count(lsx > 0.); nyc = count(lsy > 0.); nzc = count(lsz > 0.); npc = count(ls > 0.)
+    e(1) = sqrt(sum(erro_u)/nxc); e(2) = sqrt(sum(erro_v)/nyc)
+    e(3) = sqrt(sum(erro_w)/nzc); e(4) = sqrt(sum(erro_p)/npc)
+    write(*,'(A,I0,5ES24.16)') 'MMS_ERR ', it, t, e
+  end subroutine mms_print
+
+  subroutine io_mms(fname)
+    character(len=*), intent(in) :: fname
+    open(unit=23, file=fname, access='stream', form='unformatted', status='replace')
+    write(23) tf_u, tf_v, tf_w, tf_px, tf_py, tf_pz
+    write(23) bxx0, bxx1, bxy0, bxz0, bxxf, bxxf1, bxyf, bxzf, &
+              byx0, byy0, byy1, byz0, byxf, byyf, byyf1, byzf, &
+              bzx0, bzy0, bzz0, bzz1, bzxf, bzyf, bzzf, bzzf1
+    write(23) erro_u, erro_v, erro_w, erro_p
+    close(23)
+  end subroutine io_mms
 
   subroutine io_state(fname, mode)
     character(len=*), intent(in) :: fname

@@ -382,7 +382,7 @@ void or_level_set(or_state *s) {
   or_heaviside(s);
   s->vol_ins = volume(s);
   int nc = 0;
-  while (fabs(s->vol_ins - s->vol_ini) / s->vol_ins > 0.1) {
+  while (fabs(s->vol_ins - s->vol_ini) / s->vol_ins > 0.1 && s->mms_t != 2) {
     for (int64_t c = 0; c < n; c++)
       s->ls[c] = s->ls[c] - s->dt1 * (s->vol_ins - s->vol_ini) * s->mod_ls[c] / s->vol_ini;
     or_mod_ls1(s);
@@ -988,8 +988,16 @@ static double or_a_dt(int t_tempo, int tt, double dt) {
   return (or_ntt(t_tempo) > 1 && tt == 1) ? 0.5 * dt : 1.0 * dt;
 }
 
+/* plane n of the prescribed-velocity walls at (a, b), first extent n1 */
+#define BP(n, a, b, n1) (s->bp[n][(a) + (int64_t)(n1) * (b)])
+enum { BXX0, BXX1, BXY0, BXZ0, BXXF, BXXF1, BXYF, BXZF, BYX0, BYY0, BYY1, BYZ0, BYXF, BYYF, BYYF1, BYZF,
+       BZX0, BZY0, BZZ0, BZZ1, BZXF, BZYF, BZZF, BZZF1 };
+
 /* contorno, 3_contorno.f90:29-443 with prd_corr (:872-950, every dp ≡ 0);
- * wall codes 0 (periodic), 1 (free slip), 2 (no slip), 4 (outflow at xf);
+ * wall codes 0 (periodic), 1 (free slip), 2 (no slip), 3 (prescribed: the
+ * ghost planes from the boundary planes bxx0..bzzf1, the wall-normal faces
+ * from bxx1, bxxf, byy1, byyf, bzz1, bzzf), 4 (outflow at xf; at x = 0 the
+ * ghost v from bxy0 and w copied from the first cells);
  * a bottom obstacle (obst_t != 0) zeroes the velocities of its columns and
  * moves the friction planes on top of it (:266-291); nlock = 3 fills the
  * level set inside it (:445-490). */
@@ -1029,6 +1037,13 @@ void or_contorno(or_state *s, int nlock) {
       for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) w[IW(i, 0, k)] = (s->ccy0 == 1) ? w[IW(i, 1, k)] : -w[IW(i, 1, k)];
       (void)sg;
     }
+    if (s->ccy0 == 3) { /* :80-84 */
+      for (k = 0; k <= nz + 1; k++) {
+        for (i = 0; i <= nx1 + 1; i++) u[IU(i, 0, k)] = BP(BYX0, i, k, nx1 + 2);
+        for (i = 0; i <= nx + 1; i++) v[IV(i, 0, k)] = BP(BYY0, i, k, nx + 2);
+      }
+      for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) w[IW(i, 0, k)] = BP(BYZ0, i, k, nx + 2);
+    }
     /* y = yf wall, :87-114 */
     if (s->ccy0 == 0 && s->ccyf == 0) {
       for (k = 0; k <= nz + 1; k++) {
@@ -1044,6 +1059,13 @@ void or_contorno(or_state *s, int nlock) {
       }
       for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) w[IW(i, ny + 1, k)] = (s->ccyf == 1) ? w[IW(i, ny, k)] : -w[IW(i, ny, k)];
     }
+    if (s->ccyf == 3) { /* :109-113 */
+      for (k = 0; k <= nz + 1; k++) {
+        for (i = 0; i <= nx1 + 1; i++) u[IU(i, ny + 1, k)] = BP(BYXF, i, k, nx1 + 2);
+        for (i = 0; i <= nx + 1; i++) v[IV(i, ny1 + 1, k)] = BP(BYYF1, i, k, nx + 2);
+      }
+      for (k = 0; k <= nz1 + 1; k++) for (i = 0; i <= nx + 1; i++) w[IW(i, ny + 1, k)] = BP(BYZF, i, k, nx + 2);
+    }
     /* x = 0 wall, :116-161 */
     if (s->ccx0 == 0 && s->ccxf == 0) {
       for (k = 0; k <= nz + 1; k++) {
@@ -1058,6 +1080,17 @@ void or_contorno(or_state *s, int nlock) {
         for (j = 0; j <= ny1 + 1; j++) v[IV(0, j, k)] = (s->ccx0 == 1) ? v[IV(1, j, k)] : -v[IV(1, j, k)];
       }
       for (k = 0; k <= nz1 + 1; k++) for (j = 0; j <= ny + 1; j++) w[IW(0, j, k)] = (s->ccx0 == 1) ? w[IW(1, j, k)] : -w[IW(1, j, k)];
+    }
+    if (s->ccx0 == 3) { /* :136-140 */
+      for (k = 0; k <= nz + 1; k++) {
+        for (j = 0; j <= ny + 1; j++) u[IU(0, j, k)] = BP(BXX0, j, k, ny + 2);
+        for (j = 0; j <= ny1 + 1; j++) v[IV(0, j, k)] = BP(BXY0, j, k, ny1 + 2);
+      }
+      for (k = 0; k <= nz1 + 1; k++) for (j = 0; j <= ny + 1; j++) w[IW(0, j, k)] = BP(BXZ0, j, k, ny + 2);
+    }
+    if (s->ccx0 == 4) { /* :143-146 */
+      for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny1 + 1; j++) v[IV(0, j, k)] = BP(BXY0, j, k, ny1 + 2);
+      for (k = 0; k <= nz1 + 1; k++) for (j = 0; j <= ny + 1; j++) w[IW(0, j, k)] = w[IW(1, j, k)];
     }
     /* x = xf wall, :163-197 */
     if (s->ccx0 == 0 && s->ccxf == 0) {
@@ -1076,17 +1109,34 @@ void or_contorno(or_state *s, int nlock) {
       for (k = 0; k <= nz1 + 1; k++)
         for (j = 0; j <= ny + 1; j++) w[IW(nx + 1, j, k)] = (s->ccxf == 2) ? -w[IW(nx, j, k)] : w[IW(nx, j, k)];
     }
+    if (s->ccxf == 3) { /* :185-189 */
+      for (k = 0; k <= nz + 1; k++) {
+        for (j = 0; j <= ny + 1; j++) u[IU(nx1 + 1, j, k)] = BP(BXXF1, j, k, ny + 2);
+        for (j = 0; j <= ny1 + 1; j++) v[IV(nx + 1, j, k)] = BP(BXYF, j, k, ny1 + 2);
+      }
+      for (k = 0; k <= nz1 + 1; k++) for (j = 0; j <= ny + 1; j++) w[IW(nx + 1, j, k)] = BP(BXZF, j, k, ny + 2);
+    }
     /* bottom, :199-219 */
     if (s->ccz0 == 1 || s->ccz0 == 2) {
       for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx1 + 1; i++) u[IU(i, j, 0)] = (s->ccz0 == 1) ? u[IU(i, j, 1)] : -u[IU(i, j, 1)];
       for (j = 0; j <= ny1 + 1; j++) for (i = 0; i <= nx + 1; i++) v[IV(i, j, 0)] = (s->ccz0 == 1) ? v[IV(i, j, 1)] : -v[IV(i, j, 1)];
       for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, 0)] = (s->ccz0 == 1) ? -w[IW(i, j, 2)] : w[IW(i, j, 2)];
     }
+    if (s->ccz0 == 3) { /* :214-218 */
+      for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx1 + 1; i++) u[IU(i, j, 0)] = BP(BZX0, i, j, nx1 + 2);
+      for (j = 0; j <= ny1 + 1; j++) for (i = 0; i <= nx + 1; i++) v[IV(i, j, 0)] = BP(BZY0, i, j, nx + 2);
+      for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, 0)] = BP(BZZ0, i, j, nx + 2);
+    }
     /* top, :221-241 */
     if (s->cczf == 1 || s->cczf == 2) {
       for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx1 + 1; i++) u[IU(i, j, nz + 1)] = (s->cczf == 1) ? u[IU(i, j, nz)] : -u[IU(i, j, nz)];
       for (j = 0; j <= ny1 + 1; j++) for (i = 0; i <= nx + 1; i++) v[IV(i, j, nz + 1)] = (s->cczf == 1) ? v[IV(i, j, nz)] : -v[IV(i, j, nz)];
       for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, nz1 + 1)] = (s->cczf == 1) ? -w[IW(i, j, nz1 - 1)] : w[IW(i, j, nz1 - 1)];
+    }
+    if (s->cczf == 3) { /* :236-240 */
+      for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx1 + 1; i++) u[IU(i, j, nz + 1)] = BP(BZXF, i, j, nx1 + 2);
+      for (j = 0; j <= ny1 + 1; j++) for (i = 0; i <= nx + 1; i++) v[IV(i, j, nz + 1)] = BP(BZYF, i, j, nx + 2);
+      for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, nz1 + 1)] = BP(BZZF1, i, j, nx + 2);
     }
     if (s->obst_t != 0) { /* bottom obstacle, :266-291 */
 #define KU(i, j) ((int)s->ku[(i) + (int64_t)(nx + 3) * (j)])
@@ -1126,14 +1176,17 @@ void or_contorno(or_state *s, int nlock) {
       for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) v[IV(i, 1, k)] = v[IV(i, ny1, k)];
     }
     if (s->ccy0 == 1 || s->ccy0 == 2) for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) v[IV(i, 1, k)] = 0. + 0.;
+    if (s->ccy0 == 3) for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) v[IV(i, 1, k)] = BP(BYY1, i, k, nx + 2) + 0.;
     if (s->ccy0 == 0 && s->ccyf == 0) {
       for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) v[IV(i, ny1, k)] = v[IV(i, 1, k)];
     }
     if (s->ccyf == 1 || s->ccyf == 2) for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) v[IV(i, ny1, k)] = 0. + 0.;
+    if (s->ccyf == 3) for (k = 0; k <= nz + 1; k++) for (i = 0; i <= nx + 1; i++) v[IV(i, ny1, k)] = BP(BYYF, i, k, nx + 2) + 0.;
     if (s->ccx0 == 0 && s->ccxf == 0) {
       for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) u[IU(1, j, k)] = u[IU(nx1, j, k)];
     }
     if (s->ccx0 == 1 || s->ccx0 == 2) for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) u[IU(1, j, k)] = 0. + 0.;
+    if (s->ccx0 == 3) for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) u[IU(1, j, k)] = BP(BXX1, j, k, ny + 2) + 0.;
     if (s->ccx0 == 0 && s->ccxf == 0) {
       for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) u[IU(nx1, j, k)] = u[IU(1, j, k)];
     }
@@ -1146,9 +1199,12 @@ void or_contorno(or_state *s, int nlock) {
       for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny1 + 1; j++) v[IV(nx + 1, j, k)] = -v[IV(nx, j, k)];
       for (k = 0; k <= nz1 + 1; k++) for (j = 0; j <= ny + 1; j++) w[IW(nx + 1, j, k)] = -w[IW(nx, j, k)];
     }
+    if (s->ccxf == 3) for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) u[IU(nx1, j, k)] = BP(BXXF, j, k, ny + 2) + 0.;
     if (s->ccxf == 4) for (k = 0; k <= nz + 1; k++) for (j = 0; j <= ny + 1; j++) u[IU(nx1, j, k)] = u[IU(nx1 - 1, j, k)];
     if (s->ccz0 == 1 || s->ccz0 == 2) for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, 1)] = 0. + 0.;
+    if (s->ccz0 == 3) for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, 1)] = BP(BZZ1, i, j, nx + 2) + 0.;
     if (s->cczf == 1 || s->cczf == 2) for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, nz1)] = 0. + 0.;
+    if (s->cczf == 3) for (j = 0; j <= ny + 1; j++) for (i = 0; i <= nx + 1; i++) w[IW(i, j, nz1)] = BP(BZZF, i, j, nx + 2) + 0.;
   }
   if (nlock == 3 && s->obst_t != 0) {
     /* level set inside the obstacle from its neighbours, :445-490; every
@@ -1342,6 +1398,17 @@ void or_stage(or_state *s, int tt, double dt_base) {
   or_graddin(s);
   or_posdin(s);
   or_contorno(s, 1);
+}
+
+/* one stage of the manufactured-solution mode (mms_t > 0): no projection,
+ * 1_principal.f90:76-94 */
+void or_stage_noproj(or_state *s, int tt, double dt_base) {
+  s->tt = tt;
+  s->dt = or_a_dt(s->t_tempo, tt, dt_base);
+  or_visco(s);
+  or_convdiff(s);
+  or_tempo(s);
+  or_contorno(s, 2);
 }
 
 /* one time step, 1_principal.f90:74-95 (level set with the full dt) */

@@ -51,6 +51,14 @@ typedef struct {
   double *ku, *kv, *kw;
   /* optional momentum sources (manufactured solutions); NULL means zero */
   double *tf_u, *tf_v, *tf_w;
+  /* manufactured-solution mode (0_modules.f90:37): 2 skips the volume
+   * correction of level_set (14_level_set.f90:257) */
+  int mms_t;
+  /* boundary planes of the prescribed-velocity walls (wall code 3), in the
+   * declaration order of 0_modules.f90:96-120 (bxx0, bxx1, bxy0, bxz0, bxxf,
+   * bxxf1, bxyf, bxzf, byx0, ..., bzzf1), each in Fortran order from index 0
+   * with its module extents; NULL when no wall has code 3 */
+  double *bp[24];
 } or_state;
 
 /* number of float64 values in the flat state buffer for an nx*ny*nz grid */
@@ -77,6 +85,7 @@ void or_posdin(or_state *s);
 double or_est_div(const or_state *s);
 /* one RK stage (visco..contorno(1)) and one full step, 1_principal.f90:74-95 */
 void or_stage(or_state *s, int tt, double dt_base);
+void or_stage_noproj(or_state *s, int tt, double dt_base);
 void or_step(or_state *s, double dt_base);
 
 #ifdef __cplusplus

@@ -88,11 +88,11 @@ extern "C" int suli_init(const suli_params *p) {
     suli_set_error("suli_init: every grid dimension must be at least 4 (WENO5 stencils)");
     return 1;
   }
-  auto okc = [](int v, bool x) { return v == 0 || v == 1 || v == 2 || (x && v == 4); };
-  if (!okc(p->ccx0, false) || !okc(p->ccxf, true) || !okc(p->ccy0, false) || !okc(p->ccyf, false) ||
-      p->ccz0 < 1 || p->ccz0 > 2 || p->cczf < 1 || p->cczf > 2) {
-    suli_set_error("suli_init: supported wall codes are 0 periodic, 1 free slip, 2 no slip, 4 outflow at xf; "
-                   "z walls 1 or 2");
+  auto okc = [](int v, bool x) { return (v >= 0 && v <= 3) || (x && v == 4); };
+  if (!okc(p->ccx0, true) || !okc(p->ccxf, true) || !okc(p->ccy0, false) || !okc(p->ccyf, false) ||
+      p->ccz0 < 1 || p->ccz0 > 3 || p->cczf < 1 || p->cczf > 3) {
+    suli_set_error("suli_init: supported wall codes are 0 periodic, 1 free slip, 2 no slip, 3 prescribed, "
+                   "4 (x walls only: inflow at x0, outflow at xf); z walls 1, 2 or 3");
     return 1;
   }
   if ((p->ccx0 == 0) != (p->ccxf == 0) || (p->ccy0 == 0) != (p->ccyf == 0)) {
@@ -172,6 +172,7 @@ extern "C" int suli_init(const suli_params *p) {
   for (double **q : td)
     if (dalloc(*c, (void **)q, db)) return 1;
   if (dalloc(*c, (void **)&c->ctl, sizeof(CGCtl))) return 1;
+  if (dalloc(*c, &c->bp, (size_t)bplanes_layout(g.nx, g.ny, g.nz).off[BP_N] * c->esz)) return 1;
   const size_t maxblocks = (size_t)((g.nx + 3 + 63) / 64) * ((g.ny + 3 + 3) / 4) * (g.nz + 3) + 1024;
   if (dalloc(*c, (void **)&c->partials, maxblocks * 4 * sizeof(double))) return 1;
   if (dalloc(*c, (void **)&c->res, RS_NSLOT * sizeof(double))) return 1;
@@ -351,8 +352,33 @@ static bool field_ref(Ctx &c, int f, FieldRef &r) {
   }
 }
 
+// boundary plane n (field id SULI_BXX0 + n) between host doubles in its
+// module extents and the device buffer of the field precision
+static int plane_io(Ctx &c, int n, double *host, bool up) {
+  const BPlanes L = bplanes_layout(c.g.nx, c.g.ny, c.g.nz);
+  const int64_t m = (int64_t)L.e1[n] * L.e2[n];
+  char *dev = (char *)c.bp + L.off[n] * (int64_t)c.esz;
+  std::vector<float> tmp;
+  void *h = host;
+  if (c.esz == 4) {
+    tmp.resize((size_t)m);
+    if (up) for (int64_t q = 0; q < m; q++) tmp[q] = (float)host[q];
+    h = tmp.data();
+  }
+  HIPCHK(hipMemcpyAsync(up ? (void *)dev : h, up ? h : (void *)dev, (size_t)m * c.esz,
+                        up ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, c.st));
+  HIPCHK(hipStreamSynchronize(c.st));
+  if (c.esz == 4 && !up) for (int64_t q = 0; q < m; q++) host[q] = tmp[q];
+  return 0;
+}
+static bool is_plane(int field) { return field >= SULI_BXX0 && field < SULI_BXX0 + BP_N; }
+
 extern "C" int64_t suli_field_len(int field) {
   if (!g_ctx) return -1;
+  if (is_plane(field)) {
+    const BPlanes L = bplanes_layout(g_ctx->g.nx, g_ctx->g.ny, g_ctx->g.nz);
+    return (int64_t)L.e1[field - SULI_BXX0] * L.e2[field - SULI_BXX0];
+  }
   FieldRef r;
   if (!field_ref(*g_ctx, field, r)) return -1;
   return ext_len(ext_of(g_ctx->g, r.kind));
@@ -390,6 +416,14 @@ static int host_to_frame(Ctx &c, void *dev, bool f64, const std::vector<double> 
 extern "C" int suli_upload(int field, const double *host, int64_t n) {
   NEEDCTX();
   Ctx &c = *g_ctx;
+  if (is_plane(field)) {
+    if (n != suli_field_len(field)) {
+      suli_set_error("suli_upload: plane " + std::to_string(field) + " expects " +
+                     std::to_string(suli_field_len(field)) + " values, got " + std::to_string(n));
+      return 1;
+    }
+    return plane_io(c, field - SULI_BXX0, const_cast<double *>(host), true);
+  }
   FieldRef r;
   if (!field_ref(c, field, r) || r.derived) {
     suli_set_error("suli_upload: field " + std::to_string(field) + " cannot be uploaded");
@@ -417,6 +451,14 @@ extern "C" int suli_upload(int field, const double *host, int64_t n) {
 extern "C" int suli_download(int field, double *host, int64_t n) {
   NEEDCTX();
   Ctx &c = *g_ctx;
+  if (is_plane(field)) {
+    if (n != suli_field_len(field)) {
+      suli_set_error("suli_download: plane " + std::to_string(field) + " expects " +
+                     std::to_string(suli_field_len(field)) + " values, got " + std::to_string(n));
+      return 1;
+    }
+    return plane_io(c, field - SULI_BXX0, host, false);
+  }
   FieldRef r;
   if (!field_ref(c, field, r)) {
     suli_set_error("suli_download: unknown field " + std::to_string(field));
@@ -555,7 +597,8 @@ extern "C" int suli_level_set(double dt, suli_ls_stats *st) {
   double vol = 0;
   if (heavi_curv(c, false, &vol)) return 1;
   int nc = 0;
-  while (std::fabs(vol - c.vol_ini) / vol > 0.1) {
+  // (the unsteady manufactured solution corrects nothing, :257)
+  while (std::fabs(vol - c.vol_ini) / vol > 0.1 && c.mms.t != 2) {
     { ProfScope ps_(c, "volume_correction"); DISPATCH(c, Launch<TT>::masscorr(g, c.st, c.ls, c.modls, dt1 * (vol - c.vol_ini), c.vol_ini)); }
     if (heavi_curv(c, false, &vol)) return 1;
     nc++;
@@ -766,8 +809,9 @@ extern "C" int suli_contorno(int nlock) {
     HIPCHK(hipGetLastError());
     return 0;
   }
+  if (int rc = mms_planes_now(c)) return rc;  // contorno(1|2) calls mms_bc first (3_contorno.f90:30)
   { ProfScope ps_(c, "contorno");
-    DISPATCH(c, Launch<TT>::contorno(c.g, c.st, nlock, c.u, c.v, c.w, c.prd1, c.obst ? c.ku_d : nullptr,
+    DISPATCH(c, Launch<TT>::contorno(c.g, c.st, nlock, c.u, c.v, c.w, c.prd1, c.bp, c.obst ? c.ku_d : nullptr,
                                      c.obst ? c.kv_d : nullptr, c.obst ? c.kw_d : nullptr)); }
   HIPCHK(hipGetLastError());
   return 0;
@@ -898,6 +942,10 @@ extern "C" int suli_stage(int tt, double dt_base, suli_poisson_stats *st) {
   if ((rc = suli_convdiff())) return rc;
   if ((rc = suli_tempo(tt, dt))) return rc;
   if ((rc = suli_contorno(2))) return rc;
+  if (g_ctx->mms.t > 0) {  // the manufactured-solution run has no projection (1_principal.f90:89-93)
+    if (st) *st = suli_poisson_stats{};
+    return 0;
+  }
   if ((rc = suli_graddin(dt, st))) return rc;
   if ((rc = suli_posdin(dt))) return rc;
   if ((rc = suli_contorno(1))) return rc;
@@ -927,6 +975,10 @@ extern "C" int suli_step(double dt, suli_ls_stats *ls, suli_poisson_stats *ps) {
   NEEDCTX();
   g_ctx->t_now += dt;  // t = it*dt of the step (1_principal.f90:65) when started from t = 0
   int rc;
+  // manufactured-solution sources: the steady ones at the first step, the
+  // unsteady ones every step (1_principal.f90:70-71)
+  if (g_ctx->mms.t == 2 || (g_ctx->mms.t == 1 && !g_ctx->mms.src_done))
+    if ((rc = mms_sources_now(*g_ctx))) return rc;
   if ((rc = suli_level_set(dt, ls))) return rc;
   if ((rc = suli_contorno(3))) return rc;
   const int ntt = time_stages(g_ctx->t_tempo);

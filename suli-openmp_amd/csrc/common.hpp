@@ -49,6 +49,44 @@ struct Grid {
   }
 };
 
+// Boundary planes of the prescribed-velocity walls (wall code 3; bxy0 also
+// serves code 4 at x = 0), in the declaration order of 0_modules.f90:96-120.
+// All 24 live in one buffer of the field precision; plane n holds its
+// module extents (0:e1-1, 0:e2-1) in Fortran order at off[n].
+enum BPlane {
+  BXX0, BXX1, BXY0, BXZ0, BXXF, BXXF1, BXYF, BXZF,
+  BYX0, BYY0, BYY1, BYZ0, BYXF, BYYF, BYYF1, BYZF,
+  BZX0, BZY0, BZZ0, BZZ1, BZXF, BZYF, BZZF, BZZF1, BP_N
+};
+struct BPlanes {
+  int64_t off[BP_N + 1];  // element offsets; off[BP_N] = total
+  int e1[BP_N], e2[BP_N]; // extents along the plane's two axes
+  __host__ __device__ __forceinline__ int64_t at(int n, int a, int b) const { return off[n] + a + (int64_t)e1[n] * b; }
+};
+// wall axis (0 x, 1 y, 2 z) and velocity component (0 u, 1 v, 2 w) of plane n
+__host__ __device__ __forceinline__ int bplane_wall(int n) { return n / 8; }
+__host__ __device__ __forceinline__ int bplane_comp(int n) {
+  // per wall: the normal component (4 planes: ghost, first face, last face,
+  // outer ghost) and the two tangential components at each end
+  constexpr int c[BP_N] = {0, 0, 1, 2, 0, 0, 1, 2, 0, 1, 1, 2, 0, 1, 1, 2, 0, 1, 2, 2, 0, 1, 2, 2};
+  return c[n];
+}
+inline BPlanes bplanes_layout(int nx, int ny, int nz) {
+  BPlanes L{};
+  const int n[3] = {nx, ny, nz};
+  int64_t o = 0;
+  for (int q = 0; q < BP_N; q++) {
+    const int wall = bplane_wall(q), comp = bplane_comp(q);
+    const int a0 = wall == 0 ? 1 : 0, a1 = wall == 2 ? 1 : 2;  // the two in-plane axes
+    L.e1[q] = n[a0] + (comp == a0 ? 3 : 2);
+    L.e2[q] = n[a1] + (comp == a1 ? 3 : 2);
+    L.off[q] = o;
+    o += (int64_t)L.e1[q] * L.e2[q];
+  }
+  L.off[BP_N] = o;
+  return L;
+}
+
 // A frame field read through 32-bit byte offsets: base + zext(c * sizeof(T)).
 // Frame fields hold fewer than 2^29 elements (checked at set-up), so the byte
 // offset fits in 32 bits, and the load takes the scalar-base + 32-bit-offset

@@ -73,6 +73,17 @@ struct Ctx {
   std::vector<int32_t> ku_h, kv_h, kw_h;
   // face viscosities xnut, ynut, znut of the directional model (m_turb = 2)
   void *xnf = nullptr, *ynf = nullptr, *znf = nullptr;
+  // boundary planes of the prescribed-velocity walls (wall code 3; layout
+  // bplanes_layout in common.hpp), field precision, zero until set
+  void *bp = nullptr;
+  // manufactured solution of 13_mms.f90 (t = mms_t: 0 off, 1 steady, 2 unsteady)
+  struct {
+    int t = 0;
+    double a = 0.0, h0 = 2.0, coef = 0.0;
+    bool src_done = false;     // the steady sources are computed once (it = 1)
+    double *erro_p = nullptr;  // squared pressure errors (nx*ny*nz), kept between calls
+    double *part = nullptr;    // block partials of the error sums
+  } mms;
   bool ab2_started = false;     // fu0 holds the forces of the last AB2 step
   bool tf_on = false;                                    // set by the first upload of a source
   double *prd1 = nullptr;
@@ -111,6 +122,11 @@ void suli_set_error(const std::string &m);
 // address of the current device's scratch buffer of suli_est_div_noalloc
 // (est_noalloc.hip), looked up once per device
 int suli_noalloc_setup(double **out);
+// manufactured-solution mode (mms.hip): the boundary planes and the momentum
+// sources at the context's time t_now (no-ops when the mode is off; the
+// steady planes only at t = 0)
+int mms_planes_now(Ctx &c);
+int mms_sources_now(Ctx &c);
 
 #define HIPCHK(x)                                                                    \
   do {                                                                               \

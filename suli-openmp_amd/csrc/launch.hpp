@@ -42,8 +42,11 @@ struct Launch {
   // mode: MomUpdate (stencils.hpp)
   static void momentum(const Grid &g, const Phys &ph, hipStream_t s, const MomPtrs &m, double dt,
                        int mode);
+  // bp: the boundary planes of wall code 3 (layout bplanes_layout; read only
+  // where a wall has code 3, or 4 at x = 0)
   static void contorno(const Grid &g, hipStream_t s, int nlock, void *u, void *v, void *w, double *p,
-                       const int *ku = nullptr, const int *kv = nullptr, const int *kw = nullptr);
+                       const void *bp, const int *ku = nullptr, const int *kv = nullptr,
+                       const int *kw = nullptr);
   static void obst_ls(const Grid &g, hipStream_t s, void *ls, const int *kw);
   // directional Smagorinsky face viscosities (m_turb = 2)
   static void visco_dir(const Grid &g, const Phys &ph, hipStream_t s, const void *u, const void *v, const void *w,

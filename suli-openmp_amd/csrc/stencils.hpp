@@ -700,7 +700,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 // ---------------------------------------------------------------------------
 // Boundary conditions contorno(nlock) (3_contorno.f90:29-443) with prd_corr
 // (:872-950, whose corrections are all zero).  Wall codes: 0 periodic,
-// 1 free slip, 2 no slip, 4 outflow (x = xf only).  The reference fills the
+// 1 free slip, 2 no slip, 3 prescribed (ghosts and wall-normal faces from the
+// boundary planes, :80-84,109-113,136-140,185-189,214-218,236-240 and their
+// nlock = 2 counterparts), 4 outflow at x = xf and, at x = 0, the inflow of
+// :143-146 (v from the plane bxy0, w copied).  The reference fills the
 // ghost planes in the order y walls, x walls, z walls (pressure: x, y, z;
 // plain copies commute), so each phase is one launch over its ghost planes.
 // Bottom obstacle in contorno(1), contorno(2) (3_contorno.f90:266-291):
@@ -755,7 +758,7 @@ __global__ void k_obst_ls(Grid g, T *__restrict__ ls, const int *__restrict__ kw
 
 template <typename T>
 __global__ void k_bc_y(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
-                       double *__restrict__ p) {
+                       double *__restrict__ p, const T *__restrict__ bp, BPlanes L) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx + 2 || k > nz + 2) return;
@@ -768,31 +771,37 @@ __global__ void k_bc_y(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restri
     if (per) u[g.idx(i, 0, k)] = u[g.idx(i, ny, k)];
     else if (g.ccy0 == 1) u[g.idx(i, 0, k)] = u[g.idx(i, 1, k)];
     else if (g.ccy0 == 2) u[g.idx(i, 0, k)] = -u[g.idx(i, 1, k)];
+    else if (g.ccy0 == 3) u[g.idx(i, 0, k)] = bp[L.at(BYX0, i, k)];
     if (per) u[g.idx(i, ny + 1, k)] = u[g.idx(i, 1, k)];
     else if (g.ccyf == 1) u[g.idx(i, ny + 1, k)] = u[g.idx(i, ny, k)];
     else if (g.ccyf == 2) u[g.idx(i, ny + 1, k)] = -u[g.idx(i, ny, k)];
+    else if (g.ccyf == 3) u[g.idx(i, ny + 1, k)] = bp[L.at(BYXF, i, k)];
   }
   if (i <= nx + 1 && k <= nz + 1) {  // v
     if (per) { v[g.idx(i, 0, k)] = v[g.idx(i, ny, k)]; v[g.idx(i, 1, k)] = v[g.idx(i, ny + 1, k)]; }
     else if (g.ccy0 == 1) v[g.idx(i, 0, k)] = -v[g.idx(i, 2, k)];
     else if (g.ccy0 == 2) v[g.idx(i, 0, k)] = v[g.idx(i, 2, k)];
+    else if (g.ccy0 == 3) v[g.idx(i, 0, k)] = bp[L.at(BYY0, i, k)];
     if (per) v[g.idx(i, ny + 2, k)] = v[g.idx(i, 2, k)];
     else if (g.ccyf == 1) v[g.idx(i, ny + 2, k)] = -v[g.idx(i, ny, k)];
     else if (g.ccyf == 2) v[g.idx(i, ny + 2, k)] = v[g.idx(i, ny, k)];
+    else if (g.ccyf == 3) v[g.idx(i, ny + 2, k)] = bp[L.at(BYYF1, i, k)];
   }
   if (i <= nx + 1) {  // w: k in [0,nz+2]
     if (per) w[g.idx(i, 0, k)] = w[g.idx(i, ny, k)];
     else if (g.ccy0 == 1) w[g.idx(i, 0, k)] = w[g.idx(i, 1, k)];
     else if (g.ccy0 == 2) w[g.idx(i, 0, k)] = -w[g.idx(i, 1, k)];
+    else if (g.ccy0 == 3) w[g.idx(i, 0, k)] = bp[L.at(BYZ0, i, k)];
     if (per) w[g.idx(i, ny + 1, k)] = w[g.idx(i, 1, k)];
     else if (g.ccyf == 1) w[g.idx(i, ny + 1, k)] = w[g.idx(i, ny, k)];
     else if (g.ccyf == 2) w[g.idx(i, ny + 1, k)] = -w[g.idx(i, ny, k)];
+    else if (g.ccyf == 3) w[g.idx(i, ny + 1, k)] = bp[L.at(BYZF, i, k)];
   }
 }
 
 template <typename T>
 __global__ void k_bc_x(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
-                       double *__restrict__ p) {
+                       double *__restrict__ p, const T *__restrict__ bp, BPlanes L) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (j > ny + 2 || k > nz + 2) return;
@@ -804,32 +813,39 @@ __global__ void k_bc_x(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restri
     if (per) u[g.idx(0, j, k)] = u[g.idx(nx, j, k)];
     else if (g.ccx0 == 1) u[g.idx(0, j, k)] = -u[g.idx(2, j, k)];
     else if (g.ccx0 == 2) u[g.idx(0, j, k)] = u[g.idx(2, j, k)];
+    else if (g.ccx0 == 3) u[g.idx(0, j, k)] = bp[L.at(BXX0, j, k)];
     if (per) u[g.idx(nx + 2, j, k)] = u[g.idx(2, j, k)];
     else if (g.ccxf == 1) u[g.idx(nx + 2, j, k)] = -u[g.idx(nx, j, k)];
     else if (g.ccxf == 2) u[g.idx(nx + 2, j, k)] = u[g.idx(nx, j, k)];
+    else if (g.ccxf == 3) u[g.idx(nx + 2, j, k)] = bp[L.at(BXXF1, j, k)];
     else if (g.ccxf == 4) u[g.idx(nx + 2, j, k)] = u[g.idx(nx + 1, j, k)];
   }
   if (k <= nz + 1) {  // v: j in [0,ny+2]
     if (per) v[g.idx(0, j, k)] = v[g.idx(nx, j, k)];
     else if (g.ccx0 == 1) v[g.idx(0, j, k)] = v[g.idx(1, j, k)];
     else if (g.ccx0 == 2) v[g.idx(0, j, k)] = -v[g.idx(1, j, k)];
+    else if (g.ccx0 == 3 || g.ccx0 == 4) v[g.idx(0, j, k)] = bp[L.at(BXY0, j, k)];
     if (per) v[g.idx(nx + 1, j, k)] = v[g.idx(1, j, k)];
     else if (g.ccxf == 1 || g.ccxf == 4) v[g.idx(nx + 1, j, k)] = v[g.idx(nx, j, k)];
     else if (g.ccxf == 2) v[g.idx(nx + 1, j, k)] = -v[g.idx(nx, j, k)];
+    else if (g.ccxf == 3) v[g.idx(nx + 1, j, k)] = bp[L.at(BXYF, j, k)];
   }
   if (j <= ny + 1) {  // w: k in [0,nz+2]
     if (per) w[g.idx(0, j, k)] = w[g.idx(nx, j, k)];
     else if (g.ccx0 == 1) w[g.idx(0, j, k)] = w[g.idx(1, j, k)];
     else if (g.ccx0 == 2) w[g.idx(0, j, k)] = -w[g.idx(1, j, k)];
+    else if (g.ccx0 == 3) w[g.idx(0, j, k)] = bp[L.at(BXZ0, j, k)];
+    else if (g.ccx0 == 4) w[g.idx(0, j, k)] = w[g.idx(1, j, k)];
     if (per) w[g.idx(nx + 1, j, k)] = w[g.idx(1, j, k)];
     else if (g.ccxf == 1 || g.ccxf == 4) w[g.idx(nx + 1, j, k)] = w[g.idx(nx, j, k)];
     else if (g.ccxf == 2) w[g.idx(nx + 1, j, k)] = -w[g.idx(nx, j, k)];
+    else if (g.ccxf == 3) w[g.idx(nx + 1, j, k)] = bp[L.at(BXZF, j, k)];
   }
 }
 
 template <typename T>
 __global__ void k_bc_z(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
-                       double *__restrict__ p) {
+                       double *__restrict__ p, const T *__restrict__ bp, BPlanes L) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx + 2 || j > ny + 2) return;
@@ -840,37 +856,46 @@ __global__ void k_bc_z(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restri
   if (j <= ny + 1) {  // u: i in [0,nx+2]
     if (g.ccz0 == 1) u[g.idx(i, j, 0)] = u[g.idx(i, j, 1)];
     else if (g.ccz0 == 2) u[g.idx(i, j, 0)] = -u[g.idx(i, j, 1)];
+    else if (g.ccz0 == 3) u[g.idx(i, j, 0)] = bp[L.at(BZX0, i, j)];
     if (g.cczf == 1) u[g.idx(i, j, nz + 1)] = u[g.idx(i, j, nz)];
     else if (g.cczf == 2) u[g.idx(i, j, nz + 1)] = -u[g.idx(i, j, nz)];
+    else if (g.cczf == 3) u[g.idx(i, j, nz + 1)] = bp[L.at(BZXF, i, j)];
   }
   if (i <= nx + 1) {  // v: j in [0,ny+2]
     if (g.ccz0 == 1) v[g.idx(i, j, 0)] = v[g.idx(i, j, 1)];
     else if (g.ccz0 == 2) v[g.idx(i, j, 0)] = -v[g.idx(i, j, 1)];
+    else if (g.ccz0 == 3) v[g.idx(i, j, 0)] = bp[L.at(BZY0, i, j)];
     if (g.cczf == 1) v[g.idx(i, j, nz + 1)] = v[g.idx(i, j, nz)];
     else if (g.cczf == 2) v[g.idx(i, j, nz + 1)] = -v[g.idx(i, j, nz)];
+    else if (g.cczf == 3) v[g.idx(i, j, nz + 1)] = bp[L.at(BZYF, i, j)];
   }
   if (i <= nx + 1 && j <= ny + 1) {  // w
     if (g.ccz0 == 1) w[g.idx(i, j, 0)] = -w[g.idx(i, j, 2)];
     else if (g.ccz0 == 2) w[g.idx(i, j, 0)] = w[g.idx(i, j, 2)];
+    else if (g.ccz0 == 3) w[g.idx(i, j, 0)] = bp[L.at(BZZ0, i, j)];
     if (g.cczf == 1) w[g.idx(i, j, nz + 2)] = -w[g.idx(i, j, nz)];
     else if (g.cczf == 2) w[g.idx(i, j, nz + 2)] = w[g.idx(i, j, nz)];
+    else if (g.cczf == 3) w[g.idx(i, j, nz + 2)] = bp[L.at(BZZF1, i, j)];
   }
 }
 
 // contorno(2) wall-normal velocities (3_contorno.f90:297-443), in three
 // launches: y walls, x walls (with the ccxf=2 ghost refresh), z walls.
 template <typename T>
-__global__ void k_bcn_y(Grid g, T *__restrict__ v) {
+__global__ void k_bcn_y(Grid g, T *__restrict__ v, const T *__restrict__ bp, BPlanes L) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.nx + 1 || k > g.nz + 1) return;
   const bool per = g.ccy0 == 0 && g.ccyf == 0;
   if (per) v[g.idx(i, 1, k)] = v[g.idx(i, g.ny + 1, k)];
   else if (g.ccy0 == 1 || g.ccy0 == 2) v[g.idx(i, 1, k)] = T(0.) + T(0.);
+  else if (g.ccy0 == 3) v[g.idx(i, 1, k)] = bp[L.at(BYY1, i, k)] + T(0.);
   if (per) v[g.idx(i, g.ny + 1, k)] = v[g.idx(i, 1, k)];
   else if (g.ccyf == 1 || g.ccyf == 2) v[g.idx(i, g.ny + 1, k)] = T(0.) + T(0.);
+  else if (g.ccyf == 3) v[g.idx(i, g.ny + 1, k)] = bp[L.at(BYYF, i, k)] + T(0.);
 }
 template <typename T>
-__global__ void k_bcn_x(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w) {
+__global__ void k_bcn_x(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
+                        const T *__restrict__ bp, BPlanes L) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
   const int nx = g.nx;
   if (j > g.ny + 2 || k > g.nz + 2) return;
@@ -878,8 +903,10 @@ __global__ void k_bcn_x(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restr
   if (j <= g.ny + 1 && k <= g.nz + 1) {
     if (per) u[g.idx(1, j, k)] = u[g.idx(nx + 1, j, k)];
     else if (g.ccx0 == 1 || g.ccx0 == 2) u[g.idx(1, j, k)] = T(0.) + T(0.);
+    else if (g.ccx0 == 3) u[g.idx(1, j, k)] = bp[L.at(BXX1, j, k)] + T(0.);
     if (per) u[g.idx(nx + 1, j, k)] = u[g.idx(1, j, k)];
     else if (g.ccxf == 1) u[g.idx(nx + 1, j, k)] = T(0.) + T(0.);
+    else if (g.ccxf == 3) u[g.idx(nx + 1, j, k)] = bp[L.at(BXXF, j, k)] + T(0.);
     else if (g.ccxf == 2) {
       u[g.idx(nx + 1, j, k)] = T(0.) + T(0.);
       u[g.idx(nx + 2, j, k)] = u[g.idx(nx, j, k)];
@@ -891,11 +918,13 @@ __global__ void k_bcn_x(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restr
   }
 }
 template <typename T>
-__global__ void k_bcn_z(Grid g, T *__restrict__ w) {
+__global__ void k_bcn_z(Grid g, T *__restrict__ w, const T *__restrict__ bp, BPlanes L) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.nx + 1 || j > g.ny + 1) return;
   if (g.ccz0 == 1 || g.ccz0 == 2) w[g.idx(i, j, 1)] = T(0.) + T(0.);
+  else if (g.ccz0 == 3) w[g.idx(i, j, 1)] = bp[L.at(BZZ1, i, j)] + T(0.);
   if (g.cczf == 1 || g.cczf == 2) w[g.idx(i, j, g.nz + 1)] = T(0.) + T(0.);
+  else if (g.cczf == 3) w[g.idx(i, j, g.nz + 1)] = bp[L.at(BZZF, i, j)] + T(0.);
 }
 
 // ---------------------------------------------------------------------------

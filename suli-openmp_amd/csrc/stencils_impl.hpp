@@ -112,18 +112,20 @@ void Launch<T>::visco_dir(const Grid &g, const Phys &ph, hipStream_t s, const vo
 
 template <typename T>
 void Launch<T>::contorno(const Grid &g, hipStream_t s, int nlock, void *u, void *v, void *w, double *p,
-                         const int *ku, const int *kv, const int *kw) {
+                         const void *bp, const int *ku, const int *kv, const int *kw) {
   if (nlock != 1 && nlock != 2) return;
   const dim3 b(64, 4, 1);
   auto gr = [&](int a, int c) { return dim3((a + 63) / 64, (c + 3) / 4, 1); };
-  SULI_LAUNCH(k_bc_y<T>, gr(g.nx + 3, g.nz + 3), b, s, g, (T *)u, (T *)v, (T *)w, p);
-  SULI_LAUNCH(k_bc_x<T>, gr(g.ny + 3, g.nz + 3), b, s, g, (T *)u, (T *)v, (T *)w, p);
-  SULI_LAUNCH(k_bc_z<T>, gr(g.nx + 3, g.ny + 3), b, s, g, (T *)u, (T *)v, (T *)w, p);
+  const BPlanes L = bplanes_layout(g.nx, g.ny, g.nz);
+  const T *P = (const T *)bp;
+  SULI_LAUNCH(k_bc_y<T>, gr(g.nx + 3, g.nz + 3), b, s, g, (T *)u, (T *)v, (T *)w, p, P, L);
+  SULI_LAUNCH(k_bc_x<T>, gr(g.ny + 3, g.nz + 3), b, s, g, (T *)u, (T *)v, (T *)w, p, P, L);
+  SULI_LAUNCH(k_bc_z<T>, gr(g.nx + 3, g.ny + 3), b, s, g, (T *)u, (T *)v, (T *)w, p, P, L);
   if (ku) SULI_LAUNCH(k_bc_obst<T>, gr(g.nx + 3, g.ny + 3), b, s, g, (T *)u, (T *)v, (T *)w, ku, kv, kw);
   if (nlock == 2) {
-    SULI_LAUNCH(k_bcn_y<T>, gr(g.nx + 2, g.nz + 2), b, s, g, (T *)v);
-    SULI_LAUNCH(k_bcn_x<T>, gr(g.ny + 3, g.nz + 3), b, s, g, (T *)u, (T *)v, (T *)w);
-    SULI_LAUNCH(k_bcn_z<T>, gr(g.nx + 2, g.ny + 2), b, s, g, (T *)w);
+    SULI_LAUNCH(k_bcn_y<T>, gr(g.nx + 2, g.nz + 2), b, s, g, (T *)v, P, L);
+    SULI_LAUNCH(k_bcn_x<T>, gr(g.ny + 3, g.nz + 3), b, s, g, (T *)u, (T *)v, (T *)w, P, L);
+    SULI_LAUNCH(k_bcn_z<T>, gr(g.nx + 2, g.ny + 2), b, s, g, (T *)w, P, L);
   }
 }
 

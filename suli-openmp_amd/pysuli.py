@@ -29,6 +29,20 @@ KIND = {"u": "U", "v": "V", "w": "W", "prd1": "P", "xnut": "FX", "ynut": "FY", "
         "Fu": "FX", "Fv": "FY", "Fw": "FZ", "u0": "FX", "v0": "FY", "w0": "FZ", "tf_u": "FX", "tf_v": "FY",
         "tf_w": "FZ", "fu0": "FX", "fv0": "FY", "fw0": "FZ", "fu1": "FX", "fv1": "FY", "fw1": "FZ"}
 TIME_EULER, TIME_RK2, TIME_RK3, TIME_AB2 = 0, 1, 2, 3  # t_tempo, 0_modules.f90:24
+# boundary planes of the prescribed-velocity walls (wall code 3), field ids
+# 64.. in the declaration order of 0_modules.f90:96-120
+PLANES = ["bxx0", "bxx1", "bxy0", "bxz0", "bxxf", "bxxf1", "bxyf", "bxzf",
+          "byx0", "byy0", "byy1", "byz0", "byxf", "byyf", "byyf1", "byzf",
+          "bzx0", "bzy0", "bzz0", "bzz1", "bzxf", "bzyf", "bzzf", "bzzf1"]
+FIELD_ID.update({n: 64 + q for q, n in enumerate(PLANES)})
+
+
+def plane_shape(name: str, nx: int, ny: int, nz: int) -> tuple:
+    """Module extents of a boundary plane: the wall's two in-plane axes, with
+    one more entry along the axis of the plane's velocity component."""
+    n = {"x": nx, "y": ny, "z": nz}
+    wall, comp = name[1], name[2]
+    return tuple(n[a] + (3 if a == comp else 2) for a in "xyz" if a != wall)
 
 
 def extent_shape(kind: str, nx: int, ny: int, nz: int) -> tuple:
@@ -128,6 +142,8 @@ def lib():
             "suli_obstacle_columns": ([i32, i32, i32, i32, f64, f64, f64, C.c_void_p, C.c_void_p, C.c_void_p], i32),
             "suli_restart_write": ([C.c_char_p], i32), "suli_restart_read": ([C.c_char_p], i32),
             "suli_write_xdmf": ([C.c_char_p, i32, i32, i32, i32, i32, i32, f64], i32),
+            "suli_mms_set": ([i32, f64, f64, f64], i32), "suli_mms_i": ([], i32), "suli_mms_bc": ([], i32),
+            "suli_termo_fonte1": ([], i32), "suli_termo_fonte2": ([], i32), "suli_mms": ([p(f64)], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -172,6 +188,8 @@ class Solver:
         lib().suli_finalize()
 
     def shape(self, name):
+        if name in PLANES:
+            return plane_shape(name, self.nx, self.ny, self.nz)
         return extent_shape(KIND.get(name, "C"), self.nx, self.ny, self.nz)
 
     def upload(self, name, arr):
@@ -259,6 +277,25 @@ class Solver:
 
     def set_time(self, t):
         _check(lib().suli_set_time(t), "suli_set_time")
+
+    # -- the manufactured solution of 13_mms.f90 --------------------------------
+    def mms_set(self, mms_t, a=0.0, h0=2.0, coef=0.0):
+        _check(lib().suli_mms_set(mms_t, a, h0, coef), "suli_mms_set")
+
+    def mms_i(self):
+        _check(lib().suli_mms_i(), "suli_mms_i")
+
+    def mms_bc(self):
+        _check(lib().suli_mms_bc(), "suli_mms_bc")
+
+    def termo_fonte(self, mms_t):
+        fn = lib().suli_termo_fonte1 if mms_t == 1 else lib().suli_termo_fonte2
+        _check(fn(), f"suli_termo_fonte{mms_t}")
+
+    def mms(self):
+        e = (C.c_double * 4)()
+        _check(lib().suli_mms(e), "suli_mms")
+        return tuple(e)
 
     def set_solver(self, poisson, precond, tol_rel=0.0, max_it=0):
         _check(lib().suli_set_solver(poisson, precond, tol_rel, max_it), "suli_set_solver")

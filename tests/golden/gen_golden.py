@@ -14,6 +14,7 @@ the call changed in tests/golden/<case>.npz (float64, Fortran order).
     python tests/golden/gen_golden.py geometry   # obstaculo's columns per obst_t (obstaculo_*)
     python tests/golden/gen_golden.py est        # est's divergence and its maximum (est/est_*)
     python tests/golden/gen_golden.py periodic   # graddin on the x-periodic grid (per_graddin)
+    python tests/golden/gen_golden.py weng       # the manufactured solution of 13_mms.f90 (weng/*)
 
 The scheme cases use reference builds with t_tempo = 0, 2, 3
 (0_modules.f90:24) and start from nonzero kept forces fu0, fu1.
@@ -204,6 +205,54 @@ def periodic():
     print("per graddin changed:", changed)
 
 
+# the manufactured-solution cases: (mms_t, a, coef); grid 12x10x8 of
+# 0.5 x 0.6 x 0.3, every wall prescribed (code 3), no turbulence model, no
+# gravity (the mode has no projection to balance it)
+WENG_CASES = {"weng1": (1, 0.0, 0.0), "weng2": (2, 0.0, 1.0), "weng1a": (1, 0.5, 0.0), "weng2a": (2, 0.5, 1.0)}
+WENG_DT, WENG_STEPS, WENG_T_SRC = 0.01, 5, 0.7
+
+
+def weng_params(mms_t, a, coef):
+    return O.Params(nx=12, ny=10, nz=8, dx=0.5, dy=0.6, dz=0.3, ccx0=3, ccxf=3, ccy0=3, ccyf=3, ccz0=3, cczf=3,
+                    m_turb=0, gx=0.0, gz=0.0, mms_t=mms_t, mms_a=a, mms_h0=2.0, mms_coef=coef)
+
+
+def weng():
+    """The manufactured-solution mode of reference builds with mms_t = 1, 2
+    (13_mms.f90): the run of 1_principal.f90:39-108 from t = 0 for WENG_STEPS
+    steps (harness mms_run: final fields, the errors of mms() at t = 0 and
+    after every step) and, at t = WENG_T_SRC (the steady planes at t = 0),
+    mms_bc's planes and the sources of termo_fonte1/termo_fonte2 (harness
+    mms_src).  a = 0 is the reference's own setting; with a = 0.5 the
+    reference's hand-derived sources differ from the exact ones, so those
+    cases store the initial state (mms_run with 0 steps), the planes and the
+    reference's sources only."""
+    os.makedirs(os.path.join(HERE, "weng"), exist_ok=True)
+    for case, (mms_t, a, coef) in WENG_CASES.items():
+        p = weng_params(mms_t, a, coef)
+        O.ref_build(p)
+        s = O.State(p)
+        s.scal["dt"] = WENG_DT
+        nsteps = WENG_STEPS if a == 0.0 else 0
+        out, ex = O.ref_run_mms(s, "mms_run", nsteps)
+        errs = np.array([[float(x) for x in l.split()[2:]] for l in out.splitlines() if l.startswith("MMS_ERR")])
+        src = O.State(p)
+        src.scal["dt"] = WENG_DT
+        src.scal["t"] = 0.0 if mms_t == 1 else WENG_T_SRC
+        _, exs = O.ref_run_mms(src, "mms_src")
+        data = {"mms": np.array([mms_t, a, 2.0, coef]), "dt": np.array([WENG_DT]), "nsteps": np.array([nsteps]),
+                "t_src": np.array([src.scal["t"]]), "errors": errs,
+                "grid": np.array([p.nx, p.ny, p.nz]), "h": np.array([p.dx, p.dy, p.dz])}
+        for n in ("u", "v", "w", "ls", "prd1"):
+            data["out_" + n] = s.arr(n)
+        for n in O.PLANE_NAMES:
+            data["plane_" + n] = exs[n]
+        for n in ("tf_u", "tf_v", "tf_w"):
+            data["src_" + n] = exs[n]
+        np.savez_compressed(os.path.join(HERE, "weng", f"{case}.npz"), **data)
+        print(case, "errors after the last step:", errs[-1])
+
+
 def main():
     for case, p in CASES.items():
         O.ref_build(p)
@@ -260,5 +309,7 @@ if __name__ == "__main__":
         est()
     elif sys.argv[1:] == ["periodic"]:
         periodic()
+    elif sys.argv[1:] == ["weng"]:
+        weng()
     else:
         main()
