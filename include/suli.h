@@ -423,7 +423,8 @@ int suli_write_xdmf(const char *path, int nx, int ny, int nz, int first, int las
  * 11_diversos.f90:232-329).  suli_checkpoint_save/load: binary, the device
  * state of the run as it is (fields with their ghosts in the run's
  * precision, volumes, time, forcing, time scheme, the projection's guess
- * history and the bottom-obstacle columns), so that a run resumed from it
+ * history, the bottom-obstacle columns, the boundary planes and the
+ * manufactured-solution mode with its pressure errors), so that a run resumed from it
  * continues bit for bit; loads into a context of the same grid and
  * precision only, and the file's obstacle (or its absence) replaces the
  * context's.  suli_restart_write/read: the reference's own restart file
@@ -432,10 +433,11 @@ int suli_write_xdmf(const char *path, int nx, int ny, int nz, int first, int las
  * reading a file replaces the context's obstacle with the file's columns
  * (a context with an obstacle that reads all-zero columns keeps the
  * obstacle switched on, every column at k = 0, its friction planes at
- * k = 1 as the reference's).  Items outside this library's model
- * (prescribed boundary planes, d_max, d_min, ls_m, tf_u/v/w) are written
- * as zeros and skipped on reading, as are items it rebuilds (prd0, prd, ub,
- * vb, wb, xnut, ynut, znut). */
+ * k = 1 as the reference's).  The boundary planes and the momentum sources
+ * tf_u, tf_v, tf_w go both ways (sources read only when nonzero).  Items
+ * outside this library's model (d_max, d_min, ls_m) are written as zeros
+ * and skipped on reading, as are items it rebuilds (prd0, prd, ub, vb, wb,
+ * xnut, ynut, znut). */
 int suli_checkpoint_save(const char *path);
 int suli_checkpoint_load(const char *path);
 int suli_restart_write(const char *path);

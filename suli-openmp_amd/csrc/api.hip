@@ -1000,7 +1000,7 @@ static const int kCkptFields[] = {SULI_U,      SULI_V,      SULI_W,      SULI_PR
                                   SULI_HSZ,    SULI_DDLSDX, SULI_DDLSDY, SULI_DDLSDZ, SULI_NUT,  SULI_U0,
                                   SULI_V0,     SULI_W0,     SULI_TFU,    SULI_TFV,    SULI_TFW,  SULI_FU0,
                                   SULI_FV0,    SULI_FW0};
-static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '3'};
+static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '4'};
 
 struct CkptHeader {
   char magic[8];
@@ -1012,6 +1012,10 @@ struct CkptHeader {
   double p2dt;
   int32_t ab2_started, adv;
   int32_t obst, pad_;  // 1: the bottom-obstacle columns ku, kv, kw follow the pressure history
+  // manufactured-solution mode; the boundary planes (field precision) follow
+  // the obstacle columns, then, in the mode, its pressure errors (nx*ny*nz FP64)
+  int32_t mms_t, mms_src_done;
+  double mms_a, mms_h0, mms_coef;
 };
 
 // Binary checkpoint: the device frames of the state fields as they are (the
@@ -1037,6 +1041,8 @@ extern "C" int suli_checkpoint_save(const char *path) {
   h.p2dt = c.p2dt;
   h.ab2_started = c.ab2_started ? 1 : 0; h.adv = c.adv;
   h.obst = c.obst ? 1 : 0;
+  h.mms_t = c.mms.t; h.mms_src_done = c.mms.src_done ? 1 : 0;
+  h.mms_a = c.mms.a; h.mms_h0 = c.mms.h0; h.mms_coef = c.mms.coef;
   HIPCHK(hipStreamSynchronize(c.st));
   FILE *f = std::fopen(path, "wb");
   if (!f) {
@@ -1071,6 +1077,18 @@ extern "C" int suli_checkpoint_save(const char *path) {
   if (h.obst && ok)  // the obstacle columns in the reference extents, as suli_set_obstacle took them
     for (const std::vector<int32_t> *k : {&c.ku_h, &c.kv_h, &c.kw_h})
       ok = ok && std::fwrite(k->data(), 4, k->size(), f) == k->size();
+  {
+    const size_t pb = (size_t)bplanes_layout(c.g.nx, c.g.ny, c.g.nz).off[BP_N] * c.esz;
+    buf.resize(pb);
+    if (ok && hipMemcpy(buf.data(), c.bp, pb, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+    ok = ok && std::fwrite(buf.data(), 1, pb, f) == pb;
+  }
+  if (h.mms_t > 0 && ok) {
+    const size_t eb = (size_t)c.g.nx * c.g.ny * c.g.nz * 8;
+    buf.resize(eb);
+    if (hipMemcpy(buf.data(), c.mms.erro_p, eb, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+    ok = ok && std::fwrite(buf.data(), 1, eb, f) == eb;
+  }
   if (std::fclose(f) != 0 || !ok) {
     suli_set_error(std::string("suli_checkpoint_save: write failed on ") + path);
     return 1;
@@ -1127,6 +1145,16 @@ extern "C" int suli_checkpoint_load(const char *path) {
     kcol[q].resize(kn[q]);
     if (std::fread(kcol[q].data(), 4, kn[q], f) != kn[q]) err = "truncated at the obstacle columns";
   }
+  std::vector<char> planes, errp;
+  if (err.empty()) {
+    planes.resize((size_t)bplanes_layout(c.g.nx, c.g.ny, c.g.nz).off[BP_N] * c.esz);
+    if (std::fread(planes.data(), 1, planes.size(), f) != planes.size()) err = "truncated at the boundary planes";
+  }
+  if (err.empty() && (h.mms_t < 0 || h.mms_t > 2)) err = "unknown manufactured-solution mode";
+  if (err.empty() && h.mms_t > 0) {
+    errp.resize((size_t)c.g.nx * c.g.ny * c.g.nz * 8);
+    if (std::fread(errp.data(), 1, errp.size(), f) != errp.size()) err = "truncated at the pressure errors";
+  }
   std::fclose(f);
   if (!err.empty()) {
     suli_set_error(std::string("suli_checkpoint_load: ") + path + ": " + err);
@@ -1136,6 +1164,10 @@ extern "C" int suli_checkpoint_load(const char *path) {
   if (h.obst ? suli_set_obstacle(kcol[0].data(), kcol[1].data(), kcol[2].data())
              : suli_set_obstacle(nullptr, nullptr, nullptr))
     return 1;
+  HIPCHK(hipMemcpy(c.bp, planes.data(), planes.size(), hipMemcpyHostToDevice));
+  if (int rc = suli_mms_set(h.mms_t, h.mms_a, h.mms_h0, h.mms_coef)) return rc;
+  if (h.mms_t > 0) HIPCHK(hipMemcpy(c.mms.erro_p, errp.data(), errp.size(), hipMemcpyHostToDevice));
+  c.mms.src_done = h.mms_src_done != 0;
   c.tf_on = h.tf_on != 0;
   c.vol_ini = h.vol_ini; c.vol_ins = h.vol_ins; c.t_now = h.t_now;
   c.force_amp = h.force_amp; c.force_period = h.force_period; c.force_phase = h.force_phase;
@@ -1149,16 +1181,17 @@ extern "C" int suli_checkpoint_load(const char *path) {
 // arrays and scalars of restart_salva (11_diversos.f90:319-321), in that
 // order, with the reference extents (0_modules.f90).  The obstacle columns
 // ku, kv, kw are written from the context's obstacle (zeros without one)
-// and read back into it through suli_set_obstacle.  Items this library does
-// not model are written as the reference holds them in its configuration
-// here and skipped on reading: the prescribed-boundary planes bxx0..bzzf1
-// (0, no inflow), d_max, d_min, ls_m (0), the friction planes ub, vb, wb
-// (rebuilt by contorno), prd0 and prd (copies of prd1), the face
-// viscosities (rebuilt by visco) and tf_u, tf_v, tf_w (zeroed by
-// restart_ini when mms_t = 0, 11_diversos.f90:268-273).
+// and read back into it through suli_set_obstacle.  The boundary planes
+// bxx0..bzzf1 of the prescribed walls go both ways, and so do the momentum
+// sources tf_u, tf_v, tf_w (read only when nonzero, restart_ini zeroes them
+// when mms_t = 0, 11_diversos.f90:268-273).  Items this library does not
+// model are written as the reference holds them in its configuration here
+// and skipped on reading: d_max, d_min, ls_m (0), the friction planes ub,
+// vb, wb (rebuilt by contorno), prd0 and prd (copies of prd1) and the face
+// viscosities (rebuilt by visco).
 namespace {
 struct RItem {
-  int field;          // >= 0: a library field; -1: zeros; -2: scalar
+  int field;          // >= 0: a library field; -1: zeros; -2: scalar; -3 prd1 copy; -4 face viscosity; -5 source
   int n0, n1, n2;     // extents (values written); scalar: 1
   const char *name;
 };
@@ -1178,7 +1211,8 @@ std::vector<RItem> restart_items(const Grid &g) {
                              {nx + 3, nz + 2}, {nx + 2, nz + 2}, {nx + 2, nz + 2}, {nx + 2, nz + 3},   // byxf byyf byyf1 byzf
                              {nx + 3, ny + 2}, {nx + 2, ny + 3}, {nx + 2, ny + 2}, {nx + 2, ny + 2},   // bzx0 bzy0 bzz0 bzz1
                              {nx + 3, ny + 2}, {nx + 2, ny + 3}, {nx + 2, ny + 2}, {nx + 2, ny + 2}};  // bzxf bzyf bzzf bzzf1
-  for (auto &p : planes) v.push_back({-1, p[0], p[1], 1, "bc plane"});
+  // the boundary planes of the prescribed walls (wall code 3, the manufactured solution)
+  for (int q = 0; q < 24; q++) v.push_back({SULI_BXX0 + q, planes[q][0], planes[q][1], 1, "bc plane"});
   v.push_back({-1, nx + 3, ny + 2, nz + 2, "ub"});
   v.push_back({-1, nx + 2, ny + 3, nz + 2, "vb"});
   v.push_back({-1, nx + 2, ny + 2, nz + 3, "wb"});
@@ -1193,9 +1227,11 @@ std::vector<RItem> restart_items(const Grid &g) {
   v.push_back({-4, nx + 1, ny, nz, "xnut"});
   v.push_back({-4, nx, ny + 1, nz, "ynut"});
   v.push_back({-4, nx, ny, nz + 1, "znut"});
-  v.push_back({-1, nx + 1, ny, nz, "tf_u"});
-  v.push_back({-1, nx, ny + 1, nz, "tf_v"});
-  v.push_back({-1, nx, ny, nz + 1, "tf_w"});
+  // momentum sources (-5: read only when nonzero, so that a file without
+  // sources leaves the step's convdiff without the source term)
+  v.push_back({-5, nx + 1, ny, nz, "tf_u"});
+  v.push_back({-5, nx, ny + 1, nz, "tf_v"});
+  v.push_back({-5, nx, ny, nz + 1, "tf_w"});
   v.push_back({-2, 1, 1, 1, "vol_ini"});
   v.push_back({-2, 1, 1, 1, "vol_ins"});
   return v;
@@ -1222,6 +1258,7 @@ extern "C" int suli_restart_write(const char *path) {
     int field = it.field;
     if (field == -3) field = SULI_PRD1;  // prd0, prd: copies of prd1
     if (field == -4) field = it.name[0] == 'x' ? SULI_XNUT : (it.name[0] == 'y' ? SULI_YNUT : SULI_ZNUT);
+    if (field == -5) field = it.name[3] == 'u' ? SULI_TFU : (it.name[3] == 'v' ? SULI_TFV : SULI_TFW);
     if (field >= 0) {
       host.resize((size_t)n);
       if (suli_download(field, host.data(), n)) {
@@ -1315,6 +1352,15 @@ extern "C" int suli_restart_read(const char *path) {
     if (it.field >= 0 && suli_upload(it.field, host.data(), n)) {
       std::fclose(f);
       return 1;
+    }
+    if (it.field == -5) {  // tf_u, tf_v, tf_w
+      bool nz_ = false;
+      for (int64_t q = 0; q < n && !nz_; q++) nz_ = host[(size_t)q] != 0.0;
+      const int fld = it.name[3] == 'u' ? SULI_TFU : (it.name[3] == 'v' ? SULI_TFV : SULI_TFW);
+      if (nz_ && suli_upload(fld, host.data(), n)) {
+        std::fclose(f);
+        return 1;
+      }
     }
     if (it.field == -2) vol[nvol++] = host[0];
     if (it.field == -1 && it.name[0] == 'k' && it.name[2] == 0) {  // ku, kv, kw

@@ -127,6 +127,7 @@ int suli_noalloc_setup(double **out);
 // steady planes only at t = 0)
 int mms_planes_now(Ctx &c);
 int mms_sources_now(Ctx &c);
+extern "C" int suli_mms_set(int mms_t, double a, double h0, double coef);
 
 #define HIPCHK(x)                                                                    \
   do {                                                                               \

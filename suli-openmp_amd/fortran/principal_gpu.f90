@@ -2,7 +2,7 @@
 ! PROGRAM PNH (1_principal.f90:64-108) with its CALL lines unchanged; the
 ! subroutines come from module suli_gpu and run on the GPU.
 !
-!   principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME [T_TEMPO [OBST_T]]]]]
+!   principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME [T_TEMPO [OBST_T [MMS_T]]]]]]
 !
 ! Tank 1.0 x 0.5 x 0.5 m scaled to the grid (h = 1/NX), dam break of a water
 ! column WIDTH x HEIGHT in the x = 0 corner, water/air 1000/1.2 kg/m^3, no-slip
@@ -15,11 +15,15 @@
 ! with the reference extents (0_modules.f90:92-122).  T_TEMPO selects the
 ! time scheme (0_modules.f90:24): 0 Euler, 1 RK2 (default), 2 RK3, 3 AB2.
 ! OBST_T > 0 puts the reference's bottom obstacle of that type (obstaculo,
-! 3_contorno.f90:494-760) into the tank.
+! 3_contorno.f90:494-760) into the tank.  MMS_T > 0 runs the manufactured
+! solution of 13_mms.f90 instead (1 steady, 2 unsteady with coef = 1): tank
+! 2 pi x 2 pi x 2.5 on NX x NY x NZ cells, depth h0 = 2, a = 0 (the
+! reference's setting), every wall prescribed (code 3), no turbulence model,
+! no gravity, dt = 0.05 dx, no projection; prints the errors of mms() per step.
 program principal_gpu
   use suli_gpu
   implicit none
-  integer :: nx, ny, nz, ts, precond, frame, nargs, obst_t
+  integer :: nx, ny, nz, ts, precond, frame, nargs, obst_t, mms_mode
   integer(c_int32_t), allocatable :: ku(:,:), kv(:,:), kw(:,:)
   real(8) :: width, height, h, div, vol_ini
   character(len=64) :: arg
@@ -28,7 +32,7 @@ program principal_gpu
 
   nargs = command_argument_count()
   if (nargs < 4) then
-    write(*,*) 'usage: principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME [T_TEMPO [OBST_T]]]]]'
+    write(*,*) 'usage: principal_gpu NX NY NZ STEPS [PRECOND [WIDTH HEIGHT [FRAME [T_TEMPO [OBST_T [MMS_T]]]]]]'
     stop 2
   endif
   call get_command_argument(1, arg); read(arg, *) nx
@@ -56,6 +60,10 @@ program principal_gpu
   if (nargs >= 10) then
     call get_command_argument(10, arg); read(arg, *) obst_t
   endif
+  mms_mode = 0
+  if (nargs >= 11) then
+    call get_command_argument(11, arg); read(arg, *) mms_mode
+  endif
   h = 1.0d0 / nx
 
   ! iniciais (2_iniciais.f90:10-106) and level_set_ini (14_level_set.f90:10-219)
@@ -68,6 +76,10 @@ program principal_gpu
   prm%csmag = 0.1d0; prm%gx = 0.0d0; prm%gz = 9.81d0; prm%chezy = 44.5d0
   prm%precision = 1; prm%poisson = 1; prm%precond = precond; prm%max_it = 0
   prm%tol_abs = 0.0d0; prm%tol_rel = 1.0d-8; prm%device = 0
+  if (mms_mode > 0) then
+    call run_mms()
+    stop
+  endif
   call check(c_suli_init(prm), 'init')
   call check(c_suli_init_dambreak(width, height, vol_ini), 'init_dambreak')
   if (obst_t > 0) then  ! obstaculo (2_iniciais.f90:49) and the library's columns
@@ -128,4 +140,44 @@ program principal_gpu
     ' poisson_iters/solve=', real(pois_iters, 8) / max(pois_solves, 1_8)
   if (frame > 0) call visu(frame * dt)
   call check(c_suli_finalize(), 'finalize')
+
+contains
+
+  ! the manufactured-solution run of 1_principal.f90:39-108 with mms_t > 0
+  subroutine run_mms()
+    real(8), parameter :: pi = acos(-1.0d0)
+    prm%dx = 2.0d0 * pi / nx; prm%dy = 2.0d0 * pi / ny; prm%dz = 2.5d0 / nz
+    prm%ccx0 = 3; prm%ccxf = 3; prm%ccy0 = 3; prm%ccyf = 3; prm%ccz0 = 3; prm%cczf = 3
+    prm%m_turb = 0; prm%gz = 0.0d0
+    prm%precision = 0  ! FP64
+    call check(c_suli_init(prm), 'init')
+    mms_t = mms_mode
+    mms_a = 0.0d0; mms_h0 = 2.0d0
+    mms_coef = merge(1.0d0, 0.0d0, mms_t == 2)
+    dt = 0.05d0 * prm%dx
+    call tempo_ini()
+    t = 0.0d0
+    call mms_i()
+    call contorno(1)
+    call contorno(3)
+    call mms()
+    do it = 1, ts
+      t = it * dt
+      if ((mms_t == 1) .and. (it == 1)) call termo_fonte1()
+      if (mms_t == 2) call termo_fonte2()
+      call level_set()
+      call contorno(3)
+      do tt = 1, ntt
+        dt = a_dt(tt)
+        call visco()
+        call convdiff()
+        call tempo()
+        call contorno(2)
+      enddo
+      call mms()
+      write(*,'(A,I6,A,ES24.16,A,4ES24.16)') 'mms', it, ' t=', t, ' err=', mms_err
+    enddo
+    write(*,'(A,ES24.16)') 'erro total acumulado=', erro_t / ts
+    call check(c_suli_finalize(), 'finalize')
+  end subroutine run_mms
 end program principal_gpu

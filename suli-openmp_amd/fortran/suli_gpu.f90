@@ -142,6 +142,31 @@ module suli_gpu
       import :: c_int, c_char
       character(kind=c_char), dimension(*) :: path
     end function
+    integer(c_int) function c_suli_mms_set(mms_t, a, h0, coef) bind(C, name='suli_mms_set')
+      import :: c_int, c_double
+      integer(c_int), value :: mms_t
+      real(c_double), value :: a, h0, coef
+    end function
+    integer(c_int) function c_suli_mms_i() bind(C, name='suli_mms_i')
+      import :: c_int
+    end function
+    integer(c_int) function c_suli_mms_bc() bind(C, name='suli_mms_bc')
+      import :: c_int
+    end function
+    integer(c_int) function c_suli_termo_fonte1() bind(C, name='suli_termo_fonte1')
+      import :: c_int
+    end function
+    integer(c_int) function c_suli_termo_fonte2() bind(C, name='suli_termo_fonte2')
+      import :: c_int
+    end function
+    integer(c_int) function c_suli_mms(err) bind(C, name='suli_mms')
+      import :: c_int, c_double
+      real(c_double), dimension(4) :: err
+    end function
+    integer(c_int) function c_suli_level_set_ini(v) bind(C, name='suli_level_set_ini')
+      import :: c_int, c_double
+      real(c_double) :: v
+    end function
     integer(c_int) function c_suli_write_xdmf(path, nx, ny, nz, first, last, t_plot, dt_frame) &
         bind(C, name='suli_write_xdmf')
       import :: c_int, c_char, c_double
@@ -167,6 +192,12 @@ module suli_gpu
   ! advection (0_modules.f90:26-27): der 1 upwind, 2 central, 3 upwind 2nd
   ! order; adv_type 1 classic, 2 rotational, 3 antisymmetric
   integer :: der = 1, adv_type = 1
+  ! manufactured solution (0_modules.f90:37, module mms_m :204-219): mode,
+  ! constants, accumulated mean error erro_t and the last errors of mms()
+  integer :: mms_t = 0
+  real(8) :: mms_a = 0.0d0, mms_h0 = 2.0d0, mms_coef = 0.0d0
+  real(8) :: erro_t = 0.0d0
+  real(8), dimension(4) :: mms_err = 0.0d0
 
 contains
 
@@ -286,6 +317,41 @@ contains
     t = 0.
     cont = 10
   end subroutine restart_ini
+
+  ! ---- the manufactured solution (13_mms.f90) --------------------------------
+  ! mms_i (:12-212) with level_set_ini's tail (14_level_set.f90:205-217)
+  subroutine mms_i()
+    real(8) :: vol
+    call check(c_suli_mms_set(int(mms_t, c_int), mms_a, mms_h0, mms_coef), 'mms_set')
+    call check(c_suli_set_time(t), 'set_time')
+    call check(c_suli_mms_i(), 'mms_i')
+    call check(c_suli_level_set_ini(vol), 'level_set_ini')
+    erro_t = 0.0d0
+  end subroutine mms_i
+
+  ! mms_bc (:468-816); contorno(1|2) also sets the planes first
+  subroutine mms_bc()
+    call check(c_suli_set_time(t), 'set_time')
+    call check(c_suli_mms_bc(), 'mms_bc')
+  end subroutine mms_bc
+
+  subroutine termo_fonte1()
+    call check(c_suli_set_time(t), 'set_time')
+    call check(c_suli_termo_fonte1(), 'termo_fonte1')
+  end subroutine termo_fonte1
+
+  subroutine termo_fonte2()
+    call check(c_suli_set_time(t), 'set_time')
+    call check(c_suli_termo_fonte2(), 'termo_fonte2')
+  end subroutine termo_fonte2
+
+  ! mms (:218-465): the RMS errors of u, v, w, p and the running mean of the
+  ! velocity errors (erro_t, :459)
+  subroutine mms()
+    call check(c_suli_set_time(t), 'set_time')
+    call check(c_suli_mms(mms_err), 'mms')
+    erro_t = (mms_err(1) + mms_err(2) + mms_err(3)) / 3.0d0 + erro_t
+  end subroutine mms
 
   ! host copy of a field in the reference extents, for output
   subroutine download(field, a)

@@ -84,6 +84,36 @@ def test_fortran_driver_obstacle(tmp_path):
         assert abs(float(vol_err)) < 1e-2 and float(maxdiv) < 10.0 and 0 < int(iters) < 50
 
 
+@pytest.mark.parametrize("mms_t", [1, 2])
+def test_fortran_driver_manufactured_solution(tmp_path, mms_t):
+    """The driver's manufactured-solution run (13_mms.f90 through the shim's
+    mms_i, termo_fonte1/2, mms and contorno's planes; FP64, no projection)
+    against the oracle's run of the same case: the errors of mms() per step."""
+    import math
+
+    import oracle as O
+    exe = os.path.join(PKG, "principal_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("principal_gpu not built (flang absent)")
+    nx, ny, nz, ts = 16, 12, 10, 5
+    print(f"fortran driver manufactured solution mms_t {mms_t}, {nx}x{ny}x{nz}, {ts} steps", flush=True)
+    out = subprocess.run([exe, str(nx), str(ny), str(nz), str(ts), "1", "0.25", "0.25", "0", "1", "0", str(mms_t)],
+                         capture_output=True, text=True, timeout=240, cwd=str(tmp_path))
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    rows = re.findall(r"mms\s+(\d+) t=\s*(\S+) err=\s*(\S+)\s+(\S+)\s+(\S+)\s+(\S+)", out.stdout)
+    assert len(rows) == ts
+    got = np.array([[float(x) for x in r[2:]] for r in rows])
+    p = O.Params(nx=nx, ny=ny, nz=nz, dx=2 * math.pi / nx, dy=2 * math.pi / ny, dz=2.5 / nz, ccx0=3, ccxf=3,
+                 ccy0=3, ccyf=3, ccz0=3, cczf=3, m_turb=0, gx=0.0, gz=0.0, mms_t=mms_t, mms_a=0.0, mms_h0=2.0,
+                 mms_coef=1.0 if mms_t == 2 else 0.0)
+    _, errs = O.weng_run(p, 0.05 * p.dx, ts)
+    ref = np.array(errs[1:])
+    print("driver errors", got[-1], "oracle", ref[-1])
+    assert np.allclose(got[:, :3], ref[:, :3], rtol=1e-9, atol=1e-14)
+    if mms_t == 2:
+        assert np.allclose(got[:, 3], ref[:, 3], rtol=1e-9, atol=1e-14)
+
+
 def test_sloshing_forcing_drives_the_tank():
     """A flat free surface at rest stays at rest without forcing and is set
     in motion by gx(t); the volume is conserved either way."""

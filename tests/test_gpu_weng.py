@@ -160,3 +160,59 @@ def test_gpu_unsteady_converges_first_order():
     r1, r2 = out[0] / out[1], out[1] / out[2]
     print("ratios", r1, r2)
     assert 1.6 < r1 < 2.6 and 1.6 < r2 < 2.6
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_gpu_checkpoint_resumes_manufactured_run(tmp_path, prec):
+    """A checkpoint taken in the middle of an unsteady manufactured run
+    (a = 0.5) carries the mode, the boundary planes, the sources and the
+    pressure errors: the run resumed in a fresh context continues bit for bit."""
+    p, z = _case("weng2a")
+    pr = S.PREC_FP64 if prec == "fp64" else S.PREC_MIXED
+    dt = float(z["dt"][0])
+    path = str(tmp_path / "weng.ckpt")
+    sv = _solver(p, pr)
+    try:
+        _run(sv, dt, 3)
+        sv.checkpoint_save(path)
+        errs = []
+        for _ in range(2):
+            sv.step(dt)
+            errs.append(sv.mms())
+        want = {n: sv.download(n) for n in ("u", "v", "w", "ls", "prd1", "tf_u", "bzy0", "bxx1")}
+    finally:
+        sv.close()
+    sv = S.Solver.from_oracle_params(p, precision=pr)  # no suli_mms_set: the mode comes from the file
+    try:
+        sv.checkpoint_load(path)
+        got_errs = []
+        for _ in range(2):
+            sv.step(dt)
+            got_errs.append(sv.mms())
+        for n, a in want.items():
+            assert np.array_equal(sv.download(n), a), n
+        assert got_errs == errs
+    finally:
+        sv.close()
+
+
+def test_gpu_restart_file_carries_planes_and_sources(tmp_path):
+    """The reference's restart file written in a manufactured run holds the
+    boundary planes and the sources, and reading it restores them exactly."""
+    p, z = _case("weng2a")
+    path = str(tmp_path / "arquivorestart")
+    sv = _solver(p, S.PREC_FP64)
+    try:
+        _run(sv, float(z["dt"][0]), 2)
+        sv.restart_write(path)
+        want = {n: sv.download(n) for n in list(O.PLANE_NAMES) + ["tf_u", "tf_v", "tf_w", "u", "ls"]}
+    finally:
+        sv.close()
+    sv = S.Solver.from_oracle_params(p, precision=S.PREC_FP64)
+    try:
+        sv.restart_read(path)
+        for n, a in want.items():
+            assert np.array_equal(sv.download(n), a), n
+        assert np.abs(want["tf_u"]).max() > 0 and np.abs(want["bzy0"]).max() > 0
+    finally:
+        sv.close()
