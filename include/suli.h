@@ -293,6 +293,24 @@ int suli_termo_fonte1(void);
 int suli_termo_fonte2(void);
 int suli_mms(double *err);
 
+/* Wave inflow through the prescribed wall x = 0 (ccx0 = 3; wave_t of
+ * 0_modules.f90:35).  suli_waves_coef: waves_coef (3_contorno.f90:1070-1205):
+ * wave_t 0 (off), 1 Stokes I, 2 Stokes II, 5 Stokes V (Fenton's fifth-order
+ * theory) for amplitude a_w, period p_w and still depth h0_f (the
+ * reference's waves_coef fixes 0.03 m, 2.5 s, 1 m) and the params' gravity
+ * gz; coef (may be null, 14 doubles) receives the wavelength, wave number,
+ * angular frequency, celerity and the Stokes V coefficients aeta1..5,
+ * avel1..5 (zero otherwise).  Stokes V scans 10000 wavelengths as the
+ * reference does (host threads, about a second).  suli_boundary_waves:
+ * boundary_waves (:952-1068) at the library's time: the planes bxx0, bxx1,
+ * bxz0 of the wave's u and w at x = -dx/2 .. 0 (zero above its surface;
+ * bxy0 zero), the level set of the first cell column from the wave's
+ * surface and, as the reference's single-precision copy does, the rest of
+ * the level set rounded to single precision.  suli_stage calls it after
+ * tempo when a wave is set (1_principal.f90:84). */
+int suli_waves_coef(int wave_t, double a_w, double p_w, double h0_f, double *coef);
+int suli_boundary_waves(void);
+
 /* Switch the pressure solver of later suli_graddin calls (the fields stay
  * as they are).  tol_rel <= 0 or max_it <= 0 keeps the current value.  No
  * counterpart in the reference: its solver is fixed at compile time. */

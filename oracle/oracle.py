@@ -86,6 +86,10 @@ class Params:
     mms_a: float = 0.0     # its surface amplitude a, mean depth h0 and pressure scale coef
     mms_h0: float = 2.0    # (module mms_m, 0_modules.f90:208-212)
     mms_coef: float = 0.0
+    wave_t: int = 0    # wave inflow at x = 0: 0 none, 1 Stokes I, 2 Stokes II, 5 Stokes V (0_modules.f90:35)
+    wave_a: float = 0.03   # its amplitude, period and still depth (waves_coef, 3_contorno.f90:1086-1088)
+    wave_p: float = 2.5
+    wave_h0: float = 1.0
 
     @property
     def dx1(self) -> float:
@@ -114,6 +118,8 @@ class _OrState(C.Structure):
     ] + [(name, C.POINTER(C.c_double)) for name, _ in ARRAYS] + [
         ("tf_u", C.POINTER(C.c_double)), ("tf_v", C.POINTER(C.c_double)), ("tf_w", C.POINTER(C.c_double)),
         ("mms_t", C.c_int), ("bp", C.POINTER(C.c_double) * 24),
+        ("wave_t", C.c_int), ("wave_a", C.c_double), ("wave_p", C.c_double), ("wave_h0", C.c_double),
+        ("wc", C.c_double * 14),
     ]
 
 
@@ -140,6 +146,9 @@ def lib():
         L.or_stage.restype = None
         L.or_stage_noproj.argtypes = [P, C.c_int, C.c_double]
         L.or_stage_noproj.restype = None
+        for fn in ["or_waves_coef", "or_boundary_waves"]:
+            getattr(L, fn).argtypes = [P]
+            getattr(L, fn).restype = None
         L.or_step.argtypes = [P, C.c_double]
         L.or_step.restype = None
         L.or_state_len.argtypes = [C.c_int, C.c_int, C.c_int]
@@ -176,6 +185,7 @@ class State:
                          sigma=p.sigma, alpha1=p.alpha1)
         self.tf = None  # optional (tf_u, tf_v, tf_w) source arrays
         self.planes = None  # optional boundary planes of wall code 3 (name -> array, PLANE_NAMES)
+        self.wc = np.zeros(14)  # wave coefficients of waves_coef (l_w, n_w, f_w, c_w, aeta1..5, avel1..5)
         self.diag = {}
         self.cg_max_it = 0  # graddin iteration cap (0: the reference's 10000)
 
@@ -195,6 +205,7 @@ class State:
         s.scal = dict(self.scal)
         s.tf = None if self.tf is None else tuple(a.copy() for a in self.tf)
         s.planes = None if self.planes is None else {k: a.copy() for k, a in self.planes.items()}
+        s.wc = self.wc.copy()
         s.cg_max_it = self.cg_max_it
         return s
 
@@ -229,6 +240,11 @@ class State:
             self._tf_keep = [np.ascontiguousarray(a, dtype=np.float64).ravel(order="F") for a in self.tf]
             st.tf_u, st.tf_v, st.tf_w = [_dptr(a) for a in self._tf_keep]
         st.mms_t = p.mms_t
+        st.wave_t, st.wave_a, st.wave_p, st.wave_h0 = p.wave_t, p.wave_a, p.wave_p, p.wave_h0
+        for q in range(14):
+            st.wc[q] = self.wc[q]
+        if self.planes is None and p.wave_t:
+            self.planes = {n: np.zeros(plane_shape(p, n)) for n in PLANE_NAMES}
         if self.planes is not None:
             self._bp_keep = [np.ascontiguousarray(self.planes[n], dtype=np.float64).ravel(order="F")
                              for n in PLANE_NAMES]
@@ -242,6 +258,10 @@ class State:
     def _pull(self, st: _OrState) -> None:
         for k in ["dt", "t", "vol_ini", "vol_ins"]:
             self.scal[k] = float(getattr(st, k))
+        self.wc = np.array([st.wc[q] for q in range(14)])
+        if self.planes is not None:  # the C routines may write the planes (boundary_waves)
+            for n, flat in zip(PLANE_NAMES, self._bp_keep):
+                self.planes[n] = flat.reshape(plane_shape(self.p, n), order="F")
         self.diag = dict(cg_iters=st.cg_iters, cg_res2=st.cg_res2, reinit_iters=st.reinit_iters,
                          mass_corr_iters=st.mass_corr_iters)
 
@@ -275,6 +295,10 @@ class State:
             L.or_graddin(C.byref(st))
         elif cmd == "posdin":
             L.or_posdin(C.byref(st))
+        elif cmd == "waves_coef":
+            L.or_waves_coef(C.byref(st))
+        elif cmd == "boundary_waves":
+            L.or_boundary_waves(C.byref(st))
         elif cmd == "stage_noproj":  # the stage of the manufactured-solution mode
             L.or_stage_noproj(C.byref(st), tt, self.scal["dt"])
             st.dt = self.scal["dt"]
@@ -315,7 +339,8 @@ def ref_tag(p: Params) -> str:
             f"_t{p.m_turb}_cs{p.csmag:g}_g{p.gx:g}_{p.gz:g}" + (f"_tt{p.t_tempo}" if p.t_tempo != 1 else "")
             + (f"_adv{p.adv_type}{p.der}" if (p.der, p.adv_type) != (1, 1) else "")
             + (f"_ob{p.obst_t}" if p.obst_t else "")
-            + (f"_mms{p.mms_t}_a{p.mms_a:g}_h{p.mms_h0:g}_c{p.mms_coef:g}" if p.mms_t else ""))
+            + (f"_mms{p.mms_t}_a{p.mms_a:g}_h{p.mms_h0:g}_c{p.mms_coef:g}" if p.mms_t else "")
+            + (f"_wave{p.wave_t}" if p.wave_t else ""))
 
 
 def ref_build(p: Params, quiet: bool = True, variant: str = "") -> str:
@@ -335,6 +360,8 @@ def ref_build(p: Params, quiet: bool = True, variant: str = "") -> str:
     env = dict(os.environ, REF_VARIANT=variant)
     if p.mms_t:
         env["REF_MMS"] = f"{p.mms_t} {float(p.mms_a)!r} {float(p.mms_h0)!r} {float(p.mms_coef)!r}"
+    if p.wave_t:
+        env["REF_WAVE"] = str(p.wave_t)
     subprocess.run(args, check=True, stdout=subprocess.DEVNULL if quiet else None, env=env)
     return exe
 

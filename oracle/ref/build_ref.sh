@@ -37,6 +37,7 @@
 # REF_MMS="MMS_T A H0 COEF" (environment) builds the manufactured-solution
 # configuration: 0_modules.f90:37 mms_t and the module mms_m constants
 # a, h0, coef (0_modules.f90:208-212).
+# REF_WAVE=WAVE_T (environment) builds the wave inflow: 0_modules.f90:35.
 #
 # usage: build_ref.sh TAG NX NY NZ DX DY DZ CCX0 CCXF CCY0 CCYF CCZ0 CCZF MTURB CSMAG GX GZ [TTEMPO [DER ADV_TYPE [OBST_T]]]
 set -euo pipefail
@@ -89,6 +90,9 @@ if [ -n "${REF_MMS:-}" ]; then
   edit "$M" "s/^real(8), parameter :: a  = 0\.!5\$/real(8), parameter :: a  = $MMSA/" "a  = $MMSA\$"
   edit "$M" "s/^real(8), parameter :: h0 = 2\.\$/real(8), parameter :: h0 = $MMSH0/" "h0 = $MMSH0\$"
   edit "$M" "s/^real(8), parameter :: coef = 0\. ! 1\.\$/real(8), parameter :: coef = $MMSC/" "coef = $MMSC\$"
+fi
+if [ -n "${REF_WAVE:-}" ]; then
+  edit "$M" "s/^integer,parameter :: wave_t = 0 /integer,parameter :: wave_t = $REF_WAVE /" "wave_t = $REF_WAVE "
 fi
 edit "$M" "s/^module tempo\$/module tempo_mod/" "^module tempo_mod"
 edit "$M" "s/^end module tempo\$/end module tempo_mod/" "^end module tempo_mod"

@@ -10,11 +10,12 @@
 ! CMD: ls_ini | level_set | heaviside | mod_ls1 | reinic | der_weno | weno1 ARG(ihs)
 !      visco | convdiff | tempo ARG(tt) | contorno ARG(nlock) | graddin | posdin
 !      stage ARG(tt) | step ARG(nsteps) | bench | plot | visu | restart_salva | restart_ini
-!      obstaculo | est | mms_src | mms_run ARG(nsteps)
+!      obstaculo | est | mms_src | mms_run ARG(nsteps) | waves | wave_run ARG(nsteps)
 ! (plot and visu write under arquivos/, the restart commands use dados/, of
 ! the working directory)
 !
-! The mms commands (builds with mms_t > 0, 13_mms.f90) also write OUT.mms:
+! The mms and wave commands (builds with mms_t > 0, 13_mms.f90, or
+! wave_t > 0, 3_contorno.f90:952-1205) also write OUT.mms:
 ! tf_u, tf_v, tf_w, tf_px, tf_py, tf_pz, the 24 boundary planes in the order
 ! of their declaration (0_modules.f90:96-120), erro_u, erro_v, erro_w,
 ! erro_p, each in Fortran order with its module extents.
@@ -31,6 +32,7 @@ program harness
   use obst
   use tempo_mod
   use mms_m
+  use wave_c
   implicit none
   character(len=512) :: fin, fout, cmd, arg
   integer :: iarg, nsteps, k, i, j
@@ -261,6 +263,40 @@ program harness
       call mms()
       call mms_print()
     enddo
+    call io_mms(trim(fout)//'.mms')
+  case ('waves')
+    ! waves_coef (3_contorno.f90:1070-1205; 2_iniciais.f90:52-54) and one
+    ! boundary_waves (:952-1068) at the time t of the state file: the level
+    ! set (rounded through real(4), its first column from the wave) and the
+    ! inflow planes bxx0, bxx1, bxz0 (bxy0 takes v1, which the reference
+    ! never sets); the wave coefficients are printed
+    call waves_coef()
+    call boundary_waves()
+    write(*,'(A,14ES24.16)') 'WAVE_COEF ', l_w, n_w, f_w, c_w, aeta1, aeta2, aeta3, aeta4, aeta5, &
+      avel1, avel2, avel3, avel4, avel5
+    call io_mms(trim(fout)//'.mms')
+  case ('wave_run')
+    ! the time loop of 1_principal.f90:64-95 with wave_t > 0: boundary_waves
+    ! after tempo in every stage (1_principal.f90:84)
+    nsteps = max(iarg, 1)
+    call waves_coef()
+    do it = 1, nsteps
+      t = it * dt
+      call level_set()
+      call contorno(3)
+      do tt = 1, ntt
+        dt = a_dt(tt)
+        call visco()
+        call convdiff()
+        call tempo()
+        call boundary_waves()
+        call contorno(2)
+        call graddin()
+        call posdin()
+        call contorno(1)
+      enddo
+    enddo
+    dt = dt_base
     call io_mms(trim(fout)//'.mms')
   case ('restart_ini')
     ! reads dados/arquivorestart (11_diversos.f90:232-301)

@@ -1394,10 +1394,168 @@ void or_stage(or_state *s, int tt, double dt_base) {
   or_visco(s);
   or_convdiff(s);
   or_tempo(s);
+  if (s->wave_t > 0) or_boundary_waves(s); /* 1_principal.f90:84 */
   or_contorno(s, 2);
   or_graddin(s);
   or_posdin(s);
   or_contorno(s, 1);
+}
+
+/* waves_coef, 3_contorno.f90:1070-1205: wavelength, wave number and angular
+ * frequency of the linear dispersion relation by 1000 fixed-point passes
+ * (Stokes I, II); for Stokes V (Fenton 1985) the wavelength is the one of
+ * 10000 candidates l0/2 + l0 (ii/10000) 2.5 whose two estimates of the
+ * steepness parameter (10000 fixed-point passes each) agree best, then the
+ * fifth-order coefficients.  Real exponents are pow() as the reference's
+ * x**n.; integer ones are products. */
+static double p_(double x, double e) { return pow(x, e); }
+static void stokes5_b(double c, double s, double *b33, double *b35, double *b55, double *c1, double *c2) {
+  const double c2_ = c * c;
+  const double c4 = c2_ * c2_, c8 = c4 * c4, c12 = c8 * c4;  /* c**12: integer power */
+  *b33 = 3. * (8. * p_(c, 6.) + 1.) / (64. * p_(s, 6.));
+  *b35 = (88128. * p_(c, 14.) - 208224. * c12 + 70848. * p_(c, 10.) + 54000. * p_(c, 8.) - 21816. * p_(c, 6.) +
+          6264. * p_(c, 4.) - 54. * c * c - 81.) /
+         (12288. * p_(s, 12.) * (6. * c * c - 1.));
+  *b55 = (192000. * p_(c, 16.) - 262720. * p_(c, 14.) + 83680. * c12 + 20160. * p_(c, 10.) - 7280. * p_(c, 8.) +
+          7160. * p_(c, 6.) - 1800. * p_(c, 4.) - 1050. * c * c + 225.) /
+         (12288 * p_(s, 10.) * (6. * c * c - 1.) * (8. * p_(c, 4.) - 11. * c * c + 3.));
+  *c1 = (8. * p_(c, 4.) - 8. * c * c + 9.) / (8. * p_(s, 4.));
+  *c2 = (3840. * p_(c, 12.) - 4096. * p_(c, 10.) + 2592. * p_(c, 8.) - 1008. * p_(c, 6.) + 5944. * p_(c, 4.) -
+         1830. * c * c + 147.) /
+        (512. * p_(s, 10.) * (6. * c * c - 1.));
+}
+void or_waves_coef(or_state *s) {
+  const double pi = M_PI, gz = s->gz, a_w = s->wave_a, p_w = s->wave_p, h0_f = s->wave_h0;
+  const double l0_w = gz * p_w * p_w / (2. * pi);
+  double l_w = 0., lamb = 0.;
+  double *wc = s->wc;
+  for (int q = 0; q < 14; q++) wc[q] = 0.;
+  if (s->wave_t <= 2) {
+    l_w = gz * p_w * p_w / (2. * pi);
+    for (int i = 1; i <= 1000; i++) l_w = gz * p_w * p_w / (2. * pi) * tanh(2. * pi * h0_f / l_w);
+  } else {
+    double erro0 = 999.;
+    const int nxii = 10000;
+    for (int ii = 1; ii <= nxii; ii++) {
+      double lamb1 = 0., lamb2 = 0.;
+      const double l_w1 = l0_w * 0.5 + l0_w * ((double)ii / nxii) * 2.5;
+      const double sh = sinh(2. * pi * h0_f / l_w1), ch = cosh(2. * pi * h0_f / l_w1);
+      double b33, b35, b55, c1, c2;
+      stokes5_b(ch, sh, &b33, &b35, &b55, &c1, &c2);
+      for (int i = 1; i <= 10000; i++) lamb1 = pi * a_w * 2. / l_w1 - p_(lamb1, 3.) * b33 - p_(lamb1, 5.) * (b35 + b55);
+      for (int i = 1; i <= 10000; i++)
+        lamb2 = sqrt((l_w1 / (l0_w * tanh(2. * pi * h0_f / l_w1)) - 1.) / (c1 + p_(lamb2, 2.) * c2));
+      const double erro = fabs(lamb1 - lamb2);
+      if (erro < erro0) {
+        erro0 = erro;
+        lamb = lamb1;
+        l_w = l_w1;
+      }
+    }
+  }
+  const double c_w = l_w / p_w, f_w = 2. * pi / p_w, n_w = 2. * pi / l_w;
+  wc[0] = l_w; wc[1] = n_w; wc[2] = f_w; wc[3] = c_w;
+  if (s->wave_t == 5) {
+    const double sh = sinh(2. * pi * h0_f / l_w), c = cosh(2. * pi * h0_f / l_w);
+    double b33, b35, b55, c1, c2;
+    stokes5_b(c, sh, &b33, &b35, &b55, &c1, &c2);
+    const double a11 = 1. / sh;
+    const double a13 = -c * c * (5. * c * c + 1.) / (8. * p_(sh, 5.));
+    const double a15 = -(1184. * p_(c, 10.) - 1440. * p_(c, 8.) - 1992. * p_(c, 6.) + 2641. * p_(c, 4.) -
+                         249. * p_(c, 2.) + 18.) / (1536. * p_(sh, 11.));
+    const double a22 = 3. / (8. * p_(sh, 4.));
+    const double a24 = (192. * p_(c, 8.) - 424. * p_(c, 6.) - 312. * p_(c, 4.) + 480. * c * c - 17.) / (768. * p_(sh, 10.));
+    const double a33 = (13. - 4. * c * c) / (64. * p_(sh, 7.));
+    const double a35 = (512. * p_(c, 12.) + 4224. * p_(c, 10.) - 6800. * p_(c, 8.) - 12808. * p_(c, 6.) +
+                        16704. * p_(c, 4.) - 3154. * c * c + 107.) / (4096. * p_(sh, 13.) * (6. * c * c - 1.));
+    const double a44 = (80. * p_(c, 6.) - 816. * p_(c, 4.) + 1338. * c * c - 197.) /
+                       (1536. * p_(sh, 10.) * (6. * c * c - 1.));
+    const double a55 = -(2880. * p_(c, 10.) - 72480. * p_(c, 8.) + 324000. * p_(c, 6.) - 432000. * p_(c, 4.) +
+                         163470. * c * c - 16245.) /
+                       (61440. * p_(sh, 11.) * (6. * c * c - 1.) * (8. * p_(c, 4.) - 11. * c * c + 3.));
+    const double b22 = c * (2. * c * c + 1.) / (4. * p_(sh, 3.));
+    const double b24 = c * (272. * p_(c, 8.) - 504. * p_(c, 6.) - 192. * p_(c, 4.) + 322 * c * c + 21.) /
+                       (384. * p_(sh, 9.));
+    const double b44 = c * (768. * p_(c, 10.) - 448. * p_(c, 8.) - 48. * p_(c, 6.) + 48. * p_(c, 4.) + 106. * c * c -
+                            21.) / (384. * p_(sh, 9.) * (6. * c * c - 1.));
+    wc[4] = lamb / n_w;
+    wc[5] = (lamb * lamb * b22 + p_(lamb, 4.) * b24) / n_w;
+    wc[6] = (p_(lamb, 3.) * b33 + p_(lamb, 5.) * b35) / n_w;
+    wc[7] = p_(lamb, 4.) * b44 / n_w;
+    wc[8] = p_(lamb, 5.) * b55 / n_w;
+    wc[9] = 1. * 2. * pi / (p_w * n_w) * (lamb * a11 + p_(lamb, 3.) * a13 + p_(lamb, 5.) * a15);
+    wc[10] = 2. * 2. * pi / (p_w * n_w) * (p_(lamb, 2.) * a22 + p_(lamb, 4.) * a24);
+    wc[11] = 3. * 2. * pi / (p_w * n_w) * (p_(lamb, 3.) * a33 + p_(lamb, 5.) * a35);
+    wc[12] = 4. * 2. * pi / (p_w * n_w) * (p_(lamb, 4.) * a44);
+    wc[13] = 5. * 2. * pi / (p_w * n_w) * (p_(lamb, 5.) * a55);
+  }
+}
+
+/* boundary_waves, 3_contorno.f90:952-1068: the free surface h_f and the
+ * velocities u, w of the wave at i = 0, 1 (x = (i - 1/2) dx for the surface
+ * and w, (i - 1) dx for u) at the time t, zero above the surface; the level
+ * set of the first cell column becomes h_f(1) - z, and the whole level set
+ * passes through real(4) (the reference's ls1 is single precision).  The
+ * planes bxx0, bxx1, bxz0 take u(0), u(1), w(0); bxy0 takes v1, which the
+ * reference never sets (zero here, as the flang-built reference leaves it). */
+void or_boundary_waves(or_state *s) {
+  DIMS;
+  const double *wc = s->wc;
+  const double l_w = wc[0], n_w = wc[1], f_w = wc[2];
+  const double a_w = s->wave_a, h0_f = s->wave_h0, t = s->t, dx = s->dx, dz = s->dz;
+  const int nx1 = nx + 1, nz1 = nz + 1;
+  (void)l_w;
+  double *ls = s->ls;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) ls[IC(i, j, k)] = (double)(float)ls[IC(i, j, k)];
+  const double aux1 = tanh(n_w * h0_f);
+  for (int k = 0; k <= nz + 1; k++) {
+    const double kp = (k - 0.5) * dz;
+    for (int j = 0; j <= ny + 1; j++)
+      for (int i = 0; i <= 1; i++) {
+        const double aux2 = n_w * dx * (i - 0.5) - f_w * t, aux3 = n_w * dx * (i - 1.) - f_w * t;
+        double h_f, u1 = 0., w1 = 0.;
+        if (s->wave_t == 1) h_f = h0_f + a_w * cos(n_w * dx * (i - 0.5) - f_w * t);
+        else if (s->wave_t == 2)
+          h_f = h0_f + a_w * cos(aux2) + n_w * a_w * a_w * (3. - aux1 * aux1) / (4. * aux1 * aux1 * aux1) * cos(2. * aux2);
+        else
+          h_f = h0_f + wc[4] * cos(aux2) + wc[5] * cos(2. * aux2) + wc[6] * cos(3. * aux2) + wc[7] * cos(4. * aux2) +
+                wc[8] * cos(5. * aux2);
+        const float ls1 = (float)(h_f - kp);
+        if (ls1 >= 0.f) {
+          if (s->wave_t == 1) {
+            u1 = a_w * f_w * cosh(n_w * kp) / sinh(n_w * h0_f) * cos(n_w * dx * (i - 1.) - f_w * t);
+            w1 = a_w * f_w * sinh(n_w * (kp - 0.5 * dz)) / sinh(n_w * h0_f) * sin(n_w * dx * (i - 0.5) - f_w * t);
+          } else if (s->wave_t == 2) {
+            u1 = a_w * f_w * cosh(n_w * kp) / sinh(n_w * h0_f) * cos(aux3) +
+                 3. / 4. * a_w * a_w * f_w * n_w * cosh(2. * n_w * kp) / pow(sinh(n_w * h0_f), 4.) * cos(2 * aux3);
+            w1 = a_w * f_w * sinh(n_w * (kp - 0.5 * dz)) / sinh(n_w * h0_f) * sin(aux2) +
+                 3. / 4. * a_w * a_w * f_w * n_w * sinh(2. * n_w * (kp - 0.5 * dz)) / pow(sinh(n_w * h0_f), 4.) *
+                     sin(2. * aux3);
+          } else {
+            const double aux4 = n_w * kp, aux5 = n_w * (kp - 0.5 * dz);
+            u1 = wc[9] * cosh(aux4) * cos(aux3) + wc[10] * cosh(2 * aux4) * cos(2 * aux3) +
+                 wc[11] * cosh(3 * aux4) * cos(3 * aux3) + wc[12] * cosh(4 * aux4) * cos(4 * aux3) +
+                 wc[13] * cosh(5 * aux4) * cos(5 * aux3);
+            w1 = wc[9] * sinh(aux5) * sin(aux2) + wc[10] * sinh(2 * aux5) * sin(2 * aux2) +
+                 wc[11] * sinh(3 * aux5) * sin(3 * aux2) + wc[12] * sinh(4 * aux5) * sin(4 * aux2) +
+                 wc[13] * sinh(5 * aux5) * sin(5 * aux2);
+          }
+        }
+        if (i == 1 && j >= 1 && j <= ny && k >= 1 && k <= nz) ls[IC(1, j, k)] = (double)ls1;
+        if (i == 0) {
+          BP(BXX0, j, k, ny + 2) = u1;
+          BP(BXZ0, j, k, ny + 2) = w1;
+        } else {
+          BP(BXX1, j, k, ny + 2) = u1;
+        }
+      }
+  }
+  for (int j = 0; j <= ny + 1; j++) BP(BXZ0, j, nz1 + 1, ny + 2) = 0.;
+  for (int k = 0; k <= nz + 1; k++)
+    for (int j = 0; j <= ny + 2; j++) BP(BXY0, j, k, ny + 3) = 0.;
+  (void)nx1;
 }
 
 /* one stage of the manufactured-solution mode (mms_t > 0): no projection,

@@ -59,6 +59,13 @@ typedef struct {
    * bxxf1, bxyf, bxzf, byx0, ..., bzzf1), each in Fortran order from index 0
    * with its module extents; NULL when no wall has code 3 */
   double *bp[24];
+  /* wave inflow at x = 0 (0_modules.f90:35, module wave_c :164-174): 0 none,
+   * 1 Stokes I, 2 Stokes II, 5 Stokes V; amplitude, period, still depth
+   * (set by waves_coef, 3_contorno.f90:1086-1088) and the coefficients
+   * or_waves_coef computes: l_w, n_w, f_w, c_w, aeta1..5, avel1..5 */
+  int wave_t;
+  double wave_a, wave_p, wave_h0;
+  double wc[14];
 } or_state;
 
 /* number of float64 values in the flat state buffer for an nx*ny*nz grid */
@@ -86,6 +93,9 @@ double or_est_div(const or_state *s);
 /* one RK stage (visco..contorno(1)) and one full step, 1_principal.f90:74-95 */
 void or_stage(or_state *s, int tt, double dt_base);
 void or_stage_noproj(or_state *s, int tt, double dt_base);
+/* waves_coef and boundary_waves (3_contorno.f90:1070-1205, :952-1068) */
+void or_waves_coef(or_state *s);
+void or_boundary_waves(or_state *s);
 void or_step(or_state *s, double dt_base);
 
 #ifdef __cplusplus

@@ -941,6 +941,7 @@ extern "C" int suli_stage(int tt, double dt_base, suli_poisson_stats *st) {
   if ((rc = suli_visco())) return rc;
   if ((rc = suli_convdiff())) return rc;
   if ((rc = suli_tempo(tt, dt))) return rc;
+  if ((rc = waves_now(*g_ctx))) return rc;  // boundary_waves when wave_t > 0 (1_principal.f90:84)
   if ((rc = suli_contorno(2))) return rc;
   if (g_ctx->mms.t > 0) {  // the manufactured-solution run has no projection (1_principal.f90:89-93)
     if (st) *st = suli_poisson_stats{};
@@ -1000,7 +1001,7 @@ static const int kCkptFields[] = {SULI_U,      SULI_V,      SULI_W,      SULI_PR
                                   SULI_HSZ,    SULI_DDLSDX, SULI_DDLSDY, SULI_DDLSDZ, SULI_NUT,  SULI_U0,
                                   SULI_V0,     SULI_W0,     SULI_TFU,    SULI_TFV,    SULI_TFW,  SULI_FU0,
                                   SULI_FV0,    SULI_FW0};
-static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '4'};
+static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '5'};
 
 struct CkptHeader {
   char magic[8];
@@ -1016,6 +1017,9 @@ struct CkptHeader {
   // the obstacle columns, then, in the mode, its pressure errors (nx*ny*nz FP64)
   int32_t mms_t, mms_src_done;
   double mms_a, mms_h0, mms_coef;
+  // wave inflow: wave_t, amplitude, period, depth and its coefficients
+  int32_t wave_t, pad2_;
+  double wave_a, wave_p, wave_h0, wave_wc[14];
 };
 
 // Binary checkpoint: the device frames of the state fields as they are (the
@@ -1043,6 +1047,8 @@ extern "C" int suli_checkpoint_save(const char *path) {
   h.obst = c.obst ? 1 : 0;
   h.mms_t = c.mms.t; h.mms_src_done = c.mms.src_done ? 1 : 0;
   h.mms_a = c.mms.a; h.mms_h0 = c.mms.h0; h.mms_coef = c.mms.coef;
+  h.wave_t = c.wave.t; h.wave_a = c.wave.a; h.wave_p = c.wave.p; h.wave_h0 = c.wave.h0;
+  for (int q = 0; q < 14; q++) h.wave_wc[q] = c.wave.wc[q];
   HIPCHK(hipStreamSynchronize(c.st));
   FILE *f = std::fopen(path, "wb");
   if (!f) {
@@ -1168,6 +1174,8 @@ extern "C" int suli_checkpoint_load(const char *path) {
   if (int rc = suli_mms_set(h.mms_t, h.mms_a, h.mms_h0, h.mms_coef)) return rc;
   if (h.mms_t > 0) HIPCHK(hipMemcpy(c.mms.erro_p, errp.data(), errp.size(), hipMemcpyHostToDevice));
   c.mms.src_done = h.mms_src_done != 0;
+  c.wave.t = h.wave_t; c.wave.a = h.wave_a; c.wave.p = h.wave_p; c.wave.h0 = h.wave_h0;
+  for (int q = 0; q < 14; q++) c.wave.wc[q] = h.wave_wc[q];
   c.tf_on = h.tf_on != 0;
   c.vol_ini = h.vol_ini; c.vol_ins = h.vol_ins; c.t_now = h.t_now;
   c.force_amp = h.force_amp; c.force_period = h.force_period; c.force_phase = h.force_phase;

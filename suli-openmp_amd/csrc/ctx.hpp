@@ -84,6 +84,13 @@ struct Ctx {
     double *erro_p = nullptr;  // squared pressure errors (nx*ny*nz), kept between calls
     double *part = nullptr;    // block partials of the error sums
   } mms;
+  // wave inflow at x = 0 (waves.hip): t = wave_t (0 none, 1, 2, 5), amplitude,
+  // period, still depth and the coefficients l_w, n_w, f_w, c_w, aeta1..5, avel1..5
+  struct {
+    int t = 0;
+    double a = 0.03, p = 2.5, h0 = 1.0;
+    double wc[14] = {};
+  } wave;
   bool ab2_started = false;     // fu0 holds the forces of the last AB2 step
   bool tf_on = false;                                    // set by the first upload of a source
   double *prd1 = nullptr;
@@ -128,6 +135,8 @@ int suli_noalloc_setup(double **out);
 int mms_planes_now(Ctx &c);
 int mms_sources_now(Ctx &c);
 extern "C" int suli_mms_set(int mms_t, double a, double h0, double coef);
+// boundary_waves at t_now when a wave is set (waves.hip)
+int waves_now(Ctx &c);
 
 #define HIPCHK(x)                                                                    \
   do {                                                                               \

@@ -7,8 +7,14 @@
 // pass computes the true residual r = b_s - A_s x in FP64 (FP64 dot
 // products), removes its component along the null space sqrt(D) of the
 // singular Neumann operator, and solves A_s d = r approximately in FP32 with
-// FCG; x += d in FP64.  The solve stops on the FP64 relative residual
-// ||b_s - A_s x|| / ||b_s|| <= tol.
+// FCG; x += d in FP64.  The solve stops on the FP64 relative residual of the
+// solvable part, ||P(b_s - A_s x)|| / ||P b_s|| <= tol, P the projection that
+// removes the component along sqrt(D).  In a closed tank b_s has no such
+// component (the wall fluxes are zero) and this is the plain relative
+// residual; with a net flux through the walls (a prescribed inflow) the
+// system has no solution and the solve converges to the least-squares one.
+// The component r.n/|n| is the same for every iterate (A_s n = 0), so the
+// projected norms come from the r.r, r.n, n.n of the residual kernel.
 //
 // Preconditioners: Jacobi (identity in the scaled system: the reference's own
 // symmetric diagonal scaling), a geometric multigrid V-cycle on the unscaled
@@ -140,13 +146,20 @@ __global__ __launch_bounds__(256) void k_r32(Grid g, const double *__restrict__ 
 #ifndef SULI_FCG_ONEPASS_FLOOR
 #define SULI_FCG_ONEPASS_FLOOR 0.1  // and then aims for at most this fraction of it
 #endif
+// squared norms of the solvable parts of the outer residual and of b_s
+__host__ __device__ inline void proj_norms(const double *rout, double bb_full, double &rr, double &bb) {
+  const double nc = rout[2] > 0 ? rout[1] * rout[1] / rout[2] : 0.0;
+  rr = fmax(rout[0] - nc, 0.0);
+  bb = fmax(bb_full - nc, 0.0);
+}
 struct EpiR32 {
   FcgCtl *ctl;
   double rel_in, tol;
   int maxit;
   __device__ bool enabled() const { return true; }
   __device__ void operator()(const double *res) const {
-    const double rr = ctl->rout[0], bb = ctl->bb[0];
+    double rr, bb;
+    proj_norms(ctl->rout, ctl->bb[0], rr, bb);
     const bool conv = !(rr > tol * tol * bb);
     const double want = rr > 0 ? 0.3 * tol * sqrt(bb) / sqrt(rr) : 1.0;
     // one pass of at most 10x the usual reduction when that ends the solve
@@ -1627,6 +1640,15 @@ static void fcgm_iteration(Ctx &c, PoissonFCG &f, int slot, dim3 gk, const int *
   }
 }
 
+// relative residual of the solvable part from the read-back control block;
+// bnorm receives ||P b_s||
+static double proj_res(const FcgCtl *h, double &bnorm) {
+  double rr, bb;
+  proj_norms(h->rout, h->bb[0], rr, bb);
+  bnorm = std::sqrt(bb);
+  return bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
+}
+
 int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   PoissonFCG &f = *c.fcg;
   const Grid &g = c.g;
@@ -1723,9 +1745,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       }
       HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
       HIPCHK(hipStreamSynchronize(c.st));
-      bnorm = std::sqrt(f.ctl_h->bb[0]);
-      const double rn = std::sqrt(f.ctl_h->rout[0]);
-      res = bnorm > 0 ? rn / bnorm : 0.0;
+      res = proj_res(f.ctl_h, bnorm);
       if (res0 < 0) res0 = res;
       if (!(res > tol) || outer >= max_outer) {
         conv = !(res > tol);
@@ -1791,8 +1811,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       else if (chunk < 64) chunk *= 2;
     }
     if (spec) {  // this pass's outer residual, read with its first chunk
-      bnorm = std::sqrt(f.ctl_h->bb[0]);
-      res = bnorm > 0 ? std::sqrt(f.ctl_h->rout[0]) / bnorm : 0.0;
+      res = proj_res(f.ctl_h, bnorm);
       if (res0 < 0) res0 = res;
       if (f.ctl_h->done) {  // it met the tolerance: the pass did not run
         outer--;
@@ -1840,7 +1859,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       launch_finish<3>(c.st, gk, rr);
       HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
       HIPCHK(hipStreamSynchronize(c.st));
-      res = bnorm > 0 ? std::sqrt(f.ctl_h->rout[0]) / bnorm : 0.0;
+      res = proj_res(f.ctl_h, bnorm);
       conv = !(res > tol);
       break;
     }

@@ -163,6 +163,15 @@ module suli_gpu
       import :: c_int, c_double
       real(c_double), dimension(4) :: err
     end function
+    integer(c_int) function c_suli_waves_coef(wave_t, a_w, p_w, h0_f, coef) bind(C, name='suli_waves_coef')
+      import :: c_int, c_double
+      integer(c_int), value :: wave_t
+      real(c_double), value :: a_w, p_w, h0_f
+      real(c_double), dimension(14) :: coef
+    end function
+    integer(c_int) function c_suli_boundary_waves() bind(C, name='suli_boundary_waves')
+      import :: c_int
+    end function
     integer(c_int) function c_suli_level_set_ini(v) bind(C, name='suli_level_set_ini')
       import :: c_int, c_double
       real(c_double) :: v
@@ -198,6 +207,11 @@ module suli_gpu
   real(8) :: mms_a = 0.0d0, mms_h0 = 2.0d0, mms_coef = 0.0d0
   real(8) :: erro_t = 0.0d0
   real(8), dimension(4) :: mms_err = 0.0d0
+  ! wave inflow (0_modules.f90:35, module wave_c :164-174): type, the
+  ! constants of waves_coef (3_contorno.f90:1086-1088) and its coefficients
+  integer :: wave_t = 0
+  real(8) :: a_w = 0.03d0, p_w = 2.5d0, h0_f = 1.0d0
+  real(8), dimension(14) :: wave_coef = 0.0d0
 
 contains
 
@@ -352,6 +366,18 @@ contains
     call check(c_suli_mms(mms_err), 'mms')
     erro_t = (mms_err(1) + mms_err(2) + mms_err(3)) / 3.0d0 + erro_t
   end subroutine mms
+
+  ! ---- wave inflow (3_contorno.f90:952-1205) ----------------------------------
+  subroutine waves_coef()
+    call check(c_suli_waves_coef(int(wave_t, c_int), a_w, p_w, h0_f, wave_coef), 'waves_coef')
+  end subroutine waves_coef
+
+  ! boundary_waves at the step's time (the driver calls it after tempo when
+  ! wave_t > 0, 1_principal.f90:84; suli_stage does the same by itself)
+  subroutine boundary_waves()
+    call check(c_suli_set_time(t), 'set_time')
+    call check(c_suli_boundary_waves(), 'boundary_waves')
+  end subroutine boundary_waves
 
   ! host copy of a field in the reference extents, for output
   subroutine download(field, a)

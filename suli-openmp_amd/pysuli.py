@@ -144,6 +144,7 @@ def lib():
             "suli_write_xdmf": ([C.c_char_p, i32, i32, i32, i32, i32, i32, f64], i32),
             "suli_mms_set": ([i32, f64, f64, f64], i32), "suli_mms_i": ([], i32), "suli_mms_bc": ([], i32),
             "suli_termo_fonte1": ([], i32), "suli_termo_fonte2": ([], i32), "suli_mms": ([p(f64)], i32),
+            "suli_waves_coef": ([i32, f64, f64, f64, p(f64)], i32), "suli_boundary_waves": ([], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -296,6 +297,16 @@ class Solver:
         e = (C.c_double * 4)()
         _check(lib().suli_mms(e), "suli_mms")
         return tuple(e)
+
+    # -- wave inflow (3_contorno.f90:952-1205) -----------------------------------
+    def waves_coef(self, wave_t, a_w=0.03, p_w=2.5, h0_f=1.0):
+        """Returns l_w, n_w, f_w, c_w, aeta1..5, avel1..5."""
+        c = (C.c_double * 14)()
+        _check(lib().suli_waves_coef(wave_t, a_w, p_w, h0_f, c), "suli_waves_coef")
+        return np.array(c[:])
+
+    def boundary_waves(self):
+        _check(lib().suli_boundary_waves(), "suli_boundary_waves")
 
     def set_solver(self, poisson, precond, tol_rel=0.0, max_it=0):
         _check(lib().suli_set_solver(poisson, precond, tol_rel, max_it), "suli_set_solver")

@@ -15,6 +15,7 @@ the call changed in tests/golden/<case>.npz (float64, Fortran order).
     python tests/golden/gen_golden.py est        # est's divergence and its maximum (est/est_*)
     python tests/golden/gen_golden.py periodic   # graddin on the x-periodic grid (per_graddin)
     python tests/golden/gen_golden.py weng       # the manufactured solution of 13_mms.f90 (weng/*)
+    python tests/golden/gen_golden.py waves      # the wave inflow, Stokes I, II, V (waves/*)
 
 The scheme cases use reference builds with t_tempo = 0, 2, 3
 (0_modules.f90:24) and start from nonzero kept forces fu0, fu1.
@@ -253,6 +254,59 @@ def weng():
         print(case, "errors after the last step:", errs[-1])
 
 
+WAVE_STEPS = 3
+
+
+def wave_params(wave_t):
+    """A 4 x 1 x 1.2 m channel at 0.25 x 0.25 x 0.1 m cells, still water 1 m
+    deep, the wave entering through the prescribed wall x = 0 (code 3),
+    outflow at x = 4 (code 4), free-slip sides and lid, no-slip bed."""
+    return O.Params(nx=16, ny=4, nz=12, dx=0.25, dy=0.25, dz=0.1, ccx0=3, ccxf=4, ccy0=1, ccyf=1, ccz0=2, cczf=1,
+                    m_turb=1, gx=0.0, gz=9.81, wave_t=wave_t)
+
+
+def wave_still_water(p):
+    s = O.State(p)
+    z = (np.arange(1, p.nz + 1) - 0.5) * p.dz
+    s.ls[...] = 1.0 - z[None, None, :]
+    s.scal["dt"] = 0.01
+    s.run("ls_ini")
+    s.run("contorno", 1)
+    s.run("contorno", 3)
+    return s
+
+
+def waves():
+    """The wave inflow of reference builds with wave_t = 1, 2, 5
+    (3_contorno.f90:952-1205): waves_coef + one boundary_waves at t = 0.7 on
+    a still-water level set with seeded noise (harness waves: coefficients,
+    level set, planes), and WAVE_STEPS steps of the time loop from still
+    water (harness wave_run)."""
+    os.makedirs(os.path.join(HERE, "waves"), exist_ok=True)
+    for wt in (1, 2, 5):
+        p = wave_params(wt)
+        O.ref_build(p)
+        s = O.State(p)
+        z = (np.arange(1, p.nz + 1) - 0.5) * p.dz
+        s.ls[...] = 1.0 - z[None, None, :] + 1e-3 * np.random.default_rng(wt).standard_normal(s.ls.shape)
+        s.scal["dt"] = 0.01
+        s.scal["t"] = 0.7
+        ls_in = s.ls.copy()
+        out, ex = O.ref_run_mms(s, "waves")
+        coef = np.array([float(x) for x in out.split("WAVE_COEF")[1].split()[:14]])
+        data = {"coef": coef, "ls_in": ls_in, "ls_out": s.ls.copy(), "t": np.array([0.7]), "steps": np.array([WAVE_STEPS])}
+        for n in ("bxx0", "bxx1", "bxy0", "bxz0"):
+            data["plane_" + n] = ex[n]
+        r = wave_still_water(p)
+        _, exr = O.ref_run_mms(r, "wave_run", WAVE_STEPS)
+        for n in ("u", "v", "w", "prd1", "ls"):
+            data["run_" + n] = r.arr(n)
+        for n in ("bxx0", "bxx1", "bxz0"):
+            data["run_plane_" + n] = exr[n]
+        np.savez_compressed(os.path.join(HERE, "waves", f"wave{wt}.npz"), **data)
+        print("wave", wt, "l_w", coef[0], "max u after the run", np.abs(r.u).max())
+
+
 def main():
     for case, p in CASES.items():
         O.ref_build(p)
@@ -311,5 +365,7 @@ if __name__ == "__main__":
         periodic()
     elif sys.argv[1:] == ["weng"]:
         weng()
+    elif sys.argv[1:] == ["waves"]:
+        waves()
     else:
         main()
