@@ -489,6 +489,91 @@ def _windows(per_step, width=100):
     return out
 
 
+CONFIG0 = dict(nx=64, ny=32, nz=32, width=0.25, height=0.25, steps=200)
+
+
+def config0(cpu=True):
+    """The reference's own CPU-runnable case (64x32x32 dam break, column
+    0.25 x 0.25 m, dt = 0.2 h / (2 sqrt(g H)), 200 steps): wall time of the
+    200 steps on the GPU in FP64 with the reference's CG, in mixed precision
+    with GMG-FCG to 1e-10 (the case's Poisson tolerance), and of the
+    flang-built reference (-O3) on one pinned core of the host, with its CG
+    iteration counts.  A parity case, not the benchmark: printed as its own
+    JSON line."""
+    import re
+
+    import pysuli
+    c = CONFIG0
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    h = 1.0 / nx
+    dt = 0.2 * h / (2.0 * math.sqrt(9.81 * c["height"]))
+    out = {"workload": f"dam break {nx}x{ny}x{nz}, column {c['width']} x {c['height']} m, {c['steps']} steps",
+           "dt": dt, "gpu": {}}
+    for label, kw in (("fp64_reference_cg", dict(precision=pysuli.PREC_FP64, poisson=pysuli.POISSON_REF)),
+                      ("mixed_gmg_fcg_1e-10", dict(precision=pysuli.PREC_MIXED, poisson=pysuli.POISSON_FCG,
+                                                   precond=pysuli.PRECOND_GMG, tol_rel=1e-10))):
+        sv = pysuli.Solver(nx, ny, nz, h, **kw)
+        try:
+            sv.init_dambreak(c["width"], c["height"])
+            sv.contorno(1)
+            sv.contorno(3)
+            sv.step(dt)  # first-touch
+            sv.sync()
+            iters = 0
+            t0 = time.time()
+            for _ in range(c["steps"] - 1):
+                _, ps = sv.step(dt)
+                iters += sum(q["iters"] for q in ps)
+            sv.sync()
+            wall = time.time() - t0
+            out["gpu"][label] = {"seconds_per_step": wall / (c["steps"] - 1),
+                                 "mcell_steps_per_s": nx * ny * nz * (c["steps"] - 1) / wall / 1e6,
+                                 "iters_per_solve": iters / (2 * (c["steps"] - 1))}
+        finally:
+            sv.close()
+        progress(f"config 0 on the GPU ({label}): {out['gpu'][label]['seconds_per_step'] * 1e3:.2f} ms/step")
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        p = O.Params(nx=nx, ny=ny, nz=nz, dx=h, dy=h, dz=h)
+        exe = os.path.join(O.REF_DIR, O.ref_tag(p) + "_bench", "harness")
+        if os.path.exists(exe):
+            st = O.State(p)
+            st.ls[...] = O.dambreak_phi(p, c["width"], c["height"])
+            st.scal["dt"] = dt
+            st.run("ls_ini")
+            st.run("contorno", 1)
+            st.run("contorno", 3)
+            tmp = tempfile.mkdtemp(prefix="suli_c0_")
+            fin = os.path.join(tmp, "state.bin")
+            st.to_file(fin)
+
+            def _big_stack():  # the reference keeps large automatic arrays on the stack
+                import resource
+                hard = resource.getrlimit(resource.RLIMIT_STACK)[1]
+                resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+            progress("config 0 on the CPU: the reference's 200 steps on one core")
+            t0 = time.time()
+            r = subprocess.run(["taskset", "-c", "0", exe, fin, "-", "step", str(c["steps"])], capture_output=True,
+                               text=True, env=dict(os.environ, OMP_NUM_THREADS="1"), preexec_fn=_big_stack, cwd=tmp,
+                               timeout=1800)
+            os.remove(fin)
+            os.rmdir(tmp)
+            vals = {k: float(x) for k, x in re.findall(r"(\w+)=\s*([-+0-9.Ee]+)", r.stdout)}
+            its = [int(x) for x in re.findall(r"REF_CG_ITERS\s+(\d+)", r.stdout)]
+            out["cpu_reference"] = {"rc": r.returncode, "wall_s": time.time() - t0, "cores": 1,
+                                    "cpu_model": _cpu_model(),
+                                    "seconds": vals.get("seconds"), "graddin_seconds": vals.get("graddin_seconds"),
+                                    "level_set_seconds": vals.get("level_set_seconds"),
+                                    "iters_per_solve": sum(its) / len(its) if its else None,
+                                    "build": "flang -O3 -fdefault-real-8 -fdefault-double-8 (Makefile:7)"}
+            if vals.get("seconds"):
+                s = vals["seconds"] / c["steps"]
+                out["cpu_reference"].update(seconds_per_step=s, mcell_steps_per_s=nx * ny * nz / s / 1e6)
+    print(json.dumps({"config0": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -507,6 +592,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=240.0)
     ap.add_argument("--no-poisson", action="store_true", help="skip the preconditioner comparison")
     ap.add_argument("--poisson-repeats", type=int, default=3)
+    ap.add_argument("--config0", action="store_true",
+                    help="time the reference's own 64x32x32 case (GPU FP64 and mixed, and the CPU reference) instead")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="only run a 256 MiB device copy (scale factor of the rocprof byte counters)")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "CAL_FETCH_DIR", "CAL_WRITE_DIR"),
@@ -517,6 +604,9 @@ def main():
         return
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary, ncell=args.nx * (args.nx // 2) ** 2)
+        return
+    if args.config0:
+        config0(cpu=not args.no_cpu)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs the time step on one GPU; --gpus {args.gpus} is not supported "
