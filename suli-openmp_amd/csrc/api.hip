@@ -444,7 +444,7 @@ extern "C" int suli_upload(int field, const double *host, int64_t n) {
   if (host_to_frame(c, r.dev, r.f64, fr)) return 1;
   if (field == SULI_U || field == SULI_V || field == SULI_W) c.convdiff_pending = false;
   if (field == SULI_TFU || field == SULI_TFV || field == SULI_TFW) c.tf_on = true;
-  if (field == SULI_PRD1) c.p2n = 0;  // a new pressure: no history to extrapolate
+  if (field == SULI_PRD1) c.p2n = c.p1n = 0;  // a new pressure: no history to extrapolate
   return 0;
 }
 
@@ -725,7 +725,7 @@ extern "C" int suli_set_time_scheme(int t_tempo) {
   Ctx &c = *g_ctx;
   c.t_tempo = t_tempo;
   c.ab2_started = false;
-  c.p2n = 0;  // the pressure history belongs to the previous scheme's stages
+  c.p2n = c.p1n = 0;  // the pressure history belongs to the previous scheme's stages
   return 0;
 }
 
@@ -834,7 +834,7 @@ extern "C" int suli_set_guess(int mode) {
     return 1;
   }
   g_ctx->guess = mode;
-  g_ctx->p2n = 0;
+  g_ctx->p2n = g_ctx->p1n = 0;
   return 0;
 }
 
@@ -1001,7 +1001,7 @@ static const int kCkptFields[] = {SULI_U,      SULI_V,      SULI_W,      SULI_PR
                                   SULI_HSZ,    SULI_DDLSDX, SULI_DDLSDY, SULI_DDLSDZ, SULI_NUT,  SULI_U0,
                                   SULI_V0,     SULI_W0,     SULI_TFU,    SULI_TFV,    SULI_TFW,  SULI_FU0,
                                   SULI_FV0,    SULI_FW0};
-static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '5'};
+static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '6'};
 
 struct CkptHeader {
   char magic[8];
@@ -1020,6 +1020,7 @@ struct CkptHeader {
   // wave inflow: wave_t, amplitude, period, depth and its coefficients
   int32_t wave_t, pad2_;
   double wave_a, wave_p, wave_h0, wave_wc[14];
+  int32_t p1n, pad3_;  // first-stage pressures of the guess's history (after the last-stage ones)
 };
 
 // Binary checkpoint: the device frames of the state fields as they are (the
@@ -1041,7 +1042,7 @@ extern "C" int suli_checkpoint_save(const char *path) {
   h.tf_on = c.tf_on ? 1 : 0;
   h.vol_ini = c.vol_ini; h.vol_ins = c.vol_ins; h.t_now = c.t_now;
   h.force_amp = c.force_amp; h.force_period = c.force_period; h.force_phase = c.force_phase;
-  h.guess = c.guess; h.stage = c.stage; h.p2n = c.p2n; h.t_tempo = c.t_tempo;
+  h.guess = c.guess; h.stage = c.stage; h.p2n = c.p2n; h.t_tempo = c.t_tempo; h.p1n = c.p1n;
   h.p2dt = c.p2dt;
   h.ab2_started = c.ab2_started ? 1 : 0; h.adv = c.adv;
   h.obst = c.obst ? 1 : 0;
@@ -1070,10 +1071,11 @@ extern "C" int suli_checkpoint_save(const char *path) {
     const int32_t id = kCkptFields[q];
     ok = std::fwrite(&id, 4, 1, f) == 1 && std::fwrite(buf.data(), 1, bytes, f) == bytes;
   }
-  for (int q = 0; q < h.p2n && ok; q++) {  // the stage-2 pressures of the last steps (the guess's history)
+  // the last- and first-stage pressures of the last steps (the guess's history)
+  for (int q = 0; q < h.p2n + h.p1n && ok; q++) {
     const size_t bytes = (size_t)c.g.total * 8;
     buf.resize(bytes);
-    if (hipMemcpy(buf.data(), c.p2h[q], bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpy(buf.data(), q < h.p2n ? c.p2h[q] : c.p1h[q - h.p2n], bytes, hipMemcpyDeviceToHost) != hipSuccess) {
       std::fclose(f);
       suli_set_error("suli_checkpoint_save: device copy failed");
       return 1;
@@ -1133,15 +1135,17 @@ extern "C" int suli_checkpoint_load(const char *path) {
     }
     if (hipMemcpy(r.dev, buf.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) err = "device copy failed";
   }
-  for (int q = 0; q < h.p2n && q < 3 && err.empty(); q++) {
+  if (err.empty() && (h.p2n < 0 || h.p2n > 3 || h.p1n < 0 || h.p1n > 3)) err = "bad pressure history";
+  for (int q = 0; q < h.p2n + h.p1n && err.empty(); q++) {
     const size_t bytes = (size_t)c.g.total * 8;
     buf.resize(bytes);
-    if (!c.p2h[q]) {
-      if (hipMalloc((void **)&c.p2h[q], bytes) != hipSuccess) err = "device allocation failed";
-      else c.allocs.push_back(c.p2h[q]);
+    double *&slot = q < h.p2n ? c.p2h[q] : c.p1h[q - h.p2n];
+    if (!slot) {
+      if (hipMalloc((void **)&slot, bytes) != hipSuccess) err = "device allocation failed";
+      else c.allocs.push_back(slot);
     }
     if (err.empty() && std::fread(buf.data(), 1, bytes, f) != bytes) err = "truncated at the pressure history";
-    if (err.empty() && hipMemcpy(c.p2h[q], buf.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+    if (err.empty() && hipMemcpy(slot, buf.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
       err = "device copy failed";
   }
   const int nx = c.g.nx, ny = c.g.ny;
@@ -1179,7 +1183,7 @@ extern "C" int suli_checkpoint_load(const char *path) {
   c.tf_on = h.tf_on != 0;
   c.vol_ini = h.vol_ini; c.vol_ins = h.vol_ins; c.t_now = h.t_now;
   c.force_amp = h.force_amp; c.force_period = h.force_period; c.force_phase = h.force_phase;
-  c.guess = h.guess; c.stage = h.stage; c.p2n = std::min(h.p2n, 3); c.p2dt = h.p2dt;
+  c.guess = h.guess; c.stage = h.stage; c.p2n = std::min(h.p2n, 3); c.p1n = std::min(h.p1n, 3); c.p2dt = h.p2dt;
   c.t_tempo = h.t_tempo; c.ab2_started = h.ab2_started != 0; c.adv = h.adv;
   c.convdiff_pending = false;
   return 0;
@@ -1388,7 +1392,7 @@ extern "C" int suli_restart_read(const char *path) {
   c.vol_ini = vol[0];
   c.vol_ins = vol[1];
   c.convdiff_pending = false;
-  c.p2n = 0;
+  c.p2n = c.p1n = 0;
   c.ab2_started = false;  // the reference restarts its step counter: it = 1 takes the Euler update
   return 0;
 }

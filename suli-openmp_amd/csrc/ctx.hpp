@@ -107,11 +107,13 @@ struct Ctx {
   // initial guess of the FCG projection: the kind of stage set by the last
   // tempo call (1: first stage of a multi-stage step, step dt/2; 2: last
   // stage, step dt; 0: other); the last three last-stage pressures p2h[0]
-  // (newest) .. p2h[2], p2n of them valid, all of the base step p2dt
-  // (guess = 1: extrapolate from them)
-  int guess = 1, stage = 0, p2n = 0;
+  // (newest) .. p2h[2], p2n of them valid, and the last three first-stage
+  // pressures p1h[0..2], p1n valid, all of the base step p2dt (guess = 1:
+  // extrapolate from them)
+  int guess = 1, stage = 0, p2n = 0, p1n = 0;
   double p2dt = 0.0;
   double *p2h[3] = {nullptr, nullptr, nullptr};
+  double *p1h[3] = {nullptr, nullptr, nullptr};
   // field output: the record on the device, its pinned host copy, coordinate tables
   float *out_dev = nullptr, *out_host = nullptr, *out_tab = nullptr;
   int64_t out_cap = 0, out_host_cap = 0;

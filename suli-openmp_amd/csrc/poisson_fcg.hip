@@ -207,6 +207,17 @@ __global__ void k_guess(int64_t n, double *__restrict__ p, const double *__restr
   p[e] = order == 2 ? 3.0 * (h0[e] - h1[e]) + h2[e] : 2.0 * h0[e] - h1[e];
 }
 
+// Guess from a pressure and a linearly extrapolated offset between the
+// interleaved first- and last-stage pressures: p = x0 + 2 (x1 - x2) - (x3 - x4).
+// x0 may alias p.
+__global__ void k_guess_offset(int64_t n, double *p, const double *x0, const double *__restrict__ x1,
+                               const double *__restrict__ x2, const double *__restrict__ x3,
+                               const double *__restrict__ x4) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  p[e] = x0[e] + 2.0 * (x1[e] - x2[e]) - (x3[e] - x4[e]);
+}
+
 // x += d (FP64 += FP32)
 __global__ __launch_bounds__(256) void k_xupd(Grid g, double *__restrict__ x, const float *__restrict__ d) {
   CELL_IJK;
@@ -1658,16 +1669,34 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   const double tol = c.P.tol_rel;
   int pc = c.P.precond;
   // Initial guess.  The reference warm-starts every projection from the
-  // current pressure (9_grad.f90:121).  Here, once the stage-2 pressures of
-  // the last steps (same step size) are known, a projection starts from
-  // their extrapolation to this step: quadratic from three, linear from two
-  // (stage 1 only; stage 2 otherwise keeps the stage-1 pressure, already
-  // close).  On the dam break that makes the initial residual 10-25x
-  // smaller; suli_set_guess(0) restores the reference's choice.
+  // current pressure (9_grad.f90:121).  Here, once the pressures of the last
+  // steps (same step size) are known, a projection starts from an
+  // extrapolation in time.  The first- and last-stage pressures of the steps
+  // interleave (g = first stage, h = last stage, newest first: ..., g1, h1,
+  // g0, h0): a first-stage projection starts from h0 plus the offset between
+  // a first-stage pressure and the last-stage pressure before it, extrapolated
+  // linearly, h0 + 2 (g0 - h1) - (g1 - h2); a last-stage projection from the
+  // first-stage pressure just solved plus the offset between a last-stage
+  // pressure and its own step's first-stage one, extrapolated linearly,
+  // g + 2 (h0 - g0) - (h1 - g1).  With less history: the quadratic (linear)
+  // extrapolation of the last-stage pressures, or the first-stage pressure.
+  // On the dam break the offset forms take 6 and 4.75 FCG iterations at step
+  // 500 where the plain extrapolation takes 7 and 5.9 (3 h0 - 3 h1 + h2 for
+  // both) and the current pressure 8 and 7.8; suli_set_guess(0) restores the
+  // reference's choice.
   const double base_dt = c.stage == 1 ? 2.0 * dt : dt;  // a_dt(1) = dt/2, a_dt(2) = dt
-  if (c.guess == 1 && (c.stage == 1 || c.stage == 2) && c.p2n >= 2 && base_dt == c.p2dt &&
-      (c.p2n >= 3 || c.stage == 1)) {
-    hipLaunchKernelGGL(k_guess, dim3((unsigned)((g.total + 255) / 256)), dim3(256), 0, c.st, (int64_t)g.total,
+  const bool hist = c.guess == 1 && (c.stage == 1 || c.stage == 2) && base_dt == c.p2dt;
+  const unsigned gb = (unsigned)((g.total + 255) / 256);
+  if (hist && c.stage == 1 && c.p2n >= 3 && c.p1n >= 2) {
+    hipLaunchKernelGGL(k_guess_offset, dim3(gb), dim3(256), 0, c.st, (int64_t)g.total, c.prd1,
+                       (const double *)c.p2h[0], (const double *)c.p1h[0], (const double *)c.p2h[1],
+                       (const double *)c.p1h[1], (const double *)c.p2h[2]);
+  } else if (hist && c.stage == 2 && c.p2n >= 2 && c.p1n >= 3) {  // p1h[0]: this step's first stage
+    hipLaunchKernelGGL(k_guess_offset, dim3(gb), dim3(256), 0, c.st, (int64_t)g.total, c.prd1,
+                       (const double *)c.prd1, (const double *)c.p2h[0], (const double *)c.p1h[1],
+                       (const double *)c.p2h[1], (const double *)c.p1h[2]);
+  } else if (hist && c.p2n >= 2 && (c.p2n >= 3 || c.stage == 1)) {
+    hipLaunchKernelGGL(k_guess, dim3(gb), dim3(256), 0, c.st, (int64_t)g.total,
                        c.prd1, (const double *)c.p2h[0], (const double *)c.p2h[1], (const double *)c.p2h[2],
                        c.p2n >= 3 ? 2 : 1);
   }
@@ -1873,17 +1902,19 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   HIPCHK(hipGetLastError());
   double gpu_ms = 0.0;
   if (solve_elapsed_ms(c, &gpu_ms)) return 1;
-  if (c.stage == 2) {  // this solution joins the history
-    if (base_dt != c.p2dt) c.p2n = 0;
+  if (c.stage == 1 || c.stage == 2) {  // this solution joins its stage's history
+    if (base_dt != c.p2dt) c.p2n = c.p1n = 0;
     c.p2dt = base_dt;
-    for (double *&h : c.p2h)
-      if (!h && dalloc64(c, &h, g.total)) return 1;
-    double *oldest = c.p2h[2];
-    c.p2h[2] = c.p2h[1];
-    c.p2h[1] = c.p2h[0];
-    c.p2h[0] = oldest;
-    HIPCHK(hipMemcpyAsync(c.p2h[0], c.prd1, (size_t)g.total * sizeof(double), hipMemcpyDeviceToDevice, c.st));
-    c.p2n = std::min(c.p2n + 1, 3);
+    double **hh = c.stage == 2 ? c.p2h : c.p1h;
+    int &hn = c.stage == 2 ? c.p2n : c.p1n;
+    for (int q = 0; q < 3; q++)
+      if (!hh[q] && dalloc64(c, &hh[q], g.total)) return 1;
+    double *oldest = hh[2];
+    hh[2] = hh[1];
+    hh[1] = hh[0];
+    hh[0] = oldest;
+    HIPCHK(hipMemcpyAsync(hh[0], c.prd1, (size_t)g.total * sizeof(double), hipMemcpyDeviceToDevice, c.st));
+    hn = std::min(hn + 1, 3);
   }
   if (stt) {
     std::memset(stt, 0, sizeof(*stt));
