@@ -140,12 +140,25 @@ __global__ __launch_bounds__(256) void k_r32(Grid g, const double *__restrict__ 
 // overall tolerance needs), from the outer residual of this pass (rout) and
 // ||b|| (bb) already on the device; when that residual meets the tolerance
 // the pass does not run (done).
+// One FP32 pass reaches a reduction of a few 1e-7 before its single
+// precision limits it: a warm start needing at least 1e-4 times the usual
+// reduction (an initial residual below ~0.3 of the tolerance's reciprocal
+// scale) takes one pass aimed at the tolerance, with 3e-3 of the usual
+// reduction as the floor.  On the dam break that leaves one pass per
+// projection through step 700 (1.85 passes per solve at steps 506-525 with
+// the thresholds 0.03 and 0.1, 1.0 with these; 769 -> 801 Mcell-steps/s).
 #ifndef SULI_FCG_ONEPASS_WANT
-#define SULI_FCG_ONEPASS_WANT 0.03  // one pass when it needs at least this fraction of the usual reduction
+#define SULI_FCG_ONEPASS_WANT 0.0001  // one pass when it needs at least this fraction of the usual reduction
 #endif
 #ifndef SULI_FCG_ONEPASS_FLOOR
-#define SULI_FCG_ONEPASS_FLOOR 0.1  // and then aims for at most this fraction of it
+#define SULI_FCG_ONEPASS_FLOOR 0.003  // and then aims for at most this fraction of it
 #endif
+// The learned preconditioner's passes are capped (kLearnedPassCap): a deep
+// single pass would stall against the cap and hand the solve to multigrid,
+// so its passes keep to at most 10x the usual reduction, and only when that
+// ends the solve (a warm start within 3e-2 of the usual reduction).
+#define SULI_FCG_ONEPASS_WANT_CAPPED 0.03
+#define SULI_FCG_ONEPASS_FLOOR_CAPPED 0.1
 // squared norms of the solvable parts of the outer residual and of b_s
 __host__ __device__ inline void proj_norms(const double *rout, double bb_full, double &rr, double &bb) {
   const double nc = rout[2] > 0 ? rout[1] * rout[1] / rout[2] : 0.0;
@@ -156,16 +169,17 @@ struct EpiR32 {
   FcgCtl *ctl;
   double rel_in, tol;
   int maxit;
+  double want_frac, floor_frac;  // one-pass thresholds (SULI_FCG_ONEPASS_*)
   __device__ bool enabled() const { return true; }
   __device__ void operator()(const double *res) const {
     double rr, bb;
     proj_norms(ctl->rout, ctl->bb[0], rr, bb);
     const bool conv = !(rr > tol * tol * bb);
     const double want = rr > 0 ? 0.3 * tol * sqrt(bb) / sqrt(rr) : 1.0;
-    // one pass of at most 10x the usual reduction when that ends the solve
-    // (a warm start within 1e-3 of the tolerance's residual); cold starts
-    // take passes of rel_in, which cost fewer iterations in FP32
-    const double floor = want >= SULI_FCG_ONEPASS_WANT * rel_in ? SULI_FCG_ONEPASS_FLOOR * rel_in : rel_in;
+    // one pass of up to 1/floor_frac times the usual reduction when that
+    // ends the solve (a warm start); cold starts take passes of rel_in,
+    // which cost fewer iterations in FP32
+    const double floor = want >= want_frac * rel_in ? floor_frac * rel_in : rel_in;
     const double rin = fmax(floor, fmin(want, 1.0));
     ctl->thr_in = res[0] * rin * rin;
     ctl->maxit_in = maxit;
@@ -1790,7 +1804,11 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       ProfScope ps(c, "fcg_project");
       const RedSlot r3{c.partials, CTL_PTR(f.ctl, rin0)};
       hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rel_in, pass_max, r3);
-      launch_finish<1>(c.st, gk, r3, EpiR32{f.ctl, rel_in, tol, pass_max});
+      const bool capped = pc == SULI_PRECOND_LEARNED;
+      launch_finish<1>(c.st, gk, r3,
+                       EpiR32{f.ctl, rel_in, tol, pass_max,
+                              capped ? SULI_FCG_ONEPASS_WANT_CAPPED : SULI_FCG_ONEPASS_WANT,
+                              capped ? SULI_FCG_ONEPASS_FLOOR_CAPPED : SULI_FCG_ONEPASS_FLOOR});
     }
     unet_capture(c, f.unet, f.r32, active, 0);
     // Preconditioned passes take few iterations and consecutive solves of a
