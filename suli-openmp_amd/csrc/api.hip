@@ -416,6 +416,7 @@ static int host_to_frame(Ctx &c, void *dev, bool f64, const std::vector<double> 
 extern "C" int suli_upload(int field, const double *host, int64_t n) {
   NEEDCTX();
   Ctx &c = *g_ctx;
+  c.rho_epoch++;  // the projection rebuilds its operator
   if (is_plane(field)) {
     if (n != suli_field_len(field)) {
       suli_set_error("suli_upload: plane " + std::to_string(field) + " expects " +
@@ -508,6 +509,7 @@ static int read_res(Ctx &c, int off, int n) {
 }
 
 static int heavi_curv(Ctx &c, bool curv, double *vol) {
+  c.rho_epoch++;  // the density follows the heaviside
   { ProfScope ps_(c, "heaviside_normals");
   DISPATCH(c, Launch<TT>::heavi(c.g, c.ph, c.st, c.ls, c.hs, c.hsx, c.hsy, c.hsz, c.rho, c.lsnu, c.modls,
                                 c.nrx, c.nry, c.nrz, c.slot(RS_VOL, 0))); }
@@ -718,6 +720,7 @@ extern "C" int suli_tempo(int tt, double dt) {
 
 extern "C" int suli_set_time_scheme(int t_tempo) {
   NEEDCTX();
+  g_ctx->rho_epoch++;  // the projection rebuilds its operator
   if (t_tempo < SULI_TIME_EULER || t_tempo > SULI_TIME_AB2) {
     suli_set_error("suli_set_time_scheme: t_tempo must be 0 (Euler), 1 (RK2), 2 (RK3) or 3 (AB2)");
     return 1;
@@ -743,6 +746,7 @@ extern "C" int suli_set_advection(int adv_type, int der) {
 extern "C" int suli_set_obstacle(const int32_t *ku, const int32_t *kv, const int32_t *kw) {
   NEEDCTX();
   Ctx &c = *g_ctx;
+  c.rho_epoch++;  // the projection rebuilds its operator
   const Grid &g = c.g;
   if (!ku && !kv && !kw) {
     c.obst = false;
@@ -803,6 +807,7 @@ extern "C" int suli_contorno(int nlock) {
     { ProfScope ps_(c, "contorno"); DISPATCH(c, Launch<TT>::obst_ls(c.g, c.st, c.ls, c.kw_d)); }
     // heaviside (14_level_set.f90:611); mod_ls is not recomputed there, so
     // the fused mod_ls output goes to the free reinitialisation buffer
+    c.rho_epoch++;
     { ProfScope ps_(c, "heaviside_normals");
       DISPATCH(c, Launch<TT>::heavi(c.g, c.ph, c.st, c.ls, c.hs, c.hsx, c.hsy, c.hsz, c.rho, c.lsnu, c.S, c.nrx,
                                     c.nry, c.nrz, c.slot(RS_VOL, 0))); }
@@ -823,7 +828,10 @@ extern "C" int suli_contorno(int nlock) {
 extern "C" int suli_graddin(double dt, suli_poisson_stats *st) {
   NEEDCTX();
   Ctx &c = *g_ctx;
-  if (c.P.poisson == SULI_POISSON_REF) return poisson_ref_solve(c, dt, st);
+  if (c.P.poisson == SULI_POISSON_REF) {
+    c.rho_epoch++;  // its set-up rewrites the operator arrays the FCG solve keeps
+    return poisson_ref_solve(c, dt, st);
+  }
   return fcg_solve(c, dt, st);
 }
 
@@ -841,6 +849,7 @@ extern "C" int suli_set_guess(int mode) {
 extern "C" int suli_set_solver(int poisson, int precond, double tol_rel, int max_it) {
   NEEDCTX();
   Ctx &c = *g_ctx;
+  c.rho_epoch++;  // the projection rebuilds its operator
   if (poisson != SULI_POISSON_REF && poisson != SULI_POISSON_FCG) {
     suli_set_error("suli_set_solver: unknown poisson solver");
     return 1;
@@ -858,6 +867,7 @@ extern "C" int suli_set_solver(int poisson, int precond, double tol_rel, int max
 
 extern "C" int suli_set_mg(int64_t fuse_max_cells, int coarse_sweeps, int fuse_fine) {
   NEEDCTX();
+  g_ctx->rho_epoch++;  // the projection rebuilds its operator
   if (fuse_max_cells >= 0) g_ctx->mg_fuse_max = fuse_max_cells;
   if (coarse_sweeps >= 0) g_ctx->mg_coarse_sweeps = coarse_sweeps;
   if (fuse_fine >= 0) g_ctx->mg_fuse_fine = fuse_fine ? 1 : 0;
@@ -1107,6 +1117,7 @@ extern "C" int suli_checkpoint_save(const char *path) {
 extern "C" int suli_checkpoint_load(const char *path) {
   NEEDCTX();
   Ctx &c = *g_ctx;
+  c.rho_epoch++;  // the projection rebuilds its operator
   FILE *f = path ? std::fopen(path, "rb") : nullptr;
   if (!f) {
     suli_set_error(std::string("suli_checkpoint_load: cannot open ") + (path ? path : "(null)"));
@@ -1340,6 +1351,7 @@ struct LdReader {
 extern "C" int suli_restart_read(const char *path) {
   NEEDCTX();
   Ctx &c = *g_ctx;
+  c.rho_epoch++;  // the projection rebuilds its operator
   FILE *f = path ? std::fopen(path, "r") : nullptr;
   if (!f) {
     suli_set_error(std::string("suli_restart_read: cannot open ") + (path ? path : "(null)"));

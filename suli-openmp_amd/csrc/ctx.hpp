@@ -104,6 +104,10 @@ struct Ctx {
   bool convdiff_pending = false;
   double vol_ini = 0, vol_ins = 0;
   PoissonFCG *fcg = nullptr;
+  // bumped whenever the density (or anything else the projection's operator
+  // is built from) may have changed: heaviside, uploads, restarts, solver
+  // settings.  The FCG projection rebuilds its operator only when it moved.
+  uint64_t rho_epoch = 1;
   // initial guess of the FCG projection: the kind of stage set by the last
   // tempo call (1: first stage of a multi-stage step, step dt/2; 2: last
   // stage, step dt; 0: other); the last three last-stage pressures p2h[0]

@@ -1256,18 +1256,21 @@ __global__ void k_pout(Grid g, const double *__restrict__ x, const double *__res
   p[g.idx(i, j, k)] = x[s] / sD[s];
 }
 
-// initial iterate x = p sqrt(D) and b_s = q/sqrt(D) with the block sums of b_s.b_s
-__global__ __launch_bounds__(256) void k_xb(Grid g, const double *__restrict__ p, const double *__restrict__ q,
-                                            const double *__restrict__ sD, double *__restrict__ x,
-                                            double *__restrict__ b, RedSlot rs) {
+// initial iterate x = p sqrt(D) and b_s = q qs/sqrt(D) with the block sums of
+// b_s.b_s; q = -div u* formed here with k_pD's expression
+template <typename T>
+__global__ __launch_bounds__(256) void k_xbq(Grid g, const double *__restrict__ p, const T *__restrict__ u,
+                                             const T *__restrict__ v, const T *__restrict__ w,
+                                             const double *__restrict__ sD, double qs, double *__restrict__ x,
+                                             double *__restrict__ b, RedSlot rs) {
   CELL_IJK;
   double acc[1] = {0.0};
   if (i <= g.nx && j <= g.ny)
     for (int k = kz0; k <= g.nz; k += gridDim.z) {
-      const int64_t c = g.idx(i, j, k);
+      const fidx c = g.idx(i, j, k);
       const double sd = sD[c];
       x[c] = p[c] * sd;
-      const double bv = q[c] / sd;
+      const double bv = div_q<T>(g, u, v, w, c) * qs / sd;
       b[c] = bv;
       acc[0] += bv * bv;
     }
@@ -1296,6 +1299,9 @@ struct PoissonFCG {
   Grid hg{};
   float *x0c = nullptr, *x1c = nullptr, *b0c = nullptr, *b1c = nullptr, *d0c = nullptr;
   bool setup_done = false;  // operator copies and hierarchy built by a solve
+  // the density epoch (Ctx::rho_epoch) and the dt they were built for
+  uint64_t op_epoch = 0;
+  double op_dt = 0.0;
   int kz = 1;               // grid.z of the k-looped reduction kernels
   UNetPrecond *unet = nullptr;
   float *pm[SULI_FCG_M] = {}, *qm[SULI_FCG_M] = {};  // FCG(m) directions and their images
@@ -1714,27 +1720,47 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
                        c.prd1, (const double *)c.p2h[0], (const double *)c.p2h[1], (const double *)c.p2h[2],
                        c.p2n >= 3 ? 2 : 1);
   }
-  // FP64 set-up shared with the reference CG
+  // Operator.  c_f = dt/(h^2 rho_f), D = sum of c_f and the scaled faces
+  // a_f = c_f/sqrt(D_P D_Q) depend on dt only through a common factor, so
+  // while the density is unchanged (the stages of one step) the operator
+  // built for dt_s serves a projection at dt: A(dt) p = q is
+  // A(dt_s) p = q dt_s/dt, and with dt = 2 dt_s (the two stages of RK2) the
+  // scaling is exact.  The faces, D, sqrt(D) and the multigrid hierarchy are
+  // then kept and only b_s is formed.
+#ifndef SULI_FCG_REUSE_OPERATOR
+#define SULI_FCG_REUSE_OPERATOR 1
+#endif
+  const bool reuse = SULI_FCG_REUSE_OPERATOR && f.setup_done && f.op_epoch == c.rho_epoch && f.op_dt > 0.0;
   {
     ProfScope ps(c, "poisson_setup");
     const RedSlot rb{c.partials, CTL_PTR(f.ctl, bb)};
-    // (the fused face pass reads mirrored cells itself: no ghost fill)
-    DISPATCH(c, Launch<TT>::pD(g, c.st, dt, c.u, c.v, c.w, c.rho, c.D, c.sD, c.q, !c.mg_fuse_fine));
-    if (c.mg_fuse_fine) {
-      // the faces in one pass with the finest multigrid level's BF16 faces
-      MgLevel &l0 = f.lv[0];
-      const FcgFaceOut fo{c.ax, c.ay, c.az, f.ax32, f.ay32, f.az32, l0.hx, l0.hy, l0.hz};
-      DISPATCH(c, Launch<TT>::fcg_faces(g, c.st, dt, c.rho, c.D, fo));
-      hipLaunchKernelGGL(k_mg_diag_bf16, cells_full(c.g), b2, 0, c.st, c.g, l0.hx, l0.hy, l0.hz, l0.dg,
-                         (const double *)c.sD, f.sD32, f.hg, f.d0c);
-    } else {
-      DISPATCH(c, Launch<TT>::pcoef(g, c.st, dt, c.rho, c.D, c.ax, c.ay, c.az));
+    if (!reuse) {
+      // FP64 set-up shared with the reference CG (q formed below)
+      // (the fused face pass reads mirrored cells itself: no ghost fill)
+      DISPATCH(c, Launch<TT>::pD(g, c.st, dt, c.u, c.v, c.w, c.rho, c.D, c.sD, nullptr, !c.mg_fuse_fine));
+      if (c.mg_fuse_fine) {
+        // the faces in one pass with the finest multigrid level's BF16 faces
+        MgLevel &l0 = f.lv[0];
+        const FcgFaceOut fo{c.ax, c.ay, c.az, f.ax32, f.ay32, f.az32, l0.hx, l0.hy, l0.hz};
+        DISPATCH(c, Launch<TT>::fcg_faces(g, c.st, dt, c.rho, c.D, fo));
+        hipLaunchKernelGGL(k_mg_diag_bf16, cells_full(c.g), b2, 0, c.st, c.g, l0.hx, l0.hy, l0.hz, l0.dg,
+                           (const double *)c.sD, f.sD32, f.hg, f.d0c);
+      } else {
+        DISPATCH(c, Launch<TT>::pcoef(g, c.st, dt, c.rho, c.D, c.ax, c.ay, c.az));
+      }
     }
-    hipLaunchKernelGGL(k_xb, gk, b2, 0, c.st, g, c.prd1, c.q, c.sD, f.x, f.b, rb);
+    const double qs = reuse ? f.op_dt / dt : 1.0;
+    DISPATCH(c, hipLaunchKernelGGL((k_xbq<TT>), gk, b2, 0, c.st, g, (const double *)c.prd1, (const TT *)c.u,
+                                   (const TT *)c.v, (const TT *)c.w, (const double *)c.sD, qs, f.x, f.b, rb));
     launch_finish<1>(c.st, gk, rb);
   }
-  if (mg_setup(c, !c.mg_fuse_fine)) return 1;
-  f.setup_done = true;
+  if (!reuse) {
+    f.setup_done = false;
+    if (mg_setup(c, !c.mg_fuse_fine)) return 1;
+    f.setup_done = true;
+    f.op_epoch = c.rho_epoch;
+    f.op_dt = dt;
+  }
   const int entry = unet_entry(f.unet);
   if (entry > 0 && entry >= (int)f.lv.size() && (pc == SULI_PRECOND_LEARNED || unet_capture_armed(f.unet))) {
     suli_set_error("learned preconditioner: entry level " + std::to_string(entry) + " but the multigrid hierarchy has " +

@@ -32,7 +32,15 @@ __device__ __forceinline__ double face_coef(const T *__restrict__ rho, fidx cf, 
   return dth2 / rf;
 }
 
-// D and q = -div u* (:80-87), sqrt(D).
+// q = -div u* at a cell (:80-87)
+template <typename T>
+__device__ __forceinline__ double div_q(const Grid &g, const T *__restrict__ u, const T *__restrict__ v,
+                                        const T *__restrict__ w, fidx c) {
+  return ((double)u[c] - (double)u[c + 1]) / g.dx + ((double)v[c] - (double)v[c + g.sx]) / g.dy +
+         ((double)w[c] - (double)w[c + g.sz]) / g.dz;
+}
+
+// D and q = -div u* (:80-87), sqrt(D); q only when q is set.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pD(Grid g, double dt, const T *__restrict__ u,
                                             const T *__restrict__ v, const T *__restrict__ w,
@@ -49,8 +57,7 @@ __global__ __launch_bounds__(256) void k_pD(Grid g, double dt, const T *__restri
                    face_coef<T>(rho, c + Z, Z, k + 1, g.nz, mz) + face_coef<T>(rho, c, Z, k, g.nz, mz);
   D[c] = d;
   sD[c] = sqrt(d);
-  q[c] = ((double)u[c] - (double)u[c + X]) / g.dx + ((double)v[c] - (double)v[c + Y]) / g.dy +
-         ((double)w[c] - (double)w[c + Z]) / g.dz;
+  if (q) q[c] = div_q<T>(g, u, v, w, c);
 }
 
 // Ghost cells of a P-extent array, (0:nx+1,0:ny+1,0:nz+1), filled with the
