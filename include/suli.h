@@ -409,12 +409,14 @@ int suli_unet_sample_loss_grad(int idx, double *loss, float *grad, float *z);
 
 /* Initial guess of the FCG projection (SULI_POISSON_FCG; the reference CG
  * always starts from the current pressure, 9_grad.f90:121).  1 (default):
- * once the stage-2 pressures of the last steps (same step size) are known,
- * a projection of RK stage 1 or 2 (the stage set by the last suli_tempo)
- * starts from their extrapolation to this step, 3 p2(n) - 3 p2(n-1) +
- * p2(n-2) (from two: 2 p2(n) - p2(n-1), stage 1 only); 0: from the current
- * pressure, as the reference.  Uploading prd1 restarts the history.  No
- * counterpart in the reference. */
+ * a projection of the first or last stage of a step (the stage set by the
+ * last suli_tempo) starts from the current pressure plus its stage's offset
+ * extrapolated linearly in time, p + 2 d(n-1) - d(n-2) (p + d(n-1) with one
+ * offset known), where d(n) = g(n) - h(n-1) for the first stage and
+ * h(n) - g(n) for the last (g, h: the first- and last-stage pressures of
+ * step n, same step size); 0: from the current pressure, as the reference.
+ * Uploading prd1, a restart file or another time scheme restarts the
+ * history.  No counterpart in the reference. */
 int suli_set_guess(int mode);
 
 /* Field output (plot_f, 4_plot.f90:260-363): the one FP32 record of an

@@ -445,7 +445,7 @@ extern "C" int suli_upload(int field, const double *host, int64_t n) {
   if (host_to_frame(c, r.dev, r.f64, fr)) return 1;
   if (field == SULI_U || field == SULI_V || field == SULI_W) c.convdiff_pending = false;
   if (field == SULI_TFU || field == SULI_TFV || field == SULI_TFW) c.tf_on = true;
-  if (field == SULI_PRD1) c.p2n = c.p1n = 0;  // a new pressure: no history to extrapolate
+  if (field == SULI_PRD1) guess_reset(c);  // a new pressure: no history to extrapolate
   return 0;
 }
 
@@ -728,7 +728,7 @@ extern "C" int suli_set_time_scheme(int t_tempo) {
   Ctx &c = *g_ctx;
   c.t_tempo = t_tempo;
   c.ab2_started = false;
-  c.p2n = c.p1n = 0;  // the pressure history belongs to the previous scheme's stages
+  guess_reset(*g_ctx);  // the pressure history belongs to the previous scheme's stages
   return 0;
 }
 
@@ -830,6 +830,7 @@ extern "C" int suli_graddin(double dt, suli_poisson_stats *st) {
   Ctx &c = *g_ctx;
   if (c.P.poisson == SULI_POISSON_REF) {
     c.rho_epoch++;  // its set-up rewrites the operator arrays the FCG solve keeps
+    c.last = 0;     // nor does its pressure continue the FCG guess's history
     return poisson_ref_solve(c, dt, st);
   }
   return fcg_solve(c, dt, st);
@@ -842,7 +843,7 @@ extern "C" int suli_set_guess(int mode) {
     return 1;
   }
   g_ctx->guess = mode;
-  g_ctx->p2n = g_ctx->p1n = 0;
+  guess_reset(*g_ctx);
   return 0;
 }
 
@@ -1011,7 +1012,7 @@ static const int kCkptFields[] = {SULI_U,      SULI_V,      SULI_W,      SULI_PR
                                   SULI_HSZ,    SULI_DDLSDX, SULI_DDLSDY, SULI_DDLSDZ, SULI_NUT,  SULI_U0,
                                   SULI_V0,     SULI_W0,     SULI_TFU,    SULI_TFV,    SULI_TFW,  SULI_FU0,
                                   SULI_FV0,    SULI_FW0};
-static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '6'};
+static const char kCkptMagic[8] = {'S', 'U', 'L', 'I', 'C', 'K', 'P', '7'};
 
 struct CkptHeader {
   char magic[8];
@@ -1019,8 +1020,8 @@ struct CkptHeader {
   int64_t frame_total;
   int32_t nfields, tf_on;
   double vol_ini, vol_ins, t_now, force_amp, force_period, force_phase;
-  int32_t guess, stage, p2n, t_tempo;
-  double p2dt;
+  int32_t guess, stage, last, t_tempo;
+  double hdt;
   int32_t ab2_started, adv;
   int32_t obst, pad_;  // 1: the bottom-obstacle columns ku, kv, kw follow the pressure history
   // manufactured-solution mode; the boundary planes (field precision) follow
@@ -1030,7 +1031,7 @@ struct CkptHeader {
   // wave inflow: wave_t, amplitude, period, depth and its coefficients
   int32_t wave_t, pad2_;
   double wave_a, wave_p, wave_h0, wave_wc[14];
-  int32_t p1n, pad3_;  // first-stage pressures of the guess's history (after the last-stage ones)
+  int32_t offn[2];  // the guess's FP32 offsets, first then last stage (after the fields)
 };
 
 // Binary checkpoint: the device frames of the state fields as they are (the
@@ -1052,8 +1053,9 @@ extern "C" int suli_checkpoint_save(const char *path) {
   h.tf_on = c.tf_on ? 1 : 0;
   h.vol_ini = c.vol_ini; h.vol_ins = c.vol_ins; h.t_now = c.t_now;
   h.force_amp = c.force_amp; h.force_period = c.force_period; h.force_phase = c.force_phase;
-  h.guess = c.guess; h.stage = c.stage; h.p2n = c.p2n; h.t_tempo = c.t_tempo; h.p1n = c.p1n;
-  h.p2dt = c.p2dt;
+  h.guess = c.guess; h.stage = c.stage; h.last = c.last; h.t_tempo = c.t_tempo;
+  h.offn[0] = c.offn[0]; h.offn[1] = c.offn[1];
+  h.hdt = c.hdt;
   h.ab2_started = c.ab2_started ? 1 : 0; h.adv = c.adv;
   h.obst = c.obst ? 1 : 0;
   h.mms_t = c.mms.t; h.mms_src_done = c.mms.src_done ? 1 : 0;
@@ -1081,11 +1083,12 @@ extern "C" int suli_checkpoint_save(const char *path) {
     const int32_t id = kCkptFields[q];
     ok = std::fwrite(&id, 4, 1, f) == 1 && std::fwrite(buf.data(), 1, bytes, f) == bytes;
   }
-  // the last- and first-stage pressures of the last steps (the guess's history)
-  for (int q = 0; q < h.p2n + h.p1n && ok; q++) {
-    const size_t bytes = (size_t)c.g.total * 8;
+  // the offsets of the first- and last-stage pressures (the guess's history)
+  for (int q = 0; q < h.offn[0] + h.offn[1] && ok; q++) {
+    const size_t bytes = (size_t)c.g.total * 4;
     buf.resize(bytes);
-    if (hipMemcpy(buf.data(), q < h.p2n ? c.p2h[q] : c.p1h[q - h.p2n], bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    const float *src = q < h.offn[0] ? c.off[0][q] : c.off[1][q - h.offn[0]];
+    if (hipMemcpy(buf.data(), src, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
       std::fclose(f);
       suli_set_error("suli_checkpoint_save: device copy failed");
       return 1;
@@ -1146,11 +1149,13 @@ extern "C" int suli_checkpoint_load(const char *path) {
     }
     if (hipMemcpy(r.dev, buf.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) err = "device copy failed";
   }
-  if (err.empty() && (h.p2n < 0 || h.p2n > 3 || h.p1n < 0 || h.p1n > 3)) err = "bad pressure history";
-  for (int q = 0; q < h.p2n + h.p1n && err.empty(); q++) {
-    const size_t bytes = (size_t)c.g.total * 8;
+  if (err.empty() && (h.offn[0] < 0 || h.offn[0] > 3 || h.offn[1] < 0 || h.offn[1] > 3 || h.last < 0 ||
+                      h.last > 2))
+    err = "bad pressure history";
+  for (int q = 0; q < h.offn[0] + h.offn[1] && err.empty(); q++) {
+    const size_t bytes = (size_t)c.g.total * 4;
     buf.resize(bytes);
-    double *&slot = q < h.p2n ? c.p2h[q] : c.p1h[q - h.p2n];
+    float *&slot = q < h.offn[0] ? c.off[0][q] : c.off[1][q - h.offn[0]];
     if (!slot) {
       if (hipMalloc((void **)&slot, bytes) != hipSuccess) err = "device allocation failed";
       else c.allocs.push_back(slot);
@@ -1194,7 +1199,8 @@ extern "C" int suli_checkpoint_load(const char *path) {
   c.tf_on = h.tf_on != 0;
   c.vol_ini = h.vol_ini; c.vol_ins = h.vol_ins; c.t_now = h.t_now;
   c.force_amp = h.force_amp; c.force_period = h.force_period; c.force_phase = h.force_phase;
-  c.guess = h.guess; c.stage = h.stage; c.p2n = std::min(h.p2n, 3); c.p1n = std::min(h.p1n, 3); c.p2dt = h.p2dt;
+  c.guess = h.guess; c.stage = h.stage; c.last = h.last; c.offn[0] = h.offn[0]; c.offn[1] = h.offn[1];
+  c.hdt = h.hdt;
   c.t_tempo = h.t_tempo; c.ab2_started = h.ab2_started != 0; c.adv = h.adv;
   c.convdiff_pending = false;
   return 0;
@@ -1404,7 +1410,7 @@ extern "C" int suli_restart_read(const char *path) {
   c.vol_ini = vol[0];
   c.vol_ins = vol[1];
   c.convdiff_pending = false;
-  c.p2n = c.p1n = 0;
+  guess_reset(c);
   c.ab2_started = false;  // the reference restarts its step counter: it = 1 takes the Euler update
   return 0;
 }

@@ -110,14 +110,18 @@ struct Ctx {
   uint64_t rho_epoch = 1;
   // initial guess of the FCG projection: the kind of stage set by the last
   // tempo call (1: first stage of a multi-stage step, step dt/2; 2: last
-  // stage, step dt; 0: other); the last three last-stage pressures p2h[0]
-  // (newest) .. p2h[2], p2n of them valid, and the last three first-stage
-  // pressures p1h[0..2], p1n valid, all of the base step p2dt (guess = 1:
-  // extrapolate from them)
-  int guess = 1, stage = 0, p2n = 0, p1n = 0;
-  double p2dt = 0.0;
-  double *p2h[3] = {nullptr, nullptr, nullptr};
-  double *p1h[3] = {nullptr, nullptr, nullptr};
+  // stage, step dt; 0: other).  Per kind k of the first and last stages, the
+  // offsets of its last solutions from the pressure each solve started from,
+  // newest first, in FP32: off[0] first stage, g(n) - h(n-1); off[1] last
+  // stage, h(n) - g(n) (g, h the first- and last-stage pressures of step n),
+  // offn[k] of them valid, all of the base step hdt.  last: the kind (1, 2)
+  // of the solve whose pressure prd1 holds, 0 after any other change of it;
+  // an offset joins its history only when the solve started from the other
+  // kind's pressure.  (guess = 1: extrapolate the offsets.)
+  int guess = 1, stage = 0, last = 0;
+  int offn[2] = {0, 0};
+  double hdt = 0.0;
+  float *off[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   // field output: the record on the device, its pinned host copy, coordinate tables
   float *out_dev = nullptr, *out_host = nullptr, *out_tab = nullptr;
   int64_t out_cap = 0, out_host_cap = 0;
@@ -153,6 +157,12 @@ int waves_now(Ctx &c);
       return 1;                                                                      \
     }                                                                                \
   } while (0)
+
+// the pressure changed other than by a projection: no history to extrapolate
+inline void guess_reset(Ctx &c) {
+  c.offn[0] = c.offn[1] = 0;
+  c.last = 0;
+}
 
 // dispatch on the field precision
 #define DISPATCH(ctx, CALL)              \

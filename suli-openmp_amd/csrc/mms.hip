@@ -457,7 +457,7 @@ extern "C" int suli_mms_i(void) {
                                  (TT *)c.u, (TT *)c.v, (TT *)c.w, c.prd1));
   HIPCHK(hipGetLastError());
   c.convdiff_pending = false;
-  c.p2n = c.p1n = 0;
+  guess_reset(c);
   c.mms.src_done = false;
   return 0;
 }

@@ -54,6 +54,7 @@ struct FcgCtl {
   int it;            // inner iterations of this pass
   int first;         // next K1 is the first of the pass (beta = 0)
   int done;          // the last outer residual met the tolerance (set with the pass's arming)
+  int out;           // the last outer residual ends the solve: the output kernel writes
   // FCG(m) with the learned preconditioner: the last SULI_FCG_M directions
   int nk;                    // directions held (0..SULI_FCG_M)
   double zqm[SULI_FCG_M];    // z.q_j of the current z
@@ -190,6 +191,21 @@ struct EpiR32 {
     ctl->active = (res[0] > 0 && !conv) ? 1 : 0;
   }
 };
+// the outer residual's verdict for the output kernel: the solve ends with it
+// when it meets the tolerance (the host's test, proj_res) or when force is set
+struct EpiOut {
+  FcgCtl *ctl;
+  double tol;
+  int force;
+  __device__ bool enabled() const { return true; }
+  __device__ void operator()(const double *res) const {
+    double rr, bb;
+    proj_norms(res, ctl->bb[0], rr, bb);
+    const double bn = sqrt(bb);
+    const double rel = bn > 0 ? sqrt(rr) / bn : 0.0;
+    ctl->out = (force || !(rel > tol)) ? 1 : 0;
+  }
+};
 struct EpiActive {  // plain result, only while the inner loop runs
   FcgCtl *ctl;
   __device__ bool enabled() const { return ctl->active != 0; }
@@ -210,27 +226,6 @@ struct EpiK2 {  // count the iteration and decide whether the next one runs
     ctl->active = stop ? 0 : 1;
   }
 };
-
-// Initial guess extrapolated in time from the last stage-2 pressures h0
-// (newest), h1, h2, over the whole frame: order 1: 2 h0 - h1; order 2:
-// 3 h0 - 3 h1 + h2
-__global__ void k_guess(int64_t n, double *__restrict__ p, const double *__restrict__ h0,
-                        const double *__restrict__ h1, const double *__restrict__ h2, int order) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  p[e] = order == 2 ? 3.0 * (h0[e] - h1[e]) + h2[e] : 2.0 * h0[e] - h1[e];
-}
-
-// Guess from a pressure and a linearly extrapolated offset between the
-// interleaved first- and last-stage pressures: p = x0 + 2 (x1 - x2) - (x3 - x4).
-// x0 may alias p.
-__global__ void k_guess_offset(int64_t n, double *p, const double *x0, const double *__restrict__ x1,
-                               const double *__restrict__ x2, const double *__restrict__ x3,
-                               const double *__restrict__ x4) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  p[e] = x0[e] + 2.0 * (x1[e] - x2[e]) - (x3[e] - x4[e]);
-}
 
 // x += d (FP64 += FP32)
 __global__ __launch_bounds__(256) void k_xupd(Grid g, double *__restrict__ x, const float *__restrict__ d) {
@@ -1246,30 +1241,50 @@ __global__ __launch_bounds__(256) void k_scale(Grid g, const float *__restrict__
   }
 }
 
-// final pressure p = x/sqrt(D) with mirrored ghosts (9_grad.f90:268-274)
-__global__ void k_pout(Grid g, const double *__restrict__ x, const double *__restrict__ sD, double *__restrict__ p) {
+// final pressure p = x/sqrt(D) with mirrored ghosts (9_grad.f90:268-274);
+// with dl, also the change of p in FP32, dl = p_new - p_old.  With cond, only
+// when *cond is set (the residual just checked ends the solve), so that a
+// solve writes p once.
+__global__ void k_pout(Grid g, const double *__restrict__ x, const double *__restrict__ sD, double *__restrict__ p,
+                       float *__restrict__ dl, const int *cond) {
+  if (cond && !*cond) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y,
             k = blockIdx.z;
   if (i > g.nx + 1 || j > g.ny + 1 || k > g.nz + 1) return;
   auto m = [](int q, int n, bool per) { return q < 1 ? (per ? n : 1) : (q > n ? (per ? 1 : n) : q); };
   const int64_t s = g.idx(m(i, g.nx, g.ccx0 == 0), m(j, g.ny, g.ccy0 == 0), m(k, g.nz, false));
-  p[g.idx(i, j, k)] = x[s] / sD[s];
+  const int64_t e = g.idx(i, j, k);
+  const double pn = x[s] / sD[s];
+  if (dl) dl[e] = (float)(pn - p[e]);
+  p[e] = pn;
 }
 
-// initial iterate x = p sqrt(D) and b_s = q qs/sqrt(D) with the block sums of
-// b_s.b_s; q = -div u* formed here with k_pD's expression
+// initial guess p + sum_m w_m o_m from the pressure and n of its FP32
+// offsets (n = 0: the pressure itself)
+struct GuessOff {
+  const float *o[3];
+  double w[3];
+  int n;
+};
+
+// initial iterate x = p_guess sqrt(D) and b_s = q qs/sqrt(D) with the block
+// sums of b_s.b_s; q = -div u* formed here with k_pD's expression
 template <typename T>
-__global__ __launch_bounds__(256) void k_xbq(Grid g, const double *__restrict__ p, const T *__restrict__ u,
-                                             const T *__restrict__ v, const T *__restrict__ w,
-                                             const double *__restrict__ sD, double qs, double *__restrict__ x,
-                                             double *__restrict__ b, RedSlot rs) {
+__global__ __launch_bounds__(256) void k_xbq(Grid g, const double *__restrict__ p, GuessOff go,
+                                             const T *__restrict__ u, const T *__restrict__ v,
+                                             const T *__restrict__ w, const double *__restrict__ sD, double qs,
+                                             double *__restrict__ x, double *__restrict__ b, RedSlot rs) {
   CELL_IJK;
   double acc[1] = {0.0};
   if (i <= g.nx && j <= g.ny)
     for (int k = kz0; k <= g.nz; k += gridDim.z) {
       const fidx c = g.idx(i, j, k);
       const double sd = sD[c];
-      x[c] = p[c] * sd;
+      double pg = p[c];
+#pragma unroll
+      for (int m = 0; m < 3; m++)
+        if (m < go.n) pg += go.w[m] * (double)go.o[m][c];
+      x[c] = pg * sd;
       const double bv = div_q<T>(g, u, v, w, c) * qs / sd;
       b[c] = bv;
       acc[0] += bv * bv;
@@ -1689,37 +1704,52 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   const double tol = c.P.tol_rel;
   int pc = c.P.precond;
   // Initial guess.  The reference warm-starts every projection from the
-  // current pressure (9_grad.f90:121).  Here, once the pressures of the last
-  // steps (same step size) are known, a projection starts from an
-  // extrapolation in time.  The first- and last-stage pressures of the steps
-  // interleave (g = first stage, h = last stage, newest first: ..., g1, h1,
-  // g0, h0): a first-stage projection starts from h0 plus the offset between
-  // a first-stage pressure and the last-stage pressure before it, extrapolated
-  // linearly, h0 + 2 (g0 - h1) - (g1 - h2); a last-stage projection from the
-  // first-stage pressure just solved plus the offset between a last-stage
-  // pressure and its own step's first-stage one, extrapolated linearly,
-  // g + 2 (h0 - g0) - (h1 - g1).  With less history: the quadratic (linear)
-  // extrapolation of the last-stage pressures, or the first-stage pressure.
-  // On the dam break the offset forms take 6 and 4.75 FCG iterations at step
-  // 500 where the plain extrapolation takes 7 and 5.9 (3 h0 - 3 h1 + h2 for
-  // both) and the current pressure 8 and 7.8; suli_set_guess(0) restores the
+  // current pressure (9_grad.f90:121).  Here a projection of the first or
+  // last stage of a step starts from the current pressure plus its stage's
+  // offset extrapolated in time.  The first- and last-stage pressures of the
+  // steps interleave (g(n), h(n)), and each solve starts from the other
+  // kind's last pressure: a first-stage solve from h(n-1), a last-stage solve
+  // from g(n).  The offsets d(n) = g(n) - h(n-1) (first stage) and
+  // h(n) - g(n) (last stage) change slowly, and with the last two of the
+  // same kind known the solve starts from p + 2 d(n-1) - d(n-2) (from
+  // p + d(n-1) with one): for the first stage h(n-1) + 2 (g(n-1) - h(n-2)) -
+  // (g(n-2) - h(n-3)).  The offsets are kept in FP32 (rounded relative to
+  // the change they hold, not to p), written by the solve's output kernel,
+  // and the guess is formed inside the b_s pass.  On the dam break the
+  // offset forms took 6 and 4.75 FCG iterations at step 500 where a
+  // quadratic extrapolation of the last-stage pressures took 7 and 5.9 and
+  // the current pressure 8 and 7.8; suli_set_guess(0) restores the
   // reference's choice.
+#ifndef SULI_GUESS_OFFSETS
+#define SULI_GUESS_OFFSETS 2  // offsets extrapolated: 1 constant, 2 linear, 3 quadratic
+#endif
   const double base_dt = c.stage == 1 ? 2.0 * dt : dt;  // a_dt(1) = dt/2, a_dt(2) = dt
-  const bool hist = c.guess == 1 && (c.stage == 1 || c.stage == 2) && base_dt == c.p2dt;
-  const unsigned gb = (unsigned)((g.total + 255) / 256);
-  if (hist && c.stage == 1 && c.p2n >= 3 && c.p1n >= 2) {
-    hipLaunchKernelGGL(k_guess_offset, dim3(gb), dim3(256), 0, c.st, (int64_t)g.total, c.prd1,
-                       (const double *)c.p2h[0], (const double *)c.p1h[0], (const double *)c.p2h[1],
-                       (const double *)c.p1h[1], (const double *)c.p2h[2]);
-  } else if (hist && c.stage == 2 && c.p2n >= 2 && c.p1n >= 3) {  // p1h[0]: this step's first stage
-    hipLaunchKernelGGL(k_guess_offset, dim3(gb), dim3(256), 0, c.st, (int64_t)g.total, c.prd1,
-                       (const double *)c.prd1, (const double *)c.p2h[0], (const double *)c.p1h[1],
-                       (const double *)c.p2h[1], (const double *)c.p1h[2]);
-  } else if (hist && c.p2n >= 2 && (c.p2n >= 3 || c.stage == 1)) {
-    hipLaunchKernelGGL(k_guess, dim3(gb), dim3(256), 0, c.st, (int64_t)g.total,
-                       c.prd1, (const double *)c.p2h[0], (const double *)c.p2h[1], (const double *)c.p2h[2],
-                       c.p2n >= 3 ? 2 : 1);
+  const int sk = c.stage == 1 ? 0 : (c.stage == 2 ? 1 : -1);  // kind: first, last stage
+  if (sk >= 0 && base_dt != c.hdt) {
+    c.offn[0] = c.offn[1] = 0;
+    c.hdt = base_dt;
   }
+  // this solve's offset joins the history when it starts from the other kind's pressure
+  const bool chain = sk >= 0 && c.last == 2 - sk;
+  if (sk >= 0 && !chain) c.offn[sk] = 0;
+  GuessOff go{};
+  if (c.guess == 1 && chain) {
+    static const double wts[4][3] = {{0, 0, 0}, {1, 0, 0}, {2, -1, 0}, {3, -3, 1}};
+    go.n = std::min(c.offn[sk], SULI_GUESS_OFFSETS);
+    for (int m = 0; m < 3; m++) {
+      go.o[m] = c.off[sk][m];
+      go.w[m] = wts[go.n][m];
+    }
+  }
+  // the output kernel writes the offset into the oldest slot, which then
+  // leaves the history
+  float *dslot = nullptr;
+  if (chain) {
+    if (!c.off[sk][2] && falloc(c, &c.off[sk][2], g.total)) return 1;
+    dslot = c.off[sk][2];
+    c.offn[sk] = std::min(c.offn[sk], 2);
+  }
+  c.last = 0;  // until this solve has written its pressure
   // Operator.  c_f = dt/(h^2 rho_f), D = sum of c_f and the scaled faces
   // a_f = c_f/sqrt(D_P D_Q) depend on dt only through a common factor, so
   // while the density is unchanged (the stages of one step) the operator
@@ -1750,7 +1780,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       }
     }
     const double qs = reuse ? f.op_dt / dt : 1.0;
-    DISPATCH(c, hipLaunchKernelGGL((k_xbq<TT>), gk, b2, 0, c.st, g, (const double *)c.prd1, (const TT *)c.u,
+    DISPATCH(c, hipLaunchKernelGGL((k_xbq<TT>), gk, b2, 0, c.st, g, (const double *)c.prd1, go, (const TT *)c.u,
                                    (const TT *)c.v, (const TT *)c.w, (const double *)c.sD, qs, f.x, f.b, rb));
     launch_finish<1>(c.st, gk, rb);
   }
@@ -1804,19 +1834,20 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
       ProfScope ps(c, "fcg_residual64");
       const RedSlot rr{c.partials, CTL_PTR(f.ctl, rout)};
       hipLaunchKernelGGL(k_res64, gk, b2, 0, c.st, g, c.ax, c.ay, c.az, f.x, f.b, c.sD, f.r, rr);
-      launch_finish<3>(c.st, gk, rr);
+      launch_finish<3>(c.st, gk, rr, EpiOut{f.ctl, tol, outer >= max_outer ? 1 : 0});
     }
     const bool spec = speculate && outer < max_outer;
     if (!spec) {
       {
-        ProfScope ps(c, "poisson_final");  // in case this residual ends the solve
-        hipLaunchKernelGGL(k_pout, pg, dim3(64, 4, 1), 0, c.st, g, f.x, c.sD, c.prd1);
+        ProfScope ps(c, "poisson_final");  // written when this residual ends the solve
+        hipLaunchKernelGGL(k_pout, pg, dim3(64, 4, 1), 0, c.st, g, f.x, c.sD, c.prd1, dslot,
+                           (const int *)((char *)f.ctl + offsetof(FcgCtl, out)));
       }
       HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
       HIPCHK(hipStreamSynchronize(c.st));
       res = proj_res(f.ctl_h, bnorm);
       if (res0 < 0) res0 = res;
-      if (!(res > tol) || outer >= max_outer) {
+      if (f.ctl_h->out) {  // the device's verdict, the test above and outer >= max_outer
         conv = !(res > tol);
         pout_done = true;
         break;
@@ -1939,27 +1970,20 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   }
   if (!pout_done) {
     ProfScope ps(c, "poisson_final");
-    hipLaunchKernelGGL(k_pout, pg, dim3(64, 4, 1), 0, c.st, g, f.x, c.sD, c.prd1);
+    hipLaunchKernelGGL(k_pout, pg, dim3(64, 4, 1), 0, c.st, g, f.x, c.sD, c.prd1, dslot, (const int *)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c.st));
   }
   HIPCHK(hipGetLastError());
   double gpu_ms = 0.0;
   if (solve_elapsed_ms(c, &gpu_ms)) return 1;
-  if (c.stage == 1 || c.stage == 2) {  // this solution joins its stage's history
-    if (base_dt != c.p2dt) c.p2n = c.p1n = 0;
-    c.p2dt = base_dt;
-    double **hh = c.stage == 2 ? c.p2h : c.p1h;
-    int &hn = c.stage == 2 ? c.p2n : c.p1n;
-    for (int q = 0; q < 3; q++)
-      if (!hh[q] && dalloc64(c, &hh[q], g.total)) return 1;
-    double *oldest = hh[2];
-    hh[2] = hh[1];
-    hh[1] = hh[0];
-    hh[0] = oldest;
-    HIPCHK(hipMemcpyAsync(hh[0], c.prd1, (size_t)g.total * sizeof(double), hipMemcpyDeviceToDevice, c.st));
-    hn = std::min(hn + 1, 3);
+  if (dslot) {  // this solve's offset is the newest of its kind
+    c.off[sk][2] = c.off[sk][1];
+    c.off[sk][1] = c.off[sk][0];
+    c.off[sk][0] = dslot;
+    c.offn[sk] = std::min(c.offn[sk] + 1, 3);
   }
+  c.last = sk + 1;
   if (stt) {
     std::memset(stt, 0, sizeof(*stt));
     stt->iters = inner_total;
