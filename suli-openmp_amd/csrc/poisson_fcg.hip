@@ -277,10 +277,19 @@ __global__ __launch_bounds__(256) void k_fcg1(Grid g, const float *__restrict__ 
   block_partial<2>(acc, rs.partials);
 }
 
-// K2: alpha = p.r/p.q; d += alpha p; r -= alpha q; sums r.r, r.q; loop control
+// the finest multigrid level's input staged from r (k_mg0_in's values, cell
+// by cell): b = sqrt(D) r split by colour, red x = b/diag
+struct StageIn {
+  Grid h;  // pair frame of the colour-split arrays
+  const float *s, *dred;
+  float *bred, *bblack, *xred;  // bred null: no staging
+};
+
+// K2: alpha = p.r/p.q; d += alpha p; r -= alpha q; sums r.r, r.q; loop control;
+// with st.bred, the next V-cycle's input from the new r
 __global__ __launch_bounds__(256) void k_fcg2(Grid g, const float *__restrict__ p, const float *__restrict__ q,
                                               float *__restrict__ r, float *__restrict__ d, FcgCtl *ctl,
-                                              RedSlot rs) {
+                                              RedSlot rs, StageIn st) {
   if (!ctl->active) return;
   CELL_IJK;
   const double pq = ctl->k1[0];
@@ -295,6 +304,16 @@ __global__ __launch_bounds__(256) void k_fcg2(Grid g, const float *__restrict__ 
       r[c] = rv;
       acc[0] += (double)rv * (double)rv;
       acc[1] += (double)rv * (double)qv;
+      if (st.bred) {
+        const float bv = st.s[c] * rv;
+        const int64_t qh = st.h.idx((i - 1) >> 1, j, k);
+        if (((i + j + k) & 1) == 0) {
+          st.bred[qh] = bv;
+          st.xred[qh] = (bv + 0.f) / st.dred[qh];
+        } else {
+          st.bblack[qh] = bv;
+        }
+      }
     }
   __shared__ double lds[2 * SULI_LDS_RED];
   block_reduce<2>(acc, lds);
@@ -1476,7 +1495,7 @@ struct ZqOut {
 // levels above e then run their own launches even where they would fit the
 // single-work-group kernel).
 static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const ZqOut *zq = nullptr,
-                      int net_level = 0) {
+                      int net_level = 0, bool staged = false) {
   PoissonFCG &f = *c.fcg;
   const dim3 b2(SULI_BX, SULI_BY, 1);
   const int nu1 = SULI_MG_NU, nu2 = SULI_MG_NU;
@@ -1512,7 +1531,9 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
                         (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.b, lf.x, color, zero_x,
                         active);
   };
-  if (ff) {
+  if (ff && staged) {
+    // the finest level's input was staged from r by the kernel that produced r
+  } else if (ff) {
     ProfScope ps(c, "mg_gs_l0_in");
     hipLaunchKernelGGL(k_mg0_in, half_cells(c.g), b2, 0, c.st, c.g, f.hg, (const float *)f.d0c,
                        (const float *)f.sD32, r, f.b0c, f.b1c, f.x0c, active);
@@ -1681,7 +1702,7 @@ static void fcgm_iteration(Ctx &c, PoissonFCG &f, int slot, dim3 gk, const int *
   {
     ProfScope ps(c, "fcg_update");
     const RedSlot r2{c.partials, CTL_PTR(f.ctl, k2)};
-    hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.pm[slot], f.qm[slot], f.r32, f.d32, f.ctl, r2);
+    hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.pm[slot], f.qm[slot], f.r32, f.d32, f.ctl, r2, StageIn{});
     launch_finish<2>(c.st, gk, r2, EpiK2{f.ctl});
   }
 }
@@ -1855,6 +1876,10 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     }
     outer++;
     const bool use_pc = pc != SULI_PRECOND_JACOBI;
+    // the V-cycles of the pass after its first iteration take their finest
+    // level's input from K2, which has the new r cell by cell
+    const bool stage = use_pc && (pc != SULI_PRECOND_LEARNED || learned_m1) && c.mg_fuse_fine && f.x0c && f.b0c &&
+                       f.b1c && f.d0c;
     const int pass_max = pc == SULI_PRECOND_LEARNED ? std::min(kLearnedPassCap, maxit_in - inner_total)
                                                     : maxit_in - inner_total;
     {
@@ -1887,7 +1912,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
         if (use_pc) {
           const ZqOut zo{f.q32, (const int *)((char *)f.ctl + offsetof(FcgCtl, first)),
                          RedSlot{c.partials, CTL_PTR(f.ctl, zq)}};
-          mg_vcycle(c, f.r32, f.z32, active, &zo, pc == SULI_PRECOND_LEARNED ? entry : 0);
+          mg_vcycle(c, f.r32, f.z32, active, &zo, pc == SULI_PRECOND_LEARNED ? entry : 0, stage && iter > 0);
           zin = f.z32;
         }
         {
@@ -1903,7 +1928,8 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
         {
           ProfScope ps(c, "fcg_update");
           const RedSlot r2{c.partials, CTL_PTR(f.ctl, k2)};
-          hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.p32, f.q32, f.r32, f.d32, f.ctl, r2);
+          hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.p32, f.q32, f.r32, f.d32, f.ctl, r2,
+                             stage ? StageIn{f.hg, f.sD32, f.d0c, f.b0c, f.b1c, f.x0c} : StageIn{});
           launch_finish<2>(c.st, gk, r2, EpiK2{f.ctl});
         }
       }
