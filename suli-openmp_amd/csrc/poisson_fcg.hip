@@ -115,10 +115,31 @@ __global__ __launch_bounds__(256) void k_res64(Grid g, const double *__restrict_
   block_partial<3>(acc, rs.partials);
 }
 
-// r32 = r - (r.n/n.n) n; d = 0; sums r32.r32 and arms the inner loop
+// the finest multigrid level's input staged from r (k_mg0_in's values, cell
+// by cell): b = sqrt(D) r split by colour, red x = b/diag
+struct StageIn {
+  Grid h;  // pair frame of the colour-split arrays
+  const float *s, *dred;
+  float *bred, *bblack, *xred;  // bred null: no staging
+};
+
+__device__ __forceinline__ void stage_in(const StageIn &st, const Grid &g, int i, int j, int k, int64_t c,
+                                         float rv) {
+  const float bv = st.s[c] * rv;
+  const int64_t qh = st.h.idx((i - 1) >> 1, j, k);
+  if (((i + j + k) & 1) == 0) {
+    st.bred[qh] = bv;
+    st.xred[qh] = (bv + 0.f) / st.dred[qh];
+  } else {
+    st.bblack[qh] = bv;
+  }
+}
+
+// r32 = r - (r.n/n.n) n; d = 0; sums r32.r32 and arms the inner loop; with
+// st.bred, the first V-cycle's input from r32
 __global__ __launch_bounds__(256) void k_r32(Grid g, const double *__restrict__ r, const double *__restrict__ sD,
                                              float *__restrict__ r32, float *__restrict__ d32, FcgCtl *ctl,
-                                             double rel_in, int maxit, RedSlot rs) {
+                                             double rel_in, int maxit, RedSlot rs, StageIn st) {
   CELL_IJK;
   const double mu = ctl->rout[1] / ctl->rout[2];
   double acc[1] = {0.0};
@@ -129,6 +150,7 @@ __global__ __launch_bounds__(256) void k_r32(Grid g, const double *__restrict__ 
       r32[c] = rv;
       d32[c] = 0.f;
       acc[0] += (double)rv * (double)rv;
+      if (st.bred) stage_in(st, g, i, j, k, c, rv);
     }
   __shared__ double lds[SULI_LDS_RED];
   block_reduce<1>(acc, lds);
@@ -277,14 +299,6 @@ __global__ __launch_bounds__(256) void k_fcg1(Grid g, const float *__restrict__ 
   block_partial<2>(acc, rs.partials);
 }
 
-// the finest multigrid level's input staged from r (k_mg0_in's values, cell
-// by cell): b = sqrt(D) r split by colour, red x = b/diag
-struct StageIn {
-  Grid h;  // pair frame of the colour-split arrays
-  const float *s, *dred;
-  float *bred, *bblack, *xred;  // bred null: no staging
-};
-
 // K2: alpha = p.r/p.q; d += alpha p; r -= alpha q; sums r.r, r.q; loop control;
 // with st.bred, the next V-cycle's input from the new r
 __global__ __launch_bounds__(256) void k_fcg2(Grid g, const float *__restrict__ p, const float *__restrict__ q,
@@ -304,16 +318,7 @@ __global__ __launch_bounds__(256) void k_fcg2(Grid g, const float *__restrict__ 
       r[c] = rv;
       acc[0] += (double)rv * (double)rv;
       acc[1] += (double)rv * (double)qv;
-      if (st.bred) {
-        const float bv = st.s[c] * rv;
-        const int64_t qh = st.h.idx((i - 1) >> 1, j, k);
-        if (((i + j + k) & 1) == 0) {
-          st.bred[qh] = bv;
-          st.xred[qh] = (bv + 0.f) / st.dred[qh];
-        } else {
-          st.bblack[qh] = bv;
-        }
-      }
+      if (st.bred) stage_in(st, g, i, j, k, c, rv);
     }
   __shared__ double lds[2 * SULI_LDS_RED];
   block_reduce<2>(acc, lds);
@@ -1876,16 +1881,18 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     }
     outer++;
     const bool use_pc = pc != SULI_PRECOND_JACOBI;
-    // the V-cycles of the pass after its first iteration take their finest
-    // level's input from K2, which has the new r cell by cell
+    // the V-cycles of the pass take their finest level's input from the
+    // kernel that wrote r: the first from the pass's projection, the others
+    // from K2
     const bool stage = use_pc && (pc != SULI_PRECOND_LEARNED || learned_m1) && c.mg_fuse_fine && f.x0c && f.b0c &&
                        f.b1c && f.d0c;
+    const StageIn sti = stage ? StageIn{f.hg, f.sD32, f.d0c, f.b0c, f.b1c, f.x0c} : StageIn{};
     const int pass_max = pc == SULI_PRECOND_LEARNED ? std::min(kLearnedPassCap, maxit_in - inner_total)
                                                     : maxit_in - inner_total;
     {
       ProfScope ps(c, "fcg_project");
       const RedSlot r3{c.partials, CTL_PTR(f.ctl, rin0)};
-      hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rel_in, pass_max, r3);
+      hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rel_in, pass_max, r3, sti);
       const bool capped = pc == SULI_PRECOND_LEARNED;
       launch_finish<1>(c.st, gk, r3,
                        EpiR32{f.ctl, rel_in, tol, pass_max,
@@ -1912,7 +1919,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
         if (use_pc) {
           const ZqOut zo{f.q32, (const int *)((char *)f.ctl + offsetof(FcgCtl, first)),
                          RedSlot{c.partials, CTL_PTR(f.ctl, zq)}};
-          mg_vcycle(c, f.r32, f.z32, active, &zo, pc == SULI_PRECOND_LEARNED ? entry : 0, stage && iter > 0);
+          mg_vcycle(c, f.r32, f.z32, active, &zo, pc == SULI_PRECOND_LEARNED ? entry : 0, stage);
           zin = f.z32;
         }
         {
@@ -1928,8 +1935,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
         {
           ProfScope ps(c, "fcg_update");
           const RedSlot r2{c.partials, CTL_PTR(f.ctl, k2)};
-          hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.p32, f.q32, f.r32, f.d32, f.ctl, r2,
-                             stage ? StageIn{f.hg, f.sD32, f.d0c, f.b0c, f.b1c, f.x0c} : StageIn{});
+          hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.p32, f.q32, f.r32, f.d32, f.ctl, r2, sti);
           launch_finish<2>(c.st, gk, r2, EpiK2{f.ctl});
         }
       }
