@@ -659,6 +659,7 @@ static MomPtrs mom_ptrs(Ctx &c) {
   m.xnf = dir ? c.xnf : nullptr; m.ynf = dir ? c.ynf : nullptr; m.znf = dir ? c.znf : nullptr;
   m.adv = c.adv;
   m.ku = c.obst ? c.ku_d : nullptr; m.kv = c.obst ? c.kv_d : nullptr; m.kw = c.obst ? c.kw_d : nullptr;
+  m.tiles = c.mom_tiles;
   return m;
 }
 
@@ -872,6 +873,12 @@ extern "C" int suli_set_mg(int64_t fuse_max_cells, int coarse_sweeps, int fuse_f
   if (fuse_max_cells >= 0) g_ctx->mg_fuse_max = fuse_max_cells;
   if (coarse_sweeps >= 0) g_ctx->mg_coarse_sweeps = coarse_sweeps;
   if (fuse_fine >= 0) g_ctx->mg_fuse_fine = fuse_fine ? 1 : 0;
+  return 0;
+}
+
+extern "C" int suli_set_momentum(int lds_tiles) {
+  NEEDCTX();
+  if (lds_tiles >= 0) g_ctx->mom_tiles = lds_tiles ? 1 : 0;
   return 0;
 }
 

@@ -530,9 +530,10 @@ __global__ __launch_bounds__(256) void k_visco_dir(Grid g, T cs, const T *__rest
   }
 }
 
+// comp: the face kind to update (0 u, 1 v, 2 w) or -1 for all three
 template <typename T, int ADV, bool FACES, bool INX = false, bool INYZ = false>
 __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, const MomArgs<T> &A, int i, int j,
-                                              int k) {
+                                              int k, int comp = -1) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const fidx c = g.idx(i, j, k);
   const fidx X = 1, Y = g.sx, Z = g.sz;
@@ -546,7 +547,7 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
   const T fr = T(ph.gz / (ph.chezy * ph.chezy));
   const T sigma = T(ph.sigma);
   // ---- u faces: i in [1,nx+1], j in [1,ny], k in [1,nz]
-  if (i <= nx + 2 && j <= ny + 1 && k <= nz + 1) {
+  if ((comp < 0 || comp == 0) && i <= nx + 2 && j <= ny + 1 && k <= nz + 1) {
     if (i >= 1 && i <= nx + 1 && j >= 1 && j <= ny && k >= 1 && k <= nz) {
       const T uc = u[c];
       const T uint_ = adv_u<T, ADV>(g, u, v, w, c, i, j, k, dx, dy, dz);
@@ -579,7 +580,7 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
     }
   }
   // ---- v faces: i in [1,nx], j in [1,ny+1], k in [1,nz]
-  if (i <= nx + 1 && j <= ny + 2 && k <= nz + 1) {
+  if ((comp < 0 || comp == 1) && i <= nx + 1 && j <= ny + 2 && k <= nz + 1) {
     if (i >= 1 && i <= nx && j >= 1 && j <= ny + 1 && k >= 1 && k <= nz) {
       const T vc = v[c];
       const T vint_ = adv_v<T, ADV>(g, u, v, w, c, i, j, k, dx, dy, dz);
@@ -612,7 +613,7 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
     }
   }
   // ---- w faces: i in [1,nx], j in [1,ny], k in [1,nz+1]
-  if (i <= nx + 1 && j <= ny + 1 && k <= nz + 2) {
+  if ((comp < 0 || comp == 2) && i <= nx + 1 && j <= ny + 1 && k <= nz + 2) {
     if (i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz + 1) {
       const T wc = w[c];
       const T wint_ = adv_w<T, ADV>(g, u, v, w, c, i, j, k, dx, dy, dz);
@@ -651,35 +652,44 @@ __device__ __forceinline__ void momentum_cell(const Grid &g, const Phys &ph, con
 // waves_per_eu(6): 80 VGPRs instead of 82 lifts the occupancy from 5 to 6
 // waves per SIMD without spills (8 would spill)
 //
-// Two launches per stage: the rows of the inner planes and rows,
-// 2 <= j <= ny-1, 2 <= k <= nz-1, over the whole i range (k_momentum_in),
-// where none of the end-face and end-cell clamps along j and k acts, so
-// they are compiled out (the repeated neighbour loads of the viscosity
-// edges then share addresses the compiler can merge); and the remaining
-// rows of the frame (k_momentum), which hold every clamp.  The same values
-// are computed either way.
+// Two launches per stage: the inner box 1 <= i <= nx, 2 <= j <= ny-1,
+// 2 <= k <= nz-1 (k_momentum_tile for the default advection form, else
+// k_momentum_in), where none of the end-face and end-cell clamps along j
+// and k acts, so they are compiled out; and the shell of the frame around
+// it (k_momentum), which holds every clamp.  The same values are computed
+// either way.
 //
-// Remaining rows in thread order: the planes k in {0, 1, nz, nz+1, nz+2} of
-// the frame, then for 2 <= k <= nz-1 the rows j in {0, 1, ny, ny+1, ny+2}.
+// Shell cells in thread order: the planes k in {0, 1, nz, nz+1, nz+2} of
+// the frame; for 2 <= k <= nz-1 the rows j in {0, 1, ny, ny+1, ny+2}; for
+// 2 <= k <= nz-1, 2 <= j <= ny-1 the cells i in {0, nx+1, nx+2}.
 __host__ __device__ __forceinline__ int64_t mom_shell_cells(const Grid &g) {
   const int64_t W = g.nx + 3, H = g.ny + 3;
-  return 5 * W * H + (int64_t)(g.nz - 2) * 5 * W;
+  const int64_t kin = g.nz > 2 ? g.nz - 2 : 0, jin = g.ny > 2 ? g.ny - 2 : 0;
+  return 5 * W * H + kin * 5 * W + kin * jin * 3;
 }
 __device__ __forceinline__ void mom_shell_cell(const Grid &g, int t, int &i, int &j, int &k) {
-  const int W = g.nx + 3, H = g.ny + 3, nZ = 5 * W * H;
+  const int W = g.nx + 3, H = g.ny + 3, nZ = 5 * W * H, nY = (g.nz > 2 ? g.nz - 2 : 0) * 5 * W;
   const auto side = [](int q, int n) { return q < 2 ? q : n + (q - 2); };  // 0, 1, n, n+1, n+2
   if (t < nZ) {
     k = side(t / (W * H), g.nz);
     const int r = t % (W * H);
     j = r / W;
     i = r % W;
-  } else {
+  } else if (t < nZ + nY) {
     const int t2 = t - nZ, r = t2 % (5 * W);
     k = 2 + t2 / (5 * W);
     j = side(r / W, g.ny);
     i = r % W;
+  } else {
+    const int t3 = t - nZ - nY, jin = g.ny - 2, r = t3 % (3 * jin);
+    k = 2 + t3 / (3 * jin);
+    j = 2 + r / 3;
+    i = r % 3 == 0 ? 0 : g.nx + r % 3;  // 0, nx+1, nx+2
   }
 }
+// the shell: one face kind per block row (blockIdx.y = 0 u, 1 v, 2 w), so
+// that each thread walks one of the three stencils (the shell is a few
+// hundred thousand cells: latency, not throughput, sets its time)
 template <typename T, int ADV, bool FACES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_momentum(Grid g, Phys ph,
                                                                                           MomArgs<T> A) {
@@ -687,14 +697,303 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   if (t >= mom_shell_cells(g)) return;
   int i, j, k;
   mom_shell_cell(g, t, i, j, k);
-  momentum_cell<T, ADV, FACES>(g, ph, A, i, j, k);
+  momentum_cell<T, ADV, FACES>(g, ph, A, i, j, k, (int)blockIdx.y);
 }
 template <typename T, int ADV, bool FACES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_momentum_in(Grid g, Phys ph,
                                                                                              MomArgs<T> A) {
-  const int i = blockIdx.x * SULI_BX + threadIdx.x, j = 2 + blockIdx.y * SULI_BY + threadIdx.y;
-  if (i > g.nx + 2 || j > g.ny - 1) return;
+  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 2 + blockIdx.y * SULI_BY + threadIdx.y;
+  if (i > g.nx || j > g.ny - 1) return;
   momentum_cell<T, ADV, FACES, false, true>(g, ph, A, i, j, 2 + blockIdx.z);
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tiled momentum predictor of the inner box 1 <= i <= nx,
+// 2 <= j <= ny-1, 2 <= k <= nz-1 for the default advection form (classico
+// upwind, ADV = 11) with the viscosities rebuilt from the cell arrays: the
+// same values as momentum_cell<T, 11, false, false, true>, term for term in
+// the same order.  The end-cell clamps along x of that function act only at
+// i = 1 and i = nx, on the face interpolations of nut, ls_nu, rho and the
+// surface terms: there the pair of cells (1,0) becomes (2,1) and (nx+1,nx)
+// becomes (nx,nx-1).  The tile therefore holds those four fields of column
+// i = 0 mirrored from column 2 and of column nx+1 from column nx-1: the
+// unclamped expression then adds the same two values in the other order,
+// which IEEE addition rounds identically.  (The one plain read of cell i-1,
+// nut at i = 1, is clamped to cell i through the lane's LDS offset; a
+// select there would change how the FP32 instantiation contracts the
+// expression into FMAs.)
+//
+// A work-group of 64 x MT_TJ threads owns a tile of 64 i by MT_TJ j and
+// walks a chunk of k planes.  Every input value is read from HBM once per
+// tile (plus its one-cell halo) instead of once per stencil use:
+// - ring: u, v, w, nut, ls_nu of planes k-1, k, k+1 in three LDS slots, the
+//   slot of plane k-1 refilled with plane k+2 after plane k is done;
+// - plane k of rho and of the surface-tension cell terms
+//   hs_x kurv/rho, hs_y kurv/rho, hs_z kurv/rho (their -x, -y neighbours);
+//   the -z neighbour of rho and hs_z kurv/rho is the thread's own value of
+//   the previous plane, kept in registers.
+// The loads of the next planes are issued before the arithmetic of the
+// current one and land in LDS after it, so their latency hides behind it.
+#ifndef MT_TJ
+#define MT_TJ 4
+#endif
+template <typename T>
+struct MomTile {
+  static constexpr int LW = SULI_BX + 2;            // row of the tile with its i-1 and i+64 halo
+  static constexpr int LH = MT_TJ + 2;              // rows with the j-1 and j+MT_TJ halo
+  static constexpr int HP = LW * LH;                // one plane of one field
+  static constexpr int HS = HP + 1;                 // its stride in LDS: a spare slot for idle lanes
+  static constexpr int NT = SULI_BX * MT_TJ;        // threads
+  static constexpr int NL = (HP + NT - 1) / NT;     // plane elements per thread
+  static constexpr int NR = 5;                      // ring fields: u, v, w, nut, ls_nu
+  static constexpr int NS = 4;                      // plane fields: rho, sx, sy, sz
+};
+
+// MODE: the stage update as a compile-time MomUpdate (MOM_RK_FIRST and
+// MOM_FROM_U0, the two RK2 stages), or 0 for A.mode at run time.
+template <typename T, int MODE>
+__global__ __launch_bounds__(SULI_BX *MT_TJ) __attribute__((amdgpu_waves_per_eu(4))) void k_momentum_tile(
+    Grid g, Phys ph, MomArgs<T> A, int kc) {
+  using M = MomTile<T>;
+  constexpr int LW = M::LW, HP = M::HP, HS = M::HS, NL = M::NL, NR = M::NR, NS = M::NS, NT = M::NT;
+  __shared__ T ring[3][NR][HS];
+  __shared__ T pk[NS][HS];
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + SULI_BX * ty;
+  const int i0 = 1 + blockIdx.x * SULI_BX, j0 = 2 + blockIdx.y * MT_TJ, kb = 2 + blockIdx.z * kc;
+  const int ke = min(kb + kc, nz);  // planes kb..ke-1 computed, kb-1..ke read
+  const int i = i0 + tx, j = j0 + ty;
+  const bool act = i <= nx && j <= ny - 1;
+  const fidx Z = g.sz;
+  const Fld<T> ringsrc[NR] = {{A.u}, {A.v}, {A.w}, {A.nut}, {A.lsnu}};
+  const Fld<T> plsrc[5] = {{A.rho}, {A.hsx}, {A.hsy}, {A.hsz}, {A.kurv}};
+
+  // the plane elements this thread loads: frame offset in plane 0 (clamped
+  // into the frame: the values of positions outside it are never used) and
+  // LDS position (the spare slot HP for lanes past the plane)
+  // and, for nut, ls_nu and the plane fields, with columns 0 and nx+1
+  // mirrored from 2 and nx-1
+  fidx goff[NL], gofm[NL];
+  int lpos[NL];
+#pragma unroll
+  for (int q = 0; q < NL; q++) {
+    const int e = tid + q * NT;
+    const int gi = min(i0 - 1 + e % LW, nx + 2), gj = min(j0 - 1 + e / LW, ny + 2);
+    const int gm = gi == 0 ? 2 : (gi == nx + 1 ? nx - 1 : gi);
+    goff[q] = g.idx(gi, gj, 0);
+    gofm[q] = g.idx(gm, gj, 0);
+    lpos[q] = e < HP ? e : HP;
+  }
+  T rr[NR][NL], rs[5][NL];  // in flight: ring fields of one plane, plane fields of another
+  auto load_ring = [&](int kk) {
+#pragma unroll
+    for (int f = 0; f < NR; f++)
+#pragma unroll
+      for (int q = 0; q < NL; q++) rr[f][q] = ringsrc[f][(f < 3 ? goff[q] : gofm[q]) + (fidx)kk * Z];
+  };
+  auto store_ring = [&](int slot) {
+#pragma unroll
+    for (int f = 0; f < NR; f++)
+#pragma unroll
+      for (int q = 0; q < NL; q++) ring[slot][f][lpos[q]] = rr[f][q];
+  };
+  auto load_plane = [&](int kk) {
+#pragma unroll
+    for (int f = 0; f < 5; f++)
+#pragma unroll
+      for (int q = 0; q < NL; q++) rs[f][q] = plsrc[f][gofm[q] + (fidx)kk * Z];
+  };
+  auto store_plane = [&]() {
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+      const T r = rs[0][q];
+      pk[0][lpos[q]] = r;
+      // surf_cell: hs_d * kurv / rho
+      pk[1][lpos[q]] = rs[1][q] * rs[4][q] / r;
+      pk[2][lpos[q]] = rs[2][q] * rs[4][q] / r;
+      pk[3][lpos[q]] = rs[3][q] * rs[4][q] / r;
+    }
+  };
+
+  // prologue: planes kb-1, kb, kb+1 in slots 0, 1, 2; plane kb's cell terms
+  load_ring(kb - 1);
+  store_ring(0);
+  load_ring(kb);
+  store_ring(1);
+  load_ring(kb + 1);
+  store_ring(2);
+  load_plane(kb);
+  store_plane();
+  const fidx c00 = g.idx(min(i, nx), min(j, ny), 0);  // (idle lanes: a cell inside the frame)
+  T rho_m, sz_m;  // rho and hs_z kurv/rho of the thread's cell in the plane below
+  {
+    const fidx cm = c00 + (fidx)(kb - 1) * Z;
+    rho_m = A.rho[cm];
+    sz_m = A.hsz[cm] * A.kurv[cm] / rho_m;
+  }
+  __syncthreads();
+
+  const T dx = T(g.dx), dy = T(g.dy), dz = T(g.dz);
+  const T fr = T(ph.gz / (ph.chezy * ph.chezy));
+  const T sigma = T(ph.sigma);
+  const int l = (ty + 1) * LW + (tx + 1);  // the thread's cell in a tile plane
+  const int am = i == 1 ? 0 : -1;          // x offset of nut's cell i-1, clamped to 1
+  const int cp = i + g.sx * j;             // its column in the obstacle planes
+  for (int k = kb; k < ke; k++) {
+    const int n = k - kb;
+    const fidx c = c00 + (fidx)k * Z;
+    // the thread's own inputs from memory first, then the next planes
+    T u0c = T(0), v0c = T(0), w0c = T(0);
+    if (MODE == MOM_FROM_U0) { u0c = A.u0[c]; v0c = A.v0[c]; w0c = A.w0[c]; }
+    const T tfu = A.tfu ? A.tfu[c] : T(0), tfv = A.tfv ? A.tfv[c] : T(0), tfw = A.tfw ? A.tfw[c] : T(0);
+    const int kuc = A.ku && act ? A.ku[cp] : -9, kvc = A.kv && act ? A.kv[cp] : -9,
+              kwc = A.kw && act ? A.kw[cp] : -9;
+    if (k + 2 <= ke) load_ring(k + 2);
+    if (k + 1 < ke) load_plane(k + 1);
+    const T *Pm = &ring[n % 3][0][l], *P0 = &ring[(n + 1) % 3][0][l], *Pp = &ring[(n + 2) % 3][0][l];
+    // ring field f at (i+a, j+b, k+d)
+    auto R = [&](int f, int a, int b, int d) -> T {
+      const T *B = d < 0 ? Pm : (d > 0 ? Pp : P0);
+      return B[f * HS + b * LW + a];
+    };
+    auto u = [&](int a, int b, int d) { return R(0, a, b, d); };
+    auto v = [&](int a, int b, int d) { return R(1, a, b, d); };
+    auto w = [&](int a, int b, int d) { return R(2, a, b, d); };
+    auto nu = [&](int a, int b, int d) { return R(3, a, b, d); };
+    auto ln = [&](int a, int b, int d) { return R(4, a, b, d); };
+    auto P = [&](int f, int a, int b) { return pk[f][l + b * LW + a]; };
+    // face viscosity (ls_nu + nut interpolated to the face) of the face of
+    // cell (a,b,d) along x, y, z: face_visc
+    auto fvx = [&](int a, int b, int d) {
+      return (ln(a, b, d) + ln(a - 1, b, d)) * T(0.5) + (nu(a, b, d) + nu(a - 1, b, d)) * T(0.5);
+    };
+    auto fvy = [&](int a, int b, int d) {
+      return (ln(a, b, d) + ln(a, b - 1, d)) * T(0.5) + (nu(a, b, d) + nu(a, b - 1, d)) * T(0.5);
+    };
+    auto fvz = [&](int a, int b, int d) {
+      return (ln(a, b, d) + ln(a, b, d - 1)) * T(0.5) + (nu(a, b, d) + nu(a, b, d - 1)) * T(0.5);
+    };
+    auto update = [&](T uc, T F, T u0v, const T *u0, T *u0o, T *un, T *fa, T *fb) {
+      if (MODE == MOM_RK_FIRST) {
+        u0o[c] = uc;
+        un[c] = uc + A.dt * F;
+      } else if (MODE == MOM_FROM_U0) {
+        un[c] = u0v + A.dt * F;
+      } else {
+        stage_update<T>(A.mode, A.dt, c, uc, F, u0, u0o, un, fa, fb);
+      }
+    };
+    const T rho0 = P(0, 0, 0), sz0 = P(3, 0, 0);
+    const bool pl = (g.ccz0 == 2 && k == 2) || (g.ccy0 == 2 && j == 2) || (g.ccyf == 2 && j == ny);
+    if (act) {
+      // ---- u face (momentum_cell, adv_u with ADV = 11)
+      {
+        const T uc = u(0, 0, 0);
+        const T bma_a = (v(0, 0, 0) + v(-1, 0, 0) + v(0, 1, 0) + v(-1, 1, 0)) * T(0.25);
+        const T dma_a = (w(0, 0, 0) + w(-1, 0, 0) + w(0, 0, 1) + w(-1, 0, 1)) * T(0.25);
+        T a1 = tmax(uc, T(0)), a2 = tmin(uc, T(0));
+        const T dudx = a1 * (uc - u(-1, 0, 0)) / dx + a2 * (u(1, 0, 0) - uc) / dx;
+        a1 = tmax(bma_a, T(0)); a2 = tmin(bma_a, T(0));
+        const T dudy = a1 * (uc - u(0, -1, 0)) / dy + a2 * (u(0, 1, 0) - uc) / dy;
+        a1 = tmax(dma_a, T(0)); a2 = tmin(dma_a, T(0));
+        const T dudz = a1 * (uc - u(0, 0, -1)) / dz + a2 * (u(0, 0, 1) - uc) / dz;
+        const T uint_ = dudx + dudy + dudz;
+        const T amai = nu(0, 0, 0) / dx;
+        const T amam = nu(am, 0, 0) / dx;  // cell_clamp of cell i-1
+        const T bmap = ((fvx(0, 1, 0) + fvx(0, 0, 0)) * T(0.5)) / dy;
+        const T bmac = ((fvx(0, 0, 0) + fvx(0, -1, 0)) * T(0.5)) / dy;
+        const T dmap = ((fvx(0, 0, 1) + fvx(0, 0, 0)) * T(0.5)) / dz;
+        const T dmac = ((fvx(0, 0, 0) + fvx(0, 0, -1)) * T(0.5)) / dz;
+        const T rhox = (rho0 + P(0, -1, 0)) * T(0.5);
+        const T dhsdx = (P(1, 0, 0) + P(1, -1, 0)) * T(0.5);
+        const bool plu = A.ku ? k == kuc + 1 : pl;
+        const T ub = plu ? uc : T(0);
+        const T F = -uint_ +
+                    ((amai * T(2.) * (u(1, 0, 0) - uc) - amam * T(2.) * (uc - u(-1, 0, 0))) / dx +
+                     bmap * ((u(0, 1, 0) - uc) / dy + (v(0, 1, 0) - v(-1, 1, 0)) / dx) -
+                     bmac * ((uc - u(0, -1, 0)) / dy + (v(0, 0, 0) - v(-1, 0, 0)) / dx) +
+                     dmap * ((u(0, 0, 1) - uc) / dz + (w(0, 0, 1) - w(-1, 0, 1)) / dx) -
+                     dmac * ((uc - u(0, 0, -1)) / dz + (w(0, 0, 0) - w(-1, 0, 0)) / dx)) /
+                        rhox +
+                    T(ph.gx) - sigma * dhsdx + tfu - fr * sqrt(ub * ub) * uc;
+        if (A.Fu) A.Fu[c] = F;
+        update(uc, F, u0c, A.u0, A.u0o, A.un, A.fu0, A.fu1);
+      }
+      // ---- v face (adv_v with ADV = 11)
+      {
+        const T vc = v(0, 0, 0);
+        const T amb_a = (u(0, 0, 0) + u(1, 0, 0) + u(0, -1, 0) + u(1, -1, 0)) * T(0.25);
+        const T dmb_a = (w(0, 0, 0) + w(0, -1, 0) + w(0, 0, 1) + w(0, -1, 1)) * T(0.25);
+        T a1 = tmax(amb_a, T(0)), a2 = tmin(amb_a, T(0));
+        const T dvdx = a1 * (vc - v(-1, 0, 0)) / dx + a2 * (v(1, 0, 0) - vc) / dx;
+        a1 = tmax(vc, T(0)); a2 = tmin(vc, T(0));
+        const T dvdy = a1 * (vc - v(0, -1, 0)) / dy + a2 * (v(0, 1, 0) - vc) / dy;
+        a1 = tmax(dmb_a, T(0)); a2 = tmin(dmb_a, T(0));
+        const T dvdz = a1 * (vc - v(0, 0, -1)) / dz + a2 * (v(0, 0, 1) - vc) / dz;
+        const T vint_ = dvdx + dvdy + dvdz;
+        const T ambp = ((fvy(1, 0, 0) + fvy(0, 0, 0)) * T(0.5)) / dx;
+        const T ambc = ((fvy(0, 0, 0) + fvy(-1, 0, 0)) * T(0.5)) / dx;
+        const T bmbj = nu(0, 0, 0) / dy;
+        const T bmbm = nu(0, -1, 0) / dy;
+        const T dmbp = ((fvy(0, 0, 1) + fvy(0, 0, 0)) * T(0.5)) / dz;
+        const T dmbc = ((fvy(0, 0, 0) + fvy(0, 0, -1)) * T(0.5)) / dz;
+        const T rhoy = (rho0 + P(0, 0, -1)) * T(0.5);
+        const T dhsdy = (P(2, 0, 0) + P(2, 0, -1)) * T(0.5);
+        const bool plv = A.kv ? k == kvc + 1 : pl;
+        const T vb = plv ? vc : T(0);
+        const T F = -vint_ +
+                    (ambp * ((v(1, 0, 0) - vc) / dx + (u(1, 0, 0) - u(1, -1, 0)) / dy) -
+                     ambc * ((vc - v(-1, 0, 0)) / dx + (u(0, 0, 0) - u(0, -1, 0)) / dy) +
+                     (bmbj * T(2.) * (v(0, 1, 0) - vc) - bmbm * T(2.) * (vc - v(0, -1, 0))) / dy +
+                     dmbp * ((v(0, 0, 1) - vc) / dz + (w(0, 0, 1) - w(0, -1, 1)) / dy) -
+                     dmbc * ((vc - v(0, 0, -1)) / dz + (w(0, 0, 0) - w(0, -1, 0)) / dy)) /
+                        rhoy -
+                    sigma * dhsdy + tfv - fr * sqrt(vb * vb) * vc;
+        if (A.Fv) A.Fv[c] = F;
+        update(vc, F, v0c, A.v0, A.v0o, A.vn, A.fv0, A.fv1);
+      }
+      // ---- w face (adv_w with ADV = 11)
+      {
+        const T wc = w(0, 0, 0);
+        const T amd_a = (u(0, 0, 0) + u(1, 0, 0) + u(0, 0, -1) + u(1, 0, -1)) * T(0.25);
+        const T bmd_a = (v(0, 0, 0) + v(0, 1, 0) + v(0, 0, -1) + v(0, 1, -1)) * T(0.25);
+        T a1 = tmax(amd_a, T(0)), a2 = tmin(amd_a, T(0));
+        const T dwdx = a1 * (wc - w(-1, 0, 0)) / dx + a2 * (w(1, 0, 0) - wc) / dx;
+        a1 = tmax(bmd_a, T(0)); a2 = tmin(bmd_a, T(0));
+        const T dwdy = a1 * (wc - w(0, -1, 0)) / dy + a2 * (w(0, 1, 0) - wc) / dy;
+        a1 = tmax(wc, T(0)); a2 = tmin(wc, T(0));
+        const T dwdz = a1 * (wc - w(0, 0, -1)) / dz + a2 * (w(0, 0, 1) - wc) / dz;
+        const T wint_ = dwdx + dwdy + dwdz;
+        const T amdp = ((fvz(1, 0, 0) + fvz(0, 0, 0)) * T(0.5)) / dx;
+        const T amdc = ((fvz(0, 0, 0) + fvz(-1, 0, 0)) * T(0.5)) / dx;
+        const T bmdp = ((fvz(0, 1, 0) + fvz(0, 0, 0)) * T(0.5)) / dy;
+        const T bmdc = ((fvz(0, 0, 0) + fvz(0, -1, 0)) * T(0.5)) / dy;
+        const T dmdk = nu(0, 0, 0) / dz;
+        const T dmdm = nu(0, 0, -1) / dz;
+        const T rhoz = (rho0 + rho_m) * T(0.5);
+        const T dhsdz = (sz0 + sz_m) * T(0.5);
+        const bool plw = A.kw ? k == kwc + 1 : pl;
+        const T wb = plw ? wc : T(0);
+        const T epis_z = T(0);
+        const T F = -wint_ +
+                    (amdp * ((w(1, 0, 0) - wc) / dx + (u(1, 0, 0) - u(1, 0, -1)) / dz) -
+                     amdc * ((wc - w(-1, 0, 0)) / dx + (u(0, 0, 0) - u(0, 0, -1)) / dz) +
+                     bmdp * ((w(0, 1, 0) - wc) / dy + (v(0, 1, 0) - v(0, 1, -1)) / dz) -
+                     bmdc * ((wc - w(0, -1, 0)) / dy + (v(0, 0, 0) - v(0, 0, -1)) / dz) +
+                     (dmdk * T(2.) * (w(0, 0, 1) - wc) - dmdm * T(2.) * (wc - w(0, 0, -1))) / dz) /
+                        rhoz -
+                    (wc - T(0.)) * epis_z - T(ph.gz) - sigma * dhsdz + tfw - fr * sqrt(wb * wb) * wc;
+        if (A.Fw) A.Fw[c] = F;
+        update(wc, F, w0c, A.w0, A.w0o, A.wn, A.fw0, A.fw1);
+      }
+    }
+    rho_m = rho0;
+    sz_m = sz0;
+    __syncthreads();  // plane k-1's slot and plane k's cell terms are free
+    if (k + 2 <= ke) store_ring(n % 3);
+    if (k + 1 < ke) store_plane();
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -78,10 +78,26 @@ void Launch<T>::momentum(const Grid &g, const Phys &ph, hipStream_t s, const Mom
   a.ku = m.ku; a.kv = m.kv; a.kw = m.kw;
   a.dt = (T)dt;
   a.mode = mode;
-  // the inner rows (grid gi, no clamps along j and k) and the other rows (grid)
-  const dim3 grid((unsigned)((mom_shell_cells(g) + 255) / 256));
-  const dim3 gi((g.nx + 3 + SULI_BX - 1) / SULI_BX, (g.ny - 2 + SULI_BY - 1) / SULI_BY, g.nz - 2);
+  // the inner box (i in 1..nx, j and k in 2..n-1; grid gi) and the shell of
+  // the frame around it (grid)
+  const dim3 grid((unsigned)((mom_shell_cells(g) + 255) / 256), 3);
+  const dim3 gi((g.nx + SULI_BX - 1) / SULI_BX, (g.ny - 2 + SULI_BY - 1) / SULI_BY, g.nz - 2);
   const bool faces = m.xnf != nullptr;  // directional model: stored face viscosities
+  if (m.adv == 11 && !faces && m.tiles) {
+    // LDS tiles of 64 x MT_TJ cells walking chunks of MOM_KC planes
+#ifndef SULI_MOM_KC
+#define SULI_MOM_KC 16
+#endif
+    constexpr int MOM_KC = SULI_MOM_KC;
+    const dim3 gt((g.nx + SULI_BX - 1) / SULI_BX, (g.ny - 2 + MT_TJ - 1) / MT_TJ,
+                  (g.nz - 2 + MOM_KC - 1) / MOM_KC);
+    const dim3 bt(SULI_BX, MT_TJ);
+    if (mode == MOM_RK_FIRST && !a.fu0) k_momentum_tile<T, MOM_RK_FIRST><<<gt, bt, 0, s>>>(g, ph, a, MOM_KC);
+    else if (mode == MOM_FROM_U0) k_momentum_tile<T, MOM_FROM_U0><<<gt, bt, 0, s>>>(g, ph, a, MOM_KC);
+    else k_momentum_tile<T, 0><<<gt, bt, 0, s>>>(g, ph, a, MOM_KC);
+    k_momentum<T, 11, false><<<grid, 256, 0, s>>>(g, ph, a);
+    return;
+  }
 #define SULI_MOM(ADV)                                                                        \
   (faces ? ((void)(k_momentum_in<T, ADV, true><<<gi, blk2(), 0, s>>>(g, ph, a)),              \
             (void)(k_momentum<T, ADV, true><<<grid, 256, 0, s>>>(g, ph, a)))                  \

@@ -1,0 +1,73 @@
+"""The LDS-tiled momentum predictor (k_momentum_tile: the interior cells of
+convdiff + tempo, 6_convdiff.f90:147-186, 7_advectivo.f90:33-113,
+11_diversos.f90:10-228) against the kernel that reads every stencil value
+from memory, and in FP64 against the oracle.
+
+The grids split the interior box 2..n-1 into ragged tiles along every
+direction (64-cell rows, 4-row tiles, 16-plane chunks), so tile edges, the
+halo loads at the frame's end and the last partial chunk are all exercised.
+Both kernels evaluate the same expressions in the same order: the FP64 and
+FP32 results must agree bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import consistent_state, gpu_from_state, obstacle_state, rel_err
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(150, 14, 40), (70, 9, 21)]
+
+
+def _params(shape, **kw):
+    nx, ny, nz = shape
+    return O.Params(nx=nx, ny=ny, nz=nz, dx=1 / nx, dy=1 / nx, dz=1 / nx, **kw)
+
+
+def _run(st, prec, tiles):
+    """Forces, then the RK2 first-stage update, with the tile kernel on or off."""
+    sv = gpu_from_state(st, precision=prec)
+    sv.set_momentum(tiles)
+    sv.convdiff_forces()
+    out = {n: sv.download(n) for n in ("Fu", "Fv", "Fw")}
+    sv.convdiff()
+    sv.tempo(1, 1e-3)
+    out.update({n: sv.download(n) for n in ("u", "v", "w", "u0", "v0", "w0")})
+    sv.close()
+    return out
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_tiles_match_direct_loads(shape, prec):
+    import pysuli
+    pc = pysuli.PREC_FP64 if prec == "fp64" else pysuli.PREC_MIXED
+    st = consistent_state(_params(shape), seed=11)
+    a, b = _run(st, pc, 0), _run(st, pc, 1)
+    diff = {n: float(np.max(np.abs(a[n] - b[n]))) for n in a}
+    print(f"{shape} {prec}: max |tiles - direct| per field", diff)
+    assert all(v == 0.0 for v in diff.values()), diff
+
+
+def test_tiles_with_obstacle_match_direct_loads():
+    import pysuli
+    st = obstacle_state(_params((70, 9, 21), obst_t=1), seed=12)
+    a, b = _run(st, pysuli.PREC_FP64, 0), _run(st, pysuli.PREC_FP64, 1)
+    for n in a:
+        assert np.array_equal(a[n], b[n]), n
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_tiles_match_oracle(shape):
+    """FP64 forces of the tiled kernel against the oracle's convdiff (the
+    parity bar of the other stencils: 1e-12 of the field's largest value)."""
+    import pysuli
+    st = consistent_state(_params(shape), seed=13)
+    sv = gpu_from_state(st, precision=pysuli.PREC_FP64)
+    sv.set_momentum(1)
+    sv.convdiff_forces()
+    st.run("convdiff")
+    errs = {n: rel_err(sv.download(n), st.arr(n)) for n in ("Fu", "Fv", "Fw")}
+    sv.close()
+    print(f"{shape}: tiled forces vs oracle", errs)
+    assert all(e <= 1e-12 for e in errs.values()), errs
