@@ -328,13 +328,14 @@ int suli_set_solver(int poisson, int precond, double tol_rel, int max_it);
  * setting. */
 int suli_set_mg(int64_t fuse_max_cells, int coarse_sweeps, int fuse_fine);
 
-/* Momentum predictor kernel (convdiff + tempo, 6_convdiff.f90:147-186),
- * no counterpart in the reference.  lds_tiles 1 (default): the interior
- * cells 2..n-1 of the default advection form (adv_type 1, der 1) with the
- * Smagorinsky viscosity of m_turb = 1 run in LDS tiles that read each input
- * once per tile; 0: every stencil value read from memory.  Both compute the
- * same values in the same order.  A negative value keeps the setting. */
-int suli_set_momentum(int lds_tiles);
+/* Stencil kernel form, no counterpart in the reference.  on = 1 (default):
+ * the momentum predictor (convdiff + tempo, 6_convdiff.f90:147-186) of the
+ * default advection form (adv_type 1, der 1, m_turb 1) runs its inner box
+ * (1 <= i <= nx, 2 <= j <= ny-1, 2 <= k <= nz-1) in LDS tiles that read each
+ * input once per tile; 0: every stencil value read from memory per cell.
+ * Both forms compute the same values in the same order.  A negative value
+ * keeps the setting. */
+int suli_set_fast_stencils(int on);
 
 /* z = M r with the preconditioner of the current solver setting and the
  * operator of the last FCG suli_graddin (its dt and density): r and z are

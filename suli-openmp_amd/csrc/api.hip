@@ -152,6 +152,7 @@ extern "C" int suli_init(const suli_params *p) {
   g.total = (int64_t)g.off + (int64_t)g.sz * (p->nz + 3) + 64;
   g.dx = p->dx; g.dy = p->dy; g.dz = p->dz;
   g.ccx0 = p->ccx0; g.ccxf = p->ccxf; g.ccy0 = p->ccy0; g.ccyf = p->ccyf; g.ccz0 = p->ccz0; g.cczf = p->cczf;
+  g.fast = 1;
   Phys &ph = c->ph;
   ph.rho_f1 = p->rho_f1; ph.rho_f2 = p->rho_f2; ph.mi_f1 = p->mi_f1; ph.mi_f2 = p->mi_f2;
   ph.sigma = p->sigma; ph.alpha1 = p->alpha1; ph.csmag = p->csmag; ph.gx = p->gx; ph.gz = p->gz;
@@ -659,7 +660,6 @@ static MomPtrs mom_ptrs(Ctx &c) {
   m.xnf = dir ? c.xnf : nullptr; m.ynf = dir ? c.ynf : nullptr; m.znf = dir ? c.znf : nullptr;
   m.adv = c.adv;
   m.ku = c.obst ? c.ku_d : nullptr; m.kv = c.obst ? c.kv_d : nullptr; m.kw = c.obst ? c.kw_d : nullptr;
-  m.tiles = c.mom_tiles;
   return m;
 }
 
@@ -876,9 +876,9 @@ extern "C" int suli_set_mg(int64_t fuse_max_cells, int coarse_sweeps, int fuse_f
   return 0;
 }
 
-extern "C" int suli_set_momentum(int lds_tiles) {
+extern "C" int suli_set_fast_stencils(int on) {
   NEEDCTX();
-  if (lds_tiles >= 0) g_ctx->mom_tiles = lds_tiles ? 1 : 0;
+  if (on >= 0) g_ctx->g.fast = on ? 1 : 0;
   return 0;
 }
 

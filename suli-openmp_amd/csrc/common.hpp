@@ -40,6 +40,9 @@ struct Grid {
   int64_t total;     // elements per frame field (below 2^29: checked at set-up)
   double dx, dy, dz;
   int ccx0, ccxf, ccy0, ccyf, ccz0, cczf;
+  // 1: the momentum predictor's inner box in LDS tiles (k_momentum_tile),
+  // 0: every stencil value read from memory; the same values either way
+  int fast;
   // Frame offsets are 32-bit unsigned: with a uniform base pointer the loads
   // take the scalar-base + 32-bit-offset form, one VGPR per address instead
   // of two.  Neighbour offsets may wrap below zero in intermediate sums; the

@@ -51,8 +51,6 @@ struct Ctx {
   int mg_coarse_sweeps = 24;  // red-black sweeps on the coarsest level
   // input/output scaling, prolongations and z.q carried by the neighbouring half sweeps
   int mg_fuse_fine = 1;
-  // momentum predictor: LDS-tiled interior kernel (1) or direct loads (0)
-  int mom_tiles = 1;
   int prec = SULI_PREC_FP64;
   size_t esz = 8;
   hipStream_t st = nullptr;

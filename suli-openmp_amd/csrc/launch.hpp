@@ -14,7 +14,6 @@ struct MomPtrs {
   const void *xnf, *ynf, *znf;  // face viscosities of the directional model, else null
   int adv;                      // 10*adv_type + der (11, 12, 13, 21, 31)
   const int *ku, *kv, *kw;      // bottom-obstacle columns (plane layout), else null
-  int tiles;                    // 1: LDS-tiled interior kernel where it applies (adv 11, no face viscosities)
 };
 
 // outputs of the FCG face set-up (k_fcg_faces)

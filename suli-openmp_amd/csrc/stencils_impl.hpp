@@ -83,7 +83,7 @@ void Launch<T>::momentum(const Grid &g, const Phys &ph, hipStream_t s, const Mom
   const dim3 grid((unsigned)((mom_shell_cells(g) + 255) / 256), 3);
   const dim3 gi((g.nx + SULI_BX - 1) / SULI_BX, (g.ny - 2 + SULI_BY - 1) / SULI_BY, g.nz - 2);
   const bool faces = m.xnf != nullptr;  // directional model: stored face viscosities
-  if (m.adv == 11 && !faces && m.tiles) {
+  if (m.adv == 11 && !faces && g.fast) {
     // LDS tiles of 64 x MT_TJ cells walking chunks of MOM_KC planes
 #ifndef SULI_MOM_KC
 #define SULI_MOM_KC 16

@@ -116,7 +116,7 @@ def lib():
             "suli_set_vol_ini": ([f64], i32), "suli_get_volumes": ([p(f64), p(f64)], i32),
             "suli_set_solver": ([i32, i32, f64, i32], i32),
             "suli_set_mg": ([C.c_int64, i32, i32], i32),
-            "suli_set_momentum": ([i32], i32),
+            "suli_set_fast_stencils": ([i32], i32),
             "suli_precond_apply": ([p(f64), p(f64), C.c_int64], i32),
             "suli_set_forcing": ([f64, f64, f64], i32), "suli_set_time": ([f64], i32),
             "suli_prof_enable": ([i32], i32), "suli_prof_select": ([C.c_char_p, i32], i32),
@@ -318,10 +318,10 @@ class Solver:
         prolongation and z.q inside the neighbouring half sweeps (1/0); -1 keeps."""
         _check(lib().suli_set_mg(int(fuse_max_cells), int(coarse_sweeps), int(fuse_fine)), "suli_set_mg")
 
-    def set_momentum(self, lds_tiles):
-        """Momentum predictor: LDS-tiled interior kernel (1, default) or every
-        stencil value read from memory (0); the same values either way."""
-        _check(lib().suli_set_momentum(int(lds_tiles)), "suli_set_momentum")
+    def set_fast_stencils(self, on):
+        """Momentum predictor in LDS tiles (1, default) or with every stencil
+        value read from memory (0); the same values either way."""
+        _check(lib().suli_set_fast_stencils(int(on)), "suli_set_fast_stencils")
 
     def precond_apply(self, r):
         """z = M r with the current preconditioner and the operator of the last

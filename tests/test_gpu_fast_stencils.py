@@ -1,13 +1,14 @@
-"""The LDS-tiled momentum predictor (k_momentum_tile: the interior cells of
-convdiff + tempo, 6_convdiff.f90:147-186, 7_advectivo.f90:33-113,
-11_diversos.f90:10-228) against the kernel that reads every stencil value
-from memory, and in FP64 against the oracle.
+"""The LDS-tiled momentum predictor (suli_set_fast_stencils(1),
+k_momentum_tile: the inner cells of convdiff + tempo, 6_convdiff.f90:147-186,
+7_advectivo.f90:33-113, 11_diversos.f90:10-228) against the kernel that
+reads every stencil value from memory, and in FP64 against the oracle.
 
-The grids split the interior box 2..n-1 into ragged tiles along every
-direction (64-cell rows, 4-row tiles, 16-plane chunks), so tile edges, the
-halo loads at the frame's end and the last partial chunk are all exercised.
-Both kernels evaluate the same expressions in the same order: the FP64 and
-FP32 results must agree bit for bit."""
+The grids split the inner box into ragged tiles along every direction
+(64-cell rows, 4-row tiles, 16-plane chunks), so tile edges, the x ends
+(mirrored halo columns), the halo loads at the frame's end and the last
+partial chunk are all exercised.  Both kernels evaluate the same
+expressions in the same order: the FP64 and FP32 results must agree bit
+for bit."""
 import numpy as np
 import pytest
 
@@ -27,7 +28,7 @@ def _params(shape, **kw):
 def _run(st, prec, tiles):
     """Forces, then the RK2 first-stage update, with the tile kernel on or off."""
     sv = gpu_from_state(st, precision=prec)
-    sv.set_momentum(tiles)
+    sv.set_fast_stencils(tiles)
     sv.convdiff_forces()
     out = {n: sv.download(n) for n in ("Fu", "Fv", "Fw")}
     sv.convdiff()
@@ -64,10 +65,11 @@ def test_tiles_match_oracle(shape):
     import pysuli
     st = consistent_state(_params(shape), seed=13)
     sv = gpu_from_state(st, precision=pysuli.PREC_FP64)
-    sv.set_momentum(1)
+    sv.set_fast_stencils(1)
     sv.convdiff_forces()
     st.run("convdiff")
     errs = {n: rel_err(sv.download(n), st.arr(n)) for n in ("Fu", "Fv", "Fw")}
     sv.close()
     print(f"{shape}: tiled forces vs oracle", errs)
     assert all(e <= 1e-12 for e in errs.values()), errs
+
