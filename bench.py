@@ -62,7 +62,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 BYTES_PER_CELL = {
     "ls_advect_rk3": 6 * 4,        # phi, phi_n, u, v, w -> phi
     "reinit_sign": 2 * 4,          # phi -> S
-    "reinit_rk3": 4 * 4,           # phi, phi_it, S -> phi
+    "reinit_rk3": 4 * 4,           # phi, phi_it, S -> phi (the first stage: phi -> S, phi)
     "heaviside_normals": 11 * 4,   # phi -> H, Hx, Hy, Hz, rho, mu, |grad phi|, n (3)
     "curvature": 7 * 4,            # n (3) -> d (3), kappa
     "volume_correction": 3 * 4,
@@ -71,10 +71,11 @@ BYTES_PER_CELL = {
     "posdin": 8 + 7 * 4,           # p (FP64), rho, u, v, w -> u, v, w
     "est_div": 3 * 4,
     "fcg_spmv_dot": 8 * 4,         # z, p_old, r, ax, ay, az -> p, q
-    "fcg_update": 6 * 4,           # p, q, r, d -> d, r
+    "fcg_update": 6 * 4 + 12,      # p, q, r, d -> d, r; and the next V-cycle's finest input:
+                                   # s -> b (either colour), red cells D -> x (half the cells each)
     "fcg_zq": 2 * 4,
     "fcg_residual64": 7 * 8,       # x, b, sqrt(D), ax, ay, az (FP64) -> r (FP64)
-    "fcg_project": 8 + 8 + 4 + 4,  # r, sqrt(D) (FP64) -> r32, d
+    "fcg_project": 8 + 8 + 4 + 4 + 12,  # r, sqrt(D) (FP64) -> r32, d; and the first V-cycle's input as fcg_update
     "fcg_xupdate": 8 + 4 + 8,
     "mg_gs_l0": 2 + 2 + 6 + 2,     # x of the other colour, b of this one (FP32, half the cells each),
                                    # c_x,c_y,c_z (BF16; the diagonal is their sum) -> x of this colour
@@ -385,7 +386,7 @@ PMC_KERNELS = {"mg_gs_l0": "k_mg0_gs", "mg_gs_l0_in": "k_mg0_in", "mg_gs_l0_prol
                "mg_gs_l0_out": "k_mg0_out", "mg_restrict_l0": "k_mg0_restrict", "mg_prolong_l0": "k_mg_prolong",
                "mg_scale": "k_scale", "fcg_spmv_dot": "k_fcg1", "fcg_update": "k_fcg2", "fcg_zq": "k_zq",
                "fcg_residual64": "k_res64", "fcg_project": "k_r32", "fcg_xupdate": "k_xupd",
-               "convdiff_tempo": ("k_momentum_in", "k_momentum"), "ls_advect_rk3": "k_ls_adv", "reinit_rk3": "k_reinit_stage",
+               "convdiff_tempo": ("k_momentum_tile", "k_momentum"), "ls_advect_rk3": "k_ls_adv", "reinit_rk3": "k_reinit_stage",
                "reinit_sign": "k_reinit_sign", "heaviside_normals": "k_heavi", "curvature": "k_curv",
                "visco": "k_visco", "posdin": "k_posdin", "est_div": "k_divmax"}
 

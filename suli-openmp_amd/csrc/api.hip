@@ -575,13 +575,20 @@ extern "C" int suli_level_set(double dt, suli_ls_stats *st) {
   std::swap(c.ls, c.lsA);
   // reinitialisation (:386-509)
   const double dt1 = c.ph.dt1, dx1 = c.ph.dx1;
-  { ProfScope ps_(c, "reinit_sign"); DISPATCH(c, Launch<TT>::reinit_sign(g, c.ph, c.st, c.ls, c.S)); }
+  // (fast form: the sign function fused with the first stage, the same values)
+  if (!g.fast) { ProfScope ps_(c, "reinit_sign"); DISPATCH(c, Launch<TT>::reinit_sign(g, c.ph, c.st, c.ls, c.S)); }
   int il = 0;
   double error = 999.;
   while (error > dt1 * dx1 * dx1) {
     il = il + 1;
-    { ProfScope ps_(c, "reinit_rk3"); DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.ls, c.ls, c.S, c.lsA, 1., 0., 1. * dt1, 0,
-                                         c.slot(RS_BAND, 1))); }
+    if (il == 1 && g.fast) {
+      ProfScope ps_(c, "reinit_rk3");
+      DISPATCH(c, Launch<TT>::reinit_first(g, c.ph, c.st, c.ls, c.S, c.lsA, 1. * dt1));
+    } else {
+      ProfScope ps_(c, "reinit_rk3");
+      DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.ls, c.ls, c.S, c.lsA, 1., 0., 1. * dt1, 0,
+                                           c.slot(RS_BAND, 1)));
+    }
     { ProfScope ps_(c, "reinit_rk3"); DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.lsA, c.ls, c.S, c.lsB, 3. / 4., 1. / 4.,
                                          1. / 4. * dt1, 0, c.slot(RS_BAND, 1))); }
     { ProfScope ps_(c, "reinit_rk3"); DISPATCH(c, Launch<TT>::reinit_stage(g, c.ph, c.st, c.lsB, c.ls, c.S, c.lsA, 1. / 3., 2. / 3.,

@@ -31,6 +31,10 @@ struct Launch {
   static void reinit_stage(const Grid &g, const Phys &ph, hipStream_t s, const void *phi,
                            const void *phit, const void *S, void *out, double ca, double cb,
                            double cgdt, int band, RedSlot rs);
+  // reinit_sign fused with the first stage of the first reinitialisation
+  // iteration (ca = 1, cb = 0, phit = phi): S and that stage's output
+  static void reinit_first(const Grid &g, const Phys &ph, hipStream_t s, const void *phi, void *S, void *out,
+                           double cgdt);
   static void heavi(const Grid &g, const Phys &ph, hipStream_t s, const void *phi, void *hs, void *hsx,
                     void *hsy, void *hsz, void *rho, void *lsnu, void *modls, void *nrx, void *nry,
                     void *nrz, RedSlot rs);

@@ -69,12 +69,15 @@ __global__ __launch_bounds__(256) void k_reinit_sign(Grid g, T dx1, const T *__r
 // One SSP-RK3 stage of the reinitialisation (:443-484).  On the last stage
 // (band != 0) it also sums, in FP64, the change phi_new - phi_old and the
 // number of cells in the band abs(phi_new) <= 2*alpha1*dx1 (:493-503).
-template <typename T>
+// SIGN: the first stage of the first iteration, whose phi is phi0: the sign
+// function of k_reinit_sign comes from the same derivatives and the same
+// |grad phi|, so it is computed here and stored in S (dx1: as there).
+template <typename T, bool SIGN = false>
 __global__ __launch_bounds__(256) void k_reinit_stage(Grid g, const T *__restrict__ phi,
                                                       const T *__restrict__ phit,
-                                                      const T *__restrict__ S, T *__restrict__ out,
+                                                      T *__restrict__ S, T *__restrict__ out,
                                                       T ca, T cb, T cgdt, int band, double bandw,
-                                                      RedSlot rs) {
+                                                      RedSlot rs, T dx1 = T(0)) {
   const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             k = 1 + blockIdx.z;
   double acc[2] = {0.0, 0.0};
@@ -84,7 +87,14 @@ __global__ __launch_bounds__(256) void k_reinit_stage(Grid g, const T *__restric
     der_weno_cell<T>(phi, g, i, j, k, c, false, ta, tb, tc, td, te, tf);
     const T lsv = phi[c];
     const T m = godunov_mod<T>(lsv, ta, tb, tc, td, te, tf);
-    const T sy7 = S[c] * (T(1.) - m);
+    T sv;
+    if constexpr (SIGN) {
+      sv = lsv / sqrt(lsv * lsv + m * m * dx1 * dx1);
+      S[c] = sv;
+    } else {
+      sv = S[c];
+    }
+    const T sy7 = sv * (T(1.) - m);
     const T nv = ca * phit[c] + cb * lsv + cgdt * sy7;
     out[c] = nv;
     if (band) {
@@ -760,7 +770,11 @@ __global__ __launch_bounds__(SULI_BX *MT_TJ) __attribute__((amdgpu_waves_per_eu(
   __shared__ T pk[NS][HS];
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + SULI_BX * ty;
-  const int i0 = 1 + blockIdx.x * SULI_BX, j0 = 2 + blockIdx.y * MT_TJ, kb = 2 + blockIdx.z * kc;
+  // XCD-band order: the j-neighbour tiles, which read each other's halo
+  // rows, run on the same XCD and share its L2
+  int bx, by, bz;
+  xcd_band_block(bx, by, bz);
+  const int i0 = 1 + bx * SULI_BX, j0 = 2 + by * MT_TJ, kb = 2 + bz * kc;
   const int ke = min(kb + kc, nz);  // planes kb..ke-1 computed, kb-1..ke read
   const int i = i0 + tx, j = j0 + ty;
   const bool act = i <= nx && j <= ny - 1;

@@ -1,7 +1,12 @@
-"""The LDS-tiled momentum predictor (suli_set_fast_stencils(1),
-k_momentum_tile: the inner cells of convdiff + tempo, 6_convdiff.f90:147-186,
-7_advectivo.f90:33-113, 11_diversos.f90:10-228) against the kernel that
-reads every stencil value from memory, and in FP64 against the oracle.
+"""The fast forms of the stencil kernels (suli_set_fast_stencils(1)) against
+their plain forms, and in FP64 against the oracle:
+
+- the LDS-tiled momentum predictor (k_momentum_tile: the inner cells of
+  convdiff + tempo, 6_convdiff.f90:147-186, 7_advectivo.f90:33-113,
+  11_diversos.f90:10-228) against the kernel that reads every stencil value
+  from memory;
+- the reinitialisation's sign function (14_level_set.f90:400-435) computed
+  inside its first RK stage against its own pass.
 
 The grids split the inner box into ragged tiles along every direction
 (64-cell rows, 4-row tiles, 16-plane chunks), so tile edges, the x ends
@@ -9,11 +14,13 @@ The grids split the inner box into ragged tiles along every direction
 partial chunk are all exercised.  Both kernels evaluate the same
 expressions in the same order: the FP64 and FP32 results must agree bit
 for bit."""
+import dataclasses
+
 import numpy as np
 import pytest
 
 import oracle as O
-from helpers import consistent_state, gpu_from_state, obstacle_state, rel_err
+from helpers import CELL, consistent_state, gpu_from_state, obstacle_state, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -73,3 +80,36 @@ def test_tiles_match_oracle(shape):
     print(f"{shape}: tiled forces vs oracle", errs)
     assert all(e <= 1e-12 for e in errs.values()), errs
 
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("per", [False, True])
+def test_level_set_fast_matches_plain(prec, per):
+    """One whole level_set (advection, reinitialisation, volume, heaviside,
+    curvature) in both forms, with walls or periodic x; then the FP64 fast
+    form against the oracle at the level-set parity bar."""
+    import pysuli
+    pc = pysuli.PREC_FP64 if prec == "fp64" else pysuli.PREC_MIXED
+    p = _params((70, 9, 21))
+    if per:
+        p = dataclasses.replace(p, ccx0=0, ccxf=0)
+    st = consistent_state(p, seed=14, perturb=0.5)
+    out = {}
+    for fast in (0, 1):
+        sv = gpu_from_state(st, precision=pc)
+        sv.set_fast_stencils(fast)
+        sv.level_set_ini()
+        stats = sv.level_set(2e-3)
+        out[fast] = {n: sv.download(n) for n in CELL}
+        out[fast]["vol"] = np.array(sv.volumes())
+        sv.close()
+    diff = {n: float(np.max(np.abs(out[0][n] - out[1][n]))) for n in out[0]}
+    print(f"{prec} per={per}: max |fast - plain| per field", diff, "stats", stats)
+    assert all(v == 0.0 for v in diff.values()), diff
+    if prec == "fp64":
+        st.run("ls_ini")
+        st.scal["dt"] = 2e-3
+        st.run("level_set")
+        errs = {n: rel_err(out[1][n], st.arr(n)) for n in ("ls", "hs", "rho", "kurv")}
+        print("fast FP64 level set vs oracle", errs)
+        assert all(e <= 1e-10 for e in errs.values()), errs
