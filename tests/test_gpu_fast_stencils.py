@@ -83,6 +83,45 @@ def test_tiles_match_oracle(shape):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", ["rk3", "ab2", "euler", "per_xy", "sources"])
+def test_tiles_other_updates_match_direct_loads(prec, case):
+    """The tile kernel's run-time stage updates (Euler, the three RK3 stages,
+    AB2 after its Euler start), periodic x and y walls, and momentum sources
+    tf_u, tf_v, tf_w, each against the direct-load kernel."""
+    import pysuli
+    pc = pysuli.PREC_FP64 if prec == "fp64" else pysuli.PREC_MIXED
+    p = _params((70, 9, 21))
+    scheme = {"rk3": 2, "ab2": 3, "euler": 0}.get(case, 1)
+    p = dataclasses.replace(p, t_tempo=scheme)
+    if case == "per_xy":
+        p = dataclasses.replace(p, ccx0=0, ccxf=0, ccy0=0, ccyf=0)
+    st = consistent_state(p, seed=15)
+    rng = np.random.default_rng(15)
+    for n in ("fu0", "fv0", "fw0", "fu1", "fv1", "fw1"):
+        st.arr(n)[...] = 0.1 * rng.standard_normal(st.arr(n).shape)
+    names = ["u", "v", "w", "u0", "v0", "w0", "fu0", "fv0", "fw0", "fu1", "fv1", "fw1"]
+    out = {}
+    for fast in (0, 1):
+        sv = gpu_from_state(st, precision=pc)
+        sv.set_fast_stencils(fast)
+        if case == "sources":
+            for n in ("tf_u", "tf_v", "tf_w"):
+                sv.upload(n, 0.3 * np.random.default_rng(16).standard_normal(sv.shape(n)))
+        for tt in range(1, sv.time_stages() + 1):
+            sv.convdiff()
+            sv.tempo(tt, 1e-3)
+            sv.contorno(1)
+        if case == "ab2":  # the AB2 update proper after its Euler start
+            sv.convdiff()
+            sv.tempo(1, 1e-3)
+        out[fast] = {n: sv.download(n) for n in names}
+        sv.close()
+    diff = {n: float(np.max(np.abs(out[0][n] - out[1][n]))) for n in names}
+    print(f"{case} {prec}: max |tiles - direct| per field", diff)
+    assert all(v == 0.0 for v in diff.values()), diff
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("per", [False, True])
 def test_level_set_fast_matches_plain(prec, per):
     """One whole level_set (advection, reinitialisation, volume, heaviside,
