@@ -1115,44 +1115,59 @@ __global__ void k_bc_y(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restri
 template <typename T>
 __global__ void k_bc_x(Grid g, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
                        double *__restrict__ p, const T *__restrict__ bp, BPlanes L) {
+  // blockIdx.z: field (p, u, v, w) x side (x = 0, x = xf).  Every write reads
+  // only cells that no part writes, so the eight parts are independent; one
+  // thread per row touches a cache line per row, and the split gives the
+  // launch enough wavefronts to hide that latency.
   const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  const int fld = blockIdx.z >> 1, hi = blockIdx.z & 1;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (j > ny + 2 || k > nz + 2) return;
   const bool per = g.ccx0 == 0 && g.ccxf == 0;
-  if (j <= ny + 1 && k <= nz + 1) {
-    if (per) { p[g.idx(0, j, k)] = p[g.idx(nx, j, k)]; p[g.idx(nx + 1, j, k)] = p[g.idx(1, j, k)]; }
-    else { p[g.idx(0, j, k)] = p[g.idx(1, j, k)]; p[g.idx(nx + 1, j, k)] = p[g.idx(nx, j, k)]; }
-    // u: j in [0,ny+1], k in [0,nz+1]
-    if (per) u[g.idx(0, j, k)] = u[g.idx(nx, j, k)];
-    else if (g.ccx0 == 1) u[g.idx(0, j, k)] = -u[g.idx(2, j, k)];
-    else if (g.ccx0 == 2) u[g.idx(0, j, k)] = u[g.idx(2, j, k)];
-    else if (g.ccx0 == 3) u[g.idx(0, j, k)] = bp[L.at(BXX0, j, k)];
-    if (per) u[g.idx(nx + 2, j, k)] = u[g.idx(2, j, k)];
-    else if (g.ccxf == 1) u[g.idx(nx + 2, j, k)] = -u[g.idx(nx, j, k)];
-    else if (g.ccxf == 2) u[g.idx(nx + 2, j, k)] = u[g.idx(nx, j, k)];
-    else if (g.ccxf == 3) u[g.idx(nx + 2, j, k)] = bp[L.at(BXXF1, j, k)];
-    else if (g.ccxf == 4) u[g.idx(nx + 2, j, k)] = u[g.idx(nx + 1, j, k)];
+  if (fld == 0 && j <= ny + 1 && k <= nz + 1) {
+    if (!hi) p[g.idx(0, j, k)] = per ? p[g.idx(nx, j, k)] : p[g.idx(1, j, k)];
+    else p[g.idx(nx + 1, j, k)] = per ? p[g.idx(1, j, k)] : p[g.idx(nx, j, k)];
   }
-  if (k <= nz + 1) {  // v: j in [0,ny+2]
-    if (per) v[g.idx(0, j, k)] = v[g.idx(nx, j, k)];
-    else if (g.ccx0 == 1) v[g.idx(0, j, k)] = v[g.idx(1, j, k)];
-    else if (g.ccx0 == 2) v[g.idx(0, j, k)] = -v[g.idx(1, j, k)];
-    else if (g.ccx0 == 3 || g.ccx0 == 4) v[g.idx(0, j, k)] = bp[L.at(BXY0, j, k)];
-    if (per) v[g.idx(nx + 1, j, k)] = v[g.idx(1, j, k)];
-    else if (g.ccxf == 1 || g.ccxf == 4) v[g.idx(nx + 1, j, k)] = v[g.idx(nx, j, k)];
-    else if (g.ccxf == 2) v[g.idx(nx + 1, j, k)] = -v[g.idx(nx, j, k)];
-    else if (g.ccxf == 3) v[g.idx(nx + 1, j, k)] = bp[L.at(BXYF, j, k)];
+  if (fld == 1 && j <= ny + 1 && k <= nz + 1) {  // u: j in [0,ny+1], k in [0,nz+1]
+    if (!hi) {
+      if (per) u[g.idx(0, j, k)] = u[g.idx(nx, j, k)];
+      else if (g.ccx0 == 1) u[g.idx(0, j, k)] = -u[g.idx(2, j, k)];
+      else if (g.ccx0 == 2) u[g.idx(0, j, k)] = u[g.idx(2, j, k)];
+      else if (g.ccx0 == 3) u[g.idx(0, j, k)] = bp[L.at(BXX0, j, k)];
+    } else {
+      if (per) u[g.idx(nx + 2, j, k)] = u[g.idx(2, j, k)];
+      else if (g.ccxf == 1) u[g.idx(nx + 2, j, k)] = -u[g.idx(nx, j, k)];
+      else if (g.ccxf == 2) u[g.idx(nx + 2, j, k)] = u[g.idx(nx, j, k)];
+      else if (g.ccxf == 3) u[g.idx(nx + 2, j, k)] = bp[L.at(BXXF1, j, k)];
+      else if (g.ccxf == 4) u[g.idx(nx + 2, j, k)] = u[g.idx(nx + 1, j, k)];
+    }
   }
-  if (j <= ny + 1) {  // w: k in [0,nz+2]
-    if (per) w[g.idx(0, j, k)] = w[g.idx(nx, j, k)];
-    else if (g.ccx0 == 1) w[g.idx(0, j, k)] = w[g.idx(1, j, k)];
-    else if (g.ccx0 == 2) w[g.idx(0, j, k)] = -w[g.idx(1, j, k)];
-    else if (g.ccx0 == 3) w[g.idx(0, j, k)] = bp[L.at(BXZ0, j, k)];
-    else if (g.ccx0 == 4) w[g.idx(0, j, k)] = w[g.idx(1, j, k)];
-    if (per) w[g.idx(nx + 1, j, k)] = w[g.idx(1, j, k)];
-    else if (g.ccxf == 1 || g.ccxf == 4) w[g.idx(nx + 1, j, k)] = w[g.idx(nx, j, k)];
-    else if (g.ccxf == 2) w[g.idx(nx + 1, j, k)] = -w[g.idx(nx, j, k)];
-    else if (g.ccxf == 3) w[g.idx(nx + 1, j, k)] = bp[L.at(BXZF, j, k)];
+  if (fld == 2 && k <= nz + 1) {  // v: j in [0,ny+2]
+    if (!hi) {
+      if (per) v[g.idx(0, j, k)] = v[g.idx(nx, j, k)];
+      else if (g.ccx0 == 1) v[g.idx(0, j, k)] = v[g.idx(1, j, k)];
+      else if (g.ccx0 == 2) v[g.idx(0, j, k)] = -v[g.idx(1, j, k)];
+      else if (g.ccx0 == 3 || g.ccx0 == 4) v[g.idx(0, j, k)] = bp[L.at(BXY0, j, k)];
+    } else {
+      if (per) v[g.idx(nx + 1, j, k)] = v[g.idx(1, j, k)];
+      else if (g.ccxf == 1 || g.ccxf == 4) v[g.idx(nx + 1, j, k)] = v[g.idx(nx, j, k)];
+      else if (g.ccxf == 2) v[g.idx(nx + 1, j, k)] = -v[g.idx(nx, j, k)];
+      else if (g.ccxf == 3) v[g.idx(nx + 1, j, k)] = bp[L.at(BXYF, j, k)];
+    }
+  }
+  if (fld == 3 && j <= ny + 1) {  // w: k in [0,nz+2]
+    if (!hi) {
+      if (per) w[g.idx(0, j, k)] = w[g.idx(nx, j, k)];
+      else if (g.ccx0 == 1) w[g.idx(0, j, k)] = w[g.idx(1, j, k)];
+      else if (g.ccx0 == 2) w[g.idx(0, j, k)] = -w[g.idx(1, j, k)];
+      else if (g.ccx0 == 3) w[g.idx(0, j, k)] = bp[L.at(BXZ0, j, k)];
+      else if (g.ccx0 == 4) w[g.idx(0, j, k)] = w[g.idx(1, j, k)];
+    } else {
+      if (per) w[g.idx(nx + 1, j, k)] = w[g.idx(1, j, k)];
+      else if (g.ccxf == 1 || g.ccxf == 4) w[g.idx(nx + 1, j, k)] = w[g.idx(nx, j, k)];
+      else if (g.ccxf == 2) w[g.idx(nx + 1, j, k)] = -w[g.idx(nx, j, k)];
+      else if (g.ccxf == 3) w[g.idx(nx + 1, j, k)] = bp[L.at(BXZF, j, k)];
+    }
   }
 }
 

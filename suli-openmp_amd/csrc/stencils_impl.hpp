@@ -143,7 +143,11 @@ void Launch<T>::contorno(const Grid &g, hipStream_t s, int nlock, void *u, void 
   const BPlanes L = bplanes_layout(g.nx, g.ny, g.nz);
   const T *P = (const T *)bp;
   SULI_LAUNCH(k_bc_y<T>, gr(g.nx + 3, g.nz + 3), b, s, g, (T *)u, (T *)v, (T *)w, p, P, L);
-  SULI_LAUNCH(k_bc_x<T>, gr(g.ny + 3, g.nz + 3), b, s, g, (T *)u, (T *)v, (T *)w, p, P, L);
+  {
+    dim3 gx = gr(g.ny + 3, g.nz + 3);
+    gx.z = 8;  // field x side
+    SULI_LAUNCH(k_bc_x<T>, gx, b, s, g, (T *)u, (T *)v, (T *)w, p, P, L);
+  }
   SULI_LAUNCH(k_bc_z<T>, gr(g.nx + 3, g.ny + 3), b, s, g, (T *)u, (T *)v, (T *)w, p, P, L);
   if (ku) SULI_LAUNCH(k_bc_obst<T>, gr(g.nx + 3, g.ny + 3), b, s, g, (T *)u, (T *)v, (T *)w, ku, kv, kw);
   if (nlock == 2) {
