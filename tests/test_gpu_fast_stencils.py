@@ -24,7 +24,8 @@ from helpers import CELL, consistent_state, gpu_from_state, obstacle_state, rel_
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(150, 14, 40), (70, 9, 21)]
+# (the third splits into 8 tile rows, which turns the XCD-band block order on)
+SHAPES = [(150, 14, 40), (70, 9, 21), (70, 34, 20)]
 
 
 def _params(shape, **kw):
