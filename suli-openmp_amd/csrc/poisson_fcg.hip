@@ -478,39 +478,40 @@ __global__ void k_mg_diag_bf16(Grid g, const uint16_t *__restrict__ hx, const ui
   if (sD) sD32[c] = (float)sD[c];
   if (dred && ((i + j + k) & 1) == 0) dred[h.idx((i - 1) >> 1, j, k)] = d;
 }
-__global__ void k_mg_diag(Grid g, const float *__restrict__ cx, const float *__restrict__ cy,
-                          const float *__restrict__ cz, float *__restrict__ dg) {
-  const int i = 1 + blockIdx.x * SULI_BX + threadIdx.x, j = 1 + blockIdx.y * SULI_BY + threadIdx.y,
-            k = 1 + blockIdx.z;
-  if (i > g.nx || j > g.ny || k > g.nz) return;
-  const int64_t c = g.idx(i, j, k);
-  dg[c] = cx[c] + cx[c + 1] + cy[c] + cy[c + g.sx] + cz[c] + cz[c + g.sz];
-}
-
-// coarse face conductance: half the sum of the four fine faces on it
+// coarse face conductance: half the sum of the four fine faces on it; and
+// the coarse diagonal, the sum of the cell's six faces in the order x-, x+,
+// y-, y+, z-, z+ (the sum the sweeps form), with its three upper faces
+// formed again by the formula their own threads use
 template <typename CT>
 __global__ void k_mg_coarsen(Grid gf, Grid gc, const CT *__restrict__ fx, const CT *__restrict__ fy,
                              const CT *__restrict__ fz, float *__restrict__ cx, float *__restrict__ cy,
-                             float *__restrict__ cz) {
+                             float *__restrict__ cz, float *__restrict__ dg) {
   const int I = 1 + blockIdx.x * SULI_BX + threadIdx.x, J = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             K = 1 + blockIdx.z;
   if (I > gc.nx + 1 || J > gc.ny + 1 || K > gc.nz + 1) return;
   const int64_t c = gc.idx(I, J, K);
-  if (J <= gc.ny && K <= gc.nz) {
-    const int fi = 2 * I - 1, fj = 2 * J - 1, fk = 2 * K - 1;
-    cx[c] = 0.5f * (mgc(fx[gf.idx(fi, fj, fk)]) + mgc(fx[gf.idx(fi, fj + 1, fk)]) +
-                    mgc(fx[gf.idx(fi, fj, fk + 1)]) + mgc(fx[gf.idx(fi, fj + 1, fk + 1)]));
-  }
-  if (I <= gc.nx && K <= gc.nz) {
-    const int fi = 2 * I - 1, fj = 2 * J - 1, fk = 2 * K - 1;
-    cy[c] = 0.5f * (mgc(fy[gf.idx(fi, fj, fk)]) + mgc(fy[gf.idx(fi + 1, fj, fk)]) +
-                    mgc(fy[gf.idx(fi, fj, fk + 1)]) + mgc(fy[gf.idx(fi + 1, fj, fk + 1)]));
-  }
-  if (I <= gc.nx && J <= gc.ny) {
-    const int fi = 2 * I - 1, fj = 2 * J - 1, fk = 2 * K - 1;
-    cz[c] = 0.5f * (mgc(fz[gf.idx(fi, fj, fk)]) + mgc(fz[gf.idx(fi + 1, fj, fk)]) +
-                    mgc(fz[gf.idx(fi, fj + 1, fk)]) + mgc(fz[gf.idx(fi + 1, fj + 1, fk)]));
-  }
+  // the coarse faces x, y, z of coarse cell (a, b, d): its lower face in each direction
+  auto fcx = [&](int a, int b, int d) {
+    const int fi = 2 * a - 1, fj = 2 * b - 1, fk = 2 * d - 1;
+    return 0.5f * (mgc(fx[gf.idx(fi, fj, fk)]) + mgc(fx[gf.idx(fi, fj + 1, fk)]) +
+                   mgc(fx[gf.idx(fi, fj, fk + 1)]) + mgc(fx[gf.idx(fi, fj + 1, fk + 1)]));
+  };
+  auto fcy = [&](int a, int b, int d) {
+    const int fi = 2 * a - 1, fj = 2 * b - 1, fk = 2 * d - 1;
+    return 0.5f * (mgc(fy[gf.idx(fi, fj, fk)]) + mgc(fy[gf.idx(fi + 1, fj, fk)]) +
+                   mgc(fy[gf.idx(fi, fj, fk + 1)]) + mgc(fy[gf.idx(fi + 1, fj, fk + 1)]));
+  };
+  auto fcz = [&](int a, int b, int d) {
+    const int fi = 2 * a - 1, fj = 2 * b - 1, fk = 2 * d - 1;
+    return 0.5f * (mgc(fz[gf.idx(fi, fj, fk)]) + mgc(fz[gf.idx(fi + 1, fj, fk)]) +
+                   mgc(fz[gf.idx(fi, fj + 1, fk)]) + mgc(fz[gf.idx(fi + 1, fj + 1, fk)]));
+  };
+  float x0 = 0.f, y0 = 0.f, z0 = 0.f;
+  if (J <= gc.ny && K <= gc.nz) cx[c] = x0 = fcx(I, J, K);
+  if (I <= gc.nx && K <= gc.nz) cy[c] = y0 = fcy(I, J, K);
+  if (I <= gc.nx && J <= gc.ny) cz[c] = z0 = fcz(I, J, K);
+  if (dg && I <= gc.nx && J <= gc.ny && K <= gc.nz)
+    dg[c] = x0 + fcx(I + 1, J, K) + y0 + fcy(I, J + 1, K) + z0 + fcz(I, J, K + 1);
 }
 
 template <typename CT>
@@ -533,7 +534,7 @@ __device__ __forceinline__ float mg_Ax(const Grid &g, int64_t c, const CT *__res
 #define SULI_MG_NU 2  // sweeps before and after the coarse correction on the coarser levels
 #endif
 
-// The diagonal as the sum of the six face conductances, in k_mg_diag's order
+// The diagonal as the sum of the six face conductances, in k_mg_coarsen's order
 // (the stored dg, bit for bit): the sweeps read the six faces anyway, so
 // reading dg as well would only add bytes.
 template <typename CT>
@@ -1466,14 +1467,14 @@ static int mg_setup(Ctx &c, bool level0) {
     for (size_t q = 1; q < f.lv.size(); q++) {
       MgLevel &lf = f.lv[q - 1], &lc = f.lv[q];
       // the finest level coarsens from its BF16 faces (the operator its sweeps use)
+      // (the diagonal of the coarse level with its faces, one launch)
       if (lf.hx)
         hipLaunchKernelGGL(k_mg_coarsen<uint16_t>, faces_full(lc.g), b2, 0, c.st, lf.g, lc.g,
                            (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, lc.cx, lc.cy,
-                           lc.cz);
+                           lc.cz, lc.dg);
       else
         hipLaunchKernelGGL(k_mg_coarsen<float>, faces_full(lc.g), b2, 0, c.st, lf.g, lc.g, (const float *)lf.cx,
-                           (const float *)lf.cy, (const float *)lf.cz, lc.cx, lc.cy, lc.cz);
-      hipLaunchKernelGGL(k_mg_diag, cells_full(lc.g), b2, 0, c.st, lc.g, lc.cx, lc.cy, lc.cz, lc.dg);
+                           (const float *)lf.cy, (const float *)lf.cz, lc.cx, lc.cy, lc.cz, lc.dg);
     }
   }
   HIPCHK(hipGetLastError());
