@@ -299,6 +299,34 @@ __device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5) {
                  o3 * (2 * v3 + 5 * v4 - v5));
 }
 
+// The mixed-precision form: the same smoothness indicators and stencils,
+// with the weights 0.1/d1, 0.6/d2, 0.3/d3 (d = (eps + IS)^2) normalised
+// through the products 0.1 d2 d3, 0.6 d1 d3, 0.3 d1 d2 and one reciprocal
+// instead of four (a reciprocal issues at a quarter of the VALU rate).  With
+// eps = 1e-8, d >= 1e-16 and a product of two d stays above the FP32 normal
+// range's floor; the divided differences of a level set are O(1), far from
+// its ceiling.
+template <>
+__device__ __forceinline__ float weno5<float>(float v1, float v2, float v3, float v4, float v5) {
+  const float aux1 = 13.f / 12.f, aux2 = 1.f / 4.f, aux3 = 1.f / 6.f;
+  const float eps = 0.00000001f;
+  float a, b;
+  a = v1 - 2 * v2 + v3;
+  b = v1 - 4 * v2 + 3 * v3;
+  const float is1 = aux1 * a * a + aux2 * b * b;
+  a = v2 - 2 * v3 + v4;
+  b = v2 - v4;
+  const float is2 = aux1 * a * a + aux2 * b * b;
+  a = v3 - 2 * v4 + v5;
+  b = 3 * v3 - 4 * v4 + v5;
+  const float is3 = aux1 * a * a + aux2 * b * b;
+  const float d1 = (eps + is1) * (eps + is1), d2 = (eps + is2) * (eps + is2), d3 = (eps + is3) * (eps + is3);
+  const float w1 = 0.1f * (d2 * d3), w2 = 0.6f * (d1 * d3), w3 = 0.3f * (d1 * d2);
+  const float r = 1.f / (w1 + w2 + w3);
+  return aux3 * r *
+         (w1 * (2 * v1 - 7 * v2 + 11 * v3) + w2 * (-v2 + 5 * v3 + 2 * v4) + w3 * (2 * v3 + 5 * v4 - v5));
+}
+
 // Seven values phi(pos-3..pos+3) of a pencil of n points with stride st,
 // centred on the point at address c (1 <= pos <= n).  Points outside 1..n are
 // the ghost values of 14_level_set.f90:532-570: ihs 0 periodic with period
