@@ -71,18 +71,18 @@ BYTES_PER_CELL = {
     "posdin": 8 + 7 * 4,           # p (FP64), rho, u, v, w -> u, v, w
     "est_div": 3 * 4,
     "fcg_spmv_dot": 8 * 4,         # z, p_old, r, ax, ay, az -> p, q
-    "fcg_update": 6 * 4 + 12,      # p, q, r, d -> d, r; and the next V-cycle's finest input:
-                                   # s -> b (either colour), red cells D -> x (half the cells each)
+    "fcg_update": 6 * 4 + 9,       # p, q, r, d -> d, r; and the next V-cycle's finest input:
+                                   # s -> b (FP16, either colour), red cells D (FP32) -> x (FP16; half the cells each)
     "fcg_zq": 2 * 4,
     "fcg_residual64": 7 * 8,       # x, b, sqrt(D), ax, ay, az (FP64) -> r (FP64)
-    "fcg_project": 8 + 8 + 4 + 4 + 12,  # r, sqrt(D) (FP64) -> r32, d; and the first V-cycle's input as fcg_update
+    "fcg_project": 8 + 8 + 4 + 4 + 9,  # r, sqrt(D) (FP64) -> r32, d; and the first V-cycle's input as fcg_update
     "fcg_xupdate": 8 + 4 + 8,
-    "mg_gs_l0": 2 + 2 + 6 + 2,     # x of the other colour, b of this one (FP32, half the cells each),
+    "mg_gs_l0": 1 + 1 + 6 + 1,     # x of the other colour, b of this one (FP16, half the cells each),
                                    # c_x,c_y,c_z (BF16; the diagonal is their sum) -> x of this colour
-    "mg_gs_l0_in": 4 + 4 + 2 + 4 + 2,  # r, sqrt(D) (FP32), red dg (half) -> b (both colours), red x
-    "mg_gs_l0_prolong": 12 + 1,    # a half sweep + the coarse x (1/8 of the cells)
-    "mg_gs_l0_out": 2 + 6 + 2 + 3 * 4,  # black x, faces, red b, sqrt(D), q -> z
-    "mg_restrict_l0": 4 + 4 + 3 * 2 + 1,  # b, x (FP32), BF16 faces -> coarse b (1/8)
+    "mg_gs_l0_in": 4 + 4 + 2 + 4 + 2,  # r, sqrt(D), red dg (half) -> b (both colours), red x (FP32: unstaged input)
+    "mg_gs_l0_prolong": 9 + 0.5,   # a half sweep + the coarse x (FP32, 1/8 of the cells)
+    "mg_gs_l0_out": 1 + 6 + 1 + 3 * 4,  # black x, faces, red b (FP16), sqrt(D), q -> z
+    "mg_restrict_l0": 2 + 2 + 3 * 2 + 1,  # b, x (FP16), BF16 faces -> coarse b, x (FP32, 1/8)
     "mg_prolong_l0": 2 * 4,
     "mg_scale": 3 * 4,
 }

@@ -321,11 +321,14 @@ int suli_set_solver(int poisson, int precond, double tol_rel, int max_it);
  * side of the V-cycle in one launch (LDS tiles with a halo; the same cell
  * updates as one launch per half sweep; 0 turns it off; default 65536).
  * coarse_sweeps: red-black Gauss-Seidel sweeps on the coarsest level
- * (default 24).  fuse_fine 1 (default): the operator set-up runs in two
- * passes (cells, faces) and the input and output scaling of the V-cycle, the
- * prolongations and the z.q sum of FCG run inside the neighbouring half
- * sweeps; 0: separate launches.  A negative value keeps the current
- * setting. */
+ * (default 24).  fuse_fine 1: the operator set-up runs in two passes (cells,
+ * faces) and the input and output scaling of the V-cycle, the prolongations
+ * and the z.q sum of FCG run inside the neighbouring half sweeps; 2 (default):
+ * as 1, with the finest level's iterate and right-hand side held in FP16 in
+ * the V-cycles of the FCG iteration (their input scaled to a unit RMS
+ * residual; the preconditioner changes by about 2^-11 relative, the solve
+ * still meets its FP64 tolerance); 0: separate launches.  A negative value
+ * keeps the current setting; a value above 2 is an error. */
 int suli_set_mg(int64_t fuse_max_cells, int coarse_sweeps, int fuse_fine);
 
 /* Stencil kernel form, no counterpart in the reference.  on = 1 (default):

@@ -876,10 +876,14 @@ extern "C" int suli_set_solver(int poisson, int precond, double tol_rel, int max
 
 extern "C" int suli_set_mg(int64_t fuse_max_cells, int coarse_sweeps, int fuse_fine) {
   NEEDCTX();
+  if (fuse_fine > 2) {
+    suli_set_error("suli_set_mg: fuse_fine must be 0, 1 or 2 (or negative to keep it)");
+    return 1;
+  }
   g_ctx->rho_epoch++;  // the projection rebuilds its operator
   if (fuse_max_cells >= 0) g_ctx->mg_fuse_max = fuse_max_cells;
   if (coarse_sweeps >= 0) g_ctx->mg_coarse_sweeps = coarse_sweeps;
-  if (fuse_fine >= 0) g_ctx->mg_fuse_fine = fuse_fine ? 1 : 0;
+  if (fuse_fine >= 0) g_ctx->mg_fuse_fine = fuse_fine;
   return 0;
 }
 

@@ -49,8 +49,10 @@ struct Ctx {
   // one fused launch per side of the V-cycle (0: every half sweep a launch)
   int64_t mg_fuse_max = 65536;
   int mg_coarse_sweeps = 24;  // red-black sweeps on the coarsest level
-  // input/output scaling, prolongations and z.q carried by the neighbouring half sweeps
-  int mg_fuse_fine = 1;
+  // input/output scaling, prolongations and z.q carried by the neighbouring half
+  // sweeps (1), and the finest level's colour-split iterate and right-hand side
+  // in BF16 (2); 0: separate launches
+  int mg_fuse_fine = 2;
   int prec = SULI_PREC_FP64;
   size_t esz = 8;
   hipStream_t st = nullptr;

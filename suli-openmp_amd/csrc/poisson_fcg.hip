@@ -56,6 +56,7 @@ struct FcgCtl {
   int done;          // the last outer residual met the tolerance (set with the pass's arming)
   int out;           // the last outer residual ends the solve: the output kernel writes
   // FCG(m) with the learned preconditioner: the last SULI_FCG_M directions
+  float sig;                 // input scale of the finest level's staged V-cycle input (FP16 form)
   int nk;                    // directions held (0..SULI_FCG_M)
   double zqm[SULI_FCG_M];    // z.q_j of the current z
   double pqm[SULI_FCG_M];    // p_j.q_j of the held directions
@@ -117,22 +118,61 @@ __global__ __launch_bounds__(256) void k_res64(Grid g, const double *__restrict_
 
 // the finest multigrid level's input staged from r (k_mg0_in's values, cell
 // by cell): b = sqrt(D) r split by colour, red x = b/diag
+// Storage of the finest multigrid level's colour-split iterate and
+// right-hand side: FP32, or FP16 with the V-cycle's input scaled to a unit
+// RMS residual (the scale from the last residual norm the device holds; the
+// output kernel divides it out).  FP16 rounding (2^-11 relative) perturbs
+// the preconditioner far below its own error; it cuts the bytes of the
+// finest-level half sweeps from 12 to 9 per cell.  Values beyond the FP16
+// range saturate (the V-cycle stays finite; FCG corrects a poor z).
+__device__ __forceinline__ float mgc(_Float16 v) { return (float)v; }
+template <typename XT> __device__ __forceinline__ XT to_x(float v);
+template <> __device__ __forceinline__ float to_x<float>(float v) { return v; }
+template <> __device__ __forceinline__ _Float16 to_x<_Float16>(float v) {
+  return (_Float16)fminf(fmaxf(v, -65504.f), 65504.f);
+}
+// 1/RMS of a residual with squared norm rr over n cells (1 when rr is not
+// a positive finite number)
+__device__ __forceinline__ float unit_rms_scale(double rr, int64_t n) {
+  return (rr > 0.0 && rr < 1e300) ? (float)(1.0 / sqrt(rr / (double)n)) : 1.f;
+}
+
 struct StageIn {
   Grid h;  // pair frame of the colour-split arrays
   const float *s, *dred;
-  float *bred, *bblack, *xred;  // bred null: no staging
+  void *bred, *bblack, *xred;  // bred null: no staging
+  int f16;                     // the three arrays hold FP16 of the scaled input (else FP32)
 };
 
-__device__ __forceinline__ void stage_in(const StageIn &st, const Grid &g, int i, int j, int k, int64_t c,
-                                         float rv) {
-  const float bv = st.s[c] * rv;
-  const int64_t qh = st.h.idx((i - 1) >> 1, j, k);
-  if (((i + j + k) & 1) == 0) {
-    st.bred[qh] = bv;
-    st.xred[qh] = (bv + 0.f) / st.dred[qh];
+template <typename XT>
+__device__ __forceinline__ void stage_cell(XT *bred, XT *bblack, XT *xred, const float *dred, int64_t qh, bool red,
+                                           float bv) {
+  if (red) {
+    bred[qh] = to_x<XT>(bv);
+    xred[qh] = to_x<XT>((bv + 0.f) / dred[qh]);
   } else {
-    st.bblack[qh] = bv;
+    bblack[qh] = to_x<XT>(bv);
   }
+}
+
+// (sg: the input scale, 1 for FP32 storage)
+__device__ __forceinline__ void stage_in(const StageIn &st, const Grid &g, int i, int j, int k, int64_t c,
+                                         float rv, float sg) {
+  const float bv = st.s[c] * rv * sg;
+  const int64_t qh = st.h.idx((i - 1) >> 1, j, k);
+  const bool red = ((i + j + k) & 1) == 0;
+  if (st.f16)
+    stage_cell((_Float16 *)st.bred, (_Float16 *)st.bblack, (_Float16 *)st.xred, st.dred, qh, red, bv);
+  else
+    stage_cell((float *)st.bred, (float *)st.bblack, (float *)st.xred, st.dred, qh, red, bv);
+}
+// the input scale of the V-cycle that the staging kernel feeds, kept for its
+// output kernel (one thread writes it)
+__device__ __forceinline__ float stage_scale(const StageIn &st, FcgCtl *ctl, const Grid &g, double rr) {
+  const float sg = st.f16 ? unit_rms_scale(rr, (int64_t)g.nx * g.ny * g.nz) : 1.f;
+  if (st.bred && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0)
+    ctl->sig = sg;
+  return sg;
 }
 
 // r32 = r - (r.n/n.n) n; d = 0; sums r32.r32 and arms the inner loop; with
@@ -142,6 +182,7 @@ __global__ __launch_bounds__(256) void k_r32(Grid g, const double *__restrict__ 
                                              double rel_in, int maxit, RedSlot rs, StageIn st) {
   CELL_IJK;
   const double mu = ctl->rout[1] / ctl->rout[2];
+  const float sg = stage_scale(st, ctl, g, ctl->rout[0]);
   double acc[1] = {0.0};
   if (i <= g.nx && j <= g.ny)
     for (int k = kz0; k <= g.nz; k += gridDim.z) {
@@ -150,7 +191,7 @@ __global__ __launch_bounds__(256) void k_r32(Grid g, const double *__restrict__ 
       r32[c] = rv;
       d32[c] = 0.f;
       acc[0] += (double)rv * (double)rv;
-      if (st.bred) stage_in(st, g, i, j, k, c, rv);
+      if (st.bred) stage_in(st, g, i, j, k, c, rv, sg);
     }
   __shared__ double lds[SULI_LDS_RED];
   block_reduce<1>(acc, lds);
@@ -308,6 +349,8 @@ __global__ __launch_bounds__(256) void k_fcg2(Grid g, const float *__restrict__ 
   CELL_IJK;
   const double pq = ctl->k1[0];
   const float alpha = pq != 0.0 ? (float)(ctl->k1[1] / pq) : 0.f;
+  // (the last residual norm on the device: the pass's initial one before its first update)
+  const float sg = stage_scale(st, ctl, g, ctl->it == 0 ? ctl->rin0[0] : ctl->k2[0]);
   double acc[2] = {0.0, 0.0};
   if (i <= g.nx && j <= g.ny)
     for (int k = kz0; k <= g.nz; k += gridDim.z) {
@@ -318,7 +361,7 @@ __global__ __launch_bounds__(256) void k_fcg2(Grid g, const float *__restrict__ 
       r[c] = rv;
       acc[0] += (double)rv * (double)rv;
       acc[1] += (double)rv * (double)qv;
-      if (st.bred) stage_in(st, g, i, j, k, c, rv);
+      if (st.bred) stage_in(st, g, i, j, k, c, rv, sg);
     }
   __shared__ double lds[2 * SULI_LDS_RED];
   block_reduce<2>(acc, lds);
@@ -697,10 +740,11 @@ __global__ __launch_bounds__(256) void k_mg_gs_out(Grid g, const CT *__restrict_
 // y- and z-neighbours are pair p of the neighbouring rows.
 
 // first pre-smoothing half sweep from x = 0, with the input scaling b = s r
+template <typename XT>
 __global__ __launch_bounds__(256) void k_mg0_in(Grid g, Grid h, const float *__restrict__ dred,
                                                 const float *__restrict__ s, const float *__restrict__ r,
-                                                float *__restrict__ bred, float *__restrict__ bblack,
-                                                float *__restrict__ xred, const int *active) {
+                                                XT *__restrict__ bred, XT *__restrict__ bblack,
+                                                XT *__restrict__ xred, const int *active) {
   if (!*active) return;
   int bx, by, bz;
   xcd_band_block(bx, by, bz);
@@ -711,23 +755,20 @@ __global__ __launch_bounds__(256) void k_mg0_in(Grid g, Grid h, const float *__r
   const bool two = i1 < g.nx;
   const float b1 = s[c1] * r[c1], b2 = two ? s[c1 + 1] * r[c1 + 1] : 0.f;
   if (((i1 + j + k) & 1) == 0) {  // the first cell is red
-    bred[q] = b1;
-    if (two) bblack[q] = b2;
-    xred[q] = (b1 + 0.f) / dred[q];
+    stage_cell(bred, bblack, xred, dred, q, true, b1);
+    if (two) stage_cell(bred, bblack, xred, dred, q, false, b2);
   } else {
-    bblack[q] = b1;
-    if (two) {
-      bred[q] = b2;
-      xred[q] = (b2 + 0.f) / dred[q];
-    }
+    stage_cell(bred, bblack, xred, dred, q, false, b1);
+    if (two) stage_cell(bred, bblack, xred, dred, q, true, b2);
   }
 }
 
 // half sweep of colour `color` (x of the colour from b and the other colour's x)
+template <typename XT>
 __global__ __launch_bounds__(256) void k_mg0_gs(Grid g, Grid h, const uint16_t *__restrict__ cx,
                                                 const uint16_t *__restrict__ cy, const uint16_t *__restrict__ cz,
-                                                const float *__restrict__ bcol, const float *__restrict__ xo,
-                                                float *__restrict__ xcol, int color, const int *active) {
+                                                const XT *__restrict__ bcol, const XT *__restrict__ xo,
+                                                XT *__restrict__ xcol, int color, const int *active) {
   if (!*active) return;
   int bx, by, bz;
   xcd_band_block(bx, by, bz);
@@ -737,17 +778,19 @@ __global__ __launch_bounds__(256) void k_mg0_gs(Grid g, Grid h, const uint16_t *
   const int i = 1 + 2 * p + sec;
   if (i > g.nx) return;
   const int64_t c = g.idx(i, j, k), q = h.idx(p, j, k);
-  const float s = mgc(cx[c + 1]) * xo[q + sec] + mgc(cx[c]) * xo[q - 1 + sec] + mgc(cy[c + g.sx]) * xo[q + h.sx] +
-                  mgc(cy[c]) * xo[q - h.sx] + mgc(cz[c + g.sz]) * xo[q + h.sz] + mgc(cz[c]) * xo[q - h.sz];
-  xcol[q] = (bcol[q] + s) / mg_dg(g, c, cx, cy, cz);
+  const float s = mgc(cx[c + 1]) * mgc(xo[q + sec]) + mgc(cx[c]) * mgc(xo[q - 1 + sec]) +
+                  mgc(cy[c + g.sx]) * mgc(xo[q + h.sx]) + mgc(cy[c]) * mgc(xo[q - h.sx]) +
+                  mgc(cz[c + g.sz]) * mgc(xo[q + h.sz]) + mgc(cz[c]) * mgc(xo[q - h.sz]);
+  xcol[q] = to_x<XT>((mgc(bcol[q]) + s) / mg_dg(g, c, cx, cy, cz));
 }
 
 // first post-smoothing half sweep (black) with the prolongation folded into
 // the red neighbours it reads (k_mg_gs_prolong)
+template <typename XT>
 __global__ __launch_bounds__(256) void k_mg0_prolong(Grid g, Grid h, Grid gc, const uint16_t *__restrict__ cx,
                                                      const uint16_t *__restrict__ cy,
-                                                     const uint16_t *__restrict__ cz, const float *__restrict__ bblack,
-                                                     const float *__restrict__ xred, float *__restrict__ xblack,
+                                                     const uint16_t *__restrict__ cz, const XT *__restrict__ bblack,
+                                                     const XT *__restrict__ xred, XT *__restrict__ xblack,
                                                      const float *__restrict__ xc, const int *active) {
   if (!*active) return;
   int bx, by, bz;
@@ -759,22 +802,24 @@ __global__ __launch_bounds__(256) void k_mg0_prolong(Grid g, Grid h, Grid gc, co
   if (i > g.nx) return;
   const int64_t c = g.idx(i, j, k), q = h.idx(p, j, k);
   const auto X = [&](int64_t qn, int in, int jn, int kn) {
-    return xred[qn] + xc[gc.idx((in + 1) >> 1, (jn + 1) >> 1, (kn + 1) >> 1)];
+    return mgc(xred[qn]) + xc[gc.idx((in + 1) >> 1, (jn + 1) >> 1, (kn + 1) >> 1)];
   };
   const float s = mgc(cx[c + 1]) * X(q + sec, i + 1, j, k) + mgc(cx[c]) * X(q - 1 + sec, i - 1, j, k) +
                   mgc(cy[c + g.sx]) * X(q + h.sx, i, j + 1, k) + mgc(cy[c]) * X(q - h.sx, i, j - 1, k) +
                   mgc(cz[c + g.sz]) * X(q + h.sz, i, j, k + 1) + mgc(cz[c]) * X(q - h.sz, i, j, k - 1);
-  xblack[q] = (bblack[q] + s) / mg_dg(g, c, cx, cy, cz);
+  xblack[q] = to_x<XT>((mgc(bblack[q]) + s) / mg_dg(g, c, cx, cy, cz));
 }
 
 // last post-smoothing half sweep (red) with the output scaling z = s x on both
-// cells of the pair (frame layout) and the block sums of z.q (k_mg_gs_out)
+// cells of the pair (frame layout) and the block sums of z.q (k_mg_gs_out);
+// with sig set, z = s x / *sig
+template <typename XT>
 __global__ __launch_bounds__(256) void k_mg0_out(Grid g, Grid h, const uint16_t *__restrict__ cx,
                                                  const uint16_t *__restrict__ cy, const uint16_t *__restrict__ cz,
-                                                 const float *__restrict__ bred, const float *__restrict__ xblack,
+                                                 const XT *__restrict__ bred, const XT *__restrict__ xblack,
                                                  const float *__restrict__ s, float *__restrict__ z,
                                                  const float *__restrict__ q, const int *first, double *partials,
-                                                 const int *active) {
+                                                 const int *active, const float *sig) {
   if (!*active) return;
   int bx, by, bz;
   xcd_band_block(bx, by, bz);
@@ -782,6 +827,7 @@ __global__ __launch_bounds__(256) void k_mg0_out(Grid g, Grid h, const uint16_t 
   const int i1 = 1 + 2 * p;
   double acc[1] = {0.0};
   const bool dot = partials && !*first;
+  const float isg = sig ? 1.f / *sig : 1.f;  // the input scale divided out (FP16 storage)
   if (j <= g.ny && k <= g.nz && i1 <= g.nx) {
     const int sec = (1 + j + k) & 1;  // colour 0
     const int i = i1 + sec;
@@ -789,20 +835,20 @@ __global__ __launch_bounds__(256) void k_mg0_out(Grid g, Grid h, const uint16_t 
     const bool two = i1 < g.nx;
     if (i <= g.nx) {
       const int64_t c = c1 + sec, cb = sec ? c1 : c1 + 1;
-      const float sx = mgc(cx[c + 1]) * xblack[qh + sec] + mgc(cx[c]) * xblack[qh - 1 + sec] +
-                       mgc(cy[c + g.sx]) * xblack[qh + h.sx] + mgc(cy[c]) * xblack[qh - h.sx] +
-                       mgc(cz[c + g.sz]) * xblack[qh + h.sz] + mgc(cz[c]) * xblack[qh - h.sz];
-      const float xr = (bred[qh] + sx) / mg_dg(g, c, cx, cy, cz);
-      const float zr = s[c] * xr;
+      const float sx = mgc(cx[c + 1]) * mgc(xblack[qh + sec]) + mgc(cx[c]) * mgc(xblack[qh - 1 + sec]) +
+                       mgc(cy[c + g.sx]) * mgc(xblack[qh + h.sx]) + mgc(cy[c]) * mgc(xblack[qh - h.sx]) +
+                       mgc(cz[c + g.sz]) * mgc(xblack[qh + h.sz]) + mgc(cz[c]) * mgc(xblack[qh - h.sz]);
+      const float xr = (mgc(bred[qh]) + sx) / mg_dg(g, c, cx, cy, cz);
+      const float zr = s[c] * xr * isg;
       z[c] = zr;
       if (dot) acc[0] += (double)zr * (double)q[c];
       if (two) {
-        const float zb = s[cb] * xblack[qh];
+        const float zb = s[cb] * mgc(xblack[qh]) * isg;
         z[cb] = zb;
         if (dot) acc[0] += (double)zb * (double)q[cb];
       }
     } else {  // the pair's only cell is black
-      const float zb = s[c1] * xblack[qh];
+      const float zb = s[c1] * mgc(xblack[qh]) * isg;
       z[c1] = zb;
       if (dot) acc[0] += (double)zb * (double)q[c1];
     }
@@ -814,11 +860,12 @@ __global__ __launch_bounds__(256) void k_mg0_out(Grid g, Grid h, const uint16_t 
 }
 
 // restriction of the residual to level 1 (k_mg_restrict on the split arrays)
+template <typename XT>
 __global__ __launch_bounds__(256) void k_mg0_restrict(Grid gf, Grid h, Grid gc, const uint16_t *__restrict__ cx,
                                                       const uint16_t *__restrict__ cy,
-                                                      const uint16_t *__restrict__ cz, const float *__restrict__ b0,
-                                                      const float *__restrict__ b1, const float *__restrict__ x0,
-                                                      const float *__restrict__ x1, float *__restrict__ bc,
+                                                      const uint16_t *__restrict__ cz, const XT *__restrict__ b0,
+                                                      const XT *__restrict__ b1, const XT *__restrict__ x0,
+                                                      const XT *__restrict__ x1, float *__restrict__ bc,
                                                       float *__restrict__ xc, const int *active) {
   if (!*active) return;
   int bx, by, bz;
@@ -833,11 +880,12 @@ __global__ __launch_bounds__(256) void k_mg0_restrict(Grid gf, Grid h, Grid gc, 
     const int64_t c = c0 + di + dj * (int64_t)gf.sx + dk * (int64_t)gf.sz;
     const int64_t q = q0 + dj * (int64_t)h.sx + dk * (int64_t)h.sz;
     const bool red = ((di + dj + dk + 2 * I - 1 + 2 * J - 1 + 2 * K - 1) & 1) == 0;
-    const float *xs = red ? x0 : x1, *xo = red ? x1 : x0;
-    const float bv = red ? b0[q] : b1[q];
-    const float ax = mgc(cx[c + 1]) * xo[q + di] + mgc(cx[c]) * xo[q - 1 + di] + mgc(cy[c + gf.sx]) * xo[q + h.sx] +
-                     mgc(cy[c]) * xo[q - h.sx] + mgc(cz[c + gf.sz]) * xo[q + h.sz] + mgc(cz[c]) * xo[q - h.sz];
-    s += bv - (mg_dg(gf, c, cx, cy, cz) * xs[q] - ax);
+    const XT *xs = red ? x0 : x1, *xo = red ? x1 : x0;
+    const float bv = mgc(red ? b0[q] : b1[q]);
+    const float ax = mgc(cx[c + 1]) * mgc(xo[q + di]) + mgc(cx[c]) * mgc(xo[q - 1 + di]) +
+                     mgc(cy[c + gf.sx]) * mgc(xo[q + h.sx]) + mgc(cy[c]) * mgc(xo[q - h.sx]) +
+                     mgc(cz[c + gf.sz]) * mgc(xo[q + h.sz]) + mgc(cz[c]) * mgc(xo[q - h.sz]);
+    s += bv - (mg_dg(gf, c, cx, cy, cz) * mgc(xs[q]) - ax);
   }
   const int64_t cc = gc.idx(I, J, K);
   bc[cc] = s;
@@ -1338,6 +1386,8 @@ struct PoissonFCG {
   // finest level split by colour (fused kernels): pair frame, x and b per colour, red diagonal
   Grid hg{};
   float *x0c = nullptr, *x1c = nullptr, *b0c = nullptr, *b1c = nullptr, *d0c = nullptr;
+  // the same in FP16 (arrays of their own, so each form's ghost pairs stay zero)
+  _Float16 *x0h = nullptr, *x1h = nullptr, *b0h = nullptr, *b1h = nullptr;
   bool setup_done = false;  // operator copies and hierarchy built by a solve
   // the density epoch (Ctx::rho_epoch) and the dt they were built for
   uint64_t op_epoch = 0;
@@ -1404,6 +1454,11 @@ int fcg_create(Ctx &c) {
     if (falloc(c, &f->x0c, h.total) || falloc(c, &f->x1c, h.total) || falloc(c, &f->b0c, h.total) ||
         falloc(c, &f->b1c, h.total) || falloc(c, &f->d0c, h.total))
       return 1;
+    for (_Float16 **a : {&f->x0h, &f->x1h, &f->b0h, &f->b1h}) {
+      HIPCHK(hipMalloc((void **)a, (size_t)h.total * sizeof(_Float16)));
+      HIPCHK(hipMemsetAsync(*a, 0, (size_t)h.total * sizeof(_Float16), c.st));
+      c.allocs.push_back(*a);
+    }
   }
   for (;;) {
     const Grid &gf = f->lv.back().g;
@@ -1481,6 +1536,17 @@ static int mg_setup(Ctx &c, bool level0) {
   return 0;
 }
 
+// Whether the V-cycle of inner iteration `iter` of a pass holds the finest
+// level's iterate and right-hand side in FP16 (when its input is staged by
+// the kernel that produced the residual, which also scales it).  The
+// rounding makes the preconditioner slightly nonlinear, which FCG(1)
+// tolerates only down to a residual reduction of about the rounding: with
+// BF16 (2^-8) the benchmark's cold starts took 14 -> 16 iterations.
+#ifndef SULI_MG_F16_ITERS
+#define SULI_MG_F16_ITERS 1000
+#endif
+static bool fine_f16(const Ctx &c, int iter) { return c.mg_fuse_fine == 2 && iter < SULI_MG_F16_ITERS; }
+
 // z.q of the preconditioned FCG beta, computed by the V-cycle's last kernel
 struct ZqOut {
   const float *q;
@@ -1501,7 +1567,7 @@ struct ZqOut {
 // levels above e then run their own launches even where they would fit the
 // single-work-group kernel).
 static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const ZqOut *zq = nullptr,
-                      int net_level = 0, bool staged = false) {
+                      int net_level = 0, bool staged = false, bool f16 = false) {
   PoissonFCG &f = *c.fcg;
   const dim3 b2(SULI_BX, SULI_BY, 1);
   const int nu1 = SULI_MG_NU, nu2 = SULI_MG_NU;
@@ -1519,13 +1585,19 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     }
   };
   const bool ff = c.mg_fuse_fine && S >= 1;  // the finest level has launches of its own
+  const bool xb = ff && staged && f16;  // its colour-split x and b in FP16, the input scaled
   // one red-black half sweep of level lf (BF16 faces where the level has them)
   auto gs = [&](MgLevel &lf, const char *name, int color, int zero_x) {
     ProfKernel pk(c, name);
     if (ff && &lf == &f.lv[0]) {  // finest level, colour-split arrays (never from x = 0 here)
-      SULI_TIMED_LAUNCH(pk, k_mg0_gs, half_cells(lf.g), b2, 0, c.st, lf.g, f.hg, (const uint16_t *)lf.hx,
-                        (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)(color ? f.b1c : f.b0c),
-                        (const float *)(color ? f.x0c : f.x1c), color ? f.x1c : f.x0c, color, active);
+      if (xb)
+        SULI_TIMED_LAUNCH(pk, k_mg0_gs<_Float16>, half_cells(lf.g), b2, 0, c.st, lf.g, f.hg, (const uint16_t *)lf.hx,
+                          (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const _Float16 *)(color ? f.b1h : f.b0h),
+                          (const _Float16 *)(color ? f.x0h : f.x1h), color ? f.x1h : f.x0h, color, active);
+      else
+        SULI_TIMED_LAUNCH(pk, k_mg0_gs<float>, half_cells(lf.g), b2, 0, c.st, lf.g, f.hg, (const uint16_t *)lf.hx,
+                          (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)(color ? f.b1c : f.b0c),
+                          (const float *)(color ? f.x0c : f.x1c), color ? f.x1c : f.x0c, color, active);
       return;
     }
     if (lf.hx)
@@ -1541,7 +1613,7 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     // the finest level's input was staged from r by the kernel that produced r
   } else if (ff) {
     ProfScope ps(c, "mg_gs_l0_in");
-    hipLaunchKernelGGL(k_mg0_in, half_cells(c.g), b2, 0, c.st, c.g, f.hg, (const float *)f.d0c,
+    hipLaunchKernelGGL(k_mg0_in<float>, half_cells(c.g), b2, 0, c.st, c.g, f.hg, (const float *)f.d0c,
                        (const float *)f.sD32, r, f.b0c, f.b1c, f.x0c, active);
   } else {
     ProfScope ps(c, "mg_scale");
@@ -1570,10 +1642,16 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
       gs(lf, gsn, 1, 0);
     }
     ProfScope ps(c, l == 0 ? "mg_restrict_l0" : "mg_restrict");
-    if (ff && l == 0)
-      hipLaunchKernelGGL(k_mg0_restrict, cells_full(lc.g), b2, 0, c.st, lf.g, f.hg, lc.g, (const uint16_t *)lf.hx,
-                         (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)f.b0c, (const float *)f.b1c,
-                         (const float *)f.x0c, (const float *)f.x1c, lc.b, lc.x, active);
+    if (ff && l == 0 && xb)
+      hipLaunchKernelGGL(k_mg0_restrict<_Float16>, cells_full(lc.g), b2, 0, c.st, lf.g, f.hg, lc.g,
+                         (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
+                         (const _Float16 *)f.b0h, (const _Float16 *)f.b1h, (const _Float16 *)f.x0h,
+                         (const _Float16 *)f.x1h, lc.b, lc.x, active);
+    else if (ff && l == 0)
+      hipLaunchKernelGGL(k_mg0_restrict<float>, cells_full(lc.g), b2, 0, c.st, lf.g, f.hg, lc.g,
+                         (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
+                         (const float *)f.b0c, (const float *)f.b1c, (const float *)f.x0c, (const float *)f.x1c,
+                         lc.b, lc.x, active);
     else if (lf.hx)
       hipLaunchKernelGGL((k_mg_restrict<uint16_t>), cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, (const uint16_t *)lf.hx,
                          (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)lf.b,
@@ -1625,10 +1703,14 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     const int nu = l == 0 ? SULI_MG_NU0 : (l == 1 ? SULI_MG_NU1 : nu2);
     if (c.mg_fuse_fine) {
       ProfScope ps(c, l == 0 ? "mg_gs_l0_prolong" : "mg_gs_prolong");
-      if (l == 0)  // (c.mg_fuse_fine and S >= 1: ff)
-        hipLaunchKernelGGL(k_mg0_prolong, half_cells(lf.g), b2, 0, c.st, lf.g, f.hg, lc.g, (const uint16_t *)lf.hx,
-                           (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)f.b1c,
-                           (const float *)f.x0c, f.x1c, xcoarse, active);
+      if (l == 0 && xb)  // (c.mg_fuse_fine and S >= 1: ff)
+        hipLaunchKernelGGL(k_mg0_prolong<_Float16>, half_cells(lf.g), b2, 0, c.st, lf.g, f.hg, lc.g,
+                           (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
+                           (const _Float16 *)f.b1h, (const _Float16 *)f.x0h, f.x1h, xcoarse, active);
+      else if (l == 0)
+        hipLaunchKernelGGL(k_mg0_prolong<float>, half_cells(lf.g), b2, 0, c.st, lf.g, f.hg, lc.g,
+                           (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
+                           (const float *)f.b1c, (const float *)f.x0c, f.x1c, xcoarse, active);
       else if (lf.hx)
         hipLaunchKernelGGL((k_mg_gs_prolong<uint16_t>), half_cells(lf.g), b2, 0, c.st, lf.g, lc.g,
                            (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
@@ -1652,9 +1734,15 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     double *part = zq ? zq->rs.partials : nullptr;
     const float *q = zq ? zq->q : nullptr;
     const int *first = zq ? zq->first : active;
-    hipLaunchKernelGGL(k_mg0_out, half_cells(c.g), b2, 0, c.st, c.g, f.hg, (const uint16_t *)l0.hx,
-                       (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const float *)f.b0c, (const float *)f.x1c,
-                       (const float *)f.sD32, z, q, first, part, active);
+    if (xb)
+      hipLaunchKernelGGL(k_mg0_out<_Float16>, half_cells(c.g), b2, 0, c.st, c.g, f.hg, (const uint16_t *)l0.hx,
+                         (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const _Float16 *)f.b0h,
+                         (const _Float16 *)f.x1h, (const float *)f.sD32, z, q, first, part, active,
+                         (const float *)((const char *)f.ctl + offsetof(FcgCtl, sig)));
+    else
+      hipLaunchKernelGGL(k_mg0_out<float>, half_cells(c.g), b2, 0, c.st, c.g, f.hg, (const uint16_t *)l0.hx,
+                         (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const float *)f.b0c,
+                         (const float *)f.x1c, (const float *)f.sD32, z, q, first, part, active, nullptr);
     if (zq) launch_finish<1>(c.st, half_cells(c.g), zq->rs, EpiActive{f.ctl});
     return;
   }
@@ -1670,7 +1758,8 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
   }
 }
 
-// Apply the preconditioner selected for this solve: z = M r.
+// Apply the preconditioner selected for this solve: z = M r
+// (not staged: the FP32 form)
 static void apply_precond(Ctx &c, int pc, const float *r, float *z, const int *active) {
   if (pc == SULI_PRECOND_LEARNED) {
     const int e = unet_entry(c.fcg->unet);
@@ -1687,9 +1776,10 @@ static constexpr int kLearnedPassCap = 64;
 
 // One FCG(m) iteration with the learned preconditioner; the new direction
 // goes to slot `slot` of the ring of held directions.
-static void fcgm_iteration(Ctx &c, PoissonFCG &f, int slot, dim3 gk, const int *active) {
+static void fcgm_iteration(Ctx &c, PoissonFCG &f, int iter, dim3 gk, const int *active) {
   const Grid &g = c.g;
   const dim3 b2(SULI_BX, SULI_BY, 1);
+  const int slot = iter % SULI_FCG_M;
   apply_precond(c, SULI_PRECOND_LEARNED, f.r32, f.z32, active);
   {
     ProfScope ps(c, "fcgm_spmv_dots");
@@ -1887,13 +1977,16 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     // from K2
     const bool stage = use_pc && (pc != SULI_PRECOND_LEARNED || learned_m1) && c.mg_fuse_fine && f.x0c && f.b0c &&
                        f.b1c && f.d0c;
-    const StageIn sti = stage ? StageIn{f.hg, f.sD32, f.d0c, f.b0c, f.b1c, f.x0c} : StageIn{};
+    // (in the form of the V-cycle that reads it)
+    const StageIn sti_f = stage ? StageIn{f.hg, f.sD32, f.d0c, f.b0c, f.b1c, f.x0c, 0} : StageIn{};
+    const StageIn sti_h = stage ? StageIn{f.hg, f.sD32, f.d0c, f.b0h, f.b1h, f.x0h, 1} : StageIn{};
+    const auto sti = [&](int it) { return fine_f16(c, it) ? sti_h : sti_f; };
     const int pass_max = pc == SULI_PRECOND_LEARNED ? std::min(kLearnedPassCap, maxit_in - inner_total)
                                                     : maxit_in - inner_total;
     {
       ProfScope ps(c, "fcg_project");
       const RedSlot r3{c.partials, CTL_PTR(f.ctl, rin0)};
-      hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rel_in, pass_max, r3, sti);
+      hipLaunchKernelGGL(k_r32, gk, b2, 0, c.st, g, f.r, c.sD, f.r32, f.d32, f.ctl, rel_in, pass_max, r3, sti(0));
       const bool capped = pc == SULI_PRECOND_LEARNED;
       launch_finish<1>(c.st, gk, r3,
                        EpiR32{f.ctl, rel_in, tol, pass_max,
@@ -1913,14 +2006,14 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
         if (iter > 0) unet_capture(c, f.unet, f.r32, active, iter);
         f.cur_iter = iter;
         if (pc == SULI_PRECOND_LEARNED && !learned_m1) {
-          fcgm_iteration(c, f, iter % SULI_FCG_M, gk, active);
+          fcgm_iteration(c, f, iter, gk, active);
           continue;
         }
         const float *zin = f.r32;
         if (use_pc) {
           const ZqOut zo{f.q32, (const int *)((char *)f.ctl + offsetof(FcgCtl, first)),
                          RedSlot{c.partials, CTL_PTR(f.ctl, zq)}};
-          mg_vcycle(c, f.r32, f.z32, active, &zo, pc == SULI_PRECOND_LEARNED ? entry : 0, stage);
+          mg_vcycle(c, f.r32, f.z32, active, &zo, pc == SULI_PRECOND_LEARNED ? entry : 0, stage, fine_f16(c, iter));
           zin = f.z32;
         }
         {
@@ -1936,7 +2029,7 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
         {
           ProfScope ps(c, "fcg_update");
           const RedSlot r2{c.partials, CTL_PTR(f.ctl, k2)};
-          hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.p32, f.q32, f.r32, f.d32, f.ctl, r2, sti);
+          hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.p32, f.q32, f.r32, f.d32, f.ctl, r2, sti(iter + 1));
           launch_finish<2>(c.st, gk, r2, EpiK2{f.ctl});
         }
       }

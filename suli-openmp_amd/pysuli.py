@@ -315,7 +315,9 @@ class Solver:
     def set_mg(self, fuse_max_cells=-1, coarse_sweeps=-1, fuse_fine=-1):
         """Multigrid tuning: fuse the half sweeps of levels of at most
         fuse_max_cells cells (0: off); sweeps on the coarsest level; scaling,
-        prolongation and z.q inside the neighbouring half sweeps (1/0); -1 keeps."""
+        prolongation and z.q inside the neighbouring half sweeps (1; 2, the
+        default: also the finest level's iterate and right-hand side in FP16
+        in the FCG iteration's V-cycles; 0: separate launches); -1 keeps."""
         _check(lib().suli_set_mg(int(fuse_max_cells), int(coarse_sweeps), int(fuse_fine)), "suli_set_mg")
 
     def set_fast_stencils(self, on):

@@ -134,6 +134,46 @@ def test_fused_multigrid_matches_launch_per_sweep(grid):
         sv.close()
 
 
+@pytest.mark.parametrize("grid", [(128, 64, 64), (96, 48, 40)])
+def test_f16_finest_level_iterate(grid):
+    """The finest multigrid level's colour-split iterate and right-hand side
+    in FP16 with the V-cycle's input scaled (suli_set_mg fuse_fine 2, the
+    default) against FP32 (fuse_fine 1), on cold solves to the 1e-8 FP64
+    tolerance: the same FCG iteration count to one, both converged, the same
+    pressure to the solve's accuracy; a repeated solve gives the same
+    pressure bit for bit; fuse_fine above 2 is refused."""
+    nx, ny, nz = grid
+    h = 1.0 / nx
+    sv = pysuli.Solver(nx, ny, nz, h, precision=pysuli.PREC_MIXED, poisson=pysuli.POISSON_FCG,
+                       precond=pysuli.PRECOND_GMG, tol_rel=1e-8)
+    try:
+        sv.init_dambreak(0.3, 0.4)
+        sv.contorno(1)
+        sv.contorno(3)
+        sv.step(2e-4)
+        sv.set_guess(0)
+        p0 = sv.download("prd1")
+        its, p = {}, {}
+        for form in (1, 2, 2):
+            sv.set_mg(fuse_fine=form)
+            sv.upload("prd1", 0 * p0)
+            st = sv.graddin(2e-4)
+            if form in its:  # the repeated FP16 solve
+                assert st["iters"] == its[form]["iters"] and np.array_equal(sv.download("prd1"), p[form])
+                continue
+            its[form], p[form] = st, sv.download("prd1")
+        dp = np.abs(p[2] - p[1]).max() / np.abs(p[1]).max()
+        print(f"{grid}: iterations FP32 {its[1]['iters']} ({its[1]['outer']} passes), FP16 {its[2]['iters']} "
+              f"({its[2]['outer']} passes); max |p_f16 - p_f32| / max |p| = {dp:.2e}")
+        assert its[1]["converged"] and its[2]["converged"]
+        assert abs(its[2]["iters"] - its[1]["iters"]) <= 1
+        assert dp <= 1e-6, dp
+        with pytest.raises(pysuli.SuliError):
+            sv.set_mg(fuse_fine=3)
+    finally:
+        sv.close()
+
+
 def test_extrapolated_guess_saves_iterations():
     """The projections started from the stage-2 pressures extrapolated in
     time (suli_set_guess(1), the default) begin with a smaller residual than
