@@ -727,6 +727,14 @@ __global__ __launch_bounds__(256) void k_mg_gs_out(Grid g, const CT *__restrict_
 }
 
 // ---- the finest level with its iterate and right-hand side split by colour ---------
+// sum_f c_f x_f of a half sweep as explicit fused multiply-adds in a fixed
+// order, so that every kernel forming it (one or two cells per thread)
+// rounds it the same way
+__device__ __forceinline__ float nb_sum6(float c1, float x1, float c2, float x2, float c3, float x3, float c4,
+                                         float x4, float c5, float x5, float c6, float x6) {
+  return fmaf(c6, x6, fmaf(c5, x5, fmaf(c4, x4, fmaf(c3, x3, fmaf(c2, x2, c1 * x1)))));
+}
+
 // A half sweep reads the other colour's x and writes its own colour's x and
 // reads its b: in the frame layout each of those touches every cache line of
 // the array, twice the bytes the colour needs.  With the fused kernels the
@@ -778,10 +786,84 @@ __global__ __launch_bounds__(256) void k_mg0_gs(Grid g, Grid h, const uint16_t *
   const int i = 1 + 2 * p + sec;
   if (i > g.nx) return;
   const int64_t c = g.idx(i, j, k), q = h.idx(p, j, k);
-  const float s = mgc(cx[c + 1]) * mgc(xo[q + sec]) + mgc(cx[c]) * mgc(xo[q - 1 + sec]) +
-                  mgc(cy[c + g.sx]) * mgc(xo[q + h.sx]) + mgc(cy[c]) * mgc(xo[q - h.sx]) +
-                  mgc(cz[c + g.sz]) * mgc(xo[q + h.sz]) + mgc(cz[c]) * mgc(xo[q - h.sz]);
+  const float s = nb_sum6(mgc(cx[c + 1]), mgc(xo[q + sec]), mgc(cx[c]), mgc(xo[q - 1 + sec]), mgc(cy[c + g.sx]),
+                          mgc(xo[q + h.sx]), mgc(cy[c]), mgc(xo[q - h.sx]), mgc(cz[c + g.sz]), mgc(xo[q + h.sz]),
+                          mgc(cz[c]), mgc(xo[q - h.sz]));
   xcol[q] = to_x<XT>((mgc(bcol[q]) + s) / mg_dg(g, c, cx, cy, cz));
+}
+
+// The same half sweep with two cells of the colour per thread: those of
+// pairs 2t and 2t+1 of a row, i0 = 2(2t)+1+sec and i0+2.  In a row sec is
+// uniform, the pair index 2t is even and the face index of i0 is sec modulo
+// 4 (the frame's x offset is 3 modulo 4, its pitches multiples of 4; checked
+// on the host), so the two cells' x, b and other-colour neighbours in y and
+// z come as aligned pairs, and each face array's values as one aligned
+// group of four BF16 (x faces of i0..i0+3, or of i0-1..i0+2 and i0+3 when
+// sec = 1; y and z faces at positions sec and sec+2).  Each cell's update is
+// k_mg0_gs's, term for term (nb_sum6 fixes the rounding of both).
+template <typename XT> struct Pair2;
+template <> struct Pair2<float> { typedef float __attribute__((ext_vector_type(2))) t; };
+template <> struct Pair2<_Float16> { typedef _Float16 __attribute__((ext_vector_type(2))) t; };
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+// the four BF16 of the aligned group at a, element e (0..3)
+struct Bf4 {
+  uint2 w;
+  __device__ __forceinline__ float at(int e) const {
+    const uint32_t h = e >= 2 ? w.y : w.x;
+    return (e & 1) ? bf_hi(h) : bf_lo(h);
+  }
+};
+__device__ __forceinline__ Bf4 ld_bf4(const uint16_t *__restrict__ p, fidx a) {
+  return Bf4{*(const uint2 *)(p + a)};
+}
+
+template <typename XT>
+__global__ __launch_bounds__(256) void k_mg0_gs2(Grid g, Grid h, const uint16_t *__restrict__ cx,
+                                                 const uint16_t *__restrict__ cy, const uint16_t *__restrict__ cz,
+                                                 const XT *__restrict__ bcol, const XT *__restrict__ xo,
+                                                 XT *__restrict__ xcol, int color, const int *active) {
+  typedef typename Pair2<XT>::t X2;
+  if (!*active) return;
+  int bx, by, bz;
+  xcd_band_block(bx, by, bz);
+  const int p0 = 2 * (bx * SULI_BX + threadIdx.x), j = 1 + by * SULI_BY + threadIdx.y, k = 1 + bz;
+  if (j > g.ny || k > g.nz) return;
+  const int sec = (1 + j + k + color) & 1;
+  const int i0 = 1 + 2 * p0 + sec;
+  if (i0 > g.nx) return;
+  const bool two = i0 + 2 <= g.nx;
+  const fidx c0 = g.idx(i0, j, k), a = c0 - sec, q0 = h.idx(p0, j, k);
+  const X2 xm = *(const X2 *)(xo + q0);
+  const float xe = mgc(xo[q0 - 1 + 3 * sec]);  // the x-neighbour outside the pair of pairs
+  const X2 ysn = *(const X2 *)(xo + q0 - h.sx), ynn = *(const X2 *)(xo + q0 + h.sx);
+  const X2 zsn = *(const X2 *)(xo + q0 - h.sz), znn = *(const X2 *)(xo + q0 + h.sz);
+  const X2 bb = *(const X2 *)(bcol + q0);
+  const Bf4 fx = ld_bf4(cx, a), fy = ld_bf4(cy, a), fyu = ld_bf4(cy, a + g.sx), fz = ld_bf4(cz, a),
+            fzu = ld_bf4(cz, a + g.sz);
+  const float fx4 = sec ? mgc(cx[c0 + 3]) : 0.f;  // x face i0+3 (the next group when sec = 1)
+  // x faces of cell i0: lower c0, upper c0+1; of cell i0+2: c0+2, c0+3
+  const float ax0 = fx.at(sec), ax1 = fx.at(sec + 1), ax2 = fx.at(sec + 2), ax3 = sec ? fx4 : fx.at(3);
+  // other-colour x neighbours: i0-1, i0+1, i0+3 are xo[q0-1+sec], xo[q0+sec], xo[q0+1+sec]
+  const float xw0 = sec ? mgc(xm.x) : xe, xe0 = sec ? mgc(xm.y) : mgc(xm.x);
+  const float xw1 = xe0, xe1 = sec ? xe : mgc(xm.y);
+  {
+    const float s = nb_sum6(ax1, xe0, ax0, xw0, fyu.at(sec), mgc(ynn.x), fy.at(sec), mgc(ysn.x), fzu.at(sec),
+                            mgc(znn.x), fz.at(sec), mgc(zsn.x));
+    const float dg = ax0 + ax1 + fy.at(sec) + fyu.at(sec) + fz.at(sec) + fzu.at(sec);
+    const float x0 = (mgc(bb.x) + s) / dg;
+    if (!two) {
+      xcol[q0] = to_x<XT>(x0);
+      return;
+    }
+    const float s1 = nb_sum6(ax3, xe1, ax2, xw1, fyu.at(sec + 2), mgc(ynn.y), fy.at(sec + 2), mgc(ysn.y),
+                             fzu.at(sec + 2), mgc(znn.y), fz.at(sec + 2), mgc(zsn.y));
+    const float dg1 = ax2 + ax3 + fy.at(sec + 2) + fyu.at(sec + 2) + fz.at(sec + 2) + fzu.at(sec + 2);
+    X2 out;
+    out.x = to_x<XT>(x0);
+    out.y = to_x<XT>((mgc(bb.y) + s1) / dg1);
+    *(X2 *)(xcol + q0) = out;
+  }
 }
 
 // first post-smoothing half sweep (black) with the prolongation folded into
@@ -1503,6 +1585,11 @@ static dim3 faces_full(const Grid &g) {
 static dim3 half_cells(const Grid &g) {
   return dim3(((g.nx + 1) / 2 + SULI_BX - 1) / SULI_BX, (g.ny + SULI_BY - 1) / SULI_BY, g.nz);
 }
+// two cells of a colour per thread (pairs of the colour-split frame in twos)
+static dim3 half2_cells(const Grid &g) {
+  const int threads = ((g.nx + 1) / 2 + 1) / 2;
+  return dim3((threads + SULI_BX - 1) / SULI_BX, (g.ny + SULI_BY - 1) / SULI_BY, g.nz);
+}
 
 // Build the FP32 operator copies and the multigrid hierarchy for this dt/rho.
 // (level 0 and the FP32 operator copies here unless the fused set-up made them)
@@ -1586,9 +1673,24 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
   };
   const bool ff = c.mg_fuse_fine && S >= 1;  // the finest level has launches of its own
   const bool xb = ff && staged && f16;  // its colour-split x and b in FP16, the input scaled
+  // its half sweeps with two cells per thread (k_mg0_gs2; the frame alignment it assumes)
+  const bool pair2 = ff && c.g.fast && (c.g.off + 1) % 4 == 0 && c.g.sx % 4 == 0 && f.hg.off % 2 == 0 &&
+                     f.hg.sx % 2 == 0 && f.hg.sz % 2 == 0;
   // one red-black half sweep of level lf (BF16 faces where the level has them)
   auto gs = [&](MgLevel &lf, const char *name, int color, int zero_x) {
     ProfKernel pk(c, name);
+    if (ff && &lf == &f.lv[0] && pair2) {  // two cells per thread
+      if (xb)
+        SULI_TIMED_LAUNCH(pk, k_mg0_gs2<_Float16>, half2_cells(lf.g), b2, 0, c.st, lf.g, f.hg,
+                          (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
+                          (const _Float16 *)(color ? f.b1h : f.b0h), (const _Float16 *)(color ? f.x0h : f.x1h),
+                          color ? f.x1h : f.x0h, color, active);
+      else
+        SULI_TIMED_LAUNCH(pk, k_mg0_gs2<float>, half2_cells(lf.g), b2, 0, c.st, lf.g, f.hg, (const uint16_t *)lf.hx,
+                          (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)(color ? f.b1c : f.b0c),
+                          (const float *)(color ? f.x0c : f.x1c), color ? f.x1c : f.x0c, color, active);
+      return;
+    }
     if (ff && &lf == &f.lv[0]) {  // finest level, colour-split arrays (never from x = 0 here)
       if (xb)
         SULI_TIMED_LAUNCH(pk, k_mg0_gs<_Float16>, half_cells(lf.g), b2, 0, c.st, lf.g, f.hg, (const uint16_t *)lf.hx,

@@ -6,7 +6,9 @@ their plain forms, and in FP64 against the oracle:
   11_diversos.f90:10-228) against the kernel that reads every stencil value
   from memory;
 - the reinitialisation's sign function (14_level_set.f90:400-435) computed
-  inside its first RK stage against its own pass.
+  inside its first RK stage against its own pass;
+- the finest multigrid level's half sweeps with two cells per thread
+  (k_mg0_gs2) against one cell per thread (k_mg0_gs), in the projection.
 
 The grids split the inner box into ragged tiles along every direction
 (64-cell rows, 4-row tiles, 16-plane chunks), so tile edges, the x ends
@@ -153,3 +155,37 @@ def test_level_set_fast_matches_plain(prec, per):
         errs = {n: rel_err(out[1][n], st.arr(n)) for n in ("ls", "hs", "rho", "kurv")}
         print("fast FP64 level set vs oracle", errs)
         assert all(e <= 1e-10 for e in errs.values()), errs
+
+
+@pytest.mark.parametrize("grid", [(128, 64, 64), (70, 34, 20), (96, 48, 40)])
+@pytest.mark.parametrize("form", [1, 2])
+def test_multigrid_pair_sweeps_match_single(grid, form):
+    """Projections with the finest level's half sweeps two cells per thread
+    (fast) and one (plain), colour-split arrays in FP32 (form 1) and FP16
+    (form 2): the same iterations and the same pressure bit for bit.  nx = 70
+    gives 35 pairs per row, so the last thread's second pair is the ghost
+    pair and its second cell is missing in every other row."""
+    import pysuli
+    nx, ny, nz = grid
+    sv = pysuli.Solver(nx, ny, nz, 1.0 / nx, precision=pysuli.PREC_MIXED, poisson=pysuli.POISSON_FCG,
+                       precond=pysuli.PRECOND_GMG, tol_rel=1e-8)
+    try:
+        sv.set_mg(fuse_fine=form)
+        sv.init_dambreak(0.3, 0.4)
+        sv.contorno(1)
+        sv.contorno(3)
+        sv.step(2e-4)
+        sv.set_guess(0)
+        p0 = sv.download("prd1")
+        out = {}
+        for fast in (0, 1):
+            sv.set_fast_stencils(fast)
+            sv.upload("prd1", p0)
+            st = sv.graddin(2e-4)
+            out[fast] = (st, sv.download("prd1"))
+        print(f"{grid} form {form}: iterations {out[0][0]['iters']} / {out[1][0]['iters']}")
+        assert out[0][0]["converged"] and out[0][0]["iters"] > 0
+        assert out[1][0]["iters"] == out[0][0]["iters"]
+        assert np.array_equal(out[1][1], out[0][1])
+    finally:
+        sv.close()
