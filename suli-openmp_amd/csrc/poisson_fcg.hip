@@ -818,6 +818,57 @@ __device__ __forceinline__ Bf4 ld_bf4(const uint16_t *__restrict__ p, fidx a) {
   return Bf4{*(const uint2 *)(p + a)};
 }
 
+// The faces and other-colour values around the two cells i0 = 1+2p0+sec
+// and i0+2 (pairs p0 = 2t and p0+1) of row (j, k) that a thread of the pair
+// kernels updates; xo is the other colour's array, q0 = h.idx(p0, j, k).
+struct PairNb {
+  float ax[4];                            // x faces at c0, c0+1, c0+2, c0+3
+  float fy[2], fyu[2], fz[2], fzu[2];     // y and z faces (lower, upper) of each cell
+  float xw0, xe0, xe1;                    // other colour at i0-1, i0+1, i0+3
+  float ys[2], yn[2], zs[2], zn[2];       // other colour at j-1, j+1, k-1, k+1 of each cell
+  // the half-sweep update of cell n (0: i0, 1: i0+2) from its b, term for
+  // term k_mg0_gs's
+  __device__ __forceinline__ float update(int n, float b) const {
+    const float xw = n ? xe0 : xw0, xe = n ? xe1 : xe0;
+    const float s = nb_sum6(ax[2 * n + 1], xe, ax[2 * n], xw, fyu[n], yn[n], fy[n], ys[n], fzu[n], zn[n], fz[n],
+                            zs[n]);
+    return (b + s) / (ax[2 * n] + ax[2 * n + 1] + fy[n] + fyu[n] + fz[n] + fzu[n]);
+  }
+};
+template <typename XT>
+__device__ __forceinline__ PairNb load_pair_nb(const Grid &g, const Grid &h, const uint16_t *__restrict__ cx,
+                                               const uint16_t *__restrict__ cy, const uint16_t *__restrict__ cz,
+                                               const XT *__restrict__ xo, fidx c0, fidx q0, int sec) {
+  typedef typename Pair2<XT>::t X2;
+  PairNb r;
+  const fidx a = c0 - sec;
+  const X2 xm = *(const X2 *)(xo + q0);
+  const float xe = mgc(xo[q0 - 1 + 3 * sec]);  // the x-neighbour outside the two pairs
+  const X2 ysn = *(const X2 *)(xo + q0 - h.sx), ynn = *(const X2 *)(xo + q0 + h.sx);
+  const X2 zsn = *(const X2 *)(xo + q0 - h.sz), znn = *(const X2 *)(xo + q0 + h.sz);
+  const Bf4 fx = ld_bf4(cx, a), fy = ld_bf4(cy, a), fyu = ld_bf4(cy, a + g.sx), fz = ld_bf4(cz, a),
+            fzu = ld_bf4(cz, a + g.sz);
+  const float fx4 = sec ? mgc(cx[c0 + 3]) : 0.f;  // x face c0+3 (the next group when sec = 1)
+  r.ax[0] = fx.at(sec);
+  r.ax[1] = fx.at(sec + 1);
+  r.ax[2] = fx.at(sec + 2);
+  r.ax[3] = sec ? fx4 : fx.at(3);
+  // i0-1, i0+1, i0+3 hold xo[q0-1+sec], xo[q0+sec], xo[q0+1+sec]
+  r.xw0 = sec ? mgc(xm.x) : xe;
+  r.xe0 = sec ? mgc(xm.y) : mgc(xm.x);
+  r.xe1 = sec ? xe : mgc(xm.y);
+#pragma unroll
+  for (int n = 0; n < 2; n++) {
+    r.fy[n] = fy.at(sec + 2 * n);
+    r.fyu[n] = fyu.at(sec + 2 * n);
+    r.fz[n] = fz.at(sec + 2 * n);
+    r.fzu[n] = fzu.at(sec + 2 * n);
+  }
+  r.ys[0] = mgc(ysn.x), r.ys[1] = mgc(ysn.y), r.yn[0] = mgc(ynn.x), r.yn[1] = mgc(ynn.y);
+  r.zs[0] = mgc(zsn.x), r.zs[1] = mgc(zsn.y), r.zn[0] = mgc(znn.x), r.zn[1] = mgc(znn.y);
+  return r;
+}
+
 template <typename XT>
 __global__ __launch_bounds__(256) void k_mg0_gs2(Grid g, Grid h, const uint16_t *__restrict__ cx,
                                                  const uint16_t *__restrict__ cy, const uint16_t *__restrict__ cz,
@@ -832,38 +883,96 @@ __global__ __launch_bounds__(256) void k_mg0_gs2(Grid g, Grid h, const uint16_t 
   const int sec = (1 + j + k + color) & 1;
   const int i0 = 1 + 2 * p0 + sec;
   if (i0 > g.nx) return;
-  const bool two = i0 + 2 <= g.nx;
-  const fidx c0 = g.idx(i0, j, k), a = c0 - sec, q0 = h.idx(p0, j, k);
-  const X2 xm = *(const X2 *)(xo + q0);
-  const float xe = mgc(xo[q0 - 1 + 3 * sec]);  // the x-neighbour outside the pair of pairs
-  const X2 ysn = *(const X2 *)(xo + q0 - h.sx), ynn = *(const X2 *)(xo + q0 + h.sx);
-  const X2 zsn = *(const X2 *)(xo + q0 - h.sz), znn = *(const X2 *)(xo + q0 + h.sz);
+  const fidx c0 = g.idx(i0, j, k), q0 = h.idx(p0, j, k);
+  const PairNb nb = load_pair_nb(g, h, cx, cy, cz, xo, c0, q0, sec);
   const X2 bb = *(const X2 *)(bcol + q0);
-  const Bf4 fx = ld_bf4(cx, a), fy = ld_bf4(cy, a), fyu = ld_bf4(cy, a + g.sx), fz = ld_bf4(cz, a),
-            fzu = ld_bf4(cz, a + g.sz);
-  const float fx4 = sec ? mgc(cx[c0 + 3]) : 0.f;  // x face i0+3 (the next group when sec = 1)
-  // x faces of cell i0: lower c0, upper c0+1; of cell i0+2: c0+2, c0+3
-  const float ax0 = fx.at(sec), ax1 = fx.at(sec + 1), ax2 = fx.at(sec + 2), ax3 = sec ? fx4 : fx.at(3);
-  // other-colour x neighbours: i0-1, i0+1, i0+3 are xo[q0-1+sec], xo[q0+sec], xo[q0+1+sec]
-  const float xw0 = sec ? mgc(xm.x) : xe, xe0 = sec ? mgc(xm.y) : mgc(xm.x);
-  const float xw1 = xe0, xe1 = sec ? xe : mgc(xm.y);
-  {
-    const float s = nb_sum6(ax1, xe0, ax0, xw0, fyu.at(sec), mgc(ynn.x), fy.at(sec), mgc(ysn.x), fzu.at(sec),
-                            mgc(znn.x), fz.at(sec), mgc(zsn.x));
-    const float dg = ax0 + ax1 + fy.at(sec) + fyu.at(sec) + fz.at(sec) + fzu.at(sec);
-    const float x0 = (mgc(bb.x) + s) / dg;
-    if (!two) {
-      xcol[q0] = to_x<XT>(x0);
-      return;
-    }
-    const float s1 = nb_sum6(ax3, xe1, ax2, xw1, fyu.at(sec + 2), mgc(ynn.y), fy.at(sec + 2), mgc(ysn.y),
-                             fzu.at(sec + 2), mgc(znn.y), fz.at(sec + 2), mgc(zsn.y));
-    const float dg1 = ax2 + ax3 + fy.at(sec + 2) + fyu.at(sec + 2) + fz.at(sec + 2) + fzu.at(sec + 2);
-    X2 out;
-    out.x = to_x<XT>(x0);
-    out.y = to_x<XT>((mgc(bb.y) + s1) / dg1);
-    *(X2 *)(xcol + q0) = out;
+  const float x0 = nb.update(0, mgc(bb.x));
+  if (i0 + 2 > g.nx) {  // the second cell is missing
+    xcol[q0] = to_x<XT>(x0);
+    return;
   }
+  X2 out;
+  out.x = to_x<XT>(x0);
+  out.y = to_x<XT>(nb.update(1, mgc(bb.y)));
+  *(X2 *)(xcol + q0) = out;
+}
+
+// k_mg0_out with two pairs per thread: the red cells i0 = i1+sec and i0+2 of
+// cells i1 = 4t+1 .. i1+3 updated as in k_mg0_gs2, z = s x (/ *sig) of the
+// four cells (an aligned float4 of the frame: the x offset 3 modulo 4) and
+// their z.q; a row end with fewer than four cells takes the cells one by one
+template <typename XT>
+__global__ __launch_bounds__(256) void k_mg0_out2(Grid g, Grid h, const uint16_t *__restrict__ cx,
+                                                  const uint16_t *__restrict__ cy, const uint16_t *__restrict__ cz,
+                                                  const XT *__restrict__ bred, const XT *__restrict__ xblack,
+                                                  const float *__restrict__ s, float *__restrict__ z,
+                                                  const float *__restrict__ q, const int *first, double *partials,
+                                                  const int *active, const float *sig) {
+  typedef typename Pair2<XT>::t X2;
+  if (!*active) return;
+  int bx, by, bz;
+  xcd_band_block(bx, by, bz);
+  const int p0 = 2 * (bx * SULI_BX + threadIdx.x), j = 1 + by * SULI_BY + threadIdx.y, k = 1 + bz;
+  const int i1 = 1 + 2 * p0;
+  double acc[1] = {0.0};
+  const bool dot = partials && !*first;
+  const float isg = sig ? 1.f / *sig : 1.f;
+  if (j <= g.ny && k <= g.nz && i1 <= g.nx) {
+    const int sec = (1 + j + k) & 1;  // colour 0
+    const int i0 = i1 + sec;
+    const fidx c1 = g.idx(i1, j, k), q0 = h.idx(p0, j, k);
+    // black values of the two pairs (also the x-neighbours the red updates read)
+    if (i1 + 3 <= g.nx) {
+      const PairNb nb = load_pair_nb(g, h, cx, cy, cz, xblack, c1 + sec, q0, sec);
+      const X2 bb = *(const X2 *)(bred + q0);
+      const X2 xb = *(const X2 *)(xblack + q0);
+      const float xr0 = nb.update(0, mgc(bb.x)), xr1 = nb.update(1, mgc(bb.y));
+      const float4 sv = *(const float4 *)(s + c1);
+      // cells i1..i1+3: red at i1+sec and i1+2+sec, black at the others
+      float4 zv;
+      zv.x = sv.x * (sec ? mgc(xb.x) : xr0) * isg;
+      zv.y = sv.y * (sec ? xr0 : mgc(xb.x)) * isg;
+      zv.z = sv.z * (sec ? mgc(xb.y) : xr1) * isg;
+      zv.w = sv.w * (sec ? xr1 : mgc(xb.y)) * isg;
+      *(float4 *)(z + c1) = zv;
+      if (dot) {
+        const float4 qv = *(const float4 *)(q + c1);
+        acc[0] += (double)zv.x * (double)qv.x + (double)zv.y * (double)qv.y + (double)zv.z * (double)qv.z +
+                  (double)zv.w * (double)qv.w;
+      }
+    } else {  // fewer than four cells: each pair as k_mg0_out takes it
+      for (int pp = 0; pp < 2; pp++) {
+        const int ia = i1 + 2 * pp;
+        if (ia > g.nx) break;
+        const fidx ca = c1 + 2 * pp, qh = q0 + pp;
+        const bool twoc = ia < g.nx;
+        const int ir = ia + sec;
+        if (ir <= g.nx) {
+          const fidx c = ca + sec, cb = sec ? ca : ca + 1;
+          const float sx = nb_sum6(mgc(cx[c + 1]), mgc(xblack[qh + sec]), mgc(cx[c]), mgc(xblack[qh - 1 + sec]),
+                                   mgc(cy[c + g.sx]), mgc(xblack[qh + h.sx]), mgc(cy[c]), mgc(xblack[qh - h.sx]),
+                                   mgc(cz[c + g.sz]), mgc(xblack[qh + h.sz]), mgc(cz[c]), mgc(xblack[qh - h.sz]));
+          const float xr = (mgc(bred[qh]) + sx) / mg_dg(g, c, cx, cy, cz);
+          const float zr = s[c] * xr * isg;
+          z[c] = zr;
+          if (dot) acc[0] += (double)zr * (double)q[c];
+          if (twoc) {
+            const float zb = s[cb] * mgc(xblack[qh]) * isg;
+            z[cb] = zb;
+            if (dot) acc[0] += (double)zb * (double)q[cb];
+          }
+        } else {
+          const float zb = s[ca] * mgc(xblack[qh]) * isg;
+          z[ca] = zb;
+          if (dot) acc[0] += (double)zb * (double)q[ca];
+        }
+      }
+    }
+  }
+  if (!partials) return;
+  __shared__ double lds[SULI_LDS_RED];
+  block_reduce<1>(acc, lds);
+  block_partial<1>(acc, partials);
 }
 
 // first post-smoothing half sweep (black) with the prolongation folded into
@@ -886,9 +995,9 @@ __global__ __launch_bounds__(256) void k_mg0_prolong(Grid g, Grid h, Grid gc, co
   const auto X = [&](int64_t qn, int in, int jn, int kn) {
     return mgc(xred[qn]) + xc[gc.idx((in + 1) >> 1, (jn + 1) >> 1, (kn + 1) >> 1)];
   };
-  const float s = mgc(cx[c + 1]) * X(q + sec, i + 1, j, k) + mgc(cx[c]) * X(q - 1 + sec, i - 1, j, k) +
-                  mgc(cy[c + g.sx]) * X(q + h.sx, i, j + 1, k) + mgc(cy[c]) * X(q - h.sx, i, j - 1, k) +
-                  mgc(cz[c + g.sz]) * X(q + h.sz, i, j, k + 1) + mgc(cz[c]) * X(q - h.sz, i, j, k - 1);
+  const float s = nb_sum6(mgc(cx[c + 1]), X(q + sec, i + 1, j, k), mgc(cx[c]), X(q - 1 + sec, i - 1, j, k),
+                          mgc(cy[c + g.sx]), X(q + h.sx, i, j + 1, k), mgc(cy[c]), X(q - h.sx, i, j - 1, k),
+                          mgc(cz[c + g.sz]), X(q + h.sz, i, j, k + 1), mgc(cz[c]), X(q - h.sz, i, j, k - 1));
   xblack[q] = to_x<XT>((mgc(bblack[q]) + s) / mg_dg(g, c, cx, cy, cz));
 }
 
@@ -917,9 +1026,9 @@ __global__ __launch_bounds__(256) void k_mg0_out(Grid g, Grid h, const uint16_t 
     const bool two = i1 < g.nx;
     if (i <= g.nx) {
       const int64_t c = c1 + sec, cb = sec ? c1 : c1 + 1;
-      const float sx = mgc(cx[c + 1]) * mgc(xblack[qh + sec]) + mgc(cx[c]) * mgc(xblack[qh - 1 + sec]) +
-                       mgc(cy[c + g.sx]) * mgc(xblack[qh + h.sx]) + mgc(cy[c]) * mgc(xblack[qh - h.sx]) +
-                       mgc(cz[c + g.sz]) * mgc(xblack[qh + h.sz]) + mgc(cz[c]) * mgc(xblack[qh - h.sz]);
+      const float sx = nb_sum6(mgc(cx[c + 1]), mgc(xblack[qh + sec]), mgc(cx[c]), mgc(xblack[qh - 1 + sec]),
+                               mgc(cy[c + g.sx]), mgc(xblack[qh + h.sx]), mgc(cy[c]), mgc(xblack[qh - h.sx]),
+                               mgc(cz[c + g.sz]), mgc(xblack[qh + h.sz]), mgc(cz[c]), mgc(xblack[qh - h.sz]));
       const float xr = (mgc(bred[qh]) + sx) / mg_dg(g, c, cx, cy, cz);
       const float zr = s[c] * xr * isg;
       z[c] = zr;
@@ -1805,6 +1914,8 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     const int nu = l == 0 ? SULI_MG_NU0 : (l == 1 ? SULI_MG_NU1 : nu2);
     if (c.mg_fuse_fine) {
       ProfScope ps(c, l == 0 ? "mg_gs_l0_prolong" : "mg_gs_prolong");
+      // (one cell per thread: a two-cell form of this sweep, with its eleven
+      // coarse-parent loads, measured 12.8 against 11.5 us)
       if (l == 0 && xb)  // (c.mg_fuse_fine and S >= 1: ff)
         hipLaunchKernelGGL(k_mg0_prolong<_Float16>, half_cells(lf.g), b2, 0, c.st, lf.g, f.hg, lc.g,
                            (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
@@ -1836,16 +1947,25 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     double *part = zq ? zq->rs.partials : nullptr;
     const float *q = zq ? zq->q : nullptr;
     const int *first = zq ? zq->first : active;
-    if (xb)
+    const float *sig = (const float *)((const char *)f.ctl + offsetof(FcgCtl, sig));
+    const dim3 go = pair2 ? half2_cells(c.g) : half_cells(c.g);
+    if (pair2 && xb)
+      hipLaunchKernelGGL(k_mg0_out2<_Float16>, go, b2, 0, c.st, c.g, f.hg, (const uint16_t *)l0.hx,
+                         (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const _Float16 *)f.b0h,
+                         (const _Float16 *)f.x1h, (const float *)f.sD32, z, q, first, part, active, sig);
+    else if (pair2)
+      hipLaunchKernelGGL(k_mg0_out2<float>, go, b2, 0, c.st, c.g, f.hg, (const uint16_t *)l0.hx,
+                         (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const float *)f.b0c,
+                         (const float *)f.x1c, (const float *)f.sD32, z, q, first, part, active, nullptr);
+    else if (xb)
       hipLaunchKernelGGL(k_mg0_out<_Float16>, half_cells(c.g), b2, 0, c.st, c.g, f.hg, (const uint16_t *)l0.hx,
                          (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const _Float16 *)f.b0h,
-                         (const _Float16 *)f.x1h, (const float *)f.sD32, z, q, first, part, active,
-                         (const float *)((const char *)f.ctl + offsetof(FcgCtl, sig)));
+                         (const _Float16 *)f.x1h, (const float *)f.sD32, z, q, first, part, active, sig);
     else
       hipLaunchKernelGGL(k_mg0_out<float>, half_cells(c.g), b2, 0, c.st, c.g, f.hg, (const uint16_t *)l0.hx,
                          (const uint16_t *)l0.hy, (const uint16_t *)l0.hz, (const float *)f.b0c,
                          (const float *)f.x1c, (const float *)f.sD32, z, q, first, part, active, nullptr);
-    if (zq) launch_finish<1>(c.st, half_cells(c.g), zq->rs, EpiActive{f.ctl});
+    if (zq) launch_finish<1>(c.st, go, zq->rs, EpiActive{f.ctl});
     return;
   }
   {
