@@ -382,8 +382,8 @@ def hbm_traffic(kernel):
 
 
 # timing scope -> kernel of that scope (the finest-grid dispatches of it)
-PMC_KERNELS = {"mg_gs_l0": "k_mg0_gs", "mg_gs_l0_in": "k_mg0_in", "mg_gs_l0_prolong": "k_mg0_prolong",
-               "mg_gs_l0_out": "k_mg0_out", "mg_restrict_l0": "k_mg0_restrict", "mg_prolong_l0": "k_mg_prolong",
+PMC_KERNELS = {"mg_gs_l0": "k_mg0_gs2", "mg_gs_l0_in": "k_mg0_in", "mg_gs_l0_prolong": "k_mg0_prolong",
+               "mg_gs_l0_out": "k_mg0_out2", "mg_restrict_l0": "k_mg0_restrict", "mg_prolong_l0": "k_mg_prolong",
                "mg_scale": "k_scale", "fcg_spmv_dot": "k_fcg1", "fcg_update": "k_fcg2", "fcg_zq": "k_zq",
                "fcg_residual64": "k_res64", "fcg_project": "k_r32", "fcg_xupdate": "k_xupd",
                "convdiff_tempo": ("k_momentum_tile", "k_momentum"), "ls_advect_rk3": "k_ls_adv", "reinit_rk3": "k_reinit_stage",
@@ -392,8 +392,13 @@ PMC_KERNELS = {"mg_gs_l0": "k_mg0_gs", "mg_gs_l0_in": "k_mg0_in", "mg_gs_l0_prol
 
 
 def _kernel_ident(name):
-    """'void k_momentum<float>(Grid, ...)' -> 'k_momentum'."""
+    """'void k_momentum<float>(Grid, ...)' -> 'k_momentum'; a name the profiler
+    left mangled ('_Z9k_mg0_gs2IDF16_E...', the _Float16 instantiations) ->
+    'k_mg0_gs2'."""
     import re
+    m = re.match(r"_Z(\d+)", name)
+    if m:
+        return name[m.end():m.end() + int(m.group(1))]
     n = name[5:] if name.startswith("void ") else name
     return re.split(r"[<(]", n, maxsplit=1)[0].strip()
 
@@ -459,6 +464,42 @@ def pmc_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, ncell):
     with open(TRAFFIC_FILE, "w") as f:
         json.dump(res, f, indent=1)
     print(json.dumps(res, indent=1))
+
+
+def stream_probe():
+    """Practical streaming rates of this GPU for the roofline fractions:
+    torch's device copy, sum and fill over working sets of 16 MB to 1 GiB,
+    each launched 50 times back to back (sets up to about 256 MB stay in the
+    Infinity Cache between launches; 1 GiB ones stream from HBM)."""
+    import torch
+    out = []
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for mb in (16, 38, 64, 150, 256, 1024):
+        n = (mb << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        half_a, half_b = a[: n // 2], torch.empty(n // 2, dtype=torch.float32, device="cuda")
+        for kind in ("copy", "sum", "fill"):
+            def run():
+                if kind == "copy":
+                    half_b.copy_(half_a)  # mb moved: half read, half written
+                elif kind == "sum":
+                    a.sum()
+                else:
+                    a.fill_(1.0)
+            for _ in range(3):
+                run()
+            ev0.record()
+            for _ in range(50):
+                run()
+            ev1.record()
+            torch.cuda.synchronize()
+            us = ev0.elapsed_time(ev1) * 1e3 / 50
+            rec = {"kind": kind, "mb_moved": mb, "us_per_launch": round(us, 2), "tb_per_s": round((n * 4) / us / 1e6, 2)}
+            out.append(rec)
+            print(json.dumps(rec), flush=True)
+        del a, half_a, half_b
+        torch.cuda.empty_cache()
+    return out
 
 
 def pmc_calibrate():
@@ -595,11 +636,16 @@ def main():
     ap.add_argument("--poisson-repeats", type=int, default=3)
     ap.add_argument("--config0", action="store_true",
                     help="time the reference's own 64x32x32 case (GPU FP64 and mixed, and the CPU reference) instead")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="only time device copy/sum/fill over 16 MB..1 GiB (practical streaming rates)")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="only run a 256 MiB device copy (scale factor of the rocprof byte counters)")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "CAL_FETCH_DIR", "CAL_WRITE_DIR"),
                     help="summarise rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs into " + TRAFFIC_FILE)
     args = ap.parse_args()
+    if args.stream_probe:
+        stream_probe()
+        return
     if args.pmc_calibrate:
         pmc_calibrate()
         return
