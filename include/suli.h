@@ -347,6 +347,12 @@ int suli_set_fast_stencils(int on);
  * reference. */
 int suli_precond_apply(const double *r, double *z, int64_t n);
 
+/* Reads of the FCG loop's control block by the host since the solver was
+ * set up: published = those that found the copy the finish kernel wrote
+ * into pinned host memory, copied = those that fell back to a device-to-host
+ * copy.  Diagnostics, no counterpart in the reference. */
+int suli_fcg_readbacks(int64_t *published, int64_t *copied);
+
 /* Learned preconditioner (SULI_PRECOND_LEARNED).  No counterpart in the
  * reference (9_grad.f90 is Jacobi-scaled CG).  An L-level 3-D convolutional
  * U-Net (channels 16/32/64/64.., 3x3x3 kernels, ReLU, average pooling, nearest

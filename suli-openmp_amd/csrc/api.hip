@@ -893,6 +893,15 @@ extern "C" int suli_set_fast_stencils(int on) {
   return 0;
 }
 
+extern "C" int suli_fcg_readbacks(int64_t *published, int64_t *copied) {
+  NEEDCTX();
+  if (!published || !copied) {
+    suli_set_error("suli_fcg_readbacks: null output pointer");
+    return 1;
+  }
+  return fcg_readbacks(*g_ctx, published, copied);
+}
+
 extern "C" int suli_precond_apply(const double *r, double *z, int64_t n) {
   NEEDCTX();
   Ctx &c = *g_ctx;

@@ -225,6 +225,12 @@ __device__ __forceinline__ void block_partial(const double (&v)[NV], double *par
   }
 }
 
+// After an epilogue (or in place of one that is not enabled) thread 0 may
+// also publish loop-control state to the host: epi_publish overloads for the
+// epilogue types that do; none by default.
+template <typename Epi>
+__device__ __forceinline__ void epi_publish(const Epi &) {}
+
 // Epilogue of a plain reduction: nothing beyond storing the result.
 struct EpiNone {
   __device__ bool enabled() const { return true; }
@@ -240,7 +246,10 @@ struct EpiNone {
 template <int NV, bool MAX, typename Epi>
 __global__ __launch_bounds__(SULI_FINISH_THREADS) void k_finish(const double *__restrict__ partials, int nb,
                                                                 double *__restrict__ result, Epi epi) {
-  if (!epi.enabled()) return;
+  if (!epi.enabled()) {
+    if (threadIdx.x == 0) epi_publish(epi);
+    return;
+  }
   __shared__ double lds[NV * SULI_LDS_RED];
   double acc[NV];
   const double init = MAX ? -1.0e300 : 0.0;
@@ -262,6 +271,7 @@ __global__ __launch_bounds__(SULI_FINISH_THREADS) void k_finish(const double *__
 #pragma unroll
     for (int q = 0; q < NV; q++) result[q] = acc[q];
     epi(result);
+    epi_publish(epi);
   }
 }
 

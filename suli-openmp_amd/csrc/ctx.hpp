@@ -236,3 +236,4 @@ void fcg_destroy(Ctx &c);
 int fcg_solve(Ctx &c, double dt, suli_poisson_stats *st);
 // z = M r (device FP32 frames) with the operator of the last fcg_solve
 int fcg_precond_apply(Ctx &c, const float *r, float *z);
+int fcg_readbacks(Ctx &c, int64_t *published, int64_t *copied);

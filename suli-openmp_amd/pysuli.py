@@ -118,6 +118,7 @@ def lib():
             "suli_set_mg": ([C.c_int64, i32, i32], i32),
             "suli_set_fast_stencils": ([i32], i32),
             "suli_precond_apply": ([p(f64), p(f64), C.c_int64], i32),
+            "suli_fcg_readbacks": ([p(C.c_int64), p(C.c_int64)], i32),
             "suli_set_forcing": ([f64, f64, f64], i32), "suli_set_time": ([f64], i32),
             "suli_prof_enable": ([i32], i32), "suli_prof_select": ([C.c_char_p, i32], i32),
             "suli_prof_count": ([], i32), "suli_prof_get": ([i32, C.c_char_p, i32, p(C.c_int64), p(f64)], i32),
@@ -324,6 +325,13 @@ class Solver:
         """Momentum predictor in LDS tiles (1, default) or with every stencil
         value read from memory (0); the same values either way."""
         _check(lib().suli_set_fast_stencils(int(on)), "suli_set_fast_stencils")
+
+    def fcg_readbacks(self):
+        """(published, copied): host reads of the FCG control block that found
+        the finish kernel's copy in pinned memory, and those that copied."""
+        a, b = C.c_int64(), C.c_int64()
+        _check(lib().suli_fcg_readbacks(C.byref(a), C.byref(b)), "suli_fcg_readbacks")
+        return a.value, b.value
 
     def precond_apply(self, r):
         """z = M r with the current preconditioner and the operator of the last

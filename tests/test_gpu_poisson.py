@@ -174,6 +174,30 @@ def test_f16_finest_level_iterate(grid):
         sv.close()
 
 
+def test_control_block_published_to_host():
+    """The FCG loop's host reads of its control block find the copy that the
+    chunk's last finish kernel wrote into pinned host memory (no device to
+    host copy), with multigrid and Jacobi passes, and the solves converge."""
+    nx, ny, nz = 64, 32, 32
+    sv = pysuli.Solver(nx, ny, nz, 1.0 / nx, precision=pysuli.PREC_MIXED, poisson=pysuli.POISSON_FCG,
+                       precond=pysuli.PRECOND_GMG, tol_rel=1e-8)
+    try:
+        sv.init_dambreak(0.3, 0.4)
+        sv.contorno(1)
+        sv.contorno(3)
+        for _ in range(3):
+            _, ps = sv.step(2e-4)
+            assert all(p["converged"] for p in ps)
+        sv.set_solver(pysuli.POISSON_FCG, pysuli.PRECOND_JACOBI)
+        _, ps = sv.step(2e-4)
+        assert all(p["converged"] for p in ps)
+        published, copied = sv.fcg_readbacks()
+        print(f"control-block reads: {published} published, {copied} copied")
+        assert published >= 8 and copied == 0
+    finally:
+        sv.close()
+
+
 def test_extrapolated_guess_saves_iterations():
     """The projections started from the stage-2 pressures extrapolated in
     time (suli_set_guess(1), the default) begin with a smaller residual than
