@@ -57,6 +57,7 @@ struct FcgCtl {
   int out;           // the last outer residual ends the solve: the output kernel writes
   // FCG(m) with the learned preconditioner: the last SULI_FCG_M directions
   float sig;                 // input scale of the finest level's staged V-cycle input (FP16 form)
+  unsigned seq;              // (host copy only) the number of the publish that wrote it
   int nk;                    // directions held (0..SULI_FCG_M)
   double zqm[SULI_FCG_M];    // z.q_j of the current z
   double pqm[SULI_FCG_M];    // p_j.q_j of the held directions
@@ -260,6 +261,8 @@ struct EpiOut {
   FcgCtl *ctl;
   double tol;
   int force;
+  FcgCtl *pub;   // set: publish the control block to this host copy (epi_publish)
+  unsigned seq;
   __device__ bool enabled() const { return true; }
   __device__ void operator()(const double *res) const {
     double rr, bb;
@@ -281,6 +284,8 @@ struct EpiK1 {
 };
 struct EpiK2 {  // count the iteration and decide whether the next one runs
   FcgCtl *ctl;
+  FcgCtl *pub;   // set on the last iteration of an enqueued chunk (epi_publish)
+  unsigned seq;
   __device__ bool enabled() const { return ctl->active != 0; }
   __device__ void operator()(const double *res) const {
     ctl->it += 1;
@@ -289,6 +294,25 @@ struct EpiK2 {  // count the iteration and decide whether the next one runs
     ctl->active = stop ? 0 : 1;
   }
 };
+
+// The control block written straight into its pinned, coherent host copy
+// by the finish kernel that ends an enqueued chunk (in place of a device to
+// host copy, which the runtime runs as a kernel of its own): the host reads
+// it after the stream synchronisation and checks seq.
+__device__ __forceinline__ void publish_ctl(const FcgCtl *ctl, FcgCtl *pub, unsigned seq) {
+  static_assert(sizeof(FcgCtl) % 8 == 0, "the control block is copied in 8-byte words");
+  const uint64_t *s = (const uint64_t *)ctl;
+  uint64_t *d = (uint64_t *)pub;
+  for (int w = 0; w < (int)(sizeof(FcgCtl) / 8); w++) d[w] = s[w];
+  pub->seq = seq;
+  __threadfence_system();
+}
+__device__ __forceinline__ void epi_publish(const EpiK2 &e) {
+  if (e.pub) publish_ctl(e.ctl, e.pub, e.seq);
+}
+__device__ __forceinline__ void epi_publish(const EpiOut &e) {
+  if (e.pub) publish_ctl(e.ctl, e.pub, e.seq);
+}
 
 // x += d (FP64 += FP32)
 __global__ __launch_bounds__(256) void k_xupd(Grid g, double *__restrict__ x, const float *__restrict__ d) {
@@ -991,13 +1015,19 @@ __global__ __launch_bounds__(256) void k_mg0_prolong(Grid g, Grid h, Grid gc, co
   const int sec = (j + k) & 1;  // colour 1
   const int i = 1 + 2 * p + sec;
   if (i > g.nx) return;
-  const int64_t c = g.idx(i, j, k), q = h.idx(p, j, k);
-  const auto X = [&](int64_t qn, int in, int jn, int kn) {
-    return mgc(xred[qn]) + xc[gc.idx((in + 1) >> 1, (jn + 1) >> 1, (kn + 1) >> 1)];
-  };
-  const float s = nb_sum6(mgc(cx[c + 1]), X(q + sec, i + 1, j, k), mgc(cx[c]), X(q - 1 + sec, i - 1, j, k),
-                          mgc(cy[c + g.sx]), X(q + h.sx, i, j + 1, k), mgc(cy[c]), X(q - h.sx, i, j - 1, k),
-                          mgc(cz[c + g.sz]), X(q + h.sz, i, j, k + 1), mgc(cz[c]), X(q - h.sz, i, j, k - 1));
+  const fidx c = g.idx(i, j, k), q = h.idx(p, j, k);
+  // The six neighbours' coarse parents are four cells: the cell's own parent
+  // (that of one neighbour on each axis) and, on each axis, the parent across
+  // the coarse face (the lower one from an odd index, the upper from an even)
+  const fidx P = gc.idx((i + 1) >> 1, (j + 1) >> 1, (k + 1) >> 1);
+  const float xp = xc[P], xpx = xc[P + ((i & 1) ? (fidx)-1 : (fidx)1)],
+              xpy = xc[P + ((j & 1) ? (fidx)0 - gc.sx : (fidx)gc.sx)],
+              xpz = xc[P + ((k & 1) ? (fidx)0 - gc.sz : gc.sz)];
+  const float xe = mgc(xred[q + sec]) + ((i & 1) ? xp : xpx), xw = mgc(xred[q - 1 + sec]) + ((i & 1) ? xpx : xp);
+  const float xn = mgc(xred[q + h.sx]) + ((j & 1) ? xp : xpy), xs = mgc(xred[q - h.sx]) + ((j & 1) ? xpy : xp);
+  const float xt = mgc(xred[q + h.sz]) + ((k & 1) ? xp : xpz), xb = mgc(xred[q - h.sz]) + ((k & 1) ? xpz : xp);
+  const float s = nb_sum6(mgc(cx[c + 1]), xe, mgc(cx[c]), xw, mgc(cy[c + g.sx]), xn, mgc(cy[c]), xs,
+                          mgc(cz[c + g.sz]), xt, mgc(cz[c]), xb);
   xblack[q] = to_x<XT>((mgc(bblack[q]) + s) / mg_dg(g, c, cx, cy, cz));
 }
 
@@ -1570,6 +1600,10 @@ struct MgLevel {
 
 struct PoissonFCG {
   FcgCtl *ctl = nullptr, *ctl_h = nullptr;
+  FcgCtl *ctl_hd = nullptr;  // ctl_h as the device addresses it (the finish kernels publish into it)
+  unsigned pub_seq = 0;      // publishes so far
+  int64_t pub_hits = 0;      // control-block reads that found their publish
+  int64_t pub_misses = 0;    // reads that found no fresh publish (and copied instead)
   double *b = nullptr, *x = nullptr, *r = nullptr;          // FP64 outer vectors (frame)
   float *r32 = nullptr, *d32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr, *z32 = nullptr;
   float *ax32 = nullptr, *ay32 = nullptr, *az32 = nullptr, *sD32 = nullptr;
@@ -1613,7 +1647,8 @@ int fcg_create(Ctx &c) {
   HIPCHK(hipMalloc((void **)&f->ctl, sizeof(FcgCtl)));
   c.allocs.push_back(f->ctl);
   HIPCHK(hipMemsetAsync(f->ctl, 0, sizeof(FcgCtl), c.st));
-  HIPCHK(hipHostMalloc((void **)&f->ctl_h, sizeof(FcgCtl), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&f->ctl_h, sizeof(FcgCtl), hipHostMallocCoherent | hipHostMallocMapped));
+  HIPCHK(hipHostGetDevicePointer((void **)&f->ctl_hd, f->ctl_h, 0));
   if (dalloc64(c, &f->b, n) || dalloc64(c, &f->x, n) || dalloc64(c, &f->r, n)) return 1;
   float **fl[] = {&f->r32, &f->d32, &f->p32, &f->p32b, &f->q32, &f->z32, &f->ax32, &f->ay32, &f->az32, &f->sD32};
   for (float **q : fl)
@@ -1998,7 +2033,23 @@ static constexpr int kLearnedPassCap = 64;
 
 // One FCG(m) iteration with the learned preconditioner; the new direction
 // goes to slot `slot` of the ring of held directions.
-static void fcgm_iteration(Ctx &c, PoissonFCG &f, int iter, dim3 gk, const int *active) {
+// Wait for the stream and read the control block that the last finish kernel
+// published (publish number seq) into the host copy; a copy that is not that
+// publish is read from the device instead.
+static int ctl_read(Ctx &c, PoissonFCG &f, unsigned seq) {
+  HIPCHK(hipStreamSynchronize(c.st));
+  if (f.ctl_h->seq == seq) {
+    f.pub_hits++;
+  } else {
+    f.pub_misses++;
+    HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+  }
+  return 0;
+}
+
+static void fcgm_iteration(Ctx &c, PoissonFCG &f, int iter, dim3 gk, const int *active, FcgCtl *pub = nullptr,
+                           unsigned seq = 0) {
   const Grid &g = c.g;
   const dim3 b2(SULI_BX, SULI_BY, 1);
   const int slot = iter % SULI_FCG_M;
@@ -2021,7 +2072,7 @@ static void fcgm_iteration(Ctx &c, PoissonFCG &f, int iter, dim3 gk, const int *
     ProfScope ps(c, "fcg_update");
     const RedSlot r2{c.partials, CTL_PTR(f.ctl, k2)};
     hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.pm[slot], f.qm[slot], f.r32, f.d32, f.ctl, r2, StageIn{});
-    launch_finish<2>(c.st, gk, r2, EpiK2{f.ctl});
+    launch_finish<2>(c.st, gk, r2, EpiK2{f.ctl, pub, seq});
   }
 }
 
@@ -2169,21 +2220,21 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
   const dim3 pg((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4, g.nz + 2);
   bool pout_done = false;
   for (;;) {
+    const bool spec = speculate && outer < max_outer;
+    const unsigned seq_out = spec ? 0u : ++f.pub_seq;
     {
       ProfScope ps(c, "fcg_residual64");
       const RedSlot rr{c.partials, CTL_PTR(f.ctl, rout)};
       hipLaunchKernelGGL(k_res64, gk, b2, 0, c.st, g, c.ax, c.ay, c.az, f.x, f.b, c.sD, f.r, rr);
-      launch_finish<3>(c.st, gk, rr, EpiOut{f.ctl, tol, outer >= max_outer ? 1 : 0});
+      launch_finish<3>(c.st, gk, rr, EpiOut{f.ctl, tol, outer >= max_outer ? 1 : 0, spec ? nullptr : f.ctl_hd, seq_out});
     }
-    const bool spec = speculate && outer < max_outer;
     if (!spec) {
       {
         ProfScope ps(c, "poisson_final");  // written when this residual ends the solve
         hipLaunchKernelGGL(k_pout, pg, dim3(64, 4, 1), 0, c.st, g, f.x, c.sD, c.prd1, dslot,
                            (const int *)((char *)f.ctl + offsetof(FcgCtl, out)));
       }
-      HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
-      HIPCHK(hipStreamSynchronize(c.st));
+      if (ctl_read(c, f, seq_out)) return 1;
       res = proj_res(f.ctl_h, bnorm);
       if (res0 < 0) res0 = res;
       if (f.ctl_h->out) {  // the device's verdict, the test above and outer >= max_outer
@@ -2224,11 +2275,14 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     int chunk = use_pc ? (pits[pidx] > 0 ? pits[pidx] : 4) : 16;
     int iter = 0;
     for (;;) {
+      const unsigned seq_chunk = ++f.pub_seq;
       for (int n = 0; n < chunk; n++, iter++) {
         if (iter > 0) unet_capture(c, f.unet, f.r32, active, iter);
         f.cur_iter = iter;
+        // the chunk's last finish publishes the control block
+        FcgCtl *const pub = n == chunk - 1 ? f.ctl_hd : nullptr;
         if (pc == SULI_PRECOND_LEARNED && !learned_m1) {
-          fcgm_iteration(c, f, iter, gk, active);
+          fcgm_iteration(c, f, iter, gk, active, pub, seq_chunk);
           continue;
         }
         const float *zin = f.r32;
@@ -2252,12 +2306,11 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
           ProfScope ps(c, "fcg_update");
           const RedSlot r2{c.partials, CTL_PTR(f.ctl, k2)};
           hipLaunchKernelGGL(k_fcg2, gk, b2, 0, c.st, g, f.p32, f.q32, f.r32, f.d32, f.ctl, r2, sti(iter + 1));
-          launch_finish<2>(c.st, gk, r2, EpiK2{f.ctl});
+          launch_finish<2>(c.st, gk, r2, EpiK2{f.ctl, pub, seq_chunk});
         }
       }
       HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(f.ctl_h, f.ctl, sizeof(FcgCtl), hipMemcpyDeviceToHost, c.st));
-      HIPCHK(hipStreamSynchronize(c.st));
+      if (ctl_read(c, f, seq_chunk)) return 1;
       if (!f.ctl_h->active) break;
       if (use_pc) chunk = 2;
       else if (chunk < 64) chunk *= 2;
@@ -2342,6 +2395,12 @@ int fcg_solve(Ctx &c, double dt, suli_poisson_stats *stt) {
     stt->res = res;
     stt->ms = gpu_ms;
   }
+  return 0;
+}
+
+int fcg_readbacks(Ctx &c, int64_t *published, int64_t *copied) {
+  *published = c.fcg ? c.fcg->pub_hits : 0;
+  *copied = c.fcg ? c.fcg->pub_misses : 0;
   return 0;
 }
 
