@@ -630,13 +630,28 @@ __global__ __launch_bounds__(256) void k_mg_gs(Grid g, const CT *__restrict__ cx
   x[c] = (b[c] + s) / mg_dg(g, c, cx, cy, cz);
 }
 
+// The coarse level's x after restriction: 0, or with the coarse FP32 faces
+// given, the first red half sweep from x = 0 on its red cells (colour 0:
+// i + j + k even), (b + 0) / diagonal as k_mg_gs computes it with zero_x.
+// That saves the coarse level's first launch; its black cells stay 0 until
+// the black half sweep that follows.
+__device__ __forceinline__ float coarse_red_x(const Grid &gc, int64_t cc, int ijk, float bcv,
+                                              const float *__restrict__ ccx, const float *__restrict__ ccy,
+                                              const float *__restrict__ ccz) {
+  if (!ccx || (ijk & 1)) return 0.f;
+  const float s = 0.f;
+  return (bcv + s) / mg_dg(gc, cc, ccx, ccy, ccz);
+}
+
 // coarse rhs = sum over the 8 children of the fine residual b - A x; coarse x = 0
+// (or its first red half sweep, coarse_red_x)
 template <typename CT>
 __global__ __launch_bounds__(256) void k_mg_restrict(Grid gf, Grid gc, const CT *__restrict__ cx,
                                                      const CT *__restrict__ cy, const CT *__restrict__ cz,
                                                      const float *__restrict__ b, const float *__restrict__ x,
                                                      float *__restrict__ bc, float *__restrict__ xc,
-                                                     const int *active) {
+                                                     const float *__restrict__ ccx, const float *__restrict__ ccy,
+                                                     const float *__restrict__ ccz, const int *active) {
   if (!*active) return;
   const int I = 1 + blockIdx.x * SULI_BX + threadIdx.x, J = 1 + blockIdx.y * SULI_BY + threadIdx.y,
             K = 1 + blockIdx.z;
@@ -650,7 +665,7 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Grid gf, Grid gc, const CT 
   }
   const int64_t cc = gc.idx(I, J, K);
   bc[cc] = s;
-  xc[cc] = 0.f;
+  xc[cc] = coarse_red_x(gc, cc, I + J + K, s, ccx, ccy, ccz);
 }
 
 // ---- half sweeps fused with their neighbours in the V-cycle ----------------------
@@ -942,8 +957,7 @@ __global__ __launch_bounds__(256) void k_mg0_out2(Grid g, Grid h, const uint16_t
   const bool dot = partials && !*first;
   const float isg = sig ? 1.f / *sig : 1.f;
   if (j <= g.ny && k <= g.nz && i1 <= g.nx) {
-    const int sec = (1 + j + k) & 1;  // colour 0
-    const int i0 = i1 + sec;
+    const int sec = (1 + j + k) & 1;  // colour 0: the red cells are i1 + sec and i1 + 2 + sec
     const fidx c1 = g.idx(i1, j, k), q0 = h.idx(p0, j, k);
     // black values of the two pairs (also the x-neighbours the red updates read)
     if (i1 + 3 <= g.nx) {
@@ -1087,7 +1101,9 @@ __global__ __launch_bounds__(256) void k_mg0_restrict(Grid gf, Grid h, Grid gc, 
                                                       const uint16_t *__restrict__ cz, const XT *__restrict__ b0,
                                                       const XT *__restrict__ b1, const XT *__restrict__ x0,
                                                       const XT *__restrict__ x1, float *__restrict__ bc,
-                                                      float *__restrict__ xc, const int *active) {
+                                                      float *__restrict__ xc, const float *__restrict__ ccx,
+                                                      const float *__restrict__ ccy, const float *__restrict__ ccz,
+                                                      const int *active) {
   if (!*active) return;
   int bx, by, bz;
   xcd_band_block(bx, by, bz);
@@ -1110,7 +1126,7 @@ __global__ __launch_bounds__(256) void k_mg0_restrict(Grid gf, Grid h, Grid gc, 
   }
   const int64_t cc = gc.idx(I, J, K);
   bc[cc] = s;
-  xc[cc] = 0.f;
+  xc[cc] = coarse_red_x(gc, cc, I + J + K, s, ccx, ccy, ccz);
 }
 
 // x_fine += x_coarse(parent)
@@ -1872,8 +1888,17 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     return dim3((g.nx + SULI_MG_FT - 1) / SULI_MG_FT, (g.ny + SULI_MG_FT - 1) / SULI_MG_FT,
                 (g.nz + SULI_MG_FT - 1) / SULI_MG_FT);
   };
+  // levels 1..S-1 with launches of their own and FP32 faces take their first
+  // red half sweep from the restriction that fills them (coarse_red_x;
+  // with the fast stencil forms, bit for bit the separate launch)
+  const auto red_in_restrict = [&](int l) {
+    return c.g.fast && l >= 1 && l < S && !fused(l) && !f.lv[l].hx && f.lv[l].cx;
+  };
   for (int l = 0; l < S; l++) {
     MgLevel &lf = f.lv[l], &lc = f.lv[l + 1];
+    const bool rz = red_in_restrict(l + 1);
+    const float *ccx = rz ? (const float *)lc.cx : nullptr, *ccy = rz ? (const float *)lc.cy : nullptr,
+                *ccz = rz ? (const float *)lc.cz : nullptr;
     if (fused(l)) {
       ProfScope ps(c, "mg_pre_fused");
       hipLaunchKernelGGL((k_mg_pre_fused<SULI_MG_FT, SULI_MG_FNH>), tiles(lf.g), bt, 0, c.st, lf.g, lc.g,
@@ -1884,7 +1909,7 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
     }
     const char *gsn = l == 0 ? "mg_gs_l0" : "mg_gs";
     for (int s = 0; s < (l == 0 ? SULI_MG_NU0 : (l == 1 ? SULI_MG_NU1 : nu1)); s++) {
-      if (!(ff && l == 0 && s == 0)) gs(lf, gsn, 0, s == 0 ? 1 : 0);
+      if (!(s == 0 && ((ff && l == 0) || red_in_restrict(l)))) gs(lf, gsn, 0, s == 0 ? 1 : 0);
       gs(lf, gsn, 1, 0);
     }
     ProfScope ps(c, l == 0 ? "mg_restrict_l0" : "mg_restrict");
@@ -1892,20 +1917,20 @@ static void mg_vcycle(Ctx &c, const float *r, float *z, const int *active, const
       hipLaunchKernelGGL(k_mg0_restrict<_Float16>, cells_full(lc.g), b2, 0, c.st, lf.g, f.hg, lc.g,
                          (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
                          (const _Float16 *)f.b0h, (const _Float16 *)f.b1h, (const _Float16 *)f.x0h,
-                         (const _Float16 *)f.x1h, lc.b, lc.x, active);
+                         (const _Float16 *)f.x1h, lc.b, lc.x, ccx, ccy, ccz, active);
     else if (ff && l == 0)
       hipLaunchKernelGGL(k_mg0_restrict<float>, cells_full(lc.g), b2, 0, c.st, lf.g, f.hg, lc.g,
                          (const uint16_t *)lf.hx, (const uint16_t *)lf.hy, (const uint16_t *)lf.hz,
                          (const float *)f.b0c, (const float *)f.b1c, (const float *)f.x0c, (const float *)f.x1c,
-                         lc.b, lc.x, active);
+                         lc.b, lc.x, ccx, ccy, ccz, active);
     else if (lf.hx)
       hipLaunchKernelGGL((k_mg_restrict<uint16_t>), cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, (const uint16_t *)lf.hx,
                          (const uint16_t *)lf.hy, (const uint16_t *)lf.hz, (const float *)lf.b,
-                         (const float *)lf.x, lc.b, lc.x, active);
+                         (const float *)lf.x, lc.b, lc.x, ccx, ccy, ccz, active);
     else
       hipLaunchKernelGGL((k_mg_restrict<float>), cells_full(lc.g), b2, 0, c.st, lf.g, lc.g, (const float *)lf.cx,
                          (const float *)lf.cy, (const float *)lf.cz, (const float *)lf.b, (const float *)lf.x,
-                         lc.b, lc.x, active);
+                         lc.b, lc.x, ccx, ccy, ccz, active);
     capture(l + 1);
   }
   if (net_level > 0) {

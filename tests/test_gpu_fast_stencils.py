@@ -160,7 +160,8 @@ def test_level_set_fast_matches_plain(prec, per):
 
 @pytest.mark.parametrize("grid", [(128, 64, 64), (70, 34, 20), (96, 48, 40)])
 @pytest.mark.parametrize("form", [1, 2])
-def test_multigrid_pair_sweeps_match_single(grid, form):
+@pytest.mark.parametrize("fuse_max", [65536, 4096])
+def test_multigrid_pair_sweeps_match_single(grid, form, fuse_max):
     """The finest level's kernels with two cells per thread (fast: the half
     sweeps and the output sweep) against one (plain):
     the preconditioner output z = M r bit for bit (an application outside
@@ -170,13 +171,16 @@ def test_multigrid_pair_sweeps_match_single(grid, form):
     add their cells in different orders; the constant of the Neumann problem
     is left out).  nx = 70 gives 35 pairs per row, so the last thread's
     second pair is the ghost pair and the output sweep takes its row end
-    cell by cell."""
+    cell by cell.  With at most 4096 cells in a fused level, levels 1 (all
+    three grids) and 2 (128x64x64) run one launch per half sweep, their first
+    red half sweep computed by the restriction that fills them (fast) or by
+    its own launch (plain)."""
     import pysuli
     nx, ny, nz = grid
     sv = pysuli.Solver(nx, ny, nz, 1.0 / nx, precision=pysuli.PREC_MIXED, poisson=pysuli.POISSON_FCG,
                        precond=pysuli.PRECOND_GMG, tol_rel=1e-8)
     try:
-        sv.set_mg(fuse_fine=form)
+        sv.set_mg(fuse_max_cells=fuse_max, fuse_fine=form)
         sv.init_dambreak(0.3, 0.4)
         sv.contorno(1)
         sv.contorno(3)
@@ -192,7 +196,7 @@ def test_multigrid_pair_sweeps_match_single(grid, form):
             out[fast] = (st, sv.download("prd1"), sv.precond_apply(r))
         pa, pb = (out[f][1][1:-1, 1:-1, 1:-1] for f in (0, 1))
         dp = np.abs((pb - pb.mean()) - (pa - pa.mean())).max() / np.abs(pa - pa.mean()).max()
-        print(f"{grid} form {form}: iterations {out[0][0]['iters']} / {out[1][0]['iters']}, "
+        print(f"{grid} form {form} fuse_max {fuse_max}: iterations {out[0][0]['iters']} / {out[1][0]['iters']}, "
               f"max |dp| / max |p| = {dp:.2e} (mean removed)")
         assert np.array_equal(out[1][2], out[0][2])
         assert out[0][0]["converged"] and out[0][0]["iters"] > 0
